@@ -1,0 +1,164 @@
+"""Throughput benchmark of the photon-transport hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--histories H]
+
+Workload (config 2): CLARITAS small cell (bore 25 mm, water 50 mm), 40-bin
+lognormal PSD (mass median 20 um, sigma_g 1.8, m = 1.46 + 1e-4 i), Rouse
+profile on a 64x128 FP32 r-z grid, Rankine free surface (core 3 mm, depth
+5 mm), 860 nm collimated 1 mm beam, 1800-angle Mie table, plain-Mie event
+law, concentration sweep {10, 31.6, 100, 316, 1000, 3162, 5000} mg/L.
+One step = the whole sweep with H histories per concentration (default
+1e8), traced in one persistent batched launch from HBM-resident tables.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+
+
+def cpu_baseline(histories_per_conc: int, seed: int):
+    """The CPU oracle (restatement of the reference algorithm) on a bounded
+    sample of the same workload, all host cores, timed on this host."""
+    import numpy as np
+    from oracle import pyoracle
+    from tardiis_and_claritas_amd import ClaritasForwardModel, workloads
+    t_total, n_total = 0.0, 0
+    for i, c in enumerate(workloads.CONFIG2_CONCENTRATIONS_MG_L):
+        kw, mat, conc, _, _ = workloads.config2(c, 'plain_mie')
+        m = ClaritasForwardModel(**kw)
+        om = pyoracle.OracleModel(m, mat, conc)
+        t0 = time.perf_counter()
+        om.run(histories_per_conc, seed + i, 0, threads=0)
+        t_total += time.perf_counter() - t0
+        n_total += histories_per_conc
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    omp = os.environ.get('OMP_NUM_THREADS')
+    threads = int(omp) if omp and omp.isdigit() else cores
+    model = ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {'value': n_total / t_total, 'unit': 'photon histories/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{histories_per_conc} histories at each of the 7 config-2 concentrations '
+                      f'({n_total} total, plain-Mie law), CPU oracle in C with OpenMP; host CPU: {model}',
+            'seconds': t_total}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--histories', type=float, default=1e8, help='histories per concentration per step')
+    ap.add_argument('--cpu-histories', type=int, default=0, help='CPU sample per concentration (0 = auto)')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--block', type=int, default=0)
+    ap.add_argument('--blocks-per-cu', type=int, default=0)
+    ap.add_argument('--chunk', type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is '
+                                   'described in DESIGN.md, not run)'}))
+        sys.exit(2)
+    import numpy as np
+    from tardiis_and_claritas_amd import workloads
+    from tardiis_and_claritas_amd.batched import BatchedRunner, flops_per_history_model, total_counters
+
+    H = int(args.histories)
+    seed = workloads.SEED
+    launch = {'block': args.block, 'blocks_per_cu': args.blocks_per_cu, 'chunk': args.chunk}
+    runner = BatchedRunner(0, launch if any(launch.values()) else None)
+    specs = []
+    for c in workloads.CONFIG2_CONCENTRATIONS_MG_L:
+        kw, mat, conc, _, _ = workloads.config2(c, 'plain_mie')
+        specs.append((kw, mat, conc))
+    runner.prepare(specs)
+    budgets = [H] * len(specs)
+    print(f'bench: config-2 sweep, {len(specs)} concentrations x {H:.3g} histories per step', file=sys.stderr,
+          flush=True)
+    for w in range(args.warmup):
+        runner.run_prepared(budgets, seed + 1000 * (w + 1))
+        print(f'bench: warmup {w + 1}/{args.warmup} done', file=sys.stderr, flush=True)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is only used for the device-wide barrier
+        sync = lambda: None
+    sync()
+    t0 = time.perf_counter()
+    kernel_ms, flops, totals = [], 0.0, None
+    for k in range(args.steps):
+        T = runner.run_prepared(budgets, seed + k)
+        kernel_ms.append(runner.last_ms)
+        cnt = total_counters(T)
+        flops += flops_per_history_model(cnt)
+        totals = cnt if totals is None else totals + cnt
+        print(f'bench: step {k + 1}/{args.steps}: {runner.last_ms:.1f} ms', file=sys.stderr, flush=True)
+    sync()
+    elapsed = time.perf_counter() - t0
+    n_hist = args.steps * H * len(specs)
+    value = n_hist / elapsed
+    avg_ms = float(np.mean(kernel_ms))
+    achieved = flops / args.steps / (avg_ms * 1e-3) / 1e12
+    out = {
+        'metric': 'photon histories/s (1 MI355X) + surrogate-vs-MC speedup at <=1% NTU error',
+        'value': value,
+        'unit': 'photon histories/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': elapsed / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32',
+        'data': 'synthetic (seeded Philox photon histories; analytic Rouse field; lognormal PSD)',
+        'config': {
+            'workload': 'config2: lognormal PSD 40 bins (median 20 um, sigma_g 1.8), Rouse r-z FP32 64x128, '
+                        'Rankine surface (core 3 mm, depth 5 mm), 860 nm, plain-Mie event law, '
+                        '7 concentrations 10-5000 mg/L',
+            'histories_per_concentration_per_step': H,
+            'histories_per_step': H * len(specs),
+            'launch': 'one persistent batched launch per step (hipGraph)',
+        },
+        'roofline': {
+            'bound': 'mfma',
+            'achieved': achieved,
+            'peak': FP32_PEAK_TFLOPS,
+            'unit': 'TFLOP/s',
+            'frac': achieved / FP32_PEAK_TFLOPS,
+            'traffic': None,
+            'kernel': 'transport_kernel<true> (plain-Mie persistent kernel)',
+            'kernel_ms_avg': avg_ms,
+            'flop_model': 'F = 16 + 30 N_flight + 16 N_null + 60 N_scat + 40 N_iface + 10 N_abs + 45 N_exit '
+                          'per history (FP32; FMA=2); peak is the FP32 figure, issued on the VALU',
+            'flops_per_history': flops / n_hist,
+        },
+    }
+    if not args.no_cpu:
+        per = args.cpu_histories or max(20000, int(2e6))
+        out['cpu_baseline'] = cpu_baseline(per, seed)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

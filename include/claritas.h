@@ -1,0 +1,359 @@
+/*
+ * claritas.h -- C ABI of libclaritas_hip.so, the gfx950 photon-transport
+ * library for the CLARITAS sample cell.
+ *
+ * The library replaces the device side of CLARITAS's forward model: the
+ * CuPy RawModule kernels that `ClaritasForwardModel` compiles from CUDA_SRC
+ * (CLARITAS/CLARITAS.py:6000-6034) and launches from
+ * `_prepare_gpu_particle_physics` (CLARITAS/CLARITAS.py:6036-6965) and from the
+ * chunk loop of `simulate` (CLARITAS/CLARITAS.py:7614-7848).  The Python
+ * package `tardiis_and_claritas_amd` binds these entry points with ctypes and
+ * keeps the reference's `simulate(...) -> ForwardResult` surface on top.
+ *
+ * Conventions
+ *  - Plain C types only: fixed-width integers, doubles, floats, pointers.
+ *  - Every entry point returns 0 on success.  Positive codes reuse the
+ *    reference's precompute error numbering (1 Mie order cap, 2 invalid Mie
+ *    input, 3 phase CDF failure, 4/5 transport precompute, 31-35 sediment,
+ *    see CLARITAS/CLARITAS.py:3505,3515,3590,3868,3877,3958-4037);
+ *    negative codes are API/HIP errors (CLR_E_*).  clr_last_error() gives text.
+ *  - The library owns every device buffer inside a clr_ctx; the caller owns
+ *    the host structs and output arrays it passes in.
+ *  - Calls on one context are synchronous and must not overlap.
+ *  - Per-history failures never raise: they are counted by production
+ *    failure code (CLARITAS/CLARITAS.py:1846-1866) in clr_tallies.failures.
+ */
+#ifndef CLARITAS_H
+#define CLARITAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLR_ABI_VERSION 1
+
+#define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
+#define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
+#define CLR_MAX_AUX 4          /* extra hardware bores at arbitrary angles */
+#define CLR_N_COUNTERS 64
+#define CLR_N_ICHIST 64        /* interaction-order histogram, last bin = overflow */
+#define CLR_N_STATUS 9         /* status 0..8, CLARITAS.py:4678-5068 */
+#define CLR_N_FAILURE 19       /* PROD_* codes 0..18, CLARITAS.py:1846-1866 */
+
+/* error codes (negative) */
+#define CLR_E_ARG      (-1)
+#define CLR_E_HIP      (-2)
+#define CLR_E_NOMEM    (-3)
+#define CLR_E_STATE    (-4)
+#define CLR_E_RANGE    (-5)
+#define CLR_E_UNSUPPORTED (-6)
+
+/* ---- model codes (values identical to CLARITAS.py:5543-5575) ---- */
+/* event law, CLARITAS.py:5543-5550; CLR_EVENT_PLAIN_MIE has no reference code */
+#define CLR_EVENT_GEOMETRIC 0
+#define CLR_EVENT_MIE_QSCA 1
+#define CLR_EVENT_MIE_QEXT 2
+#define CLR_EVENT_HYBRID_EXCESS 3
+#define CLR_EVENT_HYBRID_EXACT_RATE 4
+#define CLR_EVENT_PLAIN_MIE 6
+/* source angular law, CLARITAS.py:5553-5557; 3 = collimated uniform disk */
+#define CLR_SOURCE_LEGACY_POLAR 0
+#define CLR_SOURCE_BETA_RADIANCE 1
+#define CLR_SOURCE_THESIS_PLANAR 2
+#define CLR_SOURCE_COLLIMATED_DISK 3
+/* free surface, CLARITAS.py:5565-5569; 3 = volume-preserving Rankine vortex */
+#define CLR_SURFACE_FLAT 0
+#define CLR_SURFACE_PARABOLIC 1
+#define CLR_SURFACE_SCULLY 2
+#define CLR_SURFACE_RANKINE 3
+/* sediment field, CLARITAS.py:5570-5575.  3 = per-bin r-z grid supplied by
+ * the caller (the reference fills it with its meridional solver);
+ * 4 = one FP32 r-z scale field shared by every bin (e.g. a Rouse profile);
+ * 5 = one FP32 Cartesian x-y-z scale field shared by every bin. */
+#define CLR_SED_UNIFORM 0
+#define CLR_SED_STOKES 1
+#define CLR_SED_SCHILLER 2
+#define CLR_SED_GRID_PER_BIN 3
+#define CLR_SED_RZ_FIELD 4
+#define CLR_SED_XYZ_FIELD 5
+
+/* ---- counters reported per configuration (clr_tallies.counters) ----
+ * Sums over histories that entered the water unless stated.  These are the
+ * per-ray integer arrays the reference reduces on the host
+ * (CLARITAS.py:8665-8733), reduced on the device instead. */
+enum {
+  CLR_C_HISTORIES = 0,     /* histories launched (all) */
+  CLR_C_ENTERED,           /* entered the water */
+  CLR_C_VALID_AIR,         /* status 1 with finite exit state */
+  CLR_C_SUM_IC,            /* particle interactions */
+  CLR_C_SUM_GIC,           /* geometric-channel interactions */
+  CLR_C_SUM_WIC,           /* wave-channel interactions */
+  CLR_C_SUM_WCOH,          /* coherent Mie wave interactions */
+  CLR_C_BALLISTIC,         /* entered with zero interactions */
+  CLR_C_SUM_PR,            /* particle Fresnel reflections */
+  CLR_C_RAYS_PER,          /* histories with >=1 entry reflection */
+  CLR_C_RAYS_PIR,          /* histories with >=1 internal reflection */
+  CLR_C_RAYS_PABS,         /* histories absorbed inside a particle */
+  CLR_C_SUM_CIR,           /* cell inner-wall reflections */
+  CLR_C_SUM_COR,           /* cell outer-wall reflections */
+  CLR_C_RAYS_CTIR,         /* histories with >=1 cell TIR */
+  CLR_C_SUM_TOPR,          /* free-surface reflections */
+  CLR_C_RAYS_TOPTIR,       /* histories with >=1 free-surface TIR */
+  CLR_C_SUM_BIR,           /* bottom inner reflections */
+  CLR_C_SUM_BOR,           /* bottom outer reflections */
+  CLR_C_RAYS_BTIR,         /* histories with >=1 bottom TIR */
+  /* algorithmic work counters over all histories (FLOP model) */
+  CLR_C_N_FLIGHT,          /* free-flight segments in water */
+  CLR_C_N_NULL,            /* Woodcock null collisions */
+  CLR_C_N_SCAT,            /* real particle events */
+  CLR_C_N_IFACE,           /* Fresnel interface decisions */
+  CLR_C_N_ABS,             /* absorption roulette samples */
+  CLR_C_N_EXIT,            /* scored air exits */
+  CLR_C_COND_TRIALS,       /* conditional-closure candidate trials */
+  CLR_C_COND_REJECT,       /* conditional outcome rejections */
+  CLR_C_COND_INACCESSIBLE, /* inaccessible particle placements */
+  CLR_C_RNG_DRAWS,         /* 32-bit Philox words consumed */
+  CLR_C_N_STEPS,           /* state-machine steps (device) or loop passes */
+  CLR_C_COUNT_
+};
+
+/* ---- configuration: the scalar parameter set of trace_kernel
+ * (CLARITAS.py:4551-4605) in sensor-plane coordinates (metres).
+ * Derivation from user-facing kwargs follows CLARITAS.py:5660-5680,10024-10052. */
+typedef struct clr_config {
+  double n_water, n_acrylic, n_air, bottom_external_n;
+  double rin, rout, zmin, zmean, zwall_top, bottom_disc_thickness;
+  double ring_in, throat_out, ring_out, source_r, throat_r, counterbore_r;
+  double vortex_delta_h, vortex_core_radius;
+  double alpha1, alpha2, beam_sigma, beam_radius;
+  double legacy_accept_deg;
+  double wavelength;            /* vacuum wavelength (m) */
+  double rough_alpha;           /* particle Beckmann alpha; only 0 supported */
+  double aux_angles_deg[CLR_MAX_AUX];
+  double field_zmax;            /* top of the sediment grid (m) */
+  double field_xy_half;         /* half-width of the Cartesian field (m) */
+  int32_t free_surface_model;
+  int32_t source_angular_model;
+  int32_t event_model;
+  int32_t sediment_model;
+  int32_t max_particle_bounces; /* max_internal_bounces */
+  int32_t max_cell_bounces;
+  int32_t max_iterations;
+  int32_t conditional_max_trials;
+  int32_t n_aux;
+  int32_t n_groups;             /* jackknife groups, 1..CLR_MAX_GROUPS */
+  int32_t field_n0, field_n1, field_n2; /* r-z: (nr,nz,-); xyz: (nx,ny,nz) */
+  int32_t field_nbins;          /* 1 or n_bins (CLR_SED_GRID_PER_BIN) */
+} clr_config;
+
+/* ---- particle population (one PSD, CLARITAS.py:6036-6060) ---- */
+typedef struct clr_particles {
+  const double* diameter;       /* [n] m */
+  const double* weight;         /* [n] raw PSD weights (normalised inside) */
+  const double* n_real;         /* [n] */
+  const double* k_imag;         /* [n] */
+  double concentration;         /* kg/m^3 (= g/L) */
+  double density;               /* particle density kg/m^3 */
+  double n_medium;
+  double wavelength;            /* vacuum wavelength, m */
+  double hybrid_wave_scale;
+  double wave_rms_height_m;
+  int32_t n;
+  int32_t n_theta;              /* Mie phase grid size */
+  int32_t weight_mode;          /* 0 mass fraction, 1 number fraction */
+  int32_t event_model;
+} clr_particles;
+
+/* ---- sediment transport inputs (CLARITAS.py:3940-3948) ---- */
+typedef struct clr_sediment {
+  double particle_density, water_density, water_viscosity, gravity;
+  double eddy_diffusivity;
+  int32_t n_quad, n_radial;
+} clr_sediment;
+
+/* ---- Mie outputs (CLARITAS.py:3495-3592); every array is caller-owned ---- */
+typedef struct clr_mie_out {
+  double* size_parameter;       /* [n] */
+  double* qext;                 /* [n] */
+  double* qsca;                 /* [n] */
+  double* qabs;                 /* [n] */
+  double* qback;                /* [n] */
+  double* asym_g;               /* [n] */
+  double* albedo;               /* [n] */
+  int32_t* nstop;               /* [n] */
+  double* theta;                /* [n_theta] or NULL */
+  double* phase_pdf;            /* [n*n_theta] or NULL */
+  double* phase_cdf;            /* [n*n_theta] or NULL */
+} clr_mie_out;
+
+/* ---- host copy of the prepared transport tables ---- */
+typedef struct clr_tables_out {
+  double* radius;               /* [n] */
+  double* number_density;       /* [n] */
+  double* mu_geom_bin;          /* [n] physical projected-area rate */
+  double* mu_sca_bin;           /* [n] */
+  double* mu_ext_bin;           /* [n] */
+  double* trace_geom_bin;       /* [n] traced geometric-channel rate */
+  double* trace_wave_bin;       /* [n] traced wave-channel rate */
+  double* geom_cdf;             /* [n] */
+  double* wave_cdf;             /* [n] */
+  double* sed_A;                /* [n] */
+  double* sed_B;                /* [n] */
+  double* sed_norm;             /* [n] */
+  double* sed_settling;         /* [n] settling velocity */
+  double* sed_radial_log;       /* [n*n_radial] or NULL */
+  double mu_scalars[8];         /* CLARITAS.py:3911-3913 layout */
+  double majorants[4];          /* CLARITAS.py:4035 layout */
+} clr_tables_out;
+
+/* ---- tallies for one configuration ----
+ * det[g][k][c]: k = 0 hardware native, 1 hardware mirror, 2 legacy native,
+ * 3 legacy mirror; channel c = detector at 10*c degrees (CLARITAS.py:4378-4436).
+ * aux[g][k][a]: k = 0 native, 1 mirror, extra bore at aux_angles_deg[a]. */
+typedef struct clr_tallies {
+  uint64_t det[CLR_MAX_GROUPS][4][CLR_N_CHANNELS];
+  uint64_t aux[CLR_MAX_GROUPS][2][CLR_MAX_AUX];
+  uint64_t group_histories[CLR_MAX_GROUPS];
+  double incidence_sum[2][CLR_N_CHANNELS];   /* degrees, native / mirror */
+  uint64_t status[CLR_N_STATUS];
+  uint64_t failures[CLR_N_FAILURE];
+  uint64_t counters[CLR_N_COUNTERS];
+  uint64_t ic_hist[CLR_N_ICHIST];
+  int32_t n_groups;
+  int32_t pad_;
+} clr_tallies;
+
+/* ---- per-history record, written for the first n_records histories
+ * when requested (history-by-history parity tests) ---- */
+typedef struct clr_record {
+  float x, y, z, vx, vy, vz;    /* exit state (status 1) or last state */
+  int32_t status;
+  int32_t native_id, mirror_id;
+  int32_t interactions;
+  int32_t failure;
+  int32_t draws;                /* Philox words consumed */
+} clr_record;
+
+/* ---- one item of a batched launch ---- */
+typedef struct clr_batch_item {
+  int32_t config;               /* index into the prepared configurations */
+  int32_t pad_;
+  uint64_t n_histories;
+  uint64_t seed;
+  uint64_t history_offset;
+} clr_batch_item;
+
+/* ---- launch shape (autotuned or fixed) ---- */
+typedef struct clr_launch {
+  int32_t block;                /* threads per workgroup: 64..1024 */
+  int32_t blocks_per_cu;        /* resident workgroups per CU (grid sizing) */
+  int32_t chunk;                /* histories per work-queue item */
+  int32_t pad_;
+} clr_launch;
+
+struct clr_ctx;
+typedef struct clr_ctx clr_ctx;
+
+/* ABI identification and struct sizes (arity test, cf. the reference's
+ * launch-arity check validate_claritas.py:972-1001). id: 0 config,
+ * 1 particles, 2 sediment, 3 mie_out, 4 tables_out, 5 tallies, 6 record,
+ * 7 batch_item, 8 launch. Returns 0 for an unknown id. */
+int clr_abi_version(void);
+size_t clr_sizeof(int struct_id);
+
+/* Context lifetime.  Replaces the CuPy device selection and NVRTC compile
+ * of ClaritasForwardModel.__init__ (CLARITAS.py:5653-5656, 6000-6034). */
+int clr_create(clr_ctx** out, int device);
+void clr_destroy(clr_ctx* ctx);
+const char* clr_last_error(const clr_ctx* ctx);
+/* Fill *info with: [0] CU count, [1] wavefront size, [2] LDS bytes per
+ * workgroup, [3] clock kHz, [4] gfx major*100+minor. */
+int clr_device_info(clr_ctx* ctx, int64_t info[8]);
+
+/* Mie coefficients, efficiencies, theta grid, phase PDF and CDF in FP64 on
+ * the device.  Replaces mie_coefficients_efficiencies_kernel,
+ * build_theta_grid_kernel, mie_phase_pdf_kernel and mie_phase_cdf_kernel
+ * (CLARITAS.py:3488-3592) as launched at CLARITAS.py:6186-6245, without the
+ * reference's MIE_MAX_ORDER=4096 cap (order is sized per call). */
+int clr_mie_tables(clr_ctx* ctx, const clr_particles* p, clr_mie_out* out);
+
+/* Full precompute for configuration slot `slot` (0 for single runs):
+ * Mie tables, transport split (transport_precompute_kernel,
+ * CLARITAS.py:3845-3921), sediment coefficients
+ * (sediment_transport_precompute_kernel, CLARITAS.py:3940-4038) and the
+ * device tables the transport kernel reads.  `field` is the optional FP32
+ * sediment scale grid for CLR_SED_GRID_PER_BIN / RZ / XYZ models, laid out
+ * [bin][n0][n1] (r-z) or [n0][n1][n2] (x-y-z), z fastest.
+ * Replaces _prepare_gpu_particle_physics (CLARITAS.py:6036-6965). */
+int clr_prepare(clr_ctx* ctx, int slot, const clr_config* cfg,
+                const clr_particles* p, const clr_sediment* sed,
+                const float* field, size_t field_len);
+/* Number of prepared configuration slots the context holds. */
+int clr_n_slots(clr_ctx* ctx);
+/* Copy the prepared tables of `slot` to host arrays (NULL fields skipped). */
+int clr_get_tables(clr_ctx* ctx, int slot, clr_tables_out* out);
+
+/* Trace n histories of slot 0 with Philox key `seed`, histories numbered
+ * history_offset .. history_offset+n-1 (64-bit; the reference's uint32 ray
+ * ids end at 2^32, CLARITAS.py:7191), and add detector tallies into *out
+ * (which the call zeroes first).  Replaces the trace_kernel +
+ * score_detectors_kernel chunk loop and its tally reduction
+ * (CLARITAS.py:7614-7864, 8773-8817).  `records` (optional) receives the
+ * first n_records per-history records. `launch` may be NULL (defaults). */
+int clr_run(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offset,
+            const clr_launch* launch, clr_tallies* out,
+            clr_record* records, uint64_t n_records);
+
+/* Batched launch: every item traced in one persistent kernel fed from a
+ * device work queue, captured once in a hipGraph and replayed.  out[i]
+ * receives the tallies of items[i].  No reference counterpart: the
+ * reference runs one configuration per simulate() call. */
+int clr_run_batch(clr_ctx* ctx, const clr_batch_item* items, int n_items,
+                  const clr_launch* launch, clr_tallies* out);
+
+/* Time one launch shape: trace n histories of slot 0 and return device
+ * milliseconds in *ms (HIP events on the library stream). */
+int clr_time_run(clr_ctx* ctx, uint64_t n, uint64_t seed,
+                 const clr_launch* launch, int repeats, double* ms);
+
+/* Average duration (ms) of the last transport-kernel launch, measured with
+ * HIP events on the stream the kernel runs on. */
+double clr_last_kernel_ms(clr_ctx* ctx);
+
+/* ---- surrogate: ensemble of MLPs, BF16 weights, FP32 accumulate on MFMA.
+ * No reference counterpart.  Layout: n_members networks of
+ * in_dim -> hidden -> hidden -> hidden -> out_dim with GELU (tanh form). */
+typedef struct clr_mlp_shape {
+  int32_t n_members, in_dim, hidden, out_dim, n_hidden_layers, pad_;
+} clr_mlp_shape;
+
+/* Create/replace the ensemble, weights initialised from `seed`. */
+int clr_mlp_init(clr_ctx* ctx, const clr_mlp_shape* shape, uint64_t seed);
+/* Copy master FP32 parameters out / in (flat, member-major, per layer W then b). */
+size_t clr_mlp_param_count(clr_ctx* ctx);
+int clr_mlp_get_params(clr_ctx* ctx, float* out, size_t n);
+int clr_mlp_set_params(clr_ctx* ctx, const float* in, size_t n);
+/* Train: `epochs` passes over (x[n][in_dim], y[n][out_dim]) in minibatches
+ * of `batch`, Adam(lr, b1, b2, eps), MSE loss; each member sees its own
+ * shuffle (seed). Returns final mean loss per member in loss[n_members]. */
+int clr_mlp_train(clr_ctx* ctx, const float* x, const float* y, int64_t n,
+                  int32_t batch, int32_t epochs, float lr, uint64_t seed,
+                  float* loss);
+/* One forward+backward on a minibatch without an optimizer step; gradients
+ * of the mean MSE loss copied to grads (same layout as params). */
+int clr_mlp_grad(clr_ctx* ctx, const float* x, const float* y, int32_t n,
+                 float* loss, float* grads);
+/* Inference: mean[n][out_dim] and std[n][out_dim] over the members. */
+int clr_mlp_infer(clr_ctx* ctx, const float* x, int64_t n, float* mean,
+                  float* stdev);
+/* Device-resident inference timing: n queries, `repeats` launches; *ms is
+ * the average per launch (HIP events). */
+int clr_mlp_time_infer(clr_ctx* ctx, int64_t n, int repeats, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLARITAS_H */

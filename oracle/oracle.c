@@ -1,0 +1,2398 @@
+/*
+ * oracle.c -- CPU restatement of the CLARITAS photon-transport hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may build, load or
+ * run it, and never as the thing measured or shipped.  The product library
+ * (tardiis_and_claritas_amd/csrc) does not include or link any of this.
+ *
+ * Every function restates the algorithm of the CLARITAS/CLARITAS.py lines it
+ * cites (the reference's CUDA_SRC device code and its host driver).  Two
+ * deliberate differences, shared with the HIP library:
+ *   - random numbers: one counter-based Philox4x32-10 stream per history,
+ *     key = 64-bit seed, counter = (block, 0, history lo, history hi),
+ *     instead of the reference's three xorshift32 words per ray
+ *     (CLARITAS.py:1125-1133, 4612);
+ *   - 64-bit history numbers (the reference stops at 2^32, CLARITAS.py:7191).
+ * Supported physics: the reference-parity mode (smooth particles, rough_alpha
+ * = 0, so the Smith walk reduces to CLARITAS.py:2336-2356; coherent wave
+ * fraction 1, so spheroid morphology is unreachable), absorbing bores, ideal
+ * photodiodes, plus the extensions named in claritas.h (plain-Mie event law,
+ * collimated disk source, Rankine surface, shared FP32 sediment fields,
+ * auxiliary bores).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define INF_F 3.402823466e+38F
+#define PI_F 3.14159265358979323846f
+#define PI_D 3.141592653589793238462643383279502884
+#define EPS_F 2.0e-7f
+#define DEPS 2.220446049250313080847263336181640625e-16
+#define FEPS 1.1920928955078125e-7
+#define MIE_ORDER_CAP (1 << 22)
+
+/* production failure codes, CLARITAS.py:1846-1866 */
+enum {
+  PF_OK = 0, PF_MICRO_NUMERICAL = 1, PF_MICRO_WATCHDOG = 2, PF_MACRO_GEOMETRY = 3,
+  PF_MACRO_BOUNCE = 4, PF_ACRYLIC_NO_HIT = 5, PF_ACRYLIC_BOUNCE = 6, PF_BOTTOM_GEOMETRY = 7,
+  PF_BOTTOM_BOUNCE = 8, PF_WATER_NO_BOUNDARY = 9, PF_ITERATION_LIMIT = 10,
+  PF_NONFINITE_POSITION = 11, PF_NONFINITE_DIRECTION = 12, PF_INVALID_MEDIUM = 13,
+  PF_INVALID_EVENT = 14, PF_OTHER = 15, PF_SPHEROID = 16, PF_HEADSPACE_NO_HIT = 17,
+  PF_MAJORANT = 18
+};
+/* subcodes, CLARITAS.py:1868-1895 and micro codes CLARITAS.py:1810-1839 */
+enum {
+  SUB_NONE = 0, SUB_SPHERE_ENTRY_SIDE = 1001, SUB_SPHERE_CHORD = 1002, SUB_SPHERE_EXIT_SIDE = 1003,
+  SUB_SPHERE_INTERNAL_SIDE = 1004, SUB_SPHERE_BOUNCE_LIMIT = 1005,
+  SUB_ACRYLIC_NO_HIT = 2001, SUB_ACRYLIC_BOUNCE = 2002, SUB_BOTTOM_TOPOLOGY = 2104,
+  SUB_WATER_NO_BOUNDARY = 3001, SUB_ITERATION_LIMIT = 3002, SUB_NONFINITE_POSITION = 3003,
+  SUB_NONFINITE_DIRECTION = 3004, SUB_WATER_OUTSIDE = 3005, SUB_HEADSPACE_NO_HIT = 3006,
+  SUB_MAJORANT = 4001,
+  MF_REFLECTION_INVALID = 31, MF_REFRACTION_INVALID = 32, MF_SIDE_STATE = 33,
+  MF_NONFINITE_DIRECTION = 34, MF_FRESNEL_INVALID = 35
+};
+
+/* ======================================================================
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random
+ * numbers: as easy as 1, 2, 3").  Multipliers and Weyl constants are the
+ * published ones.
+ * ====================================================================== */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+typedef struct {
+  uint32_t key0, key1, h0, h1, blk, idx, draws;
+  uint32_t buf[4];
+} rng_t;
+
+static void rng_seed(rng_t* r, uint64_t seed, uint64_t hist) {
+  r->key0 = (uint32_t)seed; r->key1 = (uint32_t)(seed >> 32);
+  r->h0 = (uint32_t)hist; r->h1 = (uint32_t)(hist >> 32);
+  r->blk = 0; r->idx = 4; r->draws = 0;
+}
+static uint32_t rng_next(rng_t* r) {
+  if (r->idx >= 4) {
+    uint32_t c[4] = {r->blk, 0u, r->h0, r->h1}, k[2] = {r->key0, r->key1};
+    orc_philox4x32_10(c, k, r->buf);
+    r->blk++;
+    r->idx = 0;
+  }
+  r->draws++;
+  return r->buf[r->idx++];
+}
+/* float uniform on the open interval: (m + 1/2) 2^-23, m = top 23 bits */
+static float U(rng_t* r) { return ((float)(rng_next(r) >> 9) + 0.5f) * 1.1920928955078125e-7f; }
+/* 53-bit [0,1) roulette from two words, CLARITAS.py:4491-4495 */
+static double U53(rng_t* r) {
+  uint32_t a = rng_next(r), b = rng_next(r);
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 1.1102230246251565404236316680908203125e-16;
+}
+/* open (0,1) double from one word, CLARITAS.py:1908-1912 */
+static double Uopen(rng_t* r) { return ((double)rng_next(r) + 0.5) * 2.3283064365386962890625e-10; }
+
+void orc_uniform_stream(uint64_t seed, uint64_t history, int n, float* out) {
+  rng_t r; rng_seed(&r, seed, history);
+  for (int i = 0; i < n; ++i) out[i] = U(&r);
+}
+
+/* ======================================================================
+ * Source samplers, CLARITAS.py:1149-1212
+ * ====================================================================== */
+static float normal_bm(rng_t* r) {
+  float u1 = fmaxf(U(r), 1e-12f), u2 = U(r);
+  return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * PI_F * u2);
+}
+/* Marsaglia & Tsang (2000) gamma sampler with the a<1 boost, CLARITAS.py:1154-1175 */
+static float gamma_draw(rng_t* r, float shape) {
+  if (shape <= 0.0f) return 0.0f;
+  if (fabsf(shape - 1.0f) < 1e-7f) return -logf(fmaxf(U(r), 1e-12f));
+  float boost = 1.0f, a = shape;
+  if (a < 1.0f) {
+    float u = fmaxf(U(r), 1e-12f);
+    boost = powf(u, 1.0f / a);
+    a += 1.0f;
+  }
+  float d = a - 0.3333333333333333f, c = 1.0f / sqrtf(9.0f * d);
+  for (int it = 0; it < 128; ++it) {
+    float x = normal_bm(r), v = 1.0f + c * x;
+    if (v <= 0.0f) continue;
+    v = v * v * v;
+    float u = U(r);
+    if (u < 1.0f - 0.0331f * x * x * x * x) return boost * d * v;
+    if (logf(fmaxf(u, 1e-12f)) < 0.5f * x * x + d * (1.0f - v + logf(v))) return boost * d * v;
+  }
+  return boost * d;
+}
+/* CLARITAS.py:1177-1191 */
+static float beta_draw(rng_t* r, float a, float b) {
+  if (a <= 0.0f || b <= 0.0f) return 0.0f;
+  if (fabsf(a - 1.0f) < 1e-7f) {
+    float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+    return 1.0f - powf(1.0f - u, 1.0f / b);
+  }
+  if (fabsf(b - 1.0f) < 1e-7f) {
+    float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+    return powf(u, 1.0f / a);
+  }
+  float ga = gamma_draw(r, a), gb = gamma_draw(r, b), s = ga + gb;
+  return s > 0.0f ? ga / s : 0.0f;
+}
+/* radiance law with the sin(theta) Jacobian by rejection, CLARITAS.py:1193-1212 */
+static float beta_radiance_polar(rng_t* r, float a, float b) {
+  const float c = 0.5f * PI_F;
+  for (int it = 0; it < 128; ++it) {
+    float u = fminf(fmaxf(beta_draw(r, a + 1.0f, b), 0.0f), 1.0f);
+    float x = c * u, acc = (x < 1.0e-7f) ? 1.0f : sinf(x) / x;
+    if (U(r) <= acc) return x;
+  }
+  float u = beta_draw(r, a + 1.0f, b);
+  return c * fminf(fmaxf(u, 0.0f), 1.0f);
+}
+
+/* ======================================================================
+ * Vector optics, CLARITAS.py:1214-1260
+ * ====================================================================== */
+static inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+  return ax * bx + ay * by + az * bz;
+}
+static void normalize3(float* x, float* y, float* z) {
+  float q = (*x) * (*x) + (*y) * (*y) + (*z) * (*z);
+  if (q <= 0.0f) { *x = 0.0f; *y = 1.0f; *z = 0.0f; return; }
+  float s = 1.0f / sqrtf(q);
+  *x *= s; *y *= s; *z *= s;
+}
+static void reflect3(float ix, float iy, float iz, float nx, float ny, float nz, float* ox,
+                     float* oy, float* oz) {
+  float d = dot3(ix, iy, iz, nx, ny, nz);
+  *ox = ix - 2.0f * d * nx; *oy = iy - 2.0f * d * ny; *oz = iz - 2.0f * d * nz;
+  normalize3(ox, oy, oz);
+}
+/* n points into the incident medium */
+static int refract3(float ix, float iy, float iz, float nx, float ny, float nz, float n1, float n2,
+                    float* ox, float* oy, float* oz) {
+  float ci = fminf(fmaxf(-dot3(ix, iy, iz, nx, ny, nz), 0.0f), 1.0f);
+  float eta = n1 / n2, k = 1.0f - eta * eta * (1.0f - ci * ci);
+  if (k <= 0.0f) return 0;
+  float ct = sqrtf(k), f = eta * ci - ct;
+  *ox = eta * ix + f * nx; *oy = eta * iy + f * ny; *oz = eta * iz + f * nz;
+  normalize3(ox, oy, oz);
+  return 1;
+}
+/* unpolarised Fresnel reflectance */
+static float fresnel_R(float ci, float n1, float n2, int* tir) {
+  ci = fminf(fmaxf(ci, 0.0f), 1.0f);
+  float eta = n1 / n2, st2 = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+  if (st2 >= 1.0f) { *tir = 1; return 1.0f; }
+  *tir = 0;
+  float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+  float ds = n1 * ci + n2 * ct, dp = n1 * ct + n2 * ci, rs = 1.0f, rp = 1.0f;
+  if (fabsf(ds) > 1e-12f) { float a = (n1 * ci - n2 * ct) / ds; rs = a * a; }
+  if (fabsf(dp) > 1e-12f) { float a = (n1 * ct - n2 * ci) / dp; rp = a * a; }
+  return fminf(fmaxf(0.5f * (rs + rp), 0.0f), 1.0f);
+}
+
+/* ---- complex-index interface, CLARITAS.py:1376-1452 ---- */
+typedef struct { float re, im; } cf;
+static cf cmk(float a, float b) { cf z = {a, b}; return z; }
+static cf cfadd(cf a, cf b) { return cmk(a.re + b.re, a.im + b.im); }
+static cf cfsub(cf a, cf b) { return cmk(a.re - b.re, a.im - b.im); }
+static cf cfmul(cf a, cf b) { return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static cf cfdiv(cf a, cf b) {
+  float d = b.re * b.re + b.im * b.im;
+  if (d < 1e-30f) d = 1e-30f;
+  return cmk((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+static float cfabs2(cf a) { return a.re * a.re + a.im * a.im; }
+/* principal square root with non-negative real part */
+static cf cfsqrt(cf z) {
+  float r = sqrtf(z.re * z.re + z.im * z.im);
+  float u = sqrtf(fmaxf(0.0f, 0.5f * (r + z.re))), v = sqrtf(fmaxf(0.0f, 0.5f * (r - z.re)));
+  if (z.im < 0.0f) v = -v;
+  if (u < 0.0f || (fabsf(u) < 1e-20f && v < 0.0f)) { u = -u; v = -v; }
+  return cmk(u, v);
+}
+/* planar complex-index Fresnel reflectance and the s/p-averaged energy-flux
+ * direction of the transmitted wave (tangential, normal components) */
+static int complex_interface(float ci, float n1r, float n1k, float n2r, float n2k, float* R,
+                             float* t_tan, float* t_nrm) {
+  if (!(n1r > 0.f && n2r > 0.f && n1k >= 0.f && n2k >= 0.f)) return 0;
+  ci = fminf(fmaxf(ci, 0.f), 1.f);
+  float beta = n1r * sqrtf(fmaxf(0.f, 1.f - ci * ci));
+  cf n1 = cmk(n1r, n1k), n2 = cmk(n2r, n2k);
+  cf e1 = cfmul(n1, n1), e2 = cfmul(n2, n2), b2 = cmk(beta * beta, 0.f);
+  cf q1 = cfsqrt(cfsub(e1, b2)), q2 = cfsqrt(cfsub(e2, b2));
+  cf rs = cfdiv(cfsub(q1, q2), cfadd(q1, q2));
+  cf a = cfmul(e2, q1), b = cfmul(e1, q2);
+  cf rp = cfdiv(cfsub(a, b), cfadd(a, b));
+  float Rv = .5f * (cfabs2(rs) + cfabs2(rp));
+  if (!isfinite(Rv)) return 0;
+  cf qe = cfdiv(q2, e2), be = cfdiv(cmk(beta, 0.f), e2);
+  float Sx = beta, Sz = q2.re, Px = be.re, Pz = qe.re;
+  float sn = hypotf(Sx, Sz), pn = hypotf(Px, Pz);
+  if (!(sn > 0.f && pn > 0.f && Sz >= 0.f && Pz >= 0.f)) return 0;
+  float tx = .5f * (Sx / sn + Px / pn), tz = .5f * (Sz / sn + Pz / pn);
+  float nm = hypotf(tx, tz);
+  if (!(nm > 0.f && isfinite(nm))) return 0;
+  *R = fminf(fmaxf(Rv, 0.f), 1.f);
+  *t_tan = tx / nm;
+  *t_nrm = tz / nm;
+  return 1;
+}
+static int energy_refract3(float ix, float iy, float iz, float nx, float ny, float nz, float n1r,
+                           float n1k, float n2r, float n2k, float* ox, float* oy, float* oz) {
+  float ci = fminf(fmaxf(-dot3(ix, iy, iz, nx, ny, nz), 0.f), 1.f);
+  float R, ts, tn;
+  if (!complex_interface(ci, n1r, n1k, n2r, n2k, &R, &ts, &tn)) return 0;
+  float px = ix + ci * nx, py = iy + ci * ny, pz = iz + ci * nz;
+  float pn = sqrtf(px * px + py * py + pz * pz);
+  if (pn > 1.e-7f) { px /= pn; py /= pn; pz /= pn; }
+  else { px = py = pz = 0.f; ts = 0.f; tn = 1.f; }
+  *ox = ts * px - tn * nx; *oy = ts * py - tn * ny; *oz = ts * pz - tn * nz;
+  normalize3(ox, oy, oz);
+  return isfinite(*ox) && isfinite(*oy) && isfinite(*oz);
+}
+
+static void perpendicular_basis(float vx, float vy, float vz, float* e1x, float* e1y, float* e1z,
+                                float* e2x, float* e2y, float* e2z) {
+  float rx, ry, rz;
+  if (fabsf(vy) < 0.9f) { rx = 0.0f; ry = 1.0f; rz = 0.0f; } else { rx = 1.0f; ry = 0.0f; rz = 0.0f; }
+  *e1x = ry * vz - rz * vy; *e1y = rz * vx - rx * vz; *e1z = rx * vy - ry * vx;
+  normalize3(e1x, e1y, e1z);
+  *e2x = vy * (*e1z) - vz * (*e1y); *e2y = vz * (*e1x) - vx * (*e1z); *e2z = vx * (*e1y) - vy * (*e1x);
+  normalize3(e2x, e2y, e2z);
+}
+
+/* ======================================================================
+ * Model: float parameters as the transport kernel receives them
+ * ====================================================================== */
+typedef struct {
+  float n_water, n_acrylic, n_air, n_bottom_ext;
+  float rin, rout, zmin, zmean, zwall_top, bdt, zlow;
+  float ring_in, throat_out, ring_out, source_r, throat_r, cb_r;
+  float dh, core_r, alpha1, alpha2, sigma, beam_r, legacy_acc, wavelength;
+  float mu_geom, mu_wave, maj_geom, maj_wave, coherent;
+  float zs_min;                 /* lowest free-surface height */
+  double rk_K, rk_C0;           /* Rankine constants */
+  float field_zmax, field_half;
+  int fs_model, src_model, event_model, sed_model, max_pb, max_iter, cond_trials, n_aux, n_groups;
+  int fn0, fn1, fn2, fnbins;
+} fparams;
+
+struct orc_model {
+  clr_config cfg;
+  fparams fp;
+  int n, nt, nrad;
+  double *d, *w, *nr, *ki;
+  double *x, *qext, *qsca, *qabs, *qback, *g, *albedo;
+  int* nstop;
+  double *theta, *pdf, *cdf;
+  float *theta_f, *cdf_f;
+  int nt_sample;
+  double *radius, *nd, *mug, *mus, *mue, *muw, *muev, *gcdf, *wcdf;
+  float *gcdf_f, *wcdf_f;
+  double *trace_g, *trace_w;
+  double sum_trace_g, sum_trace_w;
+  double mu_scalars[8];
+  double *sedA, *sedB, *sednorm, *sedws, *radlog;
+  double maj[4];
+  float* field;
+  size_t field_len;
+  float det_ux[CLR_N_CHANNELS], det_uy[CLR_N_CHANNELS];
+  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX];
+  float n_f[512], k_f[512], r_f[512];
+};
+
+/* ======================================================================
+ * Mie, CLARITAS.py:3488-3592 (FP64)
+ * ====================================================================== */
+typedef struct { double re, im; } cd;
+static cd cdm(double a, double b) { cd z = {a, b}; return z; }
+static cd cdadd(cd a, cd b) { return cdm(a.re + b.re, a.im + b.im); }
+static cd cdsub(cd a, cd b) { return cdm(a.re - b.re, a.im - b.im); }
+static cd cdmul(cd a, cd b) { return cdm(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static cd cdscale(cd a, double s) { return cdm(a.re * s, a.im * s); }
+static cd cddiv(cd a, cd b) {
+  double d = b.re * b.re + b.im * b.im;
+  if (d < 1e-300) d = 1e-300;
+  return cdm((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+
+/* Lorenz-Mie coefficients a_n, b_n by downward logarithmic derivative and
+ * upward Riccati-Bessel recurrences (Bohren & Huffman 1983, BHMIE);
+ * efficiencies and g as CLARITAS.py:3495-3559.  a, b sized [nstop]. */
+static int mie_one(double diameter, double nre, double kim, double n_med, double wl, double* xo,
+                   int* nstop_o, double* q, cd** a_out, cd** b_out) {
+  double x = PI_D * n_med * diameter / wl;
+  *xo = x;
+  if (!(x > 0.0) || !(nre > 0.0) || !(n_med > 0.0) || !(wl > 0.0)) {
+    *nstop_o = 0; q[0] = q[1] = q[2] = q[3] = q[4] = 0.0; q[5] = 1.0;
+    return 2;
+  }
+  double mr = nre / n_med, mi = kim / n_med;
+  int nstop = (int)(x + 4.0 * cbrt(x) + 2.0);
+  if (nstop < 1) nstop = 1;
+  double mz = hypot(mr * x, mi * x);
+  int nmx = (int)fmax((double)nstop, mz) + 128;
+  if (nmx < nstop + 1) nmx = nstop + 1;
+  if (nmx >= MIE_ORDER_CAP) { *nstop_o = 0; return 1; }
+  *nstop_o = nstop;
+  cd* D = (cd*)malloc(sizeof(cd) * (size_t)(nmx + 1));
+  cd* A = (cd*)malloc(sizeof(cd) * (size_t)nstop);
+  cd* B = (cd*)malloc(sizeof(cd) * (size_t)nstop);
+  D[nmx] = cdm(0.0, 0.0);
+  cd z = cdm(mr * x, mi * x), one = cdm(1.0, 0.0);
+  for (int n = nmx; n >= 1; --n) {
+    cd nz = cddiv(cdm((double)n, 0.0), z);
+    D[n - 1] = cdsub(nz, cddiv(one, cdadd(D[n], nz)));
+  }
+  double psi_m1 = sin(x), psi = sin(x) / x - cos(x);
+  double chi_m1 = cos(x), chi = cos(x) / x + sin(x);
+  cd m = cdm(mr, mi);
+  for (int j = 0; j < nstop; ++j) {
+    int n = j + 1;
+    cd xi_m1 = cdm(psi_m1, -chi_m1), xi = cdm(psi, -chi);
+    cd ca = cdadd(cddiv(D[n], m), cdm((double)n / x, 0.0));
+    cd cb = cdadd(cdmul(m, D[n]), cdm((double)n / x, 0.0));
+    A[j] = cddiv(cdsub(cdscale(ca, psi), cdm(psi_m1, 0.0)), cdsub(cdmul(ca, xi), xi_m1));
+    B[j] = cddiv(cdsub(cdscale(cb, psi), cdm(psi_m1, 0.0)), cdsub(cdmul(cb, xi), xi_m1));
+    double psi_p1 = ((2.0 * n + 1.0) / x) * psi - psi_m1;
+    double chi_p1 = ((2.0 * n + 1.0) / x) * chi - chi_m1;
+    psi_m1 = psi; psi = psi_p1; chi_m1 = chi; chi = chi_p1;
+  }
+  free(D);
+  double se = 0, ss = 0, sbr = 0, sbi = 0, t1 = 0, t2 = 0;
+  for (int j = 0; j < nstop; ++j) {
+    int n = j + 1;
+    double cn = 2.0 * n + 1.0;
+    cd aa = A[j], bb = B[j];
+    se += cn * (aa.re + bb.re);
+    ss += cn * (aa.re * aa.re + aa.im * aa.im + bb.re * bb.re + bb.im * bb.im);
+    double sg = (n & 1) ? -1.0 : 1.0;
+    sbr += cn * sg * (aa.re - bb.re);
+    sbi += cn * sg * (aa.im - bb.im);
+    t2 += (cn / ((double)n * (n + 1.0))) * (aa.re * bb.re + aa.im * bb.im);
+    if (j + 1 < nstop) {
+      cd an = A[j + 1], bn = B[j + 1];
+      t1 += ((double)n * (n + 2.0) / (n + 1.0)) *
+            ((aa.re * an.re + aa.im * an.im) + (bb.re * bn.re + bb.im * bn.im));
+    }
+  }
+  double x2 = x * x, qe = 2.0 * se / x2, qs = 2.0 * ss / x2, qb = (sbr * sbr + sbi * sbi) / x2;
+  if (qe < 0.0 && qe > -1e-12) qe = 0.0;
+  if (qs < 0.0 && qs > -1e-12) qs = 0.0;
+  double qa = qe - qs;
+  if (qa < 0.0 && qa > -1e-10) qa = 0.0;
+  double gg = qs > 0.0 ? 4.0 * (t1 + t2) / (x2 * qs) : 0.0;
+  if (gg > 1.0) gg = 1.0;
+  if (gg < -1.0) gg = -1.0;
+  q[0] = qe; q[1] = qs; q[2] = qa; q[3] = qb; q[4] = gg; q[5] = qe > 0.0 ? qs / qe : 1.0;
+  if (a_out) *a_out = A; else free(A);
+  if (b_out) *b_out = B; else free(B);
+  return 0;
+}
+
+/* theta_j = (pi/2)(1 - cos(pi j/(N-1))), CLARITAS.py:3488-3493 */
+static void theta_grid(double* th, int nt) {
+  for (int j = 0; j < nt; ++j) {
+    if (j == 0) th[j] = 0.0;
+    else if (j == nt - 1) th[j] = PI_D;
+    else th[j] = 0.5 * PI_D * (1.0 - cos(PI_D * ((double)j / (double)(nt - 1))));
+  }
+}
+/* phase PDF 0.5(|S1|^2+|S2|^2) sin(theta) by pi_n/tau_n recurrences, CLARITAS.py:3561-3581 */
+static void phase_pdf(const cd* A, const cd* B, int nstop, const double* th, int nt, double* pdf) {
+  for (int j = 0; j < nt; ++j) {
+    double mu = cos(th[j]), s1r = 0, s1i = 0, s2r = 0, s2i = 0, pm1 = 0.0, pn = 1.0;
+    for (int k = 0; k < nstop; ++k) {
+      double nf = (double)(k + 1);
+      double tau = nf * mu * pn - (nf + 1.0) * pm1;
+      double c = (2.0 * nf + 1.0) / (nf * (nf + 1.0));
+      s1r += c * (A[k].re * pn + B[k].re * tau); s1i += c * (A[k].im * pn + B[k].im * tau);
+      s2r += c * (A[k].re * tau + B[k].re * pn); s2i += c * (A[k].im * tau + B[k].im * pn);
+      double pp1 = ((2.0 * nf + 1.0) / nf) * mu * pn - ((nf + 1.0) / nf) * pm1;
+      pm1 = pn; pn = pp1;
+    }
+    pdf[j] = 0.5 * (s1r * s1r + s1i * s1i + s2r * s2r + s2i * s2i) * sin(th[j]);
+  }
+}
+/* trapezoid CDF, normalised, monotone, last = 1; CLARITAS.py:3583-3592 */
+static int phase_cdf(const double* th, const double* pdf, int nt, double* cdf) {
+  cdf[0] = 0.0;
+  double s = 0.0;
+  for (int j = 1; j < nt; ++j) {
+    double area = 0.5 * (pdf[j] + pdf[j - 1]) * (th[j] - th[j - 1]);
+    if (area < 0.0 && area > -1e-18) area = 0.0;
+    s += area;
+    cdf[j] = s;
+  }
+  if (!(s > 0.0) || !isfinite(s)) {
+    for (int j = 0; j < nt; ++j) cdf[j] = (double)j / (double)(nt - 1);
+    return 3;
+  }
+  double prev = 0.0;
+  for (int j = 0; j < nt; ++j) {
+    double v = cdf[j] / s;
+    if (v < prev) v = prev;
+    if (v > 1.0) v = 1.0;
+    cdf[j] = v;
+    prev = v;
+  }
+  cdf[nt - 1] = 1.0;
+  return 0;
+}
+
+int orc_mie(const clr_particles* p, clr_mie_out* o) {
+  if (!p || !o || p->n <= 0) return CLR_E_ARG;
+  int nt = p->n_theta, err = 0;
+  if (o->theta && nt >= 2) theta_grid(o->theta, nt);
+  for (int i = 0; i < p->n; ++i) {
+    double q[6], x;
+    int nstop = 0;
+    cd *A = NULL, *B = NULL;
+    int e = mie_one(p->diameter[i], p->n_real[i], p->k_imag[i], p->n_medium, p->wavelength, &x,
+                    &nstop, q, &A, &B);
+    if (e) { err = err ? err : e; continue; }
+    if (o->size_parameter) o->size_parameter[i] = x;
+    if (o->qext) o->qext[i] = q[0];
+    if (o->qsca) o->qsca[i] = q[1];
+    if (o->qabs) o->qabs[i] = q[2];
+    if (o->qback) o->qback[i] = q[3];
+    if (o->asym_g) o->asym_g[i] = q[4];
+    if (o->albedo) o->albedo[i] = q[5];
+    if (o->nstop) o->nstop[i] = nstop;
+    if (o->theta && o->phase_pdf && nt >= 2) {
+      phase_pdf(A, B, nstop, o->theta, nt, o->phase_pdf + (size_t)i * nt);
+      if (o->phase_cdf) {
+        int ce = phase_cdf(o->theta, o->phase_pdf + (size_t)i * nt, nt, o->phase_cdf + (size_t)i * nt);
+        if (ce && !err) err = ce;
+      }
+    }
+    free(A);
+    free(B);
+  }
+  return err;
+}
+
+/* ======================================================================
+ * Transport split, CLARITAS.py:3845-3921, with the trace-channel rates
+ * chosen as in CLARITAS.py:6368-6384.  The plain-Mie law (code 6) traces
+ * mu_ext in one channel.
+ * ====================================================================== */
+static int transport_precompute(orc_model* m, const clr_particles* p) {
+  int n = m->n, em = p->event_model;
+  double sw = 0.0;
+  for (int i = 0; i < n; ++i) sw += p->weight[i];
+  if (!(sw > 0.0)) return 4;
+  double* sg = (double*)calloc((size_t)n, sizeof(double));
+  double* ss = (double*)calloc((size_t)n, sizeof(double));
+  double* se = (double*)calloc((size_t)n, sizeof(double));
+  double* swv = (double*)calloc((size_t)n, sizeof(double));
+  double* mass = (double*)calloc((size_t)n, sizeof(double));
+  double* srcw = (double*)calloc((size_t)n, sizeof(double));
+  double avg_mass = 0.0;
+  int err = 0;
+  for (int i = 0; i < n; ++i) {
+    double r = 0.5 * p->diameter[i];
+    m->radius[i] = r;
+    srcw[i] = p->weight[i] / sw;
+    mass[i] = p->density * (4.0 / 3.0) * PI_D * r * r * r;
+    avg_mass += srcw[i] * mass[i];
+  }
+  for (int i = 0; i < n && !err; ++i) {
+    double nd = (p->weight_mode == 0) ? p->concentration * srcw[i] / mass[i]
+                                      : (avg_mass > 0.0 ? p->concentration / avg_mass : 0.0) * srcw[i];
+    m->nd[i] = nd;
+    double r = m->radius[i];
+    double g0 = PI_D * r * r, s0 = fmax(m->qsca[i], 0.0) * g0, e0 = fmax(m->qext[i], 0.0) * g0;
+    double tol = 1e-10 * fmax(1.0, fmax(fabs(m->qext[i]), fabs(m->qsca[i])));
+    double k = p->k_imag[i];
+    if (!isfinite(k) || k < 0.0 || !isfinite(m->qext[i]) || !isfinite(m->qsca[i]) ||
+        m->qext[i] < -tol || m->qsca[i] < -tol || m->qsca[i] > m->qext[i] + tol ||
+        (k == 0.0 && fabs(m->qext[i] - m->qsca[i]) > tol)) {
+      err = 5;
+      break;
+    }
+    double w = 0.0;
+    if (em == 3) w = p->hybrid_wave_scale * fmax(m->qsca[i] - 1.0, 0.0) * g0;
+    else if (em == 4) {
+      double rate = (k > 0.0) ? e0 : s0;
+      w = (k > 0.0) ? ((s0 > 0.0) ? rate * fmax(s0 - g0, 0.0) / s0 : 0.0) : fmax(s0 - g0, 0.0);
+    }
+    sg[i] = g0; ss[i] = s0; se[i] = e0; swv[i] = w;
+    m->mug[i] = nd * g0; m->mus[i] = nd * s0; m->mue[i] = nd * e0; m->muw[i] = nd * w;
+  }
+  if (!err) {
+    double mg = 0, ms = 0, me = 0, mw = 0, mt = 0;
+    for (int i = 0; i < n; ++i) { mg += m->mug[i]; ms += m->mus[i]; me += m->mue[i]; mw += m->muw[i]; }
+    for (int i = 0; i < n; ++i) {
+      double mb;
+      if (em == 0) mb = m->mug[i];
+      else if (em == 1) mb = m->mus[i];
+      else if (em == 2 || em == 6) mb = m->mue[i];
+      else if (em == 3) mb = m->mug[i] + m->muw[i];
+      else mb = (p->k_imag[i] > 0.0) ? m->mue[i] : m->mus[i];
+      m->muev[i] = mb;
+      mt += mb;
+    }
+    int hyb = (em == 3 || em == 4);
+    double mtg = (em == 4) ? fmax(mt - mw, 0.0) : ((em == 3) ? mg : mt);
+    double mtw = hyb ? mw : 0.0;
+    double cg = 0, cw = 0;
+    for (int i = 0; i < n; ++i) {
+      double mgi = (em == 4) ? fmax(m->muev[i] - m->muw[i], 0.0) : m->mug[i];
+      double pg = hyb ? (mtg > 0.0 ? mgi / mtg : 0.0) : (mt > 0.0 ? m->muev[i] / mt : 0.0);
+      double pw = (hyb && mtw > 0.0) ? m->muw[i] / mtw : 0.0;
+      cg += pg; cw += pw;
+      m->gcdf[i] = cg; m->wcdf[i] = cw;
+    }
+    if (n > 0) { if (cg > 0.0) m->gcdf[n - 1] = 1.0; if (cw > 0.0) m->wcdf[n - 1] = 1.0; }
+    m->mu_scalars[0] = mtg; m->mu_scalars[1] = mtw; m->mu_scalars[2] = mg; m->mu_scalars[3] = ms;
+    m->mu_scalars[4] = me; m->mu_scalars[5] = mtw; m->mu_scalars[6] = mt;
+    double cwave = 1.0;
+    if (hyb && p->wave_rms_height_m > 0.0 && p->wavelength > 0.0 && p->n_medium > 0.0) {
+      double a = 4.0 * PI_D * p->n_medium * p->wave_rms_height_m / p->wavelength;
+      cwave = exp(-a * a);
+      if (cwave < 0.0) cwave = 0.0;
+      if (cwave > 1.0) cwave = 1.0;
+    }
+    m->mu_scalars[7] = cwave;
+    /* traced per-bin channel rates, CLARITAS.py:6368-6384 */
+    m->sum_trace_g = 0.0; m->sum_trace_w = 0.0;
+    for (int i = 0; i < n; ++i) {
+      if (em == 3) { m->trace_g[i] = m->mug[i]; m->trace_w[i] = m->muw[i]; }
+      else if (em == 4) { m->trace_g[i] = fmax(m->muev[i] - m->muw[i], 0.0); m->trace_w[i] = m->muw[i]; }
+      else { m->trace_g[i] = m->muev[i]; m->trace_w[i] = 0.0; }
+      m->sum_trace_g += m->trace_g[i];
+      m->sum_trace_w += m->trace_w[i];
+    }
+  }
+  free(sg); free(ss); free(se); free(swv); free(mass); free(srcw);
+  return err;
+}
+
+/* ======================================================================
+ * Sediment coefficients, CLARITAS.py:3924-4038 (device kernel) and the
+ * host twin CLARITAS.py:883-1079.
+ * ====================================================================== */
+static double schiller_naumann_velocity(double v0, double d, double rho_w, double mu) {
+  if (!(v0 > 0.0)) return 0.0;
+  double v = v0, K = rho_w * d / mu;
+  for (int it = 0; it < 24; ++it) {
+    double Re = fmax(K * v, 0.0), corr = 0.15 * pow(Re, 0.687);
+    double f = v * (1.0 + corr) - v0, df = 1.0 + 1.687 * corr;
+    double vn = fmax(v - f / df, 0.0);
+    if (fabs(vn - v) <= 2e-14 * fmax(1.0, fabs(v))) { v = vn; break; }
+    v = vn;
+  }
+  return v;
+}
+
+static float field_max(const float* f, size_t n) {
+  float mx = 0.0f;
+  for (size_t i = 0; i < n; ++i) if (f[i] > mx) mx = f[i];
+  return mx;
+}
+
+static int sediment_precompute(orc_model* m, const clr_sediment* s) {
+  int n = m->n, tm = m->cfg.sediment_model, fsm = m->cfg.free_surface_model;
+  int nrad = m->nrad;
+  double R = m->cfg.rin, ZMIN = m->cfg.zmin, ZMEAN = m->cfg.zmean;
+  double DH = m->cfg.vortex_delta_h, CORE = m->cfg.vortex_core_radius;
+  for (int i = 0; i < n; ++i) {
+    m->sedA[i] = 0.0; m->sedB[i] = 0.0; m->sednorm[i] = 1.0; m->sedws[i] = 0.0;
+    for (int j = 0; j < nrad; ++j) m->radlog[(size_t)i * nrad + j] = 0.0;
+  }
+  if (tm == CLR_SED_UNIFORM) {
+    m->maj[0] = m->sum_trace_g; m->maj[1] = m->sum_trace_w; m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
+    return 0;
+  }
+  if (tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD) {
+    double mx = (double)field_max(m->field, m->field_len);
+    m->maj[0] = m->sum_trace_g * mx; m->maj[1] = m->sum_trace_w * mx;
+    m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
+    return 0;
+  }
+  /* analytic drift-diffusion fields (models 1, 2) */
+  double rho_p = s->particle_density, rho_w = s->water_density, mu = s->water_viscosity;
+  double grav = s->gravity, Dt = s->eddy_diffusivity;
+  int NQ = s->n_quad;
+  if (!(rho_p > rho_w) || !(rho_w > 0.0) || !(mu > 0.0) || !(grav > 0.0) || !(Dt > 0.0) ||
+      !(R > 0.0) || NQ < 64 || nrad < 65)
+    return 31;
+  if (fsm < 0 || fsm > 2 || (DH > 0.0 && fsm == 0) || (fsm == 2 && !(CORE > 0.0))) return 32;
+  double R2 = R * R, a2 = CORE * CORE, Hinf = 0.0, hbar = 0.0, om2 = 0.0;
+  if (DH > 0.0 && fsm == 1) { Hinf = DH; hbar = 0.5 * DH; om2 = 2.0 * grav * DH / R2; }
+  else if (DH > 0.0 && fsm == 2) {
+    Hinf = DH * (a2 + R2) / R2; hbar = Hinf * (1.0 - (a2 / R2) * log1p(R2 / a2)); om2 = 2.0 * grav * Hinf / a2;
+  }
+  double fR = (fsm == 2) ? R2 / (a2 + R2) : R2;
+  double Hmean = ZMEAN - ZMIN, dr = R / (double)NQ, drad = R / (double)(nrad - 1);
+  if (!(Hmean > 0.0)) return 33;
+  double mjg = 0.0, mjw = 0.0, maxA = 0.0;
+  double* wr = (double*)malloc(sizeof(double) * (size_t)nrad);
+  for (int i = 0; i < n; ++i) {
+    double d = m->d[i];
+    double tau = rho_p * d * d / (18.0 * mu);
+    double ws0 = (rho_p - rho_w) * grav * d * d / (18.0 * mu);
+    double ws = (tm != 1) ? schiller_naumann_velocity(ws0, d, rho_w, mu) : ws0;
+    m->sedws[i] = ws;
+    double B = ws / Dt;
+    m->sedB[i] = B;
+    double* rl = m->radlog + (size_t)i * nrad;
+    if (tm == 1) {
+      double A = tau * om2 * ((fsm == 2) ? a2 : 1.0) / (2.0 * Dt);
+      m->sedA[i] = A;
+      if (A > maxA) maxA = A;
+      for (int j = 0; j < nrad; ++j) {
+        double r = (double)j * drad, rr2 = r * r, f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2;
+        rl[j] = A * (f - fR);
+      }
+    } else {
+      double total = 0.0, prev = 0.0;
+      for (int j = 0; j < nrad; ++j) {
+        double r = (double)j * drad, acc = 0.0;
+        if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
+        wr[j] = schiller_naumann_velocity(tau * acc, d, rho_w, mu);
+        if (j > 0) total += 0.5 * (prev + wr[j]) * drad;
+        prev = wr[j];
+      }
+      double cum = 0.0;
+      prev = 0.0;
+      for (int j = 0; j < nrad; ++j) {
+        if (j > 0) cum += 0.5 * (prev + wr[j]) * drad;
+        prev = wr[j];
+        double psi = (cum - total) / Dt;
+        if (psi > 0.0) psi = 0.0;
+        rl[j] = psi;
+      }
+      rl[nrad - 1] = 0.0;
+      double Aeq = (fR > 0.0) ? -rl[0] / fR : 0.0;
+      m->sedA[i] = Aeq;
+      if (Aeq > maxA) maxA = Aeq;
+    }
+    double integ = 0.0;
+    for (int j = 0; j < NQ; ++j) {
+      double r = ((double)j + 0.5) * dr, rr2 = r * r, zs = ZMEAN;
+      if (fsm == 1 && DH > 0.0) zs = ZMEAN + DH * (rr2 / R2 - 0.5);
+      else if (fsm == 2 && DH > 0.0) zs = ZMEAN - hbar + Hinf * rr2 / (a2 + rr2);
+      double H = fmax(zs - ZMIN, 0.0), lr;
+      if (tm == 1) { double f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2; lr = m->sedA[i] * (f - fR); }
+      else {
+        double u = r / R * (double)(nrad - 1);
+        int k = (int)floor(u);
+        if (k < 0) k = 0;
+        if (k > nrad - 2) k = nrad - 2;
+        double t = u - (double)k;
+        lr = (1.0 - t) * rl[k] + t * rl[k + 1];
+        if (lr > 0.0) lr = 0.0;
+      }
+      double rad = exp(lr), iz = (fabs(B) < 1e-14) ? rad * H : rad * (-expm1(-B * H)) / B;
+      integ += r * iz;
+    }
+    double norm = (2.0 / (R2 * Hmean)) * integ * dr;
+    if (!(norm > 0.0) || !isfinite(norm)) { free(wr); return 34; }
+    m->sednorm[i] = norm;
+    mjg += m->trace_g[i] / norm;
+    mjw += m->trace_w[i] / norm;
+  }
+  free(wr);
+  if (tm == CLR_SED_GRID_PER_BIN) {
+    /* the caller's per-bin r-z grid replaces the analytic field */
+    size_t plane = (size_t)m->fp.fn0 * (size_t)m->fp.fn1;
+    mjg = 0.0; mjw = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double mx = (double)field_max(m->field + (size_t)i * plane, plane);
+      mjg += m->trace_g[i] * mx; mjw += m->trace_w[i] * mx;
+    }
+    maxA = 0.0;
+  }
+  m->maj[0] = mjg; m->maj[1] = mjw; m->maj[2] = mjg + mjw; m->maj[3] = maxA;
+  if (!(m->maj[2] >= 0.0) || !isfinite(m->maj[2])) return 35;
+  return 0;
+}
+
+/* ======================================================================
+ * Sediment sampling at a point, CLARITAS.py:1537-1609
+ * ====================================================================== */
+static double rz_grid_sample(const float* G, int NR, int NZ, float x, float y, float z, float R,
+                             float ZMIN, float ZMAX) {
+  if (NR < 2 || NZ < 2 || !(ZMAX > ZMIN)) return 1.0;
+  double rr = sqrt(fmax((double)x * (double)x + (double)y * (double)y, 0.0));
+  if (rr > (double)R) rr = (double)R;
+  double dr = (double)R / (double)NR, dz = ((double)ZMAX - (double)ZMIN) / (double)NZ;
+  double ur = rr / dr - 0.5, uz = ((double)z - (double)ZMIN) / dz - 0.5;
+  if (ur < 0.0) ur = 0.0;
+  if (uz < 0.0) uz = 0.0;
+  if (ur > (double)(NR - 1)) ur = (double)(NR - 1);
+  if (uz > (double)(NZ - 1)) uz = (double)(NZ - 1);
+  int i = (int)floor(ur), k = (int)floor(uz);
+  double tr = ur - (double)i, tz = uz - (double)k;
+  if (i >= NR - 1) { i = NR - 2; tr = 1.0; }
+  if (k >= NZ - 1) { k = NZ - 2; tz = 1.0; }
+  double q00 = G[(size_t)i * NZ + k], q10 = G[(size_t)(i + 1) * NZ + k];
+  double q01 = G[(size_t)i * NZ + k + 1], q11 = G[(size_t)(i + 1) * NZ + k + 1];
+  double q0 = (1.0 - tr) * q00 + tr * q10, q1 = (1.0 - tr) * q01 + tr * q11;
+  return fmax((1.0 - tz) * q0 + tz * q1, 0.0);
+}
+/* trilinear, cell-centred Cartesian field over [-H,H]^2 x [zmin,zmax] */
+static double xyz_grid_sample(const float* G, int NX, int NY, int NZ, float x, float y, float z,
+                              float H, float ZMIN, float ZMAX) {
+  double dx = 2.0 * (double)H / NX, dy = 2.0 * (double)H / NY, dz = ((double)ZMAX - (double)ZMIN) / NZ;
+  double u[3] = {((double)x + H) / dx - 0.5, ((double)y + H) / dy - 0.5, ((double)z - ZMIN) / dz - 0.5};
+  int nn[3] = {NX, NY, NZ}, ii[3];
+  double tt[3];
+  for (int a = 0; a < 3; ++a) {
+    double v = u[a];
+    if (v < 0.0) v = 0.0;
+    if (v > (double)(nn[a] - 1)) v = (double)(nn[a] - 1);
+    int i = (int)floor(v);
+    double t = v - (double)i;
+    if (i >= nn[a] - 1) { i = nn[a] - 2; t = 1.0; }
+    ii[a] = i; tt[a] = t;
+  }
+  size_t sy = (size_t)NZ, sx = (size_t)NY * NZ;
+  const float* b = G + (size_t)ii[0] * sx + (size_t)ii[1] * sy + ii[2];
+  double c00 = (1 - tt[2]) * b[0] + tt[2] * b[1];
+  double c01 = (1 - tt[2]) * b[sy] + tt[2] * b[sy + 1];
+  double c10 = (1 - tt[2]) * b[sx] + tt[2] * b[sx + 1];
+  double c11 = (1 - tt[2]) * b[sx + sy] + tt[2] * b[sx + sy + 1];
+  double c0 = (1 - tt[1]) * c00 + tt[1] * c01, c1 = (1 - tt[1]) * c10 + tt[1] * c11;
+  return fmax((1 - tt[0]) * c0 + tt[0] * c1, 0.0);
+}
+
+static double sediment_local_scale(const orc_model* m, int pidx, float x, float y, float z) {
+  const fparams* P = &m->fp;
+  int tm = P->sed_model;
+  if (tm == CLR_SED_UNIFORM) return 1.0;
+  if (tm == CLR_SED_GRID_PER_BIN)
+    return rz_grid_sample(m->field + (size_t)pidx * P->fn0 * P->fn1, P->fn0, P->fn1, x, y, z, P->rin,
+                          P->zmin, P->field_zmax);
+  if (tm == CLR_SED_RZ_FIELD)
+    return rz_grid_sample(m->field, P->fn0, P->fn1, x, y, z, P->rin, P->zmin, P->field_zmax);
+  if (tm == CLR_SED_XYZ_FIELD)
+    return xyz_grid_sample(m->field, P->fn0, P->fn1, P->fn2, x, y, z, P->field_half, P->zmin,
+                           P->field_zmax);
+  double rr2 = (double)x * (double)x + (double)y * (double)y;
+  double R2 = (double)P->rin * (double)P->rin;
+  if (rr2 > R2) rr2 = R2;
+  double rl = 0.0;
+  if (tm == CLR_SED_STOKES) {
+    if (P->fs_model == 1) rl = m->sedA[pidx] * (rr2 - R2);
+    else {
+      double a2 = (double)P->core_r * (double)P->core_r;
+      rl = m->sedA[pidx] * (rr2 / (a2 + rr2) - R2 / (a2 + R2));
+    }
+  } else {
+    int nrad = m->nrad;
+    double rr = sqrt(fmax(rr2, 0.0));
+    double u = (P->rin > 0.0f) ? rr / (double)P->rin * (double)(nrad - 1) : 0.0;
+    int j = (int)floor(u);
+    if (j < 0) j = 0;
+    if (j > nrad - 2) j = nrad - 2;
+    double t = u - (double)j;
+    const double* L = m->radlog + (size_t)pidx * nrad;
+    rl = (1.0 - t) * L[j] + t * L[j + 1];
+    if (rl > 0.0) rl = 0.0;
+  }
+  double h = fmax((double)z - (double)P->zmin, 0.0);
+  double ns = m->sednorm[pidx];
+  if (!(ns > 0.0)) return 0.0;
+  return exp(rl - m->sedB[pidx] * h) / ns;
+}
+
+static int sample_cdf_index(rng_t* r, const float* cdf, int n) {
+  if (n <= 1) return 0;
+  float u = U(r);
+  int lo = 0, hi = n - 1;
+  while (lo < hi) { int mid = (lo + hi) >> 1; if (u <= cdf[mid]) hi = mid; else lo = mid + 1; }
+  return lo;
+}
+/* O(bins) walk on the local rates, CLARITAS.py:1595-1609 */
+static int sample_local_bin(rng_t* r, const orc_model* m, const double* rate, float x, float y,
+                            float z, double local_total) {
+  if (!(local_total > 0.0)) return 0;
+  double target = (double)U(r) * local_total, cum = 0.0;
+  for (int i = 0; i < m->n; ++i) {
+    cum += rate[i] * sediment_local_scale(m, i, x, y, z);
+    if (target <= cum) return i;
+  }
+  return m->n > 0 ? m->n - 1 : 0;
+}
+/* bin selection for one channel: per-bin CDF when the density scale is
+ * common to every bin (uniform, shared fields), local walk otherwise */
+static int select_bin(rng_t* r, const orc_model* m, int wave, float x, float y, float z,
+                      float local_channel) {
+  int tm = m->fp.sed_model;
+  if (tm == CLR_SED_UNIFORM || tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD)
+    return sample_cdf_index(r, wave ? m->wcdf_f : m->gcdf_f, m->n);
+  return sample_local_bin(r, m, wave ? m->trace_w : m->trace_g, x, y, z, (double)local_channel);
+}
+
+/* ======================================================================
+ * Free surface, CLARITAS.py:1503-1527, 1611-1779; Rankine is an extension
+ * ====================================================================== */
+static float free_surface_z(const fparams* P, float x, float y) {
+  int M = P->fs_model;
+  if (M == 0 || P->dh <= 0.0f) return P->zmean;
+  float q = x * x + y * y, R = P->rin;
+  if (M == 1) return P->zmean + P->dh * (q / (R * R) - 0.5f);
+  if (M == 3) {
+    double qa = (double)q, a2 = (double)P->core_r * (double)P->core_r;
+    double f = (qa < a2) ? P->rk_K * qa / a2 : P->rk_K * (2.0 - a2 / qa);
+    return (float)(P->rk_C0 + f);
+  }
+  float a2 = P->core_r * P->core_r, R2 = R * R;
+  float H = P->dh * (a2 + R2) / R2;
+  float hbar = H * (1.0f - (a2 / R2) * log1pf(R2 / a2));
+  return P->zmean - hbar + H * q / (a2 + q);
+}
+
+static double quadratic_min_positive(double A, double B, double C) {
+  if (fabs(A) < 1e-24) {
+    if (fabs(B) < 1e-24) return 1.0e300;
+    double t = -C / B;
+    return t > 0.0 ? t : 1.0e300;
+  }
+  double D = B * B - 4.0 * A * C;
+  if (D < 0.0) return 1.0e300;
+  double sd = sqrt(fmax(D, 0.0)), q = -0.5 * (B + copysign(sd, B));
+  double t1 = (fabs(q) > 1e-300) ? q / A : (-B - sd) / (2.0 * A);
+  double t2 = (fabs(q) > 1e-300) ? C / q : (-B + sd) / (2.0 * A);
+  double best = 1.0e300;
+  if (t1 > 0.0 && t1 < best) best = t1;
+  if (t2 > 0.0 && t2 < best) best = t2;
+  return best;
+}
+static double cubic_min_positive(double A3, double A2, double A1, double A0) {
+  double scale = fmax(fmax(fabs(A3), fabs(A2)), fmax(fabs(A1), fabs(A0)));
+  if (scale <= 0.0) return 1.0e300;
+  if (fabs(A3) <= 1e-12) return quadratic_min_positive(A2, A1, A0);
+  double a = A2 / A3, b = A1 / A3, c = A0 / A3;
+  double p = b - a * a / 3.0, q = 2.0 * a * a * a / 27.0 - a * b / 3.0 + c;
+  double hq = 0.5 * q, tp = p / 3.0, d1 = hq * hq, d2 = tp * tp * tp, disc = d1 + d2;
+  double tol = 1e-12 * (fabs(d1) + fabs(d2) + 1e-300), best = 1.0e300;
+  if (disc > tol) {
+    double sd = sqrt(disc), t = cbrt(-hq + sd) + cbrt(-hq - sd) - a / 3.0;
+    if (t > 0.0) best = t;
+  } else if (fabs(p) < 1e-24 && fabs(q) < 1e-24) {
+    double t = -a / 3.0;
+    if (t > 0.0) best = t;
+  } else if (disc >= -tol) {
+    double u = cbrt(-hq), t1 = 2.0 * u - a / 3.0, t2 = -u - a / 3.0;
+    if (t1 > 0.0 && t1 < best) best = t1;
+    if (t2 > 0.0 && t2 < best) best = t2;
+  } else {
+    double mm = 2.0 * sqrt(fmax(-tp, 0.0)), den = sqrt(fmax(-(tp * tp * tp), 1e-300));
+    double ca = fmin(fmax(-hq / den, -1.0), 1.0), phi = acos(ca);
+    for (int k = 0; k < 3; ++k) {
+      double t = mm * cos((phi + 2.0 * PI_D * (double)k) / 3.0) - a / 3.0;
+      if (t > 0.0 && t < best) best = t;
+    }
+  }
+  return best;
+}
+/* monotone-interval bisection for the first sign change of a cubic on
+ * [lo, hi], CLARITAS.py:1690-1722 */
+static double bounded_cubic_root(double a3, double a2, double a1, double a0, double lo, double hi) {
+  if (!(hi > lo) || !isfinite(hi)) return 1.0e300;
+  double cuts[4];
+  int n = 1;
+  cuts[0] = lo;
+  double A = 3.0 * a3, B = 2.0 * a2, C = a1;
+  if (A != 0.0) {
+    double disc = B * B - 4.0 * A * C;
+    if (disc >= 0.0) {
+      double q = -0.5 * (B + copysign(sqrt(disc), B));
+      double r1 = (q != 0.0) ? q / A : -B / (2.0 * A), r2 = (q != 0.0) ? C / q : r1;
+      if (r1 > lo && r1 < hi) cuts[n++] = r1;
+      if (r2 > lo && r2 < hi && r2 != r1) cuts[n++] = r2;
+    }
+  } else if (B != 0.0) {
+    double r = -C / B;
+    if (r > lo && r < hi) cuts[n++] = r;
+  }
+  cuts[n++] = hi;
+  for (int i = 1; i < n; ++i) {
+    double v = cuts[i];
+    int j = i;
+    while (j > 0 && cuts[j - 1] > v) { cuts[j] = cuts[j - 1]; --j; }
+    cuts[j] = v;
+  }
+  for (int i = 0; i < n - 1; ++i) {
+    double l = cuts[i], r = cuts[i + 1];
+    double fl = ((a3 * l + a2) * l + a1) * l + a0, fr = ((a3 * r + a2) * r + a1) * r + a0;
+    if (l > 0.0 && fl == 0.0) return l;
+    if ((fl < 0.0 && fr > 0.0) || (fl > 0.0 && fr < 0.0)) {
+      for (int j = 0; j < 64; ++j) {
+        double mid = l + 0.5 * (r - l);
+        if (mid == l || mid == r) break;
+        double fm = ((a3 * mid + a2) * mid + a1) * mid + a0;
+        if (fm == 0.0) { l = r = mid; break; }
+        if ((fl < 0.0 && fm < 0.0) || (fl > 0.0 && fm > 0.0)) { l = mid; fl = fm; } else r = mid;
+      }
+      return l + 0.5 * (r - l);
+    }
+    if (r > 0.0 && fr == 0.0) return r;
+  }
+  return 1.0e300;
+}
+
+/* Rankine surface hit: split the ray at the core cylinder r=a, solve the
+ * core paraboloid (quadratic) and the outer a^2/r^2 sheet (cubic after
+ * multiplying by r^2) on each piece, first root wins. */
+static double rankine_hit(const fparams* P, double X, double Y, double Z, double VX, double VY,
+                          double VZ) {
+  double a2 = (double)P->core_r * (double)P->core_r, K = P->rk_K, C0 = P->rk_C0;
+  double RR = (double)P->rin * (double)P->rin;
+  double Q2 = VX * VX + VY * VY, Q1 = 2.0 * (X * VX + Y * VY), Q0 = X * X + Y * Y;
+  double tend = 1.0e300;
+  double tin = 1.0e300, tout = -1.0;
+  if (Q2 > 0.0) {
+    double dc = Q1 * Q1 - 4.0 * Q2 * (Q0 - RR);
+    if (dc >= 0.0) {
+      double q = -0.5 * (Q1 + copysign(sqrt(dc), Q1));
+      double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - RR) / q : ta;
+      tend = fmax(ta, tb);
+    }
+    double dk = Q1 * Q1 - 4.0 * Q2 * (Q0 - a2);
+    if (dk > 0.0) {
+      double q = -0.5 * (Q1 + copysign(sqrt(dk), Q1));
+      double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - a2) / q : ta;
+      tin = fmin(ta, tb);
+      tout = fmax(ta, tb);
+    }
+  } else {
+    if (Q0 < a2) { tin = -1.0e300; tout = 1.0e300; }
+  }
+  if (!(tend > 0.0)) return 1.0e300;
+  double cuts[4] = {0.0, 0.0, 0.0, 0.0};
+  int nc = 0;
+  cuts[nc++] = 0.0;
+  if (tin > 0.0 && tin < tend) cuts[nc++] = tin;
+  if (tout > 0.0 && tout < tend) cuts[nc++] = tout;
+  cuts[nc++] = (tend < 1.0e299) ? tend : 1.0;  /* vertical ray: one piece, bounded below */
+  if (!(tend < 1.0e299)) {
+    /* vertical ray: surface height at fixed r */
+    double f = (Q0 < a2) ? K * Q0 / a2 : K * (2.0 - a2 / Q0);
+    double t = (VZ != 0.0) ? (C0 + f - Z) / VZ : 1.0e300;
+    return (t > 0.0) ? t : 1.0e300;
+  }
+  for (int i = 0; i + 1 < nc; ++i) {
+    double lo = cuts[i], hi = cuts[i + 1];
+    if (!(hi > lo)) continue;
+    double mid = 0.5 * (lo + hi), qm = (Q2 * mid + Q1) * mid + Q0;
+    double t;
+    if (qm < a2) {
+      double k = K / a2;
+      double A = -k * Q2, B = VZ - k * Q1, C = Z - C0 - k * Q0;
+      t = bounded_cubic_root(0.0, A, B, C, lo, hi);
+    } else {
+      double L = Z - C0 - 2.0 * K;
+      t = bounded_cubic_root(VZ * Q2, L * Q2 + VZ * Q1, L * Q1 + VZ * Q0, L * Q0 + K * a2, lo, hi);
+    }
+    if (t < 1.0e299 && t > 0.0) return t;
+  }
+  return 1.0e300;
+}
+
+/* first positive intersection with the free surface, CLARITAS.py:1726-1766 */
+static float free_surface_hit(const fparams* P, float x, float y, float z, float vx, float vy,
+                              float vz) {
+  int M = P->fs_model;
+  float R = P->rin;
+  if (M == 0 || P->dh <= 0.0f) {
+    if (fabsf(vz) <= 1e-14f) return INF_F;
+    float t = (P->zmean - z) / vz;
+    return (t > 0.0f) ? t : INF_F;
+  }
+  if (M == 1) {
+    float aa = P->dh / (R * R);
+    double A = -(double)aa * ((double)vx * vx + (double)vy * vy);
+    double B = (double)vz - 2.0 * (double)aa * ((double)x * vx + (double)y * vy);
+    double C = (double)z - ((double)P->zmean - 0.5 * (double)P->dh) - (double)aa * ((double)x * x + (double)y * y);
+    double t = quadratic_min_positive(A, B, C);
+    return (t < 0.5 * (double)INF_F) ? (float)t : INF_F;
+  }
+  if (M == 3) {
+    double t = rankine_hit(P, x, y, z, vx, vy, vz);
+    return (t > 0.0 && t < 0.5 * (double)INF_F) ? (float)t : INF_F;
+  }
+  double X = x, Y = y, Z = z, VX = vx, VY = vy, VZ = vz;
+  double RR = (double)R * (double)R, aa = (double)P->core_r * (double)P->core_r;
+  double H = (double)P->dh * (aa + RR) / RR;
+  double hbar = H * (1.0 - (aa / RR) * log1p(RR / aa));
+  double Cs = (double)P->zmean - hbar;
+  double Q2 = VX * VX + VY * VY, Q1 = 2.0 * (X * VX + Y * VY), Q0 = X * X + Y * Y, L0 = Z - Cs;
+  double c3 = VZ * Q2, c2 = (L0 - H) * Q2 + VZ * Q1, c1 = (L0 - H) * Q1 + VZ * (aa + Q0);
+  double c0 = L0 * aa + (L0 - H) * Q0;
+  double t = cubic_min_positive(c3, c2, c1, c0);
+  double xx = X + t * VX, yy = Y + t * VY, qq = xx * xx + yy * yy;
+  double res = (Z + t * VZ) - (Cs + H * qq / (aa + qq));
+  double tol = 32.0 * DEPS * fmax(fmax(fabs(Z), fabs(Cs) + fabs(H)), 1.0e-3);
+  if (!(t < 0.5 * (double)INF_F) || qq > RR * (1.0 + 2e-6) || !isfinite(res) || fabs(res) > tol) {
+    if (Q2 > 0.0) {
+      double dc = Q1 * Q1 - 4.0 * Q2 * (Q0 - RR);
+      if (dc < 0.0) return INF_F;
+      double q = -0.5 * (Q1 + copysign(sqrt(dc), Q1));
+      double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - RR) / q : ta;
+      double lo = fmax(0.0, fmin(ta, tb)), hi = fmax(ta, tb);
+      t = bounded_cubic_root(c3, c2, c1, c0, lo, hi);
+    } else {
+      t = (VZ != 0.0) ? ((Cs + H * Q0 / (aa + Q0)) - Z) / VZ : 1.0e300;
+      if (!(t > 0.0) || Q0 > RR * (1.0 + 2e-6)) return INF_F;
+    }
+  }
+  return (t > 0.0 && t < 0.5 * (double)INF_F) ? (float)t : INF_F;
+}
+
+/* unit normal into the water, CLARITAS.py:1769-1779 */
+static void free_surface_normal(const fparams* P, float x, float y, float* nx, float* ny, float* nz) {
+  int M = P->fs_model;
+  if (M == 0 || P->dh <= 0.0f) { *nx = 0.0f; *ny = 0.0f; *nz = -1.0f; return; }
+  float R = P->rin;
+  if (M == 1) {
+    float aa = P->dh / (R * R);
+    *nx = 2.0f * aa * x; *ny = 2.0f * aa * y; *nz = -1.0f;
+    normalize3(nx, ny, nz);
+    return;
+  }
+  float q = x * x + y * y, fac;
+  if (M == 3) {
+    float a2 = P->core_r * P->core_r, K = (float)P->rk_K;
+    fac = (q < a2) ? 2.0f * K / a2 : 2.0f * K * a2 / (q * q);
+  } else {
+    float a2 = P->core_r * P->core_r, R2 = R * R;
+    float H = P->dh * (a2 + R2) / R2;
+    fac = 2.0f * H * a2 / ((a2 + q) * (a2 + q));
+  }
+  *nx = fac * x; *ny = fac * y; *nz = -1.0f;
+  normalize3(nx, ny, nz);
+}
+
+/* surface height (double) at radius^2 q, used by particle accessibility */
+static double free_surface_z_d(const fparams* P, double q) {
+  int M = P->fs_model;
+  double zm = (double)P->zmean;
+  if (M == 0 || P->dh <= 0.0f) return zm;
+  double R = (double)P->rin, dh = (double)P->dh, a = (double)P->core_r, R2 = R * R;
+  if (M == 1) return zm + dh * (q / R2 - 0.5);
+  if (M == 3) {
+    double a2 = a * a;
+    return P->rk_C0 + ((q < a2) ? P->rk_K * q / a2 : P->rk_K * (2.0 - a2 / q));
+  }
+  double a2 = a * a, H = dh * (a2 + R2) / R2, hbar = H * (1.0 - (a2 / R2) * log1p(R2 / a2));
+  return zm - hbar + H * q / (a2 + q);
+}
+
+/* ======================================================================
+ * Cell topology, CLARITAS.py:1473-1500, 2888-3293
+ * ====================================================================== */
+static void radial_normal(float x, float y, float* nx, float* ny) {
+  float r = sqrtf(fmaxf(x * x + y * y, 1e-30f));
+  *nx = x / r; *ny = y / r;
+}
+static int bottom_topology_at_top(float x, float y, float Rin, float Rout) {
+  double r2 = (double)x * (double)x + (double)y * (double)y;
+  double ri2 = (double)Rin * (double)Rin, ro2 = (double)Rout * (double)Rout;
+  if (r2 < ri2) return 1;
+  if (r2 <= ro2 * (1.0 + 8.0 * DEPS)) return 2;
+  return 0;
+}
+static int cylinder_roots_d(float x, float y, float vx, float vy, float R, double* t0, double* t1) {
+  *t0 = 1.0e300; *t1 = 1.0e300;
+  double X = x, Y = y, VX = vx, VY = vy, RR = R;
+  double a = VX * VX + VY * VY;
+  if (!(a > 1e-30) || !isfinite(a)) return 0;
+  double b = X * VX + Y * VY, c = X * X + Y * Y - RR * RR, d = b * b - a * c;
+  double dtol = 16.0 * DEPS * (fabs(b * b) + fabs(a * c) + 1e-300);
+  if (d < 0.0) { if (d >= -dtol) d = 0.0; else return 0; }
+  double sd = sqrt(fmax(d, 0.0)), q = -b - copysign(sd, b);
+  double r0 = (fabs(q) > 1e-300) ? q / a : (-b - sd) / a;
+  double r1 = (fabs(q) > 1e-300) ? c / q : (-b + sd) / a;
+  double a0 = 1.0e300, a1 = 1.0e300;
+  if (r0 > 0.0 && isfinite(r0)) a0 = r0;
+  if (r1 > 0.0 && isfinite(r1)) { if (r1 < a0) { a1 = a0; a0 = r1; } else a1 = r1; }
+  *t0 = a0; *t1 = a1;
+  return (a0 < 1.0e299) + (a1 < 1.0e299);
+}
+static double support_tol(float Rin, float Rout, float Zmin, float Zlow, float Ztop) {
+  double s = fmax(fmax(fabs((double)Rout), fabs((double)Zlow)), fabs((double)Ztop));
+  s = fmax(s, fmax(fabs((double)Rin), fabs((double)Zmin)));
+  return 8.0 * FEPS * fmax(s, 1.0e-3);
+}
+static double finite_cylinder_forward_d(float x, float y, float z, float vx, float vy, float vz,
+                                        float R, float zlo, float zhi) {
+  double r[2];
+  cylinder_roots_d(x, y, vx, vy, R, &r[0], &r[1]);
+  double tol = support_tol(R, R, zlo, zlo, zhi), best = 1.0e300;
+  for (int k = 0; k < 2; ++k) {
+    double t = r[k];
+    if (!(t > 0.0) || t >= 1.0e299) continue;
+    double zh = (double)z + t * (double)vz;
+    if (zh >= (double)zlo - tol && zh <= (double)zhi + tol && t < best) best = t;
+  }
+  return best;
+}
+/* water/headspace side exit through r=Rin, CLARITAS.py:2929-2950 */
+static double bore_wall_hit_d(float x, float y, float z, float vx, float vy, float vz, float R,
+                              float zlo, float zhi) {
+  double X = x, Y = y, VX = vx, VY = vy, RR = (double)R * R;
+  double rr2 = X * X + Y * Y, c = rr2 - RR, b = X * VX + Y * VY, a = VX * VX + VY * VY;
+  double rtol = 8.0 * FEPS * RR, ztol = support_tol(R, R, zlo, zlo, zhi);
+  if (c > rtol || !isfinite(c) || !isfinite(b) || !isfinite(a)) return -1.0;
+  if (c >= 0.0) {
+    if (b >= 0.0 && a > 0.0 && z >= zlo - ztol && z <= zhi + ztol) return 0.0;
+    b *= sqrt(RR / rr2);
+    c = 0.0;
+  }
+  if (!(a > 1e-30)) return 1.0e300;
+  double d = b * b - a * c;
+  if (d < 0.0) return 1.0e300;
+  double sd = sqrt(d), q = -b - copysign(sd, b);
+  double t = (b < 0.0) ? q / a : ((q != 0.0) ? c / q : 0.0);
+  if (!(t >= 0.0) || !isfinite(t)) return 1.0e300;
+  double zh = (double)z + t * (double)vz;
+  return (zh >= zlo - ztol && zh <= zhi + ztol) ? t : 1.0e300;
+}
+/* CLARITAS.py:2961-2978 */
+static int water_surface_offset(const fparams* P, float* x, float* y, float* z, float nx, float ny,
+                                float nz) {
+  double rr2 = (double)(*x) * (double)(*x) + (double)(*y) * (double)(*y);
+  double RR = (double)P->rin * P->rin, tol = 8.0 * FEPS * RR;
+  if (!isfinite(rr2) || rr2 > RR + tol) return 0;
+  float xx = *x + EPS_F * nx, yy = *y + EPS_F * ny, zz = *z + EPS_F * nz;
+  double rad = hypot((double)xx, (double)yy);
+  if (rad > (double)P->rin) {
+    double target = (double)P->rin * (1.0 - 2.0 * FEPS);
+    xx = (float)((double)xx * target / rad);
+    yy = (float)((double)yy * target / rad);
+    float surf = free_surface_z(P, xx, yy);
+    if (zz >= surf) zz = nextafterf(surf, -INF_F);
+  }
+  if (!isfinite(xx) || !isfinite(yy) || !isfinite(zz) || zz <= P->zmin || zz >= free_surface_z(P, xx, yy))
+    return 0;
+  *x = xx; *y = yy; *z = zz;
+  return 1;
+}
+
+/* nearest physical boundary of the acrylic union, CLARITAS.py:3034-3063 */
+static int acrylic_next_surface(const fparams* P, float x, float y, float z, float vx, float vy,
+                                float vz, double* t_out) {
+  double best = 1.0e300;
+  int surf = 0;
+  float Rin = P->rin, Rout = P->rout, Zmin = P->zmin, Zlow = P->zlow, Ztop = P->zwall_top;
+  double tol = support_tol(Rin, Rout, Zmin, Zlow, Ztop), r[2];
+  cylinder_roots_d(x, y, vx, vy, Rin, &r[0], &r[1]);
+  for (int k = 0; k < 2; ++k) {
+    double t = r[k];
+    if (t >= 1.0e299) continue;
+    double zh = (double)z + t * (double)vz;
+    if (zh >= (double)Zmin - tol && zh <= (double)Ztop + tol && t > 0.0 && t < best) { best = t; surf = 1; }
+  }
+  cylinder_roots_d(x, y, vx, vy, Rout, &r[0], &r[1]);
+  for (int k = 0; k < 2; ++k) {
+    double t = r[k];
+    if (t >= 1.0e299) continue;
+    double zh = (double)z + t * (double)vz;
+    if (zh >= (double)Zlow - tol && zh <= (double)Ztop + tol && t > 0.0 && t < best) { best = t; surf = 2; }
+  }
+  if (vz < 0.0f) {
+    double t = ((double)Zlow - (double)z) / (double)vz;
+    if (t > 0.0 && isfinite(t)) {
+      double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy;
+      if (hypot(X, Y) <= (double)Rout + tol && t < best) { best = t; surf = 3; }
+    }
+  }
+  if (vz > 0.0f) {
+    double t = ((double)Ztop - (double)z) / (double)vz;
+    if (t > 0.0 && isfinite(t)) {
+      double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy, rr = hypot(X, Y);
+      if (rr >= (double)Rin - tol && rr <= (double)Rout + tol && t < best) { best = t; surf = 4; }
+    }
+    double ta = ((double)Zmin - (double)z) / (double)vz;
+    if (ta > 0.0 && isfinite(ta)) {
+      double X = (double)x + ta * (double)vx, Y = (double)y + ta * (double)vy;
+      if (hypot(X, Y) < (double)Rin - tol && ta < best) { best = ta; surf = 5; }
+    }
+  }
+  *t_out = best;
+  return surf;
+}
+
+static int tir_for_ci(double ci, double n1, double n2) {
+  if (!(n1 > 0.0) || !(n2 > 0.0) || n1 <= n2) return 0;
+  ci = fmin(fmax(ci, 0.0), 1.0);
+  return (n1 / n2) * sqrt(fmax(0.0, 1.0 - ci * ci)) >= 1.0;
+}
+/* conserved-orbit TIR trap classifier, CLARITAS.py:3083-3135 */
+static int acrylic_trapped(const fparams* P, float x, float y, float vx, float vy, float vz) {
+  double dx = vx, dy = vy, dz = vz, sp2 = dx * dx + dy * dy + dz * dz;
+  if (!(sp2 > 0.0) || !isfinite(sp2)) return 0;
+  double vxy2 = dx * dx + dy * dy, L = (double)x * dy - (double)y * dx, rs = sqrt(sp2);
+  const double reach = 4.0 * (double)EPS_F;
+  double na = P->n_acrylic, nw = P->n_water, nair = P->n_air, nb = P->n_bottom_ext;
+  if (vxy2 > 0.0) {
+    double tg = L / (double)P->rout, rad2 = vxy2 - tg * tg;
+    if (rad2 < 0.0 && rad2 > -1.0e-12 * sp2) rad2 = 0.0;
+    if (rad2 < 0.0) return 0;
+    if (!tir_for_ci(sqrt(rad2) / rs, na, nair)) return 0;
+    double impact = fabs(L) / sqrt(vxy2);
+    if (impact <= (double)P->rin + reach) {
+      double ti = L / (double)P->rin, ri2 = vxy2 - ti * ti;
+      if (ri2 < 0.0 && ri2 > -1.0e-12 * sp2) ri2 = 0.0;
+      if (ri2 >= 0.0 && !tir_for_ci(sqrt(ri2) / rs, na, nw)) return 0;
+    }
+    if (impact < (double)P->rin - reach && fabs(dz) > 0.0 && !tir_for_ci(fabs(dz) / rs, na, nw)) return 0;
+  }
+  if (fabs(dz) > 0.0) {
+    double ca = fabs(dz) / rs;
+    if (vxy2 <= 1.0e-30 * sp2 && hypot((double)x, (double)y) < (double)P->rin - reach &&
+        !tir_for_ci(ca, na, nw))
+      return 0;
+    if (!tir_for_ci(ca, na, nair)) return 0;
+    if (!tir_for_ci(ca, na, nb)) return 0;
+  }
+  return 1;
+}
+
+typedef struct {
+  float x, y, z, vx, vy, vz;
+  int cir, cor, bir, bor, ctir, btir, tr, ttir;
+  uint32_t n_iface;
+} photon_t;
+
+/* propagate inside the acrylic union; returns 1 water, 2 external air,
+ * 3 bottom external, 4 headspace, 5 trapped, 0 failure (sub set).
+ * CLARITAS.py:3140-3198 */
+static int acrylic_solid(const fparams* P, rng_t* r, photon_t* p, int* sub) {
+  *sub = SUB_NONE;
+  int stuck = 0, last = -1;
+  for (int it = 0; it < 16384; ++it) {
+    double td = 1.0e300;
+    int s = acrylic_next_surface(P, p->x, p->y, p->z, p->vx, p->vy, p->vz, &td);
+    if (!s || !(td > 0.0) || !isfinite(td) || td >= 1.0e299) { *sub = SUB_ACRYLIC_NO_HIT; return 0; }
+    float t = (float)td, px = p->x, py = p->y, pz = p->z, pvx = p->vx, pvy = p->vy, pvz = p->vz;
+    p->x += t * p->vx; p->y += t * p->vy; p->z += t * p->vz;
+    float a, b, c, ci, R;
+    int tir = 0;
+    p->n_iface++;
+    if (s == 1) {
+      float nx, ny;
+      radial_normal(p->x, p->y, &nx, &ny);
+      float zs = free_surface_z(P, p->x, p->y);
+      float n_in = (p->z <= zs + EPS_F) ? P->n_water : P->n_air;
+      ci = fminf(fmaxf(-dot3(p->vx, p->vy, p->vz, nx, ny, 0.0f), 0.0f), 1.0f);
+      R = fresnel_R(ci, P->n_acrylic, n_in, &tir);
+      if (!tir && U(r) >= R && refract3(p->vx, p->vy, p->vz, nx, ny, 0.0f, P->n_acrylic, n_in, &a, &b, &c)) {
+        p->vx = a; p->vy = b; p->vz = c; p->x -= EPS_F * nx; p->y -= EPS_F * ny;
+        return (n_in == P->n_water) ? 1 : 4;
+      }
+      reflect3(p->vx, p->vy, p->vz, nx, ny, 0.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->cir++; if (tir) p->ctir++;
+      p->x += EPS_F * nx; p->y += EPS_F * ny;
+    } else if (s == 2) {
+      float nx, ny;
+      radial_normal(p->x, p->y, &nx, &ny);
+      ci = fminf(fmaxf(dot3(p->vx, p->vy, p->vz, nx, ny, 0.0f), 0.0f), 1.0f);
+      R = fresnel_R(ci, P->n_acrylic, P->n_air, &tir);
+      if (!tir && U(r) >= R && refract3(p->vx, p->vy, p->vz, -nx, -ny, 0.0f, P->n_acrylic, P->n_air, &a, &b, &c)) {
+        p->vx = a; p->vy = b; p->vz = c; p->x += EPS_F * nx; p->y += EPS_F * ny;
+        return 2;
+      }
+      reflect3(p->vx, p->vy, p->vz, nx, ny, 0.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->cor++; if (tir) p->ctir++;
+      p->x -= EPS_F * nx; p->y -= EPS_F * ny;
+    } else if (s == 3) {
+      p->z = P->zlow;
+      ci = fminf(fmaxf(-p->vz, 0.0f), 1.0f);
+      R = fresnel_R(ci, P->n_acrylic, P->n_bottom_ext, &tir);
+      if (!tir && U(r) >= R && refract3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, P->n_acrylic, P->n_bottom_ext, &a, &b, &c)) {
+        p->vx = a; p->vy = b; p->vz = c; p->z = P->zlow - EPS_F;
+        return 3;
+      }
+      reflect3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->bor++; if (tir) p->btir++;
+      p->z = P->zlow + EPS_F;
+    } else if (s == 4) {
+      p->z = P->zwall_top;
+      ci = fminf(fmaxf(p->vz, 0.0f), 1.0f);
+      R = fresnel_R(ci, P->n_acrylic, P->n_air, &tir);
+      if (!tir && U(r) >= R && refract3(p->vx, p->vy, p->vz, 0.0f, 0.0f, -1.0f, P->n_acrylic, P->n_air, &a, &b, &c)) {
+        p->vx = a; p->vy = b; p->vz = c; p->z = P->zwall_top + EPS_F;
+        return 2;
+      }
+      reflect3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->cor++; if (tir) p->ctir++;
+      p->z = P->zwall_top - EPS_F;
+    } else {
+      p->z = P->zmin;
+      ci = fminf(fmaxf(p->vz, 0.0f), 1.0f);
+      R = fresnel_R(ci, P->n_acrylic, P->n_water, &tir);
+      if (!tir && U(r) >= R && refract3(p->vx, p->vy, p->vz, 0.0f, 0.0f, -1.0f, P->n_acrylic, P->n_water, &a, &b, &c)) {
+        p->vx = a; p->vy = b; p->vz = c; p->z = P->zmin + EPS_F;
+        return 1;
+      }
+      reflect3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->bir++; if (tir) p->btir++;
+      p->z = P->zmin - EPS_F;
+    }
+    float pd = fabsf(p->x - px) + fabsf(p->y - py) + fabsf(p->z - pz);
+    float vd = fabsf(p->vx - pvx) + fabsf(p->vy - pvy) + fabsf(p->vz - pvz);
+    stuck = (s == last && pd <= 1.0e-12f && vd <= 1.0e-7f) ? stuck + 1 : 0;
+    if (stuck >= 8) { *sub = SUB_ACRYLIC_BOUNCE; return 0; }
+    last = s;
+  }
+  if (acrylic_trapped(P, p->x, p->y, p->vx, p->vy, p->vz)) return 5;
+  *sub = SUB_ACRYLIC_BOUNCE;
+  return 0;
+}
+
+/* water -> bottom disc, CLARITAS.py:3211-3228 */
+static int bottom_disc(const fparams* P, rng_t* r, photon_t* p, int* sub) {
+  *sub = SUB_NONE;
+  if (bottom_topology_at_top(p->x, p->y, P->rin, P->rout) != 1) { *sub = SUB_BOTTOM_TOPOLOGY; return 0; }
+  float a, b, c;
+  int tir = 0;
+  float ci = fminf(fmaxf(-p->vz, 0.0f), 1.0f);
+  float R = fresnel_R(ci, P->n_water, P->n_acrylic, &tir);
+  p->n_iface++;
+  if (tir || U(r) < R || !refract3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, P->n_water, P->n_acrylic, &a, &b, &c)) {
+    reflect3(p->vx, p->vy, p->vz, 0.0f, 0.0f, 1.0f, &a, &b, &c);
+    p->vx = a; p->vy = b; p->vz = c; p->bir++; if (tir) p->btir++;
+    p->z = P->zmin + EPS_F;
+    return 1;
+  }
+  p->vx = a; p->vy = b; p->vz = c; p->z = P->zmin - EPS_F;
+  return acrylic_solid(P, r, p, sub);
+}
+
+/* headspace inside the open tube; returns 1 water, 2 external, 3 bottom
+ * external, 4 open-top escape, 5 trapped, 0 failure.  CLARITAS.py:3235-3293 */
+static int headspace(const fparams* P, rng_t* r, photon_t* p, int* sub) {
+  *sub = SUB_NONE;
+  double tol = support_tol(P->rin, P->rout, P->zmin, P->zlow, P->zwall_top);
+  for (int step = 0; step < 16384; ++step) {
+    if (!isfinite(p->x) || !isfinite(p->y) || !isfinite(p->z) || !isfinite(p->vx) || !isfinite(p->vy) || !isfinite(p->vz)) {
+      *sub = SUB_HEADSPACE_NO_HIT; return 0;
+    }
+    float zs = free_surface_z(P, p->x, p->y);
+    double rr = hypot((double)p->x, (double)p->y);
+    if ((double)p->z <= (double)zs - tol && rr < (double)P->rin - tol) return 1;
+    if ((double)p->z > (double)P->zwall_top + tol) return 4;
+    double tside = bore_wall_hit_d(p->x, p->y, p->z, p->vx, p->vy, p->vz, P->rin, P->zmin, P->zwall_top);
+    if (tside < 0.0) { *sub = SUB_WATER_OUTSIDE; return 0; }
+    double tsurf = (double)free_surface_hit(P, p->x, p->y, p->z, p->vx, p->vy, p->vz);
+    double ttop = 1.0e300;
+    if (p->vz > 1.0e-14f) {
+      double t = ((double)P->zwall_top - (double)p->z) / (double)p->vz;
+      if (t > 0.0) {
+        double X = (double)p->x + t * (double)p->vx, Y = (double)p->y + t * (double)p->vy;
+        if (hypot(X, Y) < (double)P->rin + tol) ttop = t;
+      }
+    }
+    int which = 0;
+    double best = 1.0e300;
+    if (tsurf > 0.0 && tsurf < best) { best = tsurf; which = 1; }
+    if (tside >= 0.0 && tside < best) { best = tside; which = 2; }
+    if (ttop > 0.0 && ttop < best) { best = ttop; which = 3; }
+    if (!which || !isfinite(best) || best >= 1.0e299) { *sub = SUB_HEADSPACE_NO_HIT; return 0; }
+    float t = (float)best;
+    p->x += t * p->vx; p->y += t * p->vy; p->z += t * p->vz;
+    if (which == 3) { p->z = P->zwall_top + EPS_F; return 4; }
+    float a, b, c;
+    int tir = 0;
+    p->n_iface++;
+    if (which == 1) {
+      float snx, sny, snz;
+      free_surface_normal(P, p->x, p->y, &snx, &sny, &snz);
+      float anx = -snx, any = -sny, anz = -snz;
+      float ci = fminf(fmaxf(-dot3(p->vx, p->vy, p->vz, anx, any, anz), 0.0f), 1.0f);
+      float R = fresnel_R(ci, P->n_air, P->n_water, &tir);
+      if (tir || U(r) < R || !refract3(p->vx, p->vy, p->vz, anx, any, anz, P->n_air, P->n_water, &a, &b, &c)) {
+        reflect3(p->vx, p->vy, p->vz, anx, any, anz, &a, &b, &c);
+        p->vx = a; p->vy = b; p->vz = c; p->tr++; if (tir) p->ttir++;
+        p->x += EPS_F * anx; p->y += EPS_F * any; p->z += EPS_F * anz;
+        continue;
+      }
+      p->vx = a; p->vy = b; p->vz = c;
+      if (!water_surface_offset(P, &p->x, &p->y, &p->z, snx, sny, snz)) { *sub = SUB_WATER_OUTSIDE; return 0; }
+      return 1;
+    }
+    float nx, ny;
+    radial_normal(p->x, p->y, &nx, &ny);
+    float inx = -nx, iny = -ny;
+    float ci = fminf(fmaxf(-dot3(p->vx, p->vy, p->vz, inx, iny, 0.0f), 0.0f), 1.0f);
+    float R = fresnel_R(ci, P->n_air, P->n_acrylic, &tir);
+    if (tir || U(r) < R || !refract3(p->vx, p->vy, p->vz, inx, iny, 0.0f, P->n_air, P->n_acrylic, &a, &b, &c)) {
+      reflect3(p->vx, p->vy, p->vz, inx, iny, 0.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->cir++; if (tir) p->ctir++;
+      p->x -= EPS_F * nx; p->y -= EPS_F * ny;
+      continue;
+    }
+    p->vx = a; p->vy = b; p->vz = c; p->x += EPS_F * nx; p->y += EPS_F * ny;
+    int asub = SUB_NONE;
+    int where = acrylic_solid(P, r, p, &asub);
+    if (where == 4) continue;
+    *sub = asub;
+    return where;
+  }
+  *sub = SUB_HEADSPACE_NO_HIT;
+  return 0;
+}
+
+/* ======================================================================
+ * Particle events
+ * ====================================================================== */
+/* impact-parameter sampling of the struck surface point, CLARITAS.py:2436-2454 */
+static void sphere_sample(rng_t* r, float vx, float vy, float vz, float* gnx, float* gny, float* gnz) {
+  float rho = sqrtf(fminf(fmaxf(U(r), 0.0f), 0.99999994f));
+  float ci = sqrtf(fmaxf(0.0f, 1.0f - rho * rho));
+  float phi = U(r) * 2.0f * PI_F;
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(vx, vy, vz, &e1x, &e1y, &e1z, &e2x, &e2y, &e2z);
+  float cp = cosf(phi), sp = sinf(phi);
+  float qx = cp * e1x + sp * e2x, qy = cp * e1y + sp * e2y, qz = cp * e1z + sp * e2z;
+  *gnx = -ci * vx + rho * qx; *gny = -ci * vy + rho * qy; *gnz = -ci * vz + rho * qz;
+  normalize3(gnx, gny, gnz);
+}
+/* CLARITAS.py:2459-2495 */
+static int sphere_accessible(const fparams* P, double cx, double cy, double cz, double radius) {
+  double rc = hypot(cx, cy), rr = radius;
+  double wall = ((double)P->rin - rc) - rr, bottom = cz - rr - (double)P->zmin;
+  double rmin = fmax(0.0, rc - rr);
+  double top = free_surface_z_d(P, rmin * rmin) - (cz + rr);
+  double scale = fmax(fmax(fabs((double)P->rin), fabs(cz)), fmax(rr, 1.0e-6));
+  double tol = 128.0 * DEPS * scale;
+  return (wall >= -tol && bottom >= -tol && top >= -tol) ? 1 : 0;
+}
+/* smooth-interface limit of the microsurface walk, CLARITAS.py:2323-2356 */
+static int smooth_interface(rng_t* r, float bx, float by, float bz, float n_top, float k_top,
+                            float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top,
+                            int* fc) {
+  *fc = 0;
+  normalize3(&bx, &by, &bz);
+  normalize3(dx, dy, dz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(*dx) || !isfinite(*dy) || !isfinite(*dz)) {
+    *fc = MF_NONFINITE_DIRECTION; return 0;
+  }
+  if (dot3(*dx, *dy, *dz, bx, by, bz) >= -1.0e-7f) { *fc = MF_SIDE_STATE; return 0; }
+  float ci = fminf(fmaxf(dot3(-(*dx), -(*dy), -(*dz), bx, by, bz), 0.0f), 1.0f);
+  float R, ts, tn;
+  if (!complex_interface(ci, n_top, k_top, n_bot, k_bot, &R, &ts, &tn) || !isfinite(R)) {
+    *fc = MF_FRESNEL_INVALID; return 0;
+  }
+  float ox, oy, oz;
+  if (U(r) < R) {
+    reflect3(*dx, *dy, *dz, bx, by, bz, &ox, &oy, &oz);
+    *out_top = 1;
+    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_REFLECTION_INVALID; return 0; }
+  } else {
+    if (!energy_refract3(*dx, *dy, *dz, bx, by, bz, n_top, k_top, n_bot, k_bot, &ox, &oy, &oz)) {
+      *fc = MF_REFRACTION_INVALID; return 0;
+    }
+    *out_top = 0;
+  }
+  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_NONFINITE_DIRECTION; return 0; }
+  *dx = ox; *dy = oy; *dz = oz;
+  return 1;
+}
+
+typedef struct { int pr, per, pir, pabs; uint32_t n_abs; } pcount_t;
+
+/* smooth Fresnel sphere: entry, internal chords with Beer-Lambert
+ * absorption 4 pi k / lambda, exit or internal reflection.
+ * Returns 1 escape, 2 absorbed, 0 failure.  CLARITAS.py:2497-2599 */
+static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, float k_p,
+                              float radius, int max_bounces, float gnx, float gny, float gnz,
+                              double cx, double cy, double cz, float* x, float* y, float* z,
+                              float* vx, float* vy, float* vz, pcount_t* pc, int* pf, int* pfs) {
+  *pf = PF_OK; *pfs = SUB_NONE;
+  if (radius <= 0.0f) return 1;
+  int top = 1, fc = 0;
+  if (!smooth_interface(r, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc)) {
+    *pf = PF_MICRO_NUMERICAL; *pfs = fc; return 0;
+  }
+  if (top) { pc->pr++; pc->per++; return 1; }
+  if (dot3(*vx, *vy, *vz, gnx, gny, gnz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_ENTRY_SIDE; return 0; }
+  const double rr = (double)radius, rr2 = rr * rr;
+  double qx = rr * (double)gnx, qy = rr * (double)gny, qz = rr * (double)gnz;
+  double qn0 = sqrt(qx * qx + qy * qy + qz * qz);
+  if (!(qn0 > 0.0) || !isfinite(qn0)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
+  qx *= rr / qn0; qy *= rr / qn0; qz *= rr / qn0;
+  float alpha_abs = (P->wavelength > 0.0f && k_p > 0.0f) ? (4.0f * PI_F * k_p / P->wavelength) : 0.0f;
+  for (int bounce = 0; bounce <= max_bounces; ++bounce) {
+    double dvx = *vx, dvy = *vy, dvz = *vz;
+    double A = dvx * dvx + dvy * dvy + dvz * dvz, q2 = qx * qx + qy * qy + qz * qz;
+    if (!(A > 0.0) || !isfinite(A) || !isfinite(q2)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
+    double qnorm = sqrt(q2);
+    if (!(qnorm > 0.0) || !isfinite(qnorm)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
+    double qp = rr / qnorm;
+    qx *= qp; qy *= qp; qz *= qp;
+    double B = qx * dvx + qy * dvy + qz * dvz;
+    if (!(B < 0.0) || !isfinite(B)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
+    double chord = -2.0 * B / A;
+    if (!(chord > 0.0) || !isfinite(chord)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
+    double tqx = qx + chord * dvx, tqy = qy + chord * dvy, tqz = qz + chord * dvz;
+    double arr = tqx * dvx + tqy * dvy + tqz * dvz, tq2 = tqx * tqx + tqy * tqy + tqz * tqz;
+    if (!(arr > 0.0) || !isfinite(arr) || !isfinite(tq2) || fabs(tq2 - rr2) > 512.0 * DEPS * fmax(rr2, 1.0e-30)) {
+      *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0;
+    }
+    if (alpha_abs > 0.0f) {
+      pc->n_abs++;
+      double dist = -log(Uopen(r)) / (double)alpha_abs;
+      if (dist < chord) {
+        qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
+        *x = (float)(cx + qx); *y = (float)(cy + qy); *z = (float)(cz + qz);
+        pc->pabs++;
+        return 2;
+      }
+    }
+    qx = tqx; qy = tqy; qz = tqz;
+    double qn = sqrt(qx * qx + qy * qy + qz * qz);
+    if (!(qn > 0.0) || !isfinite(qn)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
+    double pj = rr / qn;
+    qx *= pj; qy *= pj; qz *= pj;
+    *x = (float)(cx + qx); *y = (float)(cy + qy); *z = (float)(cz + qz);
+    float nox = (float)(qx / rr), noy = (float)(qy / rr), noz = (float)(qz / rr);
+    normalize3(&nox, &noy, &noz);
+    top = 1;
+    if (!smooth_interface(r, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc)) {
+      *pf = PF_MICRO_NUMERICAL; *pfs = fc; return 0;
+    }
+    if (!top) {
+      if (dot3(*vx, *vy, *vz, nox, noy, noz) <= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_EXIT_SIDE; return 0; }
+      return 1;
+    }
+    if (dot3(*vx, *vy, *vz, nox, noy, noz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
+    pc->pr++; pc->pir++;
+    if (bounce == max_bounces) { *pf = PF_MACRO_BOUNCE; *pfs = SUB_SPHERE_BOUNCE_LIMIT; return 0; }
+  }
+  *pf = PF_OTHER; *pfs = SUB_NONE;
+  return 0;
+}
+
+/* inverse-CDF polar angle with linear interpolation on the float table,
+ * CLARITAS.py:2632-2641 */
+static float sample_mie_theta(rng_t* r, const orc_model* m, int pidx) {
+  int nt = m->nt_sample;
+  const float* cdf = m->cdf_f + (size_t)pidx * nt;
+  const float* th = m->theta_f;
+  float u = U(r);
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) { int mid = (lo + hi) >> 1; if (u <= cdf[mid]) hi = mid; else lo = mid + 1; }
+  int j = lo;
+  if (j <= 0) return th[0];
+  float c0 = cdf[j - 1], c1 = cdf[j], t0 = th[j - 1], t1 = th[j];
+  float f = (c1 > c0) ? fminf(fmaxf((u - c0) / (c1 - c0), 0.0f), 1.0f) : 0.0f;
+  return t0 + f * (t1 - t0);
+}
+/* CLARITAS.py:2643-2652 */
+static void mie_scatter(rng_t* r, const orc_model* m, int pidx, float* vx, float* vy, float* vz) {
+  float th = sample_mie_theta(r, m, pidx);
+  float phi = 2.0f * PI_F * U(r);
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(*vx, *vy, *vz, &e1x, &e1y, &e1z, &e2x, &e2y, &e2z);
+  float st = sinf(th), ct = cosf(th), cp = cosf(phi), sp = sinf(phi);
+  float ox = ct * (*vx) + st * (cp * e1x + sp * e2x);
+  float oy = ct * (*vy) + st * (cp * e1y + sp * e2y);
+  float oz = ct * (*vz) + st * (cp * e1z + sp * e2z);
+  *vx = ox; *vy = oy; *vz = oz;
+  normalize3(vx, vy, vz);
+}
+
+typedef struct {
+  float x, y, z, vx, vy, vz;
+  int status, channel, failure, subcode;
+  pcount_t pc;
+  uint32_t trials, rejects, inaccessible;
+} cond_t;
+
+/* conditional finite-particle closure: the outcome (absorbed or not) is
+ * drawn from the Mie albedo first; channel candidates are retried until
+ * one realises it.  CLARITAS.py:4479-4536 */
+static cond_t conditional_event(const orc_model* m, rng_t* r, int pidx, float x, float y, float z,
+                                float vx, float vy, float vz, int max_trials) {
+  const fparams* P = &m->fp;
+  cond_t h;
+  memset(&h, 0, sizeof(h));
+  h.failure = PF_OTHER; h.subcode = 9002;
+  double qs = m->qsca[pidx], qe = m->qext[pidx];
+  float ki = m->k_f[pidx], nr = m->n_f[pidx], radius = m->r_f[pidx];
+  double tol = 1e-10 * fmax(1.0, fmax(fabs(qs), fabs(qe)));
+  if (max_trials == 0) { h.subcode = 9001; return h; }
+  if (!isfinite(qs) || !isfinite(qe) || qe <= 0.0 || qs < -tol || qs > qe + tol || max_trials < 1) return h;
+  int lossless = (ki == 0.0f && fabs(qe - qs) <= tol);
+  double albedo = lossless ? 1.0 : fmin(fmax(qs / qe, 0.0), 1.0);
+  double pabs = lossless ? 0.0 : fmin(fmax((qe - qs) / qe, 0.0), 1.0);
+  if ((ki > 0.0f && (pabs <= 0.0 || albedo >= 1.0)) || (albedo > 0.0 && albedo < 2.3283064365386962890625e-10) ||
+      (pabs > 0.0 && pabs < 2.3283064365386962890625e-10)) {
+    h.subcode = 9003; return h;
+  }
+  int target_abs = (albedo < 1.0 && U53(r) >= albedo);
+  double wave = qs > 0.0 ? fmax(qs - 1.0, 0.0) / qs : 0.0;
+  float coherent = P->coherent;
+  pcount_t acc;
+  memset(&acc, 0, sizeof(acc));
+  uint32_t trials = 0, rejects = 0, inacc = 0;
+  for (int trial = 0; trial < max_trials; ++trial) {
+    trials++;
+    cond_t c;
+    memset(&c, 0, sizeof(c));
+    c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
+    c.channel = (U53(r) < wave) ? ((coherent >= 1.0f || U53(r) < coherent) ? 1 : 2) : 0;
+    if (c.channel == 1) {
+      mie_scatter(r, m, pidx, &c.vx, &c.vy, &c.vz);
+      c.status = 1;
+    } else if (c.channel == 2) {
+      c.status = 0; c.failure = PF_SPHEROID; c.subcode = 0;   /* morphology is refused at prepare */
+    } else {
+      float gx, gy, gz;
+      sphere_sample(r, vx, vy, vz, &gx, &gy, &gz);
+      double dcx = (double)x - (double)radius * gx, dcy = (double)y - (double)radius * gy,
+             dcz = (double)z - (double)radius * gz;
+      if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { inacc++; continue; }
+      int pf = 0, pfs = 0;
+      c.status = sphere_interaction(P, r, P->n_water, nr, ki, radius, m->fp.max_pb, gx, gy, gz, dcx, dcy,
+                                    dcz, &c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.pc, &pf, &pfs);
+      c.failure = pf; c.subcode = pfs;
+    }
+    acc.n_abs += c.pc.n_abs;
+    if (c.status != 1 && c.status != 2) {
+      c.trials = trials; c.rejects = rejects; c.inaccessible = inacc; c.pc.n_abs = acc.n_abs;
+      return c;
+    }
+    if ((c.status == 2) != target_abs) { rejects++; continue; }
+    c.trials = trials; c.rejects = rejects; c.inaccessible = inacc; c.pc.n_abs = acc.n_abs;
+    c.failure = PF_OK; c.subcode = 0;
+    return c;
+  }
+  memset(&h, 0, sizeof(h));
+  h.failure = PF_OTHER; h.subcode = 9001;
+  h.trials = trials; h.rejects = rejects; h.inaccessible = inacc; h.pc.n_abs = acc.n_abs;
+  return h;
+}
+
+/* ======================================================================
+ * Detectors, CLARITAS.py:4237-4255, 4404-4435
+ * ====================================================================== */
+static int bore_accept(const fparams* P, float x, float y, float z, float vx, float vy, float vz,
+                       float ux, float uy, float* t_out) {
+  float den = vx * ux + vy * uy;
+  if (den <= 1e-10f) return 0;
+  float pdot = x * ux + y * uy;
+  float t0 = (P->ring_in - pdot) / den, t1 = (P->throat_out - pdot) / den, t2 = (P->ring_out - pdot) / den;
+  if (!(t0 > 0.0f && t1 > t0 && t2 > t1)) return 0;
+  float rth2 = P->throat_r * P->throat_r, rcb2 = P->cb_r * P->cb_r;
+  float q0x = x + t0 * vx - P->ring_in * ux, q0y = y + t0 * vy - P->ring_in * uy, q0z = z + t0 * vz;
+  float q1x = x + t1 * vx - P->throat_out * ux, q1y = y + t1 * vy - P->throat_out * uy, q1z = z + t1 * vz;
+  float q2x = x + t2 * vx - P->ring_out * ux, q2y = y + t2 * vy - P->ring_out * uy, q2z = z + t2 * vz;
+  if (q0x * q0x + q0y * q0y + q0z * q0z <= rth2 && q1x * q1x + q1y * q1y + q1z * q1z <= rth2 &&
+      q2x * q2x + q2y * q2y + q2z * q2z <= rcb2) {
+    *t_out = t0;
+    return 1;
+  }
+  return 0;
+}
+static int hardware_detector(const orc_model* m, float x, float y, float z, float vx, float vy,
+                             float vz, int mirror) {
+  if (mirror) { x = -x; vx = -vx; }
+  float best_t = INF_F, t;
+  int best = -1;
+  for (int j = 0; j < CLR_N_CHANNELS; ++j)
+    if (bore_accept(&m->fp, x, y, z, vx, vy, vz, m->det_ux[j], m->det_uy[j], &t) && t < best_t) { best_t = t; best = j; }
+  return best;
+}
+
+/* ======================================================================
+ * One history, CLARITAS.py:4606-5068
+ * ====================================================================== */
+typedef struct {
+  int st, pf, pfs, entered;
+  int ic, gic, wic, wcoh;
+  pcount_t pc;
+  photon_t ph;
+  uint32_t n_flight, n_null, trials, rejects, inacc, steps;
+  uint32_t draws;
+  int native, mirror;
+  int aux_native[CLR_MAX_AUX], aux_mirror[CLR_MAX_AUX];
+  int legacy_native[CLR_N_CHANNELS], legacy_mirror[CLR_N_CHANNELS];
+  float inc_native, inc_mirror;
+} hist_t;
+
+static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist_t* o) {
+  const fparams* P = &m->fp;
+  rng_t r;
+  rng_seed(&r, seed, hist);
+  memset(o, 0, sizeof(*o));
+  o->native = o->mirror = -1;
+  for (int a = 0; a < CLR_MAX_AUX; ++a) o->aux_native[a] = o->aux_mirror[a] = -1;
+  photon_t* p = &o->ph;
+  int st = 0, pf = PF_OK, pfs = SUB_NONE;
+  /* ---- launch, CLARITAS.py:4650-4697 ---- */
+  float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
+  if (P->src_model == CLR_SOURCE_COLLIMATED_DISK) {
+    float u1 = U(&r), u2 = U(&r);
+    float rho = P->beam_r * sqrtf(u1), ang = 2.0f * PI_F * u2;
+    x = rho * cosf(ang);
+    z = rho * sinf(ang);
+  } else {
+    float u1 = fmaxf(U(&r), 1e-12f), u2 = U(&r);
+    float mag = sqrtf(-2.0f * logf(u1));
+    x = P->sigma * mag * cosf(2.0f * PI_F * u2);
+    z = P->sigma * mag * sinf(2.0f * PI_F * u2);
+  }
+  y = -P->source_r;
+  if (P->src_model == CLR_SOURCE_LEGACY_POLAR || P->src_model == CLR_SOURCE_BETA_RADIANCE) {
+    float polar = (P->src_model == 0) ? beta_draw(&r, P->alpha1, P->alpha2) * (PI_F / 2.0f)
+                                      : beta_radiance_polar(&r, P->alpha1, P->alpha2);
+    float az = 2.0f * PI_F * U(&r), sp = sinf(polar);
+    vx = sp * cosf(az); vy = cosf(polar); vz = sp * sinf(az);
+  } else if (P->src_model == CLR_SOURCE_THESIS_PLANAR) {
+    float h = beta_draw(&r, P->alpha1, P->alpha2) * (PI_F / 2.0f);
+    if (U(&r) < 0.5f) h = -h;
+    vx = sinf(h); vy = cosf(h); vz = 0.0f;
+  }
+  normalize3(&vx, &vy, &vz);
+  p->x = x; p->y = y; p->z = z; p->vx = vx; p->vy = vy; p->vz = vz;
+  if (vy <= 1e-12f) { st = 3; goto WRITE_OUT; }
+  /* absorbing bore: hard apertures, CLARITAS.py:4681-4692 */
+  {
+    if (x * x + z * z > P->cb_r * P->cb_r) { st = 3; goto WRITE_OUT; }
+    float t = (-P->throat_out - y) / vy;
+    if (t < -EPS_F) { st = 3; goto WRITE_OUT; }
+    if (t < 0.0f) t = 0.0f;
+    float xt = x + t * vx, zt = z + t * vz;
+    if (xt * xt + zt * zt > P->throat_r * P->throat_r) { st = 3; goto WRITE_OUT; }
+    t = (-P->ring_in - y) / vy;
+    if (t <= 0.0f) { st = 3; goto WRITE_OUT; }
+    x += t * vx; y += t * vy; z += t * vz;
+    if (x * x + z * z > P->throat_r * P->throat_r) { st = 3; goto WRITE_OUT; }
+  }
+  /* outer acrylic wall, CLARITAS.py:4700-4735 */
+  {
+    double td = finite_cylinder_forward_d(x, y, z, vx, vy, vz, P->rout, P->zlow, P->zwall_top);
+    float t = (td < 0.5 * (double)INF_F) ? (float)td : INF_F;
+    if (t >= INF_F * 0.5f) { st = 3; goto WRITE_OUT; }
+    x += t * vx; y += t * vy; z += t * vz;
+    float nx, ny, a, b, c;
+    radial_normal(x, y, &nx, &ny);
+    float ci = fminf(fmaxf(-dot3(vx, vy, vz, nx, ny, 0.0f), 0.0f), 1.0f);
+    int tir = 0;
+    float R = fresnel_R(ci, P->n_air, P->n_acrylic, &tir);
+    p->n_iface++;
+    p->x = x; p->y = y; p->z = z;
+    if (tir || U(&r) < R || !refract3(vx, vy, vz, nx, ny, 0.0f, P->n_air, P->n_acrylic, &a, &b, &c)) {
+      reflect3(vx, vy, vz, nx, ny, 0.0f, &a, &b, &c);
+      p->vx = a; p->vy = b; p->vz = c; p->cor++; if (tir) p->ctir++;
+      p->x = x + EPS_F * nx; p->y = y + EPS_F * ny;
+      st = 1;
+      goto AIR_EXIT;
+    }
+    p->vx = a; p->vy = b; p->vz = c; p->x = x - EPS_F * nx; p->y = y - EPS_F * ny;
+    int sub = 0;
+    int where = acrylic_solid(P, &r, p, &sub);
+    if (where == 2) { st = 1; goto AIR_EXIT; }
+    if (where == 3) { st = 7; goto WRITE_OUT; }
+    if (where == 5) { st = 8; goto WRITE_OUT; }
+    if (where == 4) {
+      int hsub = 0, hw = headspace(P, &r, p, &hsub);
+      if (hw == 1) o->entered = 1;
+      else if (hw == 2) { st = 1; goto AIR_EXIT; }
+      else if (hw == 3) { st = 7; goto WRITE_OUT; }
+      else if (hw == 4) { st = 6; goto WRITE_OUT; }
+      else if (hw == 5) { st = 8; goto WRITE_OUT; }
+      else { pf = PF_HEADSPACE_NO_HIT; pfs = hsub; st = 4; goto WRITE_OUT; }
+    } else if (where != 1) {
+      pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_ACRYLIC_NO_HIT; pfs = sub; st = 4;
+      goto WRITE_OUT;
+    }
+    o->entered = 1;
+  }
+  /* ---- water loop, CLARITAS.py:4734-5066 ---- */
+  {
+    const float MU_TOTAL = P->mu_geom + P->mu_wave;
+    const float MU_TRACK = (P->sed_model == CLR_SED_UNIFORM) ? MU_TOTAL : (P->maj_geom + P->maj_wave);
+    const double ri2w = (double)P->rin * (double)P->rin, wtol = 8.0 * FEPS * ri2w;
+    int step;
+    for (step = 0; step < P->max_iter; ++step) {
+      o->steps++;
+      if (!isfinite(p->x) || !isfinite(p->y) || !isfinite(p->z)) { pf = PF_NONFINITE_POSITION; pfs = SUB_NONFINITE_POSITION; st = 4; break; }
+      if (!isfinite(p->vx) || !isfinite(p->vy) || !isfinite(p->vz)) { pf = PF_NONFINITE_DIRECTION; pfs = SUB_NONFINITE_DIRECTION; st = 4; break; }
+      float zsh = free_surface_z(P, p->x, p->y);
+      if (p->z <= P->zmin || p->z >= zsh) { st = 2; break; }
+      double rr2 = (double)p->x * (double)p->x + (double)p->y * (double)p->y;
+      if (rr2 > ri2w + wtol) { pf = PF_INVALID_MEDIUM; pfs = SUB_WATER_OUTSIDE; st = 4; break; }
+      double tsd = bore_wall_hit_d(p->x, p->y, p->z, p->vx, p->vy, p->vz, P->rin, P->zmin, P->zwall_top);
+      if (tsd < 0.0) { pf = PF_INVALID_MEDIUM; pfs = SUB_WATER_OUTSIDE; st = 4; break; }
+      float tside = (tsd < 0.5 * (double)INF_F) ? (float)tsd : INF_F;
+      float ttop = INF_F, tbot = INF_F;
+      if (p->vz < -1e-14f) tbot = (P->zmin - p->z) / p->vz;
+      {
+        /* exact skip: the segment up to the side/bottom stays below the
+         * lowest point of the surface, so no surface hit can come first */
+        float tc = fminf(tside, (tbot > 0.0f) ? tbot : INF_F);
+        int skip = 0;
+        if (P->fs_model != 0 && tc < 0.5f * INF_F) {
+          float zend = p->z + tc * p->vz;
+          skip = (fmaxf(p->z, zend) < P->zs_min);
+        }
+        if (!skip) ttop = free_surface_hit(P, p->x, p->y, p->z, p->vx, p->vy, p->vz);
+      }
+      if (!(ttop > 0.0f)) ttop = INF_F;
+      if (!(tbot > 0.0f)) tbot = INF_F;
+      int axial_top = (ttop <= tbot && ttop < tside);
+      int axial_bot = (tbot < ttop && tbot < tside);
+      float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
+      if (tbound >= INF_F * 0.5f) { pf = PF_WATER_NO_BOUNDARY; pfs = SUB_WATER_NO_BOUNDARY; st = 4; break; }
+      float freep = INF_F;
+      if (tbound > 0.0f && MU_TRACK > 0.0f) {
+        float u = fmaxf(U(&r), 1e-12f);
+        freep = -logf(u) / MU_TRACK;
+      }
+      int do_particle = freep < tbound;
+      float travel = do_particle ? freep : tbound;
+      o->n_flight++;
+      p->x += travel * p->vx; p->y += travel * p->vy; p->z += travel * p->vz;
+      if (do_particle) {
+        double er2 = (double)p->x * p->x + (double)p->y * p->y;
+        if (er2 > ri2w + wtol || p->z < P->zmin - EPS_F || p->z > free_surface_z(P, p->x, p->y) + EPS_F) {
+          pf = PF_INVALID_MEDIUM; pfs = SUB_WATER_OUTSIDE; st = 4; break;
+        }
+        float LG = P->mu_geom, LW = P->mu_wave, LT = MU_TOTAL;
+        if (P->sed_model != CLR_SED_UNIFORM) {
+          double lg = 0.0, lw = 0.0;
+          if (P->sed_model == CLR_SED_RZ_FIELD || P->sed_model == CLR_SED_XYZ_FIELD) {
+            double s = sediment_local_scale(m, 0, p->x, p->y, p->z);
+            lg = m->sum_trace_g * s; lw = m->sum_trace_w * s;
+          } else {
+            for (int i = 0; i < m->n; ++i) {
+              double sc = sediment_local_scale(m, i, p->x, p->y, p->z);
+              lg += m->trace_g[i] * sc; lw += m->trace_w[i] * sc;
+            }
+          }
+          LG = (float)lg; LW = (float)lw; LT = LG + LW;
+          if (!(LT > 0.0f)) { o->n_null++; continue; }
+          float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
+          if (LT > MU_TRACK + slack) { pf = PF_MAJORANT; pfs = SUB_MAJORANT; st = 4; break; }
+          float pacc = LT / fmaxf(MU_TRACK, 1e-30f);
+          if (pacc > 1.0f) pacc = 1.0f;
+          if (U(&r) >= pacc) { o->n_null++; continue; }
+        }
+        o->ic++;
+        int use_wave = (LW > 0.0f && U(&r) * LT >= LG);
+        int em = P->event_model;
+        if (em == CLR_EVENT_PLAIN_MIE) {
+          int b = select_bin(&r, m, 0, p->x, p->y, p->z, LG);
+          o->pc.n_abs++;
+          if (U(&r) >= (float)m->albedo[b]) { o->pc.pabs++; st = 5; break; }
+          o->wic++; o->wcoh++;
+          mie_scatter(&r, m, b, &p->vx, &p->vy, &p->vz);
+          continue;
+        }
+        int cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
+        int sel = -1;
+        if (cond) sel = select_bin(&r, m, use_wave, p->x, p->y, p->z, use_wave ? LW : LG);
+        if (cond && m->ki[sel] > 0.0) {
+          cond_t h = conditional_event(m, &r, sel, p->x, p->y, p->z, p->vx, p->vy, p->vz, P->cond_trials);
+          o->trials += h.trials; o->rejects += h.rejects; o->inacc += h.inaccessible; o->pc.n_abs += h.pc.n_abs;
+          if (h.status != 1 && h.status != 2) { pf = h.failure ? h.failure : PF_OTHER; pfs = h.subcode; st = 4; break; }
+          p->x = h.x; p->y = h.y; p->z = h.z; p->vx = h.vx; p->vy = h.vy; p->vz = h.vz;
+          o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
+          if (h.channel == 0) o->gic++; else { o->wic++; if (h.channel == 1) o->wcoh++; }
+          if (h.status == 2) { st = 5; break; }
+          continue;
+        }
+        if (use_wave) {
+          o->wic++;
+          int pidx = sel >= 0 ? sel : select_bin(&r, m, 1, p->x, p->y, p->z, LW);
+          int coherent = (P->coherent >= 1.0f) || (U(&r) < P->coherent);
+          if (!coherent) { pf = PF_SPHEROID; pfs = 0; st = 4; break; }
+          o->wcoh++;
+          mie_scatter(&r, m, pidx, &p->vx, &p->vy, &p->vz);
+          continue;
+        }
+        o->gic++;
+        int pidx = sel >= 0 ? sel : select_bin(&r, m, 0, p->x, p->y, p->z, LG);
+        float rad = m->r_f[pidx], gx, gy, gz;
+        sphere_sample(&r, p->vx, p->vy, p->vz, &gx, &gy, &gz);
+        double dcx = (double)p->x - (double)rad * (double)gx;
+        double dcy = (double)p->y - (double)rad * (double)gy;
+        double dcz = (double)p->z - (double)rad * (double)gz;
+        int accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
+        if (!accessible && cond) {
+          o->inacc++;
+          cond_t h = conditional_event(m, &r, pidx, p->x, p->y, p->z, p->vx, p->vy, p->vz, P->cond_trials - 1);
+          o->trials += h.trials; o->rejects += h.rejects; o->inacc += h.inaccessible; o->pc.n_abs += h.pc.n_abs;
+          if (h.status != 1) { pf = h.failure ? h.failure : PF_OTHER; pfs = h.subcode; st = 4; break; }
+          p->x = h.x; p->y = h.y; p->z = h.z; p->vx = h.vx; p->vy = h.vy; p->vz = h.vz;
+          o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
+          o->gic--;
+          if (h.channel == 0) o->gic++; else { o->wic++; if (h.channel == 1) o->wcoh++; }
+          continue;
+        }
+        if (!accessible) { o->ic--; o->gic--; o->n_null++; continue; }
+        int ppf = 0, ppfs = 0;
+        int ps = sphere_interaction(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], rad, P->max_pb, gx, gy, gz,
+                                    dcx, dcy, dcz, &p->x, &p->y, &p->z, &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
+        if (ps == 2) { st = 5; break; }
+        if (ps != 1) { pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+        continue;
+      }
+      if (axial_top || axial_bot) {
+        float a, b, c;
+        int tir = 0;
+        if (axial_top) {
+          float snx = 0.0f, sny = 0.0f, snz = -1.0f;
+          free_surface_normal(P, p->x, p->y, &snx, &sny, &snz);
+          float ci = fminf(fmaxf(-dot3(p->vx, p->vy, p->vz, snx, sny, snz), 0.0f), 1.0f);
+          float R = fresnel_R(ci, P->n_water, P->n_air, &tir);
+          p->n_iface++;
+          if (tir || U(&r) < R || !refract3(p->vx, p->vy, p->vz, snx, sny, snz, P->n_water, P->n_air, &a, &b, &c)) {
+            reflect3(p->vx, p->vy, p->vz, snx, sny, snz, &a, &b, &c);
+            p->vx = a; p->vy = b; p->vz = c; p->tr++; if (tir) p->ttir++;
+            if (!water_surface_offset(P, &p->x, &p->y, &p->z, snx, sny, snz)) { pf = PF_INVALID_MEDIUM; pfs = SUB_WATER_OUTSIDE; st = 4; break; }
+            continue;
+          }
+          p->vx = a; p->vy = b; p->vz = c;
+          p->x -= EPS_F * snx; p->y -= EPS_F * sny; p->z -= EPS_F * snz;
+          int hsub = 0, hw = headspace(P, &r, p, &hsub);
+          if (hw == 1) continue;
+          if (hw == 2) { st = 1; goto AIR_EXIT; }
+          if (hw == 3) { st = 7; goto WRITE_OUT; }
+          if (hw == 4) { st = 6; goto WRITE_OUT; }
+          pf = PF_HEADSPACE_NO_HIT; pfs = hsub; st = 4; goto WRITE_OUT;
+        } else {
+          int bsub = 0, where = bottom_disc(P, &r, p, &bsub);
+          if (where == 1) continue;
+          if (where == 2) { st = 1; goto AIR_EXIT; }
+          if (where == 3) { st = 7; break; }
+          if (where == 5) { st = 8; break; }
+          if (where == 4) {
+            int hsub = 0, hw = headspace(P, &r, p, &hsub);
+            if (hw == 1) continue;
+            if (hw == 2) { st = 1; goto AIR_EXIT; }
+            if (hw == 3) { st = 7; goto WRITE_OUT; }
+            if (hw == 4) { st = 6; goto WRITE_OUT; }
+            pf = PF_HEADSPACE_NO_HIT; pfs = hsub; st = 4; goto WRITE_OUT;
+          }
+          pf = (bsub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_BOTTOM_GEOMETRY; pfs = bsub; st = 4;
+          break;
+        }
+      }
+      /* water -> acrylic at the side wall, CLARITAS.py:5038-5065 */
+      {
+        float nx, ny, a, b, c;
+        radial_normal(p->x, p->y, &nx, &ny);
+        float ci = fminf(fmaxf(dot3(p->vx, p->vy, p->vz, nx, ny, 0.0f), 0.0f), 1.0f);
+        int tir = 0;
+        float R = fresnel_R(ci, P->n_water, P->n_acrylic, &tir);
+        p->n_iface++;
+        if (tir || U(&r) < R || !refract3(p->vx, p->vy, p->vz, -nx, -ny, 0.0f, P->n_water, P->n_acrylic, &a, &b, &c)) {
+          reflect3(p->vx, p->vy, p->vz, nx, ny, 0.0f, &a, &b, &c);
+          p->vx = a; p->vy = b; p->vz = c; p->cir++; if (tir) p->ctir++;
+          p->x -= EPS_F * nx; p->y -= EPS_F * ny;
+          continue;
+        }
+        p->vx = a; p->vy = b; p->vz = c; p->x += EPS_F * nx; p->y += EPS_F * ny;
+        int sub = 0, where = acrylic_solid(P, &r, p, &sub);
+        if (where == 1) continue;
+        if (where == 2) { st = 1; goto AIR_EXIT; }
+        if (where == 3) { st = 7; break; }
+        if (where == 5) { st = 8; break; }
+        if (where == 4) {
+          int hsub = 0, hw = headspace(P, &r, p, &hsub);
+          if (hw == 1) continue;
+          if (hw == 2) { st = 1; goto AIR_EXIT; }
+          if (hw == 3) { st = 7; goto WRITE_OUT; }
+          if (hw == 4) { st = 6; goto WRITE_OUT; }
+          if (hw == 5) { st = 8; goto WRITE_OUT; }
+          pf = PF_HEADSPACE_NO_HIT; pfs = hsub; st = 4; goto WRITE_OUT;
+        }
+        pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_ACRYLIC_NO_HIT; pfs = sub; st = 4;
+        break;
+      }
+    }
+    if (st == 0) { pf = PF_ITERATION_LIMIT; pfs = SUB_ITERATION_LIMIT; st = 4; }
+    goto WRITE_OUT;
+  }
+AIR_EXIT:
+  /* detector scoring, CLARITAS.py:4397-4435 (absorbing bores, ideal photodiode) */
+  {
+    float ex = p->x, ey = p->y, ez = p->z, evx = p->vx, evy = p->vy, evz = p->vz;
+    if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
+      o->native = hardware_detector(m, ex, ey, ez, evx, evy, evz, 0);
+      o->mirror = hardware_detector(m, ex, ey, ez, evx, evy, evz, 1);
+      if (o->native >= 0) {
+        float c = fminf(fmaxf(evx * m->det_ux[o->native] + evy * m->det_uy[o->native], -1.0f), 1.0f);
+        o->inc_native = acosf(c) * (180.0f / PI_F);
+      }
+      if (o->mirror >= 0) {
+        float c = fminf(fmaxf(-evx * m->det_ux[o->mirror] + evy * m->det_uy[o->mirror], -1.0f), 1.0f);
+        o->inc_mirror = acosf(c) * (180.0f / PI_F);
+      }
+      for (int a = 0; a < P->n_aux; ++a) {
+        float t;
+        o->aux_native[a] = bore_accept(P, ex, ey, ez, evx, evy, evz, m->aux_ux[a], m->aux_uy[a], &t) ? 1 : 0;
+        o->aux_mirror[a] = bore_accept(P, -ex, ey, ez, -evx, evy, evz, m->aux_ux[a], m->aux_uy[a], &t) ? 1 : 0;
+      }
+      for (int mir = 0; mir < 2; ++mir) {
+        float xx = mir ? -ex : ex;
+        float ang = atan2f(xx, ey) * (180.0f / PI_F);
+        if (ang < 0.0f) ang += 360.0f;
+        if (ang < 0.0f || ang > 180.0f) continue;
+        for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+          float deg = 10.0f * j;
+          if (o->ic <= 0 && deg >= 90.0f) continue;
+          if (fabsf(ang - deg) <= P->legacy_acc) {
+            if (mir) o->legacy_mirror[j] = 1; else o->legacy_native[j] = 1;
+          }
+        }
+      }
+    }
+  }
+WRITE_OUT:
+  o->st = st; o->pf = pf; o->pfs = pfs;
+  o->draws = r.draws;
+}
+
+/* ======================================================================
+ * Tally one history
+ * ====================================================================== */
+static void tally(clr_tallies* T, const hist_t* h, int g) {
+  T->counters[CLR_C_HISTORIES]++;
+  T->group_histories[g]++;
+  if (h->st >= 0 && h->st < CLR_N_STATUS) T->status[h->st]++;
+  if (h->st == 4 && h->pf >= 0 && h->pf < CLR_N_FAILURE) T->failures[h->pf]++;
+  const photon_t* p = &h->ph;
+  if (h->st == 1 && isfinite(p->x) && isfinite(p->vx)) T->counters[CLR_C_VALID_AIR]++;
+  if (h->st == 1) T->counters[CLR_C_N_EXIT]++;
+  if (h->entered) {
+    T->counters[CLR_C_ENTERED]++;
+    T->counters[CLR_C_SUM_IC] += (uint64_t)h->ic;
+    T->counters[CLR_C_SUM_GIC] += (uint64_t)h->gic;
+    T->counters[CLR_C_SUM_WIC] += (uint64_t)h->wic;
+    T->counters[CLR_C_SUM_WCOH] += (uint64_t)h->wcoh;
+    if (h->ic == 0) T->counters[CLR_C_BALLISTIC]++;
+    T->ic_hist[h->ic < CLR_N_ICHIST - 1 ? h->ic : CLR_N_ICHIST - 1]++;
+    T->counters[CLR_C_SUM_PR] += (uint64_t)h->pc.pr;
+    if (h->pc.per > 0) T->counters[CLR_C_RAYS_PER]++;
+    if (h->pc.pir > 0) T->counters[CLR_C_RAYS_PIR]++;
+    if (h->pc.pabs > 0) T->counters[CLR_C_RAYS_PABS]++;
+    T->counters[CLR_C_SUM_CIR] += (uint64_t)p->cir;
+    T->counters[CLR_C_SUM_COR] += (uint64_t)p->cor;
+    if (p->ctir > 0) T->counters[CLR_C_RAYS_CTIR]++;
+    T->counters[CLR_C_SUM_TOPR] += (uint64_t)p->tr;
+    if (p->ttir > 0) T->counters[CLR_C_RAYS_TOPTIR]++;
+    T->counters[CLR_C_SUM_BIR] += (uint64_t)p->bir;
+    T->counters[CLR_C_SUM_BOR] += (uint64_t)p->bor;
+    if (p->btir > 0) T->counters[CLR_C_RAYS_BTIR]++;
+  }
+  T->counters[CLR_C_N_FLIGHT] += h->n_flight;
+  T->counters[CLR_C_N_NULL] += h->n_null;
+  T->counters[CLR_C_N_SCAT] += (uint64_t)(h->ic > 0 ? h->ic : 0);
+  T->counters[CLR_C_N_IFACE] += p->n_iface;
+  T->counters[CLR_C_N_ABS] += h->pc.n_abs;
+  T->counters[CLR_C_COND_TRIALS] += h->trials;
+  T->counters[CLR_C_COND_REJECT] += h->rejects;
+  T->counters[CLR_C_COND_INACCESSIBLE] += h->inacc;
+  T->counters[CLR_C_RNG_DRAWS] += h->draws;
+  T->counters[CLR_C_N_STEPS] += h->steps;
+  if (h->native >= 0) {
+    T->det[g][0][h->native]++;
+    T->incidence_sum[0][h->native] += h->inc_native;
+  }
+  if (h->mirror >= 0) {
+    T->det[g][1][h->mirror]++;
+    T->incidence_sum[1][h->mirror] += h->inc_mirror;
+  }
+  for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+    if (h->legacy_native[j]) T->det[g][2][j]++;
+    if (h->legacy_mirror[j]) T->det[g][3][j]++;
+  }
+  for (int a = 0; a < CLR_MAX_AUX; ++a) {
+    if (h->aux_native[a] > 0) T->aux[g][0][a]++;
+    if (h->aux_mirror[a] > 0) T->aux[g][1][a]++;
+  }
+}
+
+static void merge(clr_tallies* D, const clr_tallies* S) {
+  const uint64_t* s = (const uint64_t*)S->det;
+  uint64_t* d = (uint64_t*)D->det;
+  for (size_t i = 0; i < sizeof(S->det) / 8; ++i) d[i] += s[i];
+  s = (const uint64_t*)S->aux; d = (uint64_t*)D->aux;
+  for (size_t i = 0; i < sizeof(S->aux) / 8; ++i) d[i] += s[i];
+  for (int g = 0; g < CLR_MAX_GROUPS; ++g) D->group_histories[g] += S->group_histories[g];
+  for (int k = 0; k < 2; ++k)
+    for (int c = 0; c < CLR_N_CHANNELS; ++c) D->incidence_sum[k][c] += S->incidence_sum[k][c];
+  for (int i = 0; i < CLR_N_STATUS; ++i) D->status[i] += S->status[i];
+  for (int i = 0; i < CLR_N_FAILURE; ++i) D->failures[i] += S->failures[i];
+  for (int i = 0; i < CLR_N_COUNTERS; ++i) D->counters[i] += S->counters[i];
+  for (int i = 0; i < CLR_N_ICHIST; ++i) D->ic_hist[i] += S->ic_hist[i];
+}
+
+int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+            clr_tallies* out, clr_record* rec, uint64_t n_rec) {
+  if (!m || !out) return CLR_E_ARG;
+  memset(out, 0, sizeof(*out));
+  int G = m->fp.n_groups;
+  out->n_groups = G;
+  if (n == 0) return 0;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  int nt = 1;
+  (void)nthreads;
+#endif
+  clr_tallies* part = (clr_tallies*)calloc((size_t)nt, sizeof(clr_tallies));
+  if (!part) return CLR_E_NOMEM;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+  {
+#ifdef _OPENMP
+    int tid = omp_get_thread_num();
+#else
+    int tid = 0;
+#endif
+    clr_tallies* T = &part[tid];
+    hist_t h;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      uint64_t hist = offset + (uint64_t)i;
+      trace_history(m, seed, hist, &h);
+      int g = (int)(((unsigned __int128)(uint64_t)i * (unsigned)G) / n);
+      tally(T, &h, g);
+      if (rec && (uint64_t)i < n_rec) {
+        clr_record* q = &rec[i];
+        q->x = h.ph.x; q->y = h.ph.y; q->z = h.ph.z; q->vx = h.ph.vx; q->vy = h.ph.vy; q->vz = h.ph.vz;
+        q->status = h.st; q->native_id = h.native; q->mirror_id = h.mirror; q->interactions = h.ic;
+        q->failure = h.pf; q->draws = (int32_t)h.draws;
+      }
+    }
+  }
+  for (int t = 0; t < nt; ++t) merge(out, &part[t]);
+  free(part);
+  return 0;
+}
+
+/* ======================================================================
+ * Model construction
+ * ====================================================================== */
+static void fill_fparams(orc_model* m) {
+  const clr_config* c = &m->cfg;
+  fparams* P = &m->fp;
+  P->n_water = (float)c->n_water; P->n_acrylic = (float)c->n_acrylic; P->n_air = (float)c->n_air;
+  P->n_bottom_ext = (float)c->bottom_external_n;
+  P->rin = (float)c->rin; P->rout = (float)c->rout; P->zmin = (float)c->zmin; P->zmean = (float)c->zmean;
+  P->zwall_top = (float)c->zwall_top; P->bdt = (float)c->bottom_disc_thickness; P->zlow = P->zmin - P->bdt;
+  P->ring_in = (float)c->ring_in; P->throat_out = (float)c->throat_out; P->ring_out = (float)c->ring_out;
+  P->source_r = (float)c->source_r; P->throat_r = (float)c->throat_r; P->cb_r = (float)c->counterbore_r;
+  P->dh = (float)c->vortex_delta_h; P->core_r = (float)c->vortex_core_radius;
+  P->alpha1 = (float)c->alpha1; P->alpha2 = (float)c->alpha2; P->sigma = (float)c->beam_sigma;
+  P->beam_r = (float)c->beam_radius; P->legacy_acc = (float)c->legacy_accept_deg;
+  P->wavelength = (float)c->wavelength;
+  P->fs_model = c->free_surface_model; P->src_model = c->source_angular_model;
+  P->event_model = c->event_model; P->sed_model = c->sediment_model;
+  P->max_pb = c->max_particle_bounces; P->max_iter = c->max_iterations;
+  P->cond_trials = c->conditional_max_trials; P->n_aux = c->n_aux;
+  P->n_groups = (c->n_groups >= 1 && c->n_groups <= CLR_MAX_GROUPS) ? c->n_groups : CLR_MAX_GROUPS;
+  P->fn0 = c->field_n0; P->fn1 = c->field_n1; P->fn2 = c->field_n2; P->fnbins = c->field_nbins;
+  P->field_zmax = (float)c->field_zmax; P->field_half = (float)c->field_xy_half;
+  /* Rankine constants (K = wall-minus-centre drop / (2 - a^2/R^2)) */
+  if (c->free_surface_model == CLR_SURFACE_RANKINE) {
+    double R = c->rin, a = c->vortex_core_radius, R2 = R * R, a2 = a * a;
+    double K = c->vortex_delta_h / (2.0 - a2 / R2);
+    double hbar = (2.0 * K / R2) * (R2 - 0.75 * a2 - a2 * log(R / a));
+    P->rk_K = K; P->rk_C0 = c->zmean - hbar;
+  } else { P->rk_K = 0.0; P->rk_C0 = c->zmean; }
+  P->zs_min = (P->fs_model == 0 || P->dh <= 0.0f) ? P->zmean : free_surface_z(P, 0.0f, 0.0f);
+  P->zs_min = P->zs_min - 1e-6f * fmaxf(fabsf(P->zs_min), 1e-3f);
+  for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+    float th = (10.0f * (float)j) * (PI_F / 180.0f);
+    m->det_ux[j] = (float)sin((double)th);
+    m->det_uy[j] = (float)cos((double)th);
+  }
+  for (int a = 0; a < CLR_MAX_AUX; ++a) {
+    float th = (float)c->aux_angles_deg[a] * (PI_F / 180.0f);
+    m->aux_ux[a] = (float)sin((double)th);
+    m->aux_uy[a] = (float)cos((double)th);
+  }
+}
+
+void orc_free(orc_model* m) {
+  if (!m) return;
+  double* arrs[] = {m->d, m->w, m->nr, m->ki, m->x, m->qext, m->qsca, m->qabs, m->qback, m->g, m->albedo,
+                    m->theta, m->pdf, m->cdf, m->radius, m->nd, m->mug, m->mus, m->mue, m->muw, m->muev,
+                    m->gcdf, m->wcdf, m->trace_g, m->trace_w, m->sedA, m->sedB, m->sednorm, m->sedws, m->radlog};
+  for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); ++i) free(arrs[i]);
+  free(m->nstop); free(m->theta_f); free(m->cdf_f); free(m->gcdf_f); free(m->wcdf_f); free(m->field);
+  free(m);
+}
+
+int orc_prepare(const clr_config* cfg, const clr_particles* p, const clr_sediment* sed,
+                const float* field, size_t field_len, orc_model** out) {
+  if (!cfg || !p || !out || p->n <= 0 || p->n > 512 || p->n_theta < 2) return CLR_E_ARG;
+  if (cfg->rough_alpha > 1e-7) return CLR_E_UNSUPPORTED;
+  orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
+  if (!m) return CLR_E_NOMEM;
+  m->cfg = *cfg;
+  int n = p->n, nt = p->n_theta;
+  m->n = n; m->nt = nt;
+  m->nrad = (sed && sed->n_radial >= 2) ? sed->n_radial : 2049;
+#define ALLOC(f, cnt) m->f = (double*)calloc((size_t)(cnt), sizeof(double))
+  ALLOC(d, n); ALLOC(w, n); ALLOC(nr, n); ALLOC(ki, n); ALLOC(x, n); ALLOC(qext, n); ALLOC(qsca, n);
+  ALLOC(qabs, n); ALLOC(qback, n); ALLOC(g, n); ALLOC(albedo, n); ALLOC(theta, nt);
+  ALLOC(pdf, (size_t)n * nt); ALLOC(cdf, (size_t)n * nt); ALLOC(radius, n); ALLOC(nd, n); ALLOC(mug, n);
+  ALLOC(mus, n); ALLOC(mue, n); ALLOC(muw, n); ALLOC(muev, n); ALLOC(gcdf, n); ALLOC(wcdf, n);
+  ALLOC(trace_g, n); ALLOC(trace_w, n); ALLOC(sedA, n); ALLOC(sedB, n); ALLOC(sednorm, n); ALLOC(sedws, n);
+  ALLOC(radlog, (size_t)n * m->nrad);
+#undef ALLOC
+  m->nstop = (int*)calloc((size_t)n, sizeof(int));
+  memcpy(m->d, p->diameter, sizeof(double) * n);
+  memcpy(m->w, p->weight, sizeof(double) * n);
+  memcpy(m->nr, p->n_real, sizeof(double) * n);
+  memcpy(m->ki, p->k_imag, sizeof(double) * n);
+  if (field && field_len) {
+    m->field = (float*)malloc(sizeof(float) * field_len);
+    memcpy(m->field, field, sizeof(float) * field_len);
+    m->field_len = field_len;
+  }
+  fill_fparams(m);
+  int err = 0;
+  clr_mie_out mo = {m->x, m->qext, m->qsca, m->qabs, m->qback, m->g, m->albedo, m->nstop, m->theta, m->pdf, m->cdf};
+  err = orc_mie(p, &mo);
+  if (!err) err = transport_precompute(m, p);
+  if (!err && m->mu_scalars[7] < 1.0 && (p->event_model == 3 || p->event_model == 4)) err = CLR_E_UNSUPPORTED;
+  if (!err) {
+    int tm = cfg->sediment_model;
+    if ((tm == CLR_SED_GRID_PER_BIN && (!m->field || field_len < (size_t)n * cfg->field_n0 * cfg->field_n1)) ||
+        (tm == CLR_SED_RZ_FIELD && (!m->field || field_len < (size_t)cfg->field_n0 * cfg->field_n1)) ||
+        (tm == CLR_SED_XYZ_FIELD && (!m->field || field_len < (size_t)cfg->field_n0 * cfg->field_n1 * cfg->field_n2)))
+      err = CLR_E_ARG;
+  }
+  if (!err) err = sediment_precompute(m, sed);
+  if (err) { orc_free(m); return err; }
+  m->fp.mu_geom = (float)m->mu_scalars[0];
+  m->fp.mu_wave = (float)m->mu_scalars[1];
+  m->fp.maj_geom = (float)m->maj[0];
+  m->fp.maj_wave = (float)m->maj[1];
+  m->fp.coherent = fminf(fmaxf((float)m->mu_scalars[7], 0.0f), 1.0f);
+  m->gcdf_f = (float*)malloc(sizeof(float) * n);
+  m->wcdf_f = (float*)malloc(sizeof(float) * n);
+  for (int i = 0; i < n; ++i) {
+    m->gcdf_f[i] = (float)m->gcdf[i]; m->wcdf_f[i] = (float)m->wcdf[i];
+    m->n_f[i] = (float)m->nr[i]; m->k_f[i] = (float)m->ki[i]; m->r_f[i] = (float)m->radius[i];
+  }
+  m->nt_sample = nt;
+  m->theta_f = (float*)malloc(sizeof(float) * nt);
+  m->cdf_f = (float*)malloc(sizeof(float) * (size_t)n * nt);
+  for (int j = 0; j < nt; ++j) m->theta_f[j] = (float)m->theta[j];
+  for (size_t j = 0; j < (size_t)n * nt; ++j) m->cdf_f[j] = (float)m->cdf[j];
+  *out = m;
+  return 0;
+}
+
+int orc_set_phase_table(orc_model* m, const double* theta, const double* cdf, int nt) {
+  if (!m || !theta || !cdf || nt < 2) return CLR_E_ARG;
+  free(m->theta_f); free(m->cdf_f);
+  m->nt_sample = nt;
+  m->theta_f = (float*)malloc(sizeof(float) * nt);
+  m->cdf_f = (float*)malloc(sizeof(float) * (size_t)m->n * nt);
+  for (int j = 0; j < nt; ++j) m->theta_f[j] = (float)theta[j];
+  for (size_t j = 0; j < (size_t)m->n * nt; ++j) m->cdf_f[j] = (float)cdf[j];
+  return 0;
+}
+
+int orc_get_tables(const orc_model* m, clr_tables_out* o) {
+  if (!m || !o) return CLR_E_ARG;
+  size_t n = (size_t)m->n;
+#define CP(dst, src) if (o->dst) memcpy(o->dst, m->src, sizeof(double) * n)
+  CP(radius, radius); CP(number_density, nd); CP(mu_geom_bin, mug); CP(mu_sca_bin, mus);
+  CP(mu_ext_bin, mue); CP(trace_geom_bin, trace_g); CP(trace_wave_bin, trace_w); CP(geom_cdf, gcdf);
+  CP(wave_cdf, wcdf); CP(sed_A, sedA); CP(sed_B, sedB); CP(sed_norm, sednorm); CP(sed_settling, sedws);
+#undef CP
+  if (o->sed_radial_log) memcpy(o->sed_radial_log, m->radlog, sizeof(double) * n * (size_t)m->nrad);
+  memcpy(o->mu_scalars, m->mu_scalars, sizeof(m->mu_scalars));
+  memcpy(o->majorants, m->maj, sizeof(m->maj));
+  return 0;
+}
+
+int orc_free_surface_probe(const clr_config* cfg, const float* rays6, int n, float* zs0, float* th,
+                           float* zsh, float* nx, float* ny, float* nz) {
+  orc_model m;
+  memset(&m, 0, sizeof(m));
+  m.cfg = *cfg;
+  fill_fparams(&m);
+  const fparams* P = &m.fp;
+  for (int i = 0; i < n; ++i) {
+    const float* q = rays6 + 6 * i;
+    zs0[i] = free_surface_z(P, q[0], q[1]);
+    float t = free_surface_hit(P, q[0], q[1], q[2], q[3], q[4], q[5]);
+    th[i] = t;
+    if (t < INF_F * 0.5f) {
+      float hx = q[0] + t * q[3], hy = q[1] + t * q[4];
+      zsh[i] = free_surface_z(P, hx, hy);
+      free_surface_normal(P, hx, hy, &nx[i], &ny[i], &nz[i]);
+    } else {
+      zsh[i] = INF_F; nx[i] = ny[i] = nz[i] = 0.0f;
+    }
+  }
+  return 0;
+}
+
+int orc_sediment_probe(const orc_model* m, int bin, const float* xyz, int n, double* out) {
+  if (!m || bin < 0 || bin >= m->n) return CLR_E_ARG;
+  for (int i = 0; i < n; ++i) out[i] = sediment_local_scale(m, bin, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  return 0;
+}
+
+/* ---- probes for closed-form tests ---- */
+int orc_fresnel_probe(int n, const float* ci, float n1, float n2, float* R, int* tir) {
+  for (int i = 0; i < n; ++i) R[i] = fresnel_R(ci[i], n1, n2, &tir[i]);
+  return 0;
+}
+int orc_complex_probe(int n, const float* ci, float n1r, float n1k, float n2r, float n2k, float* R, float* tt,
+                      float* tn, int* ok) {
+  for (int i = 0; i < n; ++i) ok[i] = complex_interface(ci[i], n1r, n1k, n2r, n2k, &R[i], &tt[i], &tn[i]);
+  return 0;
+}
+int orc_refract_probe(int n, const float* v, const float* nrm, float n1, float n2, float* out, int* ok) {
+  for (int i = 0; i < n; ++i)
+    ok[i] = refract3(v[3 * i], v[3 * i + 1], v[3 * i + 2], nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], n1, n2,
+                     &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
+  return 0;
+}
+/* water-side exit distance through r = R on [zlo, zhi] (CLARITAS.py:2929-2950) */
+int orc_wall_probe(int n, const float* rays6, float R, float zlo, float zhi, double* t) {
+  orc_model m;
+  memset(&m, 0, sizeof(m));
+  for (int i = 0; i < n; ++i) {
+    const float* q = rays6 + 6 * i;
+    t[i] = bore_wall_hit_d(q[0], q[1], q[2], q[3], q[4], q[5], R, zlo, zhi);
+  }
+  return 0;
+}

@@ -1,0 +1,62 @@
+/*
+ * oracle.h -- CPU restatement of the CLARITAS photon-transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker: it may be built, linked,
+ * imported or executed only by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Nothing in the product
+ * (tardiis_and_claritas_amd/, include/) includes or calls it.
+ *
+ * The structs of the public ABI (include/claritas.h) are reused so that one
+ * configuration can be handed to both the HIP library and this oracle.
+ */
+#ifndef CLARITAS_ORACLE_H
+#define CLARITAS_ORACLE_H
+#include "../include/claritas.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_model orc_model;
+
+/* Philox4x32-10 block function (Salmon et al., SC'11), exposed for KATs. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* First n draws of the per-history uniform stream (float, (0,1)). */
+void orc_uniform_stream(uint64_t seed, uint64_t history, int n, float* out);
+
+/* Mie tables for a PSD (FP64). Same outputs as clr_mie_tables. */
+int orc_mie(const clr_particles* p, clr_mie_out* out);
+
+/* Build the full model (Mie + transport + sediment tables). */
+int orc_prepare(const clr_config* cfg, const clr_particles* p, const clr_sediment* sed,
+                const float* field, size_t field_len, orc_model** out);
+void orc_free(orc_model* m);
+int orc_get_tables(const orc_model* m, clr_tables_out* out);
+/* Replace the Mie phase table the trace samples from (used to run the
+ * oracle on exactly the device's float tables in history-by-history tests). */
+int orc_set_phase_table(orc_model* m, const double* theta, const double* cdf, int n_theta);
+
+/* Trace n histories; tallies zeroed then filled.  nthreads<=0 uses all. */
+int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t history_offset,
+            int nthreads, clr_tallies* out, clr_record* records, uint64_t n_records);
+
+/* Geometry probes on the reference's test-kernel semantics
+ * (free_surface_geometry_test_kernel, CLARITAS.py:4438-4455). */
+int orc_free_surface_probe(const clr_config* cfg, const float* rays6, int n,
+                           float* zs_start, float* t_hit, float* zs_hit,
+                           float* nx, float* ny, float* nz);
+/* Local sediment density scale of bin `bin` at n points (CLARITAS.py:4228-4235). */
+int orc_sediment_probe(const orc_model* m, int bin, const float* xyz, int n, double* out);
+
+/* closed-form probes: unpolarised Fresnel R, complex-index interface,
+ * Snell refraction, water-side wall exit distance */
+int orc_fresnel_probe(int n, const float* ci, float n1, float n2, float* R, int* tir);
+int orc_complex_probe(int n, const float* ci, float n1r, float n1k, float n2r, float n2k, float* R, float* tt,
+                      float* tn, int* ok);
+int orc_refract_probe(int n, const float* v, const float* nrm, float n1, float n2, float* out, int* ok);
+int orc_wall_probe(int n, const float* rays6, float R, float zlo, float zhi, double* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,214 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  It builds the same ABI structs
+as the product through ClaritasForwardModel's struct builders, so the oracle
+and the HIP library see identical inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from tardiis_and_claritas_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / 'liboracle.so'
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(['make', '-s', '-C', str(HERE)], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    P = C.POINTER
+    lib.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.orc_uniform_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_int, P(C.c_float)]
+    lib.orc_mie.argtypes = [P(abi.Particles), P(abi.MieOut)]
+    lib.orc_mie.restype = C.c_int
+    lib.orc_prepare.argtypes = [P(abi.Config), P(abi.Particles), P(abi.Sediment), P(C.c_float), C.c_size_t,
+                                P(C.c_void_p)]
+    lib.orc_prepare.restype = C.c_int
+    lib.orc_free.argtypes = [C.c_void_p]
+    lib.orc_get_tables.argtypes = [C.c_void_p, P(abi.TablesOut)]
+    lib.orc_set_phase_table.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_int]
+    lib.orc_run.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(abi.Tallies),
+                            P(abi.Record), C.c_uint64]
+    lib.orc_run.restype = C.c_int
+    lib.orc_free_surface_probe.argtypes = [P(abi.Config), P(C.c_float), C.c_int] + [P(C.c_float)] * 6
+    lib.orc_sediment_probe.argtypes = [C.c_void_p, C.c_int, P(C.c_float), C.c_int, P(C.c_double)]
+    lib.orc_fresnel_probe.argtypes = [C.c_int, P(C.c_float), C.c_float, C.c_float, P(C.c_float), P(C.c_int)]
+    lib.orc_complex_probe.argtypes = [C.c_int, P(C.c_float), C.c_float, C.c_float, C.c_float, C.c_float,
+                                      P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_int)]
+    lib.orc_refract_probe.argtypes = [C.c_int, P(C.c_float), P(C.c_float), C.c_float, C.c_float, P(C.c_float),
+                                      P(C.c_int)]
+    lib.orc_wall_probe.argtypes = [C.c_int, P(C.c_float), C.c_float, C.c_float, C.c_float, P(C.c_double)]
+    _lib = lib
+    return lib
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def fresnel(ci, n1, n2):
+    ci = np.ascontiguousarray(ci, dtype=np.float32)
+    R = np.zeros_like(ci)
+    tir = np.zeros(ci.size, dtype=np.int32)
+    load().orc_fresnel_probe(ci.size, _pf(ci), n1, n2, _pf(R), tir.ctypes.data_as(C.POINTER(C.c_int)))
+    return R, tir
+
+
+def complex_interface(ci, n1r, n1k, n2r, n2k):
+    ci = np.ascontiguousarray(ci, dtype=np.float32)
+    R, tt, tn = (np.zeros_like(ci) for _ in range(3))
+    ok = np.zeros(ci.size, dtype=np.int32)
+    load().orc_complex_probe(ci.size, _pf(ci), n1r, n1k, n2r, n2k, _pf(R), _pf(tt), _pf(tn),
+                             ok.ctypes.data_as(C.POINTER(C.c_int)))
+    return R, tt, tn, ok
+
+
+def refract(v, nrm, n1, n2):
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float32)
+    out = np.zeros_like(v)
+    ok = np.zeros(len(v), dtype=np.int32)
+    load().orc_refract_probe(len(v), _pf(v), _pf(nrm), n1, n2, _pf(out), ok.ctypes.data_as(C.POINTER(C.c_int)))
+    return out, ok
+
+
+def wall_exit(rays6, R, zlo, zhi):
+    rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+    t = np.zeros(len(rays6))
+    load().orc_wall_probe(len(rays6), _pf(rays6), R, zlo, zhi, t.ctypes.data_as(C.POINTER(C.c_double)))
+    return t
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform_stream(seed, history, n):
+    out = np.zeros(n, dtype=np.float32)
+    load().orc_uniform_stream(seed, history, n, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def mie(model, material, n_theta=None):
+    """FP64 Mie tables of the oracle for the model's medium/wavelength."""
+    p, keep = model.particles_struct(material, 0.0, None, 'mass_fraction')
+    nt = int(n_theta or model.mie_phase_grid_size)
+    p.n_theta = nt
+    n = p.n
+    out = {k: np.zeros(n) for k in ('size_parameter', 'qext', 'qsca', 'qabs', 'qback', 'asym_g', 'albedo')}
+    out['nstop'] = np.zeros(n, dtype=np.int32)
+    out['theta'] = np.zeros(nt)
+    out['phase_pdf'] = np.zeros((n, nt))
+    out['phase_cdf'] = np.zeros((n, nt))
+    m = abi.MieOut()
+    PD = C.POINTER(C.c_double)
+    for k in ('size_parameter', 'qext', 'qsca', 'qabs', 'qback', 'asym_g', 'albedo', 'theta', 'phase_pdf',
+              'phase_cdf'):
+        setattr(m, k, out[k].ctypes.data_as(PD))
+    m.nstop = out['nstop'].ctypes.data_as(C.POINTER(C.c_int32))
+    rc = load().orc_mie(C.byref(p), C.byref(m))
+    if rc:
+        raise RuntimeError(f'oracle Mie failed ({rc})')
+    del keep
+    return out
+
+
+class OracleModel:
+    """Prepared oracle model for one configuration of a ClaritasForwardModel."""
+
+    def __init__(self, model, material, concentration_g_per_L, weights=None, weight_mode='mass_fraction'):
+        self.lib = load()
+        self.material = model.resolve_material(material)
+        cfg = model.config_struct(self.material)
+        p, keep = model.particles_struct(self.material, concentration_g_per_L, weights, weight_mode)
+        sed = model.sediment_struct(self.material)
+        f = model.sediment_field
+        fptr = f.ctypes.data_as(C.POINTER(C.c_float)) if f is not None else None
+        h = C.c_void_p()
+        rc = self.lib.orc_prepare(C.byref(cfg), C.byref(p), C.byref(sed), fptr, int(f.size) if f is not None else 0,
+                                  C.byref(h))
+        del keep
+        if rc:
+            raise RuntimeError(f'oracle prepare failed with code {rc}')
+        self.h = h
+        self.n_bins = p.n
+
+    def tables(self):
+        PD = C.POINTER(C.c_double)
+        arrs = {k: np.zeros(self.n_bins) for k in ('radius', 'number_density', 'mu_geom_bin', 'mu_sca_bin',
+                                                  'mu_ext_bin', 'trace_geom_bin', 'trace_wave_bin', 'geom_cdf',
+                                                  'wave_cdf', 'sed_A', 'sed_B', 'sed_norm', 'sed_settling')}
+        t = abi.TablesOut()
+        for k, v in arrs.items():
+            setattr(t, k, v.ctypes.data_as(PD))
+        self.lib.orc_get_tables(self.h, C.byref(t))
+        arrs['mu_scalars'] = np.array(list(t.mu_scalars))
+        arrs['majorants'] = np.array(list(t.majorants))
+        return arrs
+
+    def radial_log(self, n_radial):
+        PD = C.POINTER(C.c_double)
+        out = np.zeros((self.n_bins, n_radial))
+        t = abi.TablesOut()
+        t.sed_radial_log = out.ctypes.data_as(PD)
+        self.lib.orc_get_tables(self.h, C.byref(t))
+        return out
+
+    def set_phase_table(self, theta, cdf):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        cdf = np.ascontiguousarray(cdf, dtype=np.float64)
+        PD = C.POINTER(C.c_double)
+        self.lib.orc_set_phase_table(self.h, theta.ctypes.data_as(PD), cdf.ctypes.data_as(PD), theta.size)
+
+    def run(self, n, seed, history_offset=0, threads=0, n_records=0):
+        t = abi.Tallies()
+        rec = (abi.Record * int(n_records))() if n_records else None
+        rc = self.lib.orc_run(self.h, int(n), int(seed), int(history_offset), int(threads), C.byref(t), rec,
+                              int(n_records))
+        if rc:
+            raise RuntimeError(f'oracle run failed with code {rc}')
+        recs = abi.records_to_numpy(rec, int(n_records)) if n_records else None
+        return abi.tallies_to_numpy(t), recs
+
+    def sediment_scale(self, bin_index, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        out = np.zeros(len(xyz))
+        self.lib.orc_sediment_probe(self.h, int(bin_index), xyz.ctypes.data_as(C.POINTER(C.c_float)), len(xyz),
+                                    out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def __del__(self):
+        if getattr(self, 'h', None):
+            self.lib.orc_free(self.h)
+            self.h = None
+
+
+def free_surface_probe(model, rays6):
+    rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+    n = len(rays6)
+    outs = [np.zeros(n, dtype=np.float32) for _ in range(6)]
+    cfg = model.config_struct(None)
+    PF = C.POINTER(C.c_float)
+    load().orc_free_surface_probe(C.byref(cfg), rays6.ctypes.data_as(PF), n, *[o.ctypes.data_as(PF) for o in outs])
+    return dict(zip(['zs_start', 't_hit', 'zs_hit', 'nx', 'ny', 'nz'], outs))

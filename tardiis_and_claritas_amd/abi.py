@@ -1,0 +1,217 @@
+"""ctypes mirror of include/claritas.h.
+
+The field order and types follow the C header exactly; `check_sizes` compares
+every structure with the library's own `clr_sizeof`, the ctypes counterpart of
+the reference's launch-arity check (validate_claritas.py:972-1001).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+ABI_VERSION = 1
+N_CHANNELS = 18
+MAX_GROUPS = 64
+MAX_AUX = 4
+N_COUNTERS = 64
+N_ICHIST = 64
+N_STATUS = 9
+N_FAILURE = 19
+
+# event laws (CLARITAS.py:5543-5550); 6 = plain Mie (no reference code)
+EVENT_MODEL_CODES = {
+    'geometric': 0,
+    'mie_qsca': 1,
+    'mie_qext': 2,
+    'hybrid_mie_excess': 3,
+    'hybrid_mie_exact_rate': 4,
+    'plain_mie': 6,
+}
+WEIGHT_MODE_CODES = {'mass_fraction': 0, 'number_fraction': 1}
+SOURCE_ANGULAR_MODEL_CODES = {
+    'legacy_polar_pdf': 0,
+    'beta_radiance_3d': 1,
+    'thesis_planar_beta': 2,
+    'collimated_disk': 3,
+}
+FREE_SURFACE_MODEL_CODES = {
+    'flat': 0,
+    'parabolic_vortex': 1,
+    'localized_scully_vortex': 2,
+    'rankine_vortex': 3,
+}
+SEDIMENT_TRANSPORT_MODEL_CODES = {
+    'uniform': 0,
+    'stokes_drift_diffusion': 1,
+    'schiller_naumann_drift_diffusion': 2,
+    'grid_per_bin': 3,
+    'rz_field': 4,
+    'xyz_field': 5,
+}
+
+# counter indices (claritas.h enum)
+COUNTER_NAMES = [
+    'histories', 'entered', 'valid_air', 'sum_ic', 'sum_gic', 'sum_wic', 'sum_wcoh',
+    'ballistic', 'sum_pr', 'rays_per', 'rays_pir', 'rays_pabs', 'sum_cir', 'sum_cor',
+    'rays_ctir', 'sum_topr', 'rays_toptir', 'sum_bir', 'sum_bor', 'rays_btir',
+    'n_flight', 'n_null', 'n_scat', 'n_iface', 'n_abs', 'n_exit', 'cond_trials',
+    'cond_reject', 'cond_inaccessible', 'rng_draws', 'n_steps',
+]
+C_IDX = {name: i for i, name in enumerate(COUNTER_NAMES)}
+
+# production failure names, CLARITAS.py:424-444
+PRODUCTION_FAILURE_NAMES = {
+    0: 'OK', 1: 'PARTICLE_MICROSURFACE_NUMERICAL', 2: 'PARTICLE_MICROSURFACE_WATCHDOG',
+    3: 'PARTICLE_MACRO_GEOMETRY_FAILURE', 4: 'PARTICLE_MACRO_BOUNCE_LIMIT',
+    5: 'ACRYLIC_ANNULUS_NO_INTERSECTION', 6: 'ACRYLIC_ANNULUS_BOUNCE_LIMIT',
+    7: 'BOTTOM_DISC_GEOMETRY_FAILURE', 8: 'BOTTOM_DISC_BOUNCE_LIMIT',
+    9: 'WATER_BOUNDARY_NO_INTERSECTION', 10: 'TRACE_ITERATION_LIMIT',
+    11: 'NONFINITE_POSITION', 12: 'NONFINITE_DIRECTION', 13: 'INVALID_MEDIUM_STATE',
+    14: 'INVALID_EVENT_STATE', 15: 'OTHER_EXPLICIT_FAILURE', 16: 'SPHEROID_GEOMETRY_FAILURE',
+    17: 'HEADSPACE_BOUNDARY_NO_INTERSECTION', 18: 'SPATIAL_MAJORANT_VIOLATION',
+}
+
+D = C.c_double
+I32 = C.c_int32
+U64 = C.c_uint64
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ('n_water', D), ('n_acrylic', D), ('n_air', D), ('bottom_external_n', D),
+        ('rin', D), ('rout', D), ('zmin', D), ('zmean', D), ('zwall_top', D),
+        ('bottom_disc_thickness', D),
+        ('ring_in', D), ('throat_out', D), ('ring_out', D), ('source_r', D), ('throat_r', D),
+        ('counterbore_r', D),
+        ('vortex_delta_h', D), ('vortex_core_radius', D),
+        ('alpha1', D), ('alpha2', D), ('beam_sigma', D), ('beam_radius', D),
+        ('legacy_accept_deg', D),
+        ('wavelength', D),
+        ('rough_alpha', D),
+        ('aux_angles_deg', D * MAX_AUX),
+        ('field_zmax', D),
+        ('field_xy_half', D),
+        ('free_surface_model', I32), ('source_angular_model', I32), ('event_model', I32),
+        ('sediment_model', I32), ('max_particle_bounces', I32), ('max_cell_bounces', I32),
+        ('max_iterations', I32), ('conditional_max_trials', I32), ('n_aux', I32),
+        ('n_groups', I32), ('field_n0', I32), ('field_n1', I32), ('field_n2', I32),
+        ('field_nbins', I32),
+    ]
+
+
+class Particles(C.Structure):
+    _fields_ = [
+        ('diameter', PD), ('weight', PD), ('n_real', PD), ('k_imag', PD),
+        ('concentration', D), ('density', D), ('n_medium', D), ('wavelength', D),
+        ('hybrid_wave_scale', D), ('wave_rms_height_m', D),
+        ('n', I32), ('n_theta', I32), ('weight_mode', I32), ('event_model', I32),
+    ]
+
+
+class Sediment(C.Structure):
+    _fields_ = [
+        ('particle_density', D), ('water_density', D), ('water_viscosity', D), ('gravity', D),
+        ('eddy_diffusivity', D), ('n_quad', I32), ('n_radial', I32),
+    ]
+
+
+class MieOut(C.Structure):
+    _fields_ = [
+        ('size_parameter', PD), ('qext', PD), ('qsca', PD), ('qabs', PD), ('qback', PD),
+        ('asym_g', PD), ('albedo', PD), ('nstop', PI), ('theta', PD), ('phase_pdf', PD),
+        ('phase_cdf', PD),
+    ]
+
+
+class TablesOut(C.Structure):
+    _fields_ = [
+        ('radius', PD), ('number_density', PD), ('mu_geom_bin', PD), ('mu_sca_bin', PD),
+        ('mu_ext_bin', PD), ('trace_geom_bin', PD), ('trace_wave_bin', PD), ('geom_cdf', PD),
+        ('wave_cdf', PD), ('sed_A', PD), ('sed_B', PD), ('sed_norm', PD), ('sed_settling', PD),
+        ('sed_radial_log', PD), ('mu_scalars', D * 8), ('majorants', D * 4),
+    ]
+
+
+class Tallies(C.Structure):
+    _fields_ = [
+        ('det', U64 * N_CHANNELS * 4 * MAX_GROUPS),
+        ('aux', U64 * MAX_AUX * 2 * MAX_GROUPS),
+        ('group_histories', U64 * MAX_GROUPS),
+        ('incidence_sum', D * N_CHANNELS * 2),
+        ('status', U64 * N_STATUS),
+        ('failures', U64 * N_FAILURE),
+        ('counters', U64 * N_COUNTERS),
+        ('ic_hist', U64 * N_ICHIST),
+        ('n_groups', I32),
+        ('pad_', I32),
+    ]
+
+
+class Record(C.Structure):
+    _fields_ = [
+        ('x', C.c_float), ('y', C.c_float), ('z', C.c_float),
+        ('vx', C.c_float), ('vy', C.c_float), ('vz', C.c_float),
+        ('status', I32), ('native_id', I32), ('mirror_id', I32), ('interactions', I32),
+        ('failure', I32), ('draws', I32),
+    ]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [('config', I32), ('pad_', I32), ('n_histories', U64), ('seed', U64),
+                ('history_offset', U64)]
+
+
+class Launch(C.Structure):
+    _fields_ = [('block', I32), ('blocks_per_cu', I32), ('chunk', I32), ('pad_', I32)]
+
+
+class MlpShape(C.Structure):
+    _fields_ = [('n_members', I32), ('in_dim', I32), ('hidden', I32), ('out_dim', I32),
+                ('n_hidden_layers', I32), ('pad_', I32)]
+
+
+STRUCT_IDS = {
+    0: Config, 1: Particles, 2: Sediment, 3: MieOut, 4: TablesOut, 5: Tallies, 6: Record,
+    7: BatchItem, 8: Launch,
+}
+
+
+def check_sizes(lib) -> None:
+    """Raise if any ctypes structure disagrees with the C library's sizeof."""
+    lib.clr_sizeof.restype = C.c_size_t
+    lib.clr_sizeof.argtypes = [C.c_int]
+    for sid, cls in STRUCT_IDS.items():
+        n = int(lib.clr_sizeof(sid))
+        if n != C.sizeof(cls):
+            raise RuntimeError(f'ABI mismatch for {cls.__name__}: C {n} vs ctypes {C.sizeof(cls)}')
+
+
+def tallies_to_numpy(t: Tallies):
+    """Copy a Tallies struct into a dict of numpy arrays."""
+    import numpy as np
+
+    g = int(t.n_groups)
+    det = np.ctypeslib.as_array(t.det).reshape(MAX_GROUPS, 4, N_CHANNELS).astype(np.int64)
+    aux = np.ctypeslib.as_array(t.aux).reshape(MAX_GROUPS, 2, MAX_AUX).astype(np.int64)
+    return {
+        'n_groups': g,
+        'det': det[:g].copy(),
+        'aux': aux[:g].copy(),
+        'group_histories': np.ctypeslib.as_array(t.group_histories)[:g].astype(np.int64).copy(),
+        'incidence_sum': np.ctypeslib.as_array(t.incidence_sum).reshape(2, N_CHANNELS).copy(),
+        'status': np.ctypeslib.as_array(t.status).astype(np.int64).copy(),
+        'failures': np.ctypeslib.as_array(t.failures).astype(np.int64).copy(),
+        'counters': np.ctypeslib.as_array(t.counters).astype(np.int64).copy(),
+        'ic_hist': np.ctypeslib.as_array(t.ic_hist).astype(np.int64).copy(),
+    }
+
+
+def records_to_numpy(recs, n):
+    import numpy as np
+
+    dt = np.dtype([('x', 'f4'), ('y', 'f4'), ('z', 'f4'), ('vx', 'f4'), ('vy', 'f4'),
+                   ('vz', 'f4'), ('status', 'i4'), ('native_id', 'i4'), ('mirror_id', 'i4'),
+                   ('interactions', 'i4'), ('failure', 'i4'), ('draws', 'i4')])
+    buf = (C.c_char * (C.sizeof(Record) * n)).from_address(C.addressof(recs))
+    return np.frombuffer(bytes(buf), dtype=dt, count=n)

@@ -1,0 +1,792 @@
+// api.cpp -- C ABI of libclaritas_hip.so (declared in include/claritas.h).
+//
+// Host orchestration only: argument checks, device buffers, work-queue
+// construction and launches of the HIP kernels in mie.hip, precompute.hip and
+// transport.hip.  Physics runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/claritas.h"
+#include "mie.h"
+#include "mlp.h"
+#include "precompute.h"
+#include "transport.h"
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr int kMaxBins = 4096;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device tables of one prepared configuration.
+struct Slot {
+  bool ready = false;
+  clr_config cfg{};
+  int n = 0, nt = 0, nrad = 0;
+  DevBuf in_d, in_w, in_nr, in_k;                  // inputs (double)
+  DevBuf q;                                        // [7][n] Mie efficiencies
+  DevBuf nstop;                                    // [n] expansion orders
+  DevBuf theta, theta_f, pdf, cdf, cdf_f;          // phase tables
+  DevBuf radius, nd, mug, mus, mue, muw, muev, trace_g, trace_w, gcdf, wcdf;
+  DevBuf mu_scalars, trace_sums, sedA, sedB, sednorm, sedws, radlog, majorants;
+  DevBuf radius_f, n_f, k_f, albedo_f, gcdf_f, wcdf_f;
+  DevBuf field;
+  size_t field_len = 0;
+  DevParams params{};
+  double mu_scalars_h[8]{};
+  double majorants_h[4]{};
+  void release() {
+    DevBuf* all[] = {&in_d, &in_w, &in_nr, &in_k, &q, &nstop, &theta, &theta_f, &pdf, &cdf, &cdf_f, &radius, &nd,
+                     &mug, &mus, &mue, &muw, &muev, &trace_g, &trace_w, &gcdf, &wcdf, &mu_scalars,
+                     &trace_sums, &sedA, &sedB, &sednorm, &sedws, &radlog, &majorants, &radius_f, &n_f,
+                     &k_f, &albedo_f, &gcdf_f, &wcdf_f, &field};
+    for (DevBuf* b : all) b->release();
+    ready = false;
+  }
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int n_items = -1, grid = 0, block = 0, plain = -1, n_slots = 0;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    n_items = -1;
+  }
+};
+
+}  // namespace
+
+struct clr_ctx {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::vector<Slot*> slots;
+  DevBuf params, tallies, items, counter, records, scratch_u32, mie_work;
+  double last_ms = 0.0;
+  GraphCache graph;
+  MlpState* mlp = nullptr;
+};
+
+namespace {
+
+int fail(clr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+#define HIPCHK(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((ctx), CLR_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+// Expansion order the Mie kernel will need for one bin (same rule as the
+// kernel: n_mx = max(n_stop, |m x|) + 128).
+int mie_order(double d, double nr, double k, double n_med, double wl) {
+  const double x = kPi * n_med * d / wl;
+  if (!(x > 0.0)) return 2;
+  int nstop = (int)(x + 4.0 * std::cbrt(x) + 2.0);
+  if (nstop < 1) nstop = 1;
+  int nmx = (int)std::fmax((double)nstop, std::hypot(nr / n_med * x, k / n_med * x)) + 128;
+  return std::max(nmx, nstop + 1);
+}
+
+int check_particles(clr_ctx* c, const clr_particles* p) {
+  if (!p || p->n <= 0 || p->n > kMaxBins) return fail(c, CLR_E_ARG, "particles: n must be 1..4096");
+  if (!p->diameter || !p->weight || !p->n_real || !p->k_imag) return fail(c, CLR_E_ARG, "particles: null array");
+  if (p->n_theta != 0 && p->n_theta < 2) return fail(c, CLR_E_ARG, "particles: n_theta must be >= 2");
+  if (!(p->n_medium > 0.0) || !(p->wavelength > 0.0)) return fail(c, CLR_E_ARG, "particles: n_medium, wavelength");
+  for (int i = 0; i < p->n; ++i) {
+    if (!(p->diameter[i] > 0.0) || !std::isfinite(p->diameter[i])) return fail(c, CLR_E_ARG, "particles: diameter");
+    if (!(p->n_real[i] > 0.0) || !(p->k_imag[i] >= 0.0)) return fail(c, CLR_E_ARG, "particles: complex index");
+    if (!(p->weight[i] >= 0.0) || !std::isfinite(p->weight[i])) return fail(c, CLR_E_ARG, "particles: weight");
+  }
+  return 0;
+}
+
+// Run the Mie kernels for particles p into slot-owned buffers.
+int run_mie(clr_ctx* c, Slot& s, const clr_particles* p) {
+  const int n = p->n, nt = std::max(p->n_theta, 2);
+  int stride = 2;
+  for (int i = 0; i < n; ++i)
+    stride = std::max(stride, mie_order(p->diameter[i], p->n_real[i], p->k_imag[i], p->n_medium, p->wavelength) + 2);
+  if (stride > (1 << 22)) return fail(c, 1, "Mie expansion order exceeds 2^22");
+  HIPCHK(c, s.in_d.ensure(sizeof(double) * n));
+  HIPCHK(c, s.in_w.ensure(sizeof(double) * n));
+  HIPCHK(c, s.in_nr.ensure(sizeof(double) * n));
+  HIPCHK(c, s.in_k.ensure(sizeof(double) * n));
+  HIPCHK(c, hipMemcpyAsync(s.in_d.p, p->diameter, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.in_w.p, p->weight, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.in_nr.p, p->n_real, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.in_k.p, p->k_imag, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, s.q.ensure(sizeof(double) * 7 * n));
+  HIPCHK(c, s.theta.ensure(sizeof(double) * nt));
+  HIPCHK(c, s.theta_f.ensure(sizeof(float) * nt));
+  HIPCHK(c, s.pdf.ensure(sizeof(double) * (size_t)n * nt));
+  HIPCHK(c, s.cdf.ensure(sizeof(double) * (size_t)n * nt));
+  HIPCHK(c, s.cdf_f.ensure(sizeof(float) * (size_t)n * nt));
+  // workspace: D, a, b (double2) + nstop + error flag
+  const size_t coef = sizeof(double2) * (size_t)n * stride;
+  HIPCHK(c, c->mie_work.ensure(3 * coef + sizeof(int)));
+  HIPCHK(c, s.nstop.ensure(sizeof(int) * n));
+  char* w = c->mie_work.as<char>();
+  int* nstop = s.nstop.as<int>();
+  int* eflag = reinterpret_cast<int*>(w + 3 * coef);
+  HIPCHK(c, hipMemsetAsync(eflag, 0, sizeof(int), c->stream));
+  MieLaunch L{};
+  L.diameter = s.in_d.as<double>();
+  L.n_real = s.in_nr.as<double>();
+  L.k_imag = s.in_k.as<double>();
+  L.n_medium = p->n_medium;
+  L.wavelength = p->wavelength;
+  L.n_bins = n;
+  L.n_theta = nt;
+  L.stride = stride;
+  L.D = reinterpret_cast<double2*>(w);
+  L.a = reinterpret_cast<double2*>(w + coef);
+  L.b = reinterpret_cast<double2*>(w + 2 * coef);
+  L.nstop = nstop;
+  L.q = s.q.as<double>();
+  L.theta = s.theta.as<double>();
+  L.theta_f = s.theta_f.as<float>();
+  L.pdf = s.pdf.as<double>();
+  L.cdf = s.cdf.as<double>();
+  L.cdf_f = s.cdf_f.as<float>();
+  L.error_flag = eflag;
+  HIPCHK(c, clr_launch_mie(L, c->stream));
+  int ecode = 0;
+  HIPCHK(c, hipMemcpyAsync(&ecode, eflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ecode) return fail(c, ecode, "Mie precompute failed with error code " + std::to_string(ecode));
+  s.n = n;
+  s.nt = nt;
+  return 0;
+}
+
+float det_u(double deg, bool cosine) {
+  const float th = (float)deg * (3.14159265358979323846f / 180.0f);
+  return cosine ? (float)std::cos((double)th) : (float)std::sin((double)th);
+}
+
+int validate_config(clr_ctx* c, const clr_config* g, const clr_particles* p) {
+  if (!g) return fail(c, CLR_E_ARG, "config is NULL");
+  if (g->rough_alpha > 1e-7)
+    return fail(c, CLR_E_UNSUPPORTED, "rough particle surfaces (Smith microsurface walk) are not supported; set particle_surface_rms_slope_deg=0");
+  const int em = g->event_model;
+  if (!(em == 0 || em == 1 || em == 2 || em == 3 || em == 4 || em == 6))
+    return fail(c, CLR_E_UNSUPPORTED, "unsupported particle event model");
+  if (p && p->event_model != em) return fail(c, CLR_E_ARG, "config and particles disagree on event model");
+  if (g->free_surface_model < 0 || g->free_surface_model > 3) return fail(c, CLR_E_ARG, "free surface model");
+  if (g->source_angular_model < 0 || g->source_angular_model > 3) return fail(c, CLR_E_ARG, "source model");
+  if (g->sediment_model < 0 || g->sediment_model > 5) return fail(c, CLR_E_ARG, "sediment model");
+  if (g->n_aux < 0 || g->n_aux > CLR_MAX_AUX) return fail(c, CLR_E_ARG, "n_aux");
+  if (g->n_groups < 1 || g->n_groups > CLR_MAX_GROUPS) return fail(c, CLR_E_ARG, "n_groups must be 1..64");
+  if (!(g->rin > 0.0 && g->rin < g->rout && g->rout < g->ring_in && g->ring_in < g->throat_out &&
+        g->throat_out <= g->source_r))
+    return fail(c, CLR_E_ARG, "geometry must satisfy 0 < rin < rout < ring_in < throat_out <= source_r");
+  if (!(g->zwall_top > g->zmean && g->zmean > g->zmin)) return fail(c, CLR_E_ARG, "heights: zwall_top > zmean > zmin");
+  if (!(g->bottom_disc_thickness > 0.0)) return fail(c, CLR_E_ARG, "bottom disc thickness");
+  if (!(0.0 < g->throat_r && g->throat_r < g->counterbore_r)) return fail(c, CLR_E_ARG, "bore radii");
+  if (g->free_surface_model == CLR_SURFACE_RANKINE && !(g->vortex_core_radius > 0.0 && g->vortex_core_radius < g->rin))
+    return fail(c, CLR_E_ARG, "Rankine vortex needs 0 < core radius < inner radius");
+  if (g->free_surface_model == CLR_SURFACE_SCULLY && g->vortex_delta_h > 0.0 && !(g->vortex_core_radius > 0.0))
+    return fail(c, CLR_E_ARG, "Scully vortex needs a positive core radius");
+  if (g->max_iterations < 1 || g->max_particle_bounces < 0) return fail(c, CLR_E_ARG, "iteration caps");
+  return 0;
+}
+
+int validate_sediment(clr_ctx* c, const clr_config* g, const clr_sediment* sed, size_t field_len, int n) {
+  const int tm = g->sediment_model, fsm = g->free_surface_model;
+  if (tm == CLR_SED_STOKES || tm == CLR_SED_SCHILLER || tm == CLR_SED_GRID_PER_BIN) {
+    if (!sed) return fail(c, CLR_E_ARG, "sediment parameters required");
+    if (!(sed->particle_density > sed->water_density) || !(sed->water_density > 0.0) || !(sed->water_viscosity > 0.0) ||
+        !(sed->gravity > 0.0) || !(sed->eddy_diffusivity > 0.0) || sed->n_quad < 64 || sed->n_radial < 65)
+      return fail(c, 31, "sediment precompute failed with error code 31");
+    if (fsm < 0 || fsm > 2 || (g->vortex_delta_h > 0.0 && fsm == 0) || (fsm == 2 && !(g->vortex_core_radius > 0.0)))
+      return fail(c, 32, "sediment precompute failed with error code 32");
+    if (!(g->zmean - g->zmin > 0.0)) return fail(c, 33, "sediment precompute failed with error code 33");
+  }
+  const size_t need_rz = (size_t)std::max(g->field_n0, 0) * (size_t)std::max(g->field_n1, 0);
+  if (tm == CLR_SED_GRID_PER_BIN && (g->field_n0 < 2 || g->field_n1 < 2 || field_len < need_rz * (size_t)n))
+    return fail(c, CLR_E_ARG, "per-bin grid needs n_bins*n0*n1 values");
+  if (tm == CLR_SED_RZ_FIELD && (g->field_n0 < 2 || g->field_n1 < 2 || field_len < need_rz))
+    return fail(c, CLR_E_ARG, "r-z field needs n0*n1 values");
+  if (tm == CLR_SED_XYZ_FIELD &&
+      (g->field_n0 < 2 || g->field_n1 < 2 || g->field_n2 < 2 || field_len < need_rz * (size_t)g->field_n2 ||
+       !(g->field_xy_half > 0.0)))
+    return fail(c, CLR_E_ARG, "x-y-z field needs n0*n1*n2 values and a half-width");
+  if ((tm == CLR_SED_GRID_PER_BIN || tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD) && !(g->field_zmax > g->zmin))
+    return fail(c, CLR_E_ARG, "field_zmax must exceed zmin");
+  return 0;
+}
+
+void fill_params(Slot& s, const clr_config& g) {
+  DevParams& P = s.params;
+  P.n_water = (float)g.n_water; P.n_acrylic = (float)g.n_acrylic; P.n_air = (float)g.n_air;
+  P.n_bottom_ext = (float)g.bottom_external_n;
+  P.rin = (float)g.rin; P.rout = (float)g.rout; P.zmin = (float)g.zmin; P.zmean = (float)g.zmean;
+  P.zwall_top = (float)g.zwall_top; P.zlow = P.zmin - (float)g.bottom_disc_thickness;
+  P.ring_in = (float)g.ring_in; P.throat_out = (float)g.throat_out; P.ring_out = (float)g.ring_out;
+  P.source_r = (float)g.source_r; P.throat_r = (float)g.throat_r; P.cb_r = (float)g.counterbore_r;
+  P.dh = (float)g.vortex_delta_h; P.core_r = (float)g.vortex_core_radius;
+  P.alpha1 = (float)g.alpha1; P.alpha2 = (float)g.alpha2; P.sigma = (float)g.beam_sigma;
+  P.beam_r = (float)g.beam_radius; P.legacy_acc = (float)g.legacy_accept_deg; P.wavelength = (float)g.wavelength;
+  P.mu_geom = (float)s.mu_scalars_h[0];
+  P.mu_wave = (float)s.mu_scalars_h[1];
+  P.coherent = std::fmin(std::fmax((float)s.mu_scalars_h[7], 0.0f), 1.0f);
+  P.mu_track = (g.sediment_model == CLR_SED_UNIFORM) ? (P.mu_geom + P.mu_wave)
+                                                     : ((float)s.majorants_h[0] + (float)s.majorants_h[1]);
+  P.field_zmax = (float)g.field_zmax;
+  P.field_half = (float)g.field_xy_half;
+  for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+    P.det_ux[j] = det_u(10.0 * j, false);
+    P.det_uy[j] = det_u(10.0 * j, true);
+  }
+  for (int a = 0; a < CLR_MAX_AUX; ++a) {
+    P.aux_ux[a] = det_u(g.aux_angles_deg[a], false);
+    P.aux_uy[a] = det_u(g.aux_angles_deg[a], true);
+  }
+  if (g.free_surface_model == CLR_SURFACE_RANKINE) {
+    const double R = g.rin, a = g.vortex_core_radius, R2 = R * R, a2 = a * a;
+    const double K = g.vortex_delta_h / (2.0 - a2 / R2);
+    P.rk_K = K;
+    P.rk_C0 = g.zmean - (2.0 * K / R2) * (R2 - 0.75 * a2 - a2 * std::log(R / a));
+  } else {
+    P.rk_K = 0.0;
+    P.rk_C0 = g.zmean;
+  }
+  P.fs_model = g.free_surface_model; P.src_model = g.source_angular_model;
+  P.event_model = g.event_model; P.sed_model = g.sediment_model;
+  P.max_pb = g.max_particle_bounces; P.max_iter = g.max_iterations; P.cond_trials = g.conditional_max_trials;
+  P.n_aux = g.n_aux; P.n_groups = g.n_groups; P.n_bins = s.n; P.n_theta = s.nt; P.n_radial = s.nrad;
+  P.fn0 = g.field_n0; P.fn1 = g.field_n1; P.fn2 = g.field_n2;
+  // lowest free-surface height (host float arithmetic of the device formula)
+  float zs = P.zmean;
+  if (P.fs_model == CLR_SURFACE_PARABOLIC && P.dh > 0.0f) zs = P.zmean - 0.5f * P.dh;
+  else if (P.fs_model == CLR_SURFACE_SCULLY && P.dh > 0.0f) {
+    const float a2 = P.core_r * P.core_r, R2 = P.rin * P.rin, H = P.dh * (a2 + R2) / R2;
+    zs = P.zmean - H * (1.0f - (a2 / R2) * std::log1p(R2 / a2));
+  } else if (P.fs_model == CLR_SURFACE_RANKINE && P.dh > 0.0f) zs = (float)P.rk_C0;
+  P.zs_min = zs - 1e-6f * std::fmax(std::fabs(zs), 1e-3f);
+  P.radius = s.radius_f.as<float>();
+  P.n_real = s.n_f.as<float>();
+  P.k_imag = s.k_f.as<float>();
+  P.albedo = s.albedo_f.as<float>();
+  P.gcdf = s.gcdf_f.as<float>();
+  P.wcdf = s.wcdf_f.as<float>();
+  P.qsca = s.q.as<double>() + s.n;
+  P.qext = s.q.as<double>();
+  P.trace_g = s.trace_g.as<double>();
+  P.trace_w = s.trace_w.as<double>();
+  P.sedA = s.sedA.as<double>();
+  P.sedB = s.sedB.as<double>();
+  P.sednorm = s.sednorm.as<double>();
+  P.radlog = s.radlog.as<double>();
+  P.cdf = s.cdf_f.as<float>();
+  P.theta = s.theta_f.as<float>();
+  P.field = s.field.as<float>();
+  // the reference traces the Rouse/xyz scale with the bin-sum rates
+  double sums[2] = {0.0, 0.0};
+  (void)hipMemcpy(sums, s.trace_sums.p, sizeof(sums), hipMemcpyDeviceToHost);
+  P.sum_trace_g = sums[0];
+  P.sum_trace_w = sums[1];
+}
+
+int ensure_slots(clr_ctx* c, int need) {
+  while ((int)c->slots.size() < need) c->slots.push_back(new Slot());
+  HIPCHK(c, c->params.ensure(sizeof(DevParams) * c->slots.size()));
+  HIPCHK(c, c->tallies.ensure(sizeof(clr_tallies) * c->slots.size()));
+  return 0;
+}
+
+void group_sizes(uint64_t n, int G, uint64_t* out) {
+  for (int g = 0; g < CLR_MAX_GROUPS; ++g) out[g] = 0;
+  auto lo = [&](int g) -> uint64_t {
+    // ceil(g*n/G) without overflow for n < 2^57
+    const unsigned __int128 num = (unsigned __int128)g * n;
+    return (uint64_t)((num + (unsigned)G - 1) / (unsigned)G);
+  };
+  for (int g = 0; g < G; ++g) out[g] = lo(g + 1) - lo(g);
+}
+
+__global__ void clear_kernel(unsigned long long* __restrict__ p, size_t n_words, unsigned int* __restrict__ counter) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < n_words; k += stride) p[k] = 0ull;
+  if (i == 0) *counter = 0u;
+}
+
+struct RunShape {
+  int block, grid, chunk, plain;
+};
+
+RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int plain) {
+  RunShape r{};
+  r.plain = plain;
+  r.block = (launch && launch->block > 0) ? launch->block : 256;
+  r.block = std::min(256, std::max(64, (r.block / 64) * 64));
+  const int occ = clr_transport_occupancy(r.block, plain);
+  int per_cu = (launch && launch->blocks_per_cu > 0) ? std::min(launch->blocks_per_cu, occ) : occ;
+  r.grid = std::max(1, c->n_cu * per_cu);
+  if (launch && launch->chunk > 0) r.chunk = launch->chunk;
+  else {
+    const uint64_t want = total / ((uint64_t)r.grid * 8ull);
+    r.chunk = (int)std::min<uint64_t>(65536, std::max<uint64_t>(2048, want));
+  }
+  r.chunk = std::max(r.block, (r.chunk / r.block) * r.block);
+  return r;
+}
+
+void append_items(std::vector<WorkItem>& items, int cfg, uint64_t n, uint64_t seed, uint64_t offset, int chunk) {
+  for (uint64_t s = 0; s < n; s += (uint64_t)chunk) {
+    WorkItem w{};
+    w.config = cfg;
+    w.start = s;
+    w.count = std::min<uint64_t>((uint64_t)chunk, n - s);
+    w.n_total = n;
+    w.seed = seed;
+    w.offset = offset;
+    items.push_back(w);
+  }
+}
+
+bool slots_plain(clr_ctx* c, const std::vector<int>& used) {
+  for (int k : used) {
+    const clr_config& g = c->slots[k]->cfg;
+    if (g.event_model != CLR_EVENT_PLAIN_MIE) return false;
+    if (!(g.sediment_model == CLR_SED_UNIFORM || g.sediment_model == CLR_SED_RZ_FIELD ||
+          g.sediment_model == CLR_SED_XYZ_FIELD))
+      return false;
+  }
+  return true;
+}
+
+// Upload items, clear tallies, launch the persistent kernel (optionally
+// through a cached hipGraph), time it with events on the library stream.
+int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape& sh, bool use_graph,
+                 clr_record* d_records, uint64_t n_records) {
+  const int n_slots = (int)c->slots.size();
+  HIPCHK(c, c->items.ensure(sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
+  HIPCHK(c, c->counter.ensure(sizeof(unsigned int)));
+  HIPCHK(c, hipMemcpyAsync(c->items.p, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, c->stream));
+  TransportLaunch L{};
+  L.params = c->params.as<DevParams>();
+  L.items = c->items.as<WorkItem>();
+  L.n_items = (int)items.size();
+  L.item_counter = c->counter.as<unsigned int>();
+  L.tallies = c->tallies.as<clr_tallies>();
+  L.records = d_records;
+  L.n_records = n_records;
+  L.grid = sh.grid;
+  L.block = sh.block;
+  L.plain = sh.plain;
+  const size_t words = sizeof(clr_tallies) * n_slots / 8;
+  if (use_graph) {
+    GraphCache& G = c->graph;
+    if (!(G.exec && G.n_items == L.n_items && G.grid == L.grid && G.block == L.block && G.plain == L.plain &&
+          G.n_slots == n_slots)) {
+      G.reset();
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(clear_kernel, dim3(256), dim3(256), 0, c->stream, c->tallies.as<unsigned long long>(), words,
+                         L.item_counter);
+      hipError_t le = clr_launch_transport(L, c->stream);
+      hipError_t ce = hipStreamEndCapture(c->stream, &G.graph);
+      if (le != hipSuccess) return fail(c, CLR_E_HIP, std::string("transport launch: ") + hipGetErrorString(le));
+      if (ce != hipSuccess) return fail(c, CLR_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+      G.n_items = L.n_items; G.grid = L.grid; G.block = L.block; G.plain = L.plain; G.n_slots = n_slots;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  } else {
+    hipLaunchKernelGGL(clear_kernel, dim3(256), dim3(256), 0, c->stream, c->tallies.as<unsigned long long>(), words,
+                       L.item_counter);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, clr_launch_transport(L, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  }
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.0f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clr_abi_version(void) { return CLR_ABI_VERSION; }
+
+size_t clr_sizeof(int id) {
+  switch (id) {
+    case 0: return sizeof(clr_config);
+    case 1: return sizeof(clr_particles);
+    case 2: return sizeof(clr_sediment);
+    case 3: return sizeof(clr_mie_out);
+    case 4: return sizeof(clr_tables_out);
+    case 5: return sizeof(clr_tallies);
+    case 6: return sizeof(clr_record);
+    case 7: return sizeof(clr_batch_item);
+    case 8: return sizeof(clr_launch);
+    default: return 0;
+  }
+}
+
+int clr_create(clr_ctx** out, int device) {
+  if (!out) return CLR_E_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return CLR_E_HIP;
+  if (device < 0 || device >= count) return CLR_E_ARG;
+  clr_ctx* c = new clr_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return CLR_E_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  *out = c;
+  return 0;
+}
+
+void clr_destroy(clr_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->graph.reset();
+  for (Slot* s : c->slots) { s->release(); delete s; }
+  DevBuf* bufs[] = {&c->params, &c->tallies, &c->items, &c->counter, &c->records, &c->scratch_u32, &c->mie_work};
+  for (DevBuf* b : bufs) b->release();
+  clr_mlp_free(c->mlp);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* clr_last_error(const clr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int clr_device_info(clr_ctx* c, int64_t info[8]) {
+  if (!c || !info) return CLR_E_ARG;
+  hipDeviceProp_t p;
+  HIPCHK(c, hipGetDeviceProperties(&p, c->device));
+  info[0] = p.multiProcessorCount;
+  info[1] = p.warpSize;
+  info[2] = (int64_t)p.sharedMemPerBlock;
+  info[3] = p.clockRate;
+  info[4] = (int64_t)p.major * 100 + p.minor;
+  info[5] = (int64_t)(p.totalGlobalMem >> 20);
+  info[6] = clr_transport_occupancy(256, 1);
+  info[7] = clr_transport_occupancy(256, 0);
+  return 0;
+}
+
+int clr_mie_tables(clr_ctx* c, const clr_particles* p, clr_mie_out* o) {
+  if (!c || !o) return fail(c, CLR_E_ARG, "null argument");
+  if (int e = check_particles(c, p)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
+  Slot s;
+  int e = run_mie(c, s, p);
+  if (!e) {
+    const int n = p->n, nt = s.nt;
+    std::vector<double> q((size_t)7 * n);
+    std::vector<int> nstop(n);
+    hipError_t he = hipMemcpy(q.data(), s.q.p, sizeof(double) * 7 * n, hipMemcpyDeviceToHost);
+    if (he == hipSuccess) he = hipMemcpy(nstop.data(), s.nstop.p, sizeof(int) * n, hipMemcpyDeviceToHost);
+    double* dst[7] = {o->qext, o->qsca, o->qabs, o->qback, o->asym_g, o->albedo, o->size_parameter};
+    for (int k = 0; k < 7 && he == hipSuccess; ++k)
+      if (dst[k]) memcpy(dst[k], q.data() + (size_t)k * n, sizeof(double) * n);
+    if (he == hipSuccess && o->nstop) memcpy(o->nstop, nstop.data(), sizeof(int) * n);
+    if (he == hipSuccess && o->theta && p->n_theta >= 2) he = hipMemcpy(o->theta, s.theta.p, sizeof(double) * nt, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && o->phase_pdf && p->n_theta >= 2)
+      he = hipMemcpy(o->phase_pdf, s.pdf.p, sizeof(double) * (size_t)n * nt, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && o->phase_cdf && p->n_theta >= 2)
+      he = hipMemcpy(o->phase_cdf, s.cdf.p, sizeof(double) * (size_t)n * nt, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) e = fail(c, CLR_E_HIP, std::string("copy Mie tables: ") + hipGetErrorString(he));
+  }
+  s.release();
+  return e;
+}
+
+int clr_prepare(clr_ctx* c, int slot, const clr_config* g, const clr_particles* p, const clr_sediment* sed,
+                const float* field, size_t field_len) {
+  if (!c) return CLR_E_ARG;
+  if (slot < 0 || slot > 4095) return fail(c, CLR_E_ARG, "slot out of range");
+  if (int e = check_particles(c, p)) return e;
+  if (p->n_theta < 2) return fail(c, CLR_E_ARG, "particles: n_theta must be >= 2 for transport");
+  if (int e = validate_config(c, g, p)) return e;
+  if (int e = validate_sediment(c, g, sed, field_len, p->n)) return e;
+  if ((g->sediment_model >= CLR_SED_GRID_PER_BIN) && !field) return fail(c, CLR_E_ARG, "field is NULL");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->graph.reset();
+  if (int e = ensure_slots(c, slot + 1)) return e;
+  Slot& s = *c->slots[slot];
+  s.ready = false;
+  if (int e = run_mie(c, s, p)) return e;
+  const int n = p->n;
+  s.nrad = (sed && (g->sediment_model == CLR_SED_STOKES || g->sediment_model == CLR_SED_SCHILLER ||
+                    g->sediment_model == CLR_SED_GRID_PER_BIN))
+               ? sed->n_radial
+               : 2;
+  DevBuf* dn[] = {&s.radius, &s.nd, &s.mug, &s.mus, &s.mue, &s.muw, &s.muev, &s.trace_g, &s.trace_w, &s.gcdf,
+                  &s.wcdf, &s.sedA, &s.sedB, &s.sednorm, &s.sedws};
+  for (DevBuf* b : dn) HIPCHK(c, b->ensure(sizeof(double) * n));
+  DevBuf* fn[] = {&s.radius_f, &s.n_f, &s.k_f, &s.albedo_f, &s.gcdf_f, &s.wcdf_f};
+  for (DevBuf* b : fn) HIPCHK(c, b->ensure(sizeof(float) * n));
+  HIPCHK(c, s.mu_scalars.ensure(sizeof(double) * 8));
+  HIPCHK(c, s.trace_sums.ensure(sizeof(double) * 2));
+  HIPCHK(c, s.majorants.ensure(sizeof(double) * 4));
+  HIPCHK(c, s.radlog.ensure(sizeof(double) * (size_t)n * s.nrad));
+  s.field_len = 0;
+  if (field && field_len) {
+    HIPCHK(c, s.field.ensure(sizeof(float) * field_len));
+    HIPCHK(c, hipMemcpyAsync(s.field.p, field, sizeof(float) * field_len, hipMemcpyHostToDevice, c->stream));
+    s.field_len = field_len;
+  }
+  HIPCHK(c, c->scratch_u32.ensure(sizeof(unsigned int) * (size_t)(n + 2)));
+  HIPCHK(c, hipMemsetAsync(c->scratch_u32.p, 0, sizeof(unsigned int) * (size_t)(n + 2), c->stream));
+  int* eflag = c->scratch_u32.as<int>() + n + 1;
+  PrepLaunch L{};
+  L.n = n; L.weight_mode = p->weight_mode; L.event_model = p->event_model; L.sed_model = g->sediment_model;
+  L.fs_model = g->free_surface_model; L.n_quad = sed ? sed->n_quad : 64; L.n_radial = s.nrad;
+  L.concentration = p->concentration; L.density = p->density; L.n_medium = p->n_medium; L.wavelength = p->wavelength;
+  L.hybrid_wave_scale = p->hybrid_wave_scale; L.wave_rms_height = p->wave_rms_height_m;
+  if (sed) {
+    L.particle_density = sed->particle_density; L.water_density = sed->water_density;
+    L.viscosity = sed->water_viscosity; L.gravity = sed->gravity; L.eddy_diffusivity = sed->eddy_diffusivity;
+  }
+  L.R = g->rin; L.zmin = g->zmin; L.zmean = g->zmean; L.dh = g->vortex_delta_h; L.core = g->vortex_core_radius;
+  L.diameter = s.in_d.as<double>(); L.weight = s.in_w.as<double>(); L.n_real = s.in_nr.as<double>();
+  L.k_imag = s.in_k.as<double>(); L.qext = s.q.as<double>(); L.qsca = s.q.as<double>() + n;
+  L.field = s.field_len ? s.field.as<float>() : nullptr;
+  L.field_len = s.field_len;
+  L.radius = s.radius.as<double>(); L.number_density = s.nd.as<double>(); L.mu_geom = s.mug.as<double>();
+  L.mu_sca = s.mus.as<double>(); L.mu_ext = s.mue.as<double>(); L.mu_wave = s.muw.as<double>();
+  L.mu_event = s.muev.as<double>(); L.trace_g = s.trace_g.as<double>(); L.trace_w = s.trace_w.as<double>();
+  L.geom_cdf = s.gcdf.as<double>(); L.wave_cdf = s.wcdf.as<double>(); L.mu_scalars = s.mu_scalars.as<double>();
+  L.trace_sums = s.trace_sums.as<double>(); L.sedA = s.sedA.as<double>(); L.sedB = s.sedB.as<double>();
+  L.sednorm = s.sednorm.as<double>(); L.sedws = s.sedws.as<double>(); L.radlog = s.radlog.as<double>();
+  L.majorants = s.majorants.as<double>(); L.radius_f = s.radius_f.as<float>(); L.n_real_f = s.n_f.as<float>();
+  L.k_imag_f = s.k_f.as<float>(); L.albedo_f = s.albedo_f.as<float>(); L.geom_cdf_f = s.gcdf_f.as<float>();
+  L.wave_cdf_f = s.wcdf_f.as<float>(); L.error_flag = eflag;
+  HIPCHK(c, clr_launch_precompute(L, c->scratch_u32.as<unsigned int>(), c->stream));
+  int ecode = 0;
+  HIPCHK(c, hipMemcpyAsync(&ecode, eflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.mu_scalars_h, s.mu_scalars.p, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.majorants_h, s.majorants.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ecode) return fail(c, ecode, "transport/sediment precompute failed with error code " + std::to_string(ecode));
+  if (s.mu_scalars_h[7] < 1.0 && (p->event_model == 3 || p->event_model == 4))
+    return fail(c, CLR_E_UNSUPPORTED,
+                "wave coherent fraction < 1 needs the spheroid morphology channel; set wave_surface_rms_height_m=0");
+  s.cfg = *g;
+  fill_params(s, *g);
+  HIPCHK(c, hipMemcpy(c->params.as<DevParams>() + slot, &s.params, sizeof(DevParams), hipMemcpyHostToDevice));
+  s.ready = true;
+  return 0;
+}
+
+int clr_n_slots(clr_ctx* c) { return c ? (int)c->slots.size() : 0; }
+
+int clr_get_tables(clr_ctx* c, int slot, clr_tables_out* o) {
+  if (!c || !o || slot < 0 || slot >= (int)c->slots.size() || !c->slots[slot]->ready)
+    return fail(c, CLR_E_STATE, "slot not prepared");
+  HIPCHK(c, hipSetDevice(c->device));
+  Slot& s = *c->slots[slot];
+  const size_t nb = sizeof(double) * s.n;
+  struct { double* dst; DevBuf* src; } m[] = {
+      {o->radius, &s.radius}, {o->number_density, &s.nd}, {o->mu_geom_bin, &s.mug}, {o->mu_sca_bin, &s.mus},
+      {o->mu_ext_bin, &s.mue}, {o->trace_geom_bin, &s.trace_g}, {o->trace_wave_bin, &s.trace_w},
+      {o->geom_cdf, &s.gcdf}, {o->wave_cdf, &s.wcdf}, {o->sed_A, &s.sedA}, {o->sed_B, &s.sedB},
+      {o->sed_norm, &s.sednorm}, {o->sed_settling, &s.sedws}};
+  for (auto& k : m)
+    if (k.dst) HIPCHK(c, hipMemcpy(k.dst, k.src->p, nb, hipMemcpyDeviceToHost));
+  if (o->sed_radial_log) HIPCHK(c, hipMemcpy(o->sed_radial_log, s.radlog.p, nb * s.nrad, hipMemcpyDeviceToHost));
+  memcpy(o->mu_scalars, s.mu_scalars_h, sizeof(s.mu_scalars_h));
+  memcpy(o->majorants, s.majorants_h, sizeof(s.majorants_h));
+  return 0;
+}
+
+int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_launch* launch, clr_tallies* out,
+            clr_record* records, uint64_t n_records) {
+  if (!c || !out) return fail(c, CLR_E_ARG, "null argument");
+  if (c->slots.empty() || !c->slots[0]->ready) return fail(c, CLR_E_STATE, "clr_prepare(slot 0) has not run");
+  if (n == 0) return fail(c, CLR_E_ARG, "n must be positive");
+  if (n > (1ull << 56) || offset > (1ull << 62)) return fail(c, CLR_E_RANGE, "history range too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int plain = slots_plain(c, {0}) ? 1 : 0;
+  const RunShape sh = pick_shape(c, launch, n, plain);
+  // one launch covers at most 2^31 histories so LDS u32 tallies cannot wrap
+  const uint64_t per_launch = 1ull << 31;
+  const int n_slots = (int)c->slots.size();
+  std::vector<clr_tallies> acc(1);
+  memset(&acc[0], 0, sizeof(clr_tallies));
+  std::vector<clr_tallies> host(n_slots);
+  clr_record* d_rec = nullptr;
+  const uint64_t nrec = std::min<uint64_t>(n_records, n);
+  if (records && nrec) {
+    HIPCHK(c, c->records.ensure(sizeof(clr_record) * nrec));
+    d_rec = c->records.as<clr_record>();
+  }
+  double total_ms = 0.0;
+  const int G = c->slots[0]->cfg.n_groups;
+  for (uint64_t done = 0; done < n; done += per_launch) {
+    const uint64_t m = std::min(per_launch, n - done);
+    // histories of this launch keep their global group by numbering them
+    // within the whole run: items carry start offsets into [0, n)
+    std::vector<WorkItem> items;
+    for (uint64_t s = done; s < done + m; s += (uint64_t)sh.chunk) {
+      WorkItem w{};
+      w.config = 0;
+      w.start = s;
+      w.count = std::min<uint64_t>((uint64_t)sh.chunk, done + m - s);
+      w.n_total = n;
+      w.seed = seed;
+      w.offset = offset;
+      items.push_back(w);
+    }
+    if (int e = launch_items(c, items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0)) return e;
+    total_ms += c->last_ms;
+    HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&host[0]);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&acc[0]);
+    // every field before incidence_sum and after it is u64; add field-wise
+    for (size_t k = 0; k < offsetof(clr_tallies, incidence_sum) / 8; ++k) dst[k] += src[k];
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < CLR_N_CHANNELS; ++j) acc[0].incidence_sum[k][j] += host[0].incidence_sum[k][j];
+    const size_t a0 = offsetof(clr_tallies, status) / 8, a1 = offsetof(clr_tallies, n_groups) / 8;
+    for (size_t k = a0; k < a1; ++k) dst[k] += src[k];
+  }
+  c->last_ms = total_ms;
+  *out = acc[0];
+  out->n_groups = G;
+  group_sizes(n, G, out->group_histories);
+  if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_launch* launch, clr_tallies* out) {
+  if (!c || !bi || !out || n_items <= 0) return fail(c, CLR_E_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<int> used;
+  uint64_t total = 0;
+  for (int i = 0; i < n_items; ++i) {
+    const int k = bi[i].config;
+    if (k < 0 || k >= (int)c->slots.size() || !c->slots[k]->ready) return fail(c, CLR_E_STATE, "batch item refers to an unprepared slot");
+    if (bi[i].n_histories == 0 || bi[i].n_histories > (1ull << 31)) return fail(c, CLR_E_RANGE, "batch item size must be 1..2^31");
+    if (std::find(used.begin(), used.end(), k) != used.end()) return fail(c, CLR_E_ARG, "each slot may appear once per batch");
+    used.push_back(k);
+    total += bi[i].n_histories;
+  }
+  const int plain = slots_plain(c, used) ? 1 : 0;
+  const RunShape sh = pick_shape(c, launch, total, plain);
+  std::vector<WorkItem> items;
+  for (int i = 0; i < n_items; ++i)
+    append_items(items, bi[i].config, bi[i].n_histories, bi[i].seed, bi[i].history_offset, sh.chunk);
+  if (int e = launch_items(c, items, sh, true, nullptr, 0)) return e;
+  const int n_slots = (int)c->slots.size();
+  std::vector<clr_tallies> host(n_slots);
+  HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies) * n_slots, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_items; ++i) {
+    const int k = bi[i].config;
+    out[i] = host[k];
+    out[i].n_groups = c->slots[k]->cfg.n_groups;
+    group_sizes(bi[i].n_histories, out[i].n_groups, out[i].group_histories);
+  }
+  return 0;
+}
+
+int clr_time_run(clr_ctx* c, uint64_t n, uint64_t seed, const clr_launch* launch, int repeats, double* ms) {
+  if (!c || !ms || repeats < 1) return fail(c, CLR_E_ARG, "null argument");
+  clr_tallies t;
+  double sum = 0.0;
+  for (int r = 0; r < repeats; ++r) {
+    if (int e = clr_run(c, n, seed + (uint64_t)r, 0, launch, &t, nullptr, 0)) return e;
+    sum += c->last_ms;
+  }
+  *ms = sum / repeats;
+  return 0;
+}
+
+double clr_last_kernel_ms(clr_ctx* c) { return c ? c->last_ms : 0.0; }
+
+}  // extern "C"
+
+// ---- surrogate entry points (ensemble kernels in mlp.hip) ----
+extern "C" {
+int clr_mlp_init(clr_ctx* c, const clr_mlp_shape* shape, uint64_t seed) {
+  if (!c || !shape) return fail(c, CLR_E_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  clr_mlp_free(c->mlp);
+  c->mlp = nullptr;
+  return mlp_init(&c->mlp, shape, seed, c->stream, c->err);
+}
+size_t clr_mlp_param_count(clr_ctx* c) { return (c && c->mlp) ? mlp_param_count(c->mlp) : 0; }
+int clr_mlp_get_params(clr_ctx* c, float* out, size_t n) {
+  if (!c || !c->mlp || !out) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_get_params(c->mlp, out, n, c->stream, c->err);
+}
+int clr_mlp_set_params(clr_ctx* c, const float* in, size_t n) {
+  if (!c || !c->mlp || !in) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_set_params(c->mlp, in, n, c->stream, c->err);
+}
+int clr_mlp_train(clr_ctx* c, const float* x, const float* y, int64_t n, int32_t batch, int32_t epochs, float lr,
+                  uint64_t seed, float* loss) {
+  if (!c || !c->mlp || !x || !y) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_train(c->mlp, x, y, n, batch, epochs, lr, seed, loss, c->stream, c->err);
+}
+int clr_mlp_grad(clr_ctx* c, const float* x, const float* y, int32_t n, float* loss, float* grads) {
+  if (!c || !c->mlp || !x || !y) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_grad(c->mlp, x, y, n, loss, grads, c->stream, c->err);
+}
+int clr_mlp_infer(clr_ctx* c, const float* x, int64_t n, float* mean, float* stdev) {
+  if (!c || !c->mlp || !x || !mean) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_infer(c->mlp, x, n, mean, stdev, c->stream, c->err);
+}
+int clr_mlp_time_infer(clr_ctx* c, int64_t n, int repeats, double* ms) {
+  if (!c || !c->mlp || !ms) return fail(c, CLR_E_STATE, "no ensemble");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_time_infer(c->mlp, n, repeats, ms, c->stream, c->err);
+}
+}  // extern "C"

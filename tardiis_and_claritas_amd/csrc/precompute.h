@@ -1,0 +1,27 @@
+// precompute.h -- launch description of the transport-split and sediment
+// coefficient kernels (precompute.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "../../include/claritas.h"
+
+struct PrepLaunch {
+  int n, weight_mode, event_model, sed_model, fs_model, n_quad, n_radial, pad0;
+  double concentration, density, n_medium, wavelength, hybrid_wave_scale, wave_rms_height;
+  double particle_density, water_density, viscosity, gravity, eddy_diffusivity;
+  double R, zmin, zmean, dh, core;
+  // inputs (device)
+  const double *diameter, *weight, *n_real, *k_imag, *qext, *qsca;
+  const float* field;
+  size_t field_len;
+  // outputs (device)
+  double *radius, *number_density, *mu_geom, *mu_sca, *mu_ext, *mu_wave, *mu_event;
+  double *trace_g, *trace_w, *geom_cdf, *wave_cdf, *mu_scalars, *trace_sums;
+  double *sedA, *sedB, *sednorm, *sedws, *radlog, *majorants;
+  float *radius_f, *n_real_f, *k_imag_f, *albedo_f, *geom_cdf_f, *wave_cdf_f;
+  int* error_flag;
+};
+
+// fmax_bits: device scratch of max(1, n) words, zeroed by the caller.
+hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, hipStream_t s);

@@ -1,0 +1,68 @@
+// transport.h -- device parameter block and launch interface of the
+// persistent photon-transport kernel (transport.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/claritas.h"
+
+// Everything the transport kernel reads for one configuration.  Scalars are
+// stored as the float32 values the reference passes to trace_kernel
+// (CLARITAS.py:7642-7760); tables live in device memory.
+struct DevParams {
+  float n_water, n_acrylic, n_air, n_bottom_ext;
+  float rin, rout, zmin, zmean, zwall_top, zlow;
+  float ring_in, throat_out, ring_out, source_r, throat_r, cb_r;
+  float dh, core_r, alpha1, alpha2, sigma, beam_r, legacy_acc, wavelength;
+  float mu_geom, mu_wave, mu_track, coherent, zs_min, field_zmax, field_half;
+  float det_ux[CLR_N_CHANNELS], det_uy[CLR_N_CHANNELS];
+  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX];
+  double rk_K, rk_C0, sum_trace_g, sum_trace_w;
+  int fs_model, src_model, event_model, sed_model;
+  int max_pb, max_iter, cond_trials, n_aux;
+  int n_groups, n_bins, n_theta, n_radial;
+  int fn0, fn1, fn2, pad0;
+  // per-bin tables
+  const float* radius;
+  const float* n_real;
+  const float* k_imag;
+  const float* albedo;
+  const float* gcdf;
+  const float* wcdf;
+  const double* qsca;
+  const double* qext;
+  const double* trace_g;
+  const double* trace_w;
+  const double* sedA;
+  const double* sedB;
+  const double* sednorm;
+  const double* radlog;
+  // Mie phase table (float, sampled by inverse CDF)
+  const float* cdf;    // [n_bins * n_theta]
+  const float* theta;  // [n_theta]
+  const float* field;  // sediment scale field or nullptr
+};
+
+// One unit of the device work queue: histories [start, start+count) of
+// configuration `config`, whose whole run has n_total histories.
+struct WorkItem {
+  int config;
+  int pad;
+  uint64_t start, count, n_total, seed, offset;
+};
+
+struct TransportLaunch {
+  const DevParams* params;   // [n_configs]
+  const WorkItem* items;     // [n_items]
+  int n_items;
+  unsigned int* item_counter;
+  clr_tallies* tallies;      // [n_configs], device layout == clr_tallies
+  clr_record* records;       // optional, config 0 only
+  uint64_t n_records;
+  int grid, block;
+  int plain;  // 1: every configuration uses the plain-Mie law and a shared or uniform field
+};
+
+hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s);
+// Resident workgroups per CU for `block` threads (occupancy query).
+int clr_transport_occupancy(int block, int plain);

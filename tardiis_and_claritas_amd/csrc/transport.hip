@@ -1,0 +1,1713 @@
+// transport.hip -- persistent photon-transport kernel for gfx950.
+//
+// Replaces the reference's trace_kernel + score_detectors_kernel pair
+// (CLARITAS/CLARITAS.py:4378-4436, 4551-5104) and the host-side reduction of
+// its per-ray arrays (CLARITAS.py:7849-7864, 8650-8733).  The physics follows
+// the reference routine by routine (cited at each function); the execution
+// model is MI355X-first:
+//
+//  * one photon history per lane, advanced as a state machine in short steps
+//    (one water segment + event, one acrylic surface, or one headspace
+//    segment), so a lane whose history ends takes the next history from the
+//    workgroup's counter at once: live histories stay packed in every
+//    wavefront instead of waiting for the longest history of a launch;
+//  * a persistent grid pulls work items (configuration, history range) from
+//    a device queue, so one launch serves one or hundreds of configurations;
+//  * Philox4x32-10 counter-based streams (key = 64-bit seed, counter =
+//    (block, 0, history lo, history hi)) replace per-ray xorshift seeds;
+//    64-bit history numbers replace the reference's uint32 ray ids;
+//  * tallies (detector channels x jackknife groups, status, failure codes,
+//    diagnostic sums, interaction histogram) accumulate in LDS with LDS
+//    atomics and are flushed to the per-configuration FP64/u64 tallies with
+//    one global atomic per non-zero LDS word; no per-ray arrays are written.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "transport.h"
+
+#define CLR_COLD __attribute__((noinline))
+
+namespace {
+
+constexpr float kInfF = 3.402823466e+38F;
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kEpsF = 2.0e-7f;
+constexpr double kDEps = 2.220446049250313080847263336181640625e-16;
+constexpr double kFEps = 1.1920928955078125e-7;
+constexpr int kEmergencyCap = 16384;  // acrylic / headspace loop caps, CLARITAS.py:3147,3244
+
+// production failure codes and subcodes, CLARITAS.py:1846-1895
+enum : int {
+  PF_OK = 0, PF_MICRO_NUMERICAL = 1, PF_MACRO_GEOMETRY = 3, PF_MACRO_BOUNCE = 4,
+  PF_ACRYLIC_NO_HIT = 5, PF_ACRYLIC_BOUNCE = 6, PF_BOTTOM_GEOMETRY = 7,
+  PF_WATER_NO_BOUNDARY = 9, PF_ITERATION_LIMIT = 10, PF_NONFINITE_POSITION = 11,
+  PF_NONFINITE_DIRECTION = 12, PF_INVALID_MEDIUM = 13, PF_OTHER = 15, PF_SPHEROID = 16,
+  PF_HEADSPACE_NO_HIT = 17, PF_MAJORANT = 18
+};
+enum : int {
+  SUB_NONE = 0, SUB_SPHERE_ENTRY_SIDE = 1001, SUB_SPHERE_CHORD = 1002, SUB_SPHERE_EXIT_SIDE = 1003,
+  SUB_SPHERE_INTERNAL_SIDE = 1004, SUB_SPHERE_BOUNCE_LIMIT = 1005, SUB_ACRYLIC_NO_HIT = 2001,
+  SUB_ACRYLIC_BOUNCE = 2002, SUB_BOTTOM_TOPOLOGY = 2104, SUB_WATER_NO_BOUNDARY = 3001,
+  SUB_ITERATION_LIMIT = 3002, SUB_NONFINITE_POSITION = 3003, SUB_NONFINITE_DIRECTION = 3004,
+  SUB_WATER_OUTSIDE = 3005, SUB_HEADSPACE_NO_HIT = 3006, SUB_MAJORANT = 4001,
+  MF_REFLECTION_INVALID = 31, MF_REFRACTION_INVALID = 32, MF_SIDE_STATE = 33,
+  MF_NONFINITE_DIRECTION = 34, MF_FRESNEL_INVALID = 35
+};
+
+// ------------------------------------------------------------------ RNG --
+// Philox4x32-10 (Salmon et al., SC'11) with the published constants.
+// The key (the run's seed) is the same for every lane of a work item, so it
+// lives once per workgroup in LDS rather than in two VGPRs per lane.
+__shared__ uint32_t s_key[2];
+struct Rng {
+  uint32_t h0, h1, blk, idx, draws;
+  uint32_t b0, b1, b2, b3;
+};
+__device__ __forceinline__ void rng_seed(Rng& r, uint64_t hist) {
+  r.h0 = (uint32_t)hist; r.h1 = (uint32_t)(hist >> 32);
+  r.blk = 0; r.idx = 4; r.draws = 0;
+  r.b0 = r.b1 = r.b2 = r.b3 = 0;
+}
+__device__ __forceinline__ void philox_refill(Rng& r) {
+  uint32_t c0 = r.blk, c1 = 0u, c2 = r.h0, c3 = r.h1, k0 = s_key[0], k1 = s_key[1];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  r.b0 = c0; r.b1 = c1; r.b2 = c2; r.b3 = c3;
+  r.blk++;
+  r.idx = 0;
+}
+__device__ __forceinline__ uint32_t rng_next(Rng& r) {
+  if (r.idx >= 4) philox_refill(r);
+  const uint32_t v = (r.idx == 0) ? r.b0 : (r.idx == 1) ? r.b1 : (r.idx == 2) ? r.b2 : r.b3;
+  r.idx++;
+  r.draws++;
+  return v;
+}
+// (m + 1/2) 2^-23 with m the top 23 bits: exact, strictly inside (0,1)
+__device__ __forceinline__ float U(Rng& r) {
+  return ((float)(rng_next(r) >> 9) + 0.5f) * 1.1920928955078125e-7f;
+}
+__device__ __forceinline__ double U53(Rng& r) {
+  const uint32_t a = rng_next(r), b = rng_next(r);
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) *
+         1.1102230246251565404236316680908203125e-16;
+}
+__device__ __forceinline__ double Uopen(Rng& r) {
+  return ((double)rng_next(r) + 0.5) * 2.3283064365386962890625e-10;
+}
+
+// ------------------------------------------------------ source samplers --
+// CLARITAS.py:1149-1212
+__device__ float normal_bm(Rng& r) {
+  const float u1 = fmaxf(U(r), 1e-12f), u2 = U(r);
+  return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * kPiF * u2);
+}
+// Marsaglia-Tsang gamma with the shape<1 boost
+CLR_COLD __device__ float gamma_draw(Rng& r, float shape) {
+  if (shape <= 0.0f) return 0.0f;
+  if (fabsf(shape - 1.0f) < 1e-7f) return -logf(fmaxf(U(r), 1e-12f));
+  float boost = 1.0f, a = shape;
+  if (a < 1.0f) {
+    boost = powf(fmaxf(U(r), 1e-12f), 1.0f / a);
+    a += 1.0f;
+  }
+  const float d = a - 0.3333333333333333f, c = 1.0f / sqrtf(9.0f * d);
+  for (int it = 0; it < 128; ++it) {
+    const float x = normal_bm(r);
+    float v = 1.0f + c * x;
+    if (v <= 0.0f) continue;
+    v = v * v * v;
+    const float u = U(r);
+    if (u < 1.0f - 0.0331f * x * x * x * x) return boost * d * v;
+    if (logf(fmaxf(u, 1e-12f)) < 0.5f * x * x + d * (1.0f - v + logf(v))) return boost * d * v;
+  }
+  return boost * d;
+}
+__device__ float beta_draw(Rng& r, float a, float b) {
+  if (a <= 0.0f || b <= 0.0f) return 0.0f;
+  if (fabsf(a - 1.0f) < 1e-7f) {
+    const float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+    return 1.0f - powf(1.0f - u, 1.0f / b);
+  }
+  if (fabsf(b - 1.0f) < 1e-7f) {
+    const float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+    return powf(u, 1.0f / a);
+  }
+  const float ga = gamma_draw(r, a), gb = gamma_draw(r, b), s = ga + gb;
+  return s > 0.0f ? ga / s : 0.0f;
+}
+CLR_COLD __device__ float beta_radiance_polar(Rng& r, float a, float b) {
+  const float c = 0.5f * kPiF;
+  for (int it = 0; it < 128; ++it) {
+    const float u = fminf(fmaxf(beta_draw(r, a + 1.0f, b), 0.0f), 1.0f);
+    const float x = c * u, acc = (x < 1.0e-7f) ? 1.0f : sinf(x) / x;
+    if (U(r) <= acc) return x;
+  }
+  return c * fminf(fmaxf(beta_draw(r, a + 1.0f, b), 0.0f), 1.0f);
+}
+
+// --------------------------------------------------------- vector optics --
+// CLARITAS.py:1214-1260
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+  return ax * bx + ay * by + az * bz;
+}
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+  const float q = x * x + y * y + z * z;
+  if (q <= 0.0f) { x = 0.0f; y = 1.0f; z = 0.0f; return; }
+  const float s = 1.0f / sqrtf(q);
+  x *= s; y *= s; z *= s;
+}
+__device__ __forceinline__ void reflect3(float& vx, float& vy, float& vz, float nx, float ny,
+                                         float nz) {
+  const float d = dot3(vx, vy, vz, nx, ny, nz);
+  vx = vx - 2.0f * d * nx; vy = vy - 2.0f * d * ny; vz = vz - 2.0f * d * nz;
+  normalize3(vx, vy, vz);
+}
+// n points into the incident medium; writes the refracted direction to o*
+__device__ __forceinline__ bool refract3(float ix, float iy, float iz, float nx, float ny, float nz,
+                                         float n1, float n2, float& ox, float& oy, float& oz) {
+  const float ci = fminf(fmaxf(-dot3(ix, iy, iz, nx, ny, nz), 0.0f), 1.0f);
+  const float eta = n1 / n2, k = 1.0f - eta * eta * (1.0f - ci * ci);
+  if (k <= 0.0f) return false;
+  const float f = eta * ci - sqrtf(k);
+  ox = eta * ix + f * nx; oy = eta * iy + f * ny; oz = eta * iz + f * nz;
+  normalize3(ox, oy, oz);
+  return true;
+}
+__device__ __forceinline__ float fresnel_R(float ci, float n1, float n2, bool& tir) {
+  ci = fminf(fmaxf(ci, 0.0f), 1.0f);
+  const float eta = n1 / n2, st2 = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+  if (st2 >= 1.0f) { tir = true; return 1.0f; }
+  tir = false;
+  const float ct = sqrtf(fmaxf(0.0f, 1.0f - st2));
+  const float ds = n1 * ci + n2 * ct, dp = n1 * ct + n2 * ci;
+  float rs = 1.0f, rp = 1.0f;
+  if (fabsf(ds) > 1e-12f) { const float a = (n1 * ci - n2 * ct) / ds; rs = a * a; }
+  if (fabsf(dp) > 1e-12f) { const float a = (n1 * ct - n2 * ci) / dp; rp = a * a; }
+  return fminf(fmaxf(0.5f * (rs + rp), 0.0f), 1.0f);
+}
+// Fresnel decision at an interface with normal n into the incident medium:
+// returns true (and updates v) for transmission; reflects v otherwise.
+// The uniform is drawn only when the interface is not TIR, matching the
+// short-circuit order `tir || u < R || !refract` of CLARITAS.py:4709,4986.
+__device__ __forceinline__ bool fresnel_transmit(Rng& r, float& vx, float& vy, float& vz, float nx,
+                                                 float ny, float nz, float n1, float n2,
+                                                 bool& tir) {
+  const float ci = fminf(fmaxf(-dot3(vx, vy, vz, nx, ny, nz), 0.0f), 1.0f);
+  const float R = fresnel_R(ci, n1, n2, tir);
+  float a, b, c;
+  if (!tir && !(U(r) < R) && refract3(vx, vy, vz, nx, ny, nz, n1, n2, a, b, c)) {
+    vx = a; vy = b; vz = c;
+    return true;
+  }
+  reflect3(vx, vy, vz, nx, ny, nz);
+  return false;
+}
+
+// ---------------------------------------------- complex-index interface --
+// CLARITAS.py:1376-1452 (float complex arithmetic)
+struct cf { float re, im; };
+__device__ __forceinline__ cf cmk(float a, float b) { return cf{a, b}; }
+__device__ __forceinline__ cf cadd(cf a, cf b) { return cmk(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cf csub(cf a, cf b) { return cmk(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+  return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ __forceinline__ cf cdivf(cf a, cf b) {
+  float d = b.re * b.re + b.im * b.im;
+  if (d < 1e-30f) d = 1e-30f;
+  return cmk((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+__device__ __forceinline__ float cabs2(cf a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ cf csqrtp(cf z) {
+  const float r = sqrtf(z.re * z.re + z.im * z.im);
+  float u = sqrtf(fmaxf(0.0f, 0.5f * (r + z.re))), v = sqrtf(fmaxf(0.0f, 0.5f * (r - z.re)));
+  if (z.im < 0.0f) v = -v;
+  if (u < 0.0f || (fabsf(u) < 1e-20f && v < 0.0f)) { u = -u; v = -v; }
+  return cmk(u, v);
+}
+CLR_COLD __device__ bool complex_interface(float ci, float n1r, float n1k, float n2r, float n2k, float& R,
+                                  float& t_tan, float& t_nrm) {
+  if (!(n1r > 0.f && n2r > 0.f && n1k >= 0.f && n2k >= 0.f)) return false;
+  ci = fminf(fmaxf(ci, 0.f), 1.f);
+  const float beta = n1r * sqrtf(fmaxf(0.f, 1.f - ci * ci));
+  const cf n1 = cmk(n1r, n1k), n2 = cmk(n2r, n2k);
+  const cf e1 = cmul(n1, n1), e2 = cmul(n2, n2), b2 = cmk(beta * beta, 0.f);
+  const cf q1 = csqrtp(csub(e1, b2)), q2 = csqrtp(csub(e2, b2));
+  const cf rs = cdivf(csub(q1, q2), cadd(q1, q2));
+  const cf pa = cmul(e2, q1), pb = cmul(e1, q2);
+  const cf rp = cdivf(csub(pa, pb), cadd(pa, pb));
+  const float Rv = .5f * (cabs2(rs) + cabs2(rp));
+  if (!isfinite(Rv)) return false;
+  const cf qe = cdivf(q2, e2), be = cdivf(cmk(beta, 0.f), e2);
+  const float Sx = beta, Sz = q2.re, Px = be.re, Pz = qe.re;
+  const float sn = hypotf(Sx, Sz), pn = hypotf(Px, Pz);
+  if (!(sn > 0.f && pn > 0.f && Sz >= 0.f && Pz >= 0.f)) return false;
+  const float tx = .5f * (Sx / sn + Px / pn), tz = .5f * (Sz / sn + Pz / pn);
+  const float nm = hypotf(tx, tz);
+  if (!(nm > 0.f && isfinite(nm))) return false;
+  R = fminf(fmaxf(Rv, 0.f), 1.f);
+  t_tan = tx / nm;
+  t_nrm = tz / nm;
+  return true;
+}
+__device__ bool energy_refract3(float ix, float iy, float iz, float nx, float ny, float nz,
+                                float n1r, float n1k, float n2r, float n2k, float& ox, float& oy,
+                                float& oz) {
+  const float ci = fminf(fmaxf(-dot3(ix, iy, iz, nx, ny, nz), 0.f), 1.f);
+  float R, ts, tn;
+  if (!complex_interface(ci, n1r, n1k, n2r, n2k, R, ts, tn)) return false;
+  float px = ix + ci * nx, py = iy + ci * ny, pz = iz + ci * nz;
+  const float pn = sqrtf(px * px + py * py + pz * pz);
+  if (pn > 1.e-7f) { px /= pn; py /= pn; pz /= pn; }
+  else { px = py = pz = 0.f; ts = 0.f; tn = 1.f; }
+  ox = ts * px - tn * nx; oy = ts * py - tn * ny; oz = ts * pz - tn * nz;
+  normalize3(ox, oy, oz);
+  return isfinite(ox) && isfinite(oy) && isfinite(oz);
+}
+__device__ __forceinline__ void perpendicular_basis(float vx, float vy, float vz, float& e1x,
+                                                    float& e1y, float& e1z, float& e2x, float& e2y,
+                                                    float& e2z) {
+  float rx, ry, rz;
+  if (fabsf(vy) < 0.9f) { rx = 0.0f; ry = 1.0f; rz = 0.0f; } else { rx = 1.0f; ry = 0.0f; rz = 0.0f; }
+  e1x = ry * vz - rz * vy; e1y = rz * vx - rx * vz; e1z = rx * vy - ry * vx;
+  normalize3(e1x, e1y, e1z);
+  e2x = vy * e1z - vz * e1y; e2y = vz * e1x - vx * e1z; e2z = vx * e1y - vy * e1x;
+  normalize3(e2x, e2y, e2z);
+}
+
+// ----------------------------------------------------------- free surface --
+// CLARITAS.py:1503-1527, 1611-1779; the Rankine sheet has no reference twin
+__device__ float free_surface_z(const DevParams& P, float x, float y) {
+  const int M = P.fs_model;
+  if (M == 0 || P.dh <= 0.0f) return P.zmean;
+  const float q = x * x + y * y, R = P.rin;
+  if (M == 1) return P.zmean + P.dh * (q / (R * R) - 0.5f);
+  if (M == 3) {
+    const double qa = (double)q, a2 = (double)P.core_r * (double)P.core_r;
+    const double f = (qa < a2) ? P.rk_K * qa / a2 : P.rk_K * (2.0 - a2 / qa);
+    return (float)(P.rk_C0 + f);
+  }
+  const float a2 = P.core_r * P.core_r, R2 = R * R;
+  const float H = P.dh * (a2 + R2) / R2;
+  const float hbar = H * (1.0f - (a2 / R2) * log1pf(R2 / a2));
+  return P.zmean - hbar + H * q / (a2 + q);
+}
+__device__ double quadratic_min_positive(double A, double B, double C) {
+  if (fabs(A) < 1e-24) {
+    if (fabs(B) < 1e-24) return 1.0e300;
+    const double t = -C / B;
+    return t > 0.0 ? t : 1.0e300;
+  }
+  const double D = B * B - 4.0 * A * C;
+  if (D < 0.0) return 1.0e300;
+  const double sd = sqrt(fmax(D, 0.0)), q = -0.5 * (B + copysign(sd, B));
+  const double t1 = (fabs(q) > 1e-300) ? q / A : (-B - sd) / (2.0 * A);
+  const double t2 = (fabs(q) > 1e-300) ? C / q : (-B + sd) / (2.0 * A);
+  double best = 1.0e300;
+  if (t1 > 0.0 && t1 < best) best = t1;
+  if (t2 > 0.0 && t2 < best) best = t2;
+  return best;
+}
+CLR_COLD __device__ double cubic_min_positive(double A3, double A2, double A1, double A0) {
+  const double scale = fmax(fmax(fabs(A3), fabs(A2)), fmax(fabs(A1), fabs(A0)));
+  if (scale <= 0.0) return 1.0e300;
+  if (fabs(A3) <= 1e-12) return quadratic_min_positive(A2, A1, A0);
+  const double a = A2 / A3, b = A1 / A3, c = A0 / A3;
+  const double p = b - a * a / 3.0, q = 2.0 * a * a * a / 27.0 - a * b / 3.0 + c;
+  const double hq = 0.5 * q, tp = p / 3.0, d1 = hq * hq, d2 = tp * tp * tp, disc = d1 + d2;
+  const double tol = 1e-12 * (fabs(d1) + fabs(d2) + 1e-300);
+  double best = 1.0e300;
+  if (disc > tol) {
+    const double sd = sqrt(disc), t = cbrt(-hq + sd) + cbrt(-hq - sd) - a / 3.0;
+    if (t > 0.0) best = t;
+  } else if (fabs(p) < 1e-24 && fabs(q) < 1e-24) {
+    const double t = -a / 3.0;
+    if (t > 0.0) best = t;
+  } else if (disc >= -tol) {
+    const double u = cbrt(-hq), t1 = 2.0 * u - a / 3.0, t2 = -u - a / 3.0;
+    if (t1 > 0.0 && t1 < best) best = t1;
+    if (t2 > 0.0 && t2 < best) best = t2;
+  } else {
+    const double mm = 2.0 * sqrt(fmax(-tp, 0.0)), den = sqrt(fmax(-(tp * tp * tp), 1e-300));
+    const double phi = acos(fmin(fmax(-hq / den, -1.0), 1.0));
+    for (int k = 0; k < 3; ++k) {
+      const double t = mm * cos((phi + 2.0 * 3.141592653589793238462643383279502884 * (double)k) / 3.0) - a / 3.0;
+      if (t > 0.0 && t < best) best = t;
+    }
+  }
+  return best;
+}
+// first sign change of a cubic on [lo,hi] by monotone-piece bisection,
+// CLARITAS.py:1690-1722
+CLR_COLD __device__ double bounded_cubic_root(double a3, double a2, double a1, double a0, double lo,
+                                     double hi) {
+  if (!(hi > lo) || !isfinite(hi)) return 1.0e300;
+  double c0 = lo, c1 = hi, c2 = hi, c3 = hi;
+  int n = 1;
+  const double A = 3.0 * a3, B = 2.0 * a2, C = a1;
+  double r1 = 2.0 * hi, r2 = 2.0 * hi;
+  if (A != 0.0) {
+    const double disc = B * B - 4.0 * A * C;
+    if (disc >= 0.0) {
+      const double q = -0.5 * (B + copysign(sqrt(disc), B));
+      r1 = (q != 0.0) ? q / A : -B / (2.0 * A);
+      r2 = (q != 0.0) ? C / q : r1;
+      if (r2 == r1) r2 = 2.0 * hi;
+    }
+  } else if (B != 0.0) {
+    r1 = -C / B;
+  }
+  // sorted cut list lo < (r1, r2 inside) < hi
+  const double sa = fmin(r1, r2), sb = fmax(r1, r2);
+  if (sa > lo && sa < hi) { c1 = sa; n = 2; }
+  if (sb > lo && sb < hi && sb != sa) { if (n == 2) { c2 = sb; n = 3; } else { c1 = sb; n = 2; } }
+  if (n == 1) c1 = hi; else if (n == 2) c2 = hi; else c3 = hi;
+  const int ncut = n + 1;
+  for (int i = 0; i < ncut - 1; ++i) {
+    double l = (i == 0) ? c0 : (i == 1) ? c1 : c2;
+    double r = (i == 0) ? c1 : (i == 1) ? c2 : c3;
+    double fl = ((a3 * l + a2) * l + a1) * l + a0, fr = ((a3 * r + a2) * r + a1) * r + a0;
+    if (l > 0.0 && fl == 0.0) return l;
+    if ((fl < 0.0 && fr > 0.0) || (fl > 0.0 && fr < 0.0)) {
+      for (int j = 0; j < 64; ++j) {
+        const double mid = l + 0.5 * (r - l);
+        if (mid == l || mid == r) break;
+        const double fm = ((a3 * mid + a2) * mid + a1) * mid + a0;
+        if (fm == 0.0) { l = r = mid; break; }
+        if ((fl < 0.0 && fm < 0.0) || (fl > 0.0 && fm > 0.0)) { l = mid; fl = fm; } else r = mid;
+      }
+      return l + 0.5 * (r - l);
+    }
+    if (r > 0.0 && fr == 0.0) return r;
+  }
+  return 1.0e300;
+}
+CLR_COLD __device__ double rankine_hit(const DevParams& P, double X, double Y, double Z, double VX,
+                              double VY, double VZ) {
+  const double a2 = (double)P.core_r * (double)P.core_r, K = P.rk_K, C0 = P.rk_C0;
+  const double RR = (double)P.rin * (double)P.rin;
+  const double Q2 = VX * VX + VY * VY, Q1 = 2.0 * (X * VX + Y * VY), Q0 = X * X + Y * Y;
+  if (!(Q2 > 0.0)) {
+    const double f = (Q0 < a2) ? K * Q0 / a2 : K * (2.0 - a2 / Q0);
+    const double t = (VZ != 0.0) ? (C0 + f - Z) / VZ : 1.0e300;
+    return (t > 0.0) ? t : 1.0e300;
+  }
+  double tend = 1.0e300, tin = 1.0e300, tout = -1.0;
+  const double dc = Q1 * Q1 - 4.0 * Q2 * (Q0 - RR);
+  if (dc >= 0.0) {
+    const double q = -0.5 * (Q1 + copysign(sqrt(dc), Q1));
+    const double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - RR) / q : ta;
+    tend = fmax(ta, tb);
+  }
+  const double dk = Q1 * Q1 - 4.0 * Q2 * (Q0 - a2);
+  if (dk > 0.0) {
+    const double q = -0.5 * (Q1 + copysign(sqrt(dk), Q1));
+    const double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - a2) / q : ta;
+    tin = fmin(ta, tb);
+    tout = fmax(ta, tb);
+  }
+  if (!(tend > 0.0) || !(tend < 1.0e299)) return 1.0e300;
+  // up to three pieces [0,b1) [b1,b2) [b2,tend) split where the ray crosses r=a
+  const double b1 = (tin > 0.0 && tin < tend) ? tin : tend;
+  const double b2 = (tout > 0.0 && tout < tend) ? fmax(tout, b1) : tend;
+  double lo = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double hi = (i == 0) ? b1 : (i == 1) ? b2 : tend;
+    if (hi > lo) {
+      const double mid = 0.5 * (lo + hi), qm = (Q2 * mid + Q1) * mid + Q0;
+      double t;
+      if (qm < a2) {
+        const double k = K / a2;
+        t = bounded_cubic_root(0.0, -k * Q2, VZ - k * Q1, Z - C0 - k * Q0, lo, hi);
+      } else {
+        const double L = Z - C0 - 2.0 * K;
+        t = bounded_cubic_root(VZ * Q2, L * Q2 + VZ * Q1, L * Q1 + VZ * Q0, L * Q0 + K * a2, lo, hi);
+      }
+      if (t < 1.0e299 && t > 0.0) return t;
+    }
+    lo = fmax(lo, hi);
+  }
+  return 1.0e300;
+}
+__device__ float free_surface_hit(const DevParams& P, float x, float y, float z, float vx, float vy,
+                                  float vz) {
+  const int M = P.fs_model;
+  const float R = P.rin;
+  if (M == 0 || P.dh <= 0.0f) {
+    if (fabsf(vz) <= 1e-14f) return kInfF;
+    const float t = (P.zmean - z) / vz;
+    return (t > 0.0f) ? t : kInfF;
+  }
+  if (M == 1) {
+    const float aa = P.dh / (R * R);
+    const double A = -(double)aa * ((double)vx * vx + (double)vy * vy);
+    const double B = (double)vz - 2.0 * (double)aa * ((double)x * vx + (double)y * vy);
+    const double C = (double)z - ((double)P.zmean - 0.5 * (double)P.dh) -
+                     (double)aa * ((double)x * x + (double)y * y);
+    const double t = quadratic_min_positive(A, B, C);
+    return (t < 0.5 * (double)kInfF) ? (float)t : kInfF;
+  }
+  if (M == 3) {
+    const double t = rankine_hit(P, x, y, z, vx, vy, vz);
+    return (t > 0.0 && t < 0.5 * (double)kInfF) ? (float)t : kInfF;
+  }
+  const double X = x, Y = y, Z = z, VX = vx, VY = vy, VZ = vz;
+  const double RR = (double)R * (double)R, aa = (double)P.core_r * (double)P.core_r;
+  const double H = (double)P.dh * (aa + RR) / RR;
+  const double hbar = H * (1.0 - (aa / RR) * log1p(RR / aa));
+  const double Cs = (double)P.zmean - hbar;
+  const double Q2 = VX * VX + VY * VY, Q1 = 2.0 * (X * VX + Y * VY), Q0 = X * X + Y * Y, L0 = Z - Cs;
+  const double c3 = VZ * Q2, c2 = (L0 - H) * Q2 + VZ * Q1, c1 = (L0 - H) * Q1 + VZ * (aa + Q0);
+  const double c0 = L0 * aa + (L0 - H) * Q0;
+  double t = cubic_min_positive(c3, c2, c1, c0);
+  const double xx = X + t * VX, yy = Y + t * VY, qq = xx * xx + yy * yy;
+  const double res = (Z + t * VZ) - (Cs + H * qq / (aa + qq));
+  const double tol = 32.0 * kDEps * fmax(fmax(fabs(Z), fabs(Cs) + fabs(H)), 1.0e-3);
+  if (!(t < 0.5 * (double)kInfF) || qq > RR * (1.0 + 2e-6) || !isfinite(res) || fabs(res) > tol) {
+    if (Q2 > 0.0) {
+      const double dc = Q1 * Q1 - 4.0 * Q2 * (Q0 - RR);
+      if (dc < 0.0) return kInfF;
+      const double q = -0.5 * (Q1 + copysign(sqrt(dc), Q1));
+      const double ta = (q != 0.0) ? q / Q2 : -Q1 / (2.0 * Q2), tb = (q != 0.0) ? (Q0 - RR) / q : ta;
+      t = bounded_cubic_root(c3, c2, c1, c0, fmax(0.0, fmin(ta, tb)), fmax(ta, tb));
+    } else {
+      t = (VZ != 0.0) ? ((Cs + H * Q0 / (aa + Q0)) - Z) / VZ : 1.0e300;
+      if (!(t > 0.0) || Q0 > RR * (1.0 + 2e-6)) return kInfF;
+    }
+  }
+  return (t > 0.0 && t < 0.5 * (double)kInfF) ? (float)t : kInfF;
+}
+__device__ void free_surface_normal(const DevParams& P, float x, float y, float& nx, float& ny,
+                                    float& nz) {
+  const int M = P.fs_model;
+  if (M == 0 || P.dh <= 0.0f) { nx = 0.0f; ny = 0.0f; nz = -1.0f; return; }
+  const float R = P.rin;
+  if (M == 1) {
+    const float aa = P.dh / (R * R);
+    nx = 2.0f * aa * x; ny = 2.0f * aa * y; nz = -1.0f;
+    normalize3(nx, ny, nz);
+    return;
+  }
+  const float q = x * x + y * y;
+  float fac;
+  if (M == 3) {
+    const float a2 = P.core_r * P.core_r, K = (float)P.rk_K;
+    fac = (q < a2) ? 2.0f * K / a2 : 2.0f * K * a2 / (q * q);
+  } else {
+    const float a2 = P.core_r * P.core_r, R2 = R * R, H = P.dh * (a2 + R2) / R2;
+    fac = 2.0f * H * a2 / ((a2 + q) * (a2 + q));
+  }
+  nx = fac * x; ny = fac * y; nz = -1.0f;
+  normalize3(nx, ny, nz);
+}
+__device__ double free_surface_z_d(const DevParams& P, double q) {
+  const int M = P.fs_model;
+  const double zm = (double)P.zmean;
+  if (M == 0 || P.dh <= 0.0f) return zm;
+  const double R = (double)P.rin, dh = (double)P.dh, a = (double)P.core_r, R2 = R * R, a2 = a * a;
+  if (M == 1) return zm + dh * (q / R2 - 0.5);
+  if (M == 3) return P.rk_C0 + ((q < a2) ? P.rk_K * q / a2 : P.rk_K * (2.0 - a2 / q));
+  const double H = dh * (a2 + R2) / R2, hbar = H * (1.0 - (a2 / R2) * log1p(R2 / a2));
+  return zm - hbar + H * q / (a2 + q);
+}
+
+// ----------------------------------------------------------- cell geometry --
+// CLARITAS.py:1473-1500, 2888-3063
+__device__ __forceinline__ void radial_normal(float x, float y, float& nx, float& ny) {
+  const float r = sqrtf(fmaxf(x * x + y * y, 1e-30f));
+  nx = x / r; ny = y / r;
+}
+__device__ __forceinline__ void cylinder_roots_d(float x, float y, float vx, float vy, float R,
+                                                 double& t0, double& t1) {
+  t0 = 1.0e300; t1 = 1.0e300;
+  const double X = x, Y = y, VX = vx, VY = vy, RR = R;
+  const double a = VX * VX + VY * VY;
+  if (!(a > 1e-30) || !isfinite(a)) return;
+  const double b = X * VX + Y * VY, c = X * X + Y * Y - RR * RR;
+  double d = b * b - a * c;
+  const double dtol = 16.0 * kDEps * (fabs(b * b) + fabs(a * c) + 1e-300);
+  if (d < 0.0) { if (d >= -dtol) d = 0.0; else return; }
+  const double sd = sqrt(fmax(d, 0.0)), q = -b - copysign(sd, b);
+  const double r0 = (fabs(q) > 1e-300) ? q / a : (-b - sd) / a;
+  const double r1 = (fabs(q) > 1e-300) ? c / q : (-b + sd) / a;
+  double a0 = 1.0e300, a1 = 1.0e300;
+  if (r0 > 0.0 && isfinite(r0)) a0 = r0;
+  if (r1 > 0.0 && isfinite(r1)) { if (r1 < a0) { a1 = a0; a0 = r1; } else a1 = r1; }
+  t0 = a0; t1 = a1;
+}
+__device__ __forceinline__ double support_tol(const DevParams& P, float Rin, float Rout, float Zmin,
+                                              float Zlow, float Ztop) {
+  double s = fmax(fmax(fabs((double)Rout), fabs((double)Zlow)), fabs((double)Ztop));
+  s = fmax(s, fmax(fabs((double)Rin), fabs((double)Zmin)));
+  return 8.0 * kFEps * fmax(s, 1.0e-3);
+}
+__device__ double finite_cylinder_forward_d(const DevParams& P, float x, float y, float z, float vx,
+                                            float vy, float vz, float R, float zlo, float zhi) {
+  double r0, r1;
+  cylinder_roots_d(x, y, vx, vy, R, r0, r1);
+  const double tol = support_tol(P, R, R, zlo, zlo, zhi);
+  double best = 1.0e300;
+  if (r0 > 0.0 && r0 < 1.0e299) {
+    const double zh = (double)z + r0 * (double)vz;
+    if (zh >= (double)zlo - tol && zh <= (double)zhi + tol) best = r0;
+  }
+  if (r1 > 0.0 && r1 < 1.0e299 && r1 < best) {
+    const double zh = (double)z + r1 * (double)vz;
+    if (zh >= (double)zlo - tol && zh <= (double)zhi + tol) best = r1;
+  }
+  return best;
+}
+__device__ double bore_wall_hit_d(const DevParams& P, float x, float y, float z, float vx, float vy,
+                                  float vz, float R, float zlo, float zhi) {
+  const double X = x, Y = y, VX = vx, VY = vy, RR = (double)R * R;
+  const double rr2 = X * X + Y * Y, a = VX * VX + VY * VY;
+  double c = rr2 - RR, b = X * VX + Y * VY;
+  const double rtol = 8.0 * kFEps * RR, ztol = support_tol(P, R, R, zlo, zlo, zhi);
+  if (c > rtol || !isfinite(c) || !isfinite(b) || !isfinite(a)) return -1.0;
+  if (c >= 0.0) {
+    if (b >= 0.0 && a > 0.0 && z >= zlo - ztol && z <= zhi + ztol) return 0.0;
+    b *= sqrt(RR / rr2);
+    c = 0.0;
+  }
+  if (!(a > 1e-30)) return 1.0e300;
+  const double d = b * b - a * c;
+  if (d < 0.0) return 1.0e300;
+  const double sd = sqrt(d), q = -b - copysign(sd, b);
+  const double t = (b < 0.0) ? q / a : ((q != 0.0) ? c / q : 0.0);
+  if (!(t >= 0.0) || !isfinite(t)) return 1.0e300;
+  const double zh = (double)z + t * (double)vz;
+  return (zh >= zlo - ztol && zh <= zhi + ztol) ? t : 1.0e300;
+}
+CLR_COLD __device__ bool water_surface_offset(const DevParams& P, float& x, float& y, float& z, float nx,
+                                     float ny, float nz) {
+  const double rr2 = (double)x * (double)x + (double)y * (double)y;
+  const double RR = (double)P.rin * P.rin, tol = 8.0 * kFEps * RR;
+  if (!isfinite(rr2) || rr2 > RR + tol) return false;
+  float xx = x + kEpsF * nx, yy = y + kEpsF * ny, zz = z + kEpsF * nz;
+  const double rad = hypot((double)xx, (double)yy);
+  if (rad > (double)P.rin) {
+    const double target = (double)P.rin * (1.0 - 2.0 * kFEps);
+    xx = (float)((double)xx * target / rad);
+    yy = (float)((double)yy * target / rad);
+    const float surf = free_surface_z(P, xx, yy);
+    if (zz >= surf) zz = nextafterf(surf, -kInfF);
+  }
+  if (!isfinite(xx) || !isfinite(yy) || !isfinite(zz) || zz <= P.zmin ||
+      zz >= free_surface_z(P, xx, yy))
+    return false;
+  x = xx; y = yy; z = zz;
+  return true;
+}
+__device__ int acrylic_next_surface(const DevParams& P, float x, float y, float z, float vx,
+                                    float vy, float vz, double& t_out) {
+  double best = 1.0e300;
+  int surf = 0;
+  const double tol = support_tol(P, P.rin, P.rout, P.zmin, P.zlow, P.zwall_top);
+  double r0, r1;
+  cylinder_roots_d(x, y, vx, vy, P.rin, r0, r1);
+  if (r0 < 1.0e299) {
+    const double zh = (double)z + r0 * (double)vz;
+    if (zh >= (double)P.zmin - tol && zh <= (double)P.zwall_top + tol && r0 > 0.0 && r0 < best) { best = r0; surf = 1; }
+  }
+  if (r1 < 1.0e299) {
+    const double zh = (double)z + r1 * (double)vz;
+    if (zh >= (double)P.zmin - tol && zh <= (double)P.zwall_top + tol && r1 > 0.0 && r1 < best) { best = r1; surf = 1; }
+  }
+  cylinder_roots_d(x, y, vx, vy, P.rout, r0, r1);
+  if (r0 < 1.0e299) {
+    const double zh = (double)z + r0 * (double)vz;
+    if (zh >= (double)P.zlow - tol && zh <= (double)P.zwall_top + tol && r0 > 0.0 && r0 < best) { best = r0; surf = 2; }
+  }
+  if (r1 < 1.0e299) {
+    const double zh = (double)z + r1 * (double)vz;
+    if (zh >= (double)P.zlow - tol && zh <= (double)P.zwall_top + tol && r1 > 0.0 && r1 < best) { best = r1; surf = 2; }
+  }
+  if (vz < 0.0f) {
+    const double t = ((double)P.zlow - (double)z) / (double)vz;
+    if (t > 0.0 && isfinite(t)) {
+      const double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy;
+      if (hypot(X, Y) <= (double)P.rout + tol && t < best) { best = t; surf = 3; }
+    }
+  }
+  if (vz > 0.0f) {
+    const double t = ((double)P.zwall_top - (double)z) / (double)vz;
+    if (t > 0.0 && isfinite(t)) {
+      const double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy, rr = hypot(X, Y);
+      if (rr >= (double)P.rin - tol && rr <= (double)P.rout + tol && t < best) { best = t; surf = 4; }
+    }
+    const double ta = ((double)P.zmin - (double)z) / (double)vz;
+    if (ta > 0.0 && isfinite(ta)) {
+      const double X = (double)x + ta * (double)vx, Y = (double)y + ta * (double)vy;
+      if (hypot(X, Y) < (double)P.rin - tol && ta < best) { best = ta; surf = 5; }
+    }
+  }
+  t_out = best;
+  return surf;
+}
+__device__ __forceinline__ bool tir_for_ci(double ci, double n1, double n2) {
+  if (!(n1 > 0.0) || !(n2 > 0.0) || n1 <= n2) return false;
+  ci = fmin(fmax(ci, 0.0), 1.0);
+  return (n1 / n2) * sqrt(fmax(0.0, 1.0 - ci * ci)) >= 1.0;
+}
+// conserved-orbit trap classifier at the emergency cap, CLARITAS.py:3083-3135
+CLR_COLD __device__ bool acrylic_trapped(const DevParams& P, float x, float y, float vx, float vy, float vz) {
+  const double dx = vx, dy = vy, dz = vz, sp2 = dx * dx + dy * dy + dz * dz;
+  if (!(sp2 > 0.0) || !isfinite(sp2)) return false;
+  const double vxy2 = dx * dx + dy * dy, L = (double)x * dy - (double)y * dx, rs = sqrt(sp2);
+  const double reach = 4.0 * (double)kEpsF;
+  const double na = P.n_acrylic, nw = P.n_water, nair = P.n_air, nb = P.n_bottom_ext;
+  if (vxy2 > 0.0) {
+    const double tg = L / (double)P.rout;
+    double rad2 = vxy2 - tg * tg;
+    if (rad2 < 0.0 && rad2 > -1.0e-12 * sp2) rad2 = 0.0;
+    if (rad2 < 0.0) return false;
+    if (!tir_for_ci(sqrt(rad2) / rs, na, nair)) return false;
+    const double impact = fabs(L) / sqrt(vxy2);
+    if (impact <= (double)P.rin + reach) {
+      const double ti = L / (double)P.rin;
+      double ri2 = vxy2 - ti * ti;
+      if (ri2 < 0.0 && ri2 > -1.0e-12 * sp2) ri2 = 0.0;
+      if (ri2 >= 0.0 && !tir_for_ci(sqrt(ri2) / rs, na, nw)) return false;
+    }
+    if (impact < (double)P.rin - reach && fabs(dz) > 0.0 && !tir_for_ci(fabs(dz) / rs, na, nw)) return false;
+  }
+  if (fabs(dz) > 0.0) {
+    const double ca = fabs(dz) / rs;
+    if (vxy2 <= 1.0e-30 * sp2 && hypot((double)x, (double)y) < (double)P.rin - reach &&
+        !tir_for_ci(ca, na, nw))
+      return false;
+    if (!tir_for_ci(ca, na, nair)) return false;
+    if (!tir_for_ci(ca, na, nb)) return false;
+  }
+  return true;
+}
+
+// -------------------------------------------------------- sediment field --
+// CLARITAS.py:1537-1609; shared r-z / x-y-z fields are extensions
+__device__ double rz_sample(const float* __restrict__ G, int NR, int NZ, float x, float y, float z,
+                            float R, float ZMIN, float ZMAX) {
+  if (NR < 2 || NZ < 2 || !(ZMAX > ZMIN)) return 1.0;
+  double rr = sqrt(fmax((double)x * (double)x + (double)y * (double)y, 0.0));
+  if (rr > (double)R) rr = (double)R;
+  const double dr = (double)R / (double)NR, dz = ((double)ZMAX - (double)ZMIN) / (double)NZ;
+  double ur = rr / dr - 0.5, uz = ((double)z - (double)ZMIN) / dz - 0.5;
+  ur = fmin(fmax(ur, 0.0), (double)(NR - 1));
+  uz = fmin(fmax(uz, 0.0), (double)(NZ - 1));
+  int i = (int)floor(ur), k = (int)floor(uz);
+  double tr = ur - (double)i, tz = uz - (double)k;
+  if (i >= NR - 1) { i = NR - 2; tr = 1.0; }
+  if (k >= NZ - 1) { k = NZ - 2; tz = 1.0; }
+  const float* g = G + (size_t)i * NZ + k;
+  const double q00 = g[0], q01 = g[1], q10 = g[NZ], q11 = g[NZ + 1];
+  const double q0 = (1.0 - tr) * q00 + tr * q10, q1 = (1.0 - tr) * q01 + tr * q11;
+  return fmax((1.0 - tz) * q0 + tz * q1, 0.0);
+}
+__device__ double xyz_sample(const float* __restrict__ G, int NX, int NY, int NZ, float x, float y,
+                             float z, float H, float ZMIN, float ZMAX) {
+  const double dx = 2.0 * (double)H / NX, dy = 2.0 * (double)H / NY, dz = ((double)ZMAX - (double)ZMIN) / NZ;
+  double u0 = ((double)x + H) / dx - 0.5, u1 = ((double)y + H) / dy - 0.5, u2 = ((double)z - ZMIN) / dz - 0.5;
+  u0 = fmin(fmax(u0, 0.0), (double)(NX - 1));
+  u1 = fmin(fmax(u1, 0.0), (double)(NY - 1));
+  u2 = fmin(fmax(u2, 0.0), (double)(NZ - 1));
+  int i0 = (int)floor(u0), i1 = (int)floor(u1), i2 = (int)floor(u2);
+  double t0 = u0 - i0, t1 = u1 - i1, t2 = u2 - i2;
+  if (i0 >= NX - 1) { i0 = NX - 2; t0 = 1.0; }
+  if (i1 >= NY - 1) { i1 = NY - 2; t1 = 1.0; }
+  if (i2 >= NZ - 1) { i2 = NZ - 2; t2 = 1.0; }
+  const size_t sy = (size_t)NZ, sx = (size_t)NY * NZ;
+  const float* b = G + (size_t)i0 * sx + (size_t)i1 * sy + i2;
+  const double c00 = (1 - t2) * b[0] + t2 * b[1];
+  const double c01 = (1 - t2) * b[sy] + t2 * b[sy + 1];
+  const double c10 = (1 - t2) * b[sx] + t2 * b[sx + 1];
+  const double c11 = (1 - t2) * b[sx + sy] + t2 * b[sx + sy + 1];
+  const double c0 = (1 - t1) * c00 + t1 * c01, c1 = (1 - t1) * c10 + t1 * c11;
+  return fmax((1 - t0) * c0 + t0 * c1, 0.0);
+}
+CLR_COLD __device__ double sediment_scale(const DevParams& P, int pidx, float x, float y, float z) {
+  const int tm = P.sed_model;
+  if (tm == CLR_SED_UNIFORM) return 1.0;
+  if (tm == CLR_SED_GRID_PER_BIN)
+    return rz_sample(P.field + (size_t)pidx * P.fn0 * P.fn1, P.fn0, P.fn1, x, y, z, P.rin, P.zmin, P.field_zmax);
+  if (tm == CLR_SED_RZ_FIELD) return rz_sample(P.field, P.fn0, P.fn1, x, y, z, P.rin, P.zmin, P.field_zmax);
+  if (tm == CLR_SED_XYZ_FIELD)
+    return xyz_sample(P.field, P.fn0, P.fn1, P.fn2, x, y, z, P.field_half, P.zmin, P.field_zmax);
+  double rr2 = (double)x * (double)x + (double)y * (double)y;
+  const double R2 = (double)P.rin * (double)P.rin;
+  if (rr2 > R2) rr2 = R2;
+  double rl;
+  if (tm == CLR_SED_STOKES) {
+    if (P.fs_model == 1) rl = P.sedA[pidx] * (rr2 - R2);
+    else {
+      const double a2 = (double)P.core_r * (double)P.core_r;
+      rl = P.sedA[pidx] * (rr2 / (a2 + rr2) - R2 / (a2 + R2));
+    }
+  } else {
+    const int nr = P.n_radial;
+    const double u = (P.rin > 0.0f) ? sqrt(fmax(rr2, 0.0)) / (double)P.rin * (double)(nr - 1) : 0.0;
+    int j = (int)floor(u);
+    j = min(max(j, 0), nr - 2);
+    const double t = u - (double)j;
+    const double* L = P.radlog + (size_t)pidx * nr;
+    rl = fmin((1.0 - t) * L[j] + t * L[j + 1], 0.0);
+  }
+  const double h = fmax((double)z - (double)P.zmin, 0.0), ns = P.sednorm[pidx];
+  if (!(ns > 0.0)) return 0.0;
+  return exp(rl - P.sedB[pidx] * h) / ns;
+}
+__device__ __forceinline__ int sample_cdf_index(Rng& r, const float* __restrict__ cdf, int n) {
+  if (n <= 1) return 0;
+  const float u = U(r);
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+__device__ int select_bin(Rng& r, const DevParams& P, bool wave, float x, float y, float z,
+                          float local_channel) {
+  const int tm = P.sed_model;
+  if (tm == CLR_SED_UNIFORM || tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD)
+    return sample_cdf_index(r, wave ? P.wcdf : P.gcdf, P.n_bins);
+  if (!((double)local_channel > 0.0)) return 0;
+  const double target = (double)U(r) * (double)local_channel;
+  const double* rate = wave ? P.trace_w : P.trace_g;
+  double cum = 0.0;
+  for (int i = 0; i < P.n_bins; ++i) {
+    cum += rate[i] * sediment_scale(P, i, x, y, z);
+    if (target <= cum) return i;
+  }
+  return P.n_bins > 0 ? P.n_bins - 1 : 0;
+}
+
+// --------------------------------------------------------- particle events --
+__device__ void sphere_sample(Rng& r, float vx, float vy, float vz, float& gx, float& gy, float& gz) {
+  const float rho = sqrtf(fminf(fmaxf(U(r), 0.0f), 0.99999994f));
+  const float ci = sqrtf(fmaxf(0.0f, 1.0f - rho * rho));
+  const float phi = U(r) * 2.0f * kPiF;
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(vx, vy, vz, e1x, e1y, e1z, e2x, e2y, e2z);
+  const float cp = cosf(phi), sp = sinf(phi);
+  gx = -ci * vx + rho * (cp * e1x + sp * e2x);
+  gy = -ci * vy + rho * (cp * e1y + sp * e2y);
+  gz = -ci * vz + rho * (cp * e1z + sp * e2z);
+  normalize3(gx, gy, gz);
+}
+__device__ bool sphere_accessible(const DevParams& P, double cx, double cy, double cz, double rr) {
+  const double rc = hypot(cx, cy);
+  const double wall = ((double)P.rin - rc) - rr, bottom = cz - rr - (double)P.zmin;
+  const double rmin = fmax(0.0, rc - rr);
+  const double top = free_surface_z_d(P, rmin * rmin) - (cz + rr);
+  const double scale = fmax(fmax(fabs((double)P.rin), fabs(cz)), fmax(rr, 1.0e-6));
+  const double tol = 128.0 * kDEps * scale;
+  return wall >= -tol && bottom >= -tol && top >= -tol;
+}
+// smooth limit of the microsurface walk, CLARITAS.py:2323-2356
+__device__ bool smooth_interface(Rng& r, float bx, float by, float bz, float n_top, float k_top,
+                                 float n_bot, float k_bot, float& dx, float& dy, float& dz,
+                                 bool& out_top, int& fc) {
+  fc = 0;
+  normalize3(bx, by, bz);
+  normalize3(dx, dy, dz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(dx) || !isfinite(dy) || !isfinite(dz)) {
+    fc = MF_NONFINITE_DIRECTION; return false;
+  }
+  if (dot3(dx, dy, dz, bx, by, bz) >= -1.0e-7f) { fc = MF_SIDE_STATE; return false; }
+  const float ci = fminf(fmaxf(dot3(-dx, -dy, -dz, bx, by, bz), 0.0f), 1.0f);
+  float R, ts, tn;
+  if (!complex_interface(ci, n_top, k_top, n_bot, k_bot, R, ts, tn) || !isfinite(R)) {
+    fc = MF_FRESNEL_INVALID; return false;
+  }
+  float ox = dx, oy = dy, oz = dz;
+  if (U(r) < R) {
+    reflect3(ox, oy, oz, bx, by, bz);
+    out_top = true;
+    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_REFLECTION_INVALID; return false; }
+  } else {
+    if (!energy_refract3(dx, dy, dz, bx, by, bz, n_top, k_top, n_bot, k_bot, ox, oy, oz)) {
+      fc = MF_REFRACTION_INVALID; return false;
+    }
+    out_top = false;
+  }
+  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_NONFINITE_DIRECTION; return false; }
+  dx = ox; dy = oy; dz = oz;
+  return true;
+}
+
+struct PCount {
+  int pr, per, pir, pabs;
+  uint32_t n_abs;
+};
+// smooth Fresnel sphere with internal Beer-Lambert chords, CLARITAS.py:2497-2599.
+// Returns 1 escape, 2 absorbed, 0 failure.
+CLR_COLD __device__ int sphere_interaction(const DevParams& P, Rng& r, float n_m, float n_p, float k_p,
+                                  float radius, float gx, float gy, float gz, double cx, double cy,
+                                  double cz, float& x, float& y, float& z, float& vx, float& vy,
+                                  float& vz, PCount& pc, int& pf, int& pfs) {
+  pf = PF_OK; pfs = SUB_NONE;
+  if (radius <= 0.0f) return 1;
+  bool top = true;
+  int fc = 0;
+  if (!smooth_interface(r, gx, gy, gz, n_m, 0.0f, n_p, k_p, vx, vy, vz, top, fc)) {
+    pf = PF_MICRO_NUMERICAL; pfs = fc; return 0;
+  }
+  if (top) { pc.pr++; pc.per++; return 1; }
+  if (dot3(vx, vy, vz, gx, gy, gz) >= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_ENTRY_SIDE; return 0; }
+  const double rr = (double)radius, rr2 = rr * rr;
+  double qx = rr * (double)gx, qy = rr * (double)gy, qz = rr * (double)gz;
+  const double qn0 = sqrt(qx * qx + qy * qy + qz * qz);
+  if (!(qn0 > 0.0) || !isfinite(qn0)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+  qx *= rr / qn0; qy *= rr / qn0; qz *= rr / qn0;
+  const float alpha_abs = (P.wavelength > 0.0f && k_p > 0.0f) ? (4.0f * kPiF * k_p / P.wavelength) : 0.0f;
+  for (int bounce = 0; bounce <= P.max_pb; ++bounce) {
+    const double dvx = vx, dvy = vy, dvz = vz;
+    const double A = dvx * dvx + dvy * dvy + dvz * dvz, q2 = qx * qx + qy * qy + qz * qz;
+    if (!(A > 0.0) || !isfinite(A) || !isfinite(q2)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+    const double qnorm = sqrt(q2);
+    if (!(qnorm > 0.0) || !isfinite(qnorm)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+    const double qp = rr / qnorm;
+    qx *= qp; qy *= qp; qz *= qp;
+    const double B = qx * dvx + qy * dvy + qz * dvz;
+    if (!(B < 0.0) || !isfinite(B)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
+    const double chord = -2.0 * B / A;
+    if (!(chord > 0.0) || !isfinite(chord)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+    const double tqx = qx + chord * dvx, tqy = qy + chord * dvy, tqz = qz + chord * dvz;
+    const double arr = tqx * dvx + tqy * dvy + tqz * dvz, tq2 = tqx * tqx + tqy * tqy + tqz * tqz;
+    if (!(arr > 0.0) || !isfinite(arr) || !isfinite(tq2) ||
+        fabs(tq2 - rr2) > 512.0 * kDEps * fmax(rr2, 1.0e-30)) {
+      pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0;
+    }
+    if (alpha_abs > 0.0f) {
+      pc.n_abs++;
+      const double dist = -log(Uopen(r)) / (double)alpha_abs;
+      if (dist < chord) {
+        qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
+        x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
+        pc.pabs++;
+        return 2;
+      }
+    }
+    qx = tqx; qy = tqy; qz = tqz;
+    const double qn = sqrt(qx * qx + qy * qy + qz * qz);
+    if (!(qn > 0.0) || !isfinite(qn)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+    const double pj = rr / qn;
+    qx *= pj; qy *= pj; qz *= pj;
+    x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
+    float nox = (float)(qx / rr), noy = (float)(qy / rr), noz = (float)(qz / rr);
+    normalize3(nox, noy, noz);
+    top = true;
+    if (!smooth_interface(r, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, top, fc)) {
+      pf = PF_MICRO_NUMERICAL; pfs = fc; return 0;
+    }
+    if (!top) {
+      if (dot3(vx, vy, vz, nox, noy, noz) <= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_EXIT_SIDE; return 0; }
+      return 1;
+    }
+    if (dot3(vx, vy, vz, nox, noy, noz) >= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
+    pc.pr++; pc.pir++;
+    if (bounce == P.max_pb) { pf = PF_MACRO_BOUNCE; pfs = SUB_SPHERE_BOUNCE_LIMIT; return 0; }
+  }
+  pf = PF_OTHER; pfs = SUB_NONE;
+  return 0;
+}
+// inverse-CDF polar angle on the float table, CLARITAS.py:2632-2641
+__device__ __forceinline__ float sample_mie_theta(Rng& r, const DevParams& P, int pidx) {
+  const int nt = P.n_theta;
+  const float* __restrict__ cdf = P.cdf + (size_t)pidx * nt;
+  const float u = U(r);
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  if (lo <= 0) return P.theta[0];
+  const float c0 = cdf[lo - 1], c1 = cdf[lo], t0 = P.theta[lo - 1], t1 = P.theta[lo];
+  const float f = (c1 > c0) ? fminf(fmaxf((u - c0) / (c1 - c0), 0.0f), 1.0f) : 0.0f;
+  return t0 + f * (t1 - t0);
+}
+__device__ void mie_scatter(Rng& r, const DevParams& P, int pidx, float& vx, float& vy, float& vz) {
+  const float th = sample_mie_theta(r, P, pidx);
+  const float phi = 2.0f * kPiF * U(r);
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(vx, vy, vz, e1x, e1y, e1z, e2x, e2y, e2z);
+  const float st = sinf(th), ct = cosf(th), cp = cosf(phi), sp = sinf(phi);
+  const float ox = ct * vx + st * (cp * e1x + sp * e2x);
+  const float oy = ct * vy + st * (cp * e1y + sp * e2y);
+  const float oz = ct * vz + st * (cp * e1z + sp * e2z);
+  vx = ox; vy = oy; vz = oz;
+  normalize3(vx, vy, vz);
+}
+
+struct CondOut {
+  float x, y, z, vx, vy, vz;
+  int status, channel, failure, subcode;
+  PCount pc;
+  uint32_t trials, rejects, inacc;
+};
+// conditional finite-particle closure, CLARITAS.py:4479-4536
+CLR_COLD __device__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
+                                     float vx, float vy, float vz, int max_trials) {
+  CondOut h;
+  h.x = x; h.y = y; h.z = z; h.vx = vx; h.vy = vy; h.vz = vz;
+  h.status = 0; h.channel = 0; h.failure = PF_OTHER; h.subcode = 9002;
+  h.pc = PCount{0, 0, 0, 0, 0u};
+  h.trials = h.rejects = h.inacc = 0;
+  const double qs = P.qsca[pidx], qe = P.qext[pidx];
+  const float ki = P.k_imag[pidx], nr = P.n_real[pidx], radius = P.radius[pidx];
+  const double tol = 1e-10 * fmax(1.0, fmax(fabs(qs), fabs(qe)));
+  if (max_trials == 0) { h.subcode = 9001; return h; }
+  if (!isfinite(qs) || !isfinite(qe) || qe <= 0.0 || qs < -tol || qs > qe + tol || max_trials < 1) return h;
+  const bool lossless = (ki == 0.0f && fabs(qe - qs) <= tol);
+  const double albedo = lossless ? 1.0 : fmin(fmax(qs / qe, 0.0), 1.0);
+  const double pabs = lossless ? 0.0 : fmin(fmax((qe - qs) / qe, 0.0), 1.0);
+  if ((ki > 0.0f && (pabs <= 0.0 || albedo >= 1.0)) || (albedo > 0.0 && albedo < 2.3283064365386962890625e-10) ||
+      (pabs > 0.0 && pabs < 2.3283064365386962890625e-10)) {
+    h.subcode = 9003; return h;
+  }
+  const bool target_abs = (albedo < 1.0 && U53(r) >= albedo);
+  const double wave = qs > 0.0 ? fmax(qs - 1.0, 0.0) / qs : 0.0;
+  uint32_t trials = 0, rejects = 0, inacc = 0, nabs = 0;
+  for (int trial = 0; trial < max_trials; ++trial) {
+    trials++;
+    CondOut c;
+    c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
+    c.pc = PCount{0, 0, 0, 0, 0u};
+    c.failure = PF_OK; c.subcode = 0; c.status = 0;
+    c.channel = (U53(r) < wave) ? ((P.coherent >= 1.0f || U53(r) < (double)P.coherent) ? 1 : 2) : 0;
+    if (c.channel == 1) {
+      mie_scatter(r, P, pidx, c.vx, c.vy, c.vz);
+      c.status = 1;
+    } else if (c.channel == 2) {
+      c.status = 0; c.failure = PF_SPHEROID;
+    } else {
+      float gx, gy, gz;
+      sphere_sample(r, vx, vy, vz, gx, gy, gz);
+      const double dcx = (double)x - (double)radius * gx, dcy = (double)y - (double)radius * gy,
+                   dcz = (double)z - (double)radius * gz;
+      if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { inacc++; continue; }
+      int pf = 0, pfs = 0;
+      c.status = sphere_interaction(P, r, P.n_water, nr, ki, radius, gx, gy, gz, dcx, dcy, dcz, c.x, c.y,
+                                    c.z, c.vx, c.vy, c.vz, c.pc, pf, pfs);
+      c.failure = pf; c.subcode = pfs;
+    }
+    nabs += c.pc.n_abs;
+    if (c.status != 1 && c.status != 2) {
+      c.trials = trials; c.rejects = rejects; c.inacc = inacc; c.pc.n_abs = nabs;
+      return c;
+    }
+    if ((c.status == 2) != target_abs) { rejects++; continue; }
+    c.trials = trials; c.rejects = rejects; c.inacc = inacc; c.pc.n_abs = nabs;
+    c.failure = PF_OK; c.subcode = 0;
+    return c;
+  }
+  h.status = 0; h.failure = PF_OTHER; h.subcode = 9001;
+  h.trials = trials; h.rejects = rejects; h.inacc = inacc; h.pc.n_abs = nabs;
+  return h;
+}
+
+// --------------------------------------------------------------- detectors --
+// three-plane hard aperture of one bore, CLARITAS.py:4237-4255
+__device__ __forceinline__ bool bore_accept(const DevParams& P, float x, float y, float z, float vx,
+                                            float vy, float vz, float ux, float uy, float& t_out) {
+  const float den = vx * ux + vy * uy;
+  if (den <= 1e-10f) return false;
+  const float pdot = x * ux + y * uy;
+  const float t0 = (P.ring_in - pdot) / den, t1 = (P.throat_out - pdot) / den, t2 = (P.ring_out - pdot) / den;
+  if (!(t0 > 0.0f && t1 > t0 && t2 > t1)) return false;
+  const float rth2 = P.throat_r * P.throat_r, rcb2 = P.cb_r * P.cb_r;
+  const float q0x = x + t0 * vx - P.ring_in * ux, q0y = y + t0 * vy - P.ring_in * uy, q0z = z + t0 * vz;
+  const float q1x = x + t1 * vx - P.throat_out * ux, q1y = y + t1 * vy - P.throat_out * uy, q1z = z + t1 * vz;
+  const float q2x = x + t2 * vx - P.ring_out * ux, q2y = y + t2 * vy - P.ring_out * uy, q2z = z + t2 * vz;
+  if (q0x * q0x + q0y * q0y + q0z * q0z <= rth2 && q1x * q1x + q1y * q1y + q1z * q1z <= rth2 &&
+      q2x * q2x + q2y * q2y + q2z * q2z <= rcb2) {
+    t_out = t0;
+    return true;
+  }
+  return false;
+}
+CLR_COLD __device__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy,
+                                 float vz, bool mirror) {
+  if (mirror) { x = -x; vx = -vx; }
+  float best_t = kInfF, t;
+  int best = -1;
+  for (int j = 0; j < CLR_N_CHANNELS; ++j)
+    if (bore_accept(P, x, y, z, vx, vy, vz, P.det_ux[j], P.det_uy[j], t) && t < best_t) { best_t = t; best = j; }
+  return best;
+}
+
+// ------------------------------------------------------------ lane state --
+enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4 };
+// who entered the acrylic / headspace (maps their return codes)
+enum : int { A_LAUNCH = 0, A_WATER = 1, A_BOTTOM = 2, A_HEAD = 3 };
+enum : int { H_LAUNCH = 0, H_WATERWALL = 1, H_TOP = 2, H_BOTTOM = 3 };
+enum : unsigned { F_ENTERED = 1u, F_PER = 2u, F_PIR = 4u, F_PABS = 8u, F_CTIR = 16u, F_TTIR = 32u,
+                  F_BTIR = 64u };
+
+struct Lane {
+  float x, y, z, vx, vy, vz;
+  Rng r;
+  int region, actx, hctx, it, hit, stuck, last_surf, wstep;
+  int st, pf;
+  int ic, gic, wic, wcoh, pr, cir, cor, tr, bir, bor;
+  unsigned flags;
+  int group;
+  uint64_t hidx;
+};
+// per-lane work counters accumulated over all histories of a launch
+struct Acc {
+  uint32_t n_flight, n_null, n_iface, n_abs, steps, draws, trials, rejects, inacc;
+};
+
+__device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
+  L.st = st;
+  L.pf = pf;
+  L.region = R_DONE;
+}
+
+// launch + hard apertures + air->acrylic entry, CLARITAS.py:4650-4735
+CLR_COLD __device__ void launch_history(Lane& L, const DevParams& P, Acc& A) {
+  Rng& r = L.r;
+  float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
+  if (P.src_model == CLR_SOURCE_COLLIMATED_DISK) {
+    const float u1 = U(r), u2 = U(r);
+    const float rho = P.beam_r * sqrtf(u1), ang = 2.0f * kPiF * u2;
+    x = rho * cosf(ang);
+    z = rho * sinf(ang);
+  } else {
+    const float u1 = fmaxf(U(r), 1e-12f), u2 = U(r);
+    const float mag = sqrtf(-2.0f * logf(u1));
+    x = P.sigma * mag * cosf(2.0f * kPiF * u2);
+    z = P.sigma * mag * sinf(2.0f * kPiF * u2);
+  }
+  y = -P.source_r;
+  if (P.src_model == CLR_SOURCE_LEGACY_POLAR || P.src_model == CLR_SOURCE_BETA_RADIANCE) {
+    const float polar = (P.src_model == CLR_SOURCE_LEGACY_POLAR)
+                            ? beta_draw(r, P.alpha1, P.alpha2) * (kPiF / 2.0f)
+                            : beta_radiance_polar(r, P.alpha1, P.alpha2);
+    const float az = 2.0f * kPiF * U(r), sp = sinf(polar);
+    vx = sp * cosf(az); vy = cosf(polar); vz = sp * sinf(az);
+  } else if (P.src_model == CLR_SOURCE_THESIS_PLANAR) {
+    float h = beta_draw(r, P.alpha1, P.alpha2) * (kPiF / 2.0f);
+    if (U(r) < 0.5f) h = -h;
+    vx = sinf(h); vy = cosf(h); vz = 0.0f;
+  }
+  normalize3(vx, vy, vz);
+  L.x = x; L.y = y; L.z = z; L.vx = vx; L.vy = vy; L.vz = vz;
+  if (vy <= 1e-12f) { terminate(L, 3); return; }
+  if (x * x + z * z > P.cb_r * P.cb_r) { terminate(L, 3); return; }
+  float t = (-P.throat_out - y) / vy;
+  if (t < -kEpsF) { terminate(L, 3); return; }
+  if (t < 0.0f) t = 0.0f;
+  const float xt = x + t * vx, zt = z + t * vz;
+  if (xt * xt + zt * zt > P.throat_r * P.throat_r) { terminate(L, 3); return; }
+  t = (-P.ring_in - y) / vy;
+  if (t <= 0.0f) { terminate(L, 3); return; }
+  x += t * vx; y += t * vy; z += t * vz;
+  if (x * x + z * z > P.throat_r * P.throat_r) { L.x = x; L.y = y; L.z = z; terminate(L, 3); return; }
+  const double td = finite_cylinder_forward_d(P, x, y, z, vx, vy, vz, P.rout, P.zlow, P.zwall_top);
+  const float tw = (td < 0.5 * (double)kInfF) ? (float)td : kInfF;
+  if (tw >= kInfF * 0.5f) { L.x = x; L.y = y; L.z = z; terminate(L, 3); return; }
+  x += tw * vx; y += tw * vy; z += tw * vz;
+  float nx, ny;
+  radial_normal(x, y, nx, ny);
+  bool tir;
+  A.n_iface++;
+  L.z = z;
+  if (!fresnel_transmit(r, vx, vy, vz, nx, ny, 0.0f, P.n_air, P.n_acrylic, tir)) {
+    L.cor++;
+    if (tir) L.flags |= F_CTIR;
+    L.x = x + kEpsF * nx; L.y = y + kEpsF * ny;
+    L.vx = vx; L.vy = vy; L.vz = vz;
+    terminate(L, 1);
+    return;
+  }
+  L.vx = vx; L.vy = vy; L.vz = vz;
+  L.x = x - kEpsF * nx; L.y = y - kEpsF * ny;
+  L.region = R_ACRYLIC; L.actx = A_LAUNCH; L.it = 0; L.stuck = 0; L.last_surf = -1;
+}
+
+// headspace return code -> next state (callers differ, CLARITAS.py:4720-4731,
+// 4995-5002, 5024-5031, 5051-5059)
+__device__ void headspace_return(Lane& L, int hw) {
+  if (hw == 1) { L.region = R_WATER; L.flags |= F_ENTERED; return; }
+  if (hw == 2) { terminate(L, 1); return; }
+  if (hw == 3) { terminate(L, 7); return; }
+  if (hw == 4) { terminate(L, 6); return; }
+  if (hw == 5 && (L.hctx == H_LAUNCH || L.hctx == H_WATERWALL)) { terminate(L, 8); return; }
+  terminate(L, 4, PF_HEADSPACE_NO_HIT);
+}
+// acrylic return code -> next state
+__device__ void acrylic_return(Lane& L, int where, int sub) {
+  if (L.actx == A_HEAD) {
+    if (where == 4) { L.region = R_HEADSPACE; return; }  // same headspace walk continues
+    headspace_return(L, where);
+    return;
+  }
+  if (where == 1) { L.region = R_WATER; L.flags |= F_ENTERED; return; }
+  if (where == 2) { terminate(L, 1); return; }
+  if (where == 3) { terminate(L, 7); return; }
+  if (where == 5) { terminate(L, 8); return; }
+  if (where == 4) {
+    L.region = R_HEADSPACE;
+    L.hit = 0;
+    L.hctx = (L.actx == A_LAUNCH) ? H_LAUNCH : (L.actx == A_WATER) ? H_WATERWALL : H_BOTTOM;
+    return;
+  }
+  int pf;
+  if (L.actx == A_BOTTOM) pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_BOTTOM_GEOMETRY;
+  else pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_ACRYLIC_NO_HIT;
+  terminate(L, 4, pf);
+}
+__device__ __forceinline__ void enter_acrylic(Lane& L, int ctx) {
+  L.region = R_ACRYLIC; L.actx = ctx; L.it = 0; L.stuck = 0; L.last_surf = -1;
+}
+
+// one surface interaction inside the acrylic union, CLARITAS.py:3140-3198
+__device__ void acrylic_step(Lane& L, const DevParams& P, Acc& A) {
+  if (L.it >= kEmergencyCap) {
+    if (acrylic_trapped(P, L.x, L.y, L.vx, L.vy, L.vz)) acrylic_return(L, 5, SUB_NONE);
+    else acrylic_return(L, 0, SUB_ACRYLIC_BOUNCE);
+    return;
+  }
+  L.it++;
+  double td = 1.0e300;
+  const int s = acrylic_next_surface(P, L.x, L.y, L.z, L.vx, L.vy, L.vz, td);
+  if (!s || !(td > 0.0) || !isfinite(td) || td >= 1.0e299) { acrylic_return(L, 0, SUB_ACRYLIC_NO_HIT); return; }
+  const float t = (float)td;
+  const float px = L.x, py = L.y, pz = L.z, pvx = L.vx, pvy = L.vy, pvz = L.vz;
+  L.x += t * L.vx; L.y += t * L.vy; L.z += t * L.vz;
+  bool tir = false;
+  A.n_iface++;
+  if (s == 1 || s == 2) {
+    float nx, ny;
+    radial_normal(L.x, L.y, nx, ny);
+    if (s == 1) {
+      const float zs = free_surface_z(P, L.x, L.y);
+      const float n_in = (L.z <= zs + kEpsF) ? P.n_water : P.n_air;
+      if (fresnel_transmit(L.r, L.vx, L.vy, L.vz, nx, ny, 0.0f, P.n_acrylic, n_in, tir)) {
+        // normal convention of the reference: +radial incident side here
+        L.x -= kEpsF * nx; L.y -= kEpsF * ny;
+        acrylic_return(L, (n_in == P.n_water) ? 1 : 4, SUB_NONE);
+        return;
+      }
+      L.cir++;
+      if (tir) L.flags |= F_CTIR;
+      L.x += kEpsF * nx; L.y += kEpsF * ny;
+    } else {
+      if (fresnel_transmit(L.r, L.vx, L.vy, L.vz, -nx, -ny, 0.0f, P.n_acrylic, P.n_air, tir)) {
+        L.x += kEpsF * nx; L.y += kEpsF * ny;
+        acrylic_return(L, 2, SUB_NONE);
+        return;
+      }
+      L.cor++;
+      if (tir) L.flags |= F_CTIR;
+      L.x -= kEpsF * nx; L.y -= kEpsF * ny;
+    }
+  } else {
+    // planes: 3 bottom external face, 4 annular top face, 5 water aperture
+    const float zplane = (s == 3) ? P.zlow : (s == 4) ? P.zwall_top : P.zmin;
+    const float nz = (s == 3) ? 1.0f : -1.0f;  // normal into the incident acrylic
+    const float n2 = (s == 3) ? P.n_bottom_ext : (s == 4) ? P.n_air : P.n_water;
+    L.z = zplane;
+    // the reference reflects about +z for all three planes (CLARITAS.py:3175-3189)
+    const float ci = fminf(fmaxf((s == 3) ? -L.vz : L.vz, 0.0f), 1.0f);
+    const float R = fresnel_R(ci, P.n_acrylic, n2, tir);
+    float a, b, c;
+    if (!tir && !(U(L.r) < R) && refract3(L.vx, L.vy, L.vz, 0.0f, 0.0f, nz, P.n_acrylic, n2, a, b, c)) {
+      L.vx = a; L.vy = b; L.vz = c;
+      L.z = (s == 3) ? P.zlow - kEpsF : (s == 4) ? P.zwall_top + kEpsF : P.zmin + kEpsF;
+      acrylic_return(L, (s == 3) ? 3 : (s == 4) ? 2 : 1, SUB_NONE);
+      return;
+    }
+    reflect3(L.vx, L.vy, L.vz, 0.0f, 0.0f, 1.0f);
+    if (s == 4) { L.cor++; if (tir) L.flags |= F_CTIR; }
+    else if (s == 3) { L.bor++; if (tir) L.flags |= F_BTIR; }
+    else { L.bir++; if (tir) L.flags |= F_BTIR; }
+    L.z = (s == 3) ? P.zlow + kEpsF : (s == 4) ? P.zwall_top - kEpsF : P.zmin - kEpsF;
+  }
+  const float pd = fabsf(L.x - px) + fabsf(L.y - py) + fabsf(L.z - pz);
+  const float vd = fabsf(L.vx - pvx) + fabsf(L.vy - pvy) + fabsf(L.vz - pvz);
+  L.stuck = (s == L.last_surf && pd <= 1.0e-12f && vd <= 1.0e-7f) ? L.stuck + 1 : 0;
+  if (L.stuck >= 8) { acrylic_return(L, 0, SUB_ACRYLIC_BOUNCE); return; }
+  L.last_surf = s;
+}
+
+// one headspace segment, CLARITAS.py:3235-3293
+CLR_COLD __device__ void headspace_step(Lane& L, const DevParams& P, Acc& A) {
+  if (L.hit >= kEmergencyCap) { headspace_return(L, 0); return; }
+  L.hit++;
+  if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z) || !isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) {
+    headspace_return(L, 0); return;
+  }
+  const double tol = support_tol(P, P.rin, P.rout, P.zmin, P.zlow, P.zwall_top);
+  const float zs = free_surface_z(P, L.x, L.y);
+  const double rr = hypot((double)L.x, (double)L.y);
+  if ((double)L.z <= (double)zs - tol && rr < (double)P.rin - tol) { headspace_return(L, 1); return; }
+  if ((double)L.z > (double)P.zwall_top + tol) { headspace_return(L, 4); return; }
+  const double tside = bore_wall_hit_d(P, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.rin, P.zmin, P.zwall_top);
+  if (tside < 0.0) { headspace_return(L, 0); return; }
+  const double tsurf = (double)free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
+  double ttop = 1.0e300;
+  if (L.vz > 1.0e-14f) {
+    const double t = ((double)P.zwall_top - (double)L.z) / (double)L.vz;
+    if (t > 0.0) {
+      const double X = (double)L.x + t * (double)L.vx, Y = (double)L.y + t * (double)L.vy;
+      if (hypot(X, Y) < (double)P.rin + tol) ttop = t;
+    }
+  }
+  int which = 0;
+  double best = 1.0e300;
+  if (tsurf > 0.0 && tsurf < best) { best = tsurf; which = 1; }
+  if (tside >= 0.0 && tside < best) { best = tside; which = 2; }
+  if (ttop > 0.0 && ttop < best) { best = ttop; which = 3; }
+  if (!which || !isfinite(best) || best >= 1.0e299) { headspace_return(L, 0); return; }
+  const float t = (float)best;
+  L.x += t * L.vx; L.y += t * L.vy; L.z += t * L.vz;
+  if (which == 3) { L.z = P.zwall_top + kEpsF; headspace_return(L, 4); return; }
+  bool tir = false;
+  A.n_iface++;
+  if (which == 1) {
+    float snx, sny, snz;
+    free_surface_normal(P, L.x, L.y, snx, sny, snz);
+    if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, -snx, -sny, -snz, P.n_air, P.n_water, tir)) {
+      L.tr++;
+      if (tir) L.flags |= F_TTIR;
+      L.x -= kEpsF * snx; L.y -= kEpsF * sny; L.z -= kEpsF * snz;
+      return;
+    }
+    if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) { headspace_return(L, 0); return; }
+    headspace_return(L, 1);
+    return;
+  }
+  float nx, ny;
+  radial_normal(L.x, L.y, nx, ny);
+  if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, -nx, -ny, 0.0f, P.n_air, P.n_acrylic, tir)) {
+    L.cir++;
+    if (tir) L.flags |= F_CTIR;
+    L.x -= kEpsF * nx; L.y -= kEpsF * ny;
+    return;
+  }
+  L.x += kEpsF * nx; L.y += kEpsF * ny;
+  enter_acrylic(L, A_HEAD);
+}
+
+// one pass of the water loop, CLARITAS.py:4734-5066
+template <bool kPlain>
+__device__ void water_step(Lane& L, const DevParams& P, Acc& A) {
+  if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
+  L.wstep++;
+  A.steps++;
+  if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) { terminate(L, 4, PF_NONFINITE_POSITION); return; }
+  if (!isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) { terminate(L, 4, PF_NONFINITE_DIRECTION); return; }
+  const float zsh = free_surface_z(P, L.x, L.y);
+  if (L.z <= P.zmin || L.z >= zsh) { terminate(L, 2); return; }
+  const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
+  const double rr2 = (double)L.x * (double)L.x + (double)L.y * (double)L.y;
+  if (rr2 > ri2w + wtol) { terminate(L, 4, PF_INVALID_MEDIUM); return; }
+  const double tsd = bore_wall_hit_d(P, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.rin, P.zmin, P.zwall_top);
+  if (tsd < 0.0) { terminate(L, 4, PF_INVALID_MEDIUM); return; }
+  const float tside = (tsd < 0.5 * (double)kInfF) ? (float)tsd : kInfF;
+  float ttop = kInfF, tbot = kInfF;
+  if (L.vz < -1e-14f) tbot = (P.zmin - L.z) / L.vz;
+  {
+    // exact skip: the segment to the side/bottom stays below the surface's
+    // lowest point, so no free-surface hit can come first
+    const float tc = fminf(tside, (tbot > 0.0f) ? tbot : kInfF);
+    bool skip = false;
+    if (P.fs_model != 0 && tc < 0.5f * kInfF) skip = fmaxf(L.z, L.z + tc * L.vz) < P.zs_min;
+    if (!skip) ttop = free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
+  }
+  if (!(ttop > 0.0f)) ttop = kInfF;
+  if (!(tbot > 0.0f)) tbot = kInfF;
+  const bool axial_top = (ttop <= tbot && ttop < tside);
+  const bool axial_bot = (tbot < ttop && tbot < tside);
+  const float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
+  if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY); return; }
+  const float MU_TOTAL = P.mu_geom + P.mu_wave;
+  const float MU_TRACK = P.mu_track;
+  float freep = kInfF;
+  if (tbound > 0.0f && MU_TRACK > 0.0f) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
+  const bool do_particle = freep < tbound;
+  const float travel = do_particle ? freep : tbound;
+  A.n_flight++;
+  L.x += travel * L.vx; L.y += travel * L.vy; L.z += travel * L.vz;
+  if (do_particle) {
+    const double er2 = (double)L.x * L.x + (double)L.y * L.y;
+    if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > free_surface_z(P, L.x, L.y) + kEpsF) {
+      terminate(L, 4, PF_INVALID_MEDIUM);
+      return;
+    }
+    float LG = P.mu_geom, LW = P.mu_wave, LT = MU_TOTAL;
+    if (P.sed_model != CLR_SED_UNIFORM) {
+      double lg = 0.0, lw = 0.0;
+      if (kPlain || P.sed_model == CLR_SED_RZ_FIELD || P.sed_model == CLR_SED_XYZ_FIELD) {
+        const double s = sediment_scale(P, 0, L.x, L.y, L.z);
+        lg = P.sum_trace_g * s;
+        lw = P.sum_trace_w * s;
+      } else {
+        for (int i = 0; i < P.n_bins; ++i) {
+          const double sc = sediment_scale(P, i, L.x, L.y, L.z);
+          lg += P.trace_g[i] * sc;
+          lw += P.trace_w[i] * sc;
+        }
+      }
+      LG = (float)lg; LW = (float)lw; LT = LG + LW;
+      if (!(LT > 0.0f)) { A.n_null++; return; }
+      const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
+      if (LT > MU_TRACK + slack) { terminate(L, 4, PF_MAJORANT); return; }
+      const float pacc = fminf(LT / fmaxf(MU_TRACK, 1e-30f), 1.0f);
+      if (U(L.r) >= pacc) { A.n_null++; return; }
+    }
+    L.ic++;
+    const bool use_wave = (LW > 0.0f && U(L.r) * LT >= LG);
+    const int em = P.event_model;
+    if (kPlain || em == CLR_EVENT_PLAIN_MIE) {
+      const int b = select_bin(L.r, P, false, L.x, L.y, L.z, LG);
+      A.n_abs++;
+      if (U(L.r) >= P.albedo[b]) { L.flags |= F_PABS; terminate(L, 5); return; }
+      L.wic++; L.wcoh++;
+      mie_scatter(L.r, P, b, L.vx, L.vy, L.vz);
+      return;
+    }
+    if constexpr (!kPlain) {
+    const bool cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
+    int sel = -1;
+    if (cond) sel = select_bin(L.r, P, use_wave, L.x, L.y, L.z, use_wave ? LW : LG);
+    if (cond && P.k_imag[sel] > 0.0f) {
+      CondOut h = conditional_event(P, L.r, sel, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials);
+      A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+      if (h.status != 1 && h.status != 2) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
+      L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
+      L.pr += h.pc.pr;
+      if (h.pc.per) L.flags |= F_PER;
+      if (h.pc.pir) L.flags |= F_PIR;
+      if (h.pc.pabs) L.flags |= F_PABS;
+      if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+      if (h.status == 2) terminate(L, 5);
+      return;
+    }
+    if (use_wave) {
+      L.wic++;
+      const int pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
+      const bool coherent = (P.coherent >= 1.0f) || (U(L.r) < P.coherent);
+      if (!coherent) { terminate(L, 4, PF_SPHEROID); return; }
+      L.wcoh++;
+      mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
+      return;
+    }
+    L.gic++;
+    const int pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
+    const float rad = P.radius[pidx];
+    float gx, gy, gz;
+    sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
+    const double dcx = (double)L.x - (double)rad * (double)gx;
+    const double dcy = (double)L.y - (double)rad * (double)gy;
+    const double dcz = (double)L.z - (double)rad * (double)gz;
+    const bool accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
+    if (!accessible && cond) {
+      A.inacc++;
+      CondOut h = conditional_event(P, L.r, pidx, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials - 1);
+      A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+      if (h.status != 1) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
+      L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
+      L.pr += h.pc.pr;
+      if (h.pc.per) L.flags |= F_PER;
+      if (h.pc.pir) L.flags |= F_PIR;
+      if (h.pc.pabs) L.flags |= F_PABS;
+      L.gic--;
+      if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+      return;
+    }
+    if (!accessible) { L.ic--; L.gic--; A.n_null++; return; }
+    PCount pc{0, 0, 0, 0, 0u};
+    int ppf = 0, ppfs = 0;
+    const int ps = sphere_interaction(P, L.r, P.n_water, P.n_real[pidx], P.k_imag[pidx], rad, gx, gy, gz,
+                                      dcx, dcy, dcz, L.x, L.y, L.z, L.vx, L.vy, L.vz, pc, ppf, ppfs);
+    L.pr += pc.pr;
+    if (pc.per) L.flags |= F_PER;
+    if (pc.pir) L.flags |= F_PIR;
+    if (pc.pabs) L.flags |= F_PABS;
+    A.n_abs += pc.n_abs;
+    if (ps == 2) { terminate(L, 5); return; }
+    if (ps != 1) { terminate(L, 4, ppf ? ppf : PF_OTHER); return; }
+    }
+    return;
+  }
+  bool tir = false;
+  A.n_iface++;
+  if (axial_top) {
+    float snx = 0.0f, sny = 0.0f, snz = -1.0f;
+    free_surface_normal(P, L.x, L.y, snx, sny, snz);
+    if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, snx, sny, snz, P.n_water, P.n_air, tir)) {
+      L.tr++;
+      if (tir) L.flags |= F_TTIR;
+      if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) terminate(L, 4, PF_INVALID_MEDIUM);
+      return;
+    }
+    L.x -= kEpsF * snx; L.y -= kEpsF * sny; L.z -= kEpsF * snz;
+    L.region = R_HEADSPACE; L.hctx = H_TOP; L.hit = 0;
+    return;
+  }
+  if (axial_bot) {
+    // water -> bottom disc, CLARITAS.py:3211-3228
+    const double r2 = (double)L.x * (double)L.x + (double)L.y * (double)L.y;
+    if (!(r2 < (double)P.rin * (double)P.rin)) { terminate(L, 4, PF_BOTTOM_GEOMETRY); return; }
+    if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, 0.0f, 0.0f, 1.0f, P.n_water, P.n_acrylic, tir)) {
+      L.bir++;
+      if (tir) L.flags |= F_BTIR;
+      L.z = P.zmin + kEpsF;
+      return;
+    }
+    L.z = P.zmin - kEpsF;
+    enter_acrylic(L, A_BOTTOM);
+    return;
+  }
+  // water -> acrylic at the side wall, CLARITAS.py:5038-5065
+  float nx, ny;
+  radial_normal(L.x, L.y, nx, ny);
+  if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, -nx, -ny, 0.0f, P.n_water, P.n_acrylic, tir)) {
+    L.cir++;
+    if (tir) L.flags |= F_CTIR;
+    L.x -= kEpsF * nx; L.y -= kEpsF * ny;
+    return;
+  }
+  L.x += kEpsF * nx; L.y += kEpsF * ny;
+  enter_acrylic(L, A_WATER);
+}
+
+// ------------------------------------------------------------------ tallies --
+constexpr int kDetWords = CLR_MAX_GROUPS * 4 * CLR_N_CHANNELS;
+constexpr int kAuxWords = CLR_MAX_GROUPS * 2 * CLR_MAX_AUX;
+
+struct LdsTally {
+  uint32_t det[kDetWords];
+  uint32_t aux[kAuxWords];
+  uint32_t status[16];
+  uint32_t fail[32];
+  uint32_t ich[CLR_N_ICHIST];
+  float inc[2 * CLR_N_CHANNELS];
+  unsigned long long cnt[CLR_N_COUNTERS];
+};
+
+__device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
+  if (v) atomicAdd(p, (unsigned long long)v);
+}
+
+// score an ended history, CLARITAS.py:4397-4435 and the per-chunk
+// reductions of CLARITAS.py:8659-8733
+CLR_COLD __device__ void finish_history(Lane& L, const DevParams& P, LdsTally& T, clr_record* rec,
+                               uint64_t n_rec) {
+  const int g = L.group;
+  atomicAdd(&T.status[L.st], 1u);
+  if (L.st == 4) atomicAdd(&T.fail[L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER], 1u);
+  int nat = -1, mir = -1;
+  if (L.st == 1) {
+    const float ex = L.x, ey = L.y, ez = L.z, evx = L.vx, evy = L.vy, evz = L.vz;
+    atomicAdd(&T.cnt[CLR_C_N_EXIT], 1ull);
+    if (isfinite(ex) && isfinite(evx)) atomicAdd(&T.cnt[CLR_C_VALID_AIR], 1ull);
+    if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
+      nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
+      mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+      if (nat >= 0) {
+        atomicAdd(&T.det[(g * 4 + 0) * CLR_N_CHANNELS + nat], 1u);
+        const float c = fminf(fmaxf(evx * P.det_ux[nat] + evy * P.det_uy[nat], -1.0f), 1.0f);
+        atomicAdd(&T.inc[nat], acosf(c) * (180.0f / kPiF));
+      }
+      if (mir >= 0) {
+        atomicAdd(&T.det[(g * 4 + 1) * CLR_N_CHANNELS + mir], 1u);
+        const float c = fminf(fmaxf(-evx * P.det_ux[mir] + evy * P.det_uy[mir], -1.0f), 1.0f);
+        atomicAdd(&T.inc[CLR_N_CHANNELS + mir], acosf(c) * (180.0f / kPiF));
+      }
+      for (int a = 0; a < P.n_aux; ++a) {
+        float t;
+        if (bore_accept(P, ex, ey, ez, evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t))
+          atomicAdd(&T.aux[(g * 2 + 0) * CLR_MAX_AUX + a], 1u);
+        if (bore_accept(P, -ex, ey, ez, -evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t))
+          atomicAdd(&T.aux[(g * 2 + 1) * CLR_MAX_AUX + a], 1u);
+      }
+      // legacy exit-position windows
+      for (int m = 0; m < 2; ++m) {
+        const float xx = m ? -ex : ex;
+        float ang = atan2f(xx, ey) * (180.0f / kPiF);
+        if (ang < 0.0f) ang += 360.0f;
+        if (ang < 0.0f || ang > 180.0f) continue;
+        const int jmax = (L.ic <= 0) ? 9 : CLR_N_CHANNELS;
+        for (int j = 0; j < jmax; ++j)
+          if (fabsf(ang - 10.0f * j) <= P.legacy_acc) atomicAdd(&T.det[(g * 4 + 2 + m) * CLR_N_CHANNELS + j], 1u);
+      }
+    }
+  }
+  if (L.flags & F_ENTERED) {
+    atomicAdd(&T.cnt[CLR_C_ENTERED], 1ull);
+    lds_add64(&T.cnt[CLR_C_SUM_IC], (uint32_t)L.ic);
+    lds_add64(&T.cnt[CLR_C_SUM_GIC], (uint32_t)L.gic);
+    lds_add64(&T.cnt[CLR_C_SUM_WIC], (uint32_t)L.wic);
+    lds_add64(&T.cnt[CLR_C_SUM_WCOH], (uint32_t)L.wcoh);
+    if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
+    atomicAdd(&T.ich[L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1], 1u);
+    lds_add64(&T.cnt[CLR_C_SUM_PR], (uint32_t)L.pr);
+    if (L.flags & F_PER) atomicAdd(&T.cnt[CLR_C_RAYS_PER], 1ull);
+    if (L.flags & F_PIR) atomicAdd(&T.cnt[CLR_C_RAYS_PIR], 1ull);
+    if (L.flags & F_PABS) atomicAdd(&T.cnt[CLR_C_RAYS_PABS], 1ull);
+    lds_add64(&T.cnt[CLR_C_SUM_CIR], (uint32_t)L.cir);
+    lds_add64(&T.cnt[CLR_C_SUM_COR], (uint32_t)L.cor);
+    if (L.flags & F_CTIR) atomicAdd(&T.cnt[CLR_C_RAYS_CTIR], 1ull);
+    lds_add64(&T.cnt[CLR_C_SUM_TOPR], (uint32_t)L.tr);
+    if (L.flags & F_TTIR) atomicAdd(&T.cnt[CLR_C_RAYS_TOPTIR], 1ull);
+    lds_add64(&T.cnt[CLR_C_SUM_BIR], (uint32_t)L.bir);
+    lds_add64(&T.cnt[CLR_C_SUM_BOR], (uint32_t)L.bor);
+    if (L.flags & F_BTIR) atomicAdd(&T.cnt[CLR_C_RAYS_BTIR], 1ull);
+  }
+  lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
+  lds_add64(&T.cnt[CLR_C_RNG_DRAWS], L.r.draws);
+  if (rec && L.hidx < n_rec) {
+    clr_record q;
+    q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
+    q.status = L.st; q.native_id = nat; q.mirror_id = mir; q.interactions = L.ic;
+    q.failure = L.pf; q.draws = (int32_t)L.r.draws;
+    rec[L.hidx] = q;
+  }
+}
+
+__device__ void zero_lds(LdsTally& T) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(&T);
+  const int nw = (int)(sizeof(LdsTally) / 4);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) w[i] = 0u;
+}
+
+// flush LDS tallies of the block into the configuration's global tallies
+__device__ void flush_lds(LdsTally& T, clr_tallies* __restrict__ G) {
+  __syncthreads();
+  unsigned long long* gdet = reinterpret_cast<unsigned long long*>(&G->det[0][0][0]);
+  for (int i = threadIdx.x; i < kDetWords; i += blockDim.x)
+    if (T.det[i]) { atomicAdd(gdet + i, (unsigned long long)T.det[i]); T.det[i] = 0u; }
+  unsigned long long* gaux = reinterpret_cast<unsigned long long*>(&G->aux[0][0][0]);
+  for (int i = threadIdx.x; i < kAuxWords; i += blockDim.x)
+    if (T.aux[i]) { atomicAdd(gaux + i, (unsigned long long)T.aux[i]); T.aux[i] = 0u; }
+  for (int i = threadIdx.x; i < CLR_N_STATUS; i += blockDim.x)
+    if (T.status[i]) { atomicAdd((unsigned long long*)&G->status[i], (unsigned long long)T.status[i]); T.status[i] = 0u; }
+  for (int i = threadIdx.x; i < CLR_N_FAILURE; i += blockDim.x)
+    if (T.fail[i]) { atomicAdd((unsigned long long*)&G->failures[i], (unsigned long long)T.fail[i]); T.fail[i] = 0u; }
+  for (int i = threadIdx.x; i < CLR_N_ICHIST; i += blockDim.x)
+    if (T.ich[i]) { atomicAdd((unsigned long long*)&G->ic_hist[i], (unsigned long long)T.ich[i]); T.ich[i] = 0u; }
+  for (int i = threadIdx.x; i < 2 * CLR_N_CHANNELS; i += blockDim.x)
+    if (T.inc[i] != 0.0f) { atomicAdd(&G->incidence_sum[0][0] + i, (double)T.inc[i]); T.inc[i] = 0.0f; }
+  for (int i = threadIdx.x; i < CLR_N_COUNTERS; i += blockDim.x)
+    if (T.cnt[i]) { atomicAdd((unsigned long long*)&G->counters[i], T.cnt[i]); T.cnt[i] = 0ull; }
+  __syncthreads();
+}
+
+// fold the per-lane work counters into LDS (wave shuffles, then one LDS
+// atomic per wave and counter)
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ void fold_acc(Acc& A, LdsTally& T) {
+  const uint32_t vals[9] = {A.n_flight, A.n_null, A.n_iface, A.n_abs, A.steps, A.trials, A.rejects,
+                            A.inacc, 0u};
+  const int idx[8] = {CLR_C_N_FLIGHT, CLR_C_N_NULL, CLR_C_N_IFACE, CLR_C_N_ABS, CLR_C_N_STEPS,
+                      CLR_C_COND_TRIALS, CLR_C_COND_REJECT, CLR_C_COND_INACCESSIBLE};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t s = wave_sum(vals[k]);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&T.cnt[idx[k]], (unsigned long long)s);
+  }
+  A = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+constexpr int kMaxBlock = 1024;
+
+template <bool kPlain>
+__global__ __launch_bounds__(256) void transport_kernel(TransportLaunch Lp) {
+  __shared__ LdsTally T;
+  __shared__ int s_item;
+  __shared__ unsigned int s_next;
+  zero_lds(T);
+  int cur_cfg = -1;
+  Acc A{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int it = (int)atomicAdd(Lp.item_counter, 1u);
+      s_item = it;
+      s_next = 0u;
+      if (it < Lp.n_items) {
+        s_key[0] = (uint32_t)Lp.items[it].seed;
+        s_key[1] = (uint32_t)(Lp.items[it].seed >> 32);
+      }
+    }
+    __syncthreads();
+    const int item = s_item;
+    if (item >= Lp.n_items) break;
+    const WorkItem wi = Lp.items[item];
+    if (wi.config != cur_cfg) {
+      if (cur_cfg >= 0) {
+        fold_acc(A, T);
+        flush_lds(T, Lp.tallies + cur_cfg);
+      }
+      cur_cfg = wi.config;
+    }
+    const DevParams& P = Lp.params[wi.config];
+    const uint64_t G = (uint64_t)P.n_groups;
+    clr_record* rec = (wi.config == 0) ? Lp.records : nullptr;
+    Lane L;
+    bool alive = false, exhausted = false;
+    for (;;) {
+      if (!alive && !exhausted) {
+        const unsigned int k = atomicAdd(&s_next, 1u);
+        if ((uint64_t)k < wi.count) {
+          L.hidx = wi.start + k;
+          L.group = (int)((L.hidx * G) / wi.n_total);
+          rng_seed(L.r, wi.offset + L.hidx);
+          L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
+          L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
+          L.flags = 0u;
+          L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
+          launch_history(L, P, A);
+          alive = true;
+        } else {
+          exhausted = true;
+        }
+      }
+      if (!__any(alive)) break;
+      if (alive) {
+        if (L.region == R_WATER) water_step<kPlain>(L, P, A);
+        else if (L.region == R_ACRYLIC) acrylic_step(L, P, A);
+        else if (L.region == R_HEADSPACE) headspace_step(L, P, A);
+        if (L.region == R_DONE) {
+          finish_history(L, P, T, rec, Lp.n_records);
+          alive = false;
+        }
+      }
+    }
+  }
+  if (cur_cfg >= 0) {
+    fold_acc(A, T);
+    flush_lds(T, Lp.tallies + cur_cfg);
+  }
+}
+
+}  // namespace
+
+hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s) {
+  if (L.plain) hipLaunchKernelGGL(transport_kernel<true>, dim3(L.grid), dim3(L.block), 0, s, L);
+  else hipLaunchKernelGGL(transport_kernel<false>, dim3(L.grid), dim3(L.block), 0, s, L);
+  return hipGetLastError();
+}
+
+int clr_transport_occupancy(int block, int plain) {
+  int n = 0;
+  hipError_t e = plain ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true>, block, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<false>, block, 0);
+  if (e != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}

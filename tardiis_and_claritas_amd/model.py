@@ -1,0 +1,768 @@
+"""`ClaritasForwardModel.simulate(...) -> ForwardResult` on the HIP library.
+
+The constructor keeps the reference's keyword names and defaults
+(ClaritasForwardModel.__init__, CLARITAS.py:5577-5652) and its validation
+rules for the supported physics; `simulate` keeps the reference's signature
+(CLARITAS.py:7160-7179) and fills the tally/result layout of ForwardResult
+(CLARITAS.py:5343-5533) from device tallies.  All transport runs in
+libclaritas_hip.so on the GPU; this module only builds structures and does the
+closed-form host arithmetic the reference also does on the host
+(normalisation, jackknife, per-ray means).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from ._lib import Context
+from .physics import (DETECTOR_ANGLES_DEG, Material, reference_material, vortex_surface_geometry)
+
+CONDITIONAL_MAX_TRIALS = 16384  # CLARITAS.py:1117
+
+
+def _jackknife_normalized_se(batch_scores):
+    """Leave-one-batch-out SE of the normalised 18-channel response,
+    CLARITAS.py:5244-5261 (NaN with fewer than 3 usable batches)."""
+    nch = len(DETECTOR_ANGLES_DEG)
+    if len(batch_scores) < 3:
+        return np.full(nch, np.nan)
+    s = np.asarray(batch_scores, dtype=float)
+    total = s.sum(axis=0)
+    loo = []
+    for b in range(s.shape[0]):
+        q = total - s[b]
+        den = q.sum()
+        if den > 0:
+            loo.append(q / den)
+    if len(loo) < 3:
+        return np.full(nch, np.nan)
+    loo = np.asarray(loo)
+    m = loo.mean(axis=0)
+    B = loo.shape[0]
+    return np.sqrt((B - 1.0) / B * np.sum((loo - m) ** 2, axis=0))
+
+
+def _histogram_median(hist):
+    """Median of an integer histogram, CLARITAS.py:5228-5240."""
+    hist = np.asarray(hist, dtype=np.int64)
+    n = int(hist.sum())
+    if n <= 0:
+        return float('nan')
+    cs = np.cumsum(hist)
+    if n % 2:
+        return float(np.searchsorted(cs, n // 2 + 1))
+    a = int(np.searchsorted(cs, n // 2))
+    b = int(np.searchsorted(cs, n // 2 + 1))
+    return 0.5 * (a + b)
+
+
+def _norm(v):
+    s = float(np.sum(v))
+    return v / s if s > 0 else np.zeros_like(v, dtype=float)
+
+
+@dataclass
+class ForwardResult:
+    """Tally/result layout of the reference's ForwardResult (CLARITAS.py:5343-5533)
+    for the supported physics, plus the extension fields documented below."""
+
+    detector_angles_deg: np.ndarray
+    raw_hits: np.ndarray
+    normalized_response: np.ndarray
+    native_exact_hits: np.ndarray
+    mirror_exact_hits: np.ndarray
+    symmetry_exact_scores: np.ndarray
+    normalized_native_exact_response: np.ndarray
+    normalized_mirror_exact_response: np.ndarray
+    normalized_symmetry_exact_response: np.ndarray
+    normalized_response_jackknife_se: np.ndarray
+    hardware_native_hits: np.ndarray
+    hardware_mirror_hits: np.ndarray
+    hardware_symmetry_scores: np.ndarray
+    normalized_hardware_native_response: np.ndarray
+    normalized_hardware_mirror_response: np.ndarray
+    normalized_hardware_symmetry_response: np.ndarray
+    hardware_normalized_jackknife_se: np.ndarray
+    legacy_native_channel_counts: np.ndarray
+    legacy_mirror_channel_counts: np.ndarray
+    legacy_symmetry_channel_scores: np.ndarray
+    normalized_legacy_native_response: np.ndarray
+    normalized_legacy_mirror_response: np.ndarray
+    normalized_legacy_symmetry_response: np.ndarray
+    legacy_normalized_jackknife_se: np.ndarray
+    legacy_acceptance_deg: float
+    valid_exit_count: int
+    numerical_failure_count: int
+    numerical_failure_fraction: float
+    production_failure_count: int
+    production_failure_code_counts: dict
+    n_rays: int
+    requested_n_rays: int
+    stopped_adaptively: bool
+    stop_reason: str
+    particle_event_model: str
+    particle_surface_rms_slope_deg: float
+    wave_surface_rms_height_nm: float
+    wave_coherent_fraction_nominal: float
+    mu_geom_per_m: float
+    mu_wave_per_m: float
+    mu_event_per_m: float
+    mu_sca_per_m: float
+    mu_ext_per_m: float
+    geometric_optical_depth_diameter: float
+    optical_depth_diameter: float
+    mean_interactions: float
+    mean_geometric_interactions: float
+    mean_wave_interactions: float
+    mean_coherent_wave_interactions: float
+    median_interactions: float
+    ballistic_fraction: float
+    mean_fresnel_reflections: float
+    entry_reflection_fraction: float
+    internal_reflection_fraction: float
+    particle_absorption_fraction: float
+    particle_absorbed_ray_count: int
+    mean_cell_inner_reflections: float
+    mean_cell_outer_reflections: float
+    cell_tir_fraction: float
+    mean_top_surface_reflections: float
+    top_surface_tir_fraction: float
+    mean_top_surface_interactions: float
+    free_surface_model: str
+    vortex_wall_center_height_difference_mm: float
+    vortex_core_radius_mm: float
+    vortex_center_surface_z_mm: float
+    vortex_wall_surface_z_mm: float
+    sediment_transport_model: str
+    sediment_majorant_geom_per_m: float
+    sediment_majorant_wave_per_m: float
+    sediment_majorant_total_per_m: float
+    mean_bottom_inner_reflections: float
+    mean_bottom_outer_reflections: float
+    bottom_tir_fraction: float
+    top_exit_fraction: float
+    bottom_exit_fraction: float
+    source_bore_surface_model: str
+    detector_bore_surface_model: str
+    source_collimator_acceptance_fraction: float
+    hardware_native_direct_hits: np.ndarray
+    hardware_mirror_direct_hits: np.ndarray
+    hardware_symmetry_direct_scores: np.ndarray
+    hardware_native_stray_hits: np.ndarray
+    hardware_mirror_stray_hits: np.ndarray
+    hardware_symmetry_stray_scores: np.ndarray
+    detector_stray_fraction_by_channel: np.ndarray
+    photodiode_response_model: str
+    photodiode_native_weighted_scores: np.ndarray
+    photodiode_mirror_weighted_scores: np.ndarray
+    photodiode_symmetry_weighted_scores: np.ndarray
+    detector_mean_incidence_angle_deg_by_channel: np.ndarray
+    hardware_candidate_detection_fraction: float
+    entered_water_fraction: float
+    air_exit_fraction: float
+    axial_loss_fraction: float
+    detector_detection_fraction: float
+    native_detector_detection_fraction: float
+    mirror_detector_detection_fraction: float
+    legacy_channel_score_per_ray: float
+    symmetry_variance_reduction: bool
+    batch_diagnostics: Optional[list] = None
+    particle_diagnostics: Optional[dict] = None
+    # ---- extensions (no reference field) ----
+    aux_detector_angles_deg: Optional[np.ndarray] = None
+    aux_native_hits: Optional[np.ndarray] = None
+    aux_mirror_hits: Optional[np.ndarray] = None
+    aux_symmetry_scores: Optional[np.ndarray] = None
+    status_counts: Optional[np.ndarray] = None
+    work_counters: dict = field(default_factory=dict)
+    device_ms: float = float('nan')
+
+    def to_dict(self):
+        d = {k: v for k, v in self.__dict__.items() if k != 'particle_diagnostics'}
+        for k, v in list(d.items()):
+            if isinstance(v, np.ndarray):
+                d[k] = v.tolist()
+            elif isinstance(v, np.generic):
+                d[k] = v.item()
+        return d
+
+
+class ClaritasForwardModel:
+    """GPU light-transport model of the CLARITAS sample cell.
+
+    Keyword arguments and defaults follow CLARITAS.py:5577-5652.  Supported
+    physics: every event law except the moment-constrained one, smooth
+    particle surfaces (particle_surface_rms_slope_deg must be 0), wave
+    coherent fraction 1 for the hybrid laws (wave_surface_rms_height_m = 0),
+    absorbing bores, ideal photodiodes.  Extensions: event law 'plain_mie',
+    source 'collimated_disk', surface 'rankine_vortex', sediment models
+    'rz_field' / 'xyz_field' / 'grid_per_bin' with a caller-supplied FP32
+    field, auxiliary bores at arbitrary angles, 64-bit history ids.
+    """
+
+    EVENT_MODEL_CODES = abi.EVENT_MODEL_CODES
+    WEIGHT_MODE_CODES = abi.WEIGHT_MODE_CODES
+    SOURCE_ANGULAR_MODEL_CODES = abi.SOURCE_ANGULAR_MODEL_CODES
+    FREE_SURFACE_MODEL_CODES = abi.FREE_SURFACE_MODEL_CODES
+    SEDIMENT_TRANSPORT_MODEL_CODES = abi.SEDIMENT_TRANSPORT_MODEL_CODES
+    BORE_SURFACE_MODEL_CODES = {'absorbing': 0}
+    PHOTODIODE_RESPONSE_MODEL_CODES = {'ideal_hard_aperture': 0}
+
+    def __init__(
+        self,
+        n_water=1.33,
+        n_particle=1.59,
+        n_acrylic=1.4906,
+        n_air=1.0,
+        tube_inner_radius_m=0.0465,
+        tube_outer_radius_m=0.0500,
+        tube_length_m=0.491,
+        water_height_m=0.142,
+        sensor_height_above_bottom_m=0.093,
+        sensor_ring_inner_radius_m=0.0505,
+        sensor_ring_outer_radius_m=0.0655,
+        counterbore_depth_m=0.008,
+        through_bore_diameter_m=0.004,
+        counterbore_diameter_m=0.0087,
+        source_launch_radius_m=0.0655,
+        legacy_detector_acceptance_deg=6.5,
+        source_beam_sigma_m=1.0e-5,
+        source_bore_surface_model='absorbing',
+        detector_bore_surface_model='absorbing',
+        photodiode_response_model='ideal_hard_aperture',
+        particle_wavelength_m=622e-9,
+        particle_optical_constants_by_material=None,
+        particle_event_model='geometric',
+        hybrid_wave_scale=1.0,
+        mie_phase_grid_size=2049,
+        particle_surface_rms_slope_deg=25.0,
+        wave_surface_rms_height_m=100e-9,
+        free_surface_model='localized_scully_vortex',
+        vortex_wall_center_height_difference_m=0.050,
+        vortex_core_radius_m=0.020,
+        sediment_transport_model='schiller_naumann_drift_diffusion',
+        sediment_eddy_diffusivity_m2_s=1.0e-3,
+        water_density_kg_per_m3=997.0,
+        water_dynamic_viscosity_pa_s=8.9e-4,
+        gravity_m_s2=9.81,
+        sediment_transport_quadrature_points=4096,
+        sediment_radial_lookup_points=2049,
+        alpha1=0.45,
+        alpha2=5.0,
+        source_angular_model='beta_radiance_3d',
+        density_kg_per_m3=2600.0,
+        bottom_disc_thickness_m=0.003,
+        bottom_external_refractive_index=1.0,
+        max_iterations=int(1e6),
+        max_internal_bounces=64,
+        max_cell_bounces=64,
+        chunk_size=250000,
+        statistics_batch_rays=250000,
+        symmetry_average_exact=True,
+        # ---- extensions ----
+        source_beam_radius_m=1.0e-3,
+        aux_detector_angles_deg=(),
+        jackknife_groups=64,
+        sediment_field=None,
+        sediment_field_zmax_m=None,
+        sediment_field_half_width_m=None,
+        device=0,
+        launch=None,
+    ):
+        self.n_water, self.n_particle = float(n_water), float(n_particle)
+        self.n_acrylic, self.n_air = float(n_acrylic), float(n_air)
+        self.rin, self.rout = float(tube_inner_radius_m), float(tube_outer_radius_m)
+        self.tube_length = float(tube_length_m)
+        self.water_height = float(water_height_m)
+        self.sensor_height = float(sensor_height_above_bottom_m)
+        # sensor-plane coordinates, CLARITAS.py:5667-5671
+        self.zmin = -self.sensor_height
+        self.zmean = self.water_height - self.sensor_height
+        self.zwall_top = self.tube_length - self.sensor_height
+        self.ring_in = float(sensor_ring_inner_radius_m)
+        self.ring_out = float(sensor_ring_outer_radius_m)
+        self.counterbore_depth = float(counterbore_depth_m)
+        self.throat_out = self.ring_out - self.counterbore_depth
+        self.throat_radius = 0.5 * float(through_bore_diameter_m)
+        self.counterbore_radius = 0.5 * float(counterbore_diameter_m)
+        self.source_launch_radius = float(source_launch_radius_m)
+        self.legacy_acceptance_deg = float(legacy_detector_acceptance_deg)
+        self.beam_sigma = float(source_beam_sigma_m)
+        self.beam_radius = float(source_beam_radius_m)
+        self.source_bore_surface_model = str(source_bore_surface_model).strip().lower()
+        self.detector_bore_surface_model = str(detector_bore_surface_model).strip().lower()
+        self.photodiode_response_model = str(photodiode_response_model).strip().lower()
+        self.particle_wavelength = float(particle_wavelength_m)
+        self.particle_optical_constants_by_material = particle_optical_constants_by_material
+        self.particle_event_model = str(particle_event_model).strip().lower()
+        self.hybrid_wave_scale = float(hybrid_wave_scale)
+        self.mie_phase_grid_size = int(mie_phase_grid_size)
+        self.particle_surface_rms_slope_deg = float(particle_surface_rms_slope_deg)
+        self.particle_surface_beckmann_alpha = math.tan(math.radians(self.particle_surface_rms_slope_deg))
+        self.wave_surface_rms_height_m = float(wave_surface_rms_height_m)
+        self.free_surface_model = str(free_surface_model).strip().lower()
+        self.vortex_delta_h = float(vortex_wall_center_height_difference_m)
+        self.vortex_core_radius = float(vortex_core_radius_m)
+        self.sediment_transport_model = str(sediment_transport_model).strip().lower()
+        self.sediment_eddy_diffusivity = float(sediment_eddy_diffusivity_m2_s)
+        self.water_density = float(water_density_kg_per_m3)
+        self.water_dynamic_viscosity = float(water_dynamic_viscosity_pa_s)
+        self.gravity = float(gravity_m_s2)
+        self.sediment_transport_quadrature_points = int(sediment_transport_quadrature_points)
+        self.sediment_radial_lookup_points = int(sediment_radial_lookup_points)
+        self.alpha1, self.alpha2 = float(alpha1), float(alpha2)
+        self.source_angular_model = str(source_angular_model).strip().lower()
+        self.density = float(density_kg_per_m3)
+        self.bottom_disc_thickness = float(bottom_disc_thickness_m)
+        self.bottom_external_n = float(bottom_external_refractive_index)
+        self.max_iterations = int(max_iterations)
+        self.max_internal_bounces = int(max_internal_bounces)
+        self.max_cell_bounces = int(max_cell_bounces)
+        self.chunk_size = int(chunk_size)
+        self.statistics_batch_rays = int(statistics_batch_rays)
+        self.symmetry_average_exact = bool(symmetry_average_exact)
+        self.aux_detector_angles_deg = np.asarray(list(aux_detector_angles_deg), dtype=float)
+        self.jackknife_groups = int(jackknife_groups)
+        self.sediment_field = None if sediment_field is None else np.ascontiguousarray(sediment_field, dtype=np.float32)
+        self.sediment_field_zmax = sediment_field_zmax_m
+        self.sediment_field_half_width = sediment_field_half_width_m
+        self.launch = launch
+        self._validate()
+        self.device = int(device)
+        self._ctx = None
+
+    @property
+    def ctx(self) -> Context:
+        """HIP device context, opened on first use (raises without a GPU)."""
+        if self._ctx is None:
+            self._ctx = Context(self.device)
+        return self._ctx
+
+    # ---------------------------------------------------------------- checks
+    def _validate(self):
+        """Fail-closed validation, following CLARITAS.py:5754-5999 for the
+        supported physics and refusing what the device path does not model."""
+        if self.particle_event_model not in self.EVENT_MODEL_CODES:
+            raise ValueError(f'unsupported particle_event_model {self.particle_event_model!r}')
+        if self.source_angular_model not in self.SOURCE_ANGULAR_MODEL_CODES:
+            raise ValueError(f'unsupported source_angular_model {self.source_angular_model!r}')
+        if self.free_surface_model not in self.FREE_SURFACE_MODEL_CODES:
+            raise ValueError(f'unsupported free_surface_model {self.free_surface_model!r}')
+        if self.sediment_transport_model not in self.SEDIMENT_TRANSPORT_MODEL_CODES:
+            raise ValueError(f'unsupported sediment_transport_model {self.sediment_transport_model!r}')
+        if self.source_bore_surface_model not in self.BORE_SURFACE_MODEL_CODES or \
+                self.detector_bore_surface_model not in self.BORE_SURFACE_MODEL_CODES:
+            raise ValueError('only absorbing bores are modelled')
+        if self.photodiode_response_model not in self.PHOTODIODE_RESPONSE_MODEL_CODES:
+            raise ValueError('only the ideal hard-aperture photodiode is modelled')
+        if self.particle_surface_beckmann_alpha > 1e-7:
+            raise ValueError('rough particle surfaces are not modelled: set particle_surface_rms_slope_deg=0')
+        if self.particle_event_model in ('hybrid_mie_excess', 'hybrid_mie_exact_rate') and \
+                self.wave_surface_rms_height_m > 0.0:
+            raise ValueError('the spheroid morphology channel is not modelled: set wave_surface_rms_height_m=0')
+        if not (0 < self.rin < self.rout < self.ring_in < self.throat_out <= self.source_launch_radius <= self.ring_out):
+            raise ValueError('geometry must satisfy 0<Rin<Rout<ring_in<throat_out<=source<=ring_out')
+        if not (self.zwall_top > self.zmean > self.zmin):
+            raise ValueError('tube top > mean water level > bottom is required')
+        if not (0 < self.throat_radius < self.counterbore_radius):
+            raise ValueError('bore radii must satisfy 0 < throat < counterbore')
+        if self.mie_phase_grid_size < 257:
+            raise ValueError('mie_phase_grid_size must be >= 257')
+        if not (1 <= self.jackknife_groups <= abi.MAX_GROUPS):
+            raise ValueError('jackknife_groups must be 1..64')
+        if self.aux_detector_angles_deg.size > abi.MAX_AUX:
+            raise ValueError('at most 4 auxiliary bores')
+        if self.free_surface_model != 'flat' and self.vortex_delta_h > 0.0:
+            vg = vortex_surface_geometry(self.free_surface_model, self.zmean, self.rin, self.vortex_delta_h,
+                                         self.vortex_core_radius, self.gravity)
+            if not (vg['wall_z'] < self.zwall_top and vg['center_z'] > self.zmin):
+                raise ValueError('free surface leaves the tube')
+        if self.sediment_transport_model in ('rz_field', 'xyz_field', 'grid_per_bin'):
+            if self.sediment_field is None:
+                raise ValueError(f'{self.sediment_transport_model} needs sediment_field')
+
+    # --------------------------------------------------------------- structs
+    def config_struct(self, material: Material) -> abi.Config:
+        c = abi.Config()
+        c.n_water, c.n_acrylic, c.n_air = self.n_water, self.n_acrylic, self.n_air
+        c.bottom_external_n = self.bottom_external_n
+        c.rin, c.rout, c.zmin, c.zmean, c.zwall_top = self.rin, self.rout, self.zmin, self.zmean, self.zwall_top
+        c.bottom_disc_thickness = self.bottom_disc_thickness
+        c.ring_in, c.throat_out, c.ring_out = self.ring_in, self.throat_out, self.ring_out
+        c.source_r, c.throat_r, c.counterbore_r = self.source_launch_radius, self.throat_radius, self.counterbore_radius
+        flat = self.free_surface_model == 'flat'
+        c.vortex_delta_h = 0.0 if flat else self.vortex_delta_h
+        c.vortex_core_radius = self.vortex_core_radius
+        c.alpha1, c.alpha2, c.beam_sigma, c.beam_radius = self.alpha1, self.alpha2, self.beam_sigma, self.beam_radius
+        c.legacy_accept_deg = self.legacy_acceptance_deg
+        c.wavelength = self.particle_wavelength
+        c.rough_alpha = self.particle_surface_beckmann_alpha
+        for i, a in enumerate(self.aux_detector_angles_deg):
+            c.aux_angles_deg[i] = float(a)
+        c.n_aux = int(self.aux_detector_angles_deg.size)
+        c.free_surface_model = self.FREE_SURFACE_MODEL_CODES[self.free_surface_model]
+        c.source_angular_model = self.SOURCE_ANGULAR_MODEL_CODES[self.source_angular_model]
+        c.event_model = self.EVENT_MODEL_CODES[self.particle_event_model]
+        c.sediment_model = self.SEDIMENT_TRANSPORT_MODEL_CODES[self.sediment_transport_model]
+        c.max_particle_bounces = self.max_internal_bounces
+        c.max_cell_bounces = self.max_cell_bounces
+        c.max_iterations = self.max_iterations
+        c.conditional_max_trials = CONDITIONAL_MAX_TRIALS
+        c.n_groups = self.jackknife_groups
+        f = self.sediment_field
+        if f is not None:
+            if c.sediment_model == 5:
+                c.field_n0, c.field_n1, c.field_n2 = f.shape[-3], f.shape[-2], f.shape[-1]
+            else:
+                c.field_n0, c.field_n1 = f.shape[-2], f.shape[-1]
+            c.field_nbins = f.shape[0] if (c.sediment_model == 3 and f.ndim == 3) else 1
+        c.field_zmax = float(self.sediment_field_zmax if self.sediment_field_zmax is not None else self.zmean)
+        c.field_xy_half = float(self.sediment_field_half_width if self.sediment_field_half_width is not None else self.rin)
+        return c
+
+    def particles_struct(self, material: Material, concentration_g_per_L: float, weights, weight_mode):
+        d = np.ascontiguousarray(material.diameter_m, dtype=np.float64)
+        w = np.ascontiguousarray(material.weights if weights is None else weights, dtype=np.float64)
+        if w.ndim != 1 or w.size != d.size:
+            raise ValueError(f'{material.name} requires {d.size} PSD weights')
+        if np.any(~np.isfinite(w)) or np.any(w < 0.0) or not np.any(w > 0.0):
+            raise ValueError('invalid PSD weights')
+        if weight_mode not in self.WEIGHT_MODE_CODES:
+            raise ValueError("weight_mode must be 'mass_fraction' or 'number_fraction'")
+        conc = float(concentration_g_per_L)
+        if not np.isfinite(conc) or conc < 0.0:
+            raise ValueError('concentration must be finite and >=0')
+        nr = np.ascontiguousarray(material.n_real, dtype=np.float64)
+        ki = np.ascontiguousarray(material.k_imag, dtype=np.float64)
+        keep = (d, w, nr, ki)
+        p = abi.Particles()
+        PD = C.POINTER(C.c_double)
+        p.diameter, p.weight = d.ctypes.data_as(PD), w.ctypes.data_as(PD)
+        p.n_real, p.k_imag = nr.ctypes.data_as(PD), ki.ctypes.data_as(PD)
+        p.concentration = conc
+        p.density = float(material.density_kg_m3 if material.density_kg_m3 else self.density)
+        p.n_medium = self.n_water
+        p.wavelength = self.particle_wavelength
+        p.hybrid_wave_scale = self.hybrid_wave_scale
+        p.wave_rms_height_m = self.wave_surface_rms_height_m
+        p.n = int(d.size)
+        p.n_theta = self.mie_phase_grid_size
+        p.weight_mode = self.WEIGHT_MODE_CODES[weight_mode]
+        p.event_model = self.EVENT_MODEL_CODES[self.particle_event_model]
+        return p, keep
+
+    def sediment_struct(self, material: Material) -> abi.Sediment:
+        s = abi.Sediment()
+        s.particle_density = float(material.density_kg_m3 if material.density_kg_m3 else self.density)
+        s.water_density, s.water_viscosity = self.water_density, self.water_dynamic_viscosity
+        s.gravity, s.eddy_diffusivity = self.gravity, self.sediment_eddy_diffusivity
+        s.n_quad, s.n_radial = self.sediment_transport_quadrature_points, self.sediment_radial_lookup_points
+        return s
+
+    def resolve_material(self, material) -> Material:
+        if isinstance(material, Material):
+            return material
+        return reference_material(material, self.density, self.particle_optical_constants_by_material)
+
+    # -------------------------------------------------------------- prepare
+    def prepare(self, material, concentration_g_per_L, weights=None, weight_mode='mass_fraction', slot=0):
+        """Device precompute for one configuration (Mie tables, transport
+        split, sediment coefficients); replaces _prepare_gpu_particle_physics."""
+        mat = self.resolve_material(material)
+        cfg = self.config_struct(mat)
+        p, keep = self.particles_struct(mat, concentration_g_per_L, weights, weight_mode)
+        sed = self.sediment_struct(mat)
+        f = self.sediment_field
+        fptr = f.ctypes.data_as(C.POINTER(C.c_float)) if f is not None else None
+        flen = int(f.size) if f is not None else 0
+        self.ctx.check(self.ctx.lib.clr_prepare(self.ctx.h, int(slot), C.byref(cfg), C.byref(p), C.byref(sed),
+                                                fptr, flen), 'clr_prepare')
+        del keep
+        return mat, cfg
+
+    def tables(self, n_bins: int, slot: int = 0):
+        """Host copy of the prepared per-bin tables of `slot`."""
+        PD = C.POINTER(C.c_double)
+        arrs = {k: np.zeros(n_bins) for k in ('radius', 'number_density', 'mu_geom_bin', 'mu_sca_bin',
+                                             'mu_ext_bin', 'trace_geom_bin', 'trace_wave_bin', 'geom_cdf',
+                                             'wave_cdf', 'sed_A', 'sed_B', 'sed_norm', 'sed_settling')}
+        t = abi.TablesOut()
+        for k, v in arrs.items():
+            setattr(t, k, v.ctypes.data_as(PD))
+        self.ctx.check(self.ctx.lib.clr_get_tables(self.ctx.h, int(slot), C.byref(t)), 'clr_get_tables')
+        arrs['mu_scalars'] = np.array(list(t.mu_scalars))
+        arrs['majorants'] = np.array(list(t.majorants))
+        return arrs
+
+    def mie_tables(self, material, n_theta=None):
+        """FP64 Mie efficiencies and phase tables on the device
+        (phase_table_diagnostics counterpart, CLARITAS.py:7138-7158)."""
+        mat = self.resolve_material(material)
+        p, keep = self.particles_struct(mat, 0.0, None, 'mass_fraction')
+        nt = int(n_theta or self.mie_phase_grid_size)
+        p.n_theta = nt
+        n = p.n
+        out = {k: np.zeros(n) for k in ('size_parameter', 'qext', 'qsca', 'qabs', 'qback', 'asym_g', 'albedo')}
+        out['nstop'] = np.zeros(n, dtype=np.int32)
+        out['theta'] = np.zeros(nt)
+        out['phase_pdf'] = np.zeros((n, nt))
+        out['phase_cdf'] = np.zeros((n, nt))
+        m = abi.MieOut()
+        PD = C.POINTER(C.c_double)
+        for k in ('size_parameter', 'qext', 'qsca', 'qabs', 'qback', 'asym_g', 'albedo', 'theta', 'phase_pdf',
+                  'phase_cdf'):
+            setattr(m, k, out[k].ctypes.data_as(PD))
+        m.nstop = out['nstop'].ctypes.data_as(C.POINTER(C.c_int32))
+        self.ctx.check(self.ctx.lib.clr_mie_tables(self.ctx.h, C.byref(p), C.byref(m)), 'clr_mie_tables')
+        del keep
+        return out
+
+    def _launch(self):
+        if self.launch is None:
+            return None
+        l = abi.Launch()
+        l.block, l.blocks_per_cu, l.chunk = (int(self.launch.get(k, 0)) for k in ('block', 'blocks_per_cu', 'chunk'))
+        return C.byref(l)
+
+    def run_tallies(self, n, seed, history_offset=0, n_records=0):
+        """Trace n histories of the prepared slot 0; returns (tallies dict,
+        records or None, device ms)."""
+        t = abi.Tallies()
+        rec = (abi.Record * int(n_records))() if n_records else None
+        self.ctx.check(self.ctx.lib.clr_run(self.ctx.h, int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, int(history_offset),
+                                            self._launch(), C.byref(t), rec, int(n_records)), 'clr_run')
+        ms = float(self.ctx.lib.clr_last_kernel_ms(self.ctx.h))
+        recs = abi.records_to_numpy(rec, int(n_records)) if n_records else None
+        return abi.tallies_to_numpy(t), recs, ms
+
+    # ------------------------------------------------------------- simulate
+    def simulate(self, material, concentration_g_per_L, weights=None, n_rays=100000, seed=12345,
+                 weight_mode='mass_fraction', collect_rays=False, heatmap_size=0, light_transport_config=None,
+                 target_detector_score=None, min_rays=None, max_rays=None, min_rear_channel_score=None,
+                 min_h170_score=None, stability_l1_tolerance=None, stability_window=3, ray_id_start=0):
+        """Signature of CLARITAS.py:7160-7179.  Per-ray outputs, heat maps and
+        light-transport voxel diagnostics are not produced (collect_rays,
+        heatmap_size and light_transport_config must be off)."""
+        if collect_rays or heatmap_size or (light_transport_config and light_transport_config.get('enabled')):
+            raise ValueError('per-ray outputs and light diagnostics are not produced by the device path')
+        mat, cfg = self.prepare(material, concentration_g_per_L, weights, weight_mode)
+        requested = int(n_rays)
+        ray_id_start = int(ray_id_start)
+        if ray_id_start < 0:
+            raise ValueError('ray_id_start must be >= 0')
+        adaptive = target_detector_score is not None
+        if not adaptive:
+            if requested < 1:
+                raise ValueError('number of rays must be positive')
+            T, _, ms = self.run_tallies(requested, seed, ray_id_start)
+            batches = [T['det'][g] for g in range(T['n_groups'])]
+            group_n = T['group_histories']
+            stop_reason, total = 'fixed_n_rays', requested
+            det_sum = T['det'].sum(axis=0)
+        else:
+            target = float(target_detector_score)
+            if target <= 0:
+                raise ValueError('target_detector_score must be >0')
+            maxN = int(max_rays if max_rays is not None else max(requested, self.statistics_batch_rays))
+            minN = int(min_rays if min_rays is not None else min(self.statistics_batch_rays, maxN))
+            if minN < 0 or maxN < minN or maxN < 1:
+                raise ValueError('invalid min_rays/max_rays')
+            total, ms, batches, group_n = 0, 0.0, [], []
+            acc = None
+            stop_reason = 'max_rays'
+            prev_cum, recent = None, []
+            while total < maxN:
+                b = min(self.statistics_batch_rays, maxN - total)
+                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total)
+                ms += msb
+                d = Tb['det'].sum(axis=0)
+                batches.append(d)
+                group_n.append(b)
+                acc = Tb if acc is None else self._merge(acc, Tb)
+                total += b
+                sym = self._sym(np.stack(batches).sum(axis=0))
+                l1 = float(np.sum(np.abs(_norm(sym) - prev_cum))) if prev_cum is not None else float('nan')
+                if np.isfinite(l1):
+                    recent.append(l1)
+                prev_cum = _norm(sym)
+                if total >= minN and sym.sum() >= target:
+                    ok = (min_rear_channel_score is None or float(np.min(sym[15:18])) >= float(min_rear_channel_score))
+                    ok &= (min_h170_score is None or float(sym[17]) >= float(min_h170_score))
+                    if stability_l1_tolerance is not None:
+                        w = max(1, int(stability_window))
+                        ok &= len(recent) >= w and all(q <= float(stability_l1_tolerance) for q in recent[-w:])
+                    if ok:
+                        stop_reason = 'target_detector_score'
+                        break
+            T = acc
+            T['n_groups'] = len(batches)
+            det_sum = np.stack(batches).sum(axis=0)
+            group_n = np.asarray(group_n)
+        return self._result(mat, T, det_sum, batches, group_n, requested, total, adaptive, stop_reason, ms)
+
+    @staticmethod
+    def _merge(a, b):
+        out = dict(a)
+        for k in ('aux', 'incidence_sum', 'status', 'failures', 'counters', 'ic_hist'):
+            out[k] = a[k] + b[k]
+        return out
+
+    def _sym(self, det):
+        """Symmetry score 0.5(native+mirror) per kind, CLARITAS.py:8788-8792."""
+        if self.symmetry_average_exact:
+            return 0.5 * (det[0].astype(float) + det[1].astype(float))
+        return det[0].astype(float)
+
+    def _result(self, mat, T, det, batches, group_n, requested, total, adaptive, stop_reason, ms):
+        cnt = T['counters']
+        ci = abi.C_IDX
+        native, mirror = det[0].astype(np.int64), det[1].astype(np.int64)
+        lnat, lmir = det[2].astype(np.int64), det[3].astype(np.int64)
+        sym = self._sym(det)
+        lsym = 0.5 * (lnat + lmir) if self.symmetry_average_exact else lnat.astype(float)
+        hw_batches = [self._sym(b) for b in batches]
+        lg_batches = [(0.5 * (b[2] + b[3]) if self.symmetry_average_exact else b[2].astype(float)) for b in batches]
+        se = _jackknife_normalized_se(hw_batches)
+        lse = _jackknife_normalized_se(lg_batches)
+        status = T['status']
+        fails = T['failures']
+        n_fail = int(fails.sum())
+        entered = int(cnt[ci['entered']])
+        den = max(entered, 1)
+        nan = float('nan')
+
+        def per_entered(k):
+            return cnt[ci[k]] / den if entered else nan
+
+        tables = self.tables(len(mat.diameter_m))
+        mus = tables['mu_scalars']
+        vg = vortex_surface_geometry(self.free_surface_model, self.zmean, self.rin,
+                                     0.0 if self.free_surface_model == 'flat' else self.vortex_delta_h,
+                                     self.vortex_core_radius, self.gravity)
+        inc = T['incidence_sum']
+        inc_sym = 0.5 * (inc[0] + inc[1]) if self.symmetry_average_exact else inc[0]
+        mean_inc = np.divide(inc_sym, sym, out=np.full(18, np.nan), where=sym > 0)
+        naux = self.aux_detector_angles_deg.size
+        aux = T['aux'].sum(axis=0)[:, :naux]
+        batch_rows = [{'batch': g + 1, 'batch_rays': int(group_n[g]),
+                       'batch_native_hits': int(batches[g][0].sum()), 'batch_mirror_hits': int(batches[g][1].sum()),
+                       'batch_hardware_candidate_score': float(hw_batches[g].sum())}
+                      for g in range(len(batches))]
+        zero = np.zeros(18, dtype=np.int64)
+        return ForwardResult(
+            detector_angles_deg=DETECTOR_ANGLES_DEG.copy(),
+            raw_hits=native.copy(),
+            normalized_response=_norm(sym),
+            native_exact_hits=native.copy(),
+            mirror_exact_hits=mirror.copy(),
+            symmetry_exact_scores=sym.copy(),
+            normalized_native_exact_response=_norm(native.astype(float)),
+            normalized_mirror_exact_response=_norm(mirror.astype(float)),
+            normalized_symmetry_exact_response=_norm(sym),
+            normalized_response_jackknife_se=se,
+            hardware_native_hits=native.copy(),
+            hardware_mirror_hits=mirror.copy(),
+            hardware_symmetry_scores=sym.copy(),
+            normalized_hardware_native_response=_norm(native.astype(float)),
+            normalized_hardware_mirror_response=_norm(mirror.astype(float)),
+            normalized_hardware_symmetry_response=_norm(sym),
+            hardware_normalized_jackknife_se=se,
+            legacy_native_channel_counts=lnat,
+            legacy_mirror_channel_counts=lmir,
+            legacy_symmetry_channel_scores=lsym,
+            normalized_legacy_native_response=_norm(lnat.astype(float)),
+            normalized_legacy_mirror_response=_norm(lmir.astype(float)),
+            normalized_legacy_symmetry_response=_norm(lsym),
+            legacy_normalized_jackknife_se=lse,
+            legacy_acceptance_deg=self.legacy_acceptance_deg,
+            valid_exit_count=int(cnt[ci['valid_air']]),
+            numerical_failure_count=n_fail,
+            numerical_failure_fraction=n_fail / max(total, 1),
+            production_failure_count=n_fail,
+            production_failure_code_counts={abi.PRODUCTION_FAILURE_NAMES[i]: int(fails[i])
+                                            for i in range(1, abi.N_FAILURE) if fails[i]},
+            n_rays=int(total),
+            requested_n_rays=int(requested),
+            stopped_adaptively=bool(adaptive and stop_reason == 'target_detector_score'),
+            stop_reason=stop_reason,
+            particle_event_model=self.particle_event_model,
+            particle_surface_rms_slope_deg=self.particle_surface_rms_slope_deg,
+            wave_surface_rms_height_nm=self.wave_surface_rms_height_m * 1e9,
+            wave_coherent_fraction_nominal=float(mus[7]),
+            mu_geom_per_m=float(mus[2]),
+            mu_wave_per_m=float(mus[5]),
+            mu_event_per_m=float(mus[6]),
+            mu_sca_per_m=float(mus[3]),
+            mu_ext_per_m=float(mus[4]),
+            geometric_optical_depth_diameter=float(mus[2]) * 2 * self.rin,
+            optical_depth_diameter=float(mus[6]) * 2 * self.rin,
+            mean_interactions=per_entered('sum_ic'),
+            mean_geometric_interactions=per_entered('sum_gic'),
+            mean_wave_interactions=per_entered('sum_wic'),
+            mean_coherent_wave_interactions=per_entered('sum_wcoh'),
+            median_interactions=_histogram_median(T['ic_hist']),
+            ballistic_fraction=per_entered('ballistic'),
+            mean_fresnel_reflections=per_entered('sum_pr'),
+            entry_reflection_fraction=per_entered('rays_per'),
+            internal_reflection_fraction=per_entered('rays_pir'),
+            particle_absorption_fraction=per_entered('rays_pabs'),
+            particle_absorbed_ray_count=int(cnt[ci['rays_pabs']]),
+            mean_cell_inner_reflections=per_entered('sum_cir'),
+            mean_cell_outer_reflections=per_entered('sum_cor'),
+            cell_tir_fraction=per_entered('rays_ctir'),
+            mean_top_surface_reflections=per_entered('sum_topr'),
+            top_surface_tir_fraction=per_entered('rays_toptir'),
+            mean_top_surface_interactions=((cnt[ci['sum_topr']] + status[6]) / den) if entered else nan,
+            free_surface_model=self.free_surface_model,
+            vortex_wall_center_height_difference_mm=self.vortex_delta_h * 1e3,
+            vortex_core_radius_mm=self.vortex_core_radius * 1e3,
+            vortex_center_surface_z_mm=vg['center_z'] * 1e3,
+            vortex_wall_surface_z_mm=vg['wall_z'] * 1e3,
+            sediment_transport_model=self.sediment_transport_model,
+            sediment_majorant_geom_per_m=float(tables['majorants'][0]),
+            sediment_majorant_wave_per_m=float(tables['majorants'][1]),
+            sediment_majorant_total_per_m=float(tables['majorants'][2]),
+            mean_bottom_inner_reflections=per_entered('sum_bir'),
+            mean_bottom_outer_reflections=per_entered('sum_bor'),
+            bottom_tir_fraction=per_entered('rays_btir'),
+            top_exit_fraction=status[6] / max(total, 1),
+            bottom_exit_fraction=status[7] / max(total, 1),
+            source_bore_surface_model=self.source_bore_surface_model,
+            detector_bore_surface_model=self.detector_bore_surface_model,
+            source_collimator_acceptance_fraction=(total - status[3]) / max(total, 1),
+            hardware_native_direct_hits=native.copy(),
+            hardware_mirror_direct_hits=mirror.copy(),
+            hardware_symmetry_direct_scores=sym.copy(),
+            hardware_native_stray_hits=zero.copy(),
+            hardware_mirror_stray_hits=zero.copy(),
+            hardware_symmetry_stray_scores=np.zeros(18),
+            detector_stray_fraction_by_channel=np.zeros(18),
+            photodiode_response_model=self.photodiode_response_model,
+            photodiode_native_weighted_scores=native.astype(float),
+            photodiode_mirror_weighted_scores=mirror.astype(float),
+            photodiode_symmetry_weighted_scores=sym.copy(),
+            detector_mean_incidence_angle_deg_by_channel=mean_inc,
+            hardware_candidate_detection_fraction=float(sym.sum()) / max(total, 1),
+            entered_water_fraction=entered / max(total, 1),
+            air_exit_fraction=int(cnt[ci['valid_air']]) / max(total, 1),
+            axial_loss_fraction=(status[6] + status[7]) / max(total, 1),
+            detector_detection_fraction=float(sym.sum()) / max(total, 1),
+            native_detector_detection_fraction=float(native.sum()) / max(total, 1),
+            mirror_detector_detection_fraction=float(mirror.sum()) / max(total, 1),
+            legacy_channel_score_per_ray=float(lsym.sum()) / max(total, 1),
+            symmetry_variance_reduction=self.symmetry_average_exact,
+            batch_diagnostics=batch_rows,
+            particle_diagnostics={'diameter_m': mat.diameter_m, 'weights': mat.weights, **tables},
+            aux_detector_angles_deg=self.aux_detector_angles_deg.copy(),
+            aux_native_hits=aux[0].astype(np.int64),
+            aux_mirror_hits=aux[1].astype(np.int64),
+            aux_symmetry_scores=(0.5 * (aux[0] + aux[1])) if self.symmetry_average_exact else aux[0].astype(float),
+            status_counts=status.copy(),
+            work_counters={k: int(cnt[i]) for k, i in ci.items()},
+            device_ms=ms,
+        )

@@ -1,0 +1,120 @@
+"""The five benchmark workloads as constructor kwargs + simulate arguments.
+
+Config 1: plain MC, small cell (r 12.5 mm, water 50 mm, flat surface),
+monodisperse d=10 um, m=1.46+1e-4i, 0.1 g/L uniform, 860 nm collimated
+1 mm beam, 1800-angle Mie table, 1e6 histories, seed 24620001.
+Config 2: 40-bin lognormal PSD (median 20 um, sigma_g 1.8), Rouse-profile
+FP32 r-z field 64x128, Rankine surface (core 3 mm, depth 5 mm), 860 nm,
+concentrations 10..5000 mg/L, 1e8 histories each.
+Config 3: 256 sampled configurations in one batched launch.
+Config 4: surrogate training/queries on config-3 Monte Carlo.
+Config 5: 3-D FP32 field 128x128x256, 20 g/L, detectors 0/90/135 deg.
+Each entry states its event law: 'hybrid_mie_exact_rate' with smooth
+particles is the reference-parity law; 'plain_mie' samples the Mie phase
+function at every extinction event.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .physics import cell_geometry, lognormal, monodisperse, rankine_surface_z, rouse_rz_field, \
+    segregated_xyz_field
+
+SEED = 24620001          # CLARITAS.py:61
+SMALL_CELL = dict(diameter_m=0.025, water_level_m=0.050, sensor_height_m=0.025, cell_height_m=0.100)
+CONFIG2_CONCENTRATIONS_MG_L = (10.0, 31.6, 100.0, 316.0, 1000.0, 3162.0, 5000.0)
+
+
+def base_kwargs(event_model='hybrid_mie_exact_rate', wavelength_nm=860.0):
+    kw = cell_geometry(**SMALL_CELL)
+    kw.update(
+        n_water=1.33, particle_wavelength_m=wavelength_nm * 1e-9, mie_phase_grid_size=1800,
+        particle_event_model=event_model, particle_surface_rms_slope_deg=0.0,
+        wave_surface_rms_height_m=0.0, free_surface_model='flat', sediment_transport_model='uniform',
+        source_angular_model='collimated_disk', source_beam_radius_m=1.0e-3,
+        legacy_detector_acceptance_deg=10.0, density_kg_per_m3=2650.0, max_internal_bounces=4095,
+    )
+    return kw
+
+
+def config1(event_model='hybrid_mie_exact_rate', source='collimated_disk'):
+    kw = base_kwargs(event_model)
+    kw['source_angular_model'] = source
+    mat = monodisperse(10e-6, 1.46, 1e-4)
+    return kw, mat, 0.1, 1_000_000, SEED
+
+
+def config2(conc_mg_l=1000.0, event_model='plain_mie', rouse_number=0.3, nr=64, nz=128):
+    kw = base_kwargs(event_model)
+    geo = cell_geometry(**SMALL_CELL)
+    rin = geo['tube_inner_radius_m']
+    zmin = -geo['sensor_height_above_bottom_m']
+    zmean = geo['water_height_m'] - geo['sensor_height_above_bottom_m']
+    depth, core = 5e-3, 3e-3
+    zs = lambda r: rankine_surface_z(r, zmean, rin, depth, core)
+    zmax = float(zs(np.array([rin]))[0]) + 1e-3
+    field = rouse_rz_field(rin, zmin, zmax, nr, nz, rouse_number, surface_z=zs)
+    kw.update(free_surface_model='rankine_vortex', vortex_wall_center_height_difference_m=depth,
+              vortex_core_radius_m=core, sediment_transport_model='rz_field', sediment_field=field,
+              sediment_field_zmax_m=zmax)
+    mat = lognormal(20e-6, 1.8, 40, 1.46, 1e-4)
+    return kw, mat, conc_mg_l * 1e-3, 100_000_000, SEED
+
+
+def config3_params(n=256, seed=20261203):
+    """256 configurations drawn with PCG64(20261203): PSD median log-U(2,60) um,
+    sigma_g U(1.3,2.2), m_r U(1.45,1.60), m_i log-U(1e-4,1e-2), lambda in
+    {530,860} nm, concentration log-U(10,10000) mg/L, budget log-U(1e5,1e7);
+    vortex depth U(0,6) mm (Rankine, core 3 mm) and Rouse number U(0.05,1.0)
+    are the two surrogate inputs config 3 leaves open."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    lu = lambda a, b: np.exp(g.uniform(np.log(a), np.log(b), n))
+    p = dict(
+        median_um=lu(2.0, 60.0), sigma_g=g.uniform(1.3, 2.2, n), m_r=g.uniform(1.45, 1.60, n),
+        m_i=lu(1e-4, 1e-2), wavelength_nm=np.where(g.uniform(0, 1, n) < 0.5, 530.0, 860.0),
+        conc_mg_l=lu(10.0, 10000.0), budget=np.round(lu(1e5, 1e7)).astype(np.int64),
+        vortex_depth_mm=g.uniform(0.0, 6.0, n), rouse=g.uniform(0.05, 1.0, n),
+    )
+    return p
+
+
+def config3_item(p, i, event_model='plain_mie', nr=64, nz=128, n_bins=40):
+    """Constructor kwargs + material + concentration for sample i."""
+    kw = base_kwargs(event_model, float(p['wavelength_nm'][i]))
+    geo = cell_geometry(**SMALL_CELL)
+    rin = geo['tube_inner_radius_m']
+    zmin = -geo['sensor_height_above_bottom_m']
+    zmean = geo['water_height_m'] - geo['sensor_height_above_bottom_m']
+    depth = float(p['vortex_depth_mm'][i]) * 1e-3
+    core = 3e-3
+    if depth > 0:
+        zs = lambda r: rankine_surface_z(r, zmean, rin, depth, core)
+        zmax = float(zs(np.array([rin]))[0]) + 1e-3
+        kw.update(free_surface_model='rankine_vortex', vortex_wall_center_height_difference_m=depth,
+                  vortex_core_radius_m=core)
+    else:
+        zs, zmax = None, zmean + 1e-3
+        zs = lambda r: np.full_like(np.asarray(r, dtype=float), zmean)
+    field = rouse_rz_field(rin, zmin, zmax, nr, nz, float(p['rouse'][i]), surface_z=zs)
+    kw.update(sediment_transport_model='rz_field', sediment_field=field, sediment_field_zmax_m=zmax,
+              aux_detector_angles_deg=(135.0,))
+    mat = lognormal(float(p['median_um'][i]) * 1e-6, float(p['sigma_g'][i]), n_bins, float(p['m_r'][i]),
+                    float(p['m_i'][i]))
+    return kw, mat, float(p['conc_mg_l'][i]) * 1e-3
+
+
+def config5(event_model='plain_mie', n=(128, 128, 256), seed=SEED):
+    kw = base_kwargs(event_model)
+    geo = cell_geometry(**SMALL_CELL)
+    rin = geo['tube_inner_radius_m']
+    zmin = -geo['sensor_height_above_bottom_m']
+    zmean = geo['water_height_m'] - geo['sensor_height_above_bottom_m']
+    depth, core = 5e-3, 3e-3
+    zs = lambda r: rankine_surface_z(r, zmean, rin, depth, core)
+    zmax = float(zs(np.array([rin]))[0]) + 1e-3
+    field = segregated_xyz_field(rin, zmin, zmax, *n, seed=seed, surface_z=zs)
+    kw.update(free_surface_model='rankine_vortex', vortex_wall_center_height_difference_m=depth,
+              vortex_core_radius_m=core, sediment_transport_model='xyz_field', sediment_field=field,
+              sediment_field_zmax_m=zmax, sediment_field_half_width_m=rin, aux_detector_angles_deg=(135.0,))
+    mat = lognormal(20e-6, 1.8, 40, 1.46, 1e-4)
+    return kw, mat, 20.0, 10_000_000_000, seed
