@@ -414,9 +414,13 @@ CLR_COLD __device__ double rankine_hit(const DevParams& P, double X, double Y, d
     tout = fmax(ta, tb);
   }
   if (!(tend > 0.0) || !(tend < 1.0e299)) return 1.0e300;
-  // up to three pieces [0,b1) [b1,b2) [b2,tend) split where the ray crosses r=a
-  const double b1 = (tin > 0.0 && tin < tend) ? tin : tend;
-  const double b2 = (tout > 0.0 && tout < tend) ? fmax(tout, b1) : tend;
+  // up to three pieces [0,b1) [b1,b2) [b2,tend) split where the ray crosses
+  // r = a (a ray that starts inside the core has only the exit crossing)
+  const bool vin = (tin > 0.0 && tin < tend), vout = (tout > 0.0 && tout < tend);
+  double b1 = tend, b2 = tend;
+  if (vin && vout) { b1 = tin; b2 = tout; }
+  else if (vin) b1 = tin;
+  else if (vout) b1 = tout;
   double lo = 0.0;
   for (int i = 0; i < 3; ++i) {
     const double hi = (i == 0) ? b1 : (i == 1) ? b2 : tend;
