@@ -5,7 +5,11 @@
 Every translation unit under csrc/ is compiled by hipcc for gfx950 and linked
 into one shared library next to this file; the ctypes layer loads it from
 there.  Packed FP32 instructions are disabled (-packed-fp32-ops): hipcc would
-otherwise form v_pk_* FP32 ops from scalar float code.
+otherwise form v_pk_* FP32 ops from scalar float code.  SimplifyCFG's
+common-store sinking is disabled (-simplifycfg-sink-common=false): it merges
+stores to different fields of the per-lane photon state into one store
+through a selected pointer, which keeps that state in scratch memory instead
+of VGPRs.
 """
 from __future__ import annotations
 
@@ -24,6 +28,7 @@ SOURCES = ['mie.hip', 'precompute.hip', 'transport.hip', 'mlp.hip', 'api.cpp']
 FLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-fPIC', '-std=c++17', '-munsafe-fp-atomics',
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    '-mllvm', '-simplifycfg-sink-common=false',
     '-Wno-unused-result',
 ]
 

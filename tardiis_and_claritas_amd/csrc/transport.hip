@@ -82,9 +82,13 @@ __device__ __forceinline__ void philox_refill(Rng& r) {
   r.blk++;
   r.idx = 0;
 }
+// The four words of the current block are consumed as a queue (b0 first,
+// then shifted down): a dynamically indexed pick would make the compiler
+// keep the buffer in scratch memory instead of VGPRs.
 __device__ __forceinline__ uint32_t rng_next(Rng& r) {
   if (r.idx >= 4) philox_refill(r);
-  const uint32_t v = (r.idx == 0) ? r.b0 : (r.idx == 1) ? r.b1 : (r.idx == 2) ? r.b2 : r.b3;
+  const uint32_t v = r.b0;
+  r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3;
   r.idx++;
   r.draws++;
   return v;
@@ -104,12 +108,12 @@ __device__ __forceinline__ double Uopen(Rng& r) {
 
 // ------------------------------------------------------ source samplers --
 // CLARITAS.py:1149-1212
-__device__ float normal_bm(Rng& r) {
+__device__ __forceinline__ float normal_bm(Rng& r) {
   const float u1 = fmaxf(U(r), 1e-12f), u2 = U(r);
   return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * kPiF * u2);
 }
 // Marsaglia-Tsang gamma with the shape<1 boost
-CLR_COLD __device__ float gamma_draw(Rng& r, float shape) {
+__device__ __forceinline__ float gamma_draw(Rng& r, float shape) {
   if (shape <= 0.0f) return 0.0f;
   if (fabsf(shape - 1.0f) < 1e-7f) return -logf(fmaxf(U(r), 1e-12f));
   float boost = 1.0f, a = shape;
@@ -129,7 +133,7 @@ CLR_COLD __device__ float gamma_draw(Rng& r, float shape) {
   }
   return boost * d;
 }
-__device__ float beta_draw(Rng& r, float a, float b) {
+__device__ __forceinline__ float beta_draw(Rng& r, float a, float b) {
   if (a <= 0.0f || b <= 0.0f) return 0.0f;
   if (fabsf(a - 1.0f) < 1e-7f) {
     const float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
@@ -142,7 +146,7 @@ __device__ float beta_draw(Rng& r, float a, float b) {
   const float ga = gamma_draw(r, a), gb = gamma_draw(r, b), s = ga + gb;
   return s > 0.0f ? ga / s : 0.0f;
 }
-CLR_COLD __device__ float beta_radiance_polar(Rng& r, float a, float b) {
+__device__ __forceinline__ float beta_radiance_polar(Rng& r, float a, float b) {
   const float c = 0.5f * kPiF;
   for (int it = 0; it < 128; ++it) {
     const float u = fminf(fmaxf(beta_draw(r, a + 1.0f, b), 0.0f), 1.0f);
@@ -232,7 +236,7 @@ __device__ __forceinline__ cf csqrtp(cf z) {
   if (u < 0.0f || (fabsf(u) < 1e-20f && v < 0.0f)) { u = -u; v = -v; }
   return cmk(u, v);
 }
-CLR_COLD __device__ bool complex_interface(float ci, float n1r, float n1k, float n2r, float n2k, float& R,
+__device__ __forceinline__ bool complex_interface(float ci, float n1r, float n1k, float n2r, float n2k, float& R,
                                   float& t_tan, float& t_nrm) {
   if (!(n1r > 0.f && n2r > 0.f && n1k >= 0.f && n2k >= 0.f)) return false;
   ci = fminf(fmaxf(ci, 0.f), 1.f);
@@ -257,7 +261,7 @@ CLR_COLD __device__ bool complex_interface(float ci, float n1r, float n1k, float
   t_nrm = tz / nm;
   return true;
 }
-__device__ bool energy_refract3(float ix, float iy, float iz, float nx, float ny, float nz,
+__device__ __forceinline__ bool energy_refract3(float ix, float iy, float iz, float nx, float ny, float nz,
                                 float n1r, float n1k, float n2r, float n2k, float& ox, float& oy,
                                 float& oz) {
   const float ci = fminf(fmaxf(-dot3(ix, iy, iz, nx, ny, nz), 0.f), 1.f);
@@ -284,7 +288,7 @@ __device__ __forceinline__ void perpendicular_basis(float vx, float vy, float vz
 
 // ----------------------------------------------------------- free surface --
 // CLARITAS.py:1503-1527, 1611-1779; the Rankine sheet has no reference twin
-__device__ float free_surface_z(const DevParams& P, float x, float y) {
+__device__ __forceinline__ float free_surface_z(const DevParams& P, float x, float y) {
   const int M = P.fs_model;
   if (M == 0 || P.dh <= 0.0f) return P.zmean;
   const float q = x * x + y * y, R = P.rin;
@@ -440,15 +444,21 @@ CLR_COLD __device__ double rankine_hit(const DevParams& P, double X, double Y, d
   }
   return 1.0e300;
 }
-__device__ float free_surface_hit(const DevParams& P, float x, float y, float z, float vx, float vy,
-                                  float vz) {
-  const int M = P.fs_model;
-  const float R = P.rin;
-  if (M == 0 || P.dh <= 0.0f) {
+CLR_COLD __device__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
+                                                  float vy, float vz);
+__device__ __forceinline__ float free_surface_hit(const DevParams& P, float x, float y, float z, float vx,
+                                                  float vy, float vz) {
+  if (P.fs_model == 0 || P.dh <= 0.0f) {
     if (fabsf(vz) <= 1e-14f) return kInfF;
     const float t = (P.zmean - z) / vz;
     return (t > 0.0f) ? t : kInfF;
   }
+  return free_surface_hit_curved(P, x, y, z, vx, vy, vz);
+}
+CLR_COLD __device__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
+                                                  float vy, float vz) {
+  const int M = P.fs_model;
+  const float R = P.rin;
   if (M == 1) {
     const float aa = P.dh / (R * R);
     const double A = -(double)aa * ((double)vx * vx + (double)vy * vy);
@@ -488,7 +498,7 @@ __device__ float free_surface_hit(const DevParams& P, float x, float y, float z,
   }
   return (t > 0.0 && t < 0.5 * (double)kInfF) ? (float)t : kInfF;
 }
-__device__ void free_surface_normal(const DevParams& P, float x, float y, float& nx, float& ny,
+__device__ __forceinline__ void free_surface_normal(const DevParams& P, float x, float y, float& nx, float& ny,
                                     float& nz) {
   const int M = P.fs_model;
   if (M == 0 || P.dh <= 0.0f) { nx = 0.0f; ny = 0.0f; nz = -1.0f; return; }
@@ -589,7 +599,7 @@ __device__ double bore_wall_hit_d(const DevParams& P, float x, float y, float z,
   const double zh = (double)z + t * (double)vz;
   return (zh >= zlo - ztol && zh <= zhi + ztol) ? t : 1.0e300;
 }
-CLR_COLD __device__ bool water_surface_offset(const DevParams& P, float& x, float& y, float& z, float nx,
+__device__ __forceinline__ bool water_surface_offset(const DevParams& P, float& x, float& y, float& z, float nx,
                                      float ny, float nz) {
   const double rr2 = (double)x * (double)x + (double)y * (double)y;
   const double RR = (double)P.rin * P.rin, tol = 8.0 * kFEps * RR;
@@ -609,7 +619,7 @@ CLR_COLD __device__ bool water_surface_offset(const DevParams& P, float& x, floa
   x = xx; y = yy; z = zz;
   return true;
 }
-__device__ int acrylic_next_surface(const DevParams& P, float x, float y, float z, float vx,
+__device__ __forceinline__ int acrylic_next_surface(const DevParams& P, float x, float y, float z, float vx,
                                     float vy, float vz, double& t_out) {
   double best = 1.0e300;
   int surf = 0;
@@ -695,7 +705,7 @@ CLR_COLD __device__ bool acrylic_trapped(const DevParams& P, float x, float y, f
 
 // -------------------------------------------------------- sediment field --
 // CLARITAS.py:1537-1609; shared r-z / x-y-z fields are extensions
-__device__ double rz_sample(const float* __restrict__ G, int NR, int NZ, float x, float y, float z,
+__device__ __forceinline__ double rz_sample(const float* __restrict__ G, int NR, int NZ, float x, float y, float z,
                             float R, float ZMIN, float ZMAX) {
   if (NR < 2 || NZ < 2 || !(ZMAX > ZMIN)) return 1.0;
   double rr = sqrt(fmax((double)x * (double)x + (double)y * (double)y, 0.0));
@@ -775,7 +785,7 @@ __device__ __forceinline__ int sample_cdf_index(Rng& r, const float* __restrict_
   }
   return lo;
 }
-__device__ int select_bin(Rng& r, const DevParams& P, bool wave, float x, float y, float z,
+__device__ __forceinline__ int select_bin(Rng& r, const DevParams& P, bool wave, float x, float y, float z,
                           float local_channel) {
   const int tm = P.sed_model;
   if (tm == CLR_SED_UNIFORM || tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD)
@@ -792,7 +802,7 @@ __device__ int select_bin(Rng& r, const DevParams& P, bool wave, float x, float 
 }
 
 // --------------------------------------------------------- particle events --
-__device__ void sphere_sample(Rng& r, float vx, float vy, float vz, float& gx, float& gy, float& gz) {
+__device__ __forceinline__ void sphere_sample(Rng& r, float vx, float vy, float vz, float& gx, float& gy, float& gz) {
   const float rho = sqrtf(fminf(fmaxf(U(r), 0.0f), 0.99999994f));
   const float ci = sqrtf(fmaxf(0.0f, 1.0f - rho * rho));
   const float phi = U(r) * 2.0f * kPiF;
@@ -804,7 +814,7 @@ __device__ void sphere_sample(Rng& r, float vx, float vy, float vz, float& gx, f
   gz = -ci * vz + rho * (cp * e1z + sp * e2z);
   normalize3(gx, gy, gz);
 }
-__device__ bool sphere_accessible(const DevParams& P, double cx, double cy, double cz, double rr) {
+__device__ __forceinline__ bool sphere_accessible(const DevParams& P, double cx, double cy, double cz, double rr) {
   const double rc = hypot(cx, cy);
   const double wall = ((double)P.rin - rc) - rr, bottom = cz - rr - (double)P.zmin;
   const double rmin = fmax(0.0, rc - rr);
@@ -814,7 +824,7 @@ __device__ bool sphere_accessible(const DevParams& P, double cx, double cy, doub
   return wall >= -tol && bottom >= -tol && top >= -tol;
 }
 // smooth limit of the microsurface walk, CLARITAS.py:2323-2356
-__device__ bool smooth_interface(Rng& r, float bx, float by, float bz, float n_top, float k_top,
+__device__ __forceinline__ bool smooth_interface(Rng& r, float bx, float by, float bz, float n_top, float k_top,
                                  float n_bot, float k_bot, float& dx, float& dy, float& dz,
                                  bool& out_top, int& fc) {
   fc = 0;
@@ -851,7 +861,7 @@ struct PCount {
 };
 // smooth Fresnel sphere with internal Beer-Lambert chords, CLARITAS.py:2497-2599.
 // Returns 1 escape, 2 absorbed, 0 failure.
-CLR_COLD __device__ int sphere_interaction(const DevParams& P, Rng& r, float n_m, float n_p, float k_p,
+__device__ __forceinline__ int sphere_interaction(const DevParams& P, Rng& r, float n_m, float n_p, float k_p,
                                   float radius, float gx, float gy, float gz, double cx, double cy,
                                   double cz, float& x, float& y, float& z, float& vx, float& vy,
                                   float& vz, PCount& pc, int& pf, int& pfs) {
@@ -936,7 +946,7 @@ __device__ __forceinline__ float sample_mie_theta(Rng& r, const DevParams& P, in
   const float f = (c1 > c0) ? fminf(fmaxf((u - c0) / (c1 - c0), 0.0f), 1.0f) : 0.0f;
   return t0 + f * (t1 - t0);
 }
-__device__ void mie_scatter(Rng& r, const DevParams& P, int pidx, float& vx, float& vy, float& vz) {
+__device__ __forceinline__ void mie_scatter(Rng& r, const DevParams& P, int pidx, float& vx, float& vy, float& vz) {
   const float th = sample_mie_theta(r, P, pidx);
   const float phi = 2.0f * kPiF * U(r);
   float e1x, e1y, e1z, e2x, e2y, e2z;
@@ -956,7 +966,7 @@ struct CondOut {
   uint32_t trials, rejects, inacc;
 };
 // conditional finite-particle closure, CLARITAS.py:4479-4536
-CLR_COLD __device__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
+__device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
                                      float vx, float vy, float vz, int max_trials) {
   CondOut h;
   h.x = x; h.y = y; h.z = z; h.vx = vx; h.vy = vy; h.vz = vz;
@@ -1036,13 +1046,31 @@ __device__ __forceinline__ bool bore_accept(const DevParams& P, float x, float y
   }
   return false;
 }
-CLR_COLD __device__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy,
-                                 float vz, bool mirror) {
+// First bore (smallest t0) whose three apertures the exit ray passes.  Only
+// bores within +-20 deg of the azimuth where the ray crosses r = ring_in are
+// tested: an accepted ray crosses that circle before its ring_in aperture
+// plane, at most ~0.2 mm away laterally, so its crossing azimuth lies within
+// asin(throat_r/ring_in) + 0.5 deg of the bore axis (< 10 deg for the cell
+// geometries here); the tested set therefore contains every bore the full
+// 18-bore scan of CLARITAS.py:4241-4253 could accept, in the same order.
+__device__ __forceinline__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
+                                 bool mirror) {
   if (mirror) { x = -x; vx = -vx; }
-  float best_t = kInfF, t;
+  const float a = vx * vx + vy * vy;
+  if (!(a > 0.0f)) return -1;
+  const float b = x * vx + y * vy, c = x * x + y * y - P.ring_in * P.ring_in;
+  const float disc = b * b - a * c;
+  if (!(disc >= 0.0f)) return -1;
+  const float t = (-b + sqrtf(disc)) / a;
+  const float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
+  const int jc = (int)rintf(deg * 0.1f);
+  float best_t = kInfF, tt;
   int best = -1;
-  for (int j = 0; j < CLR_N_CHANNELS; ++j)
-    if (bore_accept(P, x, y, z, vx, vy, vz, P.det_ux[j], P.det_uy[j], t) && t < best_t) { best_t = t; best = j; }
+  for (int k = -2; k <= 2; ++k) {
+    const int j = jc + k;
+    if (j < 0 || j >= CLR_N_CHANNELS) continue;
+    if (bore_accept(P, x, y, z, vx, vy, vz, P.det_ux[j], P.det_uy[j], tt) && tt < best_t) { best_t = tt; best = j; }
+  }
   return best;
 }
 
@@ -1067,6 +1095,7 @@ struct Lane {
 // per-lane work counters accumulated over all histories of a launch
 struct Acc {
   uint32_t n_flight, n_null, n_iface, n_abs, steps, draws, trials, rejects, inacc;
+  uint32_t entered, n_exit, valid_air;
 };
 
 __device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
@@ -1076,7 +1105,7 @@ __device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
 }
 
 // launch + hard apertures + air->acrylic entry, CLARITAS.py:4650-4735
-CLR_COLD __device__ void launch_history(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A) {
   Rng& r = L.r;
   float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
   if (P.src_model == CLR_SOURCE_COLLIMATED_DISK) {
@@ -1139,7 +1168,7 @@ CLR_COLD __device__ void launch_history(Lane& L, const DevParams& P, Acc& A) {
 
 // headspace return code -> next state (callers differ, CLARITAS.py:4720-4731,
 // 4995-5002, 5024-5031, 5051-5059)
-__device__ void headspace_return(Lane& L, int hw) {
+__device__ __forceinline__ void headspace_return(Lane& L, int hw) {
   if (hw == 1) { L.region = R_WATER; L.flags |= F_ENTERED; return; }
   if (hw == 2) { terminate(L, 1); return; }
   if (hw == 3) { terminate(L, 7); return; }
@@ -1148,7 +1177,7 @@ __device__ void headspace_return(Lane& L, int hw) {
   terminate(L, 4, PF_HEADSPACE_NO_HIT);
 }
 // acrylic return code -> next state
-__device__ void acrylic_return(Lane& L, int where, int sub) {
+__device__ __forceinline__ void acrylic_return(Lane& L, int where, int sub) {
   if (L.actx == A_HEAD) {
     if (where == 4) { L.region = R_HEADSPACE; return; }  // same headspace walk continues
     headspace_return(L, where);
@@ -1174,7 +1203,7 @@ __device__ __forceinline__ void enter_acrylic(Lane& L, int ctx) {
 }
 
 // one surface interaction inside the acrylic union, CLARITAS.py:3140-3198
-__device__ void acrylic_step(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void acrylic_step(Lane& L, const DevParams& P, Acc& A) {
   if (L.it >= kEmergencyCap) {
     if (acrylic_trapped(P, L.x, L.y, L.vx, L.vy, L.vz)) acrylic_return(L, 5, SUB_NONE);
     else acrylic_return(L, 0, SUB_ACRYLIC_BOUNCE);
@@ -1244,7 +1273,7 @@ __device__ void acrylic_step(Lane& L, const DevParams& P, Acc& A) {
 }
 
 // one headspace segment, CLARITAS.py:3235-3293
-CLR_COLD __device__ void headspace_step(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc& A) {
   if (L.hit >= kEmergencyCap) { headspace_return(L, 0); return; }
   L.hit++;
   if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z) || !isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) {
@@ -1304,7 +1333,7 @@ CLR_COLD __device__ void headspace_step(Lane& L, const DevParams& P, Acc& A) {
 
 // one pass of the water loop, CLARITAS.py:4734-5066
 template <bool kPlain>
-__device__ void water_step(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void water_step(Lane& L, const DevParams& P, Acc& A) {
   if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
   L.wstep++;
   A.steps++;
@@ -1352,7 +1381,10 @@ __device__ void water_step(Lane& L, const DevParams& P, Acc& A) {
     if (P.sed_model != CLR_SED_UNIFORM) {
       double lg = 0.0, lw = 0.0;
       if (kPlain || P.sed_model == CLR_SED_RZ_FIELD || P.sed_model == CLR_SED_XYZ_FIELD) {
-        const double s = sediment_scale(P, 0, L.x, L.y, L.z);
+        const double s = (P.sed_model == CLR_SED_RZ_FIELD)
+                             ? rz_sample(P.field, P.fn0, P.fn1, L.x, L.y, L.z, P.rin, P.zmin, P.field_zmax)
+                             : xyz_sample(P.field, P.fn0, P.fn1, P.fn2, L.x, L.y, L.z, P.field_half, P.zmin,
+                                          P.field_zmax);
         lg = P.sum_trace_g * s;
         lw = P.sum_trace_w * s;
       } else {
@@ -1504,79 +1536,128 @@ __device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
   if (v) atomicAdd(p, (unsigned long long)v);
 }
 
-// score an ended history, CLARITAS.py:4397-4435 and the per-chunk
-// reductions of CLARITAS.py:8659-8733
-CLR_COLD __device__ void finish_history(Lane& L, const DevParams& P, LdsTally& T, clr_record* rec,
-                               uint64_t n_rec) {
+// add 1 to base[key] for every lane with key >= 0, one LDS atomic per
+// distinct key in the wave (all lanes of the wave must call this)
+__device__ __forceinline__ void agg_count(uint32_t* base, int key) {
+  uint64_t pending = __ballot(key >= 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int k = __shfl(key, leader, 64);
+    const uint64_t same = __ballot(key == k);
+    if ((int)__lane_id() == leader) atomicAdd(base + k, (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ void agg_sum_f(float* base, int key, float v) {
+  uint64_t pending = __ballot(key >= 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int k = __shfl(key, leader, 64);
+    const bool mine = (key == k);
+    const float s = wave_sum_f(mine ? v : 0.0f);
+    if ((int)__lane_id() == leader) atomicAdd(base + k, s);
+    pending &= ~__ballot(mine);
+  }
+}
+
+// Score the histories that ended this step (CLARITAS.py:4397-4435) and fold
+// their per-ray reductions (CLARITAS.py:8659-8733) into LDS.  Called by every
+// lane of the wave; `done` marks the lanes whose history ended.
+__device__ __forceinline__ void finish_wave(Lane& L, bool done, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
+                            uint64_t n_rec) {
   const int g = L.group;
-  atomicAdd(&T.status[L.st], 1u);
-  if (L.st == 4) atomicAdd(&T.fail[L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER], 1u);
-  int nat = -1, mir = -1;
-  if (L.st == 1) {
-    const float ex = L.x, ey = L.y, ez = L.z, evx = L.vx, evy = L.vy, evz = L.vz;
-    atomicAdd(&T.cnt[CLR_C_N_EXIT], 1ull);
-    if (isfinite(ex) && isfinite(evx)) atomicAdd(&T.cnt[CLR_C_VALID_AIR], 1ull);
-    if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
-      nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
-      mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
-      if (nat >= 0) {
-        atomicAdd(&T.det[(g * 4 + 0) * CLR_N_CHANNELS + nat], 1u);
-        const float c = fminf(fmaxf(evx * P.det_ux[nat] + evy * P.det_uy[nat], -1.0f), 1.0f);
-        atomicAdd(&T.inc[nat], acosf(c) * (180.0f / kPiF));
-      }
-      if (mir >= 0) {
-        atomicAdd(&T.det[(g * 4 + 1) * CLR_N_CHANNELS + mir], 1u);
-        const float c = fminf(fmaxf(-evx * P.det_ux[mir] + evy * P.det_uy[mir], -1.0f), 1.0f);
-        atomicAdd(&T.inc[CLR_N_CHANNELS + mir], acosf(c) * (180.0f / kPiF));
-      }
-      for (int a = 0; a < P.n_aux; ++a) {
-        float t;
-        if (bore_accept(P, ex, ey, ez, evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t))
-          atomicAdd(&T.aux[(g * 2 + 0) * CLR_MAX_AUX + a], 1u);
-        if (bore_accept(P, -ex, ey, ez, -evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t))
-          atomicAdd(&T.aux[(g * 2 + 1) * CLR_MAX_AUX + a], 1u);
-      }
-      // legacy exit-position windows
-      for (int m = 0; m < 2; ++m) {
-        const float xx = m ? -ex : ex;
-        float ang = atan2f(xx, ey) * (180.0f / kPiF);
-        if (ang < 0.0f) ang += 360.0f;
-        if (ang < 0.0f || ang > 180.0f) continue;
-        const int jmax = (L.ic <= 0) ? 9 : CLR_N_CHANNELS;
-        for (int j = 0; j < jmax; ++j)
-          if (fabsf(ang - 10.0f * j) <= P.legacy_acc) atomicAdd(&T.det[(g * 4 + 2 + m) * CLR_N_CHANNELS + j], 1u);
+  int nat = -1, mir = -1, aux_bits = 0;
+  bool scored = false;
+  float inc_n = 0.0f, inc_m = 0.0f, ang_n = -1.0f, ang_m = -1.0f;
+  if (done) {
+    A.draws += L.r.draws;
+    if (L.flags & F_ENTERED) A.entered++;
+    if (L.st == 1) {
+      const float ex = L.x, ey = L.y, ez = L.z, evx = L.vx, evy = L.vy, evz = L.vz;
+      A.n_exit++;
+      if (isfinite(ex) && isfinite(evx)) A.valid_air++;
+      if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
+        nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
+        mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+        if (nat >= 0)
+          inc_n = acosf(fminf(fmaxf(evx * P.det_ux[nat] + evy * P.det_uy[nat], -1.0f), 1.0f)) * (180.0f / kPiF);
+        if (mir >= 0)
+          inc_m = acosf(fminf(fmaxf(-evx * P.det_ux[mir] + evy * P.det_uy[mir], -1.0f), 1.0f)) * (180.0f / kPiF);
+        for (int a = 0; a < P.n_aux; ++a) {
+          float t;
+          if (bore_accept(P, ex, ey, ez, evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t)) aux_bits |= 1 << a;
+          if (bore_accept(P, -ex, ey, ez, -evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t)) aux_bits |= 16 << a;
+        }
+        ang_n = atan2f(ex, ey) * (180.0f / kPiF);
+        if (ang_n < 0.0f) ang_n += 360.0f;
+        ang_m = atan2f(-ex, ey) * (180.0f / kPiF);
+        if (ang_m < 0.0f) ang_m += 360.0f;
+        scored = true;
       }
     }
+    if (L.flags & F_ENTERED) {
+      // rarely non-zero sums: plain LDS atomics
+      lds_add64(&T.cnt[CLR_C_SUM_IC], (uint32_t)L.ic);
+      lds_add64(&T.cnt[CLR_C_SUM_GIC], (uint32_t)L.gic);
+      lds_add64(&T.cnt[CLR_C_SUM_WIC], (uint32_t)L.wic);
+      lds_add64(&T.cnt[CLR_C_SUM_WCOH], (uint32_t)L.wcoh);
+      lds_add64(&T.cnt[CLR_C_SUM_PR], (uint32_t)L.pr);
+      lds_add64(&T.cnt[CLR_C_SUM_CIR], (uint32_t)L.cir);
+      lds_add64(&T.cnt[CLR_C_SUM_COR], (uint32_t)L.cor);
+      lds_add64(&T.cnt[CLR_C_SUM_TOPR], (uint32_t)L.tr);
+      lds_add64(&T.cnt[CLR_C_SUM_BIR], (uint32_t)L.bir);
+      lds_add64(&T.cnt[CLR_C_SUM_BOR], (uint32_t)L.bor);
+      if (L.flags & F_PER) atomicAdd(&T.cnt[CLR_C_RAYS_PER], 1ull);
+      if (L.flags & F_PIR) atomicAdd(&T.cnt[CLR_C_RAYS_PIR], 1ull);
+      if (L.flags & F_PABS) atomicAdd(&T.cnt[CLR_C_RAYS_PABS], 1ull);
+      if (L.flags & F_CTIR) atomicAdd(&T.cnt[CLR_C_RAYS_CTIR], 1ull);
+      if (L.flags & F_TTIR) atomicAdd(&T.cnt[CLR_C_RAYS_TOPTIR], 1ull);
+      if (L.flags & F_BTIR) atomicAdd(&T.cnt[CLR_C_RAYS_BTIR], 1ull);
+      if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
+    }
+    lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
+    if (rec && L.hidx < n_rec) {
+      clr_record q;
+      q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
+      q.status = L.st; q.native_id = nat; q.mirror_id = mir; q.interactions = L.ic;
+      q.failure = L.pf; q.draws = (int32_t)L.r.draws;
+      rec[L.hidx] = q;
+    }
   }
-  if (L.flags & F_ENTERED) {
-    atomicAdd(&T.cnt[CLR_C_ENTERED], 1ull);
-    lds_add64(&T.cnt[CLR_C_SUM_IC], (uint32_t)L.ic);
-    lds_add64(&T.cnt[CLR_C_SUM_GIC], (uint32_t)L.gic);
-    lds_add64(&T.cnt[CLR_C_SUM_WIC], (uint32_t)L.wic);
-    lds_add64(&T.cnt[CLR_C_SUM_WCOH], (uint32_t)L.wcoh);
-    if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
-    atomicAdd(&T.ich[L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1], 1u);
-    lds_add64(&T.cnt[CLR_C_SUM_PR], (uint32_t)L.pr);
-    if (L.flags & F_PER) atomicAdd(&T.cnt[CLR_C_RAYS_PER], 1ull);
-    if (L.flags & F_PIR) atomicAdd(&T.cnt[CLR_C_RAYS_PIR], 1ull);
-    if (L.flags & F_PABS) atomicAdd(&T.cnt[CLR_C_RAYS_PABS], 1ull);
-    lds_add64(&T.cnt[CLR_C_SUM_CIR], (uint32_t)L.cir);
-    lds_add64(&T.cnt[CLR_C_SUM_COR], (uint32_t)L.cor);
-    if (L.flags & F_CTIR) atomicAdd(&T.cnt[CLR_C_RAYS_CTIR], 1ull);
-    lds_add64(&T.cnt[CLR_C_SUM_TOPR], (uint32_t)L.tr);
-    if (L.flags & F_TTIR) atomicAdd(&T.cnt[CLR_C_RAYS_TOPTIR], 1ull);
-    lds_add64(&T.cnt[CLR_C_SUM_BIR], (uint32_t)L.bir);
-    lds_add64(&T.cnt[CLR_C_SUM_BOR], (uint32_t)L.bor);
-    if (L.flags & F_BTIR) atomicAdd(&T.cnt[CLR_C_RAYS_BTIR], 1ull);
+  // wave-aggregated tallies: lanes ending together share one atomic per key
+  agg_count(T.status, done ? L.st : -1);
+  agg_count(T.fail, (done && L.st == 4) ? (L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER) : -1);
+  const bool ent = done && (L.flags & F_ENTERED);
+  agg_count(T.ich, ent ? (L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1) : -1);
+  agg_count(T.det, nat >= 0 ? (g * 4 + 0) * CLR_N_CHANNELS + nat : -1);
+  agg_count(T.det, mir >= 0 ? (g * 4 + 1) * CLR_N_CHANNELS + mir : -1);
+  agg_sum_f(T.inc, nat >= 0 ? nat : -1, inc_n);
+  agg_sum_f(T.inc, mir >= 0 ? CLR_N_CHANNELS + mir : -1, inc_m);
+  if (__any(aux_bits != 0)) {
+    for (int a = 0; a < CLR_MAX_AUX; ++a) {
+      agg_count(T.aux, (aux_bits & (1 << a)) ? (g * 2 + 0) * CLR_MAX_AUX + a : -1);
+      agg_count(T.aux, (aux_bits & (16 << a)) ? (g * 2 + 1) * CLR_MAX_AUX + a : -1);
+    }
   }
-  lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
-  lds_add64(&T.cnt[CLR_C_RNG_DRAWS], L.r.draws);
-  if (rec && L.hidx < n_rec) {
-    clr_record q;
-    q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
-    q.status = L.st; q.native_id = nat; q.mirror_id = mir; q.interactions = L.ic;
-    q.failure = L.pf; q.draws = (int32_t)L.r.draws;
-    rec[L.hidx] = q;
+  // legacy exit-position windows |angle - 10 j| <= acceptance; with the
+  // acceptance capped at 15 deg only j within +-2 of angle/10 can match
+  if (__any(scored)) {
+    for (int m = 0; m < 2; ++m) {
+      const float ang = m ? ang_m : ang_n;
+      const bool valid = scored && !(ang < 0.0f || ang > 180.0f);
+      const int j0 = valid ? (int)rintf(ang * 0.1f) : 0;
+      for (int k = -2; k <= 2; ++k) {
+        const int j = j0 + k;
+        const bool hit = valid && j >= 0 && j < CLR_N_CHANNELS && !(L.ic <= 0 && j >= 9) &&
+                         fabsf(ang - 10.0f * (float)j) <= P.legacy_acc;
+        agg_count(T.det, hit ? (g * 4 + 2 + m) * CLR_N_CHANNELS + j : -1);
+      }
+    }
   }
 }
 
@@ -1615,29 +1696,41 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-__device__ void fold_acc(Acc& A, LdsTally& T) {
-  const uint32_t vals[9] = {A.n_flight, A.n_null, A.n_iface, A.n_abs, A.steps, A.trials, A.rejects,
-                            A.inacc, 0u};
-  const int idx[8] = {CLR_C_N_FLIGHT, CLR_C_N_NULL, CLR_C_N_IFACE, CLR_C_N_ABS, CLR_C_N_STEPS,
-                      CLR_C_COND_TRIALS, CLR_C_COND_REJECT, CLR_C_COND_INACCESSIBLE};
+__device__ __forceinline__ void fold_acc(Acc& A, LdsTally& T) {
+  const uint32_t vals[12] = {A.n_flight, A.n_null, A.n_iface, A.n_abs, A.steps, A.trials,
+                             A.rejects, A.inacc, A.draws, A.entered, A.n_exit, A.valid_air};
+  const int idx[12] = {CLR_C_N_FLIGHT, CLR_C_N_NULL, CLR_C_N_IFACE, CLR_C_N_ABS, CLR_C_N_STEPS,
+                       CLR_C_COND_TRIALS, CLR_C_COND_REJECT, CLR_C_COND_INACCESSIBLE, CLR_C_RNG_DRAWS,
+                       CLR_C_ENTERED, CLR_C_N_EXIT, CLR_C_VALID_AIR};
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 12; ++k) {
     const uint32_t s = wave_sum(vals[k]);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&T.cnt[idx[k]], (unsigned long long)s);
+    if (__lane_id() == 0 && s) atomicAdd(&T.cnt[idx[k]], (unsigned long long)s);
   }
-  A = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  A = Acc{};
 }
 
 constexpr int kMaxBlock = 1024;
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+#ifndef CLR_PLAIN_WAVES_PER_SIMD
+#define CLR_PLAIN_WAVES_PER_SIMD 2
+#endif
 template <bool kPlain>
-__global__ __launch_bounds__(256) void transport_kernel(TransportLaunch Lp) {
+__global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : 1) void transport_kernel(TransportLaunch Lp,
+                                                        const DevParams* __restrict__ params,
+                                                        const WorkItem* __restrict__ items) {
   __shared__ LdsTally T;
   __shared__ int s_item;
   __shared__ unsigned int s_next;
   zero_lds(T);
   int cur_cfg = -1;
-  Acc A{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  Acc A{};
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1645,53 +1738,90 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportLaunch Lp) {
       s_item = it;
       s_next = 0u;
       if (it < Lp.n_items) {
-        s_key[0] = (uint32_t)Lp.items[it].seed;
-        s_key[1] = (uint32_t)(Lp.items[it].seed >> 32);
+        s_key[0] = (uint32_t)items[it].seed;
+        s_key[1] = (uint32_t)(items[it].seed >> 32);
       }
     }
     __syncthreads();
-    const int item = s_item;
+    // item fields are block-uniform: readfirstlane makes them SGPR values so
+    // every DevParams access below is a scalar (SMEM) load
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= Lp.n_items) break;
-    const WorkItem wi = Lp.items[item];
-    if (wi.config != cur_cfg) {
+    const WorkItem* wp = items + item;
+    const int cfg = __builtin_amdgcn_readfirstlane(wp->config);
+    const uint64_t w_start = uniform64(wp->start), w_count = uniform64(wp->count);
+    const uint64_t w_total = uniform64(wp->n_total), w_offset = uniform64(wp->offset);
+    if (cfg != cur_cfg) {
       if (cur_cfg >= 0) {
         fold_acc(A, T);
         flush_lds(T, Lp.tallies + cur_cfg);
       }
-      cur_cfg = wi.config;
+      cur_cfg = cfg;
     }
-    const DevParams& P = Lp.params[wi.config];
+    const DevParams& P = params[cfg];
     const uint64_t G = (uint64_t)P.n_groups;
-    clr_record* rec = (wi.config == 0) ? Lp.records : nullptr;
+    clr_record* rec = (cfg == 0) ? Lp.records : nullptr;
     Lane L;
     bool alive = false, exhausted = false;
     for (;;) {
-      if (!alive && !exhausted) {
-        const unsigned int k = atomicAdd(&s_next, 1u);
-        if ((uint64_t)k < wi.count) {
-          L.hidx = wi.start + k;
-          L.group = (int)((L.hidx * G) / wi.n_total);
-          rng_seed(L.r, wi.offset + L.hidx);
-          L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
-          L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
-          L.flags = 0u;
-          L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
-          launch_history(L, P, A);
-          alive = true;
-        } else {
-          exhausted = true;
+      // region-coherent scheduling: every pass runs ONE handler (launch,
+      // water, acrylic or headspace) for the lanes of the largest group, so
+      // a pass costs one handler instead of the union of all of them
+      const bool need = !alive && !exhausted;
+      const uint64_t m_need = __ballot(need);
+      const uint64_t m_w = __ballot(alive && L.region == R_WATER);
+      const uint64_t m_a = __ballot(alive && L.region == R_ACRYLIC);
+      const uint64_t m_h = __ballot(alive && L.region == R_HEADSPACE);
+      if (!(m_need | m_w | m_a | m_h)) break;
+      int op = R_ACRYLIC, best = __popcll(m_a);
+      if (__popcll(m_h) > best) { op = R_HEADSPACE; best = __popcll(m_h); }
+      if (__popcll(m_w) > best) { op = R_WATER; best = __popcll(m_w); }
+      if (__popcll(m_need) > best) op = 0;
+      bool done = false;
+      if (op == 0) {
+        // refill: one LDS atomic per wave hands consecutive history numbers
+        // to the lanes that need one, in lane order
+        const uint32_t n_need = (uint32_t)__popcll(m_need);
+        const int leader = __ffsll((unsigned long long)m_need) - 1;
+        uint32_t base = 0u;
+        if ((int)__lane_id() == leader) base = atomicAdd(&s_next, n_need);
+        base = __shfl(base, leader, 64);
+        const uint32_t rank = (uint32_t)__popcll(m_need & ((1ull << __lane_id()) - 1ull));
+        if (need) {
+          const unsigned int k = base + rank;
+          if ((uint64_t)k < w_count) {
+            L.hidx = w_start + k;
+            L.group = (int)((L.hidx * G) / w_total);
+            rng_seed(L.r, w_offset + L.hidx);
+            L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
+            L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
+            L.flags = 0u;
+            L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
+            launch_history(L, P, A);
+            alive = true;
+            done = (L.region == R_DONE);
+          } else {
+            exhausted = true;
+          }
+        }
+      } else if (op == R_WATER) {
+        if (alive && L.region == R_WATER) {
+          water_step<kPlain>(L, P, A);
+          done = (L.region == R_DONE);
+        }
+      } else if (op == R_ACRYLIC) {
+        if (alive && L.region == R_ACRYLIC) {
+          acrylic_step(L, P, A);
+          done = (L.region == R_DONE);
+        }
+      } else {
+        if (alive && L.region == R_HEADSPACE) {
+          headspace_step(L, P, A);
+          done = (L.region == R_DONE);
         }
       }
-      if (!__any(alive)) break;
-      if (alive) {
-        if (L.region == R_WATER) water_step<kPlain>(L, P, A);
-        else if (L.region == R_ACRYLIC) acrylic_step(L, P, A);
-        else if (L.region == R_HEADSPACE) headspace_step(L, P, A);
-        if (L.region == R_DONE) {
-          finish_history(L, P, T, rec, Lp.n_records);
-          alive = false;
-        }
-      }
+      if (__any(done)) finish_wave(L, done, P, T, A, rec, Lp.n_records);
+      if (done) alive = false;
     }
   }
   if (cur_cfg >= 0) {
@@ -1703,8 +1833,10 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportLaunch Lp) {
 }  // namespace
 
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s) {
-  if (L.plain) hipLaunchKernelGGL(transport_kernel<true>, dim3(L.grid), dim3(L.block), 0, s, L);
-  else hipLaunchKernelGGL(transport_kernel<false>, dim3(L.grid), dim3(L.block), 0, s, L);
+  if (L.plain)
+    hipLaunchKernelGGL(transport_kernel<true>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+  else
+    hipLaunchKernelGGL(transport_kernel<false>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   return hipGetLastError();
 }
 
