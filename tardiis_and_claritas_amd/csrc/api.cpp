@@ -56,6 +56,7 @@ struct Slot {
   DevBuf mu_scalars, trace_sums, sedA, sedB, sednorm, sedws, radlog, majorants;
   DevBuf radius_f, n_f, k_f, albedo_f, gcdf_f, wcdf_f;
   DevBuf field;
+  DevBuf theta_guide, gguide, wguide;  // inverse-CDF guide tables
   size_t field_len = 0;
   DevParams params{};
   double mu_scalars_h[8]{};
@@ -64,7 +65,7 @@ struct Slot {
     DevBuf* all[] = {&in_d, &in_w, &in_nr, &in_k, &q, &nstop, &theta, &theta_f, &pdf, &cdf, &cdf_f, &radius, &nd,
                      &mug, &mus, &mue, &muw, &muev, &trace_g, &trace_w, &gcdf, &wcdf, &mu_scalars,
                      &trace_sums, &sedA, &sedB, &sednorm, &sedws, &radlog, &majorants, &radius_f, &n_f,
-                     &k_f, &albedo_f, &gcdf_f, &wcdf_f, &field};
+                     &k_f, &albedo_f, &gcdf_f, &wcdf_f, &field, &theta_guide, &gguide, &wguide};
     for (DevBuf* b : all) b->release();
     ready = false;
   }
@@ -315,6 +316,9 @@ void fill_params(Slot& s, const clr_config& g) {
   P.radlog = s.radlog.as<double>();
   P.cdf = s.cdf_f.as<float>();
   P.theta = s.theta_f.as<float>();
+  P.theta_guide = s.theta_guide.as<unsigned int>();
+  P.gguide = s.gguide.as<unsigned int>();
+  P.wguide = s.wguide.as<unsigned int>();
   P.field = s.field.as<float>();
   // the reference traces the Rouse/xyz scale with the bin-sum rates
   double sums[2] = {0.0, 0.0};
@@ -606,6 +610,13 @@ int clr_prepare(clr_ctx* c, int slot, const clr_config* g, const clr_particles* 
   L.k_imag_f = s.k_f.as<float>(); L.albedo_f = s.albedo_f.as<float>(); L.geom_cdf_f = s.gcdf_f.as<float>();
   L.wave_cdf_f = s.wcdf_f.as<float>(); L.error_flag = eflag;
   HIPCHK(c, clr_launch_precompute(L, c->scratch_u32.as<unsigned int>(), c->stream));
+  if (s.nt > 65535 || n > 65535) return fail(c, CLR_E_ARG, "transport needs n_theta and the bin count <= 65535");
+  HIPCHK(c, s.theta_guide.ensure(sizeof(unsigned int) * (size_t)n * kGuideN));
+  HIPCHK(c, s.gguide.ensure(sizeof(unsigned int) * kGuideN));
+  HIPCHK(c, s.wguide.ensure(sizeof(unsigned int) * kGuideN));
+  HIPCHK(c, clr_launch_guides(s.cdf_f.as<float>(), n, s.nt, s.theta_guide.as<unsigned int>(), c->stream));
+  HIPCHK(c, clr_launch_guides(s.gcdf_f.as<float>(), 1, n, s.gguide.as<unsigned int>(), c->stream));
+  HIPCHK(c, clr_launch_guides(s.wcdf_f.as<float>(), 1, n, s.wguide.as<unsigned int>(), c->stream));
   int ecode = 0;
   HIPCHK(c, hipMemcpyAsync(&ecode, eflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.mu_scalars_h, s.mu_scalars.p, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));

@@ -25,3 +25,14 @@ struct PrepLaunch {
 
 // fmax_bits: device scratch of max(1, n) words, zeroed by the caller.
 hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, hipStream_t s);
+
+// Guide tables for inverse-CDF sampling on the transport kernel's uniform
+// u = (m + 1/2) 2^-23 (m a 23-bit integer).  Bucket j of a row covers
+// m in [j 2^(23-kGuideBits), (j+1) 2^(23-kGuideBits)); its entry packs the
+// lower-bound index of the bucket's smallest u (low 16 bits) and of its
+// largest u (high 16 bits), so a search restricted to that index range
+// returns exactly the full binary search's index.  Rows need n <= 65535.
+constexpr int kGuideBits = 11;
+constexpr int kGuideN = 1 << kGuideBits;
+// cdf: [n_rows][n] floats, out: [n_rows][kGuideN]
+hipError_t clr_launch_guides(const float* cdf, int n_rows, int n, unsigned int* out, hipStream_t s);

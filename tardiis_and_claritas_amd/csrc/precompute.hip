@@ -273,3 +273,38 @@ hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, h
   hipLaunchKernelGGL(majorant_kernel, dim3(1), dim3(1), 0, s, L, fmax_bits);
   return hipGetLastError();
 }
+
+namespace {
+// lower bound of u in a nondecreasing float row: the smallest i with
+// u <= cdf[i], or n-1 when there is none (the sampler's own search)
+__device__ int row_lower_bound(const float* __restrict__ cdf, int n, float u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+__global__ void guide_kernel(const float* __restrict__ cdf, int n_rows, int n, unsigned int* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n_rows * kGuideN) return;
+  const int row = (int)(t / kGuideN), j = (int)(t % kGuideN);
+  const float* c = cdf + (size_t)row * n;
+  const unsigned int shift = 23u - (unsigned int)kGuideBits;
+  const unsigned int m_lo = (unsigned int)j << shift, m_hi = (((unsigned int)j + 1u) << shift) - 1u;
+  // same float expression as the kernel's uniform draw
+  const float u_lo = ((float)m_lo + 0.5f) * 1.1920928955078125e-7f;
+  const float u_hi = ((float)m_hi + 0.5f) * 1.1920928955078125e-7f;
+  const int lo = row_lower_bound(c, n, u_lo), hi = row_lower_bound(c, n, u_hi);
+  out[t] = (unsigned int)lo | ((unsigned int)hi << 16);
+}
+}  // namespace
+
+hipError_t clr_launch_guides(const float* cdf, int n_rows, int n, unsigned int* out, hipStream_t s) {
+  if (n_rows <= 0 || n <= 0 || n > 65535) return hipErrorInvalidValue;
+  const size_t total = (size_t)n_rows * kGuideN;
+  const int block = 256;
+  hipLaunchKernelGGL(guide_kernel, dim3((unsigned int)((total + block - 1) / block)), dim3(block), 0, s, cdf, n_rows,
+                     n, out);
+  return hipGetLastError();
+}

@@ -40,6 +40,10 @@ struct DevParams {
   // Mie phase table (float, sampled by inverse CDF)
   const float* cdf;    // [n_bins * n_theta]
   const float* theta;  // [n_theta]
+  // guide tables of the inverse-CDF searches (layout in precompute.h)
+  const unsigned int* theta_guide;  // [n_bins * kGuideN] over cdf rows
+  const unsigned int* gguide;       // [kGuideN] over gcdf
+  const unsigned int* wguide;       // [kGuideN] over wcdf
   const float* field;  // sediment scale field or nullptr
 };
 

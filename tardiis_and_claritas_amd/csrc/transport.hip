@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "precompute.h"
 #include "transport.h"
 
 #define CLR_COLD __attribute__((noinline))
@@ -94,9 +95,8 @@ __device__ __forceinline__ uint32_t rng_next(Rng& r) {
   return v;
 }
 // (m + 1/2) 2^-23 with m the top 23 bits: exact, strictly inside (0,1)
-__device__ __forceinline__ float U(Rng& r) {
-  return ((float)(rng_next(r) >> 9) + 0.5f) * 1.1920928955078125e-7f;
-}
+__device__ __forceinline__ float u_of(uint32_t m) { return ((float)m + 0.5f) * 1.1920928955078125e-7f; }
+__device__ __forceinline__ float U(Rng& r) { return u_of(rng_next(r) >> 9); }
 __device__ __forceinline__ double U53(Rng& r) {
   const uint32_t a = rng_next(r), b = rng_next(r);
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) *
@@ -775,21 +775,31 @@ CLR_COLD __device__ double sediment_scale(const DevParams& P, int pidx, float x,
   if (!(ns > 0.0)) return 0.0;
   return exp(rl - P.sedB[pidx] * h) / ns;
 }
-__device__ __forceinline__ int sample_cdf_index(Rng& r, const float* __restrict__ cdf, int n) {
-  if (n <= 1) return 0;
-  const float u = U(r);
-  int lo = 0, hi = n - 1;
+// Lower bound of u = u_of(m) in a CDF row: the smallest i with u <= cdf[i]
+// (or n-1).  The guide entry of m's bucket brackets the answer, so the
+// bisection runs over a few entries instead of the whole row and returns
+// the same index as a full binary search.
+__device__ __forceinline__ int guided_lower_bound(const float* __restrict__ cdf,
+                                                  const unsigned int* __restrict__ guide, uint32_t m, float u) {
+  const unsigned int g = guide[m >> (23 - kGuideBits)];
+  int lo = (int)(g & 0xffffu), hi = (int)(g >> 16);
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
   }
   return lo;
 }
+__device__ __forceinline__ int sample_cdf_index(Rng& r, const float* __restrict__ cdf,
+                                                const unsigned int* __restrict__ guide, int n) {
+  if (n <= 1) return 0;
+  const uint32_t m = rng_next(r) >> 9;
+  return guided_lower_bound(cdf, guide, m, u_of(m));
+}
 __device__ __forceinline__ int select_bin(Rng& r, const DevParams& P, bool wave, float x, float y, float z,
                           float local_channel) {
   const int tm = P.sed_model;
   if (tm == CLR_SED_UNIFORM || tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD)
-    return sample_cdf_index(r, wave ? P.wcdf : P.gcdf, P.n_bins);
+    return sample_cdf_index(r, wave ? P.wcdf : P.gcdf, wave ? P.wguide : P.gguide, P.n_bins);
   if (!((double)local_channel > 0.0)) return 0;
   const double target = (double)U(r) * (double)local_channel;
   const double* rate = wave ? P.trace_w : P.trace_g;
@@ -935,12 +945,9 @@ __device__ __forceinline__ int sphere_interaction(const DevParams& P, Rng& r, fl
 __device__ __forceinline__ float sample_mie_theta(Rng& r, const DevParams& P, int pidx) {
   const int nt = P.n_theta;
   const float* __restrict__ cdf = P.cdf + (size_t)pidx * nt;
-  const float u = U(r);
-  int lo = 0, hi = nt - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
-  }
+  const uint32_t m = rng_next(r) >> 9;
+  const float u = u_of(m);
+  const int lo = guided_lower_bound(cdf, P.theta_guide + (size_t)pidx * kGuideN, m, u);
   if (lo <= 0) return P.theta[0];
   const float c0 = cdf[lo - 1], c1 = cdf[lo], t0 = P.theta[lo - 1], t1 = P.theta[lo];
   const float f = (c1 > c0) ? fminf(fmaxf((u - c0) / (c1 - c0), 0.0f), 1.0f) : 0.0f;
@@ -1075,7 +1082,10 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
 }
 
 // ------------------------------------------------------------ lane state --
-enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4 };
+// R_WATER: next free flight; R_WPART / R_WIFACE: the flight ended at a
+// particle event / at a boundary of the water (event handled by the next pass)
+enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4, R_WPART = 5, R_WIFACE = 6 };
+enum : int { EV_TOP = 1, EV_BOTTOM = 2, EV_SIDE = 3 };
 // who entered the acrylic / headspace (maps their return codes)
 enum : int { A_LAUNCH = 0, A_WATER = 1, A_BOTTOM = 2, A_HEAD = 3 };
 enum : int { H_LAUNCH = 0, H_WATERWALL = 1, H_TOP = 2, H_BOTTOM = 3 };
@@ -1085,7 +1095,7 @@ enum : unsigned { F_ENTERED = 1u, F_PER = 2u, F_PIR = 4u, F_PABS = 8u, F_CTIR = 
 struct Lane {
   float x, y, z, vx, vy, vz;
   Rng r;
-  int region, actx, hctx, it, hit, stuck, last_surf, wstep;
+  int region, actx, hctx, it, hit, stuck, last_surf, wstep, ev;
   int st, pf;
   int ic, gic, wic, wcoh, pr, cir, cor, tr, bir, bor;
   unsigned flags;
@@ -1331,9 +1341,12 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
   enter_acrylic(L, A_HEAD);
 }
 
-// one pass of the water loop, CLARITAS.py:4734-5066
+// Water loop, CLARITAS.py:4734-5066, in three passes: the free flight to the
+// next particle event or boundary (water_flight), the particle event
+// (water_particle) and the boundary interaction (water_iface).  The wave
+// schedules each kind separately so divergent lanes do not serialise them.
 template <bool kPlain>
-__device__ __forceinline__ void water_step(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A) {
   if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
   L.wstep++;
   A.steps++;
@@ -1349,133 +1362,155 @@ __device__ __forceinline__ void water_step(Lane& L, const DevParams& P, Acc& A) 
   const float tside = (tsd < 0.5 * (double)kInfF) ? (float)tsd : kInfF;
   float ttop = kInfF, tbot = kInfF;
   if (L.vz < -1e-14f) tbot = (P.zmin - L.z) / L.vz;
+  if (!(tbot > 0.0f)) tbot = kInfF;
+  const float tc = fminf(tside, tbot);  // nearer of side wall and bottom
+  const float MU_TRACK = P.mu_track;
+  // The boundary distance is min(ttop, tbot, tside) with ttop, tbot in
+  // (0, inf], so when tside > 0 and tc is finite the free path is drawn
+  // whatever the surface gives.  Drawing it first (the surface intersection
+  // uses no random numbers) lets the surface be skipped exactly when the
+  // segment up to min(free path, tc) stays below its lowest point: then the
+  // surface can neither come first nor change the outcome of the flight.
+  const bool early = (tside > 0.0f && tc < 0.5f * kInfF && MU_TRACK > 0.0f);
+  float freep = kInfF;
+  if (early) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   {
-    // exact skip: the segment to the side/bottom stays below the surface's
-    // lowest point, so no free-surface hit can come first
-    const float tc = fminf(tside, (tbot > 0.0f) ? tbot : kInfF);
+    const float d = fminf(freep, tc);
     bool skip = false;
-    if (P.fs_model != 0 && tc < 0.5f * kInfF) skip = fmaxf(L.z, L.z + tc * L.vz) < P.zs_min;
+    if (P.fs_model != 0 && d < 0.5f * kInfF) skip = fmaxf(L.z, L.z + d * L.vz) < P.zs_min;
     if (!skip) ttop = free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
   }
   if (!(ttop > 0.0f)) ttop = kInfF;
-  if (!(tbot > 0.0f)) tbot = kInfF;
   const bool axial_top = (ttop <= tbot && ttop < tside);
   const bool axial_bot = (tbot < ttop && tbot < tside);
   const float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
   if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY); return; }
-  const float MU_TOTAL = P.mu_geom + P.mu_wave;
-  const float MU_TRACK = P.mu_track;
-  float freep = kInfF;
-  if (tbound > 0.0f && MU_TRACK > 0.0f) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
+  if (!early && tbound > 0.0f && MU_TRACK > 0.0f) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
   const float travel = do_particle ? freep : tbound;
   A.n_flight++;
   L.x += travel * L.vx; L.y += travel * L.vy; L.z += travel * L.vz;
-  if (do_particle) {
-    const double er2 = (double)L.x * L.x + (double)L.y * L.y;
-    if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > free_surface_z(P, L.x, L.y) + kEpsF) {
-      terminate(L, 4, PF_INVALID_MEDIUM);
-      return;
-    }
-    float LG = P.mu_geom, LW = P.mu_wave, LT = MU_TOTAL;
-    if (P.sed_model != CLR_SED_UNIFORM) {
-      double lg = 0.0, lw = 0.0;
-      if (kPlain || P.sed_model == CLR_SED_RZ_FIELD || P.sed_model == CLR_SED_XYZ_FIELD) {
-        const double s = (P.sed_model == CLR_SED_RZ_FIELD)
-                             ? rz_sample(P.field, P.fn0, P.fn1, L.x, L.y, L.z, P.rin, P.zmin, P.field_zmax)
-                             : xyz_sample(P.field, P.fn0, P.fn1, P.fn2, L.x, L.y, L.z, P.field_half, P.zmin,
-                                          P.field_zmax);
-        lg = P.sum_trace_g * s;
-        lw = P.sum_trace_w * s;
-      } else {
-        for (int i = 0; i < P.n_bins; ++i) {
-          const double sc = sediment_scale(P, i, L.x, L.y, L.z);
-          lg += P.trace_g[i] * sc;
-          lw += P.trace_w[i] * sc;
-        }
-      }
-      LG = (float)lg; LW = (float)lw; LT = LG + LW;
-      if (!(LT > 0.0f)) { A.n_null++; return; }
-      const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
-      if (LT > MU_TRACK + slack) { terminate(L, 4, PF_MAJORANT); return; }
-      const float pacc = fminf(LT / fmaxf(MU_TRACK, 1e-30f), 1.0f);
-      if (U(L.r) >= pacc) { A.n_null++; return; }
-    }
-    L.ic++;
-    const bool use_wave = (LW > 0.0f && U(L.r) * LT >= LG);
-    const int em = P.event_model;
-    if (kPlain || em == CLR_EVENT_PLAIN_MIE) {
-      const int b = select_bin(L.r, P, false, L.x, L.y, L.z, LG);
-      A.n_abs++;
-      if (U(L.r) >= P.albedo[b]) { L.flags |= F_PABS; terminate(L, 5); return; }
-      L.wic++; L.wcoh++;
-      mie_scatter(L.r, P, b, L.vx, L.vy, L.vz);
-      return;
-    }
-    if constexpr (!kPlain) {
-    const bool cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
-    int sel = -1;
-    if (cond) sel = select_bin(L.r, P, use_wave, L.x, L.y, L.z, use_wave ? LW : LG);
-    if (cond && P.k_imag[sel] > 0.0f) {
-      CondOut h = conditional_event(P, L.r, sel, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials);
-      A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
-      if (h.status != 1 && h.status != 2) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
-      L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
-      L.pr += h.pc.pr;
-      if (h.pc.per) L.flags |= F_PER;
-      if (h.pc.pir) L.flags |= F_PIR;
-      if (h.pc.pabs) L.flags |= F_PABS;
-      if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
-      if (h.status == 2) terminate(L, 5);
-      return;
-    }
-    if (use_wave) {
-      L.wic++;
-      const int pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
-      const bool coherent = (P.coherent >= 1.0f) || (U(L.r) < P.coherent);
-      if (!coherent) { terminate(L, 4, PF_SPHEROID); return; }
-      L.wcoh++;
-      mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
-      return;
-    }
-    L.gic++;
-    const int pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
-    const float rad = P.radius[pidx];
-    float gx, gy, gz;
-    sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
-    const double dcx = (double)L.x - (double)rad * (double)gx;
-    const double dcy = (double)L.y - (double)rad * (double)gy;
-    const double dcz = (double)L.z - (double)rad * (double)gz;
-    const bool accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
-    if (!accessible && cond) {
-      A.inacc++;
-      CondOut h = conditional_event(P, L.r, pidx, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials - 1);
-      A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
-      if (h.status != 1) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
-      L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
-      L.pr += h.pc.pr;
-      if (h.pc.per) L.flags |= F_PER;
-      if (h.pc.pir) L.flags |= F_PIR;
-      if (h.pc.pabs) L.flags |= F_PABS;
-      L.gic--;
-      if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
-      return;
-    }
-    if (!accessible) { L.ic--; L.gic--; A.n_null++; return; }
-    PCount pc{0, 0, 0, 0, 0u};
-    int ppf = 0, ppfs = 0;
-    const int ps = sphere_interaction(P, L.r, P.n_water, P.n_real[pidx], P.k_imag[pidx], rad, gx, gy, gz,
-                                      dcx, dcy, dcz, L.x, L.y, L.z, L.vx, L.vy, L.vz, pc, ppf, ppfs);
-    L.pr += pc.pr;
-    if (pc.per) L.flags |= F_PER;
-    if (pc.pir) L.flags |= F_PIR;
-    if (pc.pabs) L.flags |= F_PABS;
-    A.n_abs += pc.n_abs;
-    if (ps == 2) { terminate(L, 5); return; }
-    if (ps != 1) { terminate(L, 4, ppf ? ppf : PF_OTHER); return; }
-    }
+  if (do_particle) { L.region = R_WPART; return; }
+  L.ev = axial_top ? EV_TOP : (axial_bot ? EV_BOTTOM : EV_SIDE);
+  L.region = R_WIFACE;
+}
+
+// the particle event at the end of a flight (CLARITAS.py:4774-4981)
+template <bool kPlain>
+__device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A) {
+  L.region = R_WATER;  // a terminate() below overrides
+  const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
+  const float MU_TOTAL = P.mu_geom + P.mu_wave;
+  const float MU_TRACK = P.mu_track;
+  const double er2 = (double)L.x * L.x + (double)L.y * L.y;
+  if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > free_surface_z(P, L.x, L.y) + kEpsF) {
+    terminate(L, 4, PF_INVALID_MEDIUM);
     return;
   }
+  float LG = P.mu_geom, LW = P.mu_wave, LT = MU_TOTAL;
+  if (P.sed_model != CLR_SED_UNIFORM) {
+    double lg = 0.0, lw = 0.0;
+    if (kPlain || P.sed_model == CLR_SED_RZ_FIELD || P.sed_model == CLR_SED_XYZ_FIELD) {
+      const double s = (P.sed_model == CLR_SED_RZ_FIELD)
+                           ? rz_sample(P.field, P.fn0, P.fn1, L.x, L.y, L.z, P.rin, P.zmin, P.field_zmax)
+                           : xyz_sample(P.field, P.fn0, P.fn1, P.fn2, L.x, L.y, L.z, P.field_half, P.zmin,
+                                        P.field_zmax);
+      lg = P.sum_trace_g * s;
+      lw = P.sum_trace_w * s;
+    } else {
+      for (int i = 0; i < P.n_bins; ++i) {
+        const double sc = sediment_scale(P, i, L.x, L.y, L.z);
+        lg += P.trace_g[i] * sc;
+        lw += P.trace_w[i] * sc;
+      }
+    }
+    LG = (float)lg; LW = (float)lw; LT = LG + LW;
+    if (!(LT > 0.0f)) { A.n_null++; return; }
+    const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
+    if (LT > MU_TRACK + slack) { terminate(L, 4, PF_MAJORANT); return; }
+    const float pacc = fminf(LT / fmaxf(MU_TRACK, 1e-30f), 1.0f);
+    if (U(L.r) >= pacc) { A.n_null++; return; }
+  }
+  L.ic++;
+  const bool use_wave = (LW > 0.0f && U(L.r) * LT >= LG);
+  const int em = P.event_model;
+  if (kPlain || em == CLR_EVENT_PLAIN_MIE) {
+    const int b = select_bin(L.r, P, false, L.x, L.y, L.z, LG);
+    A.n_abs++;
+    if (U(L.r) >= P.albedo[b]) { L.flags |= F_PABS; terminate(L, 5); return; }
+    L.wic++; L.wcoh++;
+    mie_scatter(L.r, P, b, L.vx, L.vy, L.vz);
+    return;
+  }
+  if constexpr (!kPlain) {
+  const bool cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
+  int sel = -1;
+  if (cond) sel = select_bin(L.r, P, use_wave, L.x, L.y, L.z, use_wave ? LW : LG);
+  if (cond && P.k_imag[sel] > 0.0f) {
+    CondOut h = conditional_event(P, L.r, sel, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials);
+    A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+    if (h.status != 1 && h.status != 2) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
+    L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
+    L.pr += h.pc.pr;
+    if (h.pc.per) L.flags |= F_PER;
+    if (h.pc.pir) L.flags |= F_PIR;
+    if (h.pc.pabs) L.flags |= F_PABS;
+    if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+    if (h.status == 2) terminate(L, 5);
+    return;
+  }
+  if (use_wave) {
+    L.wic++;
+    const int pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
+    const bool coherent = (P.coherent >= 1.0f) || (U(L.r) < P.coherent);
+    if (!coherent) { terminate(L, 4, PF_SPHEROID); return; }
+    L.wcoh++;
+    mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
+    return;
+  }
+  L.gic++;
+  const int pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
+  const float rad = P.radius[pidx];
+  float gx, gy, gz;
+  sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
+  const double dcx = (double)L.x - (double)rad * (double)gx;
+  const double dcy = (double)L.y - (double)rad * (double)gy;
+  const double dcz = (double)L.z - (double)rad * (double)gz;
+  const bool accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
+  if (!accessible && cond) {
+    A.inacc++;
+    CondOut h = conditional_event(P, L.r, pidx, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials - 1);
+    A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+    if (h.status != 1) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
+    L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
+    L.pr += h.pc.pr;
+    if (h.pc.per) L.flags |= F_PER;
+    if (h.pc.pir) L.flags |= F_PIR;
+    if (h.pc.pabs) L.flags |= F_PABS;
+    L.gic--;
+    if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+    return;
+  }
+  if (!accessible) { L.ic--; L.gic--; A.n_null++; return; }
+  PCount pc{0, 0, 0, 0, 0u};
+  int ppf = 0, ppfs = 0;
+  const int ps = sphere_interaction(P, L.r, P.n_water, P.n_real[pidx], P.k_imag[pidx], rad, gx, gy, gz,
+                                    dcx, dcy, dcz, L.x, L.y, L.z, L.vx, L.vy, L.vz, pc, ppf, ppfs);
+  L.pr += pc.pr;
+  if (pc.per) L.flags |= F_PER;
+  if (pc.pir) L.flags |= F_PIR;
+  if (pc.pabs) L.flags |= F_PABS;
+  A.n_abs += pc.n_abs;
+  if (ps == 2) { terminate(L, 5); return; }
+  if (ps != 1) { terminate(L, 4, ppf ? ppf : PF_OTHER); return; }
+  }
+  return;
+}
+
+// the boundary interaction at the end of a flight (CLARITAS.py:4982-5066)
+__device__ __forceinline__ void water_iface(Lane& L, const DevParams& P, Acc& A) {
+  L.region = R_WATER;  // reflections stay in the water; transmissions set the region
+  const bool axial_top = (L.ev == EV_TOP), axial_bot = (L.ev == EV_BOTTOM);
   bool tir = false;
   A.n_iface++;
   if (axial_top) {
@@ -1770,11 +1805,15 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : 1) void tr
       const bool need = !alive && !exhausted;
       const uint64_t m_need = __ballot(need);
       const uint64_t m_w = __ballot(alive && L.region == R_WATER);
+      const uint64_t m_p = __ballot(alive && L.region == R_WPART);
+      const uint64_t m_i = __ballot(alive && L.region == R_WIFACE);
       const uint64_t m_a = __ballot(alive && L.region == R_ACRYLIC);
       const uint64_t m_h = __ballot(alive && L.region == R_HEADSPACE);
-      if (!(m_need | m_w | m_a | m_h)) break;
+      if (!(m_need | m_w | m_p | m_i | m_a | m_h)) break;
       int op = R_ACRYLIC, best = __popcll(m_a);
       if (__popcll(m_h) > best) { op = R_HEADSPACE; best = __popcll(m_h); }
+      if (__popcll(m_i) > best) { op = R_WIFACE; best = __popcll(m_i); }
+      if (__popcll(m_p) > best) { op = R_WPART; best = __popcll(m_p); }
       if (__popcll(m_w) > best) { op = R_WATER; best = __popcll(m_w); }
       if (__popcll(m_need) > best) op = 0;
       bool done = false;
@@ -1806,7 +1845,17 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : 1) void tr
         }
       } else if (op == R_WATER) {
         if (alive && L.region == R_WATER) {
-          water_step<kPlain>(L, P, A);
+          water_flight<kPlain>(L, P, A);
+          done = (L.region == R_DONE);
+        }
+      } else if (op == R_WPART) {
+        if (alive && L.region == R_WPART) {
+          water_particle<kPlain>(L, P, A);
+          done = (L.region == R_DONE);
+        }
+      } else if (op == R_WIFACE) {
+        if (alive && L.region == R_WIFACE) {
+          water_iface(L, P, A);
           done = (L.region == R_DONE);
         }
       } else if (op == R_ACRYLIC) {
