@@ -1086,6 +1086,8 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
 // particle event / at a boundary of the water (event handled by the next pass)
 enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4, R_WPART = 5, R_WIFACE = 6 };
 enum : int { EV_TOP = 1, EV_BOTTOM = 2, EV_SIDE = 3 };
+// kinds of scheduler pass
+enum : int { K_RECYCLE = 0, K_WATER = 1, K_IFACE = 2, K_ACRYLIC = 3, K_HEAD = 4, K_N = 5 };
 // who entered the acrylic / headspace (maps their return codes)
 enum : int { A_LAUNCH = 0, A_WATER = 1, A_BOTTOM = 2, A_HEAD = 3 };
 enum : int { H_LAUNCH = 0, H_WATERWALL = 1, H_TOP = 2, H_BOTTOM = 3 };
@@ -1571,45 +1573,17 @@ __device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
   if (v) atomicAdd(p, (unsigned long long)v);
 }
 
-// add 1 to base[key] for every lane with key >= 0, one LDS atomic per
-// distinct key in the wave (all lanes of the wave must call this)
-__device__ __forceinline__ void agg_count(uint32_t* base, int key) {
-  uint64_t pending = __ballot(key >= 0);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int k = __shfl(key, leader, 64);
-    const uint64_t same = __ballot(key == k);
-    if ((int)__lane_id() == leader) atomicAdd(base + k, (uint32_t)__popcll(same));
-    pending &= ~same;
-  }
-}
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ __forceinline__ void agg_sum_f(float* base, int key, float v) {
-  uint64_t pending = __ballot(key >= 0);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int k = __shfl(key, leader, 64);
-    const bool mine = (key == k);
-    const float s = wave_sum_f(mine ? v : 0.0f);
-    if ((int)__lane_id() == leader) atomicAdd(base + k, s);
-    pending &= ~__ballot(mine);
-  }
-}
-
-// Score the histories that ended this step (CLARITAS.py:4397-4435) and fold
-// their per-ray reductions (CLARITAS.py:8659-8733) into LDS.  Called by every
-// lane of the wave; `done` marks the lanes whose history ended.
-__device__ __forceinline__ void finish_wave(Lane& L, bool done, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
-                            uint64_t n_rec) {
+// Score a history that has ended (CLARITAS.py:4397-4435) and fold its
+// per-ray reductions (CLARITAS.py:8659-8733) into the workgroup's LDS
+// tallies.  Lanes of one wave that hit the same bin are serialised by the
+// LDS atomic unit, which is cheaper than aggregating them in software.
+__device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
+                                            uint64_t n_rec) {
   const int g = L.group;
   int nat = -1, mir = -1, aux_bits = 0;
   bool scored = false;
   float inc_n = 0.0f, inc_m = 0.0f, ang_n = -1.0f, ang_m = -1.0f;
-  if (done) {
+  {
     A.draws += L.r.draws;
     if (L.flags & F_ENTERED) A.entered++;
     if (L.st == 1) {
@@ -1664,33 +1638,34 @@ __device__ __forceinline__ void finish_wave(Lane& L, bool done, const DevParams&
       rec[L.hidx] = q;
     }
   }
-  // wave-aggregated tallies: lanes ending together share one atomic per key
-  agg_count(T.status, done ? L.st : -1);
-  agg_count(T.fail, (done && L.st == 4) ? (L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER) : -1);
-  const bool ent = done && (L.flags & F_ENTERED);
-  agg_count(T.ich, ent ? (L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1) : -1);
-  agg_count(T.det, nat >= 0 ? (g * 4 + 0) * CLR_N_CHANNELS + nat : -1);
-  agg_count(T.det, mir >= 0 ? (g * 4 + 1) * CLR_N_CHANNELS + mir : -1);
-  agg_sum_f(T.inc, nat >= 0 ? nat : -1, inc_n);
-  agg_sum_f(T.inc, mir >= 0 ? CLR_N_CHANNELS + mir : -1, inc_m);
-  if (__any(aux_bits != 0)) {
+  atomicAdd(&T.status[L.st], 1u);
+  if (L.st == 4) atomicAdd(&T.fail[L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER], 1u);
+  if (L.flags & F_ENTERED) atomicAdd(&T.ich[L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1], 1u);
+  if (nat >= 0) {
+    atomicAdd(&T.det[(g * 4 + 0) * CLR_N_CHANNELS + nat], 1u);
+    atomicAdd(&T.inc[nat], inc_n);
+  }
+  if (mir >= 0) {
+    atomicAdd(&T.det[(g * 4 + 1) * CLR_N_CHANNELS + mir], 1u);
+    atomicAdd(&T.inc[CLR_N_CHANNELS + mir], inc_m);
+  }
+  if (aux_bits) {
     for (int a = 0; a < CLR_MAX_AUX; ++a) {
-      agg_count(T.aux, (aux_bits & (1 << a)) ? (g * 2 + 0) * CLR_MAX_AUX + a : -1);
-      agg_count(T.aux, (aux_bits & (16 << a)) ? (g * 2 + 1) * CLR_MAX_AUX + a : -1);
+      if (aux_bits & (1 << a)) atomicAdd(&T.aux[(g * 2 + 0) * CLR_MAX_AUX + a], 1u);
+      if (aux_bits & (16 << a)) atomicAdd(&T.aux[(g * 2 + 1) * CLR_MAX_AUX + a], 1u);
     }
   }
   // legacy exit-position windows |angle - 10 j| <= acceptance; with the
   // acceptance capped at 15 deg only j within +-2 of angle/10 can match
-  if (__any(scored)) {
+  if (scored) {
     for (int m = 0; m < 2; ++m) {
       const float ang = m ? ang_m : ang_n;
-      const bool valid = scored && !(ang < 0.0f || ang > 180.0f);
-      const int j0 = valid ? (int)rintf(ang * 0.1f) : 0;
+      if (ang < 0.0f || ang > 180.0f) continue;
+      const int j0 = (int)rintf(ang * 0.1f);
       for (int k = -2; k <= 2; ++k) {
         const int j = j0 + k;
-        const bool hit = valid && j >= 0 && j < CLR_N_CHANNELS && !(L.ic <= 0 && j >= 9) &&
-                         fabsf(ang - 10.0f * (float)j) <= P.legacy_acc;
-        agg_count(T.det, hit ? (g * 4 + 2 + m) * CLR_N_CHANNELS + j : -1);
+        if (j >= 0 && j < CLR_N_CHANNELS && !(L.ic <= 0 && j >= 9) && fabsf(ang - 10.0f * (float)j) <= P.legacy_acc)
+          atomicAdd(&T.det[(g * 4 + 2 + m) * CLR_N_CHANNELS + j], 1u);
       }
     }
   }
@@ -1753,17 +1728,27 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Occupancy targets (waves per SIMD).  The kernel is latency bound (long
+// dependent FP64/transcendental chains and table loads), so the register
+// budget is capped to keep more waves resident: a few spills cost less than
+// the latency that one more wave per SIMD hides.
 #ifndef CLR_PLAIN_WAVES_PER_SIMD
-#define CLR_PLAIN_WAVES_PER_SIMD 2
+#define CLR_PLAIN_WAVES_PER_SIMD 3
+#endif
+#ifndef CLR_GENERIC_WAVES_PER_SIMD
+#define CLR_GENERIC_WAVES_PER_SIMD 2
 #endif
 template <bool kPlain>
-__global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : 1) void transport_kernel(TransportLaunch Lp,
+__global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERIC_WAVES_PER_SIMD) void transport_kernel(TransportLaunch Lp,
                                                         const DevParams* __restrict__ params,
                                                         const WorkItem* __restrict__ items) {
   __shared__ LdsTally T;
   __shared__ int s_item;
   __shared__ unsigned int s_next;
   zero_lds(T);
+  // per-wave pass-cost estimates in cycles (initial values are typical
+  // costs; they adapt to the configuration within a few passes)
+  uint64_t cost[K_N] = {12000, 12000, 3000, 6000, 12000};
   int cur_cfg = -1;
   Acc A{};
   for (;;) {
@@ -1799,78 +1784,75 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : 1) void tr
     Lane L;
     bool alive = false, exhausted = false;
     for (;;) {
-      // region-coherent scheduling: every pass runs ONE handler (launch,
-      // water, acrylic or headspace) for the lanes of the largest group, so
-      // a pass costs one handler instead of the union of all of them
-      const bool need = !alive && !exhausted;
-      const uint64_t m_need = __ballot(need);
-      const uint64_t m_w = __ballot(alive && L.region == R_WATER);
-      const uint64_t m_p = __ballot(alive && L.region == R_WPART);
+      // Region-coherent scheduling: each pass runs one handler for one group
+      // of lanes in the wave.  A pass costs about the same however many
+      // lanes are active, so handlers that a lane runs back to back share a
+      // pass (RECYCLE scores finished histories and starts new ones; WATER
+      // runs a pending particle event, then the next flight), and the group
+      // chosen is the one with the most lanes per cycle of its handler's
+      // measured pass cost.
+      const bool fin = alive && L.region == R_DONE;
+      const uint64_t m_rc = __ballot(fin || (!alive && !exhausted));
+      const uint64_t m_w = __ballot(alive && (L.region == R_WATER || L.region == R_WPART));
       const uint64_t m_i = __ballot(alive && L.region == R_WIFACE);
       const uint64_t m_a = __ballot(alive && L.region == R_ACRYLIC);
       const uint64_t m_h = __ballot(alive && L.region == R_HEADSPACE);
-      if (!(m_need | m_w | m_p | m_i | m_a | m_h)) break;
-      int op = R_ACRYLIC, best = __popcll(m_a);
-      if (__popcll(m_h) > best) { op = R_HEADSPACE; best = __popcll(m_h); }
-      if (__popcll(m_i) > best) { op = R_WIFACE; best = __popcll(m_i); }
-      if (__popcll(m_p) > best) { op = R_WPART; best = __popcll(m_p); }
-      if (__popcll(m_w) > best) { op = R_WATER; best = __popcll(m_w); }
-      if (__popcll(m_need) > best) op = 0;
-      bool done = false;
-      if (op == 0) {
+      if (!(m_rc | m_w | m_i | m_a | m_h)) break;
+      int op = K_ACRYLIC;
+      uint64_t bn = (uint64_t)__popcll(m_a), bc = cost[K_ACRYLIC];
+      const uint64_t cnt[K_N] = {(uint64_t)__popcll(m_rc), (uint64_t)__popcll(m_w), (uint64_t)__popcll(m_i),
+                                 (uint64_t)__popcll(m_a), (uint64_t)__popcll(m_h)};
+#pragma unroll
+      for (int k = 0; k < K_N; ++k)
+        if (cnt[k] * bc > bn * cost[k]) { op = k; bn = cnt[k]; bc = cost[k]; }
+      const uint64_t t_pass = __builtin_amdgcn_s_memtime();
+      if (op == K_RECYCLE) {
+        if (fin) {
+          finish_lane(L, P, T, A, rec, Lp.n_records);
+          alive = false;
+        }
         // refill: one LDS atomic per wave hands consecutive history numbers
         // to the lanes that need one, in lane order
-        const uint32_t n_need = (uint32_t)__popcll(m_need);
-        const int leader = __ffsll((unsigned long long)m_need) - 1;
-        uint32_t base = 0u;
-        if ((int)__lane_id() == leader) base = atomicAdd(&s_next, n_need);
-        base = __shfl(base, leader, 64);
-        const uint32_t rank = (uint32_t)__popcll(m_need & ((1ull << __lane_id()) - 1ull));
-        if (need) {
-          const unsigned int k = base + rank;
-          if ((uint64_t)k < w_count) {
-            L.hidx = w_start + k;
-            L.group = (int)((L.hidx * G) / w_total);
-            rng_seed(L.r, w_offset + L.hidx);
-            L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
-            L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
-            L.flags = 0u;
-            L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
-            launch_history(L, P, A);
-            alive = true;
-            done = (L.region == R_DONE);
-          } else {
-            exhausted = true;
+        const bool need = !alive && !exhausted;
+        const uint64_t m_need = __ballot(need);
+        if (m_need) {
+          const uint32_t n_need = (uint32_t)__popcll(m_need);
+          const int leader = __ffsll((unsigned long long)m_need) - 1;
+          uint32_t base = 0u;
+          if ((int)__lane_id() == leader) base = atomicAdd(&s_next, n_need);
+          base = __shfl(base, leader, 64);
+          const uint32_t rank = (uint32_t)__popcll(m_need & ((1ull << __lane_id()) - 1ull));
+          if (need) {
+            const unsigned int k = base + rank;
+            if ((uint64_t)k < w_count) {
+              L.hidx = w_start + k;
+              L.group = (int)((L.hidx * G) / w_total);
+              rng_seed(L.r, w_offset + L.hidx);
+              L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
+              L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
+              L.flags = 0u;
+              L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
+              launch_history(L, P, A);  // may end it at once: scored by a later RECYCLE pass
+              alive = true;
+            } else {
+              exhausted = true;
+            }
           }
         }
-      } else if (op == R_WATER) {
-        if (alive && L.region == R_WATER) {
-          water_flight<kPlain>(L, P, A);
-          done = (L.region == R_DONE);
-        }
-      } else if (op == R_WPART) {
-        if (alive && L.region == R_WPART) {
-          water_particle<kPlain>(L, P, A);
-          done = (L.region == R_DONE);
-        }
-      } else if (op == R_WIFACE) {
-        if (alive && L.region == R_WIFACE) {
-          water_iface(L, P, A);
-          done = (L.region == R_DONE);
-        }
-      } else if (op == R_ACRYLIC) {
-        if (alive && L.region == R_ACRYLIC) {
-          acrylic_step(L, P, A);
-          done = (L.region == R_DONE);
-        }
+      } else if (op == K_WATER) {
+        if (alive && L.region == R_WPART) water_particle<kPlain>(L, P, A);
+        if (alive && L.region == R_WATER) water_flight<kPlain>(L, P, A);
+      } else if (op == K_IFACE) {
+        if (alive && L.region == R_WIFACE) water_iface(L, P, A);
+      } else if (op == K_ACRYLIC) {
+        if (alive && L.region == R_ACRYLIC) acrylic_step(L, P, A);
       } else {
-        if (alive && L.region == R_HEADSPACE) {
-          headspace_step(L, P, A);
-          done = (L.region == R_DONE);
-        }
+        if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
       }
-      if (__any(done)) finish_wave(L, done, P, T, A, rec, Lp.n_records);
-      if (done) alive = false;
+      // running estimate of the pass cost (cycles, 1/8 weight to the new pass)
+      const int64_t dt = (int64_t)(__builtin_amdgcn_s_memtime() - t_pass);
+      const int64_t c0 = (int64_t)cost[op];
+      cost[op] = (uint64_t)max(c0 + ((min(dt, (int64_t)(1 << 24)) - c0) >> 3), (int64_t)64);
     }
   }
   if (cur_cfg >= 0) {
