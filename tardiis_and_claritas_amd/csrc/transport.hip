@@ -36,6 +36,7 @@ constexpr float kEpsF = 2.0e-7f;
 constexpr double kDEps = 2.220446049250313080847263336181640625e-16;
 constexpr double kFEps = 1.1920928955078125e-7;
 constexpr int kEmergencyCap = 16384;  // acrylic / headspace loop caps, CLARITAS.py:3147,3244
+constexpr int kLongChain = 128;       // acrylic bounces after which a visit is stepped to its end
 
 // production failure codes and subcodes, CLARITAS.py:1846-1895
 enum : int {
@@ -73,8 +74,10 @@ __device__ __forceinline__ void philox_refill(Rng& r) {
   uint32_t c0 = r.blk, c1 = 0u, c2 = r.h0, c3 = r.h1, k0 = s_key[0], k1 = s_key[1];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    // one 32x32->64 product per multiplier (a single v_mad_u64_u32)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
@@ -302,6 +305,15 @@ __device__ __forceinline__ float free_surface_z(const DevParams& P, float x, flo
   const float H = P.dh * (a2 + R2) / R2;
   const float hbar = H * (1.0f - (a2 / R2) * log1pf(R2 / a2));
   return P.zmean - hbar + H * q / (a2 + q);
+}
+// Free-surface height for a comparison with z: when z is below the surface's
+// lowest point (zs_min, a lower bound of every device-evaluated height) the
+// comparison's outcome is known and zs_min is returned without evaluating
+// the surface (an FP64 division for the Rankine vortex).
+__device__ __forceinline__ float surface_z_vs(const DevParams& P, float x, float y, float z) {
+  float zs = P.zs_min;
+  if (!(z < P.zs_min)) zs = free_surface_z(P, x, y);
+  return zs;
 }
 __device__ double quadratic_min_positive(double A, double B, double C) {
   if (fabs(A) < 1e-24) {
@@ -1234,7 +1246,7 @@ __device__ __forceinline__ void acrylic_step(Lane& L, const DevParams& P, Acc& A
     float nx, ny;
     radial_normal(L.x, L.y, nx, ny);
     if (s == 1) {
-      const float zs = free_surface_z(P, L.x, L.y);
+      const float zs = surface_z_vs(P, L.x, L.y, L.z);
       const float n_in = (L.z <= zs + kEpsF) ? P.n_water : P.n_air;
       if (fresnel_transmit(L.r, L.vx, L.vy, L.vz, nx, ny, 0.0f, P.n_acrylic, n_in, tir)) {
         // normal convention of the reference: +radial incident side here
@@ -1354,7 +1366,7 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   A.steps++;
   if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) { terminate(L, 4, PF_NONFINITE_POSITION); return; }
   if (!isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) { terminate(L, 4, PF_NONFINITE_DIRECTION); return; }
-  const float zsh = free_surface_z(P, L.x, L.y);
+  const float zsh = surface_z_vs(P, L.x, L.y, L.z);
   if (L.z <= P.zmin || L.z >= zsh) { terminate(L, 2); return; }
   const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
   const double rr2 = (double)L.x * (double)L.x + (double)L.y * (double)L.y;
@@ -1405,7 +1417,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   const float MU_TOTAL = P.mu_geom + P.mu_wave;
   const float MU_TRACK = P.mu_track;
   const double er2 = (double)L.x * L.x + (double)L.y * L.y;
-  if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > free_surface_z(P, L.x, L.y) + kEpsF) {
+  if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > surface_z_vs(P, L.x, L.y, L.z) + kEpsF) {
     terminate(L, 4, PF_INVALID_MEDIUM);
     return;
   }
@@ -1846,6 +1858,15 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
       } else if (op == K_ACRYLIC) {
         if (alive && L.region == R_ACRYLIC) acrylic_step(L, P, A);
+        // A chain of kLongChain bounces in one acrylic visit is almost always
+        // trapped by total internal reflection and runs on to the emergency
+        // cap; stepping it to the end now (the rest of the wave waits) keeps
+        // such a chain from trailing past the end of the launch.  The same
+        // applies when acrylic lanes are all this wave has left.
+        const bool only = !(m_rc | m_w | m_i | m_h);
+        while (__any(alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain))) {
+          if (alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain)) acrylic_step(L, P, A);
+        }
       } else {
         if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
       }
