@@ -25,6 +25,33 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+TRAFFIC_PROFILE = ROOT / 'profiles' / 'transport_kernel_config2_hbm_traffic.txt'
+
+
+def traffic_per_history():
+    """HBM bytes per history of the transport kernel on this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter
+    scaled by its reading on a plain copy of known size); None if absent."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith('hbm_bytes_per_history:'):
+                return float(line.split(':', 1)[1].split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of `mib` MiB (3 times), the known-size
+    reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_calibration_bytes': n * 4, 'repeats': 3}))
 
 
 def cpu_baseline(histories_per_conc: int, seed: int):
@@ -74,7 +101,12 @@ def main():
     ap.add_argument('--block', type=int, default=0)
     ap.add_argument('--blocks-per-cu', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=0)
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only run a plain device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is '
                                    'described in DESIGN.md, not run)'}))
@@ -118,6 +150,7 @@ def main():
     n_hist = args.steps * H * len(specs)
     value = n_hist / elapsed
     avg_ms = float(np.mean(kernel_ms))
+    tph = traffic_per_history()
     achieved = flops / args.steps / (avg_ms * 1e-3) / 1e12
     out = {
         'metric': 'photon histories/s (1 MI355X) + surrogate-vs-MC speedup at <=1% NTU error',
@@ -146,7 +179,7 @@ def main():
             'peak': FP32_PEAK_TFLOPS,
             'unit': 'TFLOP/s',
             'frac': achieved / FP32_PEAK_TFLOPS,
-            'traffic': None,
+            'traffic': (tph * H * len(specs)) if tph else None,
             'kernel': 'transport_kernel<true> (plain-Mie persistent kernel)',
             'kernel_ms_avg': avg_ms,
             'flop_model': 'F = 16 + 30 N_flight + 16 N_null + 60 N_scat + 40 N_iface + 10 N_abs + 45 N_exit '
@@ -155,7 +188,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        per = args.cpu_histories or max(20000, int(2e6))
+        per = args.cpu_histories or int(2e7)
         out['cpu_baseline'] = cpu_baseline(per, seed)
     print(json.dumps(out))
 
