@@ -1,0 +1,105 @@
+"""GPU checks of the surrogate ensemble kernels (csrc/mlp.hip) through the C ABI.
+
+Gradients and forward passes of the BF16-MFMA kernels are compared with the
+CPU oracle (oracle/mlp_oracle.py) evaluated on the same parameters with the
+same BF16 rounding points; sums differ only by FP32-vs-FP64 accumulation and
+rare BF16 tie flips, so the bars are relative norms of 5e-3 per tensor.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx():
+    from tardiis_and_claritas_amd._lib import Context
+    return Context(0)
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _tensors(shape, flat):
+    """[(name, array)] per member and layer for error reports."""
+    out = []
+    for m, layers in enumerate(shape.unpack(flat)):
+        for l, (W, b) in enumerate(layers):
+            out += [(f'm{m} W{l}', W), (f'm{m} b{l}', b)]
+    return out
+
+
+def test_params_roundtrip(gpu_available):
+    _flush('surrogate params roundtrip')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=2, hidden=64, n_hidden_layers=2, seed=3)
+    p = ens.get_params()
+    assert p.size == 2 * mo.Shape(2, 8, 64, 3, 2).per_member
+    q = np.random.default_rng(0).normal(0, 0.1, p.size).astype(np.float32)
+    ens.set_params(q)
+    np.testing.assert_array_equal(ens.get_params(), q)
+
+
+@pytest.mark.parametrize('n', [100, 256])
+def test_gradients_match_oracle(gpu_available, n):
+    _flush(f'surrogate gradient check n={n}')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=3, hidden=256, n_hidden_layers=3, seed=11)
+    shape = mo.Shape(3, 8, 256, 3, 3)
+    g = np.random.default_rng(n)
+    x = g.normal(size=(n, 8)).astype(np.float32)
+    y = g.normal(size=(n, 3)).astype(np.float32)
+    loss, grad = ens.grad(x, y)
+    flat = ens.get_params()
+    loss_o, grad_o = mo.ensemble_loss_grad(shape, flat, x.astype(np.float64), y.astype(np.float64), bf16=True)
+    np.testing.assert_allclose(loss, loss_o, rtol=1e-3)
+    for (name, a), (_, b) in zip(_tensors(shape, grad), _tensors(shape, grad_o)):
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert err < 5e-3, f'{name}: relative gradient error {err:.2e}'
+
+
+def test_forward_matches_oracle(gpu_available):
+    _flush('surrogate forward check')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=5, hidden=256, n_hidden_layers=3, seed=5)
+    shape = mo.Shape(5, 8, 256, 3, 3)
+    x = np.random.default_rng(9).normal(size=(1000, 8)).astype(np.float32)
+    mean, std = ens.predict(x)
+    mean_o, std_o = mo.ensemble_predict(shape, ens.get_params(), x.astype(np.float64), bf16=True)
+    scale = np.abs(mean_o).max()
+    assert np.abs(mean - mean_o).max() <= 2e-3 * scale
+    assert np.abs(std - std_o).max() <= 2e-3 * scale
+
+
+def test_training_fits_smooth_function(gpu_available):
+    _flush('surrogate training on a smooth function')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=5, hidden=256, n_hidden_layers=3, seed=1)
+    g = np.random.default_rng(2)
+    x = g.uniform(-1, 1, size=(16384, 8)).astype(np.float32)
+
+    def f(x):
+        return np.stack([np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2], np.exp(0.5 * x[:, 3]) - x[:, 4] ** 2,
+                         np.tanh(x[:, 5] + x[:, 6] - x[:, 7])], axis=1).astype(np.float32)
+
+    y = f(x)
+    first = ens.train(x, y, batch=512, epochs=1, lr=2e-3, seed=0)
+    ens.train(x, y, batch=512, epochs=20, lr=1e-3, seed=1)
+    last = ens.train(x, y, batch=512, epochs=10, lr=2e-4, seed=2)
+    assert np.all(last < 0.02 * first), (first, last)
+    xt = g.uniform(-1, 1, size=(2048, 8)).astype(np.float32)
+    mean, std = ens.predict(xt)
+    rmse = np.sqrt(np.mean((mean - f(xt)) ** 2))
+    assert rmse < 0.1 * np.std(f(xt)), rmse
+    # spread of independently initialised members is small where data is dense
+    assert np.median(std) < 0.1 * np.std(f(xt))
+
+
+def test_inference_timing(gpu_available):
+    _flush('surrogate inference timing')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=5, hidden=256, n_hidden_layers=3, seed=0)
+    ms = ens.time_infer(65536, 5)
+    assert 0.0 < ms < 1000.0
