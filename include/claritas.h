@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 1
+#define CLR_ABI_VERSION 2
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -212,10 +212,15 @@ typedef struct clr_tables_out {
 /* ---- tallies for one configuration ----
  * det[g][k][c]: k = 0 hardware native, 1 hardware mirror, 2 legacy native,
  * 3 legacy mirror; channel c = detector at 10*c degrees (CLARITAS.py:4378-4436).
- * aux[g][k][a]: k = 0 native, 1 mirror, extra bore at aux_angles_deg[a]. */
+ * aux[g][k][a]: k = 0 native, 1 mirror, extra bore at aux_angles_deg[a];
+ * aux_window[g][k][a]: legacy exit-position window at aux_angles_deg[a]. */
 typedef struct clr_tallies {
   uint64_t det[CLR_MAX_GROUPS][4][CLR_N_CHANNELS];
   uint64_t aux[CLR_MAX_GROUPS][2][CLR_MAX_AUX];
+  /* exit-position windows |angle - aux_angles_deg[a]| <= legacy_accept_deg
+   * (native, mirror), the legacy rule of CLARITAS.py:4426-4435 applied at the
+   * extra angles (ballistic histories skip windows at >= 90 deg) */
+  uint64_t aux_window[CLR_MAX_GROUPS][2][CLR_MAX_AUX];
   uint64_t group_histories[CLR_MAX_GROUPS];
   double incidence_sum[2][CLR_N_CHANNELS];   /* degrees, native / mirror */
   uint64_t status[CLR_N_STATUS];

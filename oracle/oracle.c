@@ -320,7 +320,7 @@ struct orc_model {
   float* field;
   size_t field_len;
   float det_ux[CLR_N_CHANNELS], det_uy[CLR_N_CHANNELS];
-  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX];
+  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX], aux_deg[CLR_MAX_AUX];
   float n_f[512], k_f[512], r_f[512];
 };
 
@@ -1737,6 +1737,7 @@ typedef struct {
   uint32_t draws;
   int native, mirror;
   int aux_native[CLR_MAX_AUX], aux_mirror[CLR_MAX_AUX];
+  int auxw_native[CLR_MAX_AUX], auxw_mirror[CLR_MAX_AUX];
   int legacy_native[CLR_N_CHANNELS], legacy_mirror[CLR_N_CHANNELS];
   float inc_native, inc_mirror;
 } hist_t;
@@ -2069,6 +2070,14 @@ AIR_EXIT:
             if (mir) o->legacy_mirror[j] = 1; else o->legacy_native[j] = 1;
           }
         }
+        /* the same window rule at the auxiliary angles (extension) */
+        for (int a = 0; a < P->n_aux; ++a) {
+          float deg = m->aux_deg[a];
+          if (o->ic <= 0 && deg >= 90.0f) continue;
+          if (fabsf(ang - deg) <= P->legacy_acc) {
+            if (mir) o->auxw_mirror[a] = 1; else o->auxw_native[a] = 1;
+          }
+        }
       }
     }
   }
@@ -2134,6 +2143,8 @@ static void tally(clr_tallies* T, const hist_t* h, int g) {
   for (int a = 0; a < CLR_MAX_AUX; ++a) {
     if (h->aux_native[a] > 0) T->aux[g][0][a]++;
     if (h->aux_mirror[a] > 0) T->aux[g][1][a]++;
+    if (h->auxw_native[a] > 0) T->aux_window[g][0][a]++;
+    if (h->auxw_mirror[a] > 0) T->aux_window[g][1][a]++;
   }
 }
 
@@ -2143,6 +2154,8 @@ static void merge(clr_tallies* D, const clr_tallies* S) {
   for (size_t i = 0; i < sizeof(S->det) / 8; ++i) d[i] += s[i];
   s = (const uint64_t*)S->aux; d = (uint64_t*)D->aux;
   for (size_t i = 0; i < sizeof(S->aux) / 8; ++i) d[i] += s[i];
+  s = (const uint64_t*)S->aux_window; d = (uint64_t*)D->aux_window;
+  for (size_t i = 0; i < sizeof(S->aux_window) / 8; ++i) d[i] += s[i];
   for (int g = 0; g < CLR_MAX_GROUPS; ++g) D->group_histories[g] += S->group_histories[g];
   for (int k = 0; k < 2; ++k)
     for (int c = 0; c < CLR_N_CHANNELS; ++c) D->incidence_sum[k][c] += S->incidence_sum[k][c];
@@ -2240,6 +2253,7 @@ static void fill_fparams(orc_model* m) {
     float th = (float)c->aux_angles_deg[a] * (PI_F / 180.0f);
     m->aux_ux[a] = (float)sin((double)th);
     m->aux_uy[a] = (float)cos((double)th);
+    m->aux_deg[a] = (float)c->aux_angles_deg[a];
   }
 }
 

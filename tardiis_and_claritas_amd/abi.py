@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -137,6 +137,7 @@ class Tallies(C.Structure):
     _fields_ = [
         ('det', U64 * N_CHANNELS * 4 * MAX_GROUPS),
         ('aux', U64 * MAX_AUX * 2 * MAX_GROUPS),
+        ('aux_window', U64 * MAX_AUX * 2 * MAX_GROUPS),
         ('group_histories', U64 * MAX_GROUPS),
         ('incidence_sum', D * N_CHANNELS * 2),
         ('status', U64 * N_STATUS),
@@ -194,10 +195,12 @@ def tallies_to_numpy(t: Tallies):
     g = int(t.n_groups)
     det = np.ctypeslib.as_array(t.det).reshape(MAX_GROUPS, 4, N_CHANNELS).astype(np.int64)
     aux = np.ctypeslib.as_array(t.aux).reshape(MAX_GROUPS, 2, MAX_AUX).astype(np.int64)
+    auxw = np.ctypeslib.as_array(t.aux_window).reshape(MAX_GROUPS, 2, MAX_AUX).astype(np.int64)
     return {
         'n_groups': g,
         'det': det[:g].copy(),
         'aux': aux[:g].copy(),
+        'aux_window': auxw[:g].copy(),
         'group_histories': np.ctypeslib.as_array(t.group_histories)[:g].astype(np.int64).copy(),
         'incidence_sum': np.ctypeslib.as_array(t.incidence_sum).reshape(2, N_CHANNELS).copy(),
         'status': np.ctypeslib.as_array(t.status).astype(np.int64).copy(),

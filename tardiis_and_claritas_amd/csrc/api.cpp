@@ -59,6 +59,7 @@ struct Slot {
   DevBuf theta_guide, gguide, wguide;  // inverse-CDF guide tables
   size_t field_len = 0;
   DevParams params{};
+  std::vector<double> mie_key;  // optical inputs of the Mie tables held by this slot
   double mu_scalars_h[8]{};
   double majorants_h[4]{};
   void release() {
@@ -67,6 +68,7 @@ struct Slot {
                      &trace_sums, &sedA, &sedB, &sednorm, &sedws, &radlog, &majorants, &radius_f, &n_f,
                      &k_f, &albedo_f, &gcdf_f, &wcdf_f, &field, &theta_guide, &gguide, &wguide};
     for (DevBuf* b : all) b->release();
+    mie_key.clear();
     ready = false;
   }
 };
@@ -139,6 +141,18 @@ int check_particles(clr_ctx* c, const clr_particles* p) {
 // Run the Mie kernels for particles p into slot-owned buffers.
 int run_mie(clr_ctx* c, Slot& s, const clr_particles* p) {
   const int n = p->n, nt = std::max(p->n_theta, 2);
+  // The Mie tables depend on the sizes, indices, medium, wavelength and grid
+  // only (not on PSD weights or concentration): a slot prepared again with
+  // the same optics keeps its tables and only takes the new weights.
+  std::vector<double> key = {(double)n, (double)nt, p->n_medium, p->wavelength};
+  key.insert(key.end(), p->diameter, p->diameter + n);
+  key.insert(key.end(), p->n_real, p->n_real + n);
+  key.insert(key.end(), p->k_imag, p->k_imag + n);
+  if (!s.mie_key.empty() && key == s.mie_key && s.n == n && s.nt == nt) {
+    HIPCHK(c, hipMemcpyAsync(s.in_w.p, p->weight, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return 0;
+  }
+  s.mie_key.clear();
   int stride = 2;
   for (int i = 0; i < n; ++i)
     stride = std::max(stride, mie_order(p->diameter[i], p->n_real[i], p->k_imag[i], p->n_medium, p->wavelength) + 2);
@@ -192,6 +206,7 @@ int run_mie(clr_ctx* c, Slot& s, const clr_particles* p) {
   if (ecode) return fail(c, ecode, "Mie precompute failed with error code " + std::to_string(ecode));
   s.n = n;
   s.nt = nt;
+  s.mie_key = std::move(key);
   return 0;
 }
 
@@ -277,6 +292,7 @@ void fill_params(Slot& s, const clr_config& g) {
   for (int a = 0; a < CLR_MAX_AUX; ++a) {
     P.aux_ux[a] = det_u(g.aux_angles_deg[a], false);
     P.aux_uy[a] = det_u(g.aux_angles_deg[a], true);
+    P.aux_deg[a] = (float)g.aux_angles_deg[a];
   }
   if (g.free_surface_model == CLR_SURFACE_RANKINE) {
     const double R = g.rin, a = g.vortex_core_radius, R2 = R * R, a2 = a * a;

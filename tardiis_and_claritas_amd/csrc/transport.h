@@ -16,7 +16,7 @@ struct DevParams {
   float dh, core_r, alpha1, alpha2, sigma, beam_r, legacy_acc, wavelength;
   float mu_geom, mu_wave, mu_track, coherent, zs_min, field_zmax, field_half;
   float det_ux[CLR_N_CHANNELS], det_uy[CLR_N_CHANNELS];
-  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX];
+  float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX], aux_deg[CLR_MAX_AUX];
   double rk_K, rk_C0, sum_trace_g, sum_trace_w;
   int fs_model, src_model, event_model, sed_model;
   int max_pb, max_iter, cond_trials, n_aux;

@@ -1574,6 +1574,7 @@ constexpr int kAuxWords = CLR_MAX_GROUPS * 2 * CLR_MAX_AUX;
 struct LdsTally {
   uint32_t det[kDetWords];
   uint32_t aux[kAuxWords];
+  uint32_t auxw[kAuxWords];
   uint32_t status[16];
   uint32_t fail[32];
   uint32_t ich[CLR_N_ICHIST];
@@ -1668,7 +1669,8 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
     }
   }
   // legacy exit-position windows |angle - 10 j| <= acceptance; with the
-  // acceptance capped at 15 deg only j within +-2 of angle/10 can match
+  // acceptance capped at 15 deg only j within +-2 of angle/10 can match.
+  // The same rule at the auxiliary angles fills aux_window.
   if (scored) {
     for (int m = 0; m < 2; ++m) {
       const float ang = m ? ang_m : ang_n;
@@ -1678,6 +1680,11 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
         const int j = j0 + k;
         if (j >= 0 && j < CLR_N_CHANNELS && !(L.ic <= 0 && j >= 9) && fabsf(ang - 10.0f * (float)j) <= P.legacy_acc)
           atomicAdd(&T.det[(g * 4 + 2 + m) * CLR_N_CHANNELS + j], 1u);
+      }
+      for (int a = 0; a < P.n_aux; ++a) {
+        const float deg = P.aux_deg[a];
+        if (!(L.ic <= 0 && deg >= 90.0f) && fabsf(ang - deg) <= P.legacy_acc)
+          atomicAdd(&T.auxw[(g * 2 + m) * CLR_MAX_AUX + a], 1u);
       }
     }
   }
@@ -1698,6 +1705,9 @@ __device__ void flush_lds(LdsTally& T, clr_tallies* __restrict__ G) {
   unsigned long long* gaux = reinterpret_cast<unsigned long long*>(&G->aux[0][0][0]);
   for (int i = threadIdx.x; i < kAuxWords; i += blockDim.x)
     if (T.aux[i]) { atomicAdd(gaux + i, (unsigned long long)T.aux[i]); T.aux[i] = 0u; }
+  unsigned long long* gauxw = reinterpret_cast<unsigned long long*>(&G->aux_window[0][0][0]);
+  for (int i = threadIdx.x; i < kAuxWords; i += blockDim.x)
+    if (T.auxw[i]) { atomicAdd(gauxw + i, (unsigned long long)T.auxw[i]); T.auxw[i] = 0u; }
   for (int i = threadIdx.x; i < CLR_N_STATUS; i += blockDim.x)
     if (T.status[i]) { atomicAdd((unsigned long long*)&G->status[i], (unsigned long long)T.status[i]); T.status[i] = 0u; }
   for (int i = threadIdx.x; i < CLR_N_FAILURE; i += blockDim.x)

@@ -6,8 +6,8 @@ the eight configuration parameters
     log10 concentration, PSD median (um), sigma_g, m_r, log10 m_i,
     wavelength (nm), vortex depth (mm), Rouse number
 
-to the 90 deg, 0 deg and 135 deg detector signals (symmetrised hardware hit
-fractions of launched histories).  It is trained on the GPU by the library's own
+to the 90 deg, 0 deg and 135 deg detector signals (symmetrised exit-position
+window counts per launched history).  It is trained on the GPU by the library's own
 BF16-MFMA forward/backward kernels (clr_mlp_*), on batched Monte Carlo of the
 config-3 parameter space.  `Surrogate.query` answers from the network where the
 members agree and falls back to Monte Carlo where their spread exceeds a
@@ -30,7 +30,7 @@ import numpy as np
 from . import abi, workloads
 from ._lib import Context
 
-FEATURES = ('log10_conc_mg_l', 'median_um', 'sigma_g', 'm_r', 'log10_m_i', 'wavelength_nm', 'vortex_depth_mm',
+FEATURES = ('log10_conc_mg_l', 'log10_median_um', 'sigma_g', 'm_r', 'log10_m_i', 'wavelength_nm', 'vortex_depth_mm',
             'rouse')
 SIGNALS = ('s90', 's0', 's135')
 P = C.POINTER
@@ -102,17 +102,20 @@ class MlpEnsemble:
 # ---------------------------------------------------------------- features --
 def features(p) -> np.ndarray:
     """[n][8] raw features from a config-3 style parameter dict."""
-    return np.stack([np.log10(p['conc_mg_l']), p['median_um'], p['sigma_g'], p['m_r'], np.log10(p['m_i']),
+    return np.stack([np.log10(p['conc_mg_l']), np.log10(p['median_um']), p['sigma_g'], p['m_r'], np.log10(p['m_i']),
                      p['wavelength_nm'], p['vortex_depth_mm'], p['rouse']], axis=1).astype(np.float64)
 
 
 def signals_from_tallies(T, n_histories):
-    """S90, S0, S135: symmetrised hardware hits / launched histories."""
-    det = T['det'].sum(0)   # [4][18]
-    aux = T['aux'].sum(0)   # [2][n_aux]
-    s90 = 0.5 * (det[0, 9] + det[1, 9]) / n_histories
-    s0 = 0.5 * (det[0, 0] + det[1, 0]) / n_histories
-    s135 = 0.5 * (aux[0, 0] + aux[1, 0]) / n_histories
+    """S90, S0, S135: symmetrised exit-position window counts (the legacy
+    +-acceptance rule at 90 and 0 deg, the aux window at 135 deg) per launched
+    history.  The collimated hardware bores accept ~1e-7 of the launched
+    photons, too few for Monte Carlo labels at affordable photon counts."""
+    det = T['det'].sum(0)           # [4][18]: hw native, hw mirror, legacy native, legacy mirror
+    auxw = T['aux_window'].sum(0)   # [2][n_aux]
+    s90 = 0.5 * (det[2, 9] + det[3, 9]) / n_histories
+    s0 = 0.5 * (det[2, 0] + det[3, 0]) / n_histories
+    s135 = 0.5 * (auxw[0, 0] + auxw[1, 0]) / n_histories
     return np.array([s90, s0, s135], dtype=np.float64)
 
 
@@ -144,6 +147,44 @@ def monte_carlo(params, photons, seed, batch=256, runner=None, progress=None):
     return out
 
 
+OPTICAL = ('median_um', 'sigma_g', 'm_r', 'm_i', 'wavelength_nm')
+TRANSPORT = ('conc_mg_l', 'vortex_depth_mm', 'rouse')
+
+
+def transport_params(n, g):
+    """Concentration, vortex depth and Rouse number from the config-3 ranges."""
+    return dict(conc_mg_l=np.exp(g.uniform(np.log(10.0), np.log(10000.0), n)),
+                vortex_depth_mm=g.uniform(0.0, 6.0, n), rouse=g.uniform(0.05, 1.0, n))
+
+
+def generate(n_optical, n_variants, photons, seed, runner=None, progress=None):
+    """Dataset of n_optical x n_variants configurations from the config-3
+    distributions: the five optical parameters are drawn per optical
+    configuration and the three transport parameters afresh for every variant.
+    A batch holds up to 256 optical configurations, one per device slot, and
+    the variants re-prepare the same slots, so each slot's Mie tables are
+    computed once.  Returns (params dict, signals [n][3])."""
+    from .batched import BatchedRunner
+    runner = runner or BatchedRunner(0)
+    p_opt = workloads.config3_params(n_optical, seed)
+    g = np.random.Generator(np.random.PCG64(seed + 7919))
+    cols = {k: [] for k in OPTICAL + TRANSPORT}
+    sig = []
+    done, total = 0, n_optical * n_variants
+    for a0 in range(0, n_optical, 256):
+        sel = np.arange(a0, min(n_optical, a0 + 256))
+        for v in range(n_variants):
+            params = {k: np.asarray(p_opt[k])[sel] for k in OPTICAL}
+            params.update(transport_params(len(sel), g))
+            sig.append(monte_carlo(params, photons, seed + 1_000_003 * (v + 1) + a0, batch=256, runner=runner))
+            for k in cols:
+                cols[k].append(params[k])
+            done += len(sel)
+            if progress:
+                progress(done, total)
+    return {k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig)
+
+
 # ---------------------------------------------------------------- surrogate --
 @dataclass
 class Normaliser:
@@ -169,10 +210,22 @@ class Surrogate:
     def _x(self, F):
         return (F - self.norm.x_mean) / self.norm.x_std
 
+    @staticmethod
+    def targets(S):
+        """log(S90/S0), log S0, log(S135/S0): the first is the NTU reading."""
+        S = np.maximum(S, FLOOR)
+        return np.stack([np.log(S[:, 0] / S[:, 1]), np.log(S[:, 1]), np.log(S[:, 2] / S[:, 1])], axis=1)
+
+    @staticmethod
+    def signals(Y):
+        s0 = np.exp(Y[:, 1])
+        return np.stack([np.exp(Y[:, 0]) * s0, s0, np.exp(Y[:, 2]) * s0], axis=1)
+
     def fit(self, F, S, epochs=40, batch=512, lr=1e-3, seed=0, schedule=(1.0, 0.3, 0.1)):
-        """Train on raw features F [n][8] and signals S [n][3] (log targets).
-        The learning rate steps down over `schedule` (equal shares of epochs)."""
-        Y = np.log(np.maximum(S, FLOOR))
+        """Train on raw features F [n][8] and signals S [n][3] (log-ratio
+        targets).  The learning rate steps down over `schedule` (equal shares
+        of the epochs)."""
+        Y = self.targets(S)
         self.norm = Normaliser(F.mean(0), F.std(0) + 1e-12, Y.mean(0), Y.std(0) + 1e-12)
         x, y = self._x(F), (Y - self.norm.y_mean) / self.norm.y_std
         losses = []
@@ -182,10 +235,10 @@ class Surrogate:
         return losses
 
     def predict(self, F):
-        """Signals [n][3] and the members' spread of log S [n][3] (relative)."""
+        """Signals [n][3] and the members' spread of the log targets [n][3];
+        column 0 is the relative spread of the NTU reading S90/S0."""
         m, s = self.ens.predict(self._x(F))
-        logS = m * self.norm.y_std + self.norm.y_mean
-        return np.exp(logS), s * self.norm.y_std
+        return self.signals(m * self.norm.y_std + self.norm.y_mean), s * self.norm.y_std
 
     def query(self, params, threshold=0.01, mc_photons=10_000_000, seed=1, runner=None):
         """Signals for a parameter dict: network where the ensemble spread of
@@ -193,8 +246,7 @@ class Surrogate:
         (signals, used_mc mask)."""
         F = features(params)
         S, spread = self.predict(F)
-        ratio_spread = np.sqrt(spread[:, 0] ** 2 + spread[:, 1] ** 2)
-        mc = ratio_spread > threshold
+        mc = spread[:, 0] > threshold
         if np.any(mc):
             sub = {k: np.asarray(v)[mc] for k, v in params.items()}
             S[mc] = monte_carlo(sub, mc_photons, seed, runner=runner)
@@ -204,3 +256,116 @@ class Surrogate:
 def ntu_relative_error(S_pred, S_ref):
     """|dNTU|/NTU with NTU = K S90/S0."""
     return np.abs((S_pred[:, 0] / S_pred[:, 1]) / (S_ref[:, 0] / S_ref[:, 1]) - 1.0)
+
+
+# ---------------------------------------------------------------- pipeline --
+def photon_budget(conc_mg_l, counts=200.0, p90_per_mg_l=1e-7, lo=200_000, hi=200_000_000):
+    """Histories per configuration so that the 90 deg window collects about
+    `counts` photons: its single-scattering yield grows ~linearly with the
+    concentration (p90 ~ 1e-7 per mg/L on the config-3 space)."""
+    n = counts / (p90_per_mg_l * np.asarray(conc_mg_l, dtype=float))
+    return np.clip(n, lo, hi).astype(np.int64)
+
+
+def mc_matched_photons(S, rel=0.01):
+    """Histories for a relative standard error `rel` on S90/S0 (binomial
+    counts; S0 and S90 are disjoint windows)."""
+    s90, s0 = np.maximum(S[:, 0], 1e-12), np.maximum(S[:, 1], 1e-12)
+    return (1.0 / s90 + 1.0 / s0) / rel ** 2
+
+
+def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
+                 test_counts=2000.0, epochs=60, seed=20261204, log=print):
+    """Generate, train, evaluate; writes summary.json and weights.npz."""
+    from .batched import BatchedRunner
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    runner = BatchedRunner(0)
+    t0 = time.time()
+
+    def gen(n_opt, n_var, cnt, sd):
+        p_opt = workloads.config3_params(n_opt, sd)
+        g = np.random.Generator(np.random.PCG64(sd + 7919))
+        cols = {k: [] for k in OPTICAL + TRANSPORT}
+        sig, hist, mc_s = [], 0, 0.0
+        for a0 in range(0, n_opt, 256):
+            sel = np.arange(a0, min(n_opt, a0 + 256))
+            for v in range(n_var):
+                params = {k: np.asarray(p_opt[k])[sel] for k in OPTICAL}
+                params.update(transport_params(len(sel), g))
+                budget = photon_budget(params['conc_mg_l'], cnt)
+                specs = [workloads.config3_item(params, i) for i in range(len(sel))]
+                runner.prepare(specs)
+                T = runner.run_prepared(budget.tolist(), sd + 1_000_003 * (v + 1) + a0)
+                hist += int(budget.sum())
+                mc_s += runner.last_ms * 1e-3
+                sig.append(np.stack([signals_from_tallies(T[k], int(budget[k])) for k in range(len(sel))]))
+                for k in cols:
+                    cols[k].append(params[k])
+                log(f'generate: {sum(len(c) for c in cols["rouse"])}/{n_opt * n_var} configs, '
+                    f'{hist:.3g} histories, {time.time() - t0:.0f} s')
+        return {k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig), hist, mc_s
+
+    P_tr, S_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed)
+    P_te, S_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1)
+    mc_rate = (h_tr + h_te) / (s_tr + s_te)
+    np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr,
+             **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te)
+    sur = Surrogate(runner.ctx, seed=seed)
+    F_tr = features(P_tr)
+    keep = np.all(S_tr > 0, axis=1)           # log targets need a count in every window
+    t1 = time.time()
+    losses = sur.fit(F_tr[keep], S_tr[keep], epochs=epochs, batch=256, lr=1e-3, seed=seed)
+    train_s = time.time() - t1
+    F_te = features(P_te)
+    S_pred, spread = sur.predict(F_te)
+    valid = np.all(S_te > 0, axis=1)
+    err = ntu_relative_error(S_pred[valid], S_te[valid])
+    # Monte Carlo statistical error of the held-out labels themselves
+    label_se = np.sqrt(1.0 / np.maximum(S_te[valid, 0] * photon_budget(P_te['conc_mg_l'][valid], test_counts), 1)
+                       + 1.0 / np.maximum(S_te[valid, 1] * photon_budget(P_te['conc_mg_l'][valid], test_counts), 1))
+    ratio_spread = spread[valid, 0]
+    ms = sur.ens.time_infer(65536, 20)
+    q_s = ms * 1e-3 / 65536
+    n_match = mc_matched_photons(S_te[valid])
+    t_mc = n_match / mc_rate
+    summary = dict(
+        n_train=int(keep.sum()), n_test=int(valid.sum()), train_histories=h_tr, test_histories=h_te,
+        mc_histories_per_s=mc_rate, train_seconds=train_s, epochs=epochs, final_member_losses=losses[-1],
+        ntu_error_median=float(np.median(err)), ntu_error_p90=float(np.percentile(err, 90)),
+        ntu_error_mean=float(np.mean(err)), frac_within_1pct=float(np.mean(err <= 0.01)),
+        label_se_median=float(np.median(label_se)),
+        surrogate_query_s=q_s, surrogate_queries_per_s=1.0 / q_s,
+        mc_matched_photons_median=float(np.median(n_match)), mc_query_s_median=float(np.median(t_mc)),
+        speedup_median=float(np.median(t_mc) / q_s),
+    )
+    # Monte Carlo fallback where the members disagree: error and effective speed-up
+    fb = {}
+    for thr in (0.005, 0.01, 0.02, 0.05):
+        use_mc = ratio_spread > thr
+        e = np.where(use_mc, label_se, err)   # a fallback answer carries MC's own 1 % error budget
+        t = q_s + np.where(use_mc, t_mc, 0.0)
+        fb[str(thr)] = dict(fallback_fraction=float(use_mc.mean()), ntu_error_p90=float(np.percentile(e, 90)),
+                            speedup=float(np.mean(t_mc) / np.mean(t)))
+    summary['fallback'] = fb
+    (out / 'summary.json').write_text(json.dumps(summary, indent=1))
+    np.savez(out / 'weights.npz', params=sur.ens.get_params(), **{k: v for k, v in sur.norm.__dict__.items()})
+    log(json.dumps(summary))
+    return summary
+
+
+if __name__ == '__main__':
+    import argparse
+    ap = argparse.ArgumentParser(description='surrogate pipeline: batched MC data, GPU training, held-out evaluation')
+    ap.add_argument('--out', default='gpurun_out/surrogate')
+    ap.add_argument('--n-optical', type=int, default=64)
+    ap.add_argument('--n-variants', type=int, default=64)
+    ap.add_argument('--n-test-optical', type=int, default=16)
+    ap.add_argument('--n-test-variants', type=int, default=32)
+    ap.add_argument('--counts', type=float, default=200.0)
+    ap.add_argument('--test-counts', type=float, default=2000.0)
+    ap.add_argument('--epochs', type=int, default=60)
+    ap.add_argument('--seed', type=int, default=20261204)
+    a = ap.parse_args()
+    run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
+                 a.epochs, a.seed, log=lambda s: print(s, flush=True))

@@ -70,6 +70,7 @@ def _compare_records(rg, ro):
 def test_history_by_history_config1(gpu_available, oracle_lib, event_model):
     _flush(f'history-by-history config1 {event_model}')
     kw, mat, conc, n, seed = workloads.config1(event_model)
+    kw['aux_detector_angles_deg'] = (135.0,)
     m = _model(kw)
     N = 20000
     m.prepare(mat, conc)
@@ -86,6 +87,8 @@ def test_history_by_history_config1(gpu_available, oracle_lib, event_model):
     assert np.mean(dv <= 1e-5) >= 0.999 and dv.max() <= 1e-3, dv.max()
     assert Tg['status'].sum() == N and To['status'].sum() == N
     assert np.abs(Tg['det'].sum(0) - To['det'].sum(0)).sum() <= 0.002 * N
+    for k in ('aux', 'aux_window'):
+        assert np.abs(Tg[k].sum(0) - To[k].sum(0)).sum() <= 0.002 * N, k
 
 
 def _sym(T):
@@ -104,6 +107,7 @@ def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
         N = 400_000
     elif builder == 'config2_plain_1000':
         kw, mat, conc, n, seed = workloads.config2(1000.0, 'plain_mie')
+        kw['aux_detector_angles_deg'] = (135.0,)
         N = 200_000
     else:
         kw, mat, conc, n, seed = workloads.config2(316.0, 'hybrid_mie_exact_rate')
@@ -117,6 +121,8 @@ def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
         sd = np.sqrt(np.maximum(a + b, 1.0))
         z = np.abs(a - b) / sd
         assert np.all(z < 4.0), f'{builder}: max z {z.max():.2f} at channel {z.argmax()}'
+    aw_g, aw_o = Tg['aux_window'].sum(0).sum(0), To['aux_window'].sum(0).sum(0)
+    assert np.all(np.abs(aw_g - aw_o) <= 4.0 * np.sqrt(np.maximum(aw_g + aw_o, 1.0))), (aw_g, aw_o)
     # the only failure the reference law produces here is the conditional
     # closure's explicit retry exhaustion (code 15, ~1e-6 per history,
     # CLARITAS.py:4534); every other code must stay at zero on both sides
