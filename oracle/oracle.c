@@ -314,6 +314,7 @@ struct orc_model {
   float *gcdf_f, *wcdf_f;
   double *trace_g, *trace_w;
   double sum_trace_g, sum_trace_w;
+  float sum_g_f, sum_w_f, finv[3];   /* shared-field rates and inverse cell sizes */
   double mu_scalars[8];
   double *sedA, *sedB, *sednorm, *sedws, *radlog;
   double maj[4];
@@ -631,6 +632,7 @@ static int sediment_precompute(orc_model* m, const clr_sediment* s) {
   if (tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD) {
     double mx = (double)field_max(m->field, m->field_len);
     m->maj[0] = m->sum_trace_g * mx; m->maj[1] = m->sum_trace_w * mx;
+    m->sum_g_f = (float)m->sum_trace_g; m->sum_w_f = (float)m->sum_trace_w;
     m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
     return 0;
   }
@@ -780,6 +782,44 @@ static double xyz_grid_sample(const float* G, int NX, int NY, int NZ, float x, f
   double c11 = (1 - tt[2]) * b[sx + sy] + tt[2] * b[sx + sy + 1];
   double c0 = (1 - tt[1]) * c00 + tt[1] * c01, c1 = (1 - tt[1]) * c10 + tt[1] * c11;
   return fmax((1 - tt[0]) * c0 + tt[0] * c1, 0.0);
+}
+
+/* Shared fields (models 4, 5; extensions with no reference counterpart) are
+ * interpolated in FP32 with float inverse cell sizes, as the device does. */
+static float shared_field_scale(const orc_model* m, float x, float y, float z) {
+  const fparams* P = &m->fp;
+  const float* G = m->field;
+  if (P->sed_model == CLR_SED_RZ_FIELD) {
+    int NR = P->fn0, NZ = P->fn1;
+    float rr = fminf(sqrtf(fmaxf(x * x + y * y, 0.0f)), P->rin);
+    float ur = fminf(fmaxf(rr * m->finv[0] - 0.5f, 0.0f), (float)(NR - 1));
+    float uz = fminf(fmaxf((z - P->zmin) * m->finv[1] - 0.5f, 0.0f), (float)(NZ - 1));
+    int i = (int)floorf(ur), k = (int)floorf(uz);
+    float tr = ur - (float)i, tz = uz - (float)k;
+    if (i >= NR - 1) { i = NR - 2; tr = 1.0f; }
+    if (k >= NZ - 1) { k = NZ - 2; tz = 1.0f; }
+    const float* g = G + (size_t)i * NZ + k;
+    float q0 = (1.0f - tr) * g[0] + tr * g[NZ], q1 = (1.0f - tr) * g[1] + tr * g[NZ + 1];
+    return fmaxf((1.0f - tz) * q0 + tz * q1, 0.0f);
+  }
+  int nn[3] = {P->fn0, P->fn1, P->fn2}, ii[3];
+  float u[3] = {(x + P->field_half) * m->finv[0] - 0.5f, (y + P->field_half) * m->finv[1] - 0.5f,
+                (z - P->zmin) * m->finv[2] - 0.5f}, tt[3];
+  for (int a = 0; a < 3; ++a) {
+    float v = fminf(fmaxf(u[a], 0.0f), (float)(nn[a] - 1));
+    int i = (int)floorf(v);
+    float t = v - (float)i;
+    if (i >= nn[a] - 1) { i = nn[a] - 2; t = 1.0f; }
+    ii[a] = i; tt[a] = t;
+  }
+  size_t sy = (size_t)nn[2], sx = (size_t)nn[1] * nn[2];
+  const float* b = G + (size_t)ii[0] * sx + (size_t)ii[1] * sy + ii[2];
+  float c00 = (1.0f - tt[2]) * b[0] + tt[2] * b[1];
+  float c01 = (1.0f - tt[2]) * b[sy] + tt[2] * b[sy + 1];
+  float c10 = (1.0f - tt[2]) * b[sx] + tt[2] * b[sx + 1];
+  float c11 = (1.0f - tt[2]) * b[sx + sy] + tt[2] * b[sx + sy + 1];
+  float c0 = (1.0f - tt[1]) * c00 + tt[1] * c01, c1 = (1.0f - tt[1]) * c10 + tt[1] * c11;
+  return fmaxf((1.0f - tt[0]) * c0 + tt[0] * c1, 0.0f);
 }
 
 static double sediment_local_scale(const orc_model* m, int pidx, float x, float y, float z) {
@@ -1883,17 +1923,18 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         }
         float LG = P->mu_geom, LW = P->mu_wave, LT = MU_TOTAL;
         if (P->sed_model != CLR_SED_UNIFORM) {
-          double lg = 0.0, lw = 0.0;
           if (P->sed_model == CLR_SED_RZ_FIELD || P->sed_model == CLR_SED_XYZ_FIELD) {
-            double s = sediment_local_scale(m, 0, p->x, p->y, p->z);
-            lg = m->sum_trace_g * s; lw = m->sum_trace_w * s;
+            float s = shared_field_scale(m, p->x, p->y, p->z);
+            LG = m->sum_g_f * s; LW = m->sum_w_f * s;
           } else {
+            double lg = 0.0, lw = 0.0;
             for (int i = 0; i < m->n; ++i) {
               double sc = sediment_local_scale(m, i, p->x, p->y, p->z);
               lg += m->trace_g[i] * sc; lw += m->trace_w[i] * sc;
             }
+            LG = (float)lg; LW = (float)lw;
           }
-          LG = (float)lg; LW = (float)lw; LT = LG + LW;
+          LT = LG + LW;
           if (!(LT > 0.0f)) { o->n_null++; continue; }
           float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
           if (LT > MU_TRACK + slack) { pf = PF_MAJORANT; pfs = SUB_MAJORANT; st = 4; break; }
@@ -2235,6 +2276,15 @@ static void fill_fparams(orc_model* m) {
   P->n_groups = (c->n_groups >= 1 && c->n_groups <= CLR_MAX_GROUPS) ? c->n_groups : CLR_MAX_GROUPS;
   P->fn0 = c->field_n0; P->fn1 = c->field_n1; P->fn2 = c->field_n2; P->fnbins = c->field_nbins;
   P->field_zmax = (float)c->field_zmax; P->field_half = (float)c->field_xy_half;
+  m->finv[0] = m->finv[1] = m->finv[2] = 0.0f;
+  if (c->sediment_model == CLR_SED_RZ_FIELD && c->field_n0 > 0 && c->field_n1 > 0) {
+    m->finv[0] = (float)c->field_n0 / P->rin;
+    m->finv[1] = (float)c->field_n1 / (P->field_zmax - P->zmin);
+  } else if (c->sediment_model == CLR_SED_XYZ_FIELD && c->field_n0 > 0) {
+    m->finv[0] = (float)c->field_n0 / (2.0f * P->field_half);
+    m->finv[1] = (float)c->field_n1 / (2.0f * P->field_half);
+    m->finv[2] = (float)c->field_n2 / (P->field_zmax - P->zmin);
+  }
   /* Rankine constants (K = wall-minus-centre drop / (2 - a^2/R^2)) */
   if (c->free_surface_model == CLR_SURFACE_RANKINE) {
     double R = c->rin, a = c->vortex_core_radius, R2 = R * R, a2 = a * a;

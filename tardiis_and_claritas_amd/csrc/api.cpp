@@ -341,6 +341,18 @@ void fill_params(Slot& s, const clr_config& g) {
   (void)hipMemcpy(sums, s.trace_sums.p, sizeof(sums), hipMemcpyDeviceToHost);
   P.sum_trace_g = sums[0];
   P.sum_trace_w = sums[1];
+  P.sum_g_f = (float)sums[0];
+  P.sum_w_f = (float)sums[1];
+  // inverse cell sizes of the shared fields, in float as the oracle computes them
+  P.field_inv0 = P.field_inv1 = P.field_inv2 = 0.0f;
+  if (g.sediment_model == CLR_SED_RZ_FIELD && g.field_n0 > 0 && g.field_n1 > 0) {
+    P.field_inv0 = (float)g.field_n0 / P.rin;
+    P.field_inv1 = (float)g.field_n1 / (P.field_zmax - P.zmin);
+  } else if (g.sediment_model == CLR_SED_XYZ_FIELD && g.field_n0 > 0) {
+    P.field_inv0 = (float)g.field_n0 / (2.0f * P.field_half);
+    P.field_inv1 = (float)g.field_n1 / (2.0f * P.field_half);
+    P.field_inv2 = (float)g.field_n2 / (P.field_zmax - P.zmin);
+  }
 }
 
 int ensure_slots(clr_ctx* c, int need) {

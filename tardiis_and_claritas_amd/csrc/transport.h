@@ -18,6 +18,9 @@ struct DevParams {
   float det_ux[CLR_N_CHANNELS], det_uy[CLR_N_CHANNELS];
   float aux_ux[CLR_MAX_AUX], aux_uy[CLR_MAX_AUX], aux_deg[CLR_MAX_AUX];
   double rk_K, rk_C0, sum_trace_g, sum_trace_w;
+  // shared FP32 fields (sediment models 4, 5): inverse cell sizes and the
+  // bin-summed channel rates in float
+  float field_inv0, field_inv1, field_inv2, sum_g_f, sum_w_f;
   int fs_model, src_model, event_model, sed_model;
   int max_pb, max_iter, cond_trials, n_aux;
   int n_groups, n_bins, n_theta, n_radial;

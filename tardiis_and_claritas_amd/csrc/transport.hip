@@ -735,6 +735,41 @@ __device__ __forceinline__ double rz_sample(const float* __restrict__ G, int NR,
   const double q0 = (1.0 - tr) * q00 + tr * q10, q1 = (1.0 - tr) * q01 + tr * q11;
   return fmax((1.0 - tz) * q0 + tz * q1, 0.0);
 }
+// Shared FP32 fields (sediment models 4 and 5, extensions with no reference
+// counterpart): cell-centred bilinear (r, z) / trilinear (x, y, z)
+// interpolation in FP32 with the inverse cell sizes precomputed on the host.
+__device__ __forceinline__ float rz_field_f32(const float* __restrict__ G, int NR, int NZ, float x, float y, float z,
+                                              float R, float ZMIN, float inv_dr, float inv_dz) {
+  const float rr = fminf(sqrtf(fmaxf(x * x + y * y, 0.0f)), R);
+  const float ur = fminf(fmaxf(rr * inv_dr - 0.5f, 0.0f), (float)(NR - 1));
+  const float uz = fminf(fmaxf((z - ZMIN) * inv_dz - 0.5f, 0.0f), (float)(NZ - 1));
+  int i = (int)floorf(ur), k = (int)floorf(uz);
+  float tr = ur - (float)i, tz = uz - (float)k;
+  if (i >= NR - 1) { i = NR - 2; tr = 1.0f; }
+  if (k >= NZ - 1) { k = NZ - 2; tz = 1.0f; }
+  const float* g = G + (size_t)i * NZ + k;
+  const float q0 = (1.0f - tr) * g[0] + tr * g[NZ], q1 = (1.0f - tr) * g[1] + tr * g[NZ + 1];
+  return fmaxf((1.0f - tz) * q0 + tz * q1, 0.0f);
+}
+__device__ __forceinline__ float xyz_field_f32(const float* __restrict__ G, int NX, int NY, int NZ, float x, float y,
+                                               float z, float H, float ZMIN, float inv_dx, float inv_dy, float inv_dz) {
+  const float u0 = fminf(fmaxf((x + H) * inv_dx - 0.5f, 0.0f), (float)(NX - 1));
+  const float u1 = fminf(fmaxf((y + H) * inv_dy - 0.5f, 0.0f), (float)(NY - 1));
+  const float u2 = fminf(fmaxf((z - ZMIN) * inv_dz - 0.5f, 0.0f), (float)(NZ - 1));
+  int i0 = (int)floorf(u0), i1 = (int)floorf(u1), i2 = (int)floorf(u2);
+  float t0 = u0 - (float)i0, t1 = u1 - (float)i1, t2 = u2 - (float)i2;
+  if (i0 >= NX - 1) { i0 = NX - 2; t0 = 1.0f; }
+  if (i1 >= NY - 1) { i1 = NY - 2; t1 = 1.0f; }
+  if (i2 >= NZ - 1) { i2 = NZ - 2; t2 = 1.0f; }
+  const size_t sy = (size_t)NZ, sx = (size_t)NY * NZ;
+  const float* b = G + (size_t)i0 * sx + (size_t)i1 * sy + i2;
+  const float c00 = (1.0f - t2) * b[0] + t2 * b[1];
+  const float c01 = (1.0f - t2) * b[sy] + t2 * b[sy + 1];
+  const float c10 = (1.0f - t2) * b[sx] + t2 * b[sx + 1];
+  const float c11 = (1.0f - t2) * b[sx + sy] + t2 * b[sx + sy + 1];
+  const float c0 = (1.0f - t1) * c00 + t1 * c01, c1 = (1.0f - t1) * c10 + t1 * c11;
+  return fmaxf((1.0f - t0) * c0 + t0 * c1, 0.0f);
+}
 __device__ double xyz_sample(const float* __restrict__ G, int NX, int NY, int NZ, float x, float y,
                              float z, float H, float ZMIN, float ZMAX) {
   const double dx = 2.0 * (double)H / NX, dy = 2.0 * (double)H / NY, dz = ((double)ZMAX - (double)ZMIN) / NZ;
@@ -1423,22 +1458,24 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   }
   float LG = P.mu_geom, LW = P.mu_wave, LT = MU_TOTAL;
   if (P.sed_model != CLR_SED_UNIFORM) {
-    double lg = 0.0, lw = 0.0;
     if (kPlain || P.sed_model == CLR_SED_RZ_FIELD || P.sed_model == CLR_SED_XYZ_FIELD) {
-      const double s = (P.sed_model == CLR_SED_RZ_FIELD)
-                           ? rz_sample(P.field, P.fn0, P.fn1, L.x, L.y, L.z, P.rin, P.zmin, P.field_zmax)
-                           : xyz_sample(P.field, P.fn0, P.fn1, P.fn2, L.x, L.y, L.z, P.field_half, P.zmin,
-                                        P.field_zmax);
-      lg = P.sum_trace_g * s;
-      lw = P.sum_trace_w * s;
+      const float s = (P.sed_model == CLR_SED_RZ_FIELD)
+                          ? rz_field_f32(P.field, P.fn0, P.fn1, L.x, L.y, L.z, P.rin, P.zmin, P.field_inv0,
+                                         P.field_inv1)
+                          : xyz_field_f32(P.field, P.fn0, P.fn1, P.fn2, L.x, L.y, L.z, P.field_half, P.zmin,
+                                          P.field_inv0, P.field_inv1, P.field_inv2);
+      LG = P.sum_g_f * s;
+      LW = P.sum_w_f * s;
     } else {
+      double lg = 0.0, lw = 0.0;
       for (int i = 0; i < P.n_bins; ++i) {
         const double sc = sediment_scale(P, i, L.x, L.y, L.z);
         lg += P.trace_g[i] * sc;
         lw += P.trace_w[i] * sc;
       }
+      LG = (float)lg; LW = (float)lw;
     }
-    LG = (float)lg; LW = (float)lw; LT = LG + LW;
+    LT = LG + LW;
     if (!(LT > 0.0f)) { A.n_null++; return; }
     const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
     if (LT > MU_TRACK + slack) { terminate(L, 4, PF_MAJORANT); return; }
