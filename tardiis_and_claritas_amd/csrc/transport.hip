@@ -61,16 +61,27 @@ enum : int {
 // The key (the run's seed) is the same for every lane of a work item, so it
 // lives once per workgroup in LDS rather than in two VGPRs per lane.
 __shared__ uint32_t s_key[2];
+// The lane keeps an 8-word queue of its stream: words are consumed from b0
+// and the queue shifts down (a dynamically indexed pick would make the
+// compiler keep it in scratch memory).  rng_topup appends the next block when
+// at most 4 words remain, at the start of a pass where the whole wave takes
+// the same path, so the draws inside the pass seldom need a block of their
+// own: a refill inside divergent code costs the whole wave a Philox block.
 struct Rng {
-  uint32_t h0, h1, blk, idx, draws;
-  uint32_t b0, b1, b2, b3;
+  uint32_t h0, h1;  // 64-bit stream counter (history number + launch offset)
+  uint32_t blk;     // next Philox block of the stream
+  uint32_t n;       // words left in the queue, 0..8
+  uint32_t b0, b1, b2, b3, b4, b5, b6, b7;
 };
 __device__ __forceinline__ void rng_seed(Rng& r, uint64_t hist) {
   r.h0 = (uint32_t)hist; r.h1 = (uint32_t)(hist >> 32);
-  r.blk = 0; r.idx = 4; r.draws = 0;
-  r.b0 = r.b1 = r.b2 = r.b3 = 0;
+  r.blk = 0; r.n = 0;
+  r.b0 = r.b1 = r.b2 = r.b3 = r.b4 = r.b5 = r.b6 = r.b7 = 0;
 }
-__device__ __forceinline__ void philox_refill(Rng& r) {
+// words consumed so far
+__device__ __forceinline__ uint32_t rng_draws(const Rng& r) { return 4u * r.blk - r.n; }
+__device__ __forceinline__ uint64_t rng_counter(const Rng& r) { return ((uint64_t)r.h1 << 32) | r.h0; }
+__device__ __forceinline__ void philox_block(Rng& r, uint32_t& w0, uint32_t& w1, uint32_t& w2, uint32_t& w3) {
   uint32_t c0 = r.blk, c1 = 0u, c2 = r.h0, c3 = r.h1, k0 = s_key[0], k1 = s_key[1];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
@@ -82,20 +93,34 @@ __device__ __forceinline__ void philox_refill(Rng& r) {
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
   }
-  r.b0 = c0; r.b1 = c1; r.b2 = c2; r.b3 = c3;
+  w0 = c0; w1 = c1; w2 = c2; w3 = c3;
   r.blk++;
-  r.idx = 0;
 }
-// The four words of the current block are consumed as a queue (b0 first,
-// then shifted down): a dynamically indexed pick would make the compiler
-// keep the buffer in scratch memory instead of VGPRs.
 __device__ __forceinline__ uint32_t rng_next(Rng& r) {
-  if (r.idx >= 4) philox_refill(r);
+  if (r.n == 0) {
+    philox_block(r, r.b0, r.b1, r.b2, r.b3);
+    r.n = 4;
+  }
   const uint32_t v = r.b0;
-  r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3;
-  r.idx++;
-  r.draws++;
+  r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3; r.b3 = r.b4; r.b4 = r.b5; r.b5 = r.b6; r.b6 = r.b7;
+  r.n--;
   return v;
+}
+// append the next block after the n <= 4 queued words
+__device__ __forceinline__ void rng_topup(Rng& r) {
+  if (r.n > 4) return;
+  uint32_t w0, w1, w2, w3;
+  philox_block(r, w0, w1, w2, w3);
+  const uint32_t n = r.n;
+  r.b7 = (n == 4) ? w3 : r.b7;
+  r.b6 = (n == 4) ? w2 : (n == 3) ? w3 : r.b6;
+  r.b5 = (n == 4) ? w1 : (n == 3) ? w2 : (n == 2) ? w3 : r.b5;
+  r.b4 = (n == 4) ? w0 : (n == 3) ? w1 : (n == 2) ? w2 : (n == 1) ? w3 : r.b4;
+  r.b3 = (n == 3) ? w0 : (n == 2) ? w1 : (n == 1) ? w2 : (n == 0) ? w3 : r.b3;
+  r.b2 = (n == 2) ? w0 : (n == 1) ? w1 : (n == 0) ? w2 : r.b2;
+  r.b1 = (n == 1) ? w0 : (n == 0) ? w1 : r.b1;
+  r.b0 = (n == 0) ? w0 : r.b0;
+  r.n = n + 4;
 }
 // (m + 1/2) 2^-23 with m the top 23 bits: exact, strictly inside (0,1)
 __device__ __forceinline__ float u_of(uint32_t m) { return ((float)m + 0.5f) * 1.1920928955078125e-7f; }
@@ -167,7 +192,9 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
 __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
   const float q = x * x + y * y + z * z;
   if (q <= 0.0f) { x = 0.0f; y = 1.0f; z = 0.0f; return; }
-  const float s = 1.0f / sqrtf(q);
+  // the reference's rsqrtf is the hardware reciprocal square root; v_rsq_f32
+  // is its gfx950 counterpart (<= 1 ulp)
+  const float s = __builtin_amdgcn_rsqf(q);
   x *= s; y *= s; z *= s;
 }
 __device__ __forceinline__ void reflect3(float& vx, float& vy, float& vz, float nx, float ny,
@@ -1148,8 +1175,6 @@ struct Lane {
   int st, pf;
   int ic, gic, wic, wcoh, pr, cir, cor, tr, bir, bor;
   unsigned flags;
-  int group;
-  uint64_t hidx;
 };
 // per-lane work counters accumulated over all histories of a launch
 struct Acc {
@@ -1628,13 +1653,12 @@ __device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
 // tallies.  Lanes of one wave that hit the same bin are serialised by the
 // LDS atomic unit, which is cheaper than aggregating them in software.
 __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
-                                            uint64_t n_rec) {
-  const int g = L.group;
+                                            uint64_t n_rec, uint64_t hidx, int g) {
   int nat = -1, mir = -1, aux_bits = 0;
   bool scored = false;
   float inc_n = 0.0f, inc_m = 0.0f, ang_n = -1.0f, ang_m = -1.0f;
   {
-    A.draws += L.r.draws;
+    A.draws += rng_draws(L.r);
     if (L.flags & F_ENTERED) A.entered++;
     if (L.st == 1) {
       const float ex = L.x, ey = L.y, ez = L.z, evx = L.vx, evy = L.vy, evz = L.vz;
@@ -1680,12 +1704,12 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
       if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
     }
     lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
-    if (rec && L.hidx < n_rec) {
+    if (rec && hidx < n_rec) {
       clr_record q;
       q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
       q.status = L.st; q.native_id = nat; q.mirror_id = mir; q.interactions = L.ic;
-      q.failure = L.pf; q.draws = (int32_t)L.r.draws;
-      rec[L.hidx] = q;
+      q.failure = L.pf; q.draws = (int32_t)rng_draws(L.r);
+      rec[hidx] = q;
     }
   }
   atomicAdd(&T.status[L.st], 1u);
@@ -1867,7 +1891,9 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
       const uint64_t t_pass = __builtin_amdgcn_s_memtime();
       if (op == K_RECYCLE) {
         if (fin) {
-          finish_lane(L, P, T, A, rec, Lp.n_records);
+          // history number and jackknife group from the stream counter
+          const uint64_t hidx = rng_counter(L.r) - w_offset;
+          finish_lane(L, P, T, A, rec, Lp.n_records, hidx, (int)((hidx * G) / w_total));
           alive = false;
         }
         // refill: one LDS atomic per wave hands consecutive history numbers
@@ -1884,9 +1910,7 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
           if (need) {
             const unsigned int k = base + rank;
             if ((uint64_t)k < w_count) {
-              L.hidx = w_start + k;
-              L.group = (int)((L.hidx * G) / w_total);
-              rng_seed(L.r, w_offset + L.hidx);
+              rng_seed(L.r, w_offset + w_start + k);
               L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
               L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
               L.flags = 0u;
@@ -1899,6 +1923,7 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
           }
         }
       } else if (op == K_WATER) {
+        if (alive && (L.region == R_WPART || L.region == R_WATER)) rng_topup(L.r);
         if (alive && L.region == R_WPART) water_particle<kPlain>(L, P, A);
         if (alive && L.region == R_WATER) water_flight<kPlain>(L, P, A);
       } else if (op == K_IFACE) {
