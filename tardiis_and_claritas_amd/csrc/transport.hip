@@ -1885,9 +1885,16 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
       uint64_t bn = (uint64_t)__popcll(m_a), bc = cost[K_ACRYLIC];
       const uint64_t cnt[K_N] = {(uint64_t)__popcll(m_rc), (uint64_t)__popcll(m_w), (uint64_t)__popcll(m_i),
                                  (uint64_t)__popcll(m_a), (uint64_t)__popcll(m_h)};
+#ifndef CLR_WATER_BIAS
+#define CLR_WATER_BIAS 8
+#endif
+      // the water pass's cost is weighted up by CLR_WATER_BIAS/4 so lanes parked
+      // at cheap handlers return to the water group sooner
 #pragma unroll
-      for (int k = 0; k < K_N; ++k)
-        if (cnt[k] * bc > bn * cost[k]) { op = k; bn = cnt[k]; bc = cost[k]; }
+      for (int k = 0; k < K_N; ++k) {
+        const uint64_t ck = (k == K_WATER) ? (cost[k] * CLR_WATER_BIAS) >> 2 : cost[k];
+        if (cnt[k] * bc > bn * ck) { op = k; bn = cnt[k]; bc = ck; }
+      }
       const uint64_t t_pass = __builtin_amdgcn_s_memtime();
       if (op == K_RECYCLE) {
         if (fin) {
