@@ -71,3 +71,86 @@ def flops_per_history_model(counters) -> float:
 
 def total_counters(tallies) -> np.ndarray:
     return np.sum([t['counters'] for t in tallies], axis=0)
+
+
+class Autotuner:
+    """Launch-shape autotuner for the persistent transport kernel.
+
+    Candidates are (threads per workgroup, resident workgroups per CU,
+    histories per work item).  Each is timed on the prepared batch with the
+    real budgets scaled down to a short sample (the same configurations and
+    the same mix, so divergence and tails are representative), and the
+    fastest shape is kept per workload key.  No reference counterpart."""
+
+    def __init__(self, candidates=None):
+        self.candidates = candidates or [
+            dict(block=256, blocks_per_cu=0, chunk=0),      # library default
+            dict(block=256, blocks_per_cu=0, chunk=4096),
+            dict(block=256, blocks_per_cu=0, chunk=16384),
+            dict(block=256, blocks_per_cu=0, chunk=65536),
+            dict(block=192, blocks_per_cu=4, chunk=0),
+            dict(block=128, blocks_per_cu=6, chunk=0),
+            dict(block=256, blocks_per_cu=2, chunk=0),
+        ]
+        self.cache = {}
+        self.last = []
+
+    def tune(self, runner: BatchedRunner, budgets, seed, key=None, sample_fraction=0.05, min_sample=20_000_000):
+        """Return the best launch dict for `runner`'s prepared batch."""
+        if key is not None and key in self.cache:
+            return self.cache[key]
+        total = float(sum(budgets))
+        f = min(1.0, max(sample_fraction, min_sample / max(total, 1.0)))
+        sample = [max(1, int(round(b * f))) for b in budgets]
+        saved = runner.launch
+        results = []
+        for cand in self.candidates:
+            runner.launch = cand
+            runner.run_prepared(sample, seed)            # warm-up: graph capture for this shape
+            runner.run_prepared(sample, seed + 1)
+            results.append((sum(sample) / (runner.last_ms * 1e-3), cand))
+        runner.launch = saved
+        self.last = results
+        best = max(results, key=lambda r: r[0])[1]
+        if key is not None:
+            self.cache[key] = best
+        return best
+
+
+def run_config3(n=256, seed=20261203, tune=True, log=print):
+    """Config 3: n sampled configurations with log-uniform budgets in one
+    persistent batched launch; returns a summary dict."""
+    import time
+    from . import workloads
+    p = workloads.config3_params(n, seed)
+    runner = BatchedRunner(0)
+    t0 = time.time()
+    specs = [workloads.config3_item(p, i) for i in range(n)]
+    runner.prepare(specs)
+    prep_s = time.time() - t0
+    budgets = [int(b) for b in p['budget']]
+    out = dict(n_configs=n, histories=int(sum(budgets)), prepare_seconds=prep_s)
+    runner.launch = None
+    runner.run_prepared(budgets, seed)
+    T = runner.run_prepared(budgets, seed + 1)
+    out['default_ms'] = runner.last_ms
+    out['default_histories_per_s'] = sum(budgets) / (runner.last_ms * 1e-3)
+    if tune:
+        tuner = Autotuner()
+        best = tuner.tune(runner, budgets, seed + 2)
+        out['autotune'] = [dict(shape=c, sample_histories_per_s=r) for r, c in tuner.last]
+        runner.launch = best
+        runner.run_prepared(budgets, seed)
+        T = runner.run_prepared(budgets, seed + 1)
+        out['tuned_shape'] = best
+        out['tuned_ms'] = runner.last_ms
+        out['tuned_histories_per_s'] = sum(budgets) / (runner.last_ms * 1e-3)
+    out['failures'] = int(sum(int(t['failures'].sum()) for t in T))
+    out['status_exit_fraction'] = float(sum(int(t['status'][1]) for t in T) / sum(budgets))
+    log(out)
+    return out
+
+
+if __name__ == '__main__':
+    import json
+    print(json.dumps(run_config3(log=lambda s: None)))

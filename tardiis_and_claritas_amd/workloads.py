@@ -118,3 +118,46 @@ def config5(event_model='plain_mie', n=(128, 128, 256), seed=SEED):
               sediment_field_zmax_m=zmax, sediment_field_half_width_m=rin, aux_detector_angles_deg=(135.0,))
     mat = lognormal(20e-6, 1.8, 40, 1.46, 1e-4)
     return kw, mat, 20.0, 10_000_000_000, seed
+
+
+def measure(name, histories=None, event_model='plain_mie'):
+    """Run one workload on the GPU and return a summary dict (throughput from
+    the library's HIP-event kernel time, detector signals, failure counts)."""
+    import time
+    from .model import ClaritasForwardModel
+    if name == 'config1':
+        kw, mat, conc, n, seed = config1(event_model)
+    elif name == 'config5':
+        kw, mat, conc, n, seed = config5(event_model)
+    elif name.startswith('config2:'):
+        kw, mat, conc, n, seed = config2(float(name.split(':')[1]), event_model)
+    else:
+        raise ValueError(name)
+    n = int(histories or n)
+    m = ClaritasForwardModel(**kw)
+    t0 = time.time()
+    m.prepare(mat, conc)
+    prep = time.time() - t0
+    t1 = time.time()
+    T, _, ms = m.run_tallies(n, seed)
+    wall = time.time() - t1
+    det = T['det'].sum(0)
+    out = dict(workload=name, event_model=event_model, histories=n, kernel_ms=ms, wall_s=wall, prepare_s=prep,
+               histories_per_s=n / (ms * 1e-3), steps_per_history=float(T['counters'][30]) / n,
+               s0=0.5 * float(det[2, 0] + det[3, 0]) / n, s90=0.5 * float(det[2, 9] + det[3, 9]) / n,
+               failures=int(T['failures'].sum()), status=[int(v) for v in T['status']])
+    if T['aux_window'].size:
+        aw = T['aux_window'].sum(0)
+        out['s135'] = 0.5 * float(aw[0, 0] + aw[1, 0]) / n
+    return out
+
+
+if __name__ == '__main__':
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description='run one benchmark workload on the GPU')
+    ap.add_argument('name', help="config1 | config5 | config2:<mg/L>")
+    ap.add_argument('--histories', type=float, default=None)
+    ap.add_argument('--event-model', default='plain_mie')
+    a = ap.parse_args()
+    print(json.dumps(measure(a.name, a.histories, a.event_model)))
