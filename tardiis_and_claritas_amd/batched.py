@@ -77,25 +77,26 @@ class Autotuner:
     """Launch-shape autotuner for the persistent transport kernel.
 
     Candidates are (threads per workgroup, resident workgroups per CU,
-    histories per work item).  Each is timed on the prepared batch with the
-    real budgets scaled down to a short sample (the same configurations and
-    the same mix, so divergence and tails are representative), and the
-    fastest shape is kept per workload key.  No reference counterpart."""
+    histories per work item; 0 = guided item sizes).  Each is timed on the
+    prepared batch with the real budgets scaled down to a sample large enough
+    that the launch is not dominated by its fixed costs and end-of-launch tail
+    (the same configurations and the same mix, so divergence is
+    representative), and the fastest shape is kept per workload key.  No
+    reference counterpart."""
 
     def __init__(self, candidates=None):
         self.candidates = candidates or [
-            dict(block=256, blocks_per_cu=0, chunk=0),      # library default
-            dict(block=256, blocks_per_cu=0, chunk=4096),
-            dict(block=256, blocks_per_cu=0, chunk=16384),
-            dict(block=256, blocks_per_cu=0, chunk=65536),
+            dict(block=256, blocks_per_cu=0, chunk=0),      # library default: guided item sizes
             dict(block=192, blocks_per_cu=4, chunk=0),
             dict(block=128, blocks_per_cu=6, chunk=0),
             dict(block=256, blocks_per_cu=2, chunk=0),
+            dict(block=256, blocks_per_cu=0, chunk=4096),
+            dict(block=256, blocks_per_cu=0, chunk=16384),
         ]
         self.cache = {}
         self.last = []
 
-    def tune(self, runner: BatchedRunner, budgets, seed, key=None, sample_fraction=0.05, min_sample=20_000_000):
+    def tune(self, runner: BatchedRunner, budgets, seed, key=None, sample_fraction=0.25, min_sample=100_000_000):
         """Return the best launch dict for `runner`'s prepared batch."""
         if key is not None and key in self.cache:
             return self.cache[key]
@@ -115,6 +116,51 @@ class Autotuner:
         if key is not None:
             self.cache[key] = best
         return best
+
+
+class Pipeline:
+    """A stream of batches over the same prepared configurations with `depth`
+    launches in flight on one GPU.
+
+    Each stage owns a library context (its own HIP stream, slots and captured
+    graph) and a host thread; batch k runs on stage k mod depth.  The device
+    overlaps the launches: the next batch's workgroups take the compute units
+    that the previous launch frees while its last histories drain, so the
+    end-of-launch tail of one batch is filled by the next.  Results are
+    returned in batch order.  No reference counterpart."""
+
+    def __init__(self, specs, depth=2, device=0, launch=None):
+        if depth < 1:
+            raise ValueError('depth must be >= 1')
+        self.stages = [BatchedRunner(device, launch) for _ in range(depth)]
+        for r in self.stages:
+            r.prepare(specs)
+        self.last_ms = []
+
+    @property
+    def depth(self):
+        return len(self.stages)
+
+    def run(self, budgets_list, seeds):
+        """budgets_list[k], seeds[k] describe batch k; returns per batch the
+        list of tally dicts; `last_ms` holds each launch's device time."""
+        from concurrent.futures import ThreadPoolExecutor
+        n = len(budgets_list)
+        if len(seeds) != n:
+            raise ValueError('one seed per batch')
+        out, ms = [None] * n, [0.0] * n
+
+        def stage(i):
+            r = self.stages[i]
+            for k in range(i, n, self.depth):
+                out[k] = r.run_prepared(budgets_list[k], seeds[k])
+                ms[k] = r.last_ms
+
+        with ThreadPoolExecutor(self.depth) as ex:
+            for f in [ex.submit(stage, i) for i in range(min(self.depth, n))]:
+                f.result()
+        self.last_ms = ms
+        return out
 
 
 def run_config3(n=256, seed=20261203, tune=True, log=print):

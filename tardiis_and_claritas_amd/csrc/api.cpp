@@ -380,8 +380,16 @@ __global__ void clear_kernel(unsigned long long* __restrict__ p, size_t n_words,
 }
 
 struct RunShape {
-  int block, grid, chunk, plain;
+  int block, grid, chunk, min_chunk, plain;
+  bool fixed;  // chunk given by the caller: every item has that size
 };
+
+// Guided item sizes: an item holds the histories queued behind it divided by
+// kGuidedSplit x grid, clamped to [kMinChunk, chunk] and rounded down to whole
+// blocks, so the items a launch hands out last are small and its blocks run out
+// of work together instead of waiting on one block's last large item.
+constexpr uint64_t kGuidedSplit = 2;
+constexpr int kMinChunk = 2048;
 
 RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int plain) {
   RunShape r{};
@@ -391,27 +399,46 @@ RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int pl
   const int occ = clr_transport_occupancy(r.block, plain);
   int per_cu = (launch && launch->blocks_per_cu > 0) ? std::min(launch->blocks_per_cu, occ) : occ;
   r.grid = std::max(1, c->n_cu * per_cu);
-  if (launch && launch->chunk > 0) r.chunk = launch->chunk;
+  r.fixed = launch && launch->chunk > 0;
+  if (r.fixed) r.chunk = launch->chunk;
   else {
     const uint64_t want = total / ((uint64_t)r.grid * 8ull);
-    r.chunk = (int)std::min<uint64_t>(65536, std::max<uint64_t>(2048, want));
+    r.chunk = (int)std::min<uint64_t>(65536, std::max<uint64_t>(kMinChunk, want));
   }
   r.chunk = std::max(r.block, (r.chunk / r.block) * r.block);
+  r.min_chunk = std::min(r.chunk, std::max(r.block, (kMinChunk / r.block) * r.block));
   return r;
 }
 
-void append_items(std::vector<WorkItem>& items, int cfg, uint64_t n, uint64_t seed, uint64_t offset, int chunk) {
-  for (uint64_t s = 0; s < n; s += (uint64_t)chunk) {
-    WorkItem w{};
-    w.config = cfg;
-    w.start = s;
-    w.count = std::min<uint64_t>((uint64_t)chunk, n - s);
-    w.n_total = n;
-    w.seed = seed;
-    w.offset = offset;
-    items.push_back(w);
+// Builds the device work queue in the order items are taken.
+struct QueueBuilder {
+  const RunShape& sh;
+  uint64_t remaining;  // histories queued from the next item on
+  std::vector<WorkItem> items;
+  QueueBuilder(const RunShape& s, uint64_t total) : sh(s), remaining(total) {}
+  // histories [start, start + n) of a configuration whose run numbers n_total
+  void add(int cfg, uint64_t start, uint64_t n, uint64_t n_total, uint64_t seed, uint64_t offset) {
+    for (uint64_t s = 0; s < n;) {
+      uint64_t c = (uint64_t)sh.chunk;
+      if (!sh.fixed) {
+        const uint64_t g = remaining / (kGuidedSplit * (uint64_t)sh.grid);
+        c = std::min<uint64_t>(c, std::max<uint64_t>((uint64_t)sh.min_chunk, g));
+        c = std::max<uint64_t>((uint64_t)sh.block, (c / sh.block) * sh.block);
+      }
+      c = std::min<uint64_t>(c, n - s);
+      WorkItem w{};
+      w.config = cfg;
+      w.start = start + s;
+      w.count = c;
+      w.n_total = n_total;
+      w.seed = seed;
+      w.offset = offset;
+      items.push_back(w);
+      s += c;
+      remaining -= std::min(remaining, c);
+    }
   }
-}
+};
 
 bool slots_plain(clr_ctx* c, const std::vector<int>& used) {
   for (int k : used) {
@@ -709,18 +736,9 @@ int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_la
     const uint64_t m = std::min(per_launch, n - done);
     // histories of this launch keep their global group by numbering them
     // within the whole run: items carry start offsets into [0, n)
-    std::vector<WorkItem> items;
-    for (uint64_t s = done; s < done + m; s += (uint64_t)sh.chunk) {
-      WorkItem w{};
-      w.config = 0;
-      w.start = s;
-      w.count = std::min<uint64_t>((uint64_t)sh.chunk, done + m - s);
-      w.n_total = n;
-      w.seed = seed;
-      w.offset = offset;
-      items.push_back(w);
-    }
-    if (int e = launch_items(c, items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0)) return e;
+    QueueBuilder q(sh, m);
+    q.add(0, done, m, n, seed, offset);
+    if (int e = launch_items(c, q.items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0)) return e;
     total_ms += c->last_ms;
     HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
     const uint64_t* src = reinterpret_cast<const uint64_t*>(&host[0]);
@@ -755,10 +773,10 @@ int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_l
   }
   const int plain = slots_plain(c, used) ? 1 : 0;
   const RunShape sh = pick_shape(c, launch, total, plain);
-  std::vector<WorkItem> items;
+  QueueBuilder q(sh, total);
   for (int i = 0; i < n_items; ++i)
-    append_items(items, bi[i].config, bi[i].n_histories, bi[i].seed, bi[i].history_offset, sh.chunk);
-  if (int e = launch_items(c, items, sh, true, nullptr, 0)) return e;
+    q.add(bi[i].config, 0, bi[i].n_histories, bi[i].n_histories, bi[i].seed, bi[i].history_offset);
+  if (int e = launch_items(c, q.items, sh, true, nullptr, 0)) return e;
   const int n_slots = (int)c->slots.size();
   std::vector<clr_tallies> host(n_slots);
   HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies) * n_slots, hipMemcpyDeviceToHost));
