@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 2
+#define CLR_ABI_VERSION 3
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -131,10 +131,12 @@ typedef struct clr_config {
   double alpha1, alpha2, beam_sigma, beam_radius;
   double legacy_accept_deg;
   double wavelength;            /* vacuum wavelength (m) */
-  double rough_alpha;           /* particle Beckmann alpha; only 0 supported */
+  double rough_alpha;           /* particle Beckmann alpha = tan(rms slope), CLARITAS.py:5701-5704 */
   double aux_angles_deg[CLR_MAX_AUX];
   double field_zmax;            /* top of the sediment grid (m) */
   double field_xy_half;         /* half-width of the Cartesian field (m) */
+  double spheroid_aspect;       /* wave_spheroid_aspect_ratio a/b, CLARITAS.py:5706 */
+  double spheroid_kappa;        /* wave_spheroid_orientation_kappa, CLARITAS.py:5710 */
   int32_t free_surface_model;
   int32_t source_angular_model;
   int32_t event_model;
@@ -147,6 +149,8 @@ typedef struct clr_config {
   int32_t n_groups;             /* jackknife groups, 1..CLR_MAX_GROUPS */
   int32_t field_n0, field_n1, field_n2; /* r-z: (nr,nz,-); xyz: (nx,ny,nz) */
   int32_t field_nbins;          /* 1 or n_bins (CLR_SED_GRID_PER_BIN) */
+  int32_t spheroid_orientation; /* 0 isotropic, 1 local_tangential_axial_vmf, CLARITAS.py:5564 */
+  int32_t pad_;
 } clr_config;
 
 /* ---- particle population (one PSD, CLARITAS.py:6036-6060) ---- */
@@ -342,15 +346,18 @@ size_t clr_mlp_param_count(clr_ctx* ctx);
 int clr_mlp_get_params(clr_ctx* ctx, float* out, size_t n);
 int clr_mlp_set_params(clr_ctx* ctx, const float* in, size_t n);
 /* Train: `epochs` passes over (x[n][in_dim], y[n][out_dim]) in minibatches
- * of `batch`, Adam(lr, b1, b2, eps), MSE loss; each member sees its own
- * shuffle (seed). Returns final mean loss per member in loss[n_members]. */
-int clr_mlp_train(clr_ctx* ctx, const float* x, const float* y, int64_t n,
-                  int32_t batch, int32_t epochs, float lr, uint64_t seed,
-                  float* loss);
+ * of `batch`, Adam(lr, b1, b2, eps), weighted MSE loss
+ * sum w (out - y)^2 / (rows * out_dim) with per-element weights w[n][out_dim]
+ * (w = NULL: all ones); each member sees its own shuffle (seed). Returns the
+ * final mean loss per member in loss[n_members]. */
+int clr_mlp_train(clr_ctx* ctx, const float* x, const float* y, const float* w,
+                  int64_t n, int32_t batch, int32_t epochs, float lr,
+                  uint64_t seed, float* loss);
 /* One forward+backward on a minibatch without an optimizer step; gradients
- * of the mean MSE loss copied to grads (same layout as params). */
-int clr_mlp_grad(clr_ctx* ctx, const float* x, const float* y, int32_t n,
-                 float* loss, float* grads);
+ * of the weighted mean MSE loss (w may be NULL) copied to grads (same layout
+ * as params). */
+int clr_mlp_grad(clr_ctx* ctx, const float* x, const float* y, const float* w,
+                 int32_t n, float* loss, float* grads);
 /* Inference: mean[n][out_dim] and std[n][out_dim] over the members. */
 int clr_mlp_infer(clr_ctx* ctx, const float* x, int64_t n, float* mean,
                   float* stdev);

@@ -6,7 +6,7 @@ surrogate); it restates the network the HIP library trains:
 
   n_members independent MLPs, in_dim -> hidden x n_hidden_layers -> out_dim,
   tanh-form GELU after every hidden layer, MSE loss averaged over rows and
-  outputs, parameters flat member-major with each layer's W [out][in] then b.
+  outputs (optionally with per-element weights), parameters flat member-major with each layer's W [out][in] then b.
 
 With `bf16=True` it rounds to bfloat16 (round to nearest even) exactly where the
 device does: every weight matrix, the activation written by each hidden layer,
@@ -87,12 +87,14 @@ def member_forward(layers, x, bf16=False):
     return h @ _rw(W, bf16).T + b, cache, h
 
 
-def member_loss_grad(layers, x, t, bf16=False):
+def member_loss_grad(layers, x, t, bf16=False, w=None):
+    """Loss sum w (out - t)^2 / (n d) (w = 1 when absent) and its gradients."""
     out, cache, h_last = member_forward(layers, x, bf16)
     n, d = out.shape
     diff = out - t
-    loss = float(np.sum(diff * diff) / (n * d))
-    dout = 2.0 * diff / (n * d)
+    wt = np.ones_like(diff) if w is None else np.asarray(w, dtype=np.float64)
+    loss = float(np.sum(wt * diff * diff) / (n * d))
+    dout = 2.0 * wt * diff / (n * d)
     grads = [None] * len(layers)
     W, b = layers[-1]
     grads[-1] = (dout.T @ h_last, dout.sum(0))
@@ -108,11 +110,11 @@ def member_loss_grad(layers, x, t, bf16=False):
     return loss, grads
 
 
-def ensemble_loss_grad(shape: Shape, flat, x, t, bf16=False):
+def ensemble_loss_grad(shape: Shape, flat, x, t, bf16=False, w=None):
     """Per-member losses [n_members] and flat gradient (same layout as flat)."""
     losses, glist = [], []
     for layers in shape.unpack(flat):
-        loss, g = member_loss_grad(layers, x, t, bf16)
+        loss, g = member_loss_grad(layers, x, t, bf16, w)
         losses.append(loss)
         glist.append(g)
     return np.array(losses), shape.pack(glist)

@@ -14,12 +14,11 @@
  *     instead of the reference's three xorshift32 words per ray
  *     (CLARITAS.py:1125-1133, 4612);
  *   - 64-bit history numbers (the reference stops at 2^32, CLARITAS.py:7191).
- * Supported physics: the reference-parity mode (smooth particles, rough_alpha
- * = 0, so the Smith walk reduces to CLARITAS.py:2336-2356; coherent wave
- * fraction 1, so spheroid morphology is unreachable), absorbing bores, ideal
- * photodiodes, plus the extensions named in claritas.h (plain-Mie event law,
- * collimated disk source, Rankine surface, shared FP32 sediment fields,
- * auxiliary bores).
+ * Supported physics: smooth and rough (Smith microsurface walk) particle
+ * surfaces, the coherent Mie and the incoherent spheroid-morphology wave
+ * channels, absorbing bores, ideal photodiodes, plus the extensions named in
+ * claritas.h (plain-Mie event law, collimated disk source, Rankine surface,
+ * shared FP32 sediment fields, auxiliary bores).
  */
 #include "oracle.h"
 
@@ -51,6 +50,8 @@ enum {
 enum {
   SUB_NONE = 0, SUB_SPHERE_ENTRY_SIDE = 1001, SUB_SPHERE_CHORD = 1002, SUB_SPHERE_EXIT_SIDE = 1003,
   SUB_SPHERE_INTERNAL_SIDE = 1004, SUB_SPHERE_BOUNCE_LIMIT = 1005,
+  SUB_SPHEROID_ASPECT = 1101, SUB_SPHEROID_AXES = 1102, SUB_SPHEROID_ENTRY = 1103, SUB_SPHEROID_CHORD = 1104,
+  SUB_SPHEROID_EXIT_SIDE = 1105, SUB_SPHEROID_INTERNAL_SIDE = 1106, SUB_SPHEROID_BOUNCE_LIMIT = 1107,
   SUB_ACRYLIC_NO_HIT = 2001, SUB_ACRYLIC_BOUNCE = 2002, SUB_BOTTOM_TOPOLOGY = 2104,
   SUB_WATER_NO_BOUNDARY = 3001, SUB_ITERATION_LIMIT = 3002, SUB_NONFINITE_POSITION = 3003,
   SUB_NONFINITE_DIRECTION = 3004, SUB_WATER_OUTSIDE = 3005, SUB_HEADSPACE_NO_HIT = 3006,
@@ -298,6 +299,8 @@ typedef struct {
   float field_zmax, field_half;
   int fs_model, src_model, event_model, sed_model, max_pb, max_iter, cond_trials, n_aux, n_groups;
   int fn0, fn1, fn2, fnbins;
+  float rough, aspect, kappa;   /* Beckmann alpha, spheroid a/b, vMF kappa */
+  int orientation;              /* spheroid axis law */
 } fparams;
 
 struct orc_model {
@@ -1527,42 +1530,398 @@ static int sphere_accessible(const fparams* P, double cx, double cy, double cz, 
   double tol = 128.0 * DEPS * scale;
   return (wall >= -tol && bottom >= -tol && top >= -tol) ? 1 : 0;
 }
-/* smooth-interface limit of the microsurface walk, CLARITAS.py:2323-2356 */
+/* ----------------------------------------------------------------------
+ * Rough particle surfaces: Smith microsurface random walk over a Beckmann
+ * slope distribution (Heitz, Hanika, d'Eon & Dachsbacher, ACM TOG 35(4),
+ * 2016), in the signed, log-domain form of CLARITAS.py:1792-2396.
+ * ---------------------------------------------------------------------- */
+enum { MS_HIT = 1, MS_ESCAPE = 2, MS_NUMERIC = 3, MS_WATCHDOG = 4 };
+/* walk failure subcodes, CLARITAS.py:1810-1839 */
+enum {
+  MF_PROJ_AREA = 10, MF_CDF = 11, MF_INV_CDF = 12, MF_VISIBILITY = 13, MF_NORMALISATION = 14,
+  MF_HEIGHT_SAMPLE = 21, MF_LAMBDA = 23, MF_C0 = 24, MF_LOG_C0 = 25, MF_G1 = 26, MF_ESCAPE_PROB = 27,
+  MF_RANDOM = 28, MF_LOG_CNEXT = 29, MF_NORMAL = 30, MF_CNEXT_RANGE = 36, MF_HEIGHT_INV = 37,
+  MF_HEIGHT_OUT = 38, MF_WATCHDOG = 40
+};
+#define MICRO_WATCHDOG_LIMIT 4096
+#define SQRT_PI_D 1.77245385090551602729816748334114518
+#define INV_2SQRTPI_D 0.28209479177387814347403972578038629
+
+/* inverse error function: Winitzki's closed-form estimate, then Halley
+ * steps on erf (|y| <= 1/2) or on erfc with the exact 1 - |y| (|y| > 1/2) */
+double orc_erfinv(double y) {
+  if (!(y > -1.0 && y < 1.0)) return (y == 1.0) ? INFINITY : (y == -1.0) ? -INFINITY : NAN;
+  if (y == 0.0) return 0.0;
+  const double s = (y < 0.0) ? -1.0 : 1.0, a = fabs(y), c = 1.0 - a;   /* c exact for a > 1/2 */
+  const double l = log(c * (1.0 + a)), k = 0.147, t = 2.0 / (PI_D * k) + 0.5 * l;
+  double x = sqrt(sqrt(t * t - l / k) - t);
+  for (int it = 0; it < 6; ++it) {
+    const double f = (a <= 0.5) ? (erf(x) - a) : (c - erfc(x));
+    const double d = 2.0 / SQRT_PI_D * exp(-x * x);           /* d/dx of erf and of -erfc */
+    if (!(d > 0.0)) break;
+    const double q = f / d, dx = q / (1.0 + x * q);           /* Halley: f'' = -2 x f' */
+    x -= dx;
+    if (fabs(dx) <= 1e-17 * fabs(x)) break;
+  }
+  return s * x;
+}
+
+/* B(x) = 1 - x sqrt(pi) erfcx(x), x >= 0; asymptotic series for x >= 5,
+ * CLARITAS.py:1926-1942 */
+static double beck_tail_B(double x) {
+  if (!(x >= 0.0) || !isfinite(x)) return -1.0;
+  double b;
+  if (x < 5.0) {
+    b = 1.0 - x * SQRT_PI_D * (exp(x * x) * erfc(x));
+  } else {
+    const double y = 1.0 / (x * x);
+    /* sum_k (-1)^(k+1) (2k-1)!! / 2^k y^k, k = 1..7 */
+    b = 0.5 * y - 0.75 * y * y + 1.875 * y * y * y - 6.5625 * y * y * y * y + 29.53125 * y * y * y * y * y -
+        162.421875 * y * y * y * y * y * y + 1055.7421875 * y * y * y * y * y * y * y;
+  }
+  return (isfinite(b) && b > 0.0) ? b : -1.0;
+}
+/* visible-slope marginal CDF for slope_i >= -1, CLARITAS.py:1947-1962 */
+static int beck_visible_cdf(double s, double si, double* out) {
+  if (!isfinite(s) || !isfinite(si)) return 0;
+  if (s >= si) { *out = 1.0; return 1; }
+  const double q = INV_2SQRTPI_D * exp(-si * si) + 0.5 * si * erfc(-si);
+  if (!(q > 0.0) || !isfinite(q)) return 0;
+  const double n = INV_2SQRTPI_D * exp(-s * s) + 0.5 * si * erfc(-s);
+  double f = n / q;
+  if (!isfinite(f)) return 0;
+  if (f < 0.0 && f > -1.0e-13) f = 0.0;
+  if (f > 1.0 && f < 1.0 + 1.0e-13) f = 1.0;
+  if (f < 0.0 || f > 1.0) return 0;
+  *out = f;
+  return 1;
+}
+/* log of the scaled tail CDF at slope -(x+u), CLARITAS.py:1986-2013 */
+static int beck_tail_logcdf(double x, double u, double Bx, double* out) {
+  const double y = x + u, By = beck_tail_B(y);
+  if (!(By > 0.0) || !isfinite(By)) return 0;
+  const double C = u / y + (x / y) * By;
+  if (!(C > 0.0) || !isfinite(C)) return 0;
+  const double lr = -(2.0 * x * u + u * u) + log(C / Bx);
+  if (!isfinite(lr)) return 0;
+  *out = lr;
+  return 1;
+}
+/* alpha = 1 visible slope (sx, sy) for signed cot(theta_i) = si,
+ * CLARITAS.py:1964-2045 */
+static int beck_sample_slopes(double si, double u1, double u2, double* sxo, double* syo, int* fc) {
+  *fc = 0;
+  if (!isfinite(u1) || !isfinite(u2) || !(u1 > 0.0 && u1 < 1.0) || !(u2 > 0.0 && u2 < 1.0)) { *fc = MF_CDF; return 0; }
+  const double COT_1E4 = 9999.999966666666;      /* cot(1e-4 rad): the visible law is the NDF */
+  if (isinf(si) || si > COT_1E4) {
+    if (si < 0.0) { *fc = MF_PROJ_AREA; return 0; }
+    const double r = sqrt(-log(u1)), ph = 2.0 * PI_D * u2, sx = r * cos(ph), sy = r * sin(ph);
+    if (!isfinite(sx) || !isfinite(sy)) { *fc = MF_INV_CDF; return 0; }
+    *sxo = sx; *syo = sy;
+    return 1;
+  }
+  if (!isfinite(si)) { *fc = MF_PROJ_AREA; return 0; }
+  double sx;
+  if (si < -1.0) {
+    const double x = -si, Bx = beck_tail_B(x);
+    if (!(Bx > 0.0) || !isfinite(Bx)) { *fc = MF_PROJ_AREA; return 0; }
+    const double target = log(u1);
+    double lo = 0.0, hi = fmax(1.0 / x, 1.0e-12), lr;
+    int ok = 0;
+    for (int it = 0; it < 64; ++it) {
+      if (!beck_tail_logcdf(x, hi, Bx, &lr)) { *fc = MF_CDF; return 0; }
+      if (lr <= target) { ok = 1; break; }
+      hi *= 2.0;
+      if (!isfinite(hi)) { *fc = MF_INV_CDF; return 0; }
+    }
+    if (!ok) { *fc = MF_INV_CDF; return 0; }
+    for (int it = 0; it < 52; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (!beck_tail_logcdf(x, mid, Bx, &lr)) { *fc = MF_CDF; return 0; }
+      if (lr > target) lo = mid; else hi = mid;
+    }
+    sx = -(x + 0.5 * (lo + hi));
+  } else {
+    double lo = fmin(-1.0, si - 1.0), hi = si, f = 1.0;
+    int ok = 0;
+    for (int it = 0; it < 64; ++it) {
+      if (!beck_visible_cdf(lo, si, &f)) { *fc = MF_CDF; return 0; }
+      if (f <= u1) { ok = 1; break; }
+      lo = (lo < 0.0) ? 2.0 * lo : lo - 1.0;
+      if (!isfinite(lo)) { *fc = MF_INV_CDF; return 0; }
+    }
+    if (!ok) { *fc = MF_INV_CDF; return 0; }
+    for (int it = 0; it < 48; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (!beck_visible_cdf(mid, si, &f)) { *fc = MF_CDF; return 0; }
+      if (f < u1) lo = mid; else hi = mid;
+    }
+    sx = 0.5 * (lo + hi);
+  }
+  const double ey = 2.0 * u2 - 1.0;
+  if (!(ey > -1.0 && ey < 1.0) || !isfinite(ey)) { *fc = MF_INV_CDF; return 0; }
+  const double sy = orc_erfinv(ey);
+  if (!isfinite(sx) || !isfinite(sy)) { *fc = MF_INV_CDF; return 0; }
+  *sxo = sx; *syo = sy;
+  return 1;
+}
+/* visible microfacet normal about the TOP axis b for direction w (signed:
+ * w.b may be negative), two open uniforms, CLARITAS.py:2073-2129 */
+static int beck_vndf_normal(rng_t* r, float alpha, float bx, float by, float bz, float wx, float wy, float wz,
+                            double* mx, double* my, double* mz, int* fc) {
+  *fc = 0;
+  if (!(alpha >= 0.0f) || !isfinite(alpha)) { *fc = MF_NORMALISATION; return 0; }
+  normalize3(&bx, &by, &bz);
+  normalize3(&wx, &wy, &wz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(wx) || !isfinite(wy) || !isfinite(wz)) {
+    *fc = MF_NONFINITE_DIRECTION; return 0;
+  }
+  if (alpha <= 1.0e-8f) {
+    const double vd = (double)wx * bx + (double)wy * by + (double)wz * bz;
+    if (!(vd > 0.0)) { *fc = MF_VISIBILITY; return 0; }
+    *mx = bx; *my = by; *mz = bz;
+    return 1;
+  }
+  float t1x, t1y, t1z, t2x, t2y, t2z;
+  perpendicular_basis(bx, by, bz, &t1x, &t1y, &t1z, &t2x, &t2y, &t2z);
+  const double lx = (double)wx * t1x + (double)wy * t1y + (double)wz * t1z;
+  const double ly = (double)wx * t2x + (double)wy * t2y + (double)wz * t2z;
+  const double lz = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  double sxv = (double)alpha * lx, syv = (double)alpha * ly, szv = lz;
+  const double vl = sqrt(sxv * sxv + syv * syv + szv * szv);
+  if (!(vl > 0.0) || !isfinite(vl)) { *fc = MF_NORMALISATION; return 0; }
+  sxv /= vl; syv /= vl; szv /= vl;
+  const double radial = hypot(sxv, syv);
+  double si;
+  if (radial == 0.0) {
+    if (szv > 0.0) si = INFINITY;
+    else { *fc = MF_PROJ_AREA; return 0; }
+  } else {
+    si = szv / radial;
+    if (!isfinite(si)) { *fc = MF_PROJ_AREA; return 0; }
+  }
+  const double u1 = Uopen(r), u2 = Uopen(r);
+  double s0x, s0y;
+  int f2 = 0;
+  if (!beck_sample_slopes(si, u1, u2, &s0x, &s0y, &f2)) { *fc = f2 ? f2 : MF_INV_CDF; return 0; }
+  const double ph = atan2(syv, sxv), cp = cos(ph), sp = sin(ph);
+  const double tx = cp * s0x - sp * s0y, ty = sp * s0x + cp * s0y;
+  const double sx = (double)alpha * tx, sy = (double)alpha * ty;
+  if (!isfinite(sx) || !isfinite(sy)) { *fc = MF_NORMAL; return 0; }
+  const double inv = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
+  if (!(inv > 0.0) || !isfinite(inv)) { *fc = MF_NORMALISATION; return 0; }
+  const double nlx = -sx * inv, nly = -sy * inv, nlz = inv;
+  double nx = nlx * t1x + nly * t2x + nlz * bx;
+  double ny = nlx * t1y + nly * t2y + nlz * by;
+  double nz = nlx * t1z + nly * t2z + nlz * bz;
+  const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+  if (!(nn > 0.0) || !isfinite(nn)) { *fc = MF_NORMALISATION; return 0; }
+  nx /= nn; ny /= nn; nz /= nn;
+  if (!(nx * bx + ny * by + nz * bz > 0.0)) { *fc = MF_NORMAL; return 0; }
+  if (!(nx * wx + ny * wy + nz * wz > 0.0)) { *fc = MF_VISIBILITY; return 0; }
+  *mx = nx; *my = ny; *mz = nz;
+  return 1;
+}
+/* signed Smith Lambda over the whole sphere of directions,
+ * CLARITAS.py:2162-2184 */
+double orc_smith_lambda(float wx, float wy, float wz, float bx, float by, float bz, float alpha) {
+  double c = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  if (!isfinite(c) || !isfinite(alpha) || !(alpha >= 0.0f)) return NAN;
+  c = fmax(-1.0, fmin(1.0, c));
+  const double ss2 = fmax(0.0, 1.0 - c * c);
+  if (ss2 == 0.0) return c >= 0.0 ? 0.0 : -1.0;
+  if (c == 0.0) return copysign(INFINITY, c);
+  if (alpha == 0.0f) return c >= 0.0 ? 0.0 : -1.0;
+  const double aa = fabs(c) / ((double)alpha * sqrt(ss2));
+  if (!(aa > 0.0) || !isfinite(aa)) return c >= 0.0 ? 0.0 : -1.0;
+  const double B = beck_tail_B(aa);
+  if (!(B > 0.0) || !isfinite(B)) return NAN;
+  const double lp = exp(-(aa * aa) - log(2.0 * SQRT_PI_D * aa)) * B;
+  if (!(lp >= 0.0) || !isfinite(lp)) return NAN;
+  return c >= 0.0 ? lp : (-1.0 - lp);
+}
+static double uniform_C1(double h) { return fmin(1.0, fmax(0.0, 0.5 * (h + 1.0))); }
+static double uniform_invC1(double u) { return 2.0 * fmin(1.0, fmax(0.0, u)) - 1.0; }
+/* next height of the walk for direction w at height h with the open uniform
+ * U, CLARITAS.py:2211-2310 */
+int orc_smith_height(double U, float wx, float wy, float wz, float bx, float by, float bz, float alpha, float h,
+                     float* out_h, int* fc) {
+  *fc = 0;
+  if (!isfinite(U) || !isfinite(wx) || !isfinite(wy) || !isfinite(wz) || !isfinite(h) || !(alpha >= 0.0f) ||
+      !isfinite(alpha) || !(U > 0.0 && U < 1.0)) {
+    *fc = MF_RANDOM; return MS_NUMERIC;
+  }
+  double c = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  if (!isfinite(c)) { *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
+  c = fmax(-1.0, fmin(1.0, c));
+  const double C0 = uniform_C1((double)h);
+  if (!(C0 >= 0.0 && C0 <= 1.0) || !isfinite(C0)) { *fc = MF_C0; return MS_NUMERIC; }
+  const double ss2 = fmax(0.0, 1.0 - c * c);
+  if (ss2 == 0.0) {
+    if (c > 0.0) return MS_ESCAPE;
+    if (c < 0.0) {
+      const double hn = uniform_invC1(U * C0);
+      if (!isfinite(hn)) { *fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+      *out_h = (float)hn;
+      return MS_HIT;
+    }
+    *out_h = h;
+    return MS_HIT;
+  }
+  if (c == 0.0) { *out_h = h; return MS_HIT; }
+  const double lam = orc_smith_lambda(wx, wy, wz, bx, by, bz, alpha);
+  if (isnan(lam)) { *fc = MF_LAMBDA; return MS_NUMERIC; }
+  if (c > 0.0) {
+    if (!(lam >= 0.0)) { *fc = MF_LAMBDA; return MS_NUMERIC; }
+    if (C0 <= 0.0 || C0 >= 1.0 || lam == 0.0) return MS_ESCAPE;
+    const double lc0 = log(C0);
+    if (!isfinite(lc0)) { *fc = MF_LOG_C0; return MS_NUMERIC; }
+    const double lg1 = lam * lc0;
+    if (!(lg1 <= 0.0) || !isfinite(lg1)) { *fc = MF_G1; return MS_NUMERIC; }
+    const double p_hit = -expm1(lg1);
+    if (!(p_hit >= 0.0 && p_hit <= 1.0) || !isfinite(p_hit)) { *fc = MF_ESCAPE_PROB; return MS_NUMERIC; }
+    if (U > p_hit) return MS_ESCAPE;
+    const double l1u = log1p(-U);
+    if (!isfinite(l1u)) { *fc = MF_RANDOM; return MS_NUMERIC; }
+    double lcn = lc0 - l1u / lam;
+    if (!isfinite(lcn)) { *fc = MF_LOG_CNEXT; return MS_NUMERIC; }
+    if (lcn > 64.0 * DEPS) { *fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+    if (lcn > 0.0) lcn = 0.0;
+    const double cn = exp(lcn);
+    if (!(cn >= 0.0 && cn <= 1.0) || !isfinite(cn)) { *fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+    const double hn = uniform_invC1(cn);
+    if (!isfinite(hn)) { *fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+    *out_h = (float)hn;
+    if (!isfinite(*out_h)) { *fc = MF_HEIGHT_OUT; return MS_NUMERIC; }
+    return MS_HIT;
+  }
+  if (!(lam <= -1.0) || !isfinite(lam)) { *fc = MF_LAMBDA; return MS_NUMERIC; }
+  if (C0 <= 0.0) { *out_h = -1.0f; return MS_HIT; }
+  const double lc0 = log(C0);
+  if (!isfinite(lc0)) { *fc = MF_LOG_C0; return MS_NUMERIC; }
+  const double l1u = log1p(-U);
+  if (!isfinite(l1u)) { *fc = MF_RANDOM; return MS_NUMERIC; }
+  const double lcn = lc0 - l1u / lam;
+  if (!isfinite(lcn)) { *fc = MF_LOG_CNEXT; return MS_NUMERIC; }
+  if (lcn > 0.0) { *fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+  const double cn = exp(lcn);
+  if (!(cn >= 0.0 && cn <= C0 * (1.0 + 8.0e-15)) || !isfinite(cn)) { *fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+  const double hn = uniform_invC1(cn);
+  if (!isfinite(hn)) { *fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+  *out_h = (float)hn;
+  if (!isfinite(*out_h)) { *fc = MF_HEIGHT_OUT; return MS_NUMERIC; }
+  return MS_HIT;
+}
+
+/* smooth-interface limit of the microsurface walk, CLARITAS.py:2336-2356 */
 static int smooth_interface(rng_t* r, float bx, float by, float bz, float n_top, float k_top,
                             float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top,
                             int* fc) {
-  *fc = 0;
-  normalize3(&bx, &by, &bz);
-  normalize3(dx, dy, dz);
-  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(*dx) || !isfinite(*dy) || !isfinite(*dz)) {
-    *fc = MF_NONFINITE_DIRECTION; return 0;
-  }
-  if (dot3(*dx, *dy, *dz, bx, by, bz) >= -1.0e-7f) { *fc = MF_SIDE_STATE; return 0; }
   float ci = fminf(fmaxf(dot3(-(*dx), -(*dy), -(*dz), bx, by, bz), 0.0f), 1.0f);
   float R, ts, tn;
   if (!complex_interface(ci, n_top, k_top, n_bot, k_bot, &R, &ts, &tn) || !isfinite(R)) {
-    *fc = MF_FRESNEL_INVALID; return 0;
+    *fc = MF_FRESNEL_INVALID; return MS_NUMERIC;
   }
   float ox, oy, oz;
   if (U(r) < R) {
     reflect3(*dx, *dy, *dz, bx, by, bz, &ox, &oy, &oz);
     *out_top = 1;
-    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_REFLECTION_INVALID; return 0; }
+    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_REFLECTION_INVALID; return MS_NUMERIC; }
   } else {
     if (!energy_refract3(*dx, *dy, *dz, bx, by, bz, n_top, k_top, n_bot, k_bot, &ox, &oy, &oz)) {
-      *fc = MF_REFRACTION_INVALID; return 0;
+      *fc = MF_REFRACTION_INVALID; return MS_NUMERIC;
     }
     *out_top = 0;
   }
-  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_NONFINITE_DIRECTION; return 0; }
+  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
   *dx = ox; *dy = oy; *dz = oz;
-  return 1;
+  return MS_ESCAPE;
+}
+
+/* one encounter of direction d with a (rough) dielectric interface whose
+ * TOP side has normal b: returns MS_ESCAPE with out_top = 1 (left on the
+ * top side) or 0 (crossed), else MS_NUMERIC / MS_WATCHDOG with fc set.
+ * CLARITAS.py:2314-2430 */
+static int micro_interface(rng_t* r, float alpha, float bx, float by, float bz, float n_top, float k_top,
+                           float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top, int* fc) {
+  *fc = 0;
+  normalize3(&bx, &by, &bz);
+  normalize3(dx, dy, dz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(*dx) || !isfinite(*dy) || !isfinite(*dz)) {
+    *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC;
+  }
+  if (dot3(*dx, *dy, *dz, bx, by, bz) >= -1.0e-7f) { *fc = MF_SIDE_STATE; return MS_NUMERIC; }
+  if (alpha <= 1.0e-7f) return smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+  float h = 1.0f + (2.0f * 0.999f - 1.0f);
+  int outside = 1;
+  for (int it = 0; it < MICRO_WATCHDOG_LIMIT; ++it) {
+    float nh = 0.0f;
+    int hs, f2 = 0;
+    const double Uh = Uopen(r);
+    if (outside) {
+      hs = orc_smith_height(Uh, *dx, *dy, *dz, bx, by, bz, alpha, h, &nh, &f2);
+      if (hs == MS_ESCAPE) { *out_top = 1; return MS_ESCAPE; }
+    } else {
+      float t = 0.0f;
+      hs = orc_smith_height(Uh, -(*dx), -(*dy), -(*dz), bx, by, bz, alpha, -h, &t, &f2);
+      if (hs == MS_ESCAPE) { *out_top = 0; return MS_ESCAPE; }
+      if (hs == MS_HIT) nh = -t;
+    }
+    if (hs != MS_HIT || !isfinite(nh)) { *fc = f2 ? f2 : MF_HEIGHT_SAMPLE; return MS_NUMERIC; }
+    h = nh;
+    const float wix = -(*dx), wiy = -(*dy), wiz = -(*dz);
+    double mx, my, mz;
+    f2 = 0;
+    if (outside) {
+      if (!beck_vndf_normal(r, alpha, bx, by, bz, wix, wiy, wiz, &mx, &my, &mz, &f2)) {
+        *fc = f2 ? f2 : MF_INV_CDF; return MS_NUMERIC;
+      }
+    } else {
+      double tx, ty, tz;
+      if (!beck_vndf_normal(r, alpha, bx, by, bz, -wix, -wiy, -wiz, &tx, &ty, &tz, &f2)) {
+        *fc = f2 ? f2 : MF_INV_CDF; return MS_NUMERIC;
+      }
+      mx = -tx; my = -ty; mz = -tz;
+    }
+    const double vis = (double)wix * mx + (double)wiy * my + (double)wiz * mz;
+    if (!isfinite(mx) || !isfinite(my) || !isfinite(mz)) { *fc = MF_NORMAL; return MS_NUMERIC; }
+    if (!(vis > 0.0)) { *fc = MF_VISIBILITY; return MS_NUMERIC; }
+    const float ci = (float)fmin(1.0, fmax(0.0, vis));
+    const float n1 = outside ? n_top : n_bot, k1 = outside ? k_top : k_bot;
+    const float n2 = outside ? n_bot : n_top, k2 = outside ? k_bot : k_top;
+    if (!(n1 > 0.0f && n2 > 0.0f)) { *fc = MF_REFRACTION_INVALID; return MS_NUMERIC; }
+    float R, ts, tn;
+    if (!complex_interface(ci, n1, k1, n2, k2, &R, &ts, &tn) || !isfinite(R)) { *fc = MF_FRESNEL_INVALID; return MS_NUMERIC; }
+    double ox, oy, oz;
+    if (U(r) < R) {
+      const double dd = (double)(*dx) * mx + (double)(*dy) * my + (double)(*dz) * mz;
+      ox = (double)(*dx) - 2.0 * dd * mx; oy = (double)(*dy) - 2.0 * dd * my; oz = (double)(*dz) - 2.0 * dd * mz;
+    } else {
+      float tx, ty, tz;
+      if (!energy_refract3(*dx, *dy, *dz, (float)mx, (float)my, (float)mz, n1, k1, n2, k2, &tx, &ty, &tz)) {
+        *fc = MF_REFRACTION_INVALID; return MS_NUMERIC;
+      }
+      ox = tx; oy = ty; oz = tz;
+      outside = 1 - outside;
+    }
+    const double qn = sqrt(ox * ox + oy * oy + oz * oz);
+    if (!(qn > 0.0) || !isfinite(qn) || !isfinite(ox) || !isfinite(oy) || !isfinite(oz)) {
+      *fc = MF_REFRACTION_INVALID; return MS_NUMERIC;
+    }
+    float fx = (float)(ox / qn), fy = (float)(oy / qn), fz = (float)(oz / qn);
+    if (!isfinite(fx) || !isfinite(fy) || !isfinite(fz)) { *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
+    normalize3(&fx, &fy, &fz);
+    if (!isfinite(fx) || !isfinite(fy) || !isfinite(fz)) { *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
+    *dx = fx; *dy = fy; *dz = fz;
+  }
+  *fc = MF_WATCHDOG;
+  return MS_WATCHDOG;
 }
 
 typedef struct { int pr, per, pir, pabs; uint32_t n_abs; } pcount_t;
 
-/* smooth Fresnel sphere: entry, internal chords with Beer-Lambert
- * absorption 4 pi k / lambda, exit or internal reflection.
+/* Fresnel sphere (smooth or rough surface): entry, internal chords with
+ * Beer-Lambert absorption 4 pi k / lambda, exit or internal reflection.
  * Returns 1 escape, 2 absorbed, 0 failure.  CLARITAS.py:2497-2599 */
 static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, float k_p,
                               float radius, int max_bounces, float gnx, float gny, float gnz,
@@ -1571,9 +1930,8 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
   *pf = PF_OK; *pfs = SUB_NONE;
   if (radius <= 0.0f) return 1;
   int top = 1, fc = 0;
-  if (!smooth_interface(r, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc)) {
-    *pf = PF_MICRO_NUMERICAL; *pfs = fc; return 0;
-  }
+  int ms = micro_interface(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc);
+  if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
   if (top) { pc->pr++; pc->per++; return 1; }
   if (dot3(*vx, *vy, *vz, gnx, gny, gnz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_ENTRY_SIDE; return 0; }
   const double rr = (double)radius, rr2 = rr * rr;
@@ -1618,9 +1976,8 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
     float nox = (float)(qx / rr), noy = (float)(qy / rr), noz = (float)(qz / rr);
     normalize3(&nox, &noy, &noz);
     top = 1;
-    if (!smooth_interface(r, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc)) {
-      *pf = PF_MICRO_NUMERICAL; *pfs = fc; return 0;
-    }
+    ms = micro_interface(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc);
+    if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
     if (!top) {
       if (dot3(*vx, *vy, *vz, nox, noy, noz) <= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_EXIT_SIDE; return 0; }
       return 1;
@@ -1628,6 +1985,155 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
     if (dot3(*vx, *vy, *vz, nox, noy, noz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
     pc->pr++; pc->pir++;
     if (bounce == max_bounces) { *pf = PF_MACRO_BOUNCE; *pfs = SUB_SPHERE_BOUNCE_LIMIT; return 0; }
+  }
+  *pf = PF_OTHER; *pfs = SUB_NONE;
+  return 0;
+}
+
+/* ----------------------------------------------------------------------
+ * Incoherent wave channel: direction-only finite-particle event on a
+ * volume-preserving spheroid a = r q^(2/3), b = r q^(-1/3) (q = a/b; a
+ * sphere when q = 1), CLARITAS.py:2655-2866
+ * ---------------------------------------------------------------------- */
+/* CLARITAS.py:2660-2665 */
+static void random_axis(rng_t* r, float* ux, float* uy, float* uz) {
+  const float cz = 2.0f * U(r) - 1.0f, az = 2.0f * PI_F * U(r), sz = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
+  *ux = sz * cosf(az); *uy = sz * sinf(az); *uz = cz;
+}
+/* isotropic, or the head-tail symmetric von Mises-Fisher law about the
+ * local flow direction f (axial density ~ cosh(kappa u.f)),
+ * CLARITAS.py:2675-2702 */
+static void spheroid_axis(rng_t* r, int model, float kappa, float fx, float fy, float fz, float* ux, float* uy,
+                          float* uz) {
+  if (model == 0 || !(kappa > 1e-7f) || !isfinite(kappa)) { random_axis(r, ux, uy, uz); return; }
+  normalize3(&fx, &fy, &fz);
+  if (!(fx * fx + fy * fy + fz * fz > 0.5f)) { random_axis(r, ux, uy, uz); return; }
+  const float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+  const float em2 = expf(-2.0f * kappa);
+  float c = 1.0f + logf(u + (1.0f - u) * em2) / kappa;
+  c = fminf(fmaxf(c, -1.0f), 1.0f);
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(fx, fy, fz, &e1x, &e1y, &e1z, &e2x, &e2y, &e2z);
+  const float ph = 2.0f * PI_F * U(r), st = sqrtf(fmaxf(0.0f, 1.0f - c * c)), cp = cosf(ph), sp = sinf(ph);
+  *ux = c * fx + st * (cp * e1x + sp * e2x);
+  *uy = c * fy + st * (cp * e1y + sp * e2y);
+  *uz = c * fz + st * (cp * e1z + sp * e2z);
+  if (U(r) < 0.5f) { *ux = -*ux; *uy = -*uy; *uz = -*uz; }
+  normalize3(ux, uy, uz);
+}
+/* gradient of F = |p|^2/b^2 + delta (p.u)^2 - 1, CLARITAS.py:2720-2727 */
+static void spheroid_normal(float x, float y, float z, float ux, float uy, float uz, float invb2, float delta,
+                            float* nx, float* ny, float* nz) {
+  const float pu = x * ux + y * uy + z * uz;
+  *nx = invb2 * x + delta * pu * ux; *ny = invb2 * y + delta * pu * uy; *nz = invb2 * z + delta * pu * uz;
+  normalize3(nx, ny, nz);
+}
+/* both roots of F(o + t d) = 0, ascending, CLARITAS.py:2729-2743 */
+static int spheroid_roots(float ox, float oy, float oz, float dx, float dy, float dz, float ux, float uy, float uz,
+                          float invb2, float delta, float* t0, float* t1) {
+  const float du = dx * ux + dy * uy + dz * uz, ou = ox * ux + oy * uy + oz * uz;
+  const float A = invb2 * (dx * dx + dy * dy + dz * dz) + delta * du * du;
+  const float B = 2.0f * (invb2 * (ox * dx + oy * dy + oz * dz) + delta * ou * du);
+  const float Cc = invb2 * (ox * ox + oy * oy + oz * oz) + delta * ou * ou - 1.0f;
+  const float disc = B * B - 4.0f * A * Cc;
+  if (!(A > 0.0f) || disc < 0.0f) return 0;
+  const float sd = sqrtf(fmaxf(0.0f, disc));
+  float r0 = (-B - sd) / (2.0f * A), r1 = (-B + sd) / (2.0f * A);
+  if (r0 > r1) { const float q = r0; r0 = r1; r1 = q; }
+  *t0 = r0; *t1 = r1;
+  return 1;
+}
+/* entry point uniform over the projected silhouette (rejection from the
+ * bounding disk, 64 attempts), CLARITAS.py:2745-2771 */
+static int spheroid_entry(rng_t* r, float vx, float vy, float vz, float a, float b, float ux, float uy, float uz,
+                          float* x, float* y, float* z, float* gnx, float* gny, float* gnz) {
+  const float invb2 = 1.0f / (b * b), delta = 1.0f / (a * a) - invb2;
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(vx, vy, vz, &e1x, &e1y, &e1z, &e2x, &e2y, &e2z);
+  const float du = dot3(ux, uy, uz, vx, vy, vz);
+  const float rproj2 = b * b + (a * a - b * b) * fmaxf(0.0f, 1.0f - du * du);
+  const float rbound = sqrtf(fmaxf(b * b, rproj2)), rmax = fmaxf(a, b);
+  for (int attempt = 0; attempt < 64; ++attempt) {
+    const float rr = rbound * sqrtf(U(r)), ph = 2.0f * PI_F * U(r);
+    const float ox = rr * (cosf(ph) * e1x + sinf(ph) * e2x) - 2.5f * rmax * vx;
+    const float oy = rr * (cosf(ph) * e1y + sinf(ph) * e2y) - 2.5f * rmax * vy;
+    const float oz = rr * (cosf(ph) * e1z + sinf(ph) * e2z) - 2.5f * rmax * vz;
+    float t0, t1;
+    if (!spheroid_roots(ox, oy, oz, vx, vy, vz, ux, uy, uz, invb2, delta, &t0, &t1)) continue;
+    const float t = (t0 > 1e-9f * rmax) ? t0 : ((t1 > 1e-9f * rmax) ? t1 : -1.0f);
+    if (t <= 0.0f) continue;
+    *x = ox + t * vx; *y = oy + t * vy; *z = oz + t * vz;
+    spheroid_normal(*x, *y, *z, ux, uy, uz, invb2, delta, gnx, gny, gnz);
+    if (dot3(vx, vy, vz, *gnx, *gny, *gnz) < -1e-6f) return 1;
+  }
+  return 0;
+}
+/* CLARITAS.py:2773-2781 */
+static float spheroid_chord(float x, float y, float z, float vx, float vy, float vz, float ux, float uy, float uz,
+                            float invb2, float delta, float scale) {
+  float t0, t1;
+  if (!spheroid_roots(x, y, z, vx, vy, vz, ux, uy, uz, invb2, delta, &t0, &t1)) return -1.0f;
+  const float eps = fmaxf(1e-5f * scale, 1e-12f);
+  float t = INF_F;
+  if (t0 > eps) t = t0;
+  if (t1 > eps && t1 < t) t = t1;
+  return t < INF_F * 0.5f ? t : -1.0f;
+}
+/* direction-only event: 1 scattered (direction updated), 2 absorbed,
+ * 0 failure (pf, pfs).  (fx, fy, fz) is the local flow direction of the
+ * vMF axis law.  CLARITAS.py:2783-2841 */
+static int wave_particle_event(const fparams* P, rng_t* r, float n_m, float n_p, float k_p, float r_eq, float fx,
+                               float fy, float fz, float* vx, float* vy, float* vz, pcount_t* pc, int* pf,
+                               int* pfs) {
+  *pf = PF_OK; *pfs = SUB_NONE;
+  if (r_eq <= 0.0f) return 1;
+  const float q = P->aspect;
+  if (!(q > 0.0f) || !isfinite(q)) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_ASPECT; return 0; }
+  if (fabsf(q - 1.0f) < 1e-7f) {
+    float gx, gy, gz, lx = 0.0f, ly = 0.0f, lz = 0.0f, lvx = *vx, lvy = *vy, lvz = *vz;
+    sphere_sample(r, *vx, *vy, *vz, &gx, &gy, &gz);
+    const double cx = -(double)r_eq * (double)gx, cy = -(double)r_eq * (double)gy, cz = -(double)r_eq * (double)gz;
+    const int ps = sphere_interaction(P, r, n_m, n_p, k_p, r_eq, P->max_pb, gx, gy, gz, cx, cy, cz, &lx, &ly, &lz,
+                                      &lvx, &lvy, &lvz, pc, pf, pfs);
+    if (ps == 1) { *vx = lvx; *vy = lvy; *vz = lvz; }
+    return ps;
+  }
+  const float q13 = powf(q, 1.0f / 3.0f), a = r_eq * q13 * q13, b = r_eq / q13;
+  if (!(a > 0.0f && b > 0.0f && isfinite(a) && isfinite(b))) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_AXES; return 0; }
+  float ux, uy, uz;
+  spheroid_axis(r, P->orientation, P->kappa, fx, fy, fz, &ux, &uy, &uz);
+  float x, y, z, gnx, gny, gnz;
+  if (!spheroid_entry(r, *vx, *vy, *vz, a, b, ux, uy, uz, &x, &y, &z, &gnx, &gny, &gnz)) {
+    *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_ENTRY; return 0;
+  }
+  const float invb2 = 1.0f / (b * b), delta = 1.0f / (a * a) - invb2, scale = fmaxf(a, b);
+  int top = 1, fc = 0;
+  int ms = micro_interface(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc);
+  if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
+  if (top) { pc->pr++; pc->per++; return 1; }
+  if (dot3(*vx, *vy, *vz, gnx, gny, gnz) >= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_ENTRY; return 0; }
+  const float alpha_abs = (P->wavelength > 0.0f && k_p > 0.0f) ? (4.0f * PI_F * k_p / P->wavelength) : 0.0f;
+  for (int bounce = 0; bounce <= P->max_pb; ++bounce) {
+    const float chord = spheroid_chord(x, y, z, *vx, *vy, *vz, ux, uy, uz, invb2, delta, scale);
+    if (chord <= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_CHORD; return 0; }
+    if (alpha_abs > 0.0f) {
+      pc->n_abs++;
+      const double dist = -log(Uopen(r)) / (double)alpha_abs;
+      if (dist < (double)chord) { pc->pabs++; return 2; }
+    }
+    x += chord * (*vx); y += chord * (*vy); z += chord * (*vz);
+    float nox, noy, noz;
+    spheroid_normal(x, y, z, ux, uy, uz, invb2, delta, &nox, &noy, &noz);
+    top = 1;
+    ms = micro_interface(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc);
+    if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
+    if (!top) {
+      if (dot3(*vx, *vy, *vz, nox, noy, noz) <= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_EXIT_SIDE; return 0; }
+      return 1;
+    }
+    if (dot3(*vx, *vy, *vz, nox, noy, noz) >= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_INTERNAL_SIDE; return 0; }
+    pc->pr++; pc->pir++;
+    if (bounce == P->max_pb) { *pf = PF_MACRO_BOUNCE; *pfs = SUB_SPHEROID_BOUNCE_LIMIT; return 0; }
   }
   *pf = PF_OTHER; *pfs = SUB_NONE;
   return 0;
@@ -1706,7 +2212,12 @@ static cond_t conditional_event(const orc_model* m, rng_t* r, int pidx, float x,
       mie_scatter(r, m, pidx, &c.vx, &c.vy, &c.vz);
       c.status = 1;
     } else if (c.channel == 2) {
-      c.status = 0; c.failure = PF_SPHEROID; c.subcode = 0;   /* morphology is refused at prepare */
+      float fr = sqrtf(x * x + y * y), ffx = 1.0f, ffy = 0.0f;
+      if (fr > 1e-12f) { ffx = -y / fr; ffy = x / fr; }
+      int pf = 0, pfs = 0;
+      c.status = wave_particle_event(P, r, P->n_water, nr, ki, radius, ffx, ffy, 0.0f, &c.vx, &c.vy, &c.vz, &c.pc,
+                                     &pf, &pfs);
+      c.failure = pf; c.subcode = pfs;
     } else {
       float gx, gy, gz;
       sphere_sample(r, vx, vy, vz, &gx, &gy, &gz);
@@ -1970,7 +2481,17 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->wic++;
           int pidx = sel >= 0 ? sel : select_bin(&r, m, 1, p->x, p->y, p->z, LW);
           int coherent = (P->coherent >= 1.0f) || (U(&r) < P->coherent);
-          if (!coherent) { pf = PF_SPHEROID; pfs = 0; st = 4; break; }
+          if (!coherent) {
+            /* morphology-wave event about the local azimuthal flow direction */
+            float fr = sqrtf(p->x * p->x + p->y * p->y), ffx = 1.0f, ffy = 0.0f;
+            if (fr > 1e-12f) { ffx = -p->y / fr; ffy = p->x / fr; }
+            int ppf = 0, ppfs = 0;
+            int ps = wave_particle_event(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], m->r_f[pidx], ffx, ffy, 0.0f,
+                                         &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
+            if (ps == 2) { st = 5; break; }
+            if (ps != 1) { pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+            continue;
+          }
           o->wcoh++;
           mie_scatter(&r, m, pidx, &p->vx, &p->vy, &p->vz);
           continue;
@@ -2272,6 +2793,8 @@ static void fill_fparams(orc_model* m) {
   P->fs_model = c->free_surface_model; P->src_model = c->source_angular_model;
   P->event_model = c->event_model; P->sed_model = c->sediment_model;
   P->max_pb = c->max_particle_bounces; P->max_iter = c->max_iterations;
+  P->rough = (float)c->rough_alpha; P->aspect = (float)c->spheroid_aspect; P->kappa = (float)c->spheroid_kappa;
+  P->orientation = c->spheroid_orientation;
   P->cond_trials = c->conditional_max_trials; P->n_aux = c->n_aux;
   P->n_groups = (c->n_groups >= 1 && c->n_groups <= CLR_MAX_GROUPS) ? c->n_groups : CLR_MAX_GROUPS;
   P->fn0 = c->field_n0; P->fn1 = c->field_n1; P->fn2 = c->field_n2; P->fnbins = c->field_nbins;
@@ -2320,7 +2843,11 @@ void orc_free(orc_model* m) {
 int orc_prepare(const clr_config* cfg, const clr_particles* p, const clr_sediment* sed,
                 const float* field, size_t field_len, orc_model** out) {
   if (!cfg || !p || !out || p->n <= 0 || p->n > 512 || p->n_theta < 2) return CLR_E_ARG;
-  if (cfg->rough_alpha > 1e-7) return CLR_E_UNSUPPORTED;
+  /* CLARITAS.py:5797-5826 */
+  if (!(cfg->rough_alpha >= 0.0) || !isfinite(cfg->rough_alpha) || !(cfg->spheroid_aspect >= 0.2 && cfg->spheroid_aspect <= 5.0) ||
+      !(cfg->spheroid_kappa >= 0.0 && cfg->spheroid_kappa <= 100.0) ||
+      (cfg->spheroid_orientation != 0 && cfg->spheroid_orientation != 1))
+    return CLR_E_ARG;
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
   if (!m) return CLR_E_NOMEM;
   m->cfg = *cfg;
@@ -2350,7 +2877,6 @@ int orc_prepare(const clr_config* cfg, const clr_particles* p, const clr_sedimen
   clr_mie_out mo = {m->x, m->qext, m->qsca, m->qabs, m->qback, m->g, m->albedo, m->nstop, m->theta, m->pdf, m->cdf};
   err = orc_mie(p, &mo);
   if (!err) err = transport_precompute(m, p);
-  if (!err && m->mu_scalars[7] < 1.0 && (p->event_model == 3 || p->event_model == 4)) err = CLR_E_UNSUPPORTED;
   if (!err) {
     int tm = cfg->sediment_model;
     if ((tm == CLR_SED_GRID_PER_BIN && (!m->field || field_len < (size_t)n * cfg->field_n0 * cfg->field_n1)) ||
@@ -2457,6 +2983,33 @@ int orc_wall_probe(int n, const float* rays6, float R, float zlo, float zhi, dou
   for (int i = 0; i < n; ++i) {
     const float* q = rays6 + 6 * i;
     t[i] = bore_wall_hit_d(q[0], q[1], q[2], q[3], q[4], q[5], R, zlo, zhi);
+  }
+  return 0;
+}
+
+int orc_beckmann_B_probe(int n, const double* x, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = beck_tail_B(x[i]);
+  return 0;
+}
+int orc_beckmann_slopes_probe(int n, const double* si, const double* u1, const double* u2, double* sx, double* sy,
+                              int* ok) {
+  for (int i = 0; i < n; ++i) {
+    int fc = 0;
+    ok[i] = beck_sample_slopes(si[i], u1[i], u2[i], &sx[i], &sy[i], &fc) ? 1 : -fc;
+  }
+  return 0;
+}
+int orc_walk_probe(int n, uint64_t seed, float alpha, const float* b, float n_top, float k_top, float n_bot,
+                   float k_bot, float* d, int* status, int* top, int* fc) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    rng_t r;
+    rng_seed(&r, seed, (uint64_t)i);
+    float* v = d + 3 * i;
+    const float* bb = b + 3 * i;
+    top[i] = -1; fc[i] = 0;
+    status[i] = micro_interface(&r, alpha, bb[0], bb[1], bb[2], n_top, k_top, n_bot, k_bot, &v[0], &v[1], &v[2],
+                                &top[i], &fc[i]);
   }
   return 0;
 }

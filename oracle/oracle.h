@@ -56,6 +56,22 @@ int orc_complex_probe(int n, const float* ci, float n1r, float n1k, float n2r, f
 int orc_refract_probe(int n, const float* v, const float* nrm, float n1, float n2, float* out, int* ok);
 int orc_wall_probe(int n, const float* rays6, float R, float zlo, float zhi, double* t);
 
+/* rough-surface probes: inverse error function, Beckmann tail function
+ * B(x), signed Smith Lambda, one Smith height step, alpha = 1 visible
+ * slopes from two uniforms, and whole microsurface walks (history i of
+ * `seed`; d is the incident direction in, the outgoing one out; status
+ * 2 escape, 3 numerical failure, 4 watchdog; top = 1 when it left on the
+ * top side; fc the failure subcode) */
+double orc_erfinv(double y);
+int orc_beckmann_B_probe(int n, const double* x, double* out);
+double orc_smith_lambda(float wx, float wy, float wz, float bx, float by, float bz, float alpha);
+int orc_smith_height(double U, float wx, float wy, float wz, float bx, float by, float bz, float alpha, float h,
+                     float* out_h, int* fc);
+int orc_beckmann_slopes_probe(int n, const double* si, const double* u1, const double* u2, double* sx, double* sy,
+                              int* ok);
+int orc_walk_probe(int n, uint64_t seed, float alpha, const float* b, float n_top, float k_top, float n_bot,
+                   float k_bot, float* d, int* status, int* top, int* fc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,16 @@ def load():
     lib.orc_refract_probe.argtypes = [C.c_int, P(C.c_float), P(C.c_float), C.c_float, C.c_float, P(C.c_float),
                                       P(C.c_int)]
     lib.orc_wall_probe.argtypes = [C.c_int, P(C.c_float), C.c_float, C.c_float, C.c_float, P(C.c_double)]
+    lib.orc_erfinv.argtypes = [C.c_double]
+    lib.orc_erfinv.restype = C.c_double
+    lib.orc_beckmann_B_probe.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
+    lib.orc_smith_lambda.argtypes = [C.c_float] * 7
+    lib.orc_smith_lambda.restype = C.c_double
+    lib.orc_smith_height.argtypes = [C.c_double] + [C.c_float] * 8 + [P(C.c_float), P(C.c_int)]
+    lib.orc_smith_height.restype = C.c_int
+    lib.orc_beckmann_slopes_probe.argtypes = [C.c_int] + [P(C.c_double)] * 5 + [P(C.c_int)]
+    lib.orc_walk_probe.argtypes = [C.c_int, C.c_uint64, C.c_float, P(C.c_float), C.c_float, C.c_float, C.c_float,
+                                   C.c_float, P(C.c_float), P(C.c_int), P(C.c_int), P(C.c_int)]
     _lib = lib
     return lib
 
@@ -93,6 +103,58 @@ def wall_exit(rays6, R, zlo, zhi):
     t = np.zeros(len(rays6))
     load().orc_wall_probe(len(rays6), _pf(rays6), R, zlo, zhi, t.ctypes.data_as(C.POINTER(C.c_double)))
     return t
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def erfinv(y):
+    lib = load()
+    return np.array([lib.orc_erfinv(float(v)) for v in np.ravel(y)])
+
+
+def beckmann_B(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    load().orc_beckmann_B_probe(x.size, _pd(x), _pd(out))
+    return out
+
+
+def smith_lambda(w, b, alpha):
+    return load().orc_smith_lambda(*(float(v) for v in w), *(float(v) for v in b), float(alpha))
+
+
+def smith_height(U, w, b, alpha, h):
+    """(status, next height, failure code) of one Smith height step."""
+    nh = C.c_float(0.0)
+    fc = C.c_int(0)
+    st = load().orc_smith_height(float(U), *(float(v) for v in w), *(float(v) for v in b), float(alpha), float(h),
+                                 C.byref(nh), C.byref(fc))
+    return st, nh.value, fc.value
+
+
+def beckmann_slopes(si, u1, u2):
+    si, u1, u2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (si, u1, u2))
+    sx, sy = np.zeros_like(si), np.zeros_like(si)
+    ok = np.zeros(si.size, dtype=np.int32)
+    load().orc_beckmann_slopes_probe(si.size, _pd(si), _pd(u1), _pd(u2), _pd(sx), _pd(sy), _pi(ok))
+    return sx, sy, ok
+
+
+def micro_walk(seed, alpha, b, d, n_top, k_top, n_bot, k_bot):
+    """Microsurface walks of directions d[n][3] on interfaces with top
+    normals b[n][3]: (outgoing directions, status, top, failure code)."""
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    d = np.array(d, dtype=np.float32, copy=True, order='C')
+    n = len(d)
+    st, top, fc = (np.zeros(n, dtype=np.int32) for _ in range(3))
+    load().orc_walk_probe(n, seed, alpha, _pf(b), n_top, k_top, n_bot, k_bot, _pf(d), _pi(st), _pi(top), _pi(fc))
+    return d, st, top, fc
 
 
 def philox(ctr, key):

@@ -37,9 +37,9 @@ _SIGS = {
     'clr_mlp_param_count': (C.c_size_t, [C.c_void_p]),
     'clr_mlp_get_params': (C.c_int, [C.c_void_p, P(C.c_float), C.c_size_t]),
     'clr_mlp_set_params': (C.c_int, [C.c_void_p, P(C.c_float), C.c_size_t]),
-    'clr_mlp_train': (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, C.c_int32, C.c_int32,
-                                C.c_float, C.c_uint64, P(C.c_float)]),
-    'clr_mlp_grad': (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float),
+    'clr_mlp_train': (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64, C.c_int32,
+                                C.c_int32, C.c_float, C.c_uint64, P(C.c_float)]),
+    'clr_mlp_grad': (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float),
                                P(C.c_float)]),
     'clr_mlp_infer': (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
     'clr_mlp_time_infer': (C.c_int, [C.c_void_p, C.c_int64, C.c_int, P(C.c_double)]),

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -92,11 +92,14 @@ class Config(C.Structure):
         ('aux_angles_deg', D * MAX_AUX),
         ('field_zmax', D),
         ('field_xy_half', D),
+        ('spheroid_aspect', D),
+        ('spheroid_kappa', D),
         ('free_surface_model', I32), ('source_angular_model', I32), ('event_model', I32),
         ('sediment_model', I32), ('max_particle_bounces', I32), ('max_cell_bounces', I32),
         ('max_iterations', I32), ('conditional_max_trials', I32), ('n_aux', I32),
         ('n_groups', I32), ('field_n0', I32), ('field_n1', I32), ('field_n2', I32),
         ('field_nbins', I32),
+        ('spheroid_orientation', I32), ('pad_', I32),
     ]
 
 

@@ -825,16 +825,16 @@ int clr_mlp_set_params(clr_ctx* c, const float* in, size_t n) {
   HIPCHK(c, hipSetDevice(c->device));
   return mlp_set_params(c->mlp, in, n, c->stream, c->err);
 }
-int clr_mlp_train(clr_ctx* c, const float* x, const float* y, int64_t n, int32_t batch, int32_t epochs, float lr,
-                  uint64_t seed, float* loss) {
+int clr_mlp_train(clr_ctx* c, const float* x, const float* y, const float* w, int64_t n, int32_t batch,
+                  int32_t epochs, float lr, uint64_t seed, float* loss) {
   if (!c || !c->mlp || !x || !y) return fail(c, CLR_E_STATE, "no ensemble");
   HIPCHK(c, hipSetDevice(c->device));
-  return mlp_train(c->mlp, x, y, n, batch, epochs, lr, seed, loss, c->stream, c->err);
+  return mlp_train(c->mlp, x, y, w, n, batch, epochs, lr, seed, loss, c->stream, c->err);
 }
-int clr_mlp_grad(clr_ctx* c, const float* x, const float* y, int32_t n, float* loss, float* grads) {
+int clr_mlp_grad(clr_ctx* c, const float* x, const float* y, const float* w, int32_t n, float* loss, float* grads) {
   if (!c || !c->mlp || !x || !y) return fail(c, CLR_E_STATE, "no ensemble");
   HIPCHK(c, hipSetDevice(c->device));
-  return mlp_grad(c->mlp, x, y, n, loss, grads, c->stream, c->err);
+  return mlp_grad(c->mlp, x, y, w, n, loss, grads, c->stream, c->err);
 }
 int clr_mlp_infer(clr_ctx* c, const float* x, int64_t n, float* mean, float* stdev) {
   if (!c || !c->mlp || !x || !mean) return fail(c, CLR_E_STATE, "no ensemble");

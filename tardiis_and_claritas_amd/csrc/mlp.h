@@ -13,9 +13,9 @@ int mlp_init(MlpState** st, const clr_mlp_shape* shape, uint64_t seed, hipStream
 size_t mlp_param_count(const MlpState* st);
 int mlp_get_params(MlpState* st, float* out, size_t n, hipStream_t s, std::string& err);
 int mlp_set_params(MlpState* st, const float* in, size_t n, hipStream_t s, std::string& err);
-int mlp_train(MlpState* st, const float* x, const float* y, int64_t n, int32_t batch, int32_t epochs, float lr,
-              uint64_t seed, float* loss, hipStream_t s, std::string& err);
-int mlp_grad(MlpState* st, const float* x, const float* y, int32_t n, float* loss, float* grads, hipStream_t s,
-             std::string& err);
+int mlp_train(MlpState* st, const float* x, const float* y, const float* w, int64_t n, int32_t batch, int32_t epochs,
+              float lr, uint64_t seed, float* loss, hipStream_t s, std::string& err);
+int mlp_grad(MlpState* st, const float* x, const float* y, const float* w, int32_t n, float* loss, float* grads,
+             hipStream_t s, std::string& err);
 int mlp_infer(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s, std::string& err);
 int mlp_time_infer(MlpState* st, int64_t n, int repeats, double* ms, hipStream_t s, std::string& err);

@@ -197,10 +197,12 @@ __global__ __launch_bounds__(256) void last_layer_kernel(LastArgs a) {
   }
 }
 
-// MSE: dout = 2 (out - target) / (n_valid dout); loss[member] += sum d^2 / (n_valid dout)
+// weighted MSE: dout = 2 w (out - target) / (n_valid dout);
+// loss[member] += sum w d^2 / (n_valid dout); w = 1 without a weight array
 struct LossArgs {
   const float* out;   // [members][B][dout]
   const float* y;     // [n_rows][dout]
+  const float* w;     // [n_rows][dout] or null
   const int* idx;
   long long n_idx, base;
   int n_valid, B, dout;
@@ -212,14 +214,15 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.B * a.dout) return;
   const int b = t / a.dout, j = t % a.dout;
-  float d = 0.0f;
+  float d = 0.0f, wt = 0.0f;
   if (b < a.n_valid) {
     const long long row = a.idx ? a.idx[member * a.n_idx + a.base + b] : a.base + b;
     d = a.out[((size_t)member * a.B + b) * a.dout + j] - a.y[row * a.dout + j];
+    wt = a.w ? a.w[row * a.dout + j] : 1.0f;
   }
   const float inv = 1.0f / (float)(a.n_valid * a.dout);
-  a.dout_buf[((size_t)member * a.B + b) * a.dout + j] = 2.0f * d * inv;
-  if (d != 0.0f) atomicAdd(a.loss + member, d * d * inv);
+  a.dout_buf[((size_t)member * a.B + b) * a.dout + j] = 2.0f * wt * d * inv;
+  if (d != 0.0f && wt != 0.0f) atomicAdd(a.loss + member, wt * d * d * inv);
 }
 
 // last-layer gradients: dW[j][o] = sum_b dout[b][j] y[b][o] (from Yt), db[j];
@@ -390,12 +393,12 @@ struct MlpState {
   DBuf<float> xg, out, dout, loss;
   std::vector<DBuf<float>> Z;       // hidden layers: [M][B][H]
   std::vector<DBuf<bf16>> Y, Yt, DZ, DZt;
-  DBuf<float> x_d, y_d;             // uploaded data
+  DBuf<float> x_d, y_d, w_d;        // uploaded data
   DBuf<int> idx_d;
   DBuf<float> mean_d, std_d;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ~MlpState() {
-    for (auto* d : {&params, &grads, &am, &av, &xg, &out, &dout, &loss, &x_d, &y_d, &mean_d, &std_d}) d->release();
+    for (auto* d : {&params, &grads, &am, &av, &xg, &out, &dout, &loss, &x_d, &y_d, &w_d, &mean_d, &std_d}) d->release();
     wbf.release();
     wtbf.release();
     idx_d.release();
@@ -481,11 +484,11 @@ void forward(MlpState* st, const float* x, const int* idx, long long n_idx, long
 }
 
 // loss and backward pass after forward(train=true); gradients into st->grads
-void backward(MlpState* st, const float* y, const int* idx, long long n_idx, long long base, int n_valid, int B,
-              hipStream_t s) {
+void backward(MlpState* st, const float* y, const float* w, const int* idx, long long n_idx, long long base,
+              int n_valid, int B, hipStream_t s) {
   const int M = st->sh.n_members, H = st->sh.hidden, L = st->n_layers, dout = st->sh.out_dim;
   LossArgs la{};
-  la.out = st->out.p; la.y = y; la.idx = idx; la.n_idx = n_idx; la.base = base; la.n_valid = n_valid; la.B = B;
+  la.out = st->out.p; la.y = y; la.w = w; la.idx = idx; la.n_idx = n_idx; la.base = base; la.n_valid = n_valid; la.B = B;
   la.dout = dout; la.dout_buf = st->dout.p; la.loss = st->loss.p;
   hipLaunchKernelGGL(loss_kernel, dim3(blocks((long long)B * dout), M), dim3(256), 0, s, la);
   LastGradArgs lg{};
@@ -629,8 +632,8 @@ int mlp_set_params(MlpState* st, const float* in, size_t n, hipStream_t s, std::
   return 0;
 }
 
-int mlp_grad(MlpState* st, const float* x, const float* y, int32_t n, float* loss, float* grads, hipStream_t s,
-             std::string& err) {
+int mlp_grad(MlpState* st, const float* x, const float* y, const float* w, int32_t n, float* loss, float* grads,
+             hipStream_t s, std::string& err) {
   if (n < 1 || n > (1 << 20)) { err = "gradient batch must hold 1..2^20 rows"; return CLR_E_ARG; }
   const int B = (int)round_up(n, kTile);
   if (int rc = ensure_batch(st, B, err)) return rc;
@@ -639,11 +642,15 @@ int mlp_grad(MlpState* st, const float* x, const float* y, int32_t n, float* los
   MLPCHK(st->y_d.ensure((size_t)n * st->sh.out_dim));
   MLPCHK(hipMemcpyAsync(st->x_d.p, x, sizeof(float) * n * st->sh.in_dim, hipMemcpyHostToDevice, s));
   MLPCHK(hipMemcpyAsync(st->y_d.p, y, sizeof(float) * n * st->sh.out_dim, hipMemcpyHostToDevice, s));
+  if (w) {
+    MLPCHK(st->w_d.ensure((size_t)n * st->sh.out_dim));
+    MLPCHK(hipMemcpyAsync(st->w_d.p, w, sizeof(float) * n * st->sh.out_dim, hipMemcpyHostToDevice, s));
+  }
   const size_t total = (size_t)M * st->sP;
   hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks((long long)total)), dim3(256), 0, s, st->grads.p, (long long)total, 0.0f);
   hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, st->loss.p, (long long)M, 0.0f);
   forward(st, st->x_d.p, nullptr, 0, 0, n, B, true, s);
-  backward(st, st->y_d.p, nullptr, 0, 0, n, B, s);
+  backward(st, st->y_d.p, w ? st->w_d.p : nullptr, nullptr, 0, 0, n, B, s);
   MLPCHK(hipGetLastError());
   if (loss) MLPCHK(hipMemcpyAsync(loss, st->loss.p, sizeof(float) * M, hipMemcpyDeviceToHost, s));
   if (grads) MLPCHK(copy_params(st, grads, st->grads.p, true, s));
@@ -651,8 +658,8 @@ int mlp_grad(MlpState* st, const float* x, const float* y, int32_t n, float* los
   return 0;
 }
 
-int mlp_train(MlpState* st, const float* x, const float* y, int64_t n, int32_t batch, int32_t epochs, float lr,
-              uint64_t seed, float* loss, hipStream_t s, std::string& err) {
+int mlp_train(MlpState* st, const float* x, const float* y, const float* w, int64_t n, int32_t batch, int32_t epochs,
+              float lr, uint64_t seed, float* loss, hipStream_t s, std::string& err) {
   if (n < 1 || n > ((int64_t)1 << 31) - 1 || batch < 1 || batch > (1 << 16) || epochs < 1 || !(lr > 0.0f)) {
     err = "train: need 1 <= n < 2^31, 1 <= batch <= 65536, epochs >= 1, lr > 0";
     return CLR_E_ARG;
@@ -666,6 +673,10 @@ int mlp_train(MlpState* st, const float* x, const float* y, int64_t n, int32_t b
   MLPCHK(st->idx_d.ensure((size_t)M * n));
   MLPCHK(hipMemcpyAsync(st->x_d.p, x, sizeof(float) * n * st->sh.in_dim, hipMemcpyHostToDevice, s));
   MLPCHK(hipMemcpyAsync(st->y_d.p, y, sizeof(float) * n * st->sh.out_dim, hipMemcpyHostToDevice, s));
+  if (w) {
+    MLPCHK(st->w_d.ensure((size_t)n * st->sh.out_dim));
+    MLPCHK(hipMemcpyAsync(st->w_d.p, w, sizeof(float) * n * st->sh.out_dim, hipMemcpyHostToDevice, s));
+  }
   const long long steps = n / batch;
   std::vector<int> perm((size_t)M * n);
   std::vector<float> epoch_loss(M, 0.0f);
@@ -682,7 +693,7 @@ int mlp_train(MlpState* st, const float* x, const float* y, int64_t n, int32_t b
     hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, st->loss.p, (long long)M, 0.0f);
     for (long long k = 0; k < steps; ++k) {
       forward(st, st->x_d.p, st->idx_d.p, n, k * batch, batch, B, true, s);
-      backward(st, st->y_d.p, st->idx_d.p, n, k * batch, batch, B, s);
+      backward(st, st->y_d.p, w ? st->w_d.p : nullptr, st->idx_d.p, n, k * batch, batch, B, s);
       st->step++;
       const float c1 = 1.0f - std::pow(b1, (float)st->step), c2 = 1.0f - std::pow(b2, (float)st->step);
       const long long total = (long long)M * st->sP;

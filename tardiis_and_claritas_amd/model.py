@@ -107,7 +107,11 @@ class ForwardResult:
     stop_reason: str
     particle_event_model: str
     particle_surface_rms_slope_deg: float
+    particle_surface_beckmann_alpha: float
     wave_surface_rms_height_nm: float
+    wave_spheroid_aspect_ratio: float
+    wave_spheroid_orientation_model: str
+    wave_spheroid_orientation_kappa: float
     wave_coherent_fraction_nominal: float
     mu_geom_per_m: float
     mu_wave_per_m: float
@@ -196,10 +200,11 @@ class ClaritasForwardModel:
     """GPU light-transport model of the CLARITAS sample cell.
 
     Keyword arguments and defaults follow CLARITAS.py:5577-5652.  Supported
-    physics: every event law except the moment-constrained one, smooth
-    particle surfaces (particle_surface_rms_slope_deg must be 0), wave
-    coherent fraction 1 for the hybrid laws (wave_surface_rms_height_m = 0),
-    absorbing bores, ideal photodiodes.  Extensions: event law 'plain_mie',
+    physics: every event law except the moment-constrained one, smooth and
+    rough (Smith microsurface walk, CLARITAS.py:2314-2430) particle surfaces,
+    the coherent Mie and the incoherent spheroid-morphology wave channels
+    (CLARITAS.py:2655-2866), absorbing bores, ideal photodiodes.
+    Extensions: event law 'plain_mie',
     source 'collimated_disk', surface 'rankine_vortex', sediment models
     'rz_field' / 'xyz_field' / 'grid_per_bin' with a caller-supplied FP32
     field, auxiliary bores at arbitrary angles, 64-bit history ids.
@@ -212,6 +217,7 @@ class ClaritasForwardModel:
     SEDIMENT_TRANSPORT_MODEL_CODES = abi.SEDIMENT_TRANSPORT_MODEL_CODES
     BORE_SURFACE_MODEL_CODES = {'absorbing': 0}
     PHOTODIODE_RESPONSE_MODEL_CODES = {'ideal_hard_aperture': 0}
+    SPHEROID_ORIENTATION_MODEL_CODES = {'isotropic': 0, 'local_tangential_axial_vmf': 1}   # CLARITAS.py:5564
 
     def __init__(
         self,
@@ -242,6 +248,9 @@ class ClaritasForwardModel:
         mie_phase_grid_size=2049,
         particle_surface_rms_slope_deg=25.0,
         wave_surface_rms_height_m=100e-9,
+        wave_spheroid_aspect_ratio=1.5,
+        wave_spheroid_orientation_model='isotropic',
+        wave_spheroid_orientation_kappa=0.0,
         free_surface_model='localized_scully_vortex',
         vortex_wall_center_height_difference_m=0.050,
         vortex_core_radius_m=0.020,
@@ -305,6 +314,9 @@ class ClaritasForwardModel:
         self.particle_surface_rms_slope_deg = float(particle_surface_rms_slope_deg)
         self.particle_surface_beckmann_alpha = math.tan(math.radians(self.particle_surface_rms_slope_deg))
         self.wave_surface_rms_height_m = float(wave_surface_rms_height_m)
+        self.wave_spheroid_aspect_ratio = float(wave_spheroid_aspect_ratio)
+        self.wave_spheroid_orientation_model = str(wave_spheroid_orientation_model).strip().lower()
+        self.wave_spheroid_orientation_kappa = float(wave_spheroid_orientation_kappa)
         self.free_surface_model = str(free_surface_model).strip().lower()
         self.vortex_delta_h = float(vortex_wall_center_height_difference_m)
         self.vortex_core_radius = float(vortex_core_radius_m)
@@ -360,11 +372,17 @@ class ClaritasForwardModel:
             raise ValueError('only absorbing bores are modelled')
         if self.photodiode_response_model not in self.PHOTODIODE_RESPONSE_MODEL_CODES:
             raise ValueError('only the ideal hard-aperture photodiode is modelled')
-        if self.particle_surface_beckmann_alpha > 1e-7:
-            raise ValueError('rough particle surfaces are not modelled: set particle_surface_rms_slope_deg=0')
-        if self.particle_event_model in ('hybrid_mie_excess', 'hybrid_mie_exact_rate') and \
-                self.wave_surface_rms_height_m > 0.0:
-            raise ValueError('the spheroid morphology channel is not modelled: set wave_surface_rms_height_m=0')
+        # CLARITAS.py:5797-5826
+        if not (math.isfinite(self.particle_surface_rms_slope_deg) and 0.0 <= self.particle_surface_rms_slope_deg < 80.0):
+            raise ValueError('particle_surface_rms_slope_deg must be finite and in [0,80) degrees')
+        if not (math.isfinite(self.wave_surface_rms_height_m) and self.wave_surface_rms_height_m >= 0.0):
+            raise ValueError('wave_surface_rms_height_m must be finite and >=0')
+        if not (math.isfinite(self.wave_spheroid_aspect_ratio) and 0.2 <= self.wave_spheroid_aspect_ratio <= 5.0):
+            raise ValueError('wave_spheroid_aspect_ratio must be finite and in [0.2,5.0]')
+        if self.wave_spheroid_orientation_model not in self.SPHEROID_ORIENTATION_MODEL_CODES:
+            raise ValueError("wave_spheroid_orientation_model must be 'isotropic' or 'local_tangential_axial_vmf'")
+        if not (math.isfinite(self.wave_spheroid_orientation_kappa) and 0.0 <= self.wave_spheroid_orientation_kappa <= 100.0):
+            raise ValueError('wave_spheroid_orientation_kappa must be finite and in [0,100]')
         if not (0 < self.rin < self.rout < self.ring_in < self.throat_out <= self.source_launch_radius <= self.ring_out):
             raise ValueError('geometry must satisfy 0<Rin<Rout<ring_in<throat_out<=source<=ring_out')
         if not (self.zwall_top > self.zmean > self.zmin):
@@ -402,6 +420,9 @@ class ClaritasForwardModel:
         c.legacy_accept_deg = self.legacy_acceptance_deg
         c.wavelength = self.particle_wavelength
         c.rough_alpha = self.particle_surface_beckmann_alpha
+        c.spheroid_aspect = self.wave_spheroid_aspect_ratio
+        c.spheroid_kappa = self.wave_spheroid_orientation_kappa
+        c.spheroid_orientation = self.SPHEROID_ORIENTATION_MODEL_CODES[self.wave_spheroid_orientation_model]
         for i, a in enumerate(self.aux_detector_angles_deg):
             c.aux_angles_deg[i] = float(a)
         c.n_aux = int(self.aux_detector_angles_deg.size)
@@ -692,7 +713,11 @@ class ClaritasForwardModel:
             stop_reason=stop_reason,
             particle_event_model=self.particle_event_model,
             particle_surface_rms_slope_deg=self.particle_surface_rms_slope_deg,
+            particle_surface_beckmann_alpha=self.particle_surface_beckmann_alpha,
             wave_surface_rms_height_nm=self.wave_surface_rms_height_m * 1e9,
+            wave_spheroid_aspect_ratio=self.wave_spheroid_aspect_ratio,
+            wave_spheroid_orientation_model=self.wave_spheroid_orientation_model,
+            wave_spheroid_orientation_kappa=self.wave_spheroid_orientation_kappa,
             wave_coherent_fraction_nominal=float(mus[7]),
             mu_geom_per_m=float(mus[2]),
             mu_wave_per_m=float(mus[5]),

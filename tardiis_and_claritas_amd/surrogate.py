@@ -64,22 +64,36 @@ class MlpEnsemble:
         self.ctx.check(self.ctx.lib.clr_mlp_set_params(self.ctx.h, flat.ctypes.data_as(P(C.c_float)), flat.size),
                        'clr_mlp_set_params')
 
-    def grad(self, x, y):
+    @staticmethod
+    def _weights(w, y):
+        if w is None:
+            return None, None
+        w = _fp(w)
+        if w.shape != y.shape:
+            raise ValueError(f'weights must have the shape of the targets {y.shape}, got {w.shape}')
+        return w, w.ctypes.data_as(P(C.c_float))
+
+    def grad(self, x, y, w=None):
+        """Per-member losses and the flat gradient of the (weighted) MSE on one batch."""
         x, y = _fp(x), _fp(y)
+        w, wp = self._weights(w, y)
         loss = np.empty(self.n_members, dtype=np.float32)
         g = np.empty(self.n_params, dtype=np.float32)
         self.ctx.check(self.ctx.lib.clr_mlp_grad(self.ctx.h, x.ctypes.data_as(P(C.c_float)),
-                                                 y.ctypes.data_as(P(C.c_float)), len(x),
+                                                 y.ctypes.data_as(P(C.c_float)), wp, len(x),
                                                  loss.ctypes.data_as(P(C.c_float)), g.ctypes.data_as(P(C.c_float))),
                        'clr_mlp_grad')
         return loss, g
 
-    def train(self, x, y, batch=512, epochs=10, lr=1e-3, seed=0):
+    def train(self, x, y, batch=512, epochs=10, lr=1e-3, seed=0, w=None):
+        """Adam epochs over (x, y); `w` holds per-element loss weights."""
         x, y = _fp(x), _fp(y)
+        w, wp = self._weights(w, y)
         loss = np.empty(self.n_members, dtype=np.float32)
         self.ctx.check(self.ctx.lib.clr_mlp_train(self.ctx.h, x.ctypes.data_as(P(C.c_float)),
-                                                  y.ctypes.data_as(P(C.c_float)), len(x), int(batch), int(epochs),
-                                                  C.c_float(lr), C.c_uint64(seed), loss.ctypes.data_as(P(C.c_float))),
+                                                  y.ctypes.data_as(P(C.c_float)), wp, len(x), int(batch),
+                                                  int(epochs), C.c_float(lr), C.c_uint64(seed),
+                                                  loss.ctypes.data_as(P(C.c_float))),
                        'clr_mlp_train')
         return loss
 
@@ -198,6 +212,23 @@ class Normaliser:
 
 
 FLOOR = 1e-9   # signal floor before the log transform
+REL_FLOOR = 0.003   # relative error floor added to every label's Monte Carlo error in the weights
+
+
+def label_variance(S, n_hist):
+    """Monte Carlo variance of the three log targets of Surrogate.targets from
+    Poisson window counts c = S * n_hist: log(S90/S0) -> 1/c90 + 1/c0,
+    log S0 -> 1/c0, log(S135/S0) -> 1/c135 + 1/c0."""
+    c = np.maximum(np.asarray(S, dtype=np.float64) * np.asarray(n_hist, dtype=np.float64)[:, None], 1.0)
+    return np.stack([1.0 / c[:, 0] + 1.0 / c[:, 1], 1.0 / c[:, 1], 1.0 / c[:, 2] + 1.0 / c[:, 1]], axis=1)
+
+
+def label_weights(S, n_hist, rel_floor=REL_FLOOR):
+    """Inverse-variance loss weights [n][3], each column scaled to mean 1.  The
+    floor keeps labels with very many counts from dominating once the
+    network's own error exceeds their statistical error."""
+    w = 1.0 / (label_variance(S, n_hist) + rel_floor ** 2)
+    return (w / w.mean(0)).astype(np.float32)
 
 
 class Surrogate:
@@ -221,17 +252,21 @@ class Surrogate:
         s0 = np.exp(Y[:, 1])
         return np.stack([np.exp(Y[:, 0]) * s0, s0, np.exp(Y[:, 2]) * s0], axis=1)
 
-    def fit(self, F, S, epochs=40, batch=512, lr=1e-3, seed=0, schedule=(1.0, 0.3, 0.1)):
+    def fit(self, F, S, epochs=40, batch=512, lr=1e-3, seed=0, schedule=(1.0, 0.3, 0.1), n_hist=None,
+            rel_floor=REL_FLOOR):
         """Train on raw features F [n][8] and signals S [n][3] (log-ratio
-        targets).  The learning rate steps down over `schedule` (equal shares
-        of the epochs)."""
+        targets).  With the histories behind each label (`n_hist`), every
+        target is weighted by its inverse Monte Carlo variance (see
+        label_weights).  The learning rate steps down over `schedule` (equal
+        shares of the epochs)."""
         Y = self.targets(S)
         self.norm = Normaliser(F.mean(0), F.std(0) + 1e-12, Y.mean(0), Y.std(0) + 1e-12)
         x, y = self._x(F), (Y - self.norm.y_mean) / self.norm.y_std
+        w = None if n_hist is None else label_weights(S, n_hist, rel_floor)
         losses = []
         per = max(1, epochs // len(schedule))
         for k, f in enumerate(schedule):
-            losses.append(self.ens.train(x, y, batch, per, lr * f, seed + k).tolist())
+            losses.append(self.ens.train(x, y, batch, per, lr * f, seed + k, w=w).tolist())
         return losses
 
     def predict(self, F):
@@ -274,9 +309,35 @@ def mc_matched_photons(S, rel=0.01):
     return (1.0 / s90 + 1.0 / s0) / rel ** 2
 
 
+def load_dataset(path, counts=None, test_counts=None):
+    """(train params, train signals, train histories, test params, test
+    signals, test histories) from a dataset.npz written by run_pipeline.
+    Files without per-sample histories recompute them from the counts they
+    were generated with."""
+    d = np.load(path)
+    out = []
+    for part, cnt in (('train', counts), ('test', test_counts)):
+        prm = {k: d[f'{part}_{k}'] for k in OPTICAL + TRANSPORT}
+        if f'{part}_hist' in d.files:
+            hist = d[f'{part}_hist']
+        elif cnt is not None:
+            hist = photon_budget(prm['conc_mg_l'], cnt)
+        else:
+            raise ValueError(f'{path}: no {part}_hist; give the counts it was generated with')
+        out += [prm, d[f'{part}_S'], hist]
+    return tuple(out)
+
+
+def _cat(parts):
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+
 def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
-                 test_counts=2000.0, epochs=60, seed=20261204, log=print):
-    """Generate, train, evaluate; writes summary.json and weights.npz."""
+                 test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True):
+    """Generate (n_optical x n_variants training and n_test_optical x
+    n_test_variants held-out configurations; 0 skips), merge earlier
+    `datasets` (paths of dataset.npz), train, evaluate; writes summary.json,
+    weights.npz and dataset.npz (this run's own samples)."""
     from .batched import BatchedRunner
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
@@ -287,7 +348,7 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         p_opt = workloads.config3_params(n_opt, sd)
         g = np.random.Generator(np.random.PCG64(sd + 7919))
         cols = {k: [] for k in OPTICAL + TRANSPORT}
-        sig, hist, mc_s = [], 0, 0.0
+        sig, hists, hist, mc_s = [], [], 0, 0.0
         for a0 in range(0, n_opt, 256):
             sel = np.arange(a0, min(n_opt, a0 + 256))
             for v in range(n_var):
@@ -300,37 +361,55 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
                 hist += int(budget.sum())
                 mc_s += runner.last_ms * 1e-3
                 sig.append(np.stack([signals_from_tallies(T[k], int(budget[k])) for k in range(len(sel))]))
+                hists.append(budget)
                 for k in cols:
                     cols[k].append(params[k])
                 log(f'generate: {sum(len(c) for c in cols["rouse"])}/{n_opt * n_var} configs, '
                     f'{hist:.3g} histories, {time.time() - t0:.0f} s')
-        return {k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig), hist, mc_s
+        if not sig:
+            return {k: np.zeros(0) for k in cols}, np.zeros((0, 3)), np.zeros(0, dtype=np.int64), 0, 0.0
+        return ({k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig), np.concatenate(hists), hist,
+                mc_s)
 
-    P_tr, S_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed)
-    P_te, S_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1)
-    mc_rate = (h_tr + h_te) / (s_tr + s_te)
-    np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr,
-             **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te)
+    P_tr, S_tr, n_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed)
+    P_te, S_te, n_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1)
+    np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr, train_hist=n_tr,
+             **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te, test_hist=n_te)
+    mc_rate = (h_tr + h_te) / (s_tr + s_te) if (s_tr + s_te) > 0 else None
+    for path in datasets:
+        a_P, a_S, a_n, b_P, b_S, b_n = load_dataset(path, counts, test_counts)
+        P_tr, S_tr, n_tr = _cat([P_tr, a_P]) if len(S_tr) else a_P, np.concatenate([S_tr, a_S]), \
+            np.concatenate([n_tr, a_n])
+        P_te, S_te, n_te = _cat([P_te, b_P]) if len(S_te) else b_P, np.concatenate([S_te, b_S]), \
+            np.concatenate([n_te, b_n])
+    if mc_rate is None:      # no generation in this run: time a held-out batch for the MC cost
+        sel = {k: v[:256] for k, v in P_te.items()}
+        budget = np.asarray(n_te[:256], dtype=np.int64)
+        runner.prepare([workloads.config3_item(sel, i) for i in range(len(budget))])
+        runner.run_prepared(budget.tolist(), seed + 99)
+        runner.run_prepared(budget.tolist(), seed + 98)
+        mc_rate = float(budget.sum()) / (runner.last_ms * 1e-3)
     sur = Surrogate(runner.ctx, seed=seed)
     F_tr = features(P_tr)
     keep = np.all(S_tr > 0, axis=1)           # log targets need a count in every window
     t1 = time.time()
-    losses = sur.fit(F_tr[keep], S_tr[keep], epochs=epochs, batch=256, lr=1e-3, seed=seed)
+    losses = sur.fit(F_tr[keep], S_tr[keep], epochs=epochs, batch=256, lr=1e-3, seed=seed,
+                     n_hist=n_tr[keep] if weighted else None)
     train_s = time.time() - t1
     F_te = features(P_te)
     S_pred, spread = sur.predict(F_te)
     valid = np.all(S_te > 0, axis=1)
     err = ntu_relative_error(S_pred[valid], S_te[valid])
     # Monte Carlo statistical error of the held-out labels themselves
-    label_se = np.sqrt(1.0 / np.maximum(S_te[valid, 0] * photon_budget(P_te['conc_mg_l'][valid], test_counts), 1)
-                       + 1.0 / np.maximum(S_te[valid, 1] * photon_budget(P_te['conc_mg_l'][valid], test_counts), 1))
+    label_se = np.sqrt(label_variance(S_te[valid], n_te[valid])[:, 0])
     ratio_spread = spread[valid, 0]
     ms = sur.ens.time_infer(65536, 20)
     q_s = ms * 1e-3 / 65536
     n_match = mc_matched_photons(S_te[valid])
     t_mc = n_match / mc_rate
     summary = dict(
-        n_train=int(keep.sum()), n_test=int(valid.sum()), train_histories=h_tr, test_histories=h_te,
+        n_train=int(keep.sum()), n_test=int(valid.sum()), train_histories=int(n_tr.sum()),
+        test_histories=int(n_te.sum()), datasets=[str(p) for p in datasets], weighted=bool(weighted),
         mc_histories_per_s=mc_rate, train_seconds=train_s, epochs=epochs, final_member_losses=losses[-1],
         ntu_error_median=float(np.median(err)), ntu_error_p90=float(np.percentile(err, 90)),
         ntu_error_mean=float(np.mean(err)), frac_within_1pct=float(np.mean(err <= 0.01)),
@@ -366,6 +445,8 @@ if __name__ == '__main__':
     ap.add_argument('--test-counts', type=float, default=2000.0)
     ap.add_argument('--epochs', type=int, default=60)
     ap.add_argument('--seed', type=int, default=20261204)
+    ap.add_argument('--dataset', action='append', default=[], help='merge an earlier dataset.npz (repeatable)')
+    ap.add_argument('--unweighted', action='store_true', help='plain MSE instead of inverse-variance weights')
     a = ap.parse_args()
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
-                 a.epochs, a.seed, log=lambda s: print(s, flush=True))
+                 a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted)

@@ -40,10 +40,13 @@ def test_no_device_fails_loudly():
 def test_model_validation_is_fail_closed():
     from tardiis_and_claritas_amd import ClaritasForwardModel, workloads
     kw = workloads.base_kwargs()
-    with pytest.raises(ValueError):
-        ClaritasForwardModel(**dict(kw, particle_surface_rms_slope_deg=25.0))
-    with pytest.raises(ValueError):
-        ClaritasForwardModel(**dict(kw, wave_surface_rms_height_m=100e-9))
+    # rough surfaces and the morphology channel are modelled; out-of-range values are not
+    ClaritasForwardModel(**dict(kw, particle_surface_rms_slope_deg=25.0, wave_surface_rms_height_m=100e-9))
+    for bad in (dict(particle_surface_rms_slope_deg=80.0), dict(wave_surface_rms_height_m=-1e-9),
+                dict(wave_spheroid_aspect_ratio=5.5), dict(wave_spheroid_orientation_model='random'),
+                dict(wave_spheroid_orientation_kappa=101.0)):
+        with pytest.raises(ValueError):
+            ClaritasForwardModel(**dict(kw, **bad))
     with pytest.raises(ValueError):
         ClaritasForwardModel(**dict(kw, particle_event_model='hybrid_mie_exact_rate_moment_constrained'))
     with pytest.raises(ValueError):

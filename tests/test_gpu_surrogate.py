@@ -60,6 +60,29 @@ def test_gradients_match_oracle(gpu_available, n):
         assert err < 5e-3, f'{name}: relative gradient error {err:.2e}'
 
 
+def test_weighted_gradients_match_oracle(gpu_available):
+    _flush('surrogate weighted gradient check n=192')
+    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
+    ens = MlpEnsemble(_ctx(), n_members=2, hidden=256, n_hidden_layers=3, seed=13)
+    shape = mo.Shape(2, 8, 256, 3, 3)
+    g = np.random.default_rng(17)
+    x = g.normal(size=(192, 8)).astype(np.float32)
+    y = g.normal(size=(192, 3)).astype(np.float32)
+    w = g.uniform(0.0, 4.0, size=(192, 3)).astype(np.float32)
+    w[::7] = 0.0                                    # rows that carry no weight at all
+    loss, grad = ens.grad(x, y, w)
+    loss_o, grad_o = mo.ensemble_loss_grad(shape, ens.get_params(), x.astype(np.float64), y.astype(np.float64),
+                                           bf16=True, w=w.astype(np.float64))
+    np.testing.assert_allclose(loss, loss_o, rtol=1e-3)
+    for (name, a), (_, b) in zip(_tensors(shape, grad), _tensors(shape, grad_o)):
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert err < 5e-3, f'{name}: relative gradient error {err:.2e}'
+    # unit weights give the unweighted gradient exactly (same kernels, same order)
+    l1, g1 = ens.grad(x, y, np.ones_like(w))
+    l0, g0 = ens.grad(x, y)
+    np.testing.assert_array_equal(g1, g0)
+
+
 def test_forward_matches_oracle(gpu_available):
     _flush('surrogate forward check')
     from tardiis_and_claritas_amd.surrogate import MlpEnsemble

@@ -87,3 +87,44 @@ def test_ensemble_statistics():
     outs = np.stack([mo.member_forward(l, x)[0] for l in shape.unpack(flat)])
     np.testing.assert_allclose(mean, outs.mean(0), rtol=1e-14)
     np.testing.assert_allclose(std, outs.std(0, ddof=1), rtol=1e-12)
+
+
+def test_weighted_oracle_matches_torch_and_finite_differences():
+    shape = mo.Shape(2, 4, 16, 3, 2)
+    flat = _random_params(shape, 8)
+    g = np.random.default_rng(9)
+    x = g.normal(size=(11, 4))
+    t = g.normal(size=(11, 3))
+    w = g.uniform(0.0, 3.0, size=(11, 3))
+    losses, grad = mo.ensemble_loss_grad(shape, flat, x, t, w=w)
+    glist = shape.unpack(grad)
+    for m, layers in enumerate(shape.unpack(flat)):
+        ps, h = [], torch.tensor(x, dtype=torch.float64)
+        for k, (W, b) in enumerate(layers):
+            Wt = torch.tensor(W, dtype=torch.float64, requires_grad=True)
+            bt = torch.tensor(b, dtype=torch.float64, requires_grad=True)
+            ps += [Wt, bt]
+            h = h @ Wt.T + bt
+            if k < len(layers) - 1:
+                h = torch.nn.functional.gelu(h, approximate='tanh')
+        loss = torch.mean(torch.tensor(w) * (h - torch.tensor(t)) ** 2)
+        loss.backward()
+        assert abs(losses[m] - float(loss)) <= 1e-12 * max(1.0, float(loss))
+        mine = [a for W, b in glist[m] for a in (W, b)]
+        for a, b in zip(mine, [p.grad.numpy() for p in ps]):
+            np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12)
+    # unit weights reduce to the plain MSE
+    l1, g1 = mo.ensemble_loss_grad(shape, flat, x, t, w=np.ones_like(w))
+    l0, g0 = mo.ensemble_loss_grad(shape, flat, x, t)
+    np.testing.assert_array_equal(l1, l0)
+    np.testing.assert_array_equal(g1, g0)
+    # finite differences on one member
+    one = mo.Shape(1, 4, 16, 3, 2)
+    f1 = flat[:one.per_member]
+    _, gr = mo.ensemble_loss_grad(one, f1, x, t, w=w)
+    h = 1e-6
+    for i in np.random.default_rng(10).choice(f1.size, 40, replace=False):
+        e = np.zeros_like(f1)
+        e[i] = h
+        num = (mo.ensemble_loss_grad(one, f1 + e, x, t, w=w)[0][0] - mo.ensemble_loss_grad(one, f1 - e, x, t, w=w)[0][0]) / (2 * h)
+        assert abs(gr[i] - num) <= 1e-6 * max(1.0, abs(num)), (i, gr[i], num)
