@@ -76,7 +76,7 @@ struct Slot {
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  int n_items = -1, grid = 0, block = 0, plain = -1, n_slots = 0;
+  int n_items = -1, grid = 0, block = 0, kind = -1, n_slots = 0;
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -217,8 +217,15 @@ float det_u(double deg, bool cosine) {
 
 int validate_config(clr_ctx* c, const clr_config* g, const clr_particles* p) {
   if (!g) return fail(c, CLR_E_ARG, "config is NULL");
-  if (g->rough_alpha > 1e-7)
-    return fail(c, CLR_E_UNSUPPORTED, "rough particle surfaces (Smith microsurface walk) are not supported; set particle_surface_rms_slope_deg=0");
+  // CLARITAS.py:5797-5826
+  if (!(g->rough_alpha >= 0.0) || !std::isfinite(g->rough_alpha))
+    return fail(c, CLR_E_ARG, "rough_alpha must be finite and >= 0");
+  if (!(g->spheroid_aspect >= 0.2 && g->spheroid_aspect <= 5.0))
+    return fail(c, CLR_E_ARG, "spheroid_aspect must be in [0.2, 5]");
+  if (!(g->spheroid_kappa >= 0.0 && g->spheroid_kappa <= 100.0))
+    return fail(c, CLR_E_ARG, "spheroid_kappa must be in [0, 100]");
+  if (g->spheroid_orientation != 0 && g->spheroid_orientation != 1)
+    return fail(c, CLR_E_ARG, "spheroid_orientation must be 0 (isotropic) or 1 (local tangential vMF)");
   const int em = g->event_model;
   if (!(em == 0 || em == 1 || em == 2 || em == 3 || em == 4 || em == 6))
     return fail(c, CLR_E_UNSUPPORTED, "unsupported particle event model");
@@ -306,6 +313,8 @@ void fill_params(Slot& s, const clr_config& g) {
   P.fs_model = g.free_surface_model; P.src_model = g.source_angular_model;
   P.event_model = g.event_model; P.sed_model = g.sediment_model;
   P.max_pb = g.max_particle_bounces; P.max_iter = g.max_iterations; P.cond_trials = g.conditional_max_trials;
+  P.rough = (float)g.rough_alpha; P.aspect = (float)g.spheroid_aspect; P.kappa = (float)g.spheroid_kappa;
+  P.orientation = g.spheroid_orientation;
   P.n_aux = g.n_aux; P.n_groups = g.n_groups; P.n_bins = s.n; P.n_theta = s.nt; P.n_radial = s.nrad;
   P.fn0 = g.field_n0; P.fn1 = g.field_n1; P.fn2 = g.field_n2;
   // lowest free-surface height (host float arithmetic of the device formula)
@@ -380,7 +389,7 @@ __global__ void clear_kernel(unsigned long long* __restrict__ p, size_t n_words,
 }
 
 struct RunShape {
-  int block, grid, chunk, min_chunk, plain;
+  int block, grid, chunk, min_chunk, kind;
   bool fixed;  // chunk given by the caller: every item has that size
 };
 
@@ -391,12 +400,12 @@ struct RunShape {
 constexpr uint64_t kGuidedSplit = 2;
 constexpr int kMinChunk = 2048;
 
-RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int plain) {
+RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int kind) {
   RunShape r{};
-  r.plain = plain;
+  r.kind = kind;
   r.block = (launch && launch->block > 0) ? launch->block : 256;
   r.block = std::min(256, std::max(64, (r.block / 64) * 64));
-  const int occ = clr_transport_occupancy(r.block, plain);
+  const int occ = clr_transport_occupancy(r.block, kind);
   int per_cu = (launch && launch->blocks_per_cu > 0) ? std::min(launch->blocks_per_cu, occ) : occ;
   r.grid = std::max(1, c->n_cu * per_cu);
   r.fixed = launch && launch->chunk > 0;
@@ -440,15 +449,19 @@ struct QueueBuilder {
   }
 };
 
-bool slots_plain(clr_ctx* c, const std::vector<int>& used) {
+// kernel variant that serves every configuration of a launch (CLR_KERNEL_*)
+int kernel_kind(clr_ctx* c, const std::vector<int>& used) {
+  bool plain = true, rough = false;
   for (int k : used) {
-    const clr_config& g = c->slots[k]->cfg;
-    if (g.event_model != CLR_EVENT_PLAIN_MIE) return false;
+    const Slot& s = *c->slots[k];
+    const clr_config& g = s.cfg;
+    if (g.event_model != CLR_EVENT_PLAIN_MIE) plain = false;
     if (!(g.sediment_model == CLR_SED_UNIFORM || g.sediment_model == CLR_SED_RZ_FIELD ||
           g.sediment_model == CLR_SED_XYZ_FIELD))
-      return false;
+      plain = false;
+    if (g.rough_alpha > 1e-7 || (g.event_model != CLR_EVENT_PLAIN_MIE && s.params.coherent < 1.0f)) rough = true;
   }
-  return true;
+  return plain ? CLR_KERNEL_PLAIN : rough ? CLR_KERNEL_ROUGH : CLR_KERNEL_GENERIC;
 }
 
 // Upload items, clear tallies, launch the persistent kernel (optionally
@@ -469,11 +482,11 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   L.n_records = n_records;
   L.grid = sh.grid;
   L.block = sh.block;
-  L.plain = sh.plain;
+  L.kind = sh.kind;
   const size_t words = sizeof(clr_tallies) * n_slots / 8;
   if (use_graph) {
     GraphCache& G = c->graph;
-    if (!(G.exec && G.n_items == L.n_items && G.grid == L.grid && G.block == L.block && G.plain == L.plain &&
+    if (!(G.exec && G.n_items == L.n_items && G.grid == L.grid && G.block == L.block && G.kind == L.kind &&
           G.n_slots == n_slots)) {
       G.reset();
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -484,7 +497,7 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
       if (le != hipSuccess) return fail(c, CLR_E_HIP, std::string("transport launch: ") + hipGetErrorString(le));
       if (ce != hipSuccess) return fail(c, CLR_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
       HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
-      G.n_items = L.n_items; G.grid = L.grid; G.block = L.block; G.plain = L.plain; G.n_slots = n_slots;
+      G.n_items = L.n_items; G.grid = L.grid; G.block = L.block; G.kind = L.kind; G.n_slots = n_slots;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
@@ -571,8 +584,8 @@ int clr_device_info(clr_ctx* c, int64_t info[8]) {
   info[3] = p.clockRate;
   info[4] = (int64_t)p.major * 100 + p.minor;
   info[5] = (int64_t)(p.totalGlobalMem >> 20);
-  info[6] = clr_transport_occupancy(256, 1);
-  info[7] = clr_transport_occupancy(256, 0);
+  info[6] = clr_transport_occupancy(256, CLR_KERNEL_PLAIN);
+  info[7] = clr_transport_occupancy(256, CLR_KERNEL_GENERIC);
   return 0;
 }
 
@@ -678,9 +691,6 @@ int clr_prepare(clr_ctx* c, int slot, const clr_config* g, const clr_particles* 
   HIPCHK(c, hipMemcpyAsync(s.majorants_h, s.majorants.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (ecode) return fail(c, ecode, "transport/sediment precompute failed with error code " + std::to_string(ecode));
-  if (s.mu_scalars_h[7] < 1.0 && (p->event_model == 3 || p->event_model == 4))
-    return fail(c, CLR_E_UNSUPPORTED,
-                "wave coherent fraction < 1 needs the spheroid morphology channel; set wave_surface_rms_height_m=0");
   s.cfg = *g;
   fill_params(s, *g);
   HIPCHK(c, hipMemcpy(c->params.as<DevParams>() + slot, &s.params, sizeof(DevParams), hipMemcpyHostToDevice));
@@ -716,8 +726,7 @@ int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_la
   if (n == 0) return fail(c, CLR_E_ARG, "n must be positive");
   if (n > (1ull << 56) || offset > (1ull << 62)) return fail(c, CLR_E_RANGE, "history range too large");
   HIPCHK(c, hipSetDevice(c->device));
-  const int plain = slots_plain(c, {0}) ? 1 : 0;
-  const RunShape sh = pick_shape(c, launch, n, plain);
+  const RunShape sh = pick_shape(c, launch, n, kernel_kind(c, {0}));
   // one launch covers at most 2^31 histories so LDS u32 tallies cannot wrap
   const uint64_t per_launch = 1ull << 31;
   const int n_slots = (int)c->slots.size();
@@ -771,8 +780,7 @@ int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_l
     used.push_back(k);
     total += bi[i].n_histories;
   }
-  const int plain = slots_plain(c, used) ? 1 : 0;
-  const RunShape sh = pick_shape(c, launch, total, plain);
+  const RunShape sh = pick_shape(c, launch, total, kernel_kind(c, used));
   QueueBuilder q(sh, total);
   for (int i = 0; i < n_items; ++i)
     q.add(bi[i].config, 0, bi[i].n_histories, bi[i].n_histories, bi[i].seed, bi[i].history_offset);

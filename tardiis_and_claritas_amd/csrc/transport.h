@@ -23,6 +23,8 @@ struct DevParams {
   float field_inv0, field_inv1, field_inv2, sum_g_f, sum_w_f;
   int fs_model, src_model, event_model, sed_model;
   int max_pb, max_iter, cond_trials, n_aux;
+  float rough, aspect, kappa;     // particle Beckmann alpha, spheroid a/b, axis-law kappa
+  int orientation;                // spheroid axis law (0 isotropic, 1 local-tangential vMF)
   int n_groups, n_bins, n_theta, n_radial;
   int fn0, fn1, fn2, pad0;
   // per-bin tables
@@ -67,9 +69,16 @@ struct TransportLaunch {
   clr_record* records;       // optional, config 0 only
   uint64_t n_records;
   int grid, block;
-  int plain;  // 1: every configuration uses the plain-Mie law and a shared or uniform field
+  int kind;   // kernel variant, CLR_KERNEL_*
 };
+
+// Kernel variants: PLAIN when every configuration uses the plain-Mie law and a
+// uniform or shared FP32 field; ROUGH when one needs the Smith microsurface
+// walk (rough particles) or the spheroid-morphology wave channel; GENERIC
+// otherwise.  The rare, FP64-heavy rough-surface code lives only in ROUGH, so
+// it costs the other variants no registers.
+enum : int { CLR_KERNEL_GENERIC = 0, CLR_KERNEL_PLAIN = 1, CLR_KERNEL_ROUGH = 2 };
 
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s);
 // Resident workgroups per CU for `block` threads (occupancy query).
-int clr_transport_occupancy(int block, int plain);
+int clr_transport_occupancy(int block, int kind);

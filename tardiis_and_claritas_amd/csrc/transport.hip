@@ -34,13 +34,15 @@ constexpr float kInfF = 3.402823466e+38F;
 constexpr float kPiF = 3.14159265358979323846f;
 constexpr float kEpsF = 2.0e-7f;
 constexpr double kDEps = 2.220446049250313080847263336181640625e-16;
+constexpr double kPiD = 3.141592653589793238462643383279502884;
 constexpr double kFEps = 1.1920928955078125e-7;
 constexpr int kEmergencyCap = 16384;  // acrylic / headspace loop caps, CLARITAS.py:3147,3244
 constexpr int kLongChain = 128;       // acrylic bounces after which a visit is stepped to its end
 
 // production failure codes and subcodes, CLARITAS.py:1846-1895
+enum : int { MS_HIT = 1, MS_ESCAPE = 2, MS_NUMERIC = 3, MS_WATCHDOG = 4 };  // walk outcomes
 enum : int {
-  PF_OK = 0, PF_MICRO_NUMERICAL = 1, PF_MACRO_GEOMETRY = 3, PF_MACRO_BOUNCE = 4,
+  PF_OK = 0, PF_MICRO_NUMERICAL = 1, PF_MICRO_WATCHDOG = 2, PF_MACRO_GEOMETRY = 3, PF_MACRO_BOUNCE = 4,
   PF_ACRYLIC_NO_HIT = 5, PF_ACRYLIC_BOUNCE = 6, PF_BOTTOM_GEOMETRY = 7,
   PF_WATER_NO_BOUNDARY = 9, PF_ITERATION_LIMIT = 10, PF_NONFINITE_POSITION = 11,
   PF_NONFINITE_DIRECTION = 12, PF_INVALID_MEDIUM = 13, PF_OTHER = 15, PF_SPHEROID = 16,
@@ -53,7 +55,14 @@ enum : int {
   SUB_ITERATION_LIMIT = 3002, SUB_NONFINITE_POSITION = 3003, SUB_NONFINITE_DIRECTION = 3004,
   SUB_WATER_OUTSIDE = 3005, SUB_HEADSPACE_NO_HIT = 3006, SUB_MAJORANT = 4001,
   MF_REFLECTION_INVALID = 31, MF_REFRACTION_INVALID = 32, MF_SIDE_STATE = 33,
-  MF_NONFINITE_DIRECTION = 34, MF_FRESNEL_INVALID = 35
+  MF_NONFINITE_DIRECTION = 34, MF_FRESNEL_INVALID = 35,
+  // microsurface walk subcodes, CLARITAS.py:1810-1839
+  MF_PROJ_AREA = 10, MF_CDF = 11, MF_INV_CDF = 12, MF_VISIBILITY = 13, MF_NORMALISATION = 14,
+  MF_HEIGHT_SAMPLE = 21, MF_LAMBDA = 23, MF_C0 = 24, MF_LOG_C0 = 25, MF_G1 = 26, MF_ESCAPE_PROB = 27,
+  MF_RANDOM = 28, MF_LOG_CNEXT = 29, MF_NORMAL = 30, MF_CNEXT_RANGE = 36, MF_HEIGHT_INV = 37,
+  MF_HEIGHT_OUT = 38, MF_WATCHDOG = 40,
+  SUB_SPHEROID_ASPECT = 1101, SUB_SPHEROID_AXES = 1102, SUB_SPHEROID_ENTRY = 1103, SUB_SPHEROID_CHORD = 1104,
+  SUB_SPHEROID_EXIT_SIDE = 1105, SUB_SPHEROID_INTERNAL_SIDE = 1106, SUB_SPHEROID_BOUNCE_LIMIT = 1107
 };
 
 // ------------------------------------------------------------------ RNG --
@@ -907,114 +916,389 @@ __device__ __forceinline__ bool sphere_accessible(const DevParams& P, double cx,
   const double tol = 128.0 * kDEps * scale;
   return wall >= -tol && bottom >= -tol && top >= -tol;
 }
-// smooth limit of the microsurface walk, CLARITAS.py:2323-2356
-__device__ __forceinline__ bool smooth_interface(Rng& r, float bx, float by, float bz, float n_top, float k_top,
-                                 float n_bot, float k_bot, float& dx, float& dy, float& dz,
-                                 bool& out_top, int& fc) {
+// ------------------------------------------------ rough particle surfaces --
+// Smith microsurface random walk over a Beckmann slope distribution (Heitz,
+// Hanika, d'Eon & Dachsbacher, ACM TOG 35(4), 2016), in the signed,
+// log-domain form of CLARITAS.py:1792-2430, FP64 where the reference uses it.
+// Everything here is force-inlined: the persistent kernels contain no
+// function calls (with calls present, the generic kernel computed wrong
+// results on ROCm 7.2 even along paths that never executed a call).  The
+// finite-particle event (particle_event) holds the only micro_interface site
+// of each caller, so the walk is inlined twice per kernel.
+constexpr int kMicroWatchdog = 4096;
+constexpr double kSqrtPi = 1.77245385090551602729816748334114518;
+constexpr double kInv2SqrtPi = 0.28209479177387814347403972578038629;
+
+// B(x) = 1 - x sqrt(pi) erfcx(x), x >= 0; asymptotic series for x >= 5,
+// CLARITAS.py:1926-1942
+__device__ double beck_tail_B(double x) {
+  if (!(x >= 0.0) || !isfinite(x)) return -1.0;
+  double b;
+  if (x < 5.0) {
+    b = 1.0 - x * kSqrtPi * (exp(x * x) * erfc(x));
+  } else {
+    const double y = 1.0 / (x * x);
+    // sum_k (-1)^(k+1) (2k-1)!! / 2^k y^k, k = 1..7
+    b = 0.5 * y - 0.75 * y * y + 1.875 * y * y * y - 6.5625 * y * y * y * y + 29.53125 * y * y * y * y * y -
+        162.421875 * y * y * y * y * y * y + 1055.7421875 * y * y * y * y * y * y * y;
+  }
+  return (isfinite(b) && b > 0.0) ? b : -1.0;
+}
+// visible-slope marginal CDF for slope_i >= -1, CLARITAS.py:1947-1962
+__device__ bool beck_visible_cdf(double s, double si, double& out) {
+  if (!isfinite(s) || !isfinite(si)) return false;
+  if (s >= si) { out = 1.0; return true; }
+  const double q = kInv2SqrtPi * exp(-si * si) + 0.5 * si * erfc(-si);
+  if (!(q > 0.0) || !isfinite(q)) return false;
+  const double n = kInv2SqrtPi * exp(-s * s) + 0.5 * si * erfc(-s);
+  double f = n / q;
+  if (!isfinite(f)) return false;
+  if (f < 0.0 && f > -1.0e-13) f = 0.0;
+  if (f > 1.0 && f < 1.0 + 1.0e-13) f = 1.0;
+  if (f < 0.0 || f > 1.0) return false;
+  out = f;
+  return true;
+}
+// log of the scaled tail CDF at slope -(x+u), CLARITAS.py:1986-2013
+__device__ bool beck_tail_logcdf(double x, double u, double Bx, double& out) {
+  const double y = x + u, By = beck_tail_B(y);
+  if (!(By > 0.0) || !isfinite(By)) return false;
+  const double C = u / y + (x / y) * By;
+  if (!(C > 0.0) || !isfinite(C)) return false;
+  const double lr = -(2.0 * x * u + u * u) + log(C / Bx);
+  if (!isfinite(lr)) return false;
+  out = lr;
+  return true;
+}
+// alpha = 1 visible slopes for signed cot(theta_i) = si, CLARITAS.py:1964-2045
+__device__ bool beck_sample_slopes(double si, double u1, double u2, double& sxo, double& syo, int& fc) {
   fc = 0;
+  if (!isfinite(u1) || !isfinite(u2) || !(u1 > 0.0 && u1 < 1.0) || !(u2 > 0.0 && u2 < 1.0)) { fc = MF_CDF; return false; }
+  const double kCot1e4 = 9999.999966666666;  // cot(1e-4 rad): the visible law is the NDF
+  if (isinf(si) || si > kCot1e4) {
+    if (si < 0.0) { fc = MF_PROJ_AREA; return false; }
+    const double r = sqrt(-log(u1)), ph = 2.0 * kPiD * u2, sx = r * cos(ph), sy = r * sin(ph);
+    if (!isfinite(sx) || !isfinite(sy)) { fc = MF_INV_CDF; return false; }
+    sxo = sx; syo = sy;
+    return true;
+  }
+  if (!isfinite(si)) { fc = MF_PROJ_AREA; return false; }
+  double sx;
+  if (si < -1.0) {
+    const double x = -si, Bx = beck_tail_B(x);
+    if (!(Bx > 0.0) || !isfinite(Bx)) { fc = MF_PROJ_AREA; return false; }
+    const double target = log(u1);
+    double lo = 0.0, hi = fmax(1.0 / x, 1.0e-12), lr = 0.0;
+    bool ok = false;
+    for (int it = 0; it < 64; ++it) {
+      if (!beck_tail_logcdf(x, hi, Bx, lr)) { fc = MF_CDF; return false; }
+      if (lr <= target) { ok = true; break; }
+      hi *= 2.0;
+      if (!isfinite(hi)) { fc = MF_INV_CDF; return false; }
+    }
+    if (!ok) { fc = MF_INV_CDF; return false; }
+    for (int it = 0; it < 52; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (!beck_tail_logcdf(x, mid, Bx, lr)) { fc = MF_CDF; return false; }
+      if (lr > target) lo = mid; else hi = mid;
+    }
+    sx = -(x + 0.5 * (lo + hi));
+  } else {
+    double lo = fmin(-1.0, si - 1.0), hi = si, f = 1.0;
+    bool ok = false;
+    for (int it = 0; it < 64; ++it) {
+      if (!beck_visible_cdf(lo, si, f)) { fc = MF_CDF; return false; }
+      if (f <= u1) { ok = true; break; }
+      lo = (lo < 0.0) ? 2.0 * lo : lo - 1.0;
+      if (!isfinite(lo)) { fc = MF_INV_CDF; return false; }
+    }
+    if (!ok) { fc = MF_INV_CDF; return false; }
+    for (int it = 0; it < 48; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (!beck_visible_cdf(mid, si, f)) { fc = MF_CDF; return false; }
+      if (f < u1) lo = mid; else hi = mid;
+    }
+    sx = 0.5 * (lo + hi);
+  }
+  const double ey = 2.0 * u2 - 1.0;
+  if (!(ey > -1.0 && ey < 1.0) || !isfinite(ey)) { fc = MF_INV_CDF; return false; }
+  const double sy = erfinv(ey);
+  if (!isfinite(sx) || !isfinite(sy)) { fc = MF_INV_CDF; return false; }
+  sxo = sx; syo = sy;
+  return true;
+}
+// visible microfacet normal about the TOP axis b for direction w (signed),
+// two open uniforms, CLARITAS.py:2073-2129
+__device__ bool beck_vndf_normal(Rng& r, float alpha, float bx, float by, float bz, float wx, float wy, float wz,
+                                 double& mx, double& my, double& mz, int& fc) {
+  fc = 0;
+  if (!(alpha >= 0.0f) || !isfinite(alpha)) { fc = MF_NORMALISATION; return false; }
   normalize3(bx, by, bz);
-  normalize3(dx, dy, dz);
-  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(dx) || !isfinite(dy) || !isfinite(dz)) {
+  normalize3(wx, wy, wz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(wx) || !isfinite(wy) || !isfinite(wz)) {
     fc = MF_NONFINITE_DIRECTION; return false;
   }
-  if (dot3(dx, dy, dz, bx, by, bz) >= -1.0e-7f) { fc = MF_SIDE_STATE; return false; }
+  if (alpha <= 1.0e-8f) {
+    const double vd = (double)wx * bx + (double)wy * by + (double)wz * bz;
+    if (!(vd > 0.0)) { fc = MF_VISIBILITY; return false; }
+    mx = bx; my = by; mz = bz;
+    return true;
+  }
+  float t1x, t1y, t1z, t2x, t2y, t2z;
+  perpendicular_basis(bx, by, bz, t1x, t1y, t1z, t2x, t2y, t2z);
+  const double lx = (double)wx * t1x + (double)wy * t1y + (double)wz * t1z;
+  const double ly = (double)wx * t2x + (double)wy * t2y + (double)wz * t2z;
+  const double lz = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  double sxv = (double)alpha * lx, syv = (double)alpha * ly, szv = lz;
+  const double vl = sqrt(sxv * sxv + syv * syv + szv * szv);
+  if (!(vl > 0.0) || !isfinite(vl)) { fc = MF_NORMALISATION; return false; }
+  sxv /= vl; syv /= vl; szv /= vl;
+  const double radial = hypot(sxv, syv);
+  double si;
+  if (radial == 0.0) {
+    if (szv > 0.0) si = __builtin_inf();
+    else { fc = MF_PROJ_AREA; return false; }
+  } else {
+    si = szv / radial;
+    if (!isfinite(si)) { fc = MF_PROJ_AREA; return false; }
+  }
+  const double u1 = Uopen(r), u2 = Uopen(r);
+  double s0x = 0.0, s0y = 0.0;
+  int f2 = 0;
+  if (!beck_sample_slopes(si, u1, u2, s0x, s0y, f2)) { fc = f2 ? f2 : MF_INV_CDF; return false; }
+  const double ph = atan2(syv, sxv), cp = cos(ph), sp = sin(ph);
+  const double tx = cp * s0x - sp * s0y, ty = sp * s0x + cp * s0y;
+  const double sx = (double)alpha * tx, sy = (double)alpha * ty;
+  if (!isfinite(sx) || !isfinite(sy)) { fc = MF_NORMAL; return false; }
+  const double inv = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
+  if (!(inv > 0.0) || !isfinite(inv)) { fc = MF_NORMALISATION; return false; }
+  const double nlx = -sx * inv, nly = -sy * inv, nlz = inv;
+  double nx = nlx * t1x + nly * t2x + nlz * bx;
+  double ny = nlx * t1y + nly * t2y + nlz * by;
+  double nz = nlx * t1z + nly * t2z + nlz * bz;
+  const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+  if (!(nn > 0.0) || !isfinite(nn)) { fc = MF_NORMALISATION; return false; }
+  nx /= nn; ny /= nn; nz /= nn;
+  if (!(nx * bx + ny * by + nz * bz > 0.0)) { fc = MF_NORMAL; return false; }
+  if (!(nx * wx + ny * wy + nz * wz > 0.0)) { fc = MF_VISIBILITY; return false; }
+  mx = nx; my = ny; mz = nz;
+  return true;
+}
+// signed Smith Lambda over the whole sphere of directions, CLARITAS.py:2162-2184
+__device__ double smith_lambda(float wx, float wy, float wz, float bx, float by, float bz, float alpha) {
+  double c = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  if (!isfinite(c) || !isfinite(alpha) || !(alpha >= 0.0f)) return __builtin_nan("");
+  c = fmax(-1.0, fmin(1.0, c));
+  const double ss2 = fmax(0.0, 1.0 - c * c);
+  if (ss2 == 0.0) return c >= 0.0 ? 0.0 : -1.0;
+  if (c == 0.0) return copysign(__builtin_inf(), c);
+  if (alpha == 0.0f) return c >= 0.0 ? 0.0 : -1.0;
+  const double aa = fabs(c) / ((double)alpha * sqrt(ss2));
+  if (!(aa > 0.0) || !isfinite(aa)) return c >= 0.0 ? 0.0 : -1.0;
+  const double B = beck_tail_B(aa);
+  if (!(B > 0.0) || !isfinite(B)) return __builtin_nan("");
+  const double lp = exp(-(aa * aa) - log(2.0 * kSqrtPi * aa)) * B;
+  if (!(lp >= 0.0) || !isfinite(lp)) return __builtin_nan("");
+  return c >= 0.0 ? lp : (-1.0 - lp);
+}
+__device__ __forceinline__ double uniform_C1(double h) { return fmin(1.0, fmax(0.0, 0.5 * (h + 1.0))); }
+__device__ __forceinline__ double uniform_invC1(double u) { return 2.0 * fmin(1.0, fmax(0.0, u)) - 1.0; }
+// next height of the walk, CLARITAS.py:2211-2310
+__device__ int smith_height(double U, float wx, float wy, float wz, float bx, float by, float bz, float alpha,
+                            float h, float& out_h, int& fc) {
+  fc = 0;
+  if (!isfinite(U) || !isfinite(wx) || !isfinite(wy) || !isfinite(wz) || !isfinite(h) || !(alpha >= 0.0f) ||
+      !isfinite(alpha) || !(U > 0.0 && U < 1.0)) {
+    fc = MF_RANDOM; return MS_NUMERIC;
+  }
+  double c = (double)wx * bx + (double)wy * by + (double)wz * bz;
+  if (!isfinite(c)) { fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
+  c = fmax(-1.0, fmin(1.0, c));
+  const double C0 = uniform_C1((double)h);
+  if (!(C0 >= 0.0 && C0 <= 1.0) || !isfinite(C0)) { fc = MF_C0; return MS_NUMERIC; }
+  const double ss2 = fmax(0.0, 1.0 - c * c);
+  if (ss2 == 0.0) {
+    if (c > 0.0) return MS_ESCAPE;
+    if (c < 0.0) {
+      const double hn = uniform_invC1(U * C0);
+      if (!isfinite(hn)) { fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+      out_h = (float)hn;
+      return MS_HIT;
+    }
+    out_h = h;
+    return MS_HIT;
+  }
+  if (c == 0.0) { out_h = h; return MS_HIT; }
+  const double lam = smith_lambda(wx, wy, wz, bx, by, bz, alpha);
+  if (isnan(lam)) { fc = MF_LAMBDA; return MS_NUMERIC; }
+  if (c > 0.0) {
+    if (!(lam >= 0.0)) { fc = MF_LAMBDA; return MS_NUMERIC; }
+    if (C0 <= 0.0 || C0 >= 1.0 || lam == 0.0) return MS_ESCAPE;
+    const double lc0 = log(C0);
+    if (!isfinite(lc0)) { fc = MF_LOG_C0; return MS_NUMERIC; }
+    const double lg1 = lam * lc0;
+    if (!(lg1 <= 0.0) || !isfinite(lg1)) { fc = MF_G1; return MS_NUMERIC; }
+    const double p_hit = -expm1(lg1);
+    if (!(p_hit >= 0.0 && p_hit <= 1.0) || !isfinite(p_hit)) { fc = MF_ESCAPE_PROB; return MS_NUMERIC; }
+    if (U > p_hit) return MS_ESCAPE;
+    const double l1u = log1p(-U);
+    if (!isfinite(l1u)) { fc = MF_RANDOM; return MS_NUMERIC; }
+    double lcn = lc0 - l1u / lam;
+    if (!isfinite(lcn)) { fc = MF_LOG_CNEXT; return MS_NUMERIC; }
+    if (lcn > 64.0 * kDEps) { fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+    if (lcn > 0.0) lcn = 0.0;
+    const double cn = exp(lcn);
+    if (!(cn >= 0.0 && cn <= 1.0) || !isfinite(cn)) { fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+    const double hn = uniform_invC1(cn);
+    if (!isfinite(hn)) { fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+    out_h = (float)hn;
+    if (!isfinite(out_h)) { fc = MF_HEIGHT_OUT; return MS_NUMERIC; }
+    return MS_HIT;
+  }
+  if (!(lam <= -1.0) || !isfinite(lam)) { fc = MF_LAMBDA; return MS_NUMERIC; }
+  if (C0 <= 0.0) { out_h = -1.0f; return MS_HIT; }
+  const double lc0 = log(C0);
+  if (!isfinite(lc0)) { fc = MF_LOG_C0; return MS_NUMERIC; }
+  const double l1u = log1p(-U);
+  if (!isfinite(l1u)) { fc = MF_RANDOM; return MS_NUMERIC; }
+  const double lcn = lc0 - l1u / lam;
+  if (!isfinite(lcn)) { fc = MF_LOG_CNEXT; return MS_NUMERIC; }
+  if (lcn > 0.0) { fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+  const double cn = exp(lcn);
+  if (!(cn >= 0.0 && cn <= C0 * (1.0 + 8.0e-15)) || !isfinite(cn)) { fc = MF_CNEXT_RANGE; return MS_NUMERIC; }
+  const double hn = uniform_invC1(cn);
+  if (!isfinite(hn)) { fc = MF_HEIGHT_INV; return MS_NUMERIC; }
+  out_h = (float)hn;
+  if (!isfinite(out_h)) { fc = MF_HEIGHT_OUT; return MS_NUMERIC; }
+  return MS_HIT;
+}
+
+// direction in/out and the outcome of one interface encounter
+struct MicroIO {
+  float dx, dy, dz;
+  int top, fc;
+};
+// rough-surface walk (alpha > 1e-7) from an incident direction that points
+// into the TOP side, CLARITAS.py:2358-2430
+__device__ __forceinline__ int rough_walk(Rng* __restrict__ rp, float alpha, float bx, float by, float bz,
+                                       float n_top, float k_top, float n_bot, float k_bot,
+                                       MicroIO* __restrict__ io) {
+  Rng r = *rp;
+  float dx = io->dx, dy = io->dy, dz = io->dz;
+  int status = MS_WATCHDOG, fc = MF_WATCHDOG, top = 0;
+  float h = 1.0f + (2.0f * 0.999f - 1.0f);
+  bool outside = true;
+  for (int it = 0; it < kMicroWatchdog; ++it) {
+    float nh = 0.0f;
+    int hs, f2 = 0;
+    const double Uh = Uopen(r);
+    if (outside) {
+      hs = smith_height(Uh, dx, dy, dz, bx, by, bz, alpha, h, nh, f2);
+      if (hs == MS_ESCAPE) { status = MS_ESCAPE; fc = 0; top = 1; break; }
+    } else {
+      float t = 0.0f;
+      hs = smith_height(Uh, -dx, -dy, -dz, bx, by, bz, alpha, -h, t, f2);
+      if (hs == MS_ESCAPE) { status = MS_ESCAPE; fc = 0; top = 0; break; }
+      if (hs == MS_HIT) nh = -t;
+    }
+    if (hs != MS_HIT || !isfinite(nh)) { status = MS_NUMERIC; fc = f2 ? f2 : MF_HEIGHT_SAMPLE; break; }
+    h = nh;
+    const float wix = -dx, wiy = -dy, wiz = -dz;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    f2 = 0;
+    bool ok;
+    if (outside) {
+      ok = beck_vndf_normal(r, alpha, bx, by, bz, wix, wiy, wiz, mx, my, mz, f2);
+    } else {
+      ok = beck_vndf_normal(r, alpha, bx, by, bz, -wix, -wiy, -wiz, mx, my, mz, f2);
+      mx = -mx; my = -my; mz = -mz;
+    }
+    if (!ok) { status = MS_NUMERIC; fc = f2 ? f2 : MF_INV_CDF; break; }
+    const double vis = (double)wix * mx + (double)wiy * my + (double)wiz * mz;
+    if (!isfinite(mx) || !isfinite(my) || !isfinite(mz)) { status = MS_NUMERIC; fc = MF_NORMAL; break; }
+    if (!(vis > 0.0)) { status = MS_NUMERIC; fc = MF_VISIBILITY; break; }
+    const float ci = (float)fmin(1.0, fmax(0.0, vis));
+    const float n1 = outside ? n_top : n_bot, k1 = outside ? k_top : k_bot;
+    const float n2 = outside ? n_bot : n_top, k2 = outside ? k_bot : k_top;
+    if (!(n1 > 0.0f && n2 > 0.0f)) { status = MS_NUMERIC; fc = MF_REFRACTION_INVALID; break; }
+    float R, ts, tn;
+    if (!complex_interface(ci, n1, k1, n2, k2, R, ts, tn) || !isfinite(R)) {
+      status = MS_NUMERIC; fc = MF_FRESNEL_INVALID; break;
+    }
+    double ox, oy, oz;
+    if (U(r) < R) {
+      const double dd = (double)dx * mx + (double)dy * my + (double)dz * mz;
+      ox = (double)dx - 2.0 * dd * mx; oy = (double)dy - 2.0 * dd * my; oz = (double)dz - 2.0 * dd * mz;
+    } else {
+      float tx, ty, tz;
+      if (!energy_refract3(dx, dy, dz, (float)mx, (float)my, (float)mz, n1, k1, n2, k2, tx, ty, tz)) {
+        status = MS_NUMERIC; fc = MF_REFRACTION_INVALID; break;
+      }
+      ox = tx; oy = ty; oz = tz;
+      outside = !outside;
+    }
+    const double qn = sqrt(ox * ox + oy * oy + oz * oz);
+    if (!(qn > 0.0) || !isfinite(qn) || !isfinite(ox) || !isfinite(oy) || !isfinite(oz)) {
+      status = MS_NUMERIC; fc = MF_REFRACTION_INVALID; break;
+    }
+    float fx = (float)(ox / qn), fy = (float)(oy / qn), fz = (float)(oz / qn);
+    if (!isfinite(fx) || !isfinite(fy) || !isfinite(fz)) { status = MS_NUMERIC; fc = MF_NONFINITE_DIRECTION; break; }
+    normalize3(fx, fy, fz);
+    if (!isfinite(fx) || !isfinite(fy) || !isfinite(fz)) { status = MS_NUMERIC; fc = MF_NONFINITE_DIRECTION; break; }
+    dx = fx; dy = fy; dz = fz;
+  }
+  *rp = r;
+  io->dx = dx; io->dy = dy; io->dz = dz; io->top = top; io->fc = fc;
+  return status;
+}
+// smooth limit, CLARITAS.py:2336-2356
+__device__ __forceinline__ int smooth_interface(Rng& r, float bx, float by, float bz, float n_top, float k_top,
+                                                float n_bot, float k_bot, float& dx, float& dy, float& dz,
+                                                bool& out_top, int& fc) {
   const float ci = fminf(fmaxf(dot3(-dx, -dy, -dz, bx, by, bz), 0.0f), 1.0f);
   float R, ts, tn;
   if (!complex_interface(ci, n_top, k_top, n_bot, k_bot, R, ts, tn) || !isfinite(R)) {
-    fc = MF_FRESNEL_INVALID; return false;
+    fc = MF_FRESNEL_INVALID; return MS_NUMERIC;
   }
   float ox = dx, oy = dy, oz = dz;
   if (U(r) < R) {
     reflect3(ox, oy, oz, bx, by, bz);
     out_top = true;
-    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_REFLECTION_INVALID; return false; }
+    if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_REFLECTION_INVALID; return MS_NUMERIC; }
   } else {
     if (!energy_refract3(dx, dy, dz, bx, by, bz, n_top, k_top, n_bot, k_bot, ox, oy, oz)) {
-      fc = MF_REFRACTION_INVALID; return false;
+      fc = MF_REFRACTION_INVALID; return MS_NUMERIC;
     }
     out_top = false;
   }
-  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_NONFINITE_DIRECTION; return false; }
+  if (!isfinite(ox) || !isfinite(oy) || !isfinite(oz)) { fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC; }
   dx = ox; dy = oy; dz = oz;
-  return true;
+  return MS_ESCAPE;
+}
+// one encounter of direction d with a (rough) dielectric interface whose TOP
+// side has normal b: MS_ESCAPE with out_top (left on the top side) or not
+// (crossed), else MS_NUMERIC / MS_WATCHDOG with fc.  CLARITAS.py:2314-2430
+template <bool kRough>
+__device__ __forceinline__ int micro_interface(Rng& r, float alpha, float bx, float by, float bz, float n_top,
+                                               float k_top, float n_bot, float k_bot, float& dx, float& dy,
+                                               float& dz, bool& out_top, int& fc) {
+  fc = 0;
+  normalize3(bx, by, bz);
+  normalize3(dx, dy, dz);
+  if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(dx) || !isfinite(dy) || !isfinite(dz)) {
+    fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC;
+  }
+  if (dot3(dx, dy, dz, bx, by, bz) >= -1.0e-7f) { fc = MF_SIDE_STATE; return MS_NUMERIC; }
+  if (!kRough || alpha <= 1.0e-7f)
+    return smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+  Rng t = r;
+  MicroIO io{dx, dy, dz, 0, 0};
+  const int st = rough_walk(&t, alpha, bx, by, bz, n_top, k_top, n_bot, k_bot, &io);
+  r = t;
+  dx = io.dx; dy = io.dy; dz = io.dz; out_top = io.top != 0; fc = io.fc;
+  return st;
 }
 
-struct PCount {
-  int pr, per, pir, pabs;
-  uint32_t n_abs;
-};
-// smooth Fresnel sphere with internal Beer-Lambert chords, CLARITAS.py:2497-2599.
-// Returns 1 escape, 2 absorbed, 0 failure.
-__device__ __forceinline__ int sphere_interaction(const DevParams& P, Rng& r, float n_m, float n_p, float k_p,
-                                  float radius, float gx, float gy, float gz, double cx, double cy,
-                                  double cz, float& x, float& y, float& z, float& vx, float& vy,
-                                  float& vz, PCount& pc, int& pf, int& pfs) {
-  pf = PF_OK; pfs = SUB_NONE;
-  if (radius <= 0.0f) return 1;
-  bool top = true;
-  int fc = 0;
-  if (!smooth_interface(r, gx, gy, gz, n_m, 0.0f, n_p, k_p, vx, vy, vz, top, fc)) {
-    pf = PF_MICRO_NUMERICAL; pfs = fc; return 0;
-  }
-  if (top) { pc.pr++; pc.per++; return 1; }
-  if (dot3(vx, vy, vz, gx, gy, gz) >= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_ENTRY_SIDE; return 0; }
-  const double rr = (double)radius, rr2 = rr * rr;
-  double qx = rr * (double)gx, qy = rr * (double)gy, qz = rr * (double)gz;
-  const double qn0 = sqrt(qx * qx + qy * qy + qz * qz);
-  if (!(qn0 > 0.0) || !isfinite(qn0)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
-  qx *= rr / qn0; qy *= rr / qn0; qz *= rr / qn0;
-  const float alpha_abs = (P.wavelength > 0.0f && k_p > 0.0f) ? (4.0f * kPiF * k_p / P.wavelength) : 0.0f;
-  for (int bounce = 0; bounce <= P.max_pb; ++bounce) {
-    const double dvx = vx, dvy = vy, dvz = vz;
-    const double A = dvx * dvx + dvy * dvy + dvz * dvz, q2 = qx * qx + qy * qy + qz * qz;
-    if (!(A > 0.0) || !isfinite(A) || !isfinite(q2)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
-    const double qnorm = sqrt(q2);
-    if (!(qnorm > 0.0) || !isfinite(qnorm)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
-    const double qp = rr / qnorm;
-    qx *= qp; qy *= qp; qz *= qp;
-    const double B = qx * dvx + qy * dvy + qz * dvz;
-    if (!(B < 0.0) || !isfinite(B)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
-    const double chord = -2.0 * B / A;
-    if (!(chord > 0.0) || !isfinite(chord)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
-    const double tqx = qx + chord * dvx, tqy = qy + chord * dvy, tqz = qz + chord * dvz;
-    const double arr = tqx * dvx + tqy * dvy + tqz * dvz, tq2 = tqx * tqx + tqy * tqy + tqz * tqz;
-    if (!(arr > 0.0) || !isfinite(arr) || !isfinite(tq2) ||
-        fabs(tq2 - rr2) > 512.0 * kDEps * fmax(rr2, 1.0e-30)) {
-      pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0;
-    }
-    if (alpha_abs > 0.0f) {
-      pc.n_abs++;
-      const double dist = -log(Uopen(r)) / (double)alpha_abs;
-      if (dist < chord) {
-        qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
-        x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
-        pc.pabs++;
-        return 2;
-      }
-    }
-    qx = tqx; qy = tqy; qz = tqz;
-    const double qn = sqrt(qx * qx + qy * qy + qz * qz);
-    if (!(qn > 0.0) || !isfinite(qn)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
-    const double pj = rr / qn;
-    qx *= pj; qy *= pj; qz *= pj;
-    x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
-    float nox = (float)(qx / rr), noy = (float)(qy / rr), noz = (float)(qz / rr);
-    normalize3(nox, noy, noz);
-    top = true;
-    if (!smooth_interface(r, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, top, fc)) {
-      pf = PF_MICRO_NUMERICAL; pfs = fc; return 0;
-    }
-    if (!top) {
-      if (dot3(vx, vy, vz, nox, noy, noz) <= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_EXIT_SIDE; return 0; }
-      return 1;
-    }
-    if (dot3(vx, vy, vz, nox, noy, noz) >= 0.0f) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
-    pc.pr++; pc.pir++;
-    if (bounce == P.max_pb) { pf = PF_MACRO_BOUNCE; pfs = SUB_SPHERE_BOUNCE_LIMIT; return 0; }
-  }
-  pf = PF_OTHER; pfs = SUB_NONE;
-  return 0;
-}
 // inverse-CDF polar angle on the float table, CLARITAS.py:2632-2641
 __device__ __forceinline__ float sample_mie_theta(Rng& r, const DevParams& P, int pidx) {
   const int nt = P.n_theta;
@@ -1040,6 +1324,209 @@ __device__ __forceinline__ void mie_scatter(Rng& r, const DevParams& P, int pidx
   normalize3(vx, vy, vz);
 }
 
+// ------------------------------------------- incoherent wave channel event --
+// direction-only finite-particle event on a volume-preserving spheroid
+// a = r q^(2/3), b = r q^(-1/3) (q = a/b), CLARITAS.py:2655-2866
+__device__ __forceinline__ void random_axis(Rng& r, float& ux, float& uy, float& uz) {
+  const float cz = 2.0f * U(r) - 1.0f, az = 2.0f * kPiF * U(r), sz = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
+  ux = sz * cosf(az); uy = sz * sinf(az); uz = cz;
+}
+// isotropic, or the head-tail symmetric von Mises-Fisher law about the local
+// flow direction f, CLARITAS.py:2675-2702
+__device__ void spheroid_axis(Rng& r, int model, float kappa, float fx, float fy, float fz, float& ux, float& uy,
+                              float& uz) {
+  if (model == 0 || !(kappa > 1e-7f) || !isfinite(kappa)) { random_axis(r, ux, uy, uz); return; }
+  normalize3(fx, fy, fz);
+  if (!(fx * fx + fy * fy + fz * fz > 0.5f)) { random_axis(r, ux, uy, uz); return; }
+  const float u = fminf(fmaxf(U(r), 1e-12f), 1.0f - 1e-7f);
+  const float em2 = expf(-2.0f * kappa);
+  float c = 1.0f + logf(u + (1.0f - u) * em2) / kappa;
+  c = fminf(fmaxf(c, -1.0f), 1.0f);
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(fx, fy, fz, e1x, e1y, e1z, e2x, e2y, e2z);
+  const float ph = 2.0f * kPiF * U(r), st = sqrtf(fmaxf(0.0f, 1.0f - c * c)), cp = cosf(ph), sp = sinf(ph);
+  ux = c * fx + st * (cp * e1x + sp * e2x);
+  uy = c * fy + st * (cp * e1y + sp * e2y);
+  uz = c * fz + st * (cp * e1z + sp * e2z);
+  if (U(r) < 0.5f) { ux = -ux; uy = -uy; uz = -uz; }
+  normalize3(ux, uy, uz);
+}
+__device__ __forceinline__ void spheroid_normal(float x, float y, float z, float ux, float uy, float uz, float invb2,
+                                                float delta, float& nx, float& ny, float& nz) {
+  const float pu = x * ux + y * uy + z * uz;
+  nx = invb2 * x + delta * pu * ux; ny = invb2 * y + delta * pu * uy; nz = invb2 * z + delta * pu * uz;
+  normalize3(nx, ny, nz);
+}
+__device__ __forceinline__ bool spheroid_roots(float ox, float oy, float oz, float dx, float dy, float dz, float ux,
+                                               float uy, float uz, float invb2, float delta, float& t0, float& t1) {
+  const float du = dx * ux + dy * uy + dz * uz, ou = ox * ux + oy * uy + oz * uz;
+  const float A = invb2 * (dx * dx + dy * dy + dz * dz) + delta * du * du;
+  const float B = 2.0f * (invb2 * (ox * dx + oy * dy + oz * dz) + delta * ou * du);
+  const float Cc = invb2 * (ox * ox + oy * oy + oz * oz) + delta * ou * ou - 1.0f;
+  const float disc = B * B - 4.0f * A * Cc;
+  if (!(A > 0.0f) || disc < 0.0f) return false;
+  const float sd = sqrtf(fmaxf(0.0f, disc));
+  float r0 = (-B - sd) / (2.0f * A), r1 = (-B + sd) / (2.0f * A);
+  if (r0 > r1) { const float q = r0; r0 = r1; r1 = q; }
+  t0 = r0; t1 = r1;
+  return true;
+}
+// entry point uniform over the projected silhouette, CLARITAS.py:2745-2771
+__device__ bool spheroid_entry(Rng& r, float vx, float vy, float vz, float a, float b, float ux, float uy, float uz,
+                               float& x, float& y, float& z, float& gnx, float& gny, float& gnz) {
+  const float invb2 = 1.0f / (b * b), delta = 1.0f / (a * a) - invb2;
+  float e1x, e1y, e1z, e2x, e2y, e2z;
+  perpendicular_basis(vx, vy, vz, e1x, e1y, e1z, e2x, e2y, e2z);
+  const float du = dot3(ux, uy, uz, vx, vy, vz);
+  const float rproj2 = b * b + (a * a - b * b) * fmaxf(0.0f, 1.0f - du * du);
+  const float rbound = sqrtf(fmaxf(b * b, rproj2)), rmax = fmaxf(a, b);
+  for (int attempt = 0; attempt < 64; ++attempt) {
+    const float rr = rbound * sqrtf(U(r)), ph = 2.0f * kPiF * U(r);
+    const float ox = rr * (cosf(ph) * e1x + sinf(ph) * e2x) - 2.5f * rmax * vx;
+    const float oy = rr * (cosf(ph) * e1y + sinf(ph) * e2y) - 2.5f * rmax * vy;
+    const float oz = rr * (cosf(ph) * e1z + sinf(ph) * e2z) - 2.5f * rmax * vz;
+    float t0, t1;
+    if (!spheroid_roots(ox, oy, oz, vx, vy, vz, ux, uy, uz, invb2, delta, t0, t1)) continue;
+    const float t = (t0 > 1e-9f * rmax) ? t0 : ((t1 > 1e-9f * rmax) ? t1 : -1.0f);
+    if (t <= 0.0f) continue;
+    x = ox + t * vx; y = oy + t * vy; z = oz + t * vz;
+    spheroid_normal(x, y, z, ux, uy, uz, invb2, delta, gnx, gny, gnz);
+    if (dot3(vx, vy, vz, gnx, gny, gnz) < -1e-6f) return true;
+  }
+  return false;
+}
+__device__ __forceinline__ float spheroid_chord(float x, float y, float z, float vx, float vy, float vz, float ux,
+                                                float uy, float uz, float invb2, float delta, float scale) {
+  float t0, t1;
+  if (!spheroid_roots(x, y, z, vx, vy, vz, ux, uy, uz, invb2, delta, t0, t1)) return -1.0f;
+  const float eps = fmaxf(1e-5f * scale, 1e-12f);
+  float t = kInfF;
+  if (t0 > eps) t = t0;
+  if (t1 > eps && t1 < t) t = t1;
+  return t < kInfF * 0.5f ? t : -1.0f;
+}
+struct PCount {
+  int pr, per, pir, pabs;
+  uint32_t n_abs;
+};
+// One finite-particle encounter with smooth or rough surfaces: a Fresnel
+// sphere of the given radius whose centre is c (the entry normal g is the
+// sampled surface point's), or a volume-preserving spheroid a = r q^(2/3),
+// b = r q^(-1/3) about the local origin (direction only; the axis, the entry
+// point and its normal are sampled here).  The entry and every internal
+// encounter share one micro_interface site, so the walk is inlined once per
+// caller.  Returns 1 escaped, 2 absorbed, 0 failure (pf, pfs).
+// CLARITAS.py:2497-2599 (sphere), 2783-2841 (spheroid)
+template <bool kRough>
+__device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool spheroid, float n_m, float n_p,
+                                              float k_p, float radius, float gx, float gy, float gz, double cx,
+                                              double cy, double cz, float fx, float fy, float fz, float& x, float& y,
+                                              float& z, float& vx, float& vy, float& vz, PCount& pc, int& pf,
+                                              int& pfs) {
+  pf = PF_OK; pfs = SUB_NONE;
+  if (radius <= 0.0f) return 1;
+  float ux = 0.0f, uy = 0.0f, uz = 0.0f, invb2 = 0.0f, delta = 0.0f, scale = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  if (!kRough) spheroid = false;  // the morphology channel is reached only in the ROUGH variant
+  if (spheroid) {
+    const float q13 = powf(P.aspect, 1.0f / 3.0f), a = radius * q13 * q13, b = radius / q13;
+    if (!(a > 0.0f && b > 0.0f && isfinite(a) && isfinite(b))) { pf = PF_SPHEROID; pfs = SUB_SPHEROID_AXES; return 0; }
+    spheroid_axis(r, P.orientation, P.kappa, fx, fy, fz, ux, uy, uz);
+    if (!spheroid_entry(r, vx, vy, vz, a, b, ux, uy, uz, sx, sy, sz, gx, gy, gz)) {
+      pf = PF_SPHEROID; pfs = SUB_SPHEROID_ENTRY; return 0;
+    }
+    invb2 = 1.0f / (b * b); delta = 1.0f / (a * a) - invb2; scale = fmaxf(a, b);
+  }
+  const int pf_geo = spheroid ? PF_SPHEROID : PF_MACRO_GEOMETRY;
+  const double rr = (double)radius, rr2 = rr * rr;
+  double qx = 0.0, qy = 0.0, qz = 0.0;  // sphere: position relative to the centre (FP64)
+  const float alpha_abs = (P.wavelength > 0.0f && k_p > 0.0f) ? (4.0f * kPiF * k_p / P.wavelength) : 0.0f;
+  float bx = gx, by = gy, bz = gz;       // TOP normal of the current encounter
+  float nox = 0.0f, noy = 0.0f, noz = 0.0f;  // outward normal at an internal encounter
+  float n_top = n_m, k_top = 0.0f, n_bot = n_p, k_bot = k_p;
+  for (int stage = 0;; ++stage) {
+    bool top = true;
+    int fc = 0;
+    const int ms = micro_interface<kRough>(r, P.rough, bx, by, bz, n_top, k_top, n_bot, k_bot, vx, vy, vz, top, fc);
+    if (ms != MS_ESCAPE) { pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; pfs = fc; return 0; }
+    if (stage == 0) {
+      if (top) { pc.pr++; pc.per++; return 1; }
+      if (dot3(vx, vy, vz, gx, gy, gz) >= 0.0f) {
+        pf = pf_geo; pfs = spheroid ? SUB_SPHEROID_ENTRY : SUB_SPHERE_ENTRY_SIDE; return 0;
+      }
+      if (!spheroid) {
+        qx = rr * (double)gx; qy = rr * (double)gy; qz = rr * (double)gz;
+        const double qn0 = sqrt(qx * qx + qy * qy + qz * qz);
+        if (!(qn0 > 0.0) || !isfinite(qn0)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+        qx *= rr / qn0; qy *= rr / qn0; qz *= rr / qn0;
+      }
+      n_top = n_p; k_top = k_p; n_bot = n_m; k_bot = 0.0f;
+    } else {
+      if (!top) {
+        if (dot3(vx, vy, vz, nox, noy, noz) <= 0.0f) {
+          pf = pf_geo; pfs = spheroid ? SUB_SPHEROID_EXIT_SIDE : SUB_SPHERE_EXIT_SIDE; return 0;
+        }
+        return 1;
+      }
+      if (dot3(vx, vy, vz, nox, noy, noz) >= 0.0f) {
+        pf = pf_geo; pfs = spheroid ? SUB_SPHEROID_INTERNAL_SIDE : SUB_SPHERE_INTERNAL_SIDE; return 0;
+      }
+      pc.pr++; pc.pir++;
+      if (stage - 1 == P.max_pb) {
+        pf = PF_MACRO_BOUNCE; pfs = spheroid ? SUB_SPHEROID_BOUNCE_LIMIT : SUB_SPHERE_BOUNCE_LIMIT; return 0;
+      }
+    }
+    if (stage > P.max_pb) { pf = PF_OTHER; pfs = SUB_NONE; return 0; }
+    // chord of internal bounce `stage` to the next encounter
+    if (kRough && spheroid) {
+      const float chord = spheroid_chord(sx, sy, sz, vx, vy, vz, ux, uy, uz, invb2, delta, scale);
+      if (chord <= 0.0f) { pf = PF_SPHEROID; pfs = SUB_SPHEROID_CHORD; return 0; }
+      if (alpha_abs > 0.0f) {
+        pc.n_abs++;
+        const double dist = -log(Uopen(r)) / (double)alpha_abs;
+        if (dist < (double)chord) { pc.pabs++; return 2; }
+      }
+      sx += chord * vx; sy += chord * vy; sz += chord * vz;
+      spheroid_normal(sx, sy, sz, ux, uy, uz, invb2, delta, nox, noy, noz);
+    } else {
+      const double dvx = vx, dvy = vy, dvz = vz;
+      const double A = dvx * dvx + dvy * dvy + dvz * dvz, q2 = qx * qx + qy * qy + qz * qz;
+      if (!(A > 0.0) || !isfinite(A) || !isfinite(q2)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+      const double qnorm = sqrt(q2);
+      if (!(qnorm > 0.0) || !isfinite(qnorm)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+      const double qp = rr / qnorm;
+      qx *= qp; qy *= qp; qz *= qp;
+      const double B = qx * dvx + qy * dvy + qz * dvz;
+      if (!(B < 0.0) || !isfinite(B)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
+      const double chord = -2.0 * B / A;
+      if (!(chord > 0.0) || !isfinite(chord)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+      const double tqx = qx + chord * dvx, tqy = qy + chord * dvy, tqz = qz + chord * dvz;
+      const double arr = tqx * dvx + tqy * dvy + tqz * dvz, tq2 = tqx * tqx + tqy * tqy + tqz * tqz;
+      if (!(arr > 0.0) || !isfinite(arr) || !isfinite(tq2) || fabs(tq2 - rr2) > 512.0 * kDEps * fmax(rr2, 1.0e-30)) {
+        pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0;
+      }
+      if (alpha_abs > 0.0f) {
+        pc.n_abs++;
+        const double dist = -log(Uopen(r)) / (double)alpha_abs;
+        if (dist < chord) {
+          qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
+          x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
+          pc.pabs++;
+          return 2;
+        }
+      }
+      qx = tqx; qy = tqy; qz = tqz;
+      const double qn = sqrt(qx * qx + qy * qy + qz * qz);
+      if (!(qn > 0.0) || !isfinite(qn)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
+      const double pj = rr / qn;
+      qx *= pj; qy *= pj; qz *= pj;
+      x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
+      nox = (float)(qx / rr); noy = (float)(qy / rr); noz = (float)(qz / rr);
+      normalize3(nox, noy, noz);
+    }
+    bx = -nox; by = -noy; bz = -noz;
+  }
+}
+
 struct CondOut {
   float x, y, z, vx, vy, vz;
   int status, channel, failure, subcode;
@@ -1047,6 +1534,7 @@ struct CondOut {
   uint32_t trials, rejects, inacc;
 };
 // conditional finite-particle closure, CLARITAS.py:4479-4536
+template <bool kRough>
 __device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
                                      float vx, float vy, float vz, int max_trials) {
   CondOut h;
@@ -1079,18 +1567,45 @@ __device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r,
     if (c.channel == 1) {
       mie_scatter(r, P, pidx, c.vx, c.vy, c.vz);
       c.status = 1;
-    } else if (c.channel == 2) {
-      c.status = 0; c.failure = PF_SPHEROID;
     } else {
-      float gx, gy, gz;
-      sphere_sample(r, vx, vy, vz, gx, gy, gz);
-      const double dcx = (double)x - (double)radius * gx, dcy = (double)y - (double)radius * gy,
-                   dcz = (double)z - (double)radius * gz;
-      if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { inacc++; continue; }
-      int pf = 0, pfs = 0;
-      c.status = sphere_interaction(P, r, P.n_water, nr, ki, radius, gx, gy, gz, dcx, dcy, dcz, c.x, c.y,
-                                    c.z, c.vx, c.vy, c.vz, c.pc, pf, pfs);
-      c.failure = pf; c.subcode = pfs;
+      // channel 0: Fresnel sphere at a sampled accessible centre; channel 2:
+      // morphology-wave event (direction only), CLARITAS.py:4512-4531
+      const bool dir_only = (c.channel == 2);
+      bool sph = false;
+      float gx = 0.0f, gy = 0.0f, gz = 0.0f, fx = 1.0f, fy = 0.0f;
+      double dcx = 0.0, dcy = 0.0, dcz = 0.0;
+      int pre = -1;  // outcome settled before the event
+      if (dir_only) {
+        const float fr = sqrtf(x * x + y * y);
+        if (fr > 1e-12f) { fx = -y / fr; fy = x / fr; }
+        if (radius <= 0.0f) {
+          pre = 1;
+        } else if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) {
+          pre = 0; c.failure = PF_SPHEROID; c.subcode = SUB_SPHEROID_ASPECT;
+        } else if (fabsf(P.aspect - 1.0f) < 1e-7f) {
+          sphere_sample(r, vx, vy, vz, gx, gy, gz);
+          dcx = -(double)radius * (double)gx; dcy = -(double)radius * (double)gy; dcz = -(double)radius * (double)gz;
+        } else {
+          sph = true;
+        }
+      } else {
+        sphere_sample(r, vx, vy, vz, gx, gy, gz);
+        dcx = (double)x - (double)radius * gx; dcy = (double)y - (double)radius * gy;
+        dcz = (double)z - (double)radius * gz;
+        if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { inacc++; continue; }
+      }
+      if (pre >= 0) {
+        c.status = pre;
+      } else {
+        float px = dir_only ? 0.0f : c.x, py = dir_only ? 0.0f : c.y, pz = dir_only ? 0.0f : c.z;
+        float pvx = c.vx, pvy = c.vy, pvz = c.vz;
+        int pf = 0, pfs = 0;
+        c.status = particle_event<kRough>(P, r, sph, P.n_water, nr, ki, radius, gx, gy, gz, dcx, dcy, dcz, fx, fy, 0.0f, px,
+                                  py, pz, pvx, pvy, pvz, c.pc, pf, pfs);
+        if (!dir_only) { c.x = px; c.y = py; c.z = pz; }
+        if (!dir_only || sph || c.status == 1) { c.vx = pvx; c.vy = pvy; c.vz = pvz; }
+        c.failure = pf; c.subcode = pfs;
+      }
     }
     nabs += c.pc.n_abs;
     if (c.status != 1 && c.status != 2) {
@@ -1419,8 +1934,9 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
 // next particle event or boundary (water_flight), the particle event
 // (water_particle) and the boundary interaction (water_iface).  The wave
 // schedules each kind separately so divergent lanes do not serialise them.
-template <bool kPlain>
+template <int kKind>
 __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A) {
+  constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
   if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
   L.wstep++;
   A.steps++;
@@ -1470,8 +1986,9 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
 }
 
 // the particle event at the end of a flight (CLARITAS.py:4774-4981)
-template <bool kPlain>
+template <int kKind>
 __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A) {
+  constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN), kRough = (kKind == CLR_KERNEL_ROUGH);
   L.region = R_WATER;  // a terminate() below overrides
   const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
   const float MU_TOTAL = P.mu_geom + P.mu_wave;
@@ -1519,59 +2036,87 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     return;
   }
   if constexpr (!kPlain) {
+  // One conditional-closure call site and one finite-particle call site serve
+  // every branch below (each inlines the microsurface walk once).
   const bool cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
   int sel = -1;
   if (cond) sel = select_bin(L.r, P, use_wave, L.x, L.y, L.z, use_wave ? LW : LG);
+  int cbin = -1, ctrials = 0;
+  bool fallback = false;  // closure reached from an inaccessible geometric candidate
+  int pidx = -1;
+  bool sph = false, dir_only = false;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, fx = 1.0f, fy = 0.0f, rad = 0.0f;
+  double dcx = 0.0, dcy = 0.0, dcz = 0.0;
   if (cond && P.k_imag[sel] > 0.0f) {
-    CondOut h = conditional_event(P, L.r, sel, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials);
-    A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
-    if (h.status != 1 && h.status != 2) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
-    L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
-    L.pr += h.pc.pr;
-    if (h.pc.per) L.flags |= F_PER;
-    if (h.pc.pir) L.flags |= F_PIR;
-    if (h.pc.pabs) L.flags |= F_PABS;
-    if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
-    if (h.status == 2) terminate(L, 5);
-    return;
-  }
-  if (use_wave) {
+    cbin = sel; ctrials = P.cond_trials;
+  } else if (use_wave) {
     L.wic++;
-    const int pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
+    pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
     const bool coherent = (P.coherent >= 1.0f) || (U(L.r) < P.coherent);
-    if (!coherent) { terminate(L, 4, PF_SPHEROID); return; }
-    L.wcoh++;
-    mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
-    return;
+    if (coherent) {
+      L.wcoh++;
+      mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
+      return;
+    }
+    // morphology-wave event about the local azimuthal flow direction,
+    // CLARITAS.py:4862-4881, 2783-2799 (configurations with a coherent
+    // fraction below 1 run on the ROUGH variant)
+    if constexpr (!kRough) { terminate(L, 4, PF_SPHEROID); return; }
+    dir_only = true;
+    const float fr = sqrtf(L.x * L.x + L.y * L.y);
+    if (fr > 1e-12f) { fx = -L.y / fr; fy = L.x / fr; }
+    rad = P.radius[pidx];
+    if (rad <= 0.0f) return;
+    if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) { terminate(L, 4, PF_SPHEROID); return; }
+    if (fabsf(P.aspect - 1.0f) < 1e-7f) {
+      sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
+      dcx = -(double)rad * (double)gx; dcy = -(double)rad * (double)gy; dcz = -(double)rad * (double)gz;
+    } else {
+      sph = true;
+    }
+  } else {
+    L.gic++;
+    pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
+    rad = P.radius[pidx];
+    sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
+    dcx = (double)L.x - (double)rad * (double)gx;
+    dcy = (double)L.y - (double)rad * (double)gy;
+    dcz = (double)L.z - (double)rad * (double)gz;
+    const bool accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
+    if (!accessible && cond) {
+      A.inacc++;
+      cbin = pidx; ctrials = P.cond_trials - 1; fallback = true;
+    } else if (!accessible) {
+      L.ic--; L.gic--; A.n_null++;
+      return;
+    }
   }
-  L.gic++;
-  const int pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
-  const float rad = P.radius[pidx];
-  float gx, gy, gz;
-  sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
-  const double dcx = (double)L.x - (double)rad * (double)gx;
-  const double dcy = (double)L.y - (double)rad * (double)gy;
-  const double dcz = (double)L.z - (double)rad * (double)gz;
-  const bool accessible = sphere_accessible(P, dcx, dcy, dcz, rad);
-  if (!accessible && cond) {
-    A.inacc++;
-    CondOut h = conditional_event(P, L.r, pidx, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.cond_trials - 1);
+  if (cbin >= 0) {
+    CondOut h = conditional_event<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials);
     A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
-    if (h.status != 1) { terminate(L, 4, h.failure ? h.failure : PF_OTHER); return; }
+    if (fallback ? (h.status != 1) : (h.status != 1 && h.status != 2)) {
+      terminate(L, 4, h.failure ? h.failure : PF_OTHER);
+      return;
+    }
     L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
     L.pr += h.pc.pr;
     if (h.pc.per) L.flags |= F_PER;
     if (h.pc.pir) L.flags |= F_PIR;
     if (h.pc.pabs) L.flags |= F_PABS;
-    L.gic--;
+    if (fallback) L.gic--;
     if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+    if (!fallback && h.status == 2) terminate(L, 5);
     return;
   }
-  if (!accessible) { L.ic--; L.gic--; A.n_null++; return; }
   PCount pc{0, 0, 0, 0, 0u};
   int ppf = 0, ppfs = 0;
-  const int ps = sphere_interaction(P, L.r, P.n_water, P.n_real[pidx], P.k_imag[pidx], rad, gx, gy, gz,
-                                    dcx, dcy, dcz, L.x, L.y, L.z, L.vx, L.vy, L.vz, pc, ppf, ppfs);
+  float px = dir_only ? 0.0f : L.x, py = dir_only ? 0.0f : L.y, pz = dir_only ? 0.0f : L.z;
+  float pvx = L.vx, pvy = L.vy, pvz = L.vz;
+  const int ps = particle_event<kRough>(P, L.r, sph, P.n_water, P.n_real[pidx], P.k_imag[pidx], rad, gx, gy, gz, dcx, dcy,
+                                dcz, fx, fy, 0.0f, px, py, pz, pvx, pvy, pvz, pc, ppf, ppfs);
+  if (!dir_only) { L.x = px; L.y = py; L.z = pz; }
+  // a direction-only sphere keeps the incoming direction unless it escaped
+  if (!dir_only || sph || ps == 1) { L.vx = pvx; L.vy = pvy; L.vz = pvz; }
   L.pr += pc.pr;
   if (pc.per) L.flags |= F_PER;
   if (pc.pir) L.flags |= F_PIR;
@@ -1821,8 +2366,8 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 #ifndef CLR_GENERIC_WAVES_PER_SIMD
 #define CLR_GENERIC_WAVES_PER_SIMD 2
 #endif
-template <bool kPlain>
-__global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERIC_WAVES_PER_SIMD) void transport_kernel(TransportLaunch Lp,
+template <int kKind>
+__global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERIC_WAVES_PER_SIMD) void transport_kernel(TransportLaunch Lp,
                                                         const DevParams* __restrict__ params,
                                                         const WorkItem* __restrict__ items) {
   __shared__ LdsTally T;
@@ -1931,8 +2476,8 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
         }
       } else if (op == K_WATER) {
         if (alive && (L.region == R_WPART || L.region == R_WATER)) rng_topup(L.r);
-        if (alive && L.region == R_WPART) water_particle<kPlain>(L, P, A);
-        if (alive && L.region == R_WATER) water_flight<kPlain>(L, P, A);
+        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A);
+        if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
       } else if (op == K_ACRYLIC) {
@@ -1964,17 +2509,25 @@ __global__ __launch_bounds__(256, kPlain ? CLR_PLAIN_WAVES_PER_SIMD : CLR_GENERI
 }  // namespace
 
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s) {
-  if (L.plain)
-    hipLaunchKernelGGL(transport_kernel<true>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+  if (L.kind == CLR_KERNEL_PLAIN)
+    hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_PLAIN>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+  else if (L.kind == CLR_KERNEL_ROUGH)
+    hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_ROUGH>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   else
-    hipLaunchKernelGGL(transport_kernel<false>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+    hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_GENERIC>, dim3(L.grid), dim3(L.block), 0, s, L, L.params,
+                       L.items);
   return hipGetLastError();
 }
 
-int clr_transport_occupancy(int block, int plain) {
+int clr_transport_occupancy(int block, int kind) {
   int n = 0;
-  hipError_t e = plain ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<true>, block, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<false>, block, 0);
+  hipError_t e;
+  if (kind == CLR_KERNEL_PLAIN)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_PLAIN>, block, 0);
+  else if (kind == CLR_KERNEL_ROUGH)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_ROUGH>, block, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_GENERIC>, block, 0);
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }

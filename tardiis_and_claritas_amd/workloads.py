@@ -44,6 +44,19 @@ def config1(event_model='hybrid_mie_exact_rate', source='collimated_disk'):
     return kw, mat, 0.1, 1_000_000, SEED
 
 
+def config1_rough(event_model='hybrid_mie_exact_rate', conc=0.5, aspect=1.5, orientation='isotropic', kappa=0.0):
+    """Config-1 cell with the reference's default particle physics
+    (CLARITAS.py:5610-5614): Beckmann rough particle surfaces at 25 deg rms
+    slope (Smith microsurface walk), 100 nm wave rms height (coherent fraction
+    ~0.02 at 860 nm, so wave events are mostly spheroid-morphology events),
+    spheroid aspect 1.5 with an isotropic axis."""
+    kw, mat, _, n, seed = config1(event_model)
+    kw.update(particle_surface_rms_slope_deg=25.0, wave_surface_rms_height_m=100e-9,
+              wave_spheroid_aspect_ratio=aspect, wave_spheroid_orientation_model=orientation,
+              wave_spheroid_orientation_kappa=kappa)
+    return kw, mat, conc, n, seed
+
+
 def config2(conc_mg_l=1000.0, event_model='plain_mie', rouse_number=0.3, nr=64, nz=128):
     kw = base_kwargs(event_model)
     geo = cell_geometry(**SMALL_CELL)

@@ -91,12 +91,39 @@ def test_history_by_history_config1(gpu_available, oracle_lib, event_model):
         assert np.abs(Tg[k].sum(0) - To[k].sum(0)).sum() <= 0.002 * N, k
 
 
+@pytest.mark.parametrize('over', [dict(), dict(aspect=1.0), dict(orientation='local_tangential_axial_vmf', kappa=4.0),
+                                  dict(event_model='hybrid_mie_excess')])
+def test_history_by_history_rough(gpu_available, oracle_lib, over):
+    """The reference's default particle physics: Smith walk on rough spheres
+    and spheroids, incoherent wave channel.  Same Philox streams, same
+    outcome per history."""
+    _flush(f'history-by-history rough {over}')
+    kw, mat, conc, n, seed = workloads.config1_rough(**over)
+    m = _model(kw)
+    N = 20000
+    m.prepare(mat, conc)
+    Tg, rg, _ = m.run_tallies(N, seed, 0, n_records=N)
+    To, ro = oracle_lib.OracleModel(m, mat, conc).run(N, seed, 0, n_records=N)
+    same = _compare_records(rg, ro)
+    frac = same.mean()
+    assert frac >= 0.999, f'only {frac:.5f} of histories agree; first mismatch {np.nonzero(~same)[0][:5]}'
+    dv = np.abs(rg['vx'][same] - ro['vx'][same])
+    assert np.mean(dv <= 1e-5) >= 0.995, np.mean(dv <= 1e-5)
+    assert Tg['status'].sum() == N and To['status'].sum() == N
+    assert np.abs(Tg['det'].sum(0) - To['det'].sum(0)).sum() <= 0.002 * N
+    for k in ('sum_ic', 'sum_wic', 'sum_wcoh', 'sum_pr'):
+        i = __import__('tardiis_and_claritas_amd.abi', fromlist=['C_IDX']).C_IDX[k]
+        a, b = Tg['counters'][i], To['counters'][i]
+        assert abs(a - b) <= 0.002 * max(a, b) + 2, (k, a, b)
+
+
 def _sym(T):
     d = T['det'].sum(0)
     return 0.5 * (d[0] + d[1]), 0.5 * (d[2] + d[3])
 
 
-@pytest.mark.parametrize('builder', ['config1_hybrid', 'config1_plain', 'config2_plain_1000', 'config2_hybrid_316'])
+@pytest.mark.parametrize('builder', ['config1_hybrid', 'config1_plain', 'config2_plain_1000', 'config2_hybrid_316',
+                                     'config1_rough'])
 def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
     _flush(f'3-sigma tallies {builder}')
     if builder == 'config1_hybrid':
@@ -108,6 +135,9 @@ def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
     elif builder == 'config2_plain_1000':
         kw, mat, conc, n, seed = workloads.config2(1000.0, 'plain_mie')
         kw['aux_detector_angles_deg'] = (135.0,)
+        N = 200_000
+    elif builder == 'config1_rough':
+        kw, mat, conc, n, seed = workloads.config1_rough()
         N = 200_000
     else:
         kw, mat, conc, n, seed = workloads.config2(316.0, 'hybrid_mie_exact_rate')
@@ -123,14 +153,22 @@ def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
         assert np.all(z < 4.0), f'{builder}: max z {z.max():.2f} at channel {z.argmax()}'
     aw_g, aw_o = Tg['aux_window'].sum(0).sum(0), To['aux_window'].sum(0).sum(0)
     assert np.all(np.abs(aw_g - aw_o) <= 4.0 * np.sqrt(np.maximum(aw_g + aw_o, 1.0))), (aw_g, aw_o)
-    # the only failure the reference law produces here is the conditional
-    # closure's explicit retry exhaustion (code 15, ~1e-6 per history,
-    # CLARITAS.py:4534); every other code must stay at zero on both sides
-    for T in (Tg, To):
-        others = np.delete(T['failures'], 15)
-        assert others.sum() == 0, f'failures by code {T["failures"]}'
-    a, b = Tg['failures'][15], To['failures'][15]
-    assert abs(a - b) <= 4 * np.sqrt(a + b + 1)
+    if builder == 'config1_rough':
+        # rough walks may end in the reference's explicit numerical/watchdog or
+        # spheroid-geometry failures: rare on both sides, same rate per code
+        for code in range(1, len(Tg['failures'])):
+            a, b = Tg['failures'][code], To['failures'][code]
+            assert abs(a - b) <= 4 * np.sqrt(a + b + 1), f'failure code {code}: {a} vs {b}'
+        assert Tg['failures'].sum() <= 1e-3 * N
+    else:
+        # the only failure the reference law produces here is the conditional
+        # closure's explicit retry exhaustion (code 15, ~1e-6 per history,
+        # CLARITAS.py:4534); every other code must stay at zero on both sides
+        for T in (Tg, To):
+            others = np.delete(T['failures'], 15)
+            assert others.sum() == 0, f'failures by code {T["failures"]}'
+        a, b = Tg['failures'][15], To['failures'][15]
+        assert abs(a - b) <= 4 * np.sqrt(a + b + 1)
     for k in range(1, 9):
         a, b = Tg['status'][k], To['status'][k]
         assert abs(a - b) <= 4 * np.sqrt(max(a + b, 1)), f'status {k}: {a} vs {b}'

@@ -206,3 +206,39 @@ def test_walk_reflection_symmetric_about_plane_of_incidence():
     for sel in (top == 1, top == 0):
         y = out[sel, 1]
         assert abs(y.mean()) <= 4 * y.std() / math.sqrt(sel.sum())
+
+
+def _rough_model(**over):
+    from tardiis_and_claritas_amd import ClaritasForwardModel, workloads
+    kw, mat, conc, n, seed = workloads.config1_rough(**over)
+    return ClaritasForwardModel(**kw), mat, conc, seed
+
+
+@pytest.mark.parametrize('over', [dict(), dict(aspect=1.0), dict(orientation='local_tangential_axial_vmf', kappa=4.0),
+                                  dict(event_model='hybrid_mie_excess')])
+def test_rough_transport_closure_and_determinism(oracle_lib, over):
+    m, mat, conc, seed = _rough_model(**over)
+    om = oracle_lib.OracleModel(m, mat, conc)
+    N = 20000
+    a, _ = om.run(N, seed, 0, threads=1)
+    b, _ = om.run(N, seed, 0, threads=4)
+    for k in ('det', 'status', 'counters', 'ic_hist', 'failures'):
+        assert np.array_equal(a[k], b[k]), k
+    assert a['status'].sum() == N
+    ci = __import__('tardiis_and_claritas_amd.abi', fromlist=['C_IDX']).C_IDX
+    c = a['counters']
+    # particle events happened, most wave events are morphology (coherent fraction ~0.02)
+    assert c[ci['sum_ic']] > 0.2 * N
+    assert c[ci['sum_wic']] - c[ci['sum_wcoh']] > 10 * c[ci['sum_wcoh']]
+    # failures (numerical walk failures, watchdogs, spheroid geometry) stay rare
+    assert a['failures'].sum() <= 1e-3 * N, a['failures']
+
+
+def test_rough_transport_offset_additivity(oracle_lib):
+    m, mat, conc, seed = _rough_model()
+    om = oracle_lib.OracleModel(m, mat, conc)
+    x, _ = om.run(6000, seed, 0)
+    y, _ = om.run(6000, seed, 6000)
+    z, _ = om.run(12000, seed, 0)
+    assert np.array_equal(x['status'] + y['status'], z['status'])
+    assert np.array_equal(x['counters'] + y['counters'], z['counters'])
