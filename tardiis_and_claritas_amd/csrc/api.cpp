@@ -365,9 +365,18 @@ void fill_params(Slot& s, const clr_config& g) {
 }
 
 int ensure_slots(clr_ctx* c, int need) {
+  const size_t before = c->slots.size();
   while ((int)c->slots.size() < need) c->slots.push_back(new Slot());
+  if (c->slots.size() == before && c->params.p) return 0;
+  // DevBuf::ensure reallocates without keeping the contents, so after growing
+  // the parameter array every prepared slot's parameters are uploaded again
+  // from their host copies (the tallies are cleared at every launch).
   HIPCHK(c, c->params.ensure(sizeof(DevParams) * c->slots.size()));
   HIPCHK(c, c->tallies.ensure(sizeof(clr_tallies) * c->slots.size()));
+  for (size_t k = 0; k < before; ++k)
+    if (c->slots[k]->ready)
+      HIPCHK(c, hipMemcpy(c->params.as<DevParams>() + k, &c->slots[k]->params, sizeof(DevParams),
+                          hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -763,6 +772,7 @@ int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_la
   *out = acc[0];
   out->n_groups = G;
   group_sizes(n, G, out->group_histories);
+  out->counters[CLR_C_HISTORIES] = n;  // launched histories (the device counts per-event work only)
   if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
   return 0;
 }
@@ -793,6 +803,7 @@ int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_l
     out[i] = host[k];
     out[i].n_groups = c->slots[k]->cfg.n_groups;
     group_sizes(bi[i].n_histories, out[i].n_groups, out[i].group_histories);
+    out[i].counters[CLR_C_HISTORIES] = bi[i].n_histories;
   }
   return 0;
 }

@@ -193,14 +193,18 @@ def test_batch_equals_single(gpu_available):
     exactly (same Philox streams, same histories)."""
     _flush('batch vs single')
     from tardiis_and_claritas_amd.batched import BatchedRunner
-    p = workloads.config3_params(6)
+    # 24 slots prepared one by one on a fresh context: the device parameter
+    # array is reallocated as it grows, and every slot must keep its own
+    p = workloads.config3_params(24)
     runner = BatchedRunner()
-    specs = [workloads.config3_item(p, i) for i in range(6)]
-    budgets = [20000 + 1000 * i for i in range(6)]
+    specs = [workloads.config3_item(p, i) for i in range(24)]
+    budgets = [6000 + 500 * i for i in range(24)]
     res = runner.run(specs, budgets, seed=77)
     for i, (kw, mat, conc) in enumerate(specs):
         m = _model(kw)
         m.prepare(mat, conc)
         T, _, _ = m.run_tallies(budgets[i], 77 + i, 0)
         assert np.array_equal(T['det'].sum(0), res[i]['det'].sum(0)), f'config {i}'
-        assert np.array_equal(T['status'], res[i]['status'])
+        assert np.array_equal(T['status'], res[i]['status']), f'config {i}'
+        assert np.array_equal(T['counters'], res[i]['counters']), f'config {i}'
+        assert res[i]['counters'][__import__('tardiis_and_claritas_amd.abi', fromlist=['C_IDX']).C_IDX['n_flight']] > 0
