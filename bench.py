@@ -127,6 +127,7 @@ def main():
     ap.add_argument('--blocks-per-cu', type=int, default=0)
     ap.add_argument('--chunk', type=int, default=0)
     ap.add_argument('--depth', type=int, default=2, help='launches in flight (separate HIP streams)')
+    ap.add_argument('--queue-order', type=int, default=0, help='0 as given, 1 most expensive configuration first')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()
@@ -143,7 +144,8 @@ def main():
 
     H = int(args.histories)
     seed = workloads.SEED
-    launch = {'block': args.block, 'blocks_per_cu': args.blocks_per_cu, 'chunk': args.chunk}
+    launch = {'block': args.block, 'blocks_per_cu': args.blocks_per_cu, 'chunk': args.chunk,
+              'queue_order': args.queue_order}
     specs = []
     for c in workloads.CONFIG2_CONCENTRATIONS_MG_L:
         kw, mat, conc, _, _ = workloads.config2(c, 'plain_mie')

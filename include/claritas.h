@@ -259,8 +259,8 @@ typedef struct clr_batch_item {
 typedef struct clr_launch {
   int32_t block;                /* threads per workgroup: 64..1024 */
   int32_t blocks_per_cu;        /* resident workgroups per CU (grid sizing) */
-  int32_t chunk;                /* histories per work-queue item */
-  int32_t pad_;
+  int32_t chunk;                /* histories per work-queue item (0: sized by the queued work) */
+  int32_t queue_order;          /* batches: 0 as given, 1 most expensive configuration first */
 } clr_launch;
 
 struct clr_ctx;

@@ -167,7 +167,7 @@ class BatchItem(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [('block', I32), ('blocks_per_cu', I32), ('chunk', I32), ('pad_', I32)]
+    _fields_ = [('block', I32), ('blocks_per_cu', I32), ('chunk', I32), ('queue_order', I32)]
 
 
 class MlpShape(C.Structure):

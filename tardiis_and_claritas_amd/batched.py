@@ -48,7 +48,8 @@ class BatchedRunner:
         lp = None
         if self.launch:
             l = abi.Launch()
-            l.block, l.blocks_per_cu, l.chunk = (int(self.launch.get(k, 0)) for k in ('block', 'blocks_per_cu', 'chunk'))
+            l.block, l.blocks_per_cu, l.chunk, l.queue_order = (
+                int(self.launch.get(k, 0)) for k in ('block', 'blocks_per_cu', 'chunk', 'queue_order'))
             lp = C.byref(l)
         self.ctx.check(self.ctx.lib.clr_run_batch(self.ctx.h, items, n, lp, out), 'clr_run_batch')
         self.last_ms = float(self.ctx.lib.clr_last_kernel_ms(self.ctx.h))
@@ -77,7 +78,8 @@ class Autotuner:
     """Launch-shape autotuner for the persistent transport kernel.
 
     Candidates are (threads per workgroup, resident workgroups per CU,
-    histories per work item; 0 = guided item sizes).  Each is timed on the
+    histories per work item, queue order; chunk 0 = item sizes guided by the
+    queued work, order 1 = most expensive configuration first).  Each is timed on the
     prepared batch with the real budgets scaled down to a sample large enough
     that the launch is not dominated by its fixed costs and end-of-launch tail
     (the same configurations and the same mix, so divergence is
@@ -86,12 +88,12 @@ class Autotuner:
 
     def __init__(self, candidates=None):
         self.candidates = candidates or [
-            dict(block=256, blocks_per_cu=0, chunk=0),      # library default: guided item sizes
-            dict(block=192, blocks_per_cu=4, chunk=0),
-            dict(block=128, blocks_per_cu=6, chunk=0),
-            dict(block=256, blocks_per_cu=2, chunk=0),
-            dict(block=256, blocks_per_cu=0, chunk=4096),
-            dict(block=256, blocks_per_cu=0, chunk=16384),
+            dict(block=256, blocks_per_cu=0, chunk=0, queue_order=0),   # library default
+            dict(block=256, blocks_per_cu=0, chunk=0, queue_order=1),
+            dict(block=192, blocks_per_cu=4, chunk=0, queue_order=1),
+            dict(block=128, blocks_per_cu=6, chunk=0, queue_order=1),
+            dict(block=256, blocks_per_cu=0, chunk=4096, queue_order=1),
+            dict(block=256, blocks_per_cu=0, chunk=16384, queue_order=0),
         ]
         self.cache = {}
         self.last = []

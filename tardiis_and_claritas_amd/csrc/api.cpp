@@ -428,19 +428,23 @@ RunShape pick_shape(clr_ctx* c, const clr_launch* launch, uint64_t total, int ki
   return r;
 }
 
-// Builds the device work queue in the order items are taken.
+// Builds the device work queue in the order items are taken.  Item sizes are
+// guided by the estimated work queued behind them (histories times a
+// per-history cost), so the items handed out last are small in work, not just
+// in histories.
 struct QueueBuilder {
   const RunShape& sh;
-  uint64_t remaining;  // histories queued from the next item on
+  double remaining;  // estimated work queued from the next item on
   std::vector<WorkItem> items;
-  QueueBuilder(const RunShape& s, uint64_t total) : sh(s), remaining(total) {}
-  // histories [start, start + n) of a configuration whose run numbers n_total
-  void add(int cfg, uint64_t start, uint64_t n, uint64_t n_total, uint64_t seed, uint64_t offset) {
+  QueueBuilder(const RunShape& s, double total_work) : sh(s), remaining(total_work) {}
+  // histories [start, start + n) of a configuration whose run numbers n_total;
+  // `cost` is its estimated work per history
+  void add(int cfg, uint64_t start, uint64_t n, uint64_t n_total, uint64_t seed, uint64_t offset, double cost = 1.0) {
     for (uint64_t s = 0; s < n;) {
       uint64_t c = (uint64_t)sh.chunk;
       if (!sh.fixed) {
-        const uint64_t g = remaining / (kGuidedSplit * (uint64_t)sh.grid);
-        c = std::min<uint64_t>(c, std::max<uint64_t>((uint64_t)sh.min_chunk, g));
+        const double g = remaining / ((double)kGuidedSplit * (double)sh.grid * cost);
+        c = std::min<uint64_t>(c, std::max<uint64_t>((uint64_t)sh.min_chunk, (uint64_t)std::min(g, 1e18)));
         c = std::max<uint64_t>((uint64_t)sh.block, (c / sh.block) * sh.block);
       }
       c = std::min<uint64_t>(c, n - s);
@@ -453,10 +457,18 @@ struct QueueBuilder {
       w.offset = offset;
       items.push_back(w);
       s += c;
-      remaining -= std::min(remaining, c);
+      remaining = std::max(0.0, remaining - cost * (double)c);
     }
   }
 };
+
+// Estimated work per history of a prepared configuration, in flights: a few
+// for launch, walls and exit, plus the majorant collisions (real and null)
+// along a path of about one cell diameter.
+double history_cost(const Slot& s) {
+  const double tau = (double)s.params.mu_track * 2.0 * (double)s.params.rin;
+  return 4.0 + (std::isfinite(tau) && tau > 0.0 ? tau : 0.0);
+}
 
 // kernel variant that serves every configuration of a launch (CLR_KERNEL_*)
 int kernel_kind(clr_ctx* c, const std::vector<int>& used) {
@@ -754,7 +766,7 @@ int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_la
     const uint64_t m = std::min(per_launch, n - done);
     // histories of this launch keep their global group by numbering them
     // within the whole run: items carry start offsets into [0, n)
-    QueueBuilder q(sh, m);
+    QueueBuilder q(sh, (double)m);
     q.add(0, done, m, n, seed, offset);
     if (int e = launch_items(c, q.items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0)) return e;
     total_ms += c->last_ms;
@@ -791,9 +803,21 @@ int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_l
     total += bi[i].n_histories;
   }
   const RunShape sh = pick_shape(c, launch, total, kernel_kind(c, used));
-  QueueBuilder q(sh, total);
-  for (int i = 0; i < n_items; ++i)
-    q.add(bi[i].config, 0, bi[i].n_histories, bi[i].n_histories, bi[i].seed, bi[i].history_offset);
+  // caller's order, or most expensive configurations first (longest
+  // processing time first) on request
+  std::vector<int> order(n_items);
+  std::vector<double> cost(n_items);
+  double work = 0.0;
+  for (int i = 0; i < n_items; ++i) {
+    order[i] = i;
+    cost[i] = history_cost(*c->slots[bi[i].config]);
+    work += cost[i] * (double)bi[i].n_histories;
+  }
+  if (launch && launch->queue_order == 1)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  QueueBuilder q(sh, work);
+  for (int i : order)
+    q.add(bi[i].config, 0, bi[i].n_histories, bi[i].n_histories, bi[i].seed, bi[i].history_offset, cost[i]);
   if (int e = launch_items(c, q.items, sh, true, nullptr, 0)) return e;
   const int n_slots = (int)c->slots.size();
   std::vector<clr_tallies> host(n_slots);
