@@ -59,6 +59,7 @@ struct Slot {
   DevBuf theta_guide, gguide, wguide;  // inverse-CDF guide tables
   size_t field_len = 0;
   DevParams params{};
+  std::vector<float> slab_max;  // field maximum per majorant slab (shared fields)
   std::vector<double> mie_key;  // optical inputs of the Mie tables held by this slot
   double mu_scalars_h[8]{};
   double majorants_h[4]{};
@@ -352,6 +353,19 @@ void fill_params(Slot& s, const clr_config& g) {
   P.sum_trace_w = sums[1];
   P.sum_g_f = (float)sums[0];
   P.sum_w_f = (float)sums[1];
+  // piecewise majorant over z slabs: mu_track's formula with each slab's maximum
+  P.n_slabs = 0;
+  P.slab_z0 = P.zmin;
+  P.slab_w = P.slab_inv_w = 0.0f;
+  for (int j = 0; j < CLR_MAJ_SLABS; ++j) P.slab_mu[j] = P.mu_track;
+  if (!s.slab_max.empty()) {
+    const int S = (int)s.slab_max.size();
+    P.n_slabs = S;
+    P.slab_w = (float)((g.field_zmax - g.zmin) / S);
+    P.slab_inv_w = (float)(S / (g.field_zmax - g.zmin));
+    for (int j = 0; j < S; ++j)
+      P.slab_mu[j] = (float)(sums[0] * (double)s.slab_max[j]) + (float)(sums[1] * (double)s.slab_max[j]);
+  }
   // inverse cell sizes of the shared fields, in float as the oracle computes them
   P.field_inv0 = P.field_inv1 = P.field_inv2 = 0.0f;
   if (g.sediment_model == CLR_SED_RZ_FIELD && g.field_n0 > 0 && g.field_n1 > 0) {
@@ -468,6 +482,35 @@ struct QueueBuilder {
 double history_cost(const Slot& s) {
   const double tau = (double)s.params.mu_track * 2.0 * (double)s.params.rin;
   return 4.0 + (std::isfinite(tau) && tau > 0.0 ? tau : 0.0);
+}
+
+// Field maximum over each z slab of the piecewise-constant Woodcock majorant
+// (shared r-z and x-y-z fields).  Slab j covers [zmin + j w, zmin + (j+1) w),
+// w = (zmax - zmin) / S, and the end slabs extend to the clamped field edges.
+// The interpolation at z uses the z nodes floor(u) and floor(u) + 1 with
+// u = (z - zmin) nz / (zmax - zmin) - 1/2; one extra node on each side keeps
+// the bound safe against rounding of u.  Each slab's maximum is over every
+// r (x, y) node, so it bounds the interpolated field anywhere in the slab.
+void slab_maxima(const clr_config& g, const float* field, size_t len, std::vector<float>& out) {
+  out.clear();
+  const bool rz = g.sediment_model == CLR_SED_RZ_FIELD, xyz = g.sediment_model == CLR_SED_XYZ_FIELD;
+  if (!(rz || xyz) || !field) return;
+  const int nz = rz ? g.field_n1 : g.field_n2;
+  const size_t planes = rz ? (size_t)g.field_n0 : (size_t)g.field_n0 * g.field_n1;
+  if (nz < 2 || planes == 0 || len < planes * nz || !(g.field_zmax > g.zmin)) return;
+  const int S = std::min(CLR_MAJ_SLABS, nz);
+  out.assign(S, 0.0f);
+  for (int j = 0; j < S; ++j) {
+    int k0 = (int)std::floor((double)j * nz / S - 0.5) - 1, k1 = (int)std::floor((double)(j + 1) * nz / S - 0.5) + 2;
+    if (j == 0) k0 = 0;
+    if (j == S - 1) k1 = nz - 1;
+    k0 = std::max(k0, 0);
+    k1 = std::min(k1, nz - 1);
+    float m = 0.0f;
+    for (size_t pl = 0; pl < planes; ++pl)
+      for (int k = k0; k <= k1; ++k) m = std::max(m, field[pl * nz + k]);
+    out[j] = m;
+  }
 }
 
 // kernel variant that serves every configuration of a launch (CLR_KERNEL_*)
@@ -672,6 +715,7 @@ int clr_prepare(clr_ctx* c, int slot, const clr_config* g, const clr_particles* 
     HIPCHK(c, hipMemcpyAsync(s.field.p, field, sizeof(float) * field_len, hipMemcpyHostToDevice, c->stream));
     s.field_len = field_len;
   }
+  slab_maxima(*g, field, field_len, s.slab_max);
   HIPCHK(c, c->scratch_u32.ensure(sizeof(unsigned int) * (size_t)(n + 2)));
   HIPCHK(c, hipMemsetAsync(c->scratch_u32.p, 0, sizeof(unsigned int) * (size_t)(n + 2), c->stream));
   int* eflag = c->scratch_u32.as<int>() + n + 1;

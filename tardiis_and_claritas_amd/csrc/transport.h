@@ -9,6 +9,8 @@
 // Everything the transport kernel reads for one configuration.  Scalars are
 // stored as the float32 values the reference passes to trace_kernel
 // (CLARITAS.py:7642-7760); tables live in device memory.
+constexpr int CLR_MAJ_SLABS = 32;
+
 struct DevParams {
   float n_water, n_acrylic, n_air, n_bottom_ext;
   float rin, rout, zmin, zmean, zwall_top, zlow;
@@ -24,6 +26,12 @@ struct DevParams {
   int fs_model, src_model, event_model, sed_model;
   int max_pb, max_iter, cond_trials, n_aux;
   float rough, aspect, kappa;     // particle Beckmann alpha, spheroid a/b, axis-law kappa
+  // shared FP32 fields: piecewise-constant majorant over CLR_MAJ_SLABS z slabs
+  // [slab_z0 + j w, slab_z0 + (j+1) w) (the end slabs are open), n_slabs = 0
+  // when the single majorant mu_track is used
+  float slab_mu[CLR_MAJ_SLABS];
+  float slab_z0, slab_w, slab_inv_w;
+  int n_slabs;
   int orientation;                // spheroid axis law (0 isotropic, 1 local-tangential vMF)
   int n_groups, n_bins, n_theta, n_radial;
   int fn0, fn1, fn2, pad0;

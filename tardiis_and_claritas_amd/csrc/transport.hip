@@ -1934,6 +1934,36 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
 // next particle event or boundary (water_flight), the particle event
 // (water_particle) and the boundary interaction (water_iface).  The wave
 // schedules each kind separately so divergent lanes do not serialise them.
+// majorant slab of height z (end slabs open)
+__device__ __forceinline__ int slab_of(const DevParams& P, float z) {
+  const int j = (int)floorf((z - P.slab_z0) * P.slab_inv_w);
+  return min(max(j, 0), P.n_slabs - 1);
+}
+// Free path from height z along direction cosine vz under the piecewise-
+// constant majorant of the z slabs: the optical depth -log u is spent slab by
+// slab (delta tracking with a spatially varying majorant; the exponential is
+// memoryless at slab faces, so the collision density is exact).  Returns
+// kInfF when the path reaches `limit` first.
+__device__ __forceinline__ float slab_free_path(const DevParams& P, float z, float vz, float u, float limit) {
+  float tau = -logf(fmaxf(u, 1e-12f));
+  int j = slab_of(P, z);
+  const int step = (vz > 0.0f) ? 1 : -1;
+  float t = 0.0f;
+  for (int n = 0; n < P.n_slabs; ++n) {
+    float t_exit = kInfF;
+    if (vz > 1e-14f && j < P.n_slabs - 1) t_exit = (P.slab_z0 + (float)(j + 1) * P.slab_w - z) / vz;
+    else if (vz < -1e-14f && j > 0) t_exit = (P.slab_z0 + (float)j * P.slab_w - z) / vz;
+    t_exit = fmaxf(t_exit, t);
+    const float mu = P.slab_mu[j];
+    if (mu > 0.0f && tau < mu * (t_exit - t)) return t + tau / mu;
+    if (t_exit >= limit) return kInfF;
+    tau -= mu * (t_exit - t);
+    t = t_exit;
+    j += step;
+  }
+  return kInfF;
+}
+
 template <int kKind>
 __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
@@ -1962,8 +1992,9 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   // segment up to min(free path, tc) stays below its lowest point: then the
   // surface can neither come first nor change the outcome of the flight.
   const bool early = (tside > 0.0f && tc < 0.5f * kInfF && MU_TRACK > 0.0f);
+  const bool slabs = P.n_slabs > 0;
   float freep = kInfF;
-  if (early) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
+  if (early) freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tc) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   {
     const float d = fminf(freep, tc);
     bool skip = false;
@@ -1975,7 +2006,8 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   const bool axial_bot = (tbot < ttop && tbot < tside);
   const float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
   if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY); return; }
-  if (!early && tbound > 0.0f && MU_TRACK > 0.0f) freep = -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
+  if (!early && tbound > 0.0f && MU_TRACK > 0.0f)
+    freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tbound) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
   const float travel = do_particle ? freep : tbound;
   A.n_flight++;
@@ -2019,9 +2051,11 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     }
     LT = LG + LW;
     if (!(LT > 0.0f)) { A.n_null++; return; }
-    const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
-    if (LT > MU_TRACK + slack) { terminate(L, 4, PF_MAJORANT); return; }
-    const float pacc = fminf(LT / fmaxf(MU_TRACK, 1e-30f), 1.0f);
+    // the majorant the collision was sampled with: the slab's, or mu_track
+    const float MU_LOC = (P.n_slabs > 0) ? P.slab_mu[slab_of(P, L.z)] : MU_TRACK;
+    const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_LOC), 1.0f);
+    if (LT > MU_LOC + slack) { terminate(L, 4, PF_MAJORANT); return; }
+    const float pacc = fminf(LT / fmaxf(MU_LOC, 1e-30f), 1.0f);
     if (U(L.r) >= pacc) { A.n_null++; return; }
   }
   L.ic++;

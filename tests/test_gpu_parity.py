@@ -123,7 +123,7 @@ def _sym(T):
 
 
 @pytest.mark.parametrize('builder', ['config1_hybrid', 'config1_plain', 'config2_plain_1000', 'config2_hybrid_316',
-                                     'config1_rough'])
+                                     'config1_rough', 'config2_plain_5000'])
 def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
     _flush(f'3-sigma tallies {builder}')
     if builder == 'config1_hybrid':
@@ -134,6 +134,12 @@ def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
         N = 400_000
     elif builder == 'config2_plain_1000':
         kw, mat, conc, n, seed = workloads.config2(1000.0, 'plain_mie')
+        kw['aux_detector_angles_deg'] = (135.0,)
+        N = 200_000
+    elif builder == 'config2_plain_5000':
+        # shared Rouse field at the sweep's densest point: the device tracks
+        # with a piecewise (z-slab) majorant, the oracle with the single one
+        kw, mat, conc, n, seed = workloads.config2(5000.0, 'plain_mie')
         kw['aux_detector_angles_deg'] = (135.0,)
         N = 200_000
     elif builder == 'config1_rough':
