@@ -15,7 +15,8 @@
  *  - Every entry point returns 0 on success.  Positive codes reuse the
  *    reference's precompute error numbering (1 Mie order cap, 2 invalid Mie
  *    input, 3 phase CDF failure, 4/5 transport precompute, 31-35 sediment,
- *    see CLARITAS/CLARITAS.py:3505,3515,3590,3868,3877,3958-4037);
+ *    41 meridional mass normalisation, see CLARITAS/CLARITAS.py:3505,3515,
+ *    3590,3868,3877,3958-4037,4180);
  *    negative codes are API/HIP errors (CLR_E_*).  clr_last_error() gives text.
  *  - The library owns every device buffer inside a clr_ctx; the caller owns
  *    the host structs and output arrays it passes in.
@@ -33,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 3
+#define CLR_ABI_VERSION 4
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -50,6 +51,7 @@ extern "C" {
 #define CLR_E_STATE    (-4)
 #define CLR_E_RANGE    (-5)
 #define CLR_E_UNSUPPORTED (-6)
+#define CLR_E_NOT_CONVERGED (-7) /* meridional solve ended above its tolerance */
 
 /* ---- model codes (values identical to CLARITAS.py:5543-5575) ---- */
 /* event law, CLARITAS.py:5543-5550; CLR_EVENT_PLAIN_MIE has no reference code */
@@ -263,13 +265,46 @@ typedef struct clr_launch {
   int32_t queue_order;          /* batches: 0 as given, 1 most expensive configuration first */
 } clr_launch;
 
+/* ---- meridional advection-diffusion solve: the producer of the per-bin
+ * r-z grid of CLR_SED_GRID_PER_BIN (the reference's sediment model
+ * 'schiller_naumann_meridional_advection_diffusion', code 3).  Parameters
+ * and limits follow CLARITAS.py:5625-5631,5873-5887. ---- */
+typedef struct clr_meridional {
+  double circulation_ratio;     /* chi in [0, 0.5]: peak meridional speed = chi * max v_theta */
+  double tolerance;             /* max relative residual, in (0, 1e-2) */
+  double sor;                   /* over-relaxation factor, in [1, 1.95) */
+  int32_t nr, nz;               /* cells in r and z, >= 16 and >= 32 */
+  int32_t max_iterations;       /* >= 250 */
+  int32_t check_interval;       /* iterations between residual checks, >= 10 */
+} clr_meridional;
+
+/* Outputs of clr_meridional_solve; every array is caller-owned and may be
+ * NULL (skipped).  Cell (i, k) has centre r = (i + 1/2) R/nr,
+ * z = zmin + (k + 1/2) (grid_zmax - zmin)/nz; arrays are [bin][i][k],
+ * k fastest. */
+typedef struct clr_meridional_out {
+  double* grid;                 /* [n][nr][nz] density scale; cells above the surface hold the column's top value */
+  double* residual;             /* [n] max relative residual at the last check */
+  double* scale_max;            /* [n] max over fluid cells (sediment_max_density_scale) */
+  double* scale_min;            /* [n] min over fluid cells */
+  double* point_scales;         /* [n][4] at the sensor centre, sensor wall (0.95 R), bottom centre, bottom wall */
+  uint8_t* active;              /* [nr][nz] 1 for fluid cells (centre below the free surface) */
+  double* coeff;                /* [5][n][nr][nz] finite-volume coefficients AP, AE, AW, AN, AS */
+  double grid_zmax;             /* top of the grid = free-surface height at the wall (m) */
+  double vtheta_peak;           /* maximum azimuthal speed of the vortex (m/s) */
+  double peak_speed;            /* chi * vtheta_peak (m/s) */
+  int32_t iterations;           /* SOR iterations run (a multiple of check_interval, or max_iterations) */
+  int32_t pad_;
+} clr_meridional_out;
+
 struct clr_ctx;
 typedef struct clr_ctx clr_ctx;
 
 /* ABI identification and struct sizes (arity test, cf. the reference's
  * launch-arity check validate_claritas.py:972-1001). id: 0 config,
  * 1 particles, 2 sediment, 3 mie_out, 4 tables_out, 5 tallies, 6 record,
- * 7 batch_item, 8 launch. Returns 0 for an unknown id. */
+ * 7 batch_item, 8 launch, 9 meridional, 10 meridional_out. Returns 0 for
+ * an unknown id. */
 int clr_abi_version(void);
 size_t clr_sizeof(int struct_id);
 
@@ -331,6 +366,24 @@ int clr_time_run(clr_ctx* ctx, uint64_t n, uint64_t seed,
 /* Average duration (ms) of the last transport-kernel launch, measured with
  * HIP events on the stream the kernel runs on. */
 double clr_last_kernel_ms(clr_ctx* ctx);
+
+/* Stationary meridional advection-diffusion sediment field, one r-z grid
+ * per bin of `p` (only p->n and p->diameter are read), on the free surface
+ * of `cfg` (parabolic or Scully vortex with vortex_delta_h > 0).  Upwind
+ * finite volumes with no-flux walls, red-black SOR with every bin's mass
+ * sum G (re^2 - rw^2)/2 dz over fluid cells renormalised to
+ * R^2 (zmean - zmin)/2 (mean scale 1) every 20 iterations and at each
+ * residual check; the solve stops at the first check at or after
+ * min(1000, max_iterations) iterations whose max residual over bins is
+ * <= tolerance.  Replaces meridional_active_mask_kernel ..
+ * meridional_finalize_kernel (CLARITAS.py:4091-4215) and their driver loop
+ * (CLARITAS.py:6451-6725).  Returns CLR_E_NOT_CONVERGED (outputs filled)
+ * when the last check is above the tolerance, 41 when a bin's mass is not
+ * positive and finite.  The grid feeds clr_prepare as the CLR_SED_GRID_PER_BIN
+ * field (converted to FP32) with field_zmax = grid_zmax. */
+int clr_meridional_solve(clr_ctx* ctx, const clr_config* cfg, const clr_particles* p,
+                         const clr_sediment* sed, const clr_meridional* mp,
+                         clr_meridional_out* out);
 
 /* ---- surrogate: ensemble of MLPs, BF16 weights, FP32 accumulate on MFMA.
  * No reference counterpart.  Layout: n_members networks of

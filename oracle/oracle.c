@@ -619,27 +619,17 @@ static float field_max(const float* f, size_t n) {
   return mx;
 }
 
-static int sediment_precompute(orc_model* m, const clr_sediment* s) {
-  int n = m->n, tm = m->cfg.sediment_model, fsm = m->cfg.free_surface_model;
-  int nrad = m->nrad;
-  double R = m->cfg.rin, ZMIN = m->cfg.zmin, ZMEAN = m->cfg.zmean;
-  double DH = m->cfg.vortex_delta_h, CORE = m->cfg.vortex_core_radius;
-  for (int i = 0; i < n; ++i) {
-    m->sedA[i] = 0.0; m->sedB[i] = 0.0; m->sednorm[i] = 1.0; m->sedws[i] = 0.0;
-    for (int j = 0; j < nrad; ++j) m->radlog[(size_t)i * nrad + j] = 0.0;
-  }
-  if (tm == CLR_SED_UNIFORM) {
-    m->maj[0] = m->sum_trace_g; m->maj[1] = m->sum_trace_w; m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
-    return 0;
-  }
-  if (tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD) {
-    double mx = (double)field_max(m->field, m->field_len);
-    m->maj[0] = m->sum_trace_g * mx; m->maj[1] = m->sum_trace_w * mx;
-    m->sum_g_f = (float)m->sum_trace_g; m->sum_w_f = (float)m->sum_trace_w;
-    m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
-    return 0;
-  }
-  /* analytic drift-diffusion fields (models 1, 2) */
+/* Per-bin coefficients of the analytic drift-diffusion fields (models 1, 2
+ * and the per-bin grid model 3, which the meridional solver starts from):
+ * settling speed, radial slip table w_r (radwr, may be NULL), radial log
+ * potential table, A, B and the volume normalisation.  Returns 0 or the
+ * reference's error code 31-34. */
+static int sediment_bins(const clr_config* cfg, const clr_sediment* s, int tm, int n, const double* dia, int nrad,
+                         double* Av, double* Bv, double* normv, double* wsv, double* radlog, double* radwr,
+                         double* maxA_out) {
+  int fsm = cfg->free_surface_model;
+  double R = cfg->rin, ZMIN = cfg->zmin, ZMEAN = cfg->zmean;
+  double DH = cfg->vortex_delta_h, CORE = cfg->vortex_core_radius;
   double rho_p = s->particle_density, rho_w = s->water_density, mu = s->water_viscosity;
   double grav = s->gravity, Dt = s->eddy_diffusivity;
   int NQ = s->n_quad;
@@ -655,24 +645,27 @@ static int sediment_precompute(orc_model* m, const clr_sediment* s) {
   double fR = (fsm == 2) ? R2 / (a2 + R2) : R2;
   double Hmean = ZMEAN - ZMIN, dr = R / (double)NQ, drad = R / (double)(nrad - 1);
   if (!(Hmean > 0.0)) return 33;
-  double mjg = 0.0, mjw = 0.0, maxA = 0.0;
+  double maxA = 0.0;
   double* wr = (double*)malloc(sizeof(double) * (size_t)nrad);
   for (int i = 0; i < n; ++i) {
-    double d = m->d[i];
+    double d = dia[i];
     double tau = rho_p * d * d / (18.0 * mu);
     double ws0 = (rho_p - rho_w) * grav * d * d / (18.0 * mu);
     double ws = (tm != 1) ? schiller_naumann_velocity(ws0, d, rho_w, mu) : ws0;
-    m->sedws[i] = ws;
+    wsv[i] = ws;
     double B = ws / Dt;
-    m->sedB[i] = B;
-    double* rl = m->radlog + (size_t)i * nrad;
+    Bv[i] = B;
+    double* rl = radlog + (size_t)i * nrad;
     if (tm == 1) {
       double A = tau * om2 * ((fsm == 2) ? a2 : 1.0) / (2.0 * Dt);
-      m->sedA[i] = A;
+      Av[i] = A;
       if (A > maxA) maxA = A;
       for (int j = 0; j < nrad; ++j) {
         double r = (double)j * drad, rr2 = r * r, f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2;
         rl[j] = A * (f - fR);
+        double acc = 0.0;
+        if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
+        wr[j] = tau * acc;
       }
     } else {
       double total = 0.0, prev = 0.0;
@@ -694,16 +687,17 @@ static int sediment_precompute(orc_model* m, const clr_sediment* s) {
       }
       rl[nrad - 1] = 0.0;
       double Aeq = (fR > 0.0) ? -rl[0] / fR : 0.0;
-      m->sedA[i] = Aeq;
+      Av[i] = Aeq;
       if (Aeq > maxA) maxA = Aeq;
     }
+    if (radwr) memcpy(radwr + (size_t)i * nrad, wr, sizeof(double) * (size_t)nrad);
     double integ = 0.0;
     for (int j = 0; j < NQ; ++j) {
       double r = ((double)j + 0.5) * dr, rr2 = r * r, zs = ZMEAN;
       if (fsm == 1 && DH > 0.0) zs = ZMEAN + DH * (rr2 / R2 - 0.5);
       else if (fsm == 2 && DH > 0.0) zs = ZMEAN - hbar + Hinf * rr2 / (a2 + rr2);
       double H = fmax(zs - ZMIN, 0.0), lr;
-      if (tm == 1) { double f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2; lr = m->sedA[i] * (f - fR); }
+      if (tm == 1) { double f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2; lr = Av[i] * (f - fR); }
       else {
         double u = r / R * (double)(nrad - 1);
         int k = (int)floor(u);
@@ -718,11 +712,47 @@ static int sediment_precompute(orc_model* m, const clr_sediment* s) {
     }
     double norm = (2.0 / (R2 * Hmean)) * integ * dr;
     if (!(norm > 0.0) || !isfinite(norm)) { free(wr); return 34; }
-    m->sednorm[i] = norm;
-    mjg += m->trace_g[i] / norm;
-    mjw += m->trace_w[i] / norm;
+    normv[i] = norm;
   }
   free(wr);
+  if (maxA_out) *maxA_out = maxA;
+  return 0;
+}
+
+int orc_sediment_bins(const clr_config* cfg, const clr_sediment* s, int n, const double* diameter, double* A,
+                      double* B, double* norm, double* ws, double* radlog, double* radwr) {
+  if (!cfg || !s || n <= 0 || !diameter || !A || !B || !norm || !ws || !radlog) return -1;
+  return sediment_bins(cfg, s, cfg->sediment_model, n, diameter, s->n_radial, A, B, norm, ws, radlog, radwr, NULL);
+}
+
+static int sediment_precompute(orc_model* m, const clr_sediment* s) {
+  int n = m->n, tm = m->cfg.sediment_model;
+  int nrad = m->nrad;
+  for (int i = 0; i < n; ++i) {
+    m->sedA[i] = 0.0; m->sedB[i] = 0.0; m->sednorm[i] = 1.0; m->sedws[i] = 0.0;
+    for (int j = 0; j < nrad; ++j) m->radlog[(size_t)i * nrad + j] = 0.0;
+  }
+  if (tm == CLR_SED_UNIFORM) {
+    m->maj[0] = m->sum_trace_g; m->maj[1] = m->sum_trace_w; m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
+    return 0;
+  }
+  if (tm == CLR_SED_RZ_FIELD || tm == CLR_SED_XYZ_FIELD) {
+    double mx = (double)field_max(m->field, m->field_len);
+    m->maj[0] = m->sum_trace_g * mx; m->maj[1] = m->sum_trace_w * mx;
+    m->sum_g_f = (float)m->sum_trace_g; m->sum_w_f = (float)m->sum_trace_w;
+    m->maj[2] = m->maj[0] + m->maj[1]; m->maj[3] = 0.0;
+    return 0;
+  }
+  /* analytic drift-diffusion fields (models 1, 2) */
+  double maxA = 0.0;
+  int e = sediment_bins(&m->cfg, s, tm, n, m->d, nrad, m->sedA, m->sedB, m->sednorm, m->sedws, m->radlog, NULL,
+                        &maxA);
+  if (e) return e;
+  double mjg = 0.0, mjw = 0.0;
+  for (int i = 0; i < n; ++i) {
+    mjg += m->trace_g[i] / m->sednorm[i];
+    mjw += m->trace_w[i] / m->sednorm[i];
+  }
   if (tm == CLR_SED_GRID_PER_BIN) {
     /* the caller's per-bin r-z grid replaces the analytic field */
     size_t plane = (size_t)m->fp.fn0 * (size_t)m->fp.fn1;

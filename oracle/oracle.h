@@ -72,6 +72,23 @@ int orc_beckmann_slopes_probe(int n, const double* si, const double* u1, const d
 int orc_walk_probe(int n, uint64_t seed, float alpha, const float* b, float n_top, float k_top, float n_bot,
                    float k_bot, float* d, int* status, int* top, int* fc);
 
+/* Per-bin sediment coefficients of cfg->sediment_model (1, 2 or 3):
+ * A, B, normalisation, settling speed [n], radial log potential and radial
+ * slip w_r [n][sed->n_radial] (radwr may be NULL); CLARITAS.py:3940-4038. */
+int orc_sediment_bins(const clr_config* cfg, const clr_sediment* s, int n, const double* diameter, double* A,
+                      double* B, double* norm, double* ws, double* radlog, double* radwr);
+
+/* meridional.c: vortex-surface diagnostics (wall_z, center_z, h_inf, h_bar,
+ * vtheta_peak; CLARITAS.py:708-792), the secondary-circulation velocity
+ * (CLARITAS.py:4059-4073) and the full meridional solve
+ * (CLARITAS.py:4091-4215, 6451-6725) with the clr_meridional_solve
+ * contract. */
+int orc_vortex_geometry(int fsm, double zmean, double R, double D, double a, double g, double out[5]);
+void orc_meridional_velocity(int n, const double* r, const double* z, double zmin, double zmean, double R, int fsm,
+                             double D, double a, double upeak, double* ur, double* uz);
+int orc_meridional_solve(const clr_config* cfg, const clr_particles* p, const clr_sediment* sed,
+                         const clr_meridional* mp, clr_meridional_out* out);
+
 #ifdef __cplusplus
 }
 #endif

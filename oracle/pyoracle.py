@@ -64,6 +64,16 @@ def load():
     lib.orc_beckmann_slopes_probe.argtypes = [C.c_int] + [P(C.c_double)] * 5 + [P(C.c_int)]
     lib.orc_walk_probe.argtypes = [C.c_int, C.c_uint64, C.c_float, P(C.c_float), C.c_float, C.c_float, C.c_float,
                                    C.c_float, P(C.c_float), P(C.c_int), P(C.c_int), P(C.c_int)]
+    lib.orc_sediment_bins.argtypes = [P(abi.Config), P(abi.Sediment), C.c_int, P(C.c_double)] + [P(C.c_double)] * 6
+    lib.orc_sediment_bins.restype = C.c_int
+    lib.orc_vortex_geometry.argtypes = [C.c_int] + [C.c_double] * 5 + [P(C.c_double)]
+    lib.orc_vortex_geometry.restype = C.c_int
+    lib.orc_meridional_velocity.argtypes = [C.c_int, P(C.c_double), P(C.c_double)] + [C.c_double] * 3 + \
+        [C.c_int] + [C.c_double] * 3 + [P(C.c_double), P(C.c_double)]
+    lib.orc_meridional_velocity.restype = None
+    lib.orc_meridional_solve.argtypes = [P(abi.Config), P(abi.Particles), P(abi.Sediment), P(abi.Meridional),
+                                         P(abi.MeridionalOut)]
+    lib.orc_meridional_solve.restype = C.c_int
     _lib = lib
     return lib
 
@@ -274,3 +284,62 @@ def free_surface_probe(model, rays6):
     PF = C.POINTER(C.c_float)
     load().orc_free_surface_probe(C.byref(cfg), rays6.ctypes.data_as(PF), n, *[o.ctypes.data_as(PF) for o in outs])
     return dict(zip(['zs_start', 't_hit', 'zs_hit', 'nx', 'ny', 'nz'], outs))
+
+
+# ---------------------------------------------------------------- meridional
+SURFACE_CODES = {'flat': 0, 'parabolic_vortex': 1, 'localized_scully_vortex': 2}
+
+
+def vortex_geometry(model, zmean, R, D, a, g=9.81):
+    """(wall_z, center_z, h_inf, h_bar, vtheta_peak) of the oracle."""
+    out = np.zeros(5)
+    rc = load().orc_vortex_geometry(SURFACE_CODES[model], zmean, R, D, a, g, _pd(out))
+    if rc:
+        raise ValueError('invalid vortex')
+    return out
+
+
+def meridional_velocity(r, z, zmean, zmin, R, D, a, chi, g=9.81, model='localized_scully_vortex'):
+    """(ur, uz) of the secondary circulation, with the host signature of the
+    reference's meridional_velocity_host (CLARITAS.py:795-867)."""
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+    vt = vortex_geometry(model, zmean, R, D, a, g)[4]
+    ur, uz = np.zeros_like(r), np.zeros_like(r)
+    load().orc_meridional_velocity(r.size, _pd(r), _pd(z), zmin, zmean, R, SURFACE_CODES[model], D, a,
+                                   chi * vt, _pd(ur), _pd(uz))
+    return ur, uz
+
+
+def sediment_bins(model, material):
+    """Per-bin sediment tables (A, B, norm, ws, radlog, radwr) for the
+    model's sediment law (the meridional law uses model 3's tables)."""
+    mat = model.resolve_material(material)
+    cfg = model.config_struct(mat)
+    sed = model.sediment_struct(mat)
+    d = np.ascontiguousarray(mat.diameter_m, dtype=np.float64)
+    n, nr = d.size, sed.n_radial
+    out = [np.zeros(n) for _ in range(4)] + [np.zeros((n, nr)), np.zeros((n, nr))]
+    rc = load().orc_sediment_bins(C.byref(cfg), C.byref(sed), n, _pd(d), *[_pd(a) for a in out])
+    if rc:
+        raise RuntimeError(f'oracle sediment bins failed with code {rc}')
+    return dict(zip(('A', 'B', 'norm', 'ws', 'radlog', 'radwr'), out))
+
+
+def meridional_solve(model, material, coeff=True, settings=None):
+    """Oracle meridional solve for `model`'s settings and the material's PSD;
+    returns the same dict as tardiis_and_claritas_amd.meridional.solve."""
+    from tardiis_and_claritas_amd.meridional import result_dict
+
+    mat = model.resolve_material(material)
+    cfg = model.config_struct(mat)
+    p, keep = model.particles_struct(mat, 0.0, None, 'mass_fraction')
+    sed = model.sediment_struct(mat)
+    st = settings or model.meridional
+    out, arrs = abi.meridional_out(int(p.n), int(st.nr), int(st.nz), coeff=coeff)
+    mp = st.struct()
+    rc = load().orc_meridional_solve(C.byref(cfg), C.byref(p), C.byref(sed), C.byref(mp), C.byref(out))
+    del keep
+    if rc and rc != -7:
+        raise RuntimeError(f'oracle meridional solve failed with code {rc}')
+    return result_dict(out, arrs, rc)

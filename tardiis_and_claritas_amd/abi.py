@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -44,6 +44,7 @@ SEDIMENT_TRANSPORT_MODEL_CODES = {
     'stokes_drift_diffusion': 1,
     'schiller_naumann_drift_diffusion': 2,
     'grid_per_bin': 3,
+    'schiller_naumann_meridional_advection_diffusion': 3,
     'rz_field': 4,
     'xyz_field': 5,
 }
@@ -175,9 +176,37 @@ class MlpShape(C.Structure):
                 ('n_hidden_layers', I32), ('pad_', I32)]
 
 
+class Meridional(C.Structure):
+    _fields_ = [('circulation_ratio', D), ('tolerance', D), ('sor', D), ('nr', I32), ('nz', I32),
+                ('max_iterations', I32), ('check_interval', I32)]
+
+
+class MeridionalOut(C.Structure):
+    _fields_ = [('grid', PD), ('residual', PD), ('scale_max', PD), ('scale_min', PD), ('point_scales', PD),
+                ('active', C.POINTER(C.c_uint8)), ('coeff', PD), ('grid_zmax', D), ('vtheta_peak', D),
+                ('peak_speed', D), ('iterations', I32), ('pad_', I32)]
+
+
+def meridional_out(n, nr, nz, coeff=False):
+    """Allocate host outputs for one meridional solve; returns (struct, arrays)."""
+    import numpy as np
+
+    arrs = {
+        'grid': np.zeros((n, nr, nz)), 'residual': np.zeros(n), 'scale_max': np.zeros(n),
+        'scale_min': np.zeros(n), 'point_scales': np.zeros((n, 4)), 'active': np.zeros((nr, nz), dtype=np.uint8),
+    }
+    if coeff:
+        arrs['coeff'] = np.zeros((5, n, nr, nz))
+    o = MeridionalOut()
+    for k, v in arrs.items():
+        ct = C.POINTER(C.c_uint8) if v.dtype == np.uint8 else PD
+        setattr(o, k, v.ctypes.data_as(ct))
+    return o, arrs
+
+
 STRUCT_IDS = {
     0: Config, 1: Particles, 2: Sediment, 3: MieOut, 4: TablesOut, 5: Tallies, 6: Record,
-    7: BatchItem, 8: Launch,
+    7: BatchItem, 8: Launch, 9: Meridional, 10: MeridionalOut,
 }
 
 

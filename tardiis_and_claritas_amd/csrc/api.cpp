@@ -1,8 +1,8 @@
 // api.cpp -- C ABI of libclaritas_hip.so (declared in include/claritas.h).
 //
 // Host orchestration only: argument checks, device buffers, work-queue
-// construction and launches of the HIP kernels in mie.hip, precompute.hip and
-// transport.hip.  Physics runs on the device.
+// construction and launches of the HIP kernels in mie.hip, precompute.hip,
+// meridional.hip and transport.hip.  Physics runs on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/claritas.h"
+#include "meridional.h"
 #include "mie.h"
 #include "mlp.h"
 #include "precompute.h"
@@ -598,6 +599,8 @@ size_t clr_sizeof(int id) {
     case 6: return sizeof(clr_record);
     case 7: return sizeof(clr_batch_item);
     case 8: return sizeof(clr_launch);
+    case 9: return sizeof(clr_meridional);
+    case 10: return sizeof(clr_meridional_out);
     default: return 0;
   }
 }
@@ -889,6 +892,154 @@ int clr_time_run(clr_ctx* c, uint64_t n, uint64_t seed, const clr_launch* launch
 }
 
 double clr_last_kernel_ms(clr_ctx* c) { return c ? c->last_ms : 0.0; }
+
+int clr_meridional_solve(clr_ctx* c, const clr_config* g, const clr_particles* p, const clr_sediment* sed,
+                         const clr_meridional* mp, clr_meridional_out* out) {
+  if (!c) return CLR_E_ARG;
+  if (!g || !p || !sed || !mp || !out) return fail(c, CLR_E_ARG, "meridional: null argument");
+  if (p->n <= 0 || p->n > kMaxBins || !p->diameter) return fail(c, CLR_E_ARG, "particles: n must be 1..4096");
+  // CLARITAS.py:5873-5887
+  if (!(mp->circulation_ratio >= 0.0 && mp->circulation_ratio <= 0.5))
+    return fail(c, CLR_E_ARG, "meridional_circulation_ratio must be finite and in [0,0.5]");
+  if (mp->nr < 16 || mp->nz < 32) return fail(c, CLR_E_ARG, "meridional solver grid must be at least 16 x 32");
+  if (mp->nr > 4096 || mp->nz > 4096 || (int64_t)mp->nr * mp->nz * p->n > (int64_t)1 << 28)
+    return fail(c, CLR_E_RANGE, "meridional grid too large");
+  if (mp->max_iterations < 250 || mp->check_interval < 10)
+    return fail(c, CLR_E_ARG, "invalid meridional solver iteration controls");
+  if (!(mp->tolerance > 0.0 && mp->tolerance < 1e-2))
+    return fail(c, CLR_E_ARG, "meridional_solver_tolerance must be in (0,1e-2)");
+  if (!(mp->sor >= 1.0 && mp->sor < 1.95)) return fail(c, CLR_E_ARG, "meridional_solver_sor must be in [1,1.95)");
+  const int fsm = g->free_surface_model;
+  const double R = g->rin, D = g->vortex_delta_h, a = g->vortex_core_radius;
+  // CLARITAS.py:5911-5921
+  if ((fsm != CLR_SURFACE_PARABOLIC && fsm != CLR_SURFACE_SCULLY) || !(D > 0.0) ||
+      (fsm == CLR_SURFACE_SCULLY && !(a > 0.0)))
+    return fail(c, CLR_E_ARG, "meridional transport requires a non-zero supported vortex");
+  if (!(R > 0.0) || !(g->zmean > g->zmin)) return fail(c, CLR_E_ARG, "geometry: rin > 0 and zmean > zmin");
+  if (!(sed->particle_density > sed->water_density) || !(sed->water_density > 0.0) || !(sed->water_viscosity > 0.0) ||
+      !(sed->gravity > 0.0) || !(sed->eddy_diffusivity > 0.0) || sed->n_quad < 64 || sed->n_radial < 65)
+    return fail(c, 31, "sediment precompute failed with error code 31");
+  // vortex_surface_geometry (CLARITAS.py:708-792): wall and centre heights,
+  // peak azimuthal speed
+  double wall_z, center_z, vtheta;
+  const double R2 = R * R, grav = sed->gravity;
+  if (fsm == CLR_SURFACE_PARABOLIC) {
+    wall_z = g->zmean + 0.5 * D;
+    center_z = g->zmean - 0.5 * D;
+    vtheta = std::sqrt(2.0 * grav * D / R2) * R;
+  } else {
+    const double a2 = a * a, h_inf = D * (a2 + R2) / R2, h_bar = h_inf * (1.0 - (a2 / R2) * std::log1p(R2 / a2));
+    wall_z = g->zmean - h_bar + D;
+    center_z = g->zmean - h_bar;
+    vtheta = 0.5 * std::sqrt(2.0 * grav * h_inf / a2) * a;
+  }
+  if (!(center_z > g->zmin))
+    return fail(c, CLR_E_ARG, "vortex depression reaches or passes the acrylic bottom");
+  if (!(wall_z < g->zwall_top)) return fail(c, CLR_E_ARG, "free surface reaches or exceeds the physical acrylic wall top");
+  out->grid_zmax = wall_z;
+  out->vtheta_peak = vtheta;
+  out->peak_speed = mp->circulation_ratio * vtheta;
+  out->iterations = 0;
+  HIPCHK(c, hipSetDevice(c->device));
+
+  const int n = p->n, NR = mp->nr, NZ = mp->nz, nrad = sed->n_radial;
+  const size_t plane = (size_t)NR * NZ, total = plane * (size_t)n;
+  struct Bufs {
+    DevBuf d, A, B, norm, ws, radlog, radwr, active, grid, coeff, res, stats, err;
+    ~Bufs() {
+      DevBuf* all[] = {&d, &A, &B, &norm, &ws, &radlog, &radwr, &active, &grid, &coeff, &res, &stats, &err};
+      for (DevBuf* b : all) b->release();
+    }
+  } b;
+  HIPCHK(c, b.d.ensure(sizeof(double) * n));
+  DevBuf* per_bin[] = {&b.A, &b.B, &b.norm, &b.ws, &b.res};
+  for (DevBuf* x : per_bin) HIPCHK(c, x->ensure(sizeof(double) * n));
+  HIPCHK(c, b.radlog.ensure(sizeof(double) * (size_t)n * nrad));
+  HIPCHK(c, b.radwr.ensure(sizeof(double) * (size_t)n * nrad));
+  HIPCHK(c, b.active.ensure(plane));
+  HIPCHK(c, b.grid.ensure(sizeof(double) * total));
+  HIPCHK(c, b.coeff.ensure(sizeof(double) * 5 * total));
+  HIPCHK(c, b.stats.ensure(sizeof(double) * 6 * n));
+  HIPCHK(c, b.err.ensure(sizeof(int) * 2));
+  HIPCHK(c, hipMemcpyAsync(b.d.p, p->diameter, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(b.err.p, 0, sizeof(int) * 2, c->stream));
+
+  // per-bin finite-Re coefficients: the reference runs
+  // sediment_transport_precompute_kernel with model 3 (CLARITAS.py:6399-6443)
+  PrepLaunch P{};
+  P.n = n; P.sed_model = CLR_SED_GRID_PER_BIN; P.fs_model = fsm; P.n_quad = sed->n_quad; P.n_radial = nrad;
+  P.particle_density = sed->particle_density; P.water_density = sed->water_density;
+  P.viscosity = sed->water_viscosity; P.gravity = grav; P.eddy_diffusivity = sed->eddy_diffusivity;
+  P.R = R; P.zmin = g->zmin; P.zmean = g->zmean; P.dh = D; P.core = a;
+  P.diameter = b.d.as<double>();
+  P.sedA = b.A.as<double>(); P.sedB = b.B.as<double>(); P.sednorm = b.norm.as<double>(); P.sedws = b.ws.as<double>();
+  P.radlog = b.radlog.as<double>(); P.radwr = b.radwr.as<double>(); P.error_flag = b.err.as<int>();
+  HIPCHK(c, clr_launch_sediment_bins(P, c->stream));
+  int ecode[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(ecode, b.err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ecode[0]) return fail(c, ecode[0], "sediment precompute failed with error code " + std::to_string(ecode[0]));
+
+  MeridionalLaunch L{};
+  L.n = n; L.nr = NR; L.nz = NZ; L.fs_model = fsm; L.n_radial = nrad;
+  L.R = R; L.zmin = g->zmin; L.zgrid = wall_z; L.zmean = g->zmean; L.dh = D; L.core = a;
+  L.upeak = out->peak_speed; L.eddy = sed->eddy_diffusivity; L.omega = mp->sor;
+  L.target = 0.5 * R2 * (g->zmean - g->zmin);
+  L.sedB = b.B.as<double>(); L.sednorm = b.norm.as<double>(); L.ws = b.ws.as<double>();
+  L.radlog = b.radlog.as<double>(); L.radwr = b.radwr.as<double>();
+  L.active = b.active.as<unsigned char>(); L.grid = b.grid.as<double>(); L.coeff = b.coeff.as<double>();
+  L.residual = b.res.as<double>(); L.stats = b.stats.as<double>(); L.error_flag = b.err.as<int>();
+  clr_meridional_shape(L);
+  HIPCHK(c, clr_meridional_setup(L, c->stream));
+
+  // driver loop, CLARITAS.py:6568-6700: one launch per check interval
+  std::vector<double> res((size_t)n, 0.0);
+  const int maxit = mp->max_iterations, check = mp->check_interval, min_it = std::min(1000, maxit);
+  int it = 0, iterations = 0;
+  double maxres = 0.0;
+  int rc = 0;
+  while (it < maxit) {
+    const int next = std::min((it / check + 1) * check, maxit);
+    HIPCHK(c, clr_meridional_sweep(L, it, next, it == 0, c->stream));
+    HIPCHK(c, hipMemcpyAsync(res.data(), b.res.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ecode, b.err.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    it = next;
+    if (ecode[0]) {
+      rc = fail(c, ecode[0], std::string(ecode[1] == 0 ? "meridional initial normalization" : "meridional normalization") +
+                                 " failed with error code " + std::to_string(ecode[0]));
+      break;
+    }
+    maxres = *std::max_element(res.begin(), res.end());
+    if (it >= min_it && maxres <= mp->tolerance) { iterations = it; break; }
+  }
+  if (iterations == 0) iterations = it;
+  out->iterations = iterations;
+  if (!rc) {
+    HIPCHK(c, clr_meridional_finish(L, c->stream));
+    if (!(maxres <= mp->tolerance)) {
+      char msg[160];
+      std::snprintf(msg, sizeof msg, "meridional transport solver did not converge: max residual=%.6g after %d iterations",
+                    maxres, iterations);
+      rc = fail(c, CLR_E_NOT_CONVERGED, msg);
+    }
+  }
+  if (out->grid) HIPCHK(c, hipMemcpyAsync(out->grid, b.grid.p, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream));
+  if (out->coeff)
+    HIPCHK(c, hipMemcpyAsync(out->coeff, b.coeff.p, sizeof(double) * 5 * total, hipMemcpyDeviceToHost, c->stream));
+  if (out->active) HIPCHK(c, hipMemcpyAsync(out->active, b.active.p, plane, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> st(6 * (size_t)n, 0.0);
+  HIPCHK(c, hipMemcpyAsync(st.data(), b.stats.p, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out->residual) std::memcpy(out->residual, res.data(), sizeof(double) * n);
+  for (int q = 0; q < n; ++q) {
+    if (out->scale_max) out->scale_max[q] = st[6 * q];
+    if (out->scale_min) out->scale_min[q] = st[6 * q + 1];
+    if (out->point_scales)
+      for (int j = 0; j < 4; ++j) out->point_scales[4 * q + j] = st[6 * q + 2 + j];
+  }
+  return rc;
+}
 
 }  // extern "C"
 

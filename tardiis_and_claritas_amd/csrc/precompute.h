@@ -19,12 +19,16 @@ struct PrepLaunch {
   double *radius, *number_density, *mu_geom, *mu_sca, *mu_ext, *mu_wave, *mu_event;
   double *trace_g, *trace_w, *geom_cdf, *wave_cdf, *mu_scalars, *trace_sums;
   double *sedA, *sedB, *sednorm, *sedws, *radlog, *majorants;
+  double* radwr;  // optional [n][n_radial] radial slip w_r (the meridional solve's face slips)
   float *radius_f, *n_real_f, *k_imag_f, *albedo_f, *geom_cdf_f, *wave_cdf_f;
   int* error_flag;
 };
 
 // fmax_bits: device scratch of max(1, n) words, zeroed by the caller.
 hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, hipStream_t s);
+// Only the per-bin sediment coefficients (settling speed, A, B, norm, radial
+// tables); reads n, the sediment/geometry scalars and `diameter`.
+hipError_t clr_launch_sediment_bins(const PrepLaunch& L, hipStream_t s);
 
 // Guide tables for inverse-CDF sampling on the transport kernel's uniform
 // u = (m + 1/2) 2^-23 (m a 23-bit integer).  Bucket j of a row covers

@@ -145,9 +145,11 @@ __global__ void sediment_bins_kernel(PrepLaunch L) {
   const int tm = L.sed_model, fsm = L.fs_model, nrad = L.n_radial, NQ = L.n_quad;
   const double R = L.R, R2 = R * R, a2 = L.core * L.core, DH = L.dh;
   double* rl = L.radlog + (size_t)i * nrad;
+  double* wrt = L.radwr ? L.radwr + (size_t)i * nrad : nullptr;
   if (tm != CLR_SED_STOKES && tm != CLR_SED_SCHILLER && tm != CLR_SED_GRID_PER_BIN) {
     L.sedA[i] = 0.0; L.sedB[i] = 0.0; L.sednorm[i] = 1.0; L.sedws[i] = 0.0;
     for (int j = 0; j < nrad; ++j) rl[j] = 0.0;
+    if (wrt) for (int j = 0; j < nrad; ++j) wrt[j] = 0.0;
     return;
   }
   double Hinf = 0.0, hbar = 0.0, om2 = 0.0;
@@ -171,6 +173,11 @@ __global__ void sediment_bins_kernel(PrepLaunch L) {
     for (int j = 0; j < nrad; ++j) {
       const double r = (double)j * drad, rr2 = r * r, f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2;
       rl[j] = A * (f - fR);
+      if (wrt) {
+        double acc = 0.0;
+        if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
+        wrt[j] = tau * acc;
+      }
     }
   } else {
     // trapezoid integral of the finite-Re radial slip; rl temporarily holds w_r
@@ -181,6 +188,7 @@ __global__ void sediment_bins_kernel(PrepLaunch L) {
       if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
       const double wr = schiller_naumann(tau * acc, d, rho_w, mu);
       rl[j] = wr;
+      if (wrt) wrt[j] = wr;
       if (j > 0) total += 0.5 * (prev + wr) * drad;
       prev = wr;
     }
@@ -259,6 +267,11 @@ __global__ void majorant_kernel(PrepLaunch L, const unsigned int* __restrict__ f
   if (!(L.majorants[2] >= 0.0) || !isfinite(L.majorants[2])) atomicExch(L.error_flag, 35);
 }
 }  // namespace
+
+hipError_t clr_launch_sediment_bins(const PrepLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL(sediment_bins_kernel, dim3((L.n + 63) / 64), dim3(64), 0, s, L);
+  return hipGetLastError();
+}
 
 hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, hipStream_t s) {
   hipLaunchKernelGGL(transport_split_kernel, dim3(1), dim3(256), 0, s, L);

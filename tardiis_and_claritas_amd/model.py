@@ -18,9 +18,12 @@ from typing import Optional
 
 import numpy as np
 
-from . import abi
+from . import abi, meridional
 from ._lib import Context
+from .meridional import MeridionalSettings
 from .physics import (DETECTOR_ANGLES_DEG, Material, reference_material, vortex_surface_geometry)
+
+MERIDIONAL_MODEL = 'schiller_naumann_meridional_advection_diffusion'
 
 CONDITIONAL_MAX_TRIALS = 16384  # CLARITAS.py:1117
 
@@ -146,6 +149,14 @@ class ForwardResult:
     sediment_majorant_geom_per_m: float
     sediment_majorant_wave_per_m: float
     sediment_majorant_total_per_m: float
+    sediment_eddy_diffusivity_m2_s: float
+    sediment_max_bin_density_scale: float
+    meridional_circulation_ratio: float
+    meridional_peak_speed_m_s: float
+    meridional_grid_nr: int
+    meridional_grid_nz: int
+    meridional_solver_iterations: int
+    meridional_solver_max_residual: float
     mean_bottom_inner_reflections: float
     mean_bottom_outer_reflections: float
     bottom_tir_fraction: float
@@ -261,6 +272,13 @@ class ClaritasForwardModel:
         gravity_m_s2=9.81,
         sediment_transport_quadrature_points=4096,
         sediment_radial_lookup_points=2049,
+        meridional_circulation_ratio=0.10,
+        meridional_grid_nr=48,
+        meridional_grid_nz=96,
+        meridional_solver_max_iterations=8000,
+        meridional_solver_check_interval=250,
+        meridional_solver_tolerance=1.0e-6,
+        meridional_solver_sor=1.75,
         alpha1=0.45,
         alpha2=5.0,
         source_angular_model='beta_radiance_3d',
@@ -327,6 +345,19 @@ class ClaritasForwardModel:
         self.gravity = float(gravity_m_s2)
         self.sediment_transport_quadrature_points = int(sediment_transport_quadrature_points)
         self.sediment_radial_lookup_points = int(sediment_radial_lookup_points)
+        self.meridional_circulation_ratio = float(meridional_circulation_ratio)
+        self.meridional = MeridionalSettings(
+            circulation_ratio=self.meridional_circulation_ratio, nr=int(meridional_grid_nr),
+            nz=int(meridional_grid_nz), max_iterations=int(meridional_solver_max_iterations),
+            check_interval=int(meridional_solver_check_interval), tolerance=float(meridional_solver_tolerance),
+            sor=float(meridional_solver_sor))
+        self.meridional_grid_nr, self.meridional_grid_nz = self.meridional.nr, self.meridional.nz
+        self.meridional_solver_max_iterations = self.meridional.max_iterations
+        self.meridional_solver_check_interval = self.meridional.check_interval
+        self.meridional_solver_tolerance, self.meridional_solver_sor = self.meridional.tolerance, self.meridional.sor
+        self.meridional_vtheta_peak = 0.0
+        self.meridional_peak_speed = 0.0
+        self.meridional_result = None
         self.alpha1, self.alpha2 = float(alpha1), float(alpha2)
         self.source_angular_model = str(source_angular_model).strip().lower()
         self.density = float(density_kg_per_m3)
@@ -403,6 +434,17 @@ class ClaritasForwardModel:
         if self.sediment_transport_model in ('rz_field', 'xyz_field', 'grid_per_bin'):
             if self.sediment_field is None:
                 raise ValueError(f'{self.sediment_transport_model} needs sediment_field')
+        # meridional circulation, CLARITAS.py:5873-5921
+        self.meridional.validate()
+        if self.free_surface_model in ('localized_scully_vortex', 'parabolic_vortex') and self.vortex_delta_h > 0.0:
+            vg = vortex_surface_geometry(self.free_surface_model, self.zmean, self.rin, self.vortex_delta_h,
+                                         self.vortex_core_radius, self.gravity)
+            self.meridional_vtheta_peak = float(vg['vtheta_peak_m_s'])
+            self.meridional_peak_speed = self.meridional_circulation_ratio * self.meridional_vtheta_peak
+        if self.sediment_transport_model == MERIDIONAL_MODEL:
+            if self.free_surface_model not in ('localized_scully_vortex', 'parabolic_vortex') or \
+                    self.vortex_delta_h <= 0.0:
+                raise ValueError('meridional transport requires a non-zero supported vortex')
 
     # --------------------------------------------------------------- structs
     def config_struct(self, material: Material) -> abi.Config:
@@ -498,6 +540,8 @@ class ClaritasForwardModel:
         cfg = self.config_struct(mat)
         p, keep = self.particles_struct(mat, concentration_g_per_L, weights, weight_mode)
         sed = self.sediment_struct(mat)
+        if self.sediment_transport_model == MERIDIONAL_MODEL:
+            cfg = self._solve_meridional(cfg, p, sed)
         f = self.sediment_field
         fptr = f.ctypes.data_as(C.POINTER(C.c_float)) if f is not None else None
         flen = int(f.size) if f is not None else 0
@@ -505,6 +549,38 @@ class ClaritasForwardModel:
                                                 fptr, flen), 'clr_prepare')
         del keep
         return mat, cfg
+
+    def _solve_meridional(self, cfg, particles, sed):
+        """Device meridional solve for the prepared PSD; its grid becomes the
+        FP32 per-bin r-z field the transport reads (sediment model 3, the
+        reference's sed_grid, CLARITAS.py:6451-6725, 7719-7722)."""
+        res = meridional.solve(self.ctx, cfg, particles, sed, self.meridional)
+        self.meridional_result = res
+        self.sediment_field = np.ascontiguousarray(res['grid'], dtype=np.float32)
+        self.sediment_field_zmax = res['grid_zmax']
+        return self.config_struct_from(cfg)
+
+    def config_struct_from(self, cfg):
+        """cfg with the current sediment field's shape and extent."""
+        f = self.sediment_field
+        cfg.field_n0, cfg.field_n1 = f.shape[-2], f.shape[-1]
+        cfg.field_nbins = f.shape[0]
+        cfg.field_zmax = float(self.sediment_field_zmax)
+        return cfg
+
+    def _max_bin_density_scale(self, tables):
+        """Largest per-bin density scale (sediment_max_density_scale,
+        CLARITAS.py:4021,4202): 1/normalisation for the analytic fields, the
+        solved or supplied grid's maximum otherwise."""
+        tm = self.sediment_transport_model
+        if tm == 'uniform':
+            return 1.0
+        if tm == MERIDIONAL_MODEL and self.meridional_result is not None:
+            return float(np.max(self.meridional_result['scale_max']))
+        if self.sediment_field is not None and tm != 'stokes_drift_diffusion' and \
+                tm != 'schiller_naumann_drift_diffusion':
+            return float(np.max(self.sediment_field))
+        return float(np.max(1.0 / np.asarray(tables['sed_norm'])))
 
     def tables(self, n_bins: int, slot: int = 0):
         """Host copy of the prepared per-bin tables of `slot`."""
@@ -661,6 +737,7 @@ class ClaritasForwardModel:
             return cnt[ci[k]] / den if entered else nan
 
         tables = self.tables(len(mat.diameter_m))
+        mres = self.meridional_result if self.sediment_transport_model == MERIDIONAL_MODEL else None
         mus = tables['mu_scalars']
         vg = vortex_surface_geometry(self.free_surface_model, self.zmean, self.rin,
                                      0.0 if self.free_surface_model == 'flat' else self.vortex_delta_h,
@@ -752,6 +829,14 @@ class ClaritasForwardModel:
             sediment_majorant_geom_per_m=float(tables['majorants'][0]),
             sediment_majorant_wave_per_m=float(tables['majorants'][1]),
             sediment_majorant_total_per_m=float(tables['majorants'][2]),
+            sediment_eddy_diffusivity_m2_s=self.sediment_eddy_diffusivity,
+            sediment_max_bin_density_scale=self._max_bin_density_scale(tables),
+            meridional_circulation_ratio=self.meridional_circulation_ratio,
+            meridional_peak_speed_m_s=self.meridional_peak_speed,
+            meridional_grid_nr=self.meridional.nr if mres else 2,
+            meridional_grid_nz=self.meridional.nz if mres else 2,
+            meridional_solver_iterations=int(mres['iterations']) if mres else 0,
+            meridional_solver_max_residual=float(mres['max_residual']) if mres else 0.0,
             mean_bottom_inner_reflections=per_entered('sum_bir'),
             mean_bottom_outer_reflections=per_entered('sum_bor'),
             bottom_tir_fraction=per_entered('rays_btir'),

@@ -74,6 +74,19 @@ def config2(conc_mg_l=1000.0, event_model='plain_mie', rouse_number=0.3, nr=64, 
     return kw, mat, conc_mg_l * 1e-3, 100_000_000, SEED
 
 
+def config2_meridional(conc_mg_l=1000.0, event_model='plain_mie', chi=0.10, nr=48, nz=96):
+    """Config-2 cell and PSD with the reference's solved sediment field: a
+    Scully vortex (wall-centre height 5 mm, core 3 mm) and the meridional
+    advection-diffusion model (CLARITAS.py:4040-4226, 6451-6725) on its
+    default 48 x 96 grid, SOR 1.75, tolerance 1e-6."""
+    kw = base_kwargs(event_model)
+    kw.update(free_surface_model='localized_scully_vortex', vortex_wall_center_height_difference_m=5e-3,
+              vortex_core_radius_m=3e-3, sediment_transport_model='schiller_naumann_meridional_advection_diffusion',
+              meridional_circulation_ratio=chi, meridional_grid_nr=nr, meridional_grid_nz=nz)
+    mat = lognormal(20e-6, 1.8, 40, 1.46, 1e-4)
+    return kw, mat, conc_mg_l * 1e-3, 100_000_000, SEED
+
+
 def config3_params(n=256, seed=20261203):
     """256 configurations drawn with PCG64(20261203): PSD median log-U(2,60) um,
     sigma_g U(1.3,2.2), m_r U(1.45,1.60), m_i log-U(1e-4,1e-2), lambda in
@@ -138,6 +151,9 @@ def measure(name, histories=None, event_model='plain_mie'):
     the library's HIP-event kernel time, detector signals, failure counts)."""
     import time
     from .model import ClaritasForwardModel
+    if name == 'meridional':
+        from .meridional import measure as measure_solve
+        return measure_solve(repeats=int(histories or 5))
     if name == 'config1':
         kw, mat, conc, n, seed = config1(event_model)
     elif name == 'config5':
@@ -169,7 +185,8 @@ if __name__ == '__main__':
     import argparse
     import json
     ap = argparse.ArgumentParser(description='run one benchmark workload on the GPU')
-    ap.add_argument('name', help="config1 | config5 | config2:<mg/L>")
+    ap.add_argument('name', help="config1 | config5 | config2:<mg/L> | meridional (device sediment solve, "
+                                 "--histories = repeats)")
     ap.add_argument('--histories', type=float, default=None)
     ap.add_argument('--event-model', default='plain_mie')
     a = ap.parse_args()
