@@ -297,14 +297,52 @@ typedef struct clr_meridional_out {
   int32_t pad_;
 } clr_meridional_out;
 
+/* ---- light-transport voxel diagnostics (the reference's
+ * light_transport_config, CLARITAS.py:7232-7300) ---- */
+typedef struct clr_light_config {
+  uint64_t max_rays;            /* histories with id < max_rays are recorded (>= 1) */
+  int32_t nx, ny, nz;           /* voxels over [-rin, rin]^2 x [zmin, zmax], each >= 4 */
+  int32_t representative_rays;  /* the first histories whose segments are stored (<= max_rays) */
+  int32_t max_segments;         /* segments stored per representative history (>= 1) */
+  int32_t pad_;
+  double zmax;                  /* top of the box: max(centre, wall) height of the free surface */
+} clr_light_config;
+
+/* Outputs; arrays are caller-owned, grids [nz][ny][nx] (x fastest),
+ * trajectory arrays [representative_rays][max_segments].  Any array may be
+ * NULL (skipped). */
+typedef struct clr_light_out {
+  float* track_length;               /* water-path track length per voxel (m) */
+  float* post_scatter_track_length;  /* the part after the first particle event */
+  float* scatter_event_weight;       /* surviving particle events (weight 1 each) */
+  uint32_t* scatter_event_count;
+  uint32_t* absorption_event_count;  /* accepted extinction encounters ending in absorption */
+  float* traj_x0; float* traj_y0; float* traj_z0;   /* clipped segment ends (NaN unused) */
+  float* traj_x1; float* traj_y1; float* traj_z1;
+  int32_t* traj_order;               /* particle events before the segment (-1 unused) */
+  int32_t* traj_event;               /* terminal event code (-1 unused; 0 propagation/boundary,
+                                        1-3 geometric external reflection / transmission /
+                                        internal reflection, 4 coherent Mie, 5-7 morphology,
+                                        8 inaccessible placement, 9 absorption, 10 failure) */
+  int32_t* traj_count;               /* [representative_rays] segments stored */
+  int32_t* traj_overflow;            /* [representative_rays] 1 when segments were dropped */
+  double segment_total;              /* sum of clipped segment lengths (m) */
+  double deposited_total;            /* sum of deposited lengths (m) */
+  uint64_t segment_count, scatter_event_total, absorption_event_total;
+  uint32_t error_count;              /* traversal-integrity failures and out-of-box sites */
+  uint32_t boundary_adjustment_count;/* benign endpoint snaps */
+  uint32_t outside[8];               /* out-of-box sites: count, first x, y, z (FP32 bits),
+                                        history id, tolerance (FP32 bits), recorded flag */
+} clr_light_out;
+
 struct clr_ctx;
 typedef struct clr_ctx clr_ctx;
 
 /* ABI identification and struct sizes (arity test, cf. the reference's
  * launch-arity check validate_claritas.py:972-1001). id: 0 config,
  * 1 particles, 2 sediment, 3 mie_out, 4 tables_out, 5 tallies, 6 record,
- * 7 batch_item, 8 launch, 9 meridional, 10 meridional_out. Returns 0 for
- * an unknown id. */
+ * 7 batch_item, 8 launch, 9 meridional, 10 meridional_out, 11 light_config,
+ * 12 light_out. Returns 0 for an unknown id. */
 int clr_abi_version(void);
 size_t clr_sizeof(int struct_id);
 
@@ -357,6 +395,29 @@ int clr_run(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offset,
  * reference runs one configuration per simulate() call. */
 int clr_run_batch(clr_ctx* ctx, const clr_batch_item* items, int n_items,
                   const clr_launch* launch, clr_tallies* out);
+
+/* clr_run plus the light-transport voxel diagnostics of the histories with
+ * id < light->max_rays: exact track-length deposition of every water
+ * segment by 3-D voxel traversal (all path, and path after the first
+ * particle event), scatter-event and absorption site grids, closure totals
+ * and the segments of the first representative histories.  The tallies
+ * equal clr_run's (same histories, same random numbers); the diagnostics
+ * run on a separate kernel variant.  Replaces trace_kernel's
+ * LIGHT_DIAG_ENABLED path (CLARITAS.py:3296-3485 at 4611, 4766, 4850,
+ * 4926-4946) and its readback (CLARITAS.py:7232-7300, 9455-9560).
+ * Diagnostic memory is capped at 2 GB. */
+int clr_run_light(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offset,
+                  const clr_launch* launch, const clr_light_config* light,
+                  clr_tallies* out, clr_light_out* light_out);
+
+/* The diagnostics' deposition on explicit inputs, as the reference's test
+ * kernels light_diag_segment_test_kernel / light_diag_event_test_kernel
+ * (CLARITAS.py:3470-3485): mode 0 deposits segment i of data[n][7]
+ * (x0, y0, z0, vx, vy, vz, dist; history id i, particle order i & 1), mode 1
+ * records event site i of data[n][3] into the scatter-event grids, in the
+ * box [-rin, rin]^2 x [zmin, light->zmax].  Outputs as clr_run_light. */
+int clr_light_probe(clr_ctx* ctx, int mode, int n, const float* data, const clr_light_config* light, double rin,
+                    double zmin, clr_light_out* light_out);
 
 /* Time one launch shape: trace n histories of slot 0 and return device
  * milliseconds in *ms (HIP events on the library stream). */

@@ -2323,7 +2323,192 @@ typedef struct {
   float inc_native, inc_mirror;
 } hist_t;
 
-static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist_t* o) {
+/* ======================================================================
+ * Light-transport voxel diagnostics, CLARITAS.py:3312-3467 (observational:
+ * no random numbers, no change to the photon).  Histories run in parallel,
+ * so grid and total updates are atomic; a history's trajectory slots are
+ * written only by its own thread.
+ * ====================================================================== */
+typedef struct {
+  const clr_light_config* c;
+  clr_light_out* o;
+  float rin, zmin, zmax;
+} light_t;
+
+static void light_count_u32(uint32_t* p, uint32_t v) {
+#pragma omp atomic
+  *p += v;
+}
+
+/* clip [t0, t1] of p + t d to [lo, hi], CLARITAS.py:3312-3316 */
+static int light_clip(double p, double d, double lo, double hi, double* t0, double* t1) {
+  if (fabs(d) < 1.0e-30f) return p >= lo && p <= hi;
+  double a = (lo - p) / d, b = (hi - p) / d;
+  if (a > b) { double q = a; a = b; b = q; }
+  *t0 = fmax(*t0, a);
+  *t1 = fmin(*t1, b);
+  return *t1 >= *t0;
+}
+
+static double light_face(double lo, int i, int st, double h, double a, double e) {
+  return (st == 0) ? (double)INF_F : (lo + (double)(st > 0 ? i + 1 : i) * h - a) / e;
+}
+
+/* exact track-length deposition by 3-D voxel traversal, CLARITAS.py:3318-3400 */
+static void light_segment(const light_t* L, uint64_t gid, float x0, float y0, float z0, float vx, float vy, float vz,
+                          float dist, int order) {
+  const clr_light_config* c = L->c;
+  clr_light_out* o = L->o;
+  if (gid >= c->max_rays || !(dist > 0.0f) || !isfinite(dist)) return;
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  const float Rin = L->rin, zmin = L->zmin, zmax = L->zmax;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !(zmax > zmin)) { light_count_u32(&o->error_count, 1); return; }
+  double dd = (double)dist, sx = (double)vx * dd, sy = (double)vy * dd, sz = (double)vz * dd;
+  double c0 = 0.0, c1 = 1.0;
+  if (!light_clip(x0, sx, -Rin, Rin, &c0, &c1) || !light_clip(y0, sy, -Rin, Rin, &c0, &c1) ||
+      !light_clip(z0, sz, zmin, zmax, &c0, &c1))
+    return;
+  double ax = x0 + sx * c0, ay = y0 + sy * c0, az = z0 + sz * c0;
+  double bx = x0 + sx * c1, by = y0 + sy * c1, bz = z0 + sz * c1;
+  double clipped = dist * (c1 - c0);
+  if (!(clipped > 0.0f)) return;
+#pragma omp atomic
+  o->segment_total += clipped;
+#pragma omp atomic
+  o->segment_count += 1;
+  if (gid < (uint64_t)c->representative_rays && c->max_segments > 0) {
+    int g = (int)gid, slot = o->traj_count ? o->traj_count[g] : 0;
+    if (slot < c->max_segments) {
+      size_t q = (size_t)g * c->max_segments + slot;
+      if (o->traj_x0) { o->traj_x0[q] = (float)ax; o->traj_y0[q] = (float)ay; o->traj_z0[q] = (float)az; }
+      if (o->traj_x1) { o->traj_x1[q] = (float)bx; o->traj_y1[q] = (float)by; o->traj_z1[q] = (float)bz; }
+      if (o->traj_order) o->traj_order[q] = order;
+      if (o->traj_event) o->traj_event[q] = 0;
+      if (o->traj_count) o->traj_count[g] = slot + 1;
+    } else if (o->traj_overflow) {
+      o->traj_overflow[g] = 1;
+    }
+  }
+  double xlo = -(double)Rin, ylo = -(double)Rin, zlo = (double)zmin;
+  double hx = (2.0 * (double)Rin) / nx, hy = (2.0 * (double)Rin) / ny, hz = ((double)zmax - zlo) / nz;
+  int ix = (int)floor((ax - xlo) / hx), iy = (int)floor((ay - ylo) / hy), iz = (int)floor((az - zlo) / hz);
+  if (ix < 0) ix = 0;
+  if (ix >= nx) ix = nx - 1;
+  if (iy < 0) iy = 0;
+  if (iy >= ny) iy = ny - 1;
+  if (iz < 0) iz = 0;
+  if (iz >= nz) iz = nz - 1;
+  double ex = bx - ax, ey = by - ay, ez = bz - az;
+  int stx = (ex > 0.0f) ? 1 : ((ex < 0.0f) ? -1 : 0);
+  int sty = (ey > 0.0f) ? 1 : ((ey < 0.0f) ? -1 : 0);
+  int stz = (ez > 0.0f) ? 1 : ((ez < 0.0f) ? -1 : 0);
+  double tmx = light_face(xlo, ix, stx, hx, ax, ex), tmy = light_face(ylo, iy, sty, hy, ay, ey);
+  double tmz = light_face(zlo, iz, stz, hz, az, ez);
+  if (tmx < 0.0f) tmx = 0.0f;
+  if (tmy < 0.0f) tmy = 0.0f;
+  if (tmz < 0.0f) tmz = 0.0f;
+  double u = 0.0, local = 0.0;
+  const float tol = 2.0e-6f;
+  int max_steps = nx + ny + nz + 8;
+  for (int k = 0; k < max_steps && u < 1.0f; ++k) {
+    if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || iz < 0 || iz >= nz) {
+      if (u >= 1.0f - tol) { light_count_u32(&o->boundary_adjustment_count, 1); u = 1.0f; break; }
+      light_count_u32(&o->error_count, 1);
+      break;
+    }
+    double un = fmin(1.0, fmin(tmx, fmin(tmy, tmz)));
+    if (un < 1.0f && un >= 1.0f - tol) { un = 1.0f; light_count_u32(&o->boundary_adjustment_count, 1); }
+    if (un < u) {
+      if (u - un <= tol) { un = u; light_count_u32(&o->boundary_adjustment_count, 1); }
+      else { light_count_u32(&o->error_count, 1); break; }
+    }
+    double len = (un - u) * clipped;
+    if (len > 0.0f) {
+      size_t q = ((size_t)iz * ny + iy) * nx + ix;
+      float lf = (float)len;
+      if (o->track_length) {
+#pragma omp atomic
+        o->track_length[q] += lf;
+      }
+      if (order > 0 && o->post_scatter_track_length) {
+#pragma omp atomic
+        o->post_scatter_track_length[q] += lf;
+      }
+      local += len;
+    }
+    if (un >= 1.0f) { u = 1.0f; break; }
+    double eps = 8.0 * 2.2204460492503131e-16 * fmax(1.0, fabs(un));
+    if (tmx <= un + eps) { ix += stx; tmx = light_face(xlo, ix, stx, hx, ax, ex); }
+    if (tmy <= un + eps) { iy += sty; tmy = light_face(ylo, iy, sty, hy, ay, ey); }
+    if (tmz <= un + eps) { iz += stz; tmz = light_face(zlo, iz, stz, hz, az, ez); }
+    u = un;
+  }
+  if (u < 1.0f - tol) light_count_u32(&o->error_count, 1);
+#pragma omp atomic
+  o->deposited_total += local;
+}
+
+/* CLARITAS.py:3402-3407 */
+static void light_mark(const light_t* L, uint64_t gid, int code) {
+  const clr_light_config* c = L->c;
+  if (gid >= c->max_rays || gid >= (uint64_t)c->representative_rays || c->max_segments <= 0) return;
+  if (!L->o->traj_count || !L->o->traj_event) return;
+  int g = (int)gid, n = L->o->traj_count[g];
+  if (n > 0 && n <= c->max_segments) L->o->traj_event[(size_t)g * c->max_segments + n - 1] = code;
+}
+
+/* one event site into a count grid (and the weight grid when given),
+ * CLARITAS.py:3409-3467 */
+static void light_site(const light_t* L, uint64_t gid, float x, float y, float z, float* weight, uint32_t* count,
+                       uint64_t* total) {
+  const clr_light_config* c = L->c;
+  clr_light_out* o = L->o;
+  if (gid >= c->max_rays) return;
+#pragma omp atomic
+  *total += 1;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  float Rin = L->rin, zmin = L->zmin, zmax = L->zmax;
+  float ctol = 2.0e-6f * fmaxf(2.0f * Rin, zmax - zmin);
+  if (!isfinite(x) || !isfinite(y) || !isfinite(z) || x < -Rin - ctol || x > Rin + ctol || y < -Rin - ctol ||
+      y > Rin + ctol || z < zmin - ctol || z > zmax + ctol) {
+#pragma omp critical(light_outside)
+    {
+      if (o->outside[0]++ == 0) {
+        memcpy(&o->outside[1], &x, 4); memcpy(&o->outside[2], &y, 4); memcpy(&o->outside[3], &z, 4);
+        o->outside[4] = (uint32_t)gid; memcpy(&o->outside[5], &ctol, 4); o->outside[6] = 1;
+      }
+    }
+    light_count_u32(&o->error_count, 1);
+    return;
+  }
+  int ix = (int)floorf(((x + Rin) / (2.0f * Rin)) * nx);
+  int iy = (int)floorf(((y + Rin) / (2.0f * Rin)) * ny);
+  int iz = (int)floorf(((z - zmin) / (zmax - zmin)) * nz);
+  int snapped = 0;
+  if (ix < 0) { ix = 0; snapped = 1; } else if (ix >= nx) { ix = nx - 1; snapped = 1; }
+  if (iy < 0) { iy = 0; snapped = 1; } else if (iy >= ny) { iy = ny - 1; snapped = 1; }
+  if (iz < 0) { iz = 0; snapped = 1; } else if (iz >= nz) { iz = nz - 1; snapped = 1; }
+  if (snapped) light_count_u32(&o->boundary_adjustment_count, 1);
+  size_t q = ((size_t)iz * ny + iy) * nx + ix;
+  if (weight) {
+#pragma omp atomic
+    weight[q] += 1.0f;
+  }
+  if (count) light_count_u32(&count[q], 1);
+}
+static void light_event(const light_t* L, uint64_t gid, float x, float y, float z, int code) {
+  light_site(L, gid, x, y, z, L->o->scatter_event_weight, L->o->scatter_event_count, &L->o->scatter_event_total);
+  light_mark(L, gid, code);
+}
+static void light_absorb(const light_t* L, uint64_t gid, float x, float y, float z) {
+  light_site(L, gid, x, y, z, NULL, L->o->absorption_event_count, &L->o->absorption_event_total);
+  light_mark(L, gid, 9);
+}
+/* Fresnel outcome code of a finite-particle event: 1 external reflection,
+ * 3 internal reflection, 2 transmission (+4 for the morphology channel) */
+static int light_code(int per, int pir, int morph) { return (per ? 1 : (pir ? 3 : 2)) + (morph ? 4 : 0); }
+
+static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist_t* o, const light_t* Ld) {
   const fparams* P = &m->fp;
   rng_t r;
   rng_seed(&r, seed, hist);
@@ -2455,6 +2640,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
       }
       int do_particle = freep < tbound;
       float travel = do_particle ? freep : tbound;
+      if (Ld) light_segment(Ld, hist, p->x, p->y, p->z, p->vx, p->vy, p->vz, travel, o->ic);
       o->n_flight++;
       p->x += travel * p->vx; p->y += travel * p->vy; p->z += travel * p->vz;
       if (do_particle) {
@@ -2484,14 +2670,21 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           if (U(&r) >= pacc) { o->n_null++; continue; }
         }
         o->ic++;
+        const float sx0 = p->x, sy0 = p->y, sz0 = p->z;  /* event site */
         int use_wave = (LW > 0.0f && U(&r) * LT >= LG);
         int em = P->event_model;
         if (em == CLR_EVENT_PLAIN_MIE) {
           int b = select_bin(&r, m, 0, p->x, p->y, p->z, LG);
           o->pc.n_abs++;
-          if (U(&r) >= (float)m->albedo[b]) { o->pc.pabs++; st = 5; break; }
+          if (U(&r) >= (float)m->albedo[b]) {
+            o->pc.pabs++;
+            if (Ld) light_absorb(Ld, hist, sx0, sy0, sz0);
+            st = 5;
+            break;
+          }
           o->wic++; o->wcoh++;
           mie_scatter(&r, m, b, &p->vx, &p->vy, &p->vz);
+          if (Ld) light_event(Ld, hist, sx0, sy0, sz0, 4);
           continue;
         }
         int cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
@@ -2504,7 +2697,12 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           p->x = h.x; p->y = h.y; p->z = h.z; p->vx = h.vx; p->vy = h.vy; p->vz = h.vz;
           o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
           if (h.channel == 0) o->gic++; else { o->wic++; if (h.channel == 1) o->wcoh++; }
-          if (h.status == 2) { st = 5; break; }
+          if (h.status == 2) {
+            if (Ld) light_absorb(Ld, hist, sx0, sy0, sz0);
+            st = 5;
+            break;
+          }
+          if (Ld) light_event(Ld, hist, sx0, sy0, sz0, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2));
           continue;
         }
         if (use_wave) {
@@ -2515,15 +2713,17 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
             /* morphology-wave event about the local azimuthal flow direction */
             float fr = sqrtf(p->x * p->x + p->y * p->y), ffx = 1.0f, ffy = 0.0f;
             if (fr > 1e-12f) { ffx = -p->y / fr; ffy = p->x / fr; }
-            int ppf = 0, ppfs = 0;
+            int ppf = 0, ppfs = 0, per_b = o->pc.per, pir_b = o->pc.pir;
             int ps = wave_particle_event(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], m->r_f[pidx], ffx, ffy, 0.0f,
                                          &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
-            if (ps == 2) { st = 5; break; }
-            if (ps != 1) { pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+            if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
+            if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+            if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 1));
             continue;
           }
           o->wcoh++;
           mie_scatter(&r, m, pidx, &p->vx, &p->vy, &p->vz);
+          if (Ld) light_event(Ld, hist, sx0, sy0, sz0, 4);
           continue;
         }
         o->gic++;
@@ -2543,14 +2743,16 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
           o->gic--;
           if (h.channel == 0) o->gic++; else { o->wic++; if (h.channel == 1) o->wcoh++; }
+          if (Ld) light_event(Ld, hist, sx0, sy0, sz0, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2));
           continue;
         }
-        if (!accessible) { o->ic--; o->gic--; o->n_null++; continue; }
-        int ppf = 0, ppfs = 0;
+        if (!accessible) { o->ic--; o->gic--; o->n_null++; if (Ld) light_mark(Ld, hist, 8); continue; }
+        int ppf = 0, ppfs = 0, per_b = o->pc.per, pir_b = o->pc.pir;
         int ps = sphere_interaction(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], rad, P->max_pb, gx, gy, gz,
                                     dcx, dcy, dcz, &p->x, &p->y, &p->z, &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
-        if (ps == 2) { st = 5; break; }
-        if (ps != 1) { pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+        if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
+        if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
+        if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 0));
         continue;
       }
       if (axial_top || axial_bot) {
@@ -2757,8 +2959,8 @@ static void merge(clr_tallies* D, const clr_tallies* S) {
   for (int i = 0; i < CLR_N_ICHIST; ++i) D->ic_hist[i] += S->ic_hist[i];
 }
 
-int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
-            clr_tallies* out, clr_record* rec, uint64_t n_rec) {
+static int run_impl(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+                    clr_tallies* out, clr_record* rec, uint64_t n_rec, const light_t* Ld) {
   if (!m || !out) return CLR_E_ARG;
   memset(out, 0, sizeof(*out));
   int G = m->fp.n_groups;
@@ -2788,7 +2990,7 @@ int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int 
 #endif
     for (int64_t i = 0; i < (int64_t)n; ++i) {
       uint64_t hist = offset + (uint64_t)i;
-      trace_history(m, seed, hist, &h);
+      trace_history(m, seed, hist, &h, Ld);
       int g = (int)(((unsigned __int128)(uint64_t)i * (unsigned)G) / n);
       tally(T, &h, g);
       if (rec && (uint64_t)i < n_rec) {
@@ -2801,6 +3003,63 @@ int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int 
   }
   for (int t = 0; t < nt; ++t) merge(out, &part[t]);
   free(part);
+  return 0;
+}
+
+int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+            clr_tallies* out, clr_record* rec, uint64_t n_rec) {
+  return run_impl(m, n, seed, offset, nthreads, out, rec, n_rec, NULL);
+}
+
+/* orc_run plus the light-transport diagnostics (clr_run_light contract):
+ * the caller's output arrays are initialised here (grids 0, trajectory
+ * coordinates NaN, orders and codes -1) and filled. */
+int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+                  const clr_light_config* lc, clr_tallies* out, clr_light_out* lo) {
+  if (!m || !lc || !lo || !out) return CLR_E_ARG;
+  if (lc->max_rays < 1 || lc->nx < 4 || lc->ny < 4 || lc->nz < 4 || lc->representative_rays < 0 ||
+      (uint64_t)lc->representative_rays > lc->max_rays || lc->max_segments < 1 || !(lc->zmax > m->cfg.zmin))
+    return CLR_E_ARG;
+  size_t vox = (size_t)lc->nx * lc->ny * lc->nz, rep = (size_t)lc->representative_rays;
+  size_t tb = rep * (size_t)lc->max_segments;
+  float* g4[] = {lo->track_length, lo->post_scatter_track_length, lo->scatter_event_weight};
+  for (int k = 0; k < 3; ++k) if (g4[k]) memset(g4[k], 0, vox * 4);
+  if (lo->scatter_event_count) memset(lo->scatter_event_count, 0, vox * 4);
+  if (lo->absorption_event_count) memset(lo->absorption_event_count, 0, vox * 4);
+  float* tr[] = {lo->traj_x0, lo->traj_y0, lo->traj_z0, lo->traj_x1, lo->traj_y1, lo->traj_z1};
+  for (int k = 0; k < 6; ++k) if (tr[k]) for (size_t i = 0; i < tb; ++i) tr[k][i] = NAN;
+  if (lo->traj_order) for (size_t i = 0; i < tb; ++i) lo->traj_order[i] = -1;
+  if (lo->traj_event) for (size_t i = 0; i < tb; ++i) lo->traj_event[i] = -1;
+  if (lo->traj_count) memset(lo->traj_count, 0, rep * 4);
+  if (lo->traj_overflow) memset(lo->traj_overflow, 0, rep * 4);
+  lo->segment_total = lo->deposited_total = 0.0;
+  lo->segment_count = lo->scatter_event_total = lo->absorption_event_total = 0;
+  lo->error_count = lo->boundary_adjustment_count = 0;
+  memset(lo->outside, 0, sizeof(lo->outside));
+  light_t L = {lc, lo, (float)m->cfg.rin, (float)m->cfg.zmin, (float)lc->zmax};
+  return run_impl(m, n, seed, offset, nthreads, out, NULL, 0, &L);
+}
+
+/* the reference's segment and event test kernels (CLARITAS.py:3470-3485)
+ * on explicit inputs: segments7 [n][7] = x0, y0, z0, vx, vy, vz, dist with
+ * particle order i & 1, xyz3 [n][3] event sites */
+int orc_light_segment_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* segments7,
+                            clr_light_out* lo) {
+  light_t L = {lc, lo, rin, zmin, (float)lc->zmax};
+  for (int i = 0; i < n; ++i) {
+    const float* q = segments7 + 7 * (size_t)i;
+    light_segment(&L, (uint64_t)i, q[0], q[1], q[2], q[3], q[4], q[5], q[6], i & 1);
+  }
+  return 0;
+}
+int orc_light_event_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* xyz3,
+                          clr_light_out* lo) {
+  light_t L = {lc, lo, rin, zmin, (float)lc->zmax};
+  for (int i = 0; i < n; ++i) {
+    const float* q = xyz3 + 3 * (size_t)i;
+    light_site(&L, (uint64_t)i, q[0], q[1], q[2], lo->scatter_event_weight, lo->scatter_event_count,
+               &lo->scatter_event_total);
+  }
   return 0;
 }
 

@@ -40,6 +40,19 @@ int orc_set_phase_table(orc_model* m, const double* theta, const double* cdf, in
 int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t history_offset,
             int nthreads, clr_tallies* out, clr_record* records, uint64_t n_records);
 
+/* orc_run plus the light-transport voxel diagnostics (clr_run_light
+ * contract; the oracle initialises the caller's output arrays). */
+int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+                  const clr_light_config* lc, clr_tallies* out, clr_light_out* lo);
+/* The reference's diagnostic test kernels (CLARITAS.py:3470-3485): segment i
+ * of segments7 [n][7] (x0, y0, z0, vx, vy, vz, dist; particle order i & 1,
+ * history id i) and event sites xyz3 [n][3], deposited into lo (which the
+ * caller zeroes; max_rays, representative_rays and max_segments from lc). */
+int orc_light_segment_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* segments7,
+                            clr_light_out* lo);
+int orc_light_event_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* xyz3,
+                          clr_light_out* lo);
+
 /* Geometry probes on the reference's test-kernel semantics
  * (free_surface_geometry_test_kernel, CLARITAS.py:4438-4455). */
 int orc_free_surface_probe(const clr_config* cfg, const float* rays6, int n,

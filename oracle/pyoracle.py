@@ -74,6 +74,13 @@ def load():
     lib.orc_meridional_solve.argtypes = [P(abi.Config), P(abi.Particles), P(abi.Sediment), P(abi.Meridional),
                                          P(abi.MeridionalOut)]
     lib.orc_meridional_solve.restype = C.c_int
+    lib.orc_run_light.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(abi.LightConfig),
+                                  P(abi.Tallies), P(abi.LightOut)]
+    lib.orc_run_light.restype = C.c_int
+    for name in ('orc_light_segment_probe', 'orc_light_event_probe'):
+        fn = getattr(lib, name)
+        fn.argtypes = [P(abi.LightConfig), C.c_float, C.c_float, C.c_int, P(C.c_float), P(abi.LightOut)]
+        fn.restype = C.c_int
     _lib = lib
     return lib
 
@@ -263,6 +270,18 @@ class OracleModel:
         recs = abi.records_to_numpy(rec, int(n_records)) if n_records else None
         return abi.tallies_to_numpy(t), recs
 
+    def run_light(self, n, seed, light_config, history_offset=0, threads=0):
+        """orc_run_light: (tallies dict, arrays dict, scalars dict) with the
+        clr_run_light contract."""
+        lc = light_config
+        out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments)
+        t = abi.Tallies()
+        rc = self.lib.orc_run_light(self.h, int(n), int(seed), int(history_offset), int(threads), C.byref(lc),
+                                    C.byref(t), C.byref(out))
+        if rc:
+            raise RuntimeError(f'oracle light run failed with code {rc}')
+        return abi.tallies_to_numpy(t), arrs, abi.light_scalars(out)
+
     def sediment_scale(self, bin_index, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float32)
         out = np.zeros(len(xyz))
@@ -343,3 +362,19 @@ def meridional_solve(model, material, coeff=True, settings=None):
     if rc and rc != -7:
         raise RuntimeError(f'oracle meridional solve failed with code {rc}')
     return result_dict(out, arrs, rc)
+
+
+# ----------------------------------------------------------- light probes
+def light_probe(mode, data, light_config, rin, zmin):
+    """The reference's diagnostic test kernels on the oracle: mode 0 deposits
+    segments [n][7], mode 1 event sites [n][3]; returns (arrays, scalars)."""
+    lc = light_config
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments)
+    for k in ('traj_x0', 'traj_y0', 'traj_z0', 'traj_x1', 'traj_y1', 'traj_z1'):
+        arrs[k][:] = np.nan
+    for k in ('traj_order', 'traj_event'):
+        arrs[k][:] = -1
+    fn = load().orc_light_segment_probe if mode == 0 else load().orc_light_event_probe
+    fn(C.byref(lc), float(rin), float(zmin), int(data.shape[0]), _pf(data), C.byref(out))
+    return arrs, abi.light_scalars(out)

@@ -33,6 +33,10 @@ _SIGS = {
     'clr_run_batch': (C.c_int, [C.c_void_p, P(abi.BatchItem), C.c_int, P(abi.Launch), P(abi.Tallies)]),
     'clr_time_run': (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, P(abi.Launch), C.c_int, P(C.c_double)]),
     'clr_last_kernel_ms': (C.c_double, [C.c_void_p]),
+    'clr_run_light': (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, P(abi.Launch), P(abi.LightConfig),
+                                P(abi.Tallies), P(abi.LightOut)]),
+    'clr_light_probe': (C.c_int, [C.c_void_p, C.c_int, C.c_int, P(C.c_float), P(abi.LightConfig), C.c_double,
+                                  C.c_double, P(abi.LightOut)]),
     'clr_meridional_solve': (C.c_int, [C.c_void_p, P(abi.Config), P(abi.Particles), P(abi.Sediment),
                                        P(abi.Meridional), P(abi.MeridionalOut)]),
     'clr_mlp_init': (C.c_int, [C.c_void_p, P(abi.MlpShape), C.c_uint64]),

@@ -204,9 +204,59 @@ def meridional_out(n, nr, nz, coeff=False):
     return o, arrs
 
 
+PF = C.POINTER(C.c_float)
+PU32 = C.POINTER(C.c_uint32)
+
+
+class LightConfig(C.Structure):
+    _fields_ = [('max_rays', U64), ('nx', I32), ('ny', I32), ('nz', I32), ('representative_rays', I32),
+                ('max_segments', I32), ('pad_', I32), ('zmax', D)]
+
+
+class LightOut(C.Structure):
+    _fields_ = [('track_length', PF), ('post_scatter_track_length', PF), ('scatter_event_weight', PF),
+                ('scatter_event_count', PU32), ('absorption_event_count', PU32),
+                ('traj_x0', PF), ('traj_y0', PF), ('traj_z0', PF), ('traj_x1', PF), ('traj_y1', PF), ('traj_z1', PF),
+                ('traj_order', PI), ('traj_event', PI), ('traj_count', PI), ('traj_overflow', PI),
+                ('segment_total', D), ('deposited_total', D), ('segment_count', U64),
+                ('scatter_event_total', U64), ('absorption_event_total', U64), ('error_count', C.c_uint32),
+                ('boundary_adjustment_count', C.c_uint32), ('outside', C.c_uint32 * 8)]
+
+
+LIGHT_GRIDS = ('track_length', 'post_scatter_track_length', 'scatter_event_weight', 'scatter_event_count',
+               'absorption_event_count')
+LIGHT_TRAJ = ('traj_x0', 'traj_y0', 'traj_z0', 'traj_x1', 'traj_y1', 'traj_z1', 'traj_order', 'traj_event')
+LIGHT_SCALARS = ('segment_total', 'deposited_total', 'segment_count', 'scatter_event_total',
+                 'absorption_event_total', 'error_count', 'boundary_adjustment_count')
+
+
+def light_out(nx, ny, nz, rep, max_segments):
+    """Allocate host outputs of one light-diagnostic call; returns (struct, arrays)."""
+    import numpy as np
+
+    arrs = {}
+    for k in LIGHT_GRIDS:
+        arrs[k] = np.zeros((nz, ny, nx), dtype=np.uint32 if k.endswith('count') else np.float32)
+    for k in LIGHT_TRAJ:
+        arrs[k] = np.zeros((rep, max_segments), dtype=np.int32 if k in ('traj_order', 'traj_event') else np.float32)
+    arrs['traj_count'] = np.zeros(rep, dtype=np.int32)
+    arrs['traj_overflow'] = np.zeros(rep, dtype=np.int32)
+    o = LightOut()
+    for k, v in arrs.items():
+        ct = {np.dtype(np.float32): PF, np.dtype(np.uint32): PU32, np.dtype(np.int32): PI}[v.dtype]
+        setattr(o, k, v.ctypes.data_as(ct) if v.size else None)
+    return o, arrs
+
+
+def light_scalars(o: LightOut) -> dict:
+    d = {k: getattr(o, k) for k in LIGHT_SCALARS}
+    d['outside'] = [int(v) for v in o.outside]
+    return d
+
+
 STRUCT_IDS = {
     0: Config, 1: Particles, 2: Sediment, 3: MieOut, 4: TablesOut, 5: Tallies, 6: Record,
-    7: BatchItem, 8: Launch, 9: Meridional, 10: MeridionalOut,
+    7: BatchItem, 8: Launch, 9: Meridional, 10: MeridionalOut, 11: LightConfig, 12: LightOut,
 }
 
 

@@ -90,6 +90,11 @@ struct GraphCache {
 
 }  // namespace
 
+// Device buffers of the light-transport diagnostics (grown on demand).
+struct LightBufs {
+  DevBuf grids, scalars, traj, desc, probe;
+};
+
 struct clr_ctx {
   int device = 0;
   int n_cu = 256;
@@ -101,6 +106,7 @@ struct clr_ctx {
   double last_ms = 0.0;
   GraphCache graph;
   MlpState* mlp = nullptr;
+  LightBufs light;
 };
 
 namespace {
@@ -532,7 +538,7 @@ int kernel_kind(clr_ctx* c, const std::vector<int>& used) {
 // Upload items, clear tallies, launch the persistent kernel (optionally
 // through a cached hipGraph), time it with events on the library stream.
 int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape& sh, bool use_graph,
-                 clr_record* d_records, uint64_t n_records) {
+                 clr_record* d_records, uint64_t n_records, const LightDiag* light = nullptr) {
   const int n_slots = (int)c->slots.size();
   HIPCHK(c, c->items.ensure(sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
   HIPCHK(c, c->counter.ensure(sizeof(unsigned int)));
@@ -548,6 +554,7 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   L.grid = sh.grid;
   L.block = sh.block;
   L.kind = sh.kind;
+  L.light = light;
   const size_t words = sizeof(clr_tallies) * n_slots / 8;
   if (use_graph) {
     GraphCache& G = c->graph;
@@ -582,6 +589,147 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   return 0;
 }
 
+// Validate a light-diagnostic configuration, size and clear its device
+// buffers and upload the descriptor (CLARITAS.py:7232-7300).
+int light_setup(clr_ctx* c, const clr_light_config* lc, float rin, float zmin, LightDiag& D) {
+  if (lc->max_rays < 1) return fail(c, CLR_E_ARG, "light diagnostic max_rays must be >=1");
+  if (std::min(lc->nx, std::min(lc->ny, lc->nz)) < 4)
+    return fail(c, CLR_E_ARG, "light diagnostic grid dimensions must each be >=4");
+  if (lc->representative_rays < 0 || (uint64_t)lc->representative_rays > lc->max_rays || lc->max_segments < 1)
+    return fail(c, CLR_E_ARG, "invalid representative-ray diagnostic configuration");
+  if (!(lc->zmax > (double)zmin) || !(rin > 0.0f)) return fail(c, CLR_E_ARG, "light diagnostic box is empty");
+  const size_t vox = (size_t)lc->nx * lc->ny * lc->nz, rep = (size_t)lc->representative_rays;
+  const size_t traj = std::max<size_t>(rep * lc->max_segments, 1), rep1 = std::max<size_t>(rep, 1);
+  const double bytes = (double)vox * 20 + (double)traj * 32 + (double)rep1 * 8 + 8;
+  if (bytes > 1e9)
+    return fail(c, CLR_E_RANGE, "light diagnostic allocation is too large; reduce grid dimensions, "
+                                "representative rays, or segments per ray");
+  HIPCHK(c, hipSetDevice(c->device));
+  LightBufs& B = c->light;
+  HIPCHK(c, B.grids.ensure(vox * 20));
+  HIPCHK(c, B.scalars.ensure(128));
+  HIPCHK(c, B.traj.ensure(traj * 32 + rep1 * 8));
+  HIPCHK(c, B.desc.ensure(sizeof(LightDiag)));
+  char* gp = B.grids.as<char>();
+  char* sp = B.scalars.as<char>();
+  D = LightDiag{};
+  D.total = reinterpret_cast<float*>(gp);
+  D.post = D.total + vox;
+  D.event_weight = D.post + vox;
+  D.event_count = reinterpret_cast<unsigned int*>(D.event_weight + vox);
+  D.absorption_count = D.event_count + vox;
+  // scalars: 2 doubles, 3 u64, error and snap counts, outside[8] (80 bytes)
+  D.segment_total = reinterpret_cast<double*>(sp);
+  D.deposited_total = D.segment_total + 1;
+  D.segment_count = reinterpret_cast<unsigned long long*>(sp + 16);
+  D.event_total = D.segment_count + 1;
+  D.absorption_total = D.segment_count + 2;
+  D.error_count = reinterpret_cast<unsigned int*>(sp + 40);
+  D.boundary_adjust = D.error_count + 1;
+  D.outside = D.error_count + 2;
+  float* tf = B.traj.as<float>();
+  D.tx0 = tf; D.ty0 = tf + traj; D.tz0 = tf + 2 * traj;
+  D.tx1 = tf + 3 * traj; D.ty1 = tf + 4 * traj; D.tz1 = tf + 5 * traj;
+  D.torder = reinterpret_cast<int*>(tf + 6 * traj);
+  D.tevent = D.torder + traj;
+  D.tcount = D.tevent + traj;
+  D.toverflow = D.tcount + rep1;
+  D.max_rays = lc->max_rays;
+  D.nx = lc->nx; D.ny = lc->ny; D.nz = lc->nz;
+  D.rep_rays = lc->representative_rays; D.max_segments = lc->max_segments;
+  D.rin = rin; D.zmin = zmin; D.zmax = (float)lc->zmax;
+  HIPCHK(c, hipMemsetAsync(gp, 0, vox * 20, c->stream));
+  HIPCHK(c, hipMemsetAsync(sp, 0, 128, c->stream));
+  HIPCHK(c, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(tf), 0x7fc00000, traj * 6, c->stream));  // NaN
+  HIPCHK(c, hipMemsetAsync(D.torder, 0xff, traj * 8, c->stream));                                         // -1
+  HIPCHK(c, hipMemsetAsync(D.tcount, 0, rep1 * 8, c->stream));
+  HIPCHK(c, hipMemcpyAsync(B.desc.p, &D, sizeof(LightDiag), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// Copy the diagnostics of D into the caller's arrays (NULL ones skipped).
+int light_readback(clr_ctx* c, const clr_light_config* lc, const LightDiag& D, clr_light_out* lo) {
+  const size_t vox = (size_t)lc->nx * lc->ny * lc->nz, rep = (size_t)lc->representative_rays;
+  auto get = [&](void* dst, const void* src, size_t nb) -> hipError_t {
+    return dst ? hipMemcpyAsync(dst, src, nb, hipMemcpyDeviceToHost, c->stream) : hipSuccess;
+  };
+  HIPCHK(c, get(lo->track_length, D.total, vox * 4));
+  HIPCHK(c, get(lo->post_scatter_track_length, D.post, vox * 4));
+  HIPCHK(c, get(lo->scatter_event_weight, D.event_weight, vox * 4));
+  HIPCHK(c, get(lo->scatter_event_count, D.event_count, vox * 4));
+  HIPCHK(c, get(lo->absorption_event_count, D.absorption_count, vox * 4));
+  if (rep) {
+    const size_t tb = rep * lc->max_segments;
+    HIPCHK(c, get(lo->traj_x0, D.tx0, tb * 4)); HIPCHK(c, get(lo->traj_y0, D.ty0, tb * 4));
+    HIPCHK(c, get(lo->traj_z0, D.tz0, tb * 4)); HIPCHK(c, get(lo->traj_x1, D.tx1, tb * 4));
+    HIPCHK(c, get(lo->traj_y1, D.ty1, tb * 4)); HIPCHK(c, get(lo->traj_z1, D.tz1, tb * 4));
+    HIPCHK(c, get(lo->traj_order, D.torder, tb * 4)); HIPCHK(c, get(lo->traj_event, D.tevent, tb * 4));
+    HIPCHK(c, get(lo->traj_count, D.tcount, rep * 4)); HIPCHK(c, get(lo->traj_overflow, D.toverflow, rep * 4));
+  }
+  unsigned char sc[80];
+  HIPCHK(c, hipMemcpyAsync(sc, D.segment_total, 80, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(&lo->segment_total, sc, 8);
+  std::memcpy(&lo->deposited_total, sc + 8, 8);
+  std::memcpy(&lo->segment_count, sc + 16, 8);
+  std::memcpy(&lo->scatter_event_total, sc + 24, 8);
+  std::memcpy(&lo->absorption_event_total, sc + 32, 8);
+  std::memcpy(&lo->error_count, sc + 40, 4);
+  std::memcpy(&lo->boundary_adjustment_count, sc + 44, 4);
+  std::memcpy(lo->outside, sc + 48, 32);
+  return 0;
+}
+
+// Trace n histories of slot 0 on kernel variant `kind` (with the light
+// diagnostics when `light` is given) and reduce the tallies into *out.
+int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_launch* launch, clr_tallies* out,
+               clr_record* records, uint64_t n_records, int kind, const LightDiag* light) {
+  if (n == 0) return fail(c, CLR_E_ARG, "n must be positive");
+  if (n > (1ull << 56) || offset > (1ull << 62)) return fail(c, CLR_E_RANGE, "history range too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  const RunShape sh = pick_shape(c, launch, n, kind);
+  // one launch covers at most 2^31 histories so LDS u32 tallies cannot wrap
+  const uint64_t per_launch = 1ull << 31;
+  const int n_slots = (int)c->slots.size();
+  std::vector<clr_tallies> acc(1);
+  memset(&acc[0], 0, sizeof(clr_tallies));
+  std::vector<clr_tallies> host(n_slots);
+  clr_record* d_rec = nullptr;
+  const uint64_t nrec = std::min<uint64_t>(n_records, n);
+  if (records && nrec) {
+    HIPCHK(c, c->records.ensure(sizeof(clr_record) * nrec));
+    d_rec = c->records.as<clr_record>();
+  }
+  double total_ms = 0.0;
+  const int G = c->slots[0]->cfg.n_groups;
+  for (uint64_t done = 0; done < n; done += per_launch) {
+    const uint64_t m = std::min(per_launch, n - done);
+    // histories of this launch keep their global group by numbering them
+    // within the whole run: items carry start offsets into [0, n)
+    QueueBuilder q(sh, (double)m);
+    q.add(0, done, m, n, seed, offset);
+    if (int e = launch_items(c, q.items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0, light))
+      return e;
+    total_ms += c->last_ms;
+    HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&host[0]);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&acc[0]);
+    // every field before incidence_sum and after it is u64; add field-wise
+    for (size_t k = 0; k < offsetof(clr_tallies, incidence_sum) / 8; ++k) dst[k] += src[k];
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < CLR_N_CHANNELS; ++j) acc[0].incidence_sum[k][j] += host[0].incidence_sum[k][j];
+    const size_t a0 = offsetof(clr_tallies, status) / 8, a1 = offsetof(clr_tallies, n_groups) / 8;
+    for (size_t k = a0; k < a1; ++k) dst[k] += src[k];
+  }
+  c->last_ms = total_ms;
+  *out = acc[0];
+  out->n_groups = G;
+  group_sizes(n, G, out->group_histories);
+  out->counters[CLR_C_HISTORIES] = n;  // launched histories (the device counts per-event work only)
+  if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -601,6 +749,8 @@ size_t clr_sizeof(int id) {
     case 8: return sizeof(clr_launch);
     case 9: return sizeof(clr_meridional);
     case 10: return sizeof(clr_meridional_out);
+    case 11: return sizeof(clr_light_config);
+    case 12: return sizeof(clr_light_out);
     default: return 0;
   }
 }
@@ -630,7 +780,9 @@ void clr_destroy(clr_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->graph.reset();
   for (Slot* s : c->slots) { s->release(); delete s; }
-  DevBuf* bufs[] = {&c->params, &c->tallies, &c->items, &c->counter, &c->records, &c->scratch_u32, &c->mie_work};
+  DevBuf* bufs[] = {&c->params,      &c->tallies,      &c->items,         &c->counter,         &c->records,
+                    &c->scratch_u32, &c->mie_work,     &c->light.grids,   &c->light.scalars,   &c->light.traj,
+                    &c->light.desc,  &c->light.probe};
   for (DevBuf* b : bufs) b->release();
   clr_mlp_free(c->mlp);
   (void)hipEventDestroy(c->ev0);
@@ -791,49 +943,32 @@ int clr_run(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_la
             clr_record* records, uint64_t n_records) {
   if (!c || !out) return fail(c, CLR_E_ARG, "null argument");
   if (c->slots.empty() || !c->slots[0]->ready) return fail(c, CLR_E_STATE, "clr_prepare(slot 0) has not run");
-  if (n == 0) return fail(c, CLR_E_ARG, "n must be positive");
-  if (n > (1ull << 56) || offset > (1ull << 62)) return fail(c, CLR_E_RANGE, "history range too large");
+  return run_single(c, n, seed, offset, launch, out, records, n_records, kernel_kind(c, {0}), nullptr);
+}
+
+int clr_run_light(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_launch* launch,
+                  const clr_light_config* lc, clr_tallies* out, clr_light_out* lo) {
+  if (!c || !out || !lc || !lo) return fail(c, CLR_E_ARG, "null argument");
+  if (c->slots.empty() || !c->slots[0]->ready) return fail(c, CLR_E_STATE, "clr_prepare(slot 0) has not run");
+  const clr_config& g = c->slots[0]->cfg;
+  LightDiag D{};
+  if (int e = light_setup(c, lc, (float)g.rin, (float)g.zmin, D)) return e;
+  if (int e = run_single(c, n, seed, offset, launch, out, nullptr, 0, CLR_KERNEL_LIGHT, c->light.desc.as<LightDiag>()))
+    return e;
+  return light_readback(c, lc, D, lo);
+}
+
+int clr_light_probe(clr_ctx* c, int mode, int n, const float* data, const clr_light_config* lc, double rin,
+                    double zmin, clr_light_out* lo) {
+  if (!c || !data || !lc || !lo || n <= 0 || (mode != 0 && mode != 1)) return fail(c, CLR_E_ARG, "null argument");
   HIPCHK(c, hipSetDevice(c->device));
-  const RunShape sh = pick_shape(c, launch, n, kernel_kind(c, {0}));
-  // one launch covers at most 2^31 histories so LDS u32 tallies cannot wrap
-  const uint64_t per_launch = 1ull << 31;
-  const int n_slots = (int)c->slots.size();
-  std::vector<clr_tallies> acc(1);
-  memset(&acc[0], 0, sizeof(clr_tallies));
-  std::vector<clr_tallies> host(n_slots);
-  clr_record* d_rec = nullptr;
-  const uint64_t nrec = std::min<uint64_t>(n_records, n);
-  if (records && nrec) {
-    HIPCHK(c, c->records.ensure(sizeof(clr_record) * nrec));
-    d_rec = c->records.as<clr_record>();
-  }
-  double total_ms = 0.0;
-  const int G = c->slots[0]->cfg.n_groups;
-  for (uint64_t done = 0; done < n; done += per_launch) {
-    const uint64_t m = std::min(per_launch, n - done);
-    // histories of this launch keep their global group by numbering them
-    // within the whole run: items carry start offsets into [0, n)
-    QueueBuilder q(sh, (double)m);
-    q.add(0, done, m, n, seed, offset);
-    if (int e = launch_items(c, q.items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0)) return e;
-    total_ms += c->last_ms;
-    HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(&host[0]);
-    uint64_t* dst = reinterpret_cast<uint64_t*>(&acc[0]);
-    // every field before incidence_sum and after it is u64; add field-wise
-    for (size_t k = 0; k < offsetof(clr_tallies, incidence_sum) / 8; ++k) dst[k] += src[k];
-    for (int k = 0; k < 2; ++k)
-      for (int j = 0; j < CLR_N_CHANNELS; ++j) acc[0].incidence_sum[k][j] += host[0].incidence_sum[k][j];
-    const size_t a0 = offsetof(clr_tallies, status) / 8, a1 = offsetof(clr_tallies, n_groups) / 8;
-    for (size_t k = a0; k < a1; ++k) dst[k] += src[k];
-  }
-  c->last_ms = total_ms;
-  *out = acc[0];
-  out->n_groups = G;
-  group_sizes(n, G, out->group_histories);
-  out->counters[CLR_C_HISTORIES] = n;  // launched histories (the device counts per-event work only)
-  if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
-  return 0;
+  LightDiag D{};
+  if (int e = light_setup(c, lc, (float)rin, (float)zmin, D)) return e;
+  const size_t nb = sizeof(float) * (size_t)n * (mode == 0 ? 7 : 3);
+  HIPCHK(c, c->light.probe.ensure(nb));
+  HIPCHK(c, hipMemcpyAsync(c->light.probe.p, data, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, clr_launch_light_probe(c->light.desc.as<LightDiag>(), c->light.probe.as<float>(), n, mode, c->stream));
+  return light_readback(c, lc, D, lo);
 }
 
 int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_launch* launch, clr_tallies* out) {

@@ -137,19 +137,38 @@ __device__ double schiller_naumann(double v0, double d, double rho_w, double mu)
   return v;
 }
 
-// One thread per bin: settling speed, radial log potential (lookup table),
-// vertical decay B = w_s / D_t and the volume normalisation.
-__global__ void sediment_bins_kernel(PrepLaunch L) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per bin: settling speed, radial slip and log-potential
+// tables, vertical decay B = w_s / D_t and the volume normalisation
+// (CLARITAS.py:3940-4031).  The radial points' Newton solves run across the
+// workgroup; the trapezoid integral of the slip is a chunked scan (each
+// thread sums a contiguous chunk, one thread scans the chunk sums, each
+// thread then writes its chunk); the normalisation quadrature is a
+// workgroup reduction.
+constexpr int kSedThreads = 256;
+__device__ __forceinline__ double sed_block_sum(double v, double* red) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kSedThreads / 64; ++w) s += red[w];
+  return s;
+}
+__global__ __launch_bounds__(kSedThreads) void sediment_bins_kernel(PrepLaunch L) {
+  __shared__ double part[kSedThreads];
+  __shared__ double red[kSedThreads / 64];
+  __shared__ double s_total;
+  const int i = blockIdx.x, tid = threadIdx.x;
   if (i >= L.n) return;
   const int tm = L.sed_model, fsm = L.fs_model, nrad = L.n_radial, NQ = L.n_quad;
   const double R = L.R, R2 = R * R, a2 = L.core * L.core, DH = L.dh;
   double* rl = L.radlog + (size_t)i * nrad;
   double* wrt = L.radwr ? L.radwr + (size_t)i * nrad : nullptr;
   if (tm != CLR_SED_STOKES && tm != CLR_SED_SCHILLER && tm != CLR_SED_GRID_PER_BIN) {
-    L.sedA[i] = 0.0; L.sedB[i] = 0.0; L.sednorm[i] = 1.0; L.sedws[i] = 0.0;
-    for (int j = 0; j < nrad; ++j) rl[j] = 0.0;
-    if (wrt) for (int j = 0; j < nrad; ++j) wrt[j] = 0.0;
+    if (tid == 0) { L.sedA[i] = 0.0; L.sedB[i] = 0.0; L.sednorm[i] = 1.0; L.sedws[i] = 0.0; }
+    for (int j = tid; j < nrad; j += kSedThreads) {
+      rl[j] = 0.0;
+      if (wrt) wrt[j] = 0.0;
+    }
     return;
   }
   double Hinf = 0.0, hbar = 0.0, om2 = 0.0;
@@ -165,47 +184,55 @@ __global__ void sediment_bins_kernel(PrepLaunch L) {
   const bool finite_re = (tm != CLR_SED_STOKES);
   const double ws = finite_re ? schiller_naumann(ws0, d, rho_w, mu) : ws0;
   const double B = ws / L.eddy_diffusivity;
-  L.sedws[i] = ws;
-  L.sedB[i] = B;
   double A;
+  auto radial_acc = [&](double r) -> double {
+    if (!(r > 0.0)) return 0.0;
+    return (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
+  };
   if (!finite_re) {
     A = tau * om2 * ((fsm == 2) ? a2 : 1.0) / (2.0 * L.eddy_diffusivity);
-    for (int j = 0; j < nrad; ++j) {
+    for (int j = tid; j < nrad; j += kSedThreads) {
       const double r = (double)j * drad, rr2 = r * r, f = (fsm == 2) ? rr2 / (a2 + rr2) : rr2;
       rl[j] = A * (f - fR);
-      if (wrt) {
-        double acc = 0.0;
-        if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
-        wrt[j] = tau * acc;
-      }
+      if (wrt) wrt[j] = tau * radial_acc(r);
     }
   } else {
-    // trapezoid integral of the finite-Re radial slip; rl temporarily holds w_r
-    double total = 0.0, prev = 0.0;
-    for (int j = 0; j < nrad; ++j) {
-      const double r = (double)j * drad;
-      double acc = 0.0;
-      if (r > 0.0) acc = (fsm == 1) ? om2 * r : ((fsm == 2) ? om2 * r / pow(1.0 + r * r / a2, 2.0) : 0.0);
-      const double wr = schiller_naumann(tau * acc, d, rho_w, mu);
+    // finite-Re slip w_r at every radial point (rl holds w_r until the scan)
+    for (int j = tid; j < nrad; j += kSedThreads) {
+      const double wr = schiller_naumann(tau * radial_acc((double)j * drad), d, rho_w, mu);
       rl[j] = wr;
       if (wrt) wrt[j] = wr;
-      if (j > 0) total += 0.5 * (prev + wr) * drad;
-      prev = wr;
     }
-    double cum = 0.0;
-    prev = 0.0;
-    for (int j = 0; j < nrad; ++j) {
+    __syncthreads();
+    // psi(r_j) = [int_0^r_j w - int_0^R w] / D_t by the trapezoid rule
+    const int chunk = (nrad + kSedThreads - 1) / kSedThreads;
+    const int j0 = min(tid * chunk, nrad), j1 = min(j0 + chunk, nrad);
+    double loc = 0.0;
+    for (int j = max(j0, 1); j < j1; ++j) loc += 0.5 * (rl[j - 1] + rl[j]) * drad;
+    part[tid] = loc;
+    const double prev0 = (j0 > 0 && j0 < nrad) ? rl[j0 - 1] : 0.0;
+    __syncthreads();
+    if (tid == 0) {
+      double run = 0.0;
+      for (int t = 0; t < kSedThreads; ++t) { const double v = part[t]; part[t] = run; run += v; }
+      s_total = run;
+    }
+    __syncthreads();
+    const double total = s_total;
+    double cum = part[tid], prev = prev0;
+    for (int j = j0; j < j1; ++j) {
       const double wr = rl[j];
       if (j > 0) cum += 0.5 * (prev + wr) * drad;
       prev = wr;
       rl[j] = fmin((cum - total) / L.eddy_diffusivity, 0.0);
     }
-    rl[nrad - 1] = 0.0;
-    A = (fR > 0.0) ? -rl[0] / fR : 0.0;
+    __syncthreads();
+    if (tid == 0) rl[nrad - 1] = 0.0;
+    A = (fR > 0.0) ? -fmin(-total / L.eddy_diffusivity, 0.0) / fR : 0.0;
+    __syncthreads();
   }
-  L.sedA[i] = A;
   double integ = 0.0;
-  for (int j = 0; j < NQ; ++j) {
+  for (int j = tid; j < NQ; j += kSedThreads) {
     const double r = ((double)j + 0.5) * dr, rr2 = r * r;
     double zs = L.zmean;
     if (fsm == 1 && DH > 0.0) zs = L.zmean + DH * (rr2 / R2 - 0.5);
@@ -223,9 +250,12 @@ __global__ void sediment_bins_kernel(PrepLaunch L) {
     const double rad = exp(lr), iz = (fabs(B) < 1e-14) ? rad * H : rad * (-expm1(-B * H)) / B;
     integ += r * iz;
   }
-  const double norm = (2.0 / (R2 * Hmean)) * integ * dr;
-  if (!(norm > 0.0) || !isfinite(norm)) atomicExch(L.error_flag, 34);
-  L.sednorm[i] = norm;
+  integ = sed_block_sum(integ, red);
+  if (tid == 0) {
+    const double norm = (2.0 / (R2 * Hmean)) * integ * dr;
+    if (!(norm > 0.0) || !isfinite(norm)) atomicExch(L.error_flag, 34);
+    L.sedA[i] = A; L.sedB[i] = B; L.sedws[i] = ws; L.sednorm[i] = norm;
+  }
 }
 
 // maximum of a non-negative float field (IEEE order == integer order)
@@ -269,13 +299,13 @@ __global__ void majorant_kernel(PrepLaunch L, const unsigned int* __restrict__ f
 }  // namespace
 
 hipError_t clr_launch_sediment_bins(const PrepLaunch& L, hipStream_t s) {
-  hipLaunchKernelGGL(sediment_bins_kernel, dim3((L.n + 63) / 64), dim3(64), 0, s, L);
+  hipLaunchKernelGGL(sediment_bins_kernel, dim3(L.n), dim3(kSedThreads), 0, s, L);
   return hipGetLastError();
 }
 
 hipError_t clr_launch_precompute(const PrepLaunch& L, unsigned int* fmax_bits, hipStream_t s) {
   hipLaunchKernelGGL(transport_split_kernel, dim3(1), dim3(256), 0, s, L);
-  hipLaunchKernelGGL(sediment_bins_kernel, dim3((L.n + 63) / 64), dim3(64), 0, s, L);
+  hipLaunchKernelGGL(sediment_bins_kernel, dim3(L.n), dim3(kSedThreads), 0, s, L);
   if (L.field && L.field_len) {
     const size_t plane = (L.sed_model == CLR_SED_GRID_PER_BIN) ? L.field_len / (size_t)L.n : 0;
     size_t blocks = (L.field_len + 255) / 256;

@@ -68,6 +68,23 @@ struct WorkItem {
   uint64_t start, count, n_total, seed, offset;
 };
 
+// Light-transport diagnostic buffers (device memory), CLARITAS.py:7232-7300.
+// Voxel grids are [nz][ny][nx] over [-rin, rin]^2 x [zmin, zmax]; the
+// trajectory arrays are [rep_rays][max_segments].  Histories with id <
+// max_rays are recorded; the first rep_rays also store their segments.
+struct LightDiag {
+  float *total, *post, *event_weight;
+  unsigned int *event_count, *absorption_count;
+  double *segment_total, *deposited_total;
+  unsigned long long *segment_count, *event_total, *absorption_total;
+  unsigned int *error_count, *boundary_adjust, *outside;  // outside[8]
+  float *tx0, *ty0, *tz0, *tx1, *ty1, *tz1;
+  int *torder, *tevent, *tcount, *toverflow;
+  uint64_t max_rays;
+  int nx, ny, nz, rep_rays, max_segments, pad_;
+  float rin, zmin, zmax, pad2_;
+};
+
 struct TransportLaunch {
   const DevParams* params;   // [n_configs]
   const WorkItem* items;     // [n_items]
@@ -78,15 +95,20 @@ struct TransportLaunch {
   uint64_t n_records;
   int grid, block;
   int kind;   // kernel variant, CLR_KERNEL_*
+  const LightDiag* light;    // CLR_KERNEL_LIGHT only
 };
 
 // Kernel variants: PLAIN when every configuration uses the plain-Mie law and a
 // uniform or shared FP32 field; ROUGH when one needs the Smith microsurface
 // walk (rough particles) or the spheroid-morphology wave channel; GENERIC
 // otherwise.  The rare, FP64-heavy rough-surface code lives only in ROUGH, so
-// it costs the other variants no registers.
-enum : int { CLR_KERNEL_GENERIC = 0, CLR_KERNEL_PLAIN = 1, CLR_KERNEL_ROUGH = 2 };
+// it costs the other variants no registers.  LIGHT is ROUGH plus the
+// light-transport voxel diagnostics; it runs only when they are requested.
+enum : int { CLR_KERNEL_GENERIC = 0, CLR_KERNEL_PLAIN = 1, CLR_KERNEL_ROUGH = 2, CLR_KERNEL_LIGHT = 3 };
 
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s);
+// Diagnostic deposition of explicit inputs (mode 0 segments [n][7], mode 1
+// event sites [n][3]) into the buffers of D (device pointers).
+hipError_t clr_launch_light_probe(const LightDiag* D, const float* data, int n, int mode, hipStream_t s);
 // Resident workgroups per CU for `block` threads (occupancy query).
 int clr_transport_occupancy(int block, int kind);

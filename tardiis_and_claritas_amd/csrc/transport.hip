@@ -1935,6 +1935,158 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
 // (water_particle) and the boundary interaction (water_iface).  The wave
 // schedules each kind separately so divergent lanes do not serialise them.
 // majorant slab of height z (end slabs open)
+// ---- light-transport voxel diagnostics (LIGHT variant only) ----
+// Observational: no random numbers, no change to the photon; writes only
+// the diagnostic buffers.  The arithmetic types (FP64 traversal of FP32
+// segments, FP32 tolerances) follow CLARITAS.py:3312-3467 so the same
+// segments deposit the same lengths and count the same snaps and errors.
+
+// clip the parameter interval [t0, t1] of p + t d to lo <= . <= hi
+__device__ __forceinline__ bool light_clip(double p, double d, double lo, double hi, double& t0, double& t1) {
+  if (fabs(d) < 1.0e-30f) return p >= lo && p <= hi;
+  double a = (lo - p) / d, b = (hi - p) / d;
+  if (a > b) { const double q = a; a = b; b = q; }
+  t0 = fmax(t0, a);
+  t1 = fmin(t1, b);
+  return t1 >= t0;
+}
+
+// Exact track-length deposition of the water segment (x0 + t v, 0 < t <
+// dist) into the voxel grid by a 3-D DDA over its clipped part; `order` is
+// the number of particle events before it (post-scatter grid when > 0).
+// CLARITAS.py:3318-3400.
+__device__ void light_segment(const LightDiag& D, uint64_t gid, float x0, float y0, float z0, float vx, float vy,
+                              float vz, float dist, int order) {
+  if (gid >= D.max_rays || !(dist > 0.0f) || !isfinite(dist)) return;
+  const int nx = D.nx, ny = D.ny, nz = D.nz;
+  const float Rin = D.rin, zmin = D.zmin, zmax = D.zmax;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !(zmax > zmin)) { atomicAdd(D.error_count, 1u); return; }
+  const double ddist = (double)dist;
+  const double sx = (double)vx * ddist, sy = (double)vy * ddist, sz = (double)vz * ddist;
+  double c0 = 0.0, c1 = 1.0;
+  if (!light_clip(x0, sx, -Rin, Rin, c0, c1) || !light_clip(y0, sy, -Rin, Rin, c0, c1) ||
+      !light_clip(z0, sz, zmin, zmax, c0, c1))
+    return;
+  const double ax = x0 + sx * c0, ay = y0 + sy * c0, az = z0 + sz * c0;
+  const double bx = x0 + sx * c1, by = y0 + sy * c1, bz = z0 + sz * c1;
+  const double clipped = dist * (c1 - c0);
+  if (!(clipped > 0.0f)) return;
+  atomicAdd(D.segment_total, clipped);
+  atomicAdd(D.segment_count, 1ull);
+  if (gid < (uint64_t)D.rep_rays && D.max_segments > 0) {
+    const int g = (int)gid, slot = D.tcount[g];
+    if (slot < D.max_segments) {
+      const size_t q = (size_t)g * D.max_segments + slot;
+      D.tx0[q] = (float)ax; D.ty0[q] = (float)ay; D.tz0[q] = (float)az;
+      D.tx1[q] = (float)bx; D.ty1[q] = (float)by; D.tz1[q] = (float)bz;
+      D.torder[q] = order; D.tevent[q] = 0; D.tcount[g] = slot + 1;
+    } else {
+      D.toverflow[g] = 1;
+    }
+  }
+  const double xlo = -(double)Rin, ylo = -(double)Rin, zlo = (double)zmin;
+  const double hx = (2.0 * (double)Rin) / nx, hy = (2.0 * (double)Rin) / ny, hz = ((double)zmax - zlo) / nz;
+  int ix = min(max((int)floor((ax - xlo) / hx), 0), nx - 1);
+  int iy = min(max((int)floor((ay - ylo) / hy), 0), ny - 1);
+  int iz = min(max((int)floor((az - zlo) / hz), 0), nz - 1);
+  const double ex = bx - ax, ey = by - ay, ez = bz - az;
+  const int stx = (ex > 0.0f) ? 1 : ((ex < 0.0f) ? -1 : 0);
+  const int sty = (ey > 0.0f) ? 1 : ((ey < 0.0f) ? -1 : 0);
+  const int stz = (ez > 0.0f) ? 1 : ((ez < 0.0f) ? -1 : 0);
+  // parameter (fraction of the clipped segment) of the next face crossing per axis
+  auto face = [](double lo, int i, int st, double h, double a, double e) -> double {
+    return (st == 0) ? (double)kInfF : (lo + (double)(st > 0 ? i + 1 : i) * h - a) / e;
+  };
+  double tmx = fmax(face(xlo, ix, stx, hx, ax, ex), 0.0);
+  double tmy = fmax(face(ylo, iy, sty, hy, ay, ey), 0.0);
+  double tmz = fmax(face(zlo, iz, stz, hz, az, ez), 0.0);
+  double u = 0.0, local = 0.0;
+  const float tol = 2.0e-6f;
+  const int max_steps = nx + ny + nz + 8;
+  for (int k = 0; k < max_steps && u < 1.0f; ++k) {
+    if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || iz < 0 || iz >= nz) {
+      if (u >= 1.0f - tol) { atomicAdd(D.boundary_adjust, 1u); u = 1.0f; break; }
+      atomicAdd(D.error_count, 1u);
+      break;
+    }
+    double un = fmin(1.0, fmin(tmx, fmin(tmy, tmz)));
+    if (un < 1.0f && un >= 1.0f - tol) { un = 1.0f; atomicAdd(D.boundary_adjust, 1u); }
+    if (un < u) {
+      if (u - un <= tol) { un = u; atomicAdd(D.boundary_adjust, 1u); }
+      else { atomicAdd(D.error_count, 1u); break; }
+    }
+    const double len = (un - u) * clipped;
+    if (len > 0.0f) {
+      const int q = (iz * ny + iy) * nx + ix;
+      atomicAdd(D.total + q, (float)len);
+      if (order > 0) atomicAdd(D.post + q, (float)len);
+      local += len;
+    }
+    if (un >= 1.0f) { u = 1.0f; break; }
+    const double eps = 8.0 * 2.2204460492503131e-16 * fmax(1.0, fabs(un));
+    if (tmx <= un + eps) { ix += stx; tmx = face(xlo, ix, stx, hx, ax, ex); }
+    if (tmy <= un + eps) { iy += sty; tmy = face(ylo, iy, sty, hy, ay, ey); }
+    if (tmz <= un + eps) { iz += stz; tmz = face(zlo, iz, stz, hz, az, ez); }
+    u = un;
+  }
+  if (u < 1.0f - tol) atomicAdd(D.error_count, 1u);
+  atomicAdd(D.deposited_total, local);
+}
+
+// terminal event code of the history's last stored segment,
+// CLARITAS.py:3402-3407 (0 propagation/boundary, 1-3 geometric external
+// reflection / transmission / internal reflection, 4 coherent Mie, 5-7 the
+// same for morphology, 8 inaccessible placement, 9 absorption, 10 failure)
+__device__ __forceinline__ void light_mark(const LightDiag& D, uint64_t gid, int code) {
+  if (gid >= D.max_rays || gid >= (uint64_t)D.rep_rays || D.max_segments <= 0) return;
+  const int g = (int)gid, n = D.tcount[g];
+  if (n > 0 && n <= D.max_segments) D.tevent[(size_t)g * D.max_segments + n - 1] = code;
+}
+
+// one event site into a count grid (and the weight grid when given),
+// CLARITAS.py:3409-3467: sites beyond the box by more than the coordinate
+// tolerance are errors (the first is recorded), sites within it are
+// snapped to the edge voxel
+__device__ void light_site(const LightDiag& D, uint64_t gid, float x, float y, float z, float* weight,
+                           unsigned int* count, unsigned long long* total) {
+  if (gid >= D.max_rays) return;
+  atomicAdd(total, 1ull);
+  const int nx = D.nx, ny = D.ny, nz = D.nz;
+  const float Rin = D.rin, zmin = D.zmin, zmax = D.zmax;
+  const float ctol = 2.0e-6f * fmaxf(2.0f * Rin, zmax - zmin);
+  if (!isfinite(x) || !isfinite(y) || !isfinite(z) || x < -Rin - ctol || x > Rin + ctol || y < -Rin - ctol ||
+      y > Rin + ctol || z < zmin - ctol || z > zmax + ctol) {
+    const unsigned int prev = atomicAdd(D.outside, 1u);
+    if (prev == 0u) {
+      D.outside[1] = __float_as_uint(x); D.outside[2] = __float_as_uint(y); D.outside[3] = __float_as_uint(z);
+      D.outside[4] = (unsigned int)gid; D.outside[5] = __float_as_uint(ctol); D.outside[6] = 1u;
+    }
+    atomicAdd(D.error_count, 1u);
+    return;
+  }
+  int ix = (int)floorf(((x + Rin) / (2.0f * Rin)) * nx);
+  int iy = (int)floorf(((y + Rin) / (2.0f * Rin)) * ny);
+  int iz = (int)floorf(((z - zmin) / (zmax - zmin)) * nz);
+  bool snapped = false;
+  if (ix < 0) { ix = 0; snapped = true; } else if (ix >= nx) { ix = nx - 1; snapped = true; }
+  if (iy < 0) { iy = 0; snapped = true; } else if (iy >= ny) { iy = ny - 1; snapped = true; }
+  if (iz < 0) { iz = 0; snapped = true; } else if (iz >= nz) { iz = nz - 1; snapped = true; }
+  if (snapped) atomicAdd(D.boundary_adjust, 1u);
+  const int q = (iz * ny + iy) * nx + ix;
+  if (weight) atomicAdd(weight + q, 1.0f);
+  atomicAdd(count + q, 1u);
+}
+// a surviving particle event (scatter-event grids) and its code
+__device__ __forceinline__ void light_event(const LightDiag& D, uint64_t gid, float x, float y, float z, int code) {
+  light_site(D, gid, x, y, z, D.event_weight, D.event_count, D.event_total);
+  light_mark(D, gid, code);
+}
+// an accepted extinction encounter that ends in absorption, code 9
+__device__ __forceinline__ void light_absorb(const LightDiag& D, uint64_t gid, float x, float y, float z) {
+  light_site(D, gid, x, y, z, nullptr, D.absorption_count, D.absorption_total);
+  light_mark(D, gid, 9);
+}
+
 __device__ __forceinline__ int slab_of(const DevParams& P, float z) {
   const int j = (int)floorf((z - P.slab_z0) * P.slab_inv_w);
   return min(max(j, 0), P.n_slabs - 1);
@@ -1965,7 +2117,7 @@ __device__ __forceinline__ float slab_free_path(const DevParams& P, float z, flo
 }
 
 template <int kKind>
-__device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A) {
+__device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
   if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
   L.wstep++;
@@ -2010,6 +2162,8 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
     freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tbound) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
   const float travel = do_particle ? freep : tbound;
+  if constexpr (kKind == CLR_KERNEL_LIGHT)
+    light_segment(*D, rng_counter(L.r), L.x, L.y, L.z, L.vx, L.vy, L.vz, travel, L.ic);
   A.n_flight++;
   L.x += travel * L.vx; L.y += travel * L.vy; L.z += travel * L.vz;
   if (do_particle) { L.region = R_WPART; return; }
@@ -2019,9 +2173,13 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
 
 // the particle event at the end of a flight (CLARITAS.py:4774-4981)
 template <int kKind>
-__device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A) {
-  constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN), kRough = (kKind == CLR_KERNEL_ROUGH);
+__device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D) {
+  constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
+  constexpr bool kLight = (kKind == CLR_KERNEL_LIGHT), kRough = (kKind == CLR_KERNEL_ROUGH) || kLight;
   L.region = R_WATER;  // a terminate() below overrides
+  // event site and history id for the light-transport diagnostics
+  const float sx0 = L.x, sy0 = L.y, sz0 = L.z;
+  const uint64_t gid = kLight ? rng_counter(L.r) : 0ull;
   const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
   const float MU_TOTAL = P.mu_geom + P.mu_wave;
   const float MU_TRACK = P.mu_track;
@@ -2064,9 +2222,15 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   if (kPlain || em == CLR_EVENT_PLAIN_MIE) {
     const int b = select_bin(L.r, P, false, L.x, L.y, L.z, LG);
     A.n_abs++;
-    if (U(L.r) >= P.albedo[b]) { L.flags |= F_PABS; terminate(L, 5); return; }
+    if (U(L.r) >= P.albedo[b]) {
+      L.flags |= F_PABS;
+      if constexpr (kLight) light_absorb(*D, gid, sx0, sy0, sz0);
+      terminate(L, 5);
+      return;
+    }
     L.wic++; L.wcoh++;
     mie_scatter(L.r, P, b, L.vx, L.vy, L.vz);
+    if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, 4);
     return;
   }
   if constexpr (!kPlain) {
@@ -2090,6 +2254,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     if (coherent) {
       L.wcoh++;
       mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
+      if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, 4);
       return;
     }
     // morphology-wave event about the local azimuthal flow direction,
@@ -2122,6 +2287,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
       cbin = pidx; ctrials = P.cond_trials - 1; fallback = true;
     } else if (!accessible) {
       L.ic--; L.gic--; A.n_null++;
+      if constexpr (kLight) light_mark(*D, gid, 8);
       return;
     }
   }
@@ -2139,7 +2305,15 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     if (h.pc.pabs) L.flags |= F_PABS;
     if (fallback) L.gic--;
     if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
-    if (!fallback && h.status == 2) terminate(L, 5);
+    if (!fallback && h.status == 2) {
+      if constexpr (kLight) light_absorb(*D, gid, sx0, sy0, sz0);
+      terminate(L, 5);
+      return;
+    }
+    if constexpr (kLight) {
+      const int fres = h.pc.per ? 1 : (h.pc.pir ? 3 : 2);  // external reflection / internal reflection / transmission
+      light_event(*D, gid, sx0, sy0, sz0, (h.channel == 1) ? 4 : (h.channel == 2 ? fres + 4 : fres));
+    }
     return;
   }
   PCount pc{0, 0, 0, 0, 0u};
@@ -2156,8 +2330,20 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   if (pc.pir) L.flags |= F_PIR;
   if (pc.pabs) L.flags |= F_PABS;
   A.n_abs += pc.n_abs;
-  if (ps == 2) { terminate(L, 5); return; }
-  if (ps != 1) { terminate(L, 4, ppf ? ppf : PF_OTHER); return; }
+  if (ps == 2) {
+    if constexpr (kLight) light_mark(*D, gid, 9);
+    terminate(L, 5);
+    return;
+  }
+  if (ps != 1) {
+    if constexpr (kLight) light_mark(*D, gid, 10);
+    terminate(L, 4, ppf ? ppf : PF_OTHER);
+    return;
+  }
+  if constexpr (kLight) {
+    const int fres = pc.per ? 1 : (pc.pir ? 3 : 2);
+    light_event(*D, gid, sx0, sy0, sz0, dir_only ? fres + 4 : fres);
+  }
   }
   return;
 }
@@ -2510,8 +2696,8 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         }
       } else if (op == K_WATER) {
         if (alive && (L.region == R_WPART || L.region == R_WATER)) rng_topup(L.r);
-        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A);
-        if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A);
+        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A, Lp.light);
+        if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
       } else if (op == K_ACRYLIC) {
@@ -2542,11 +2728,35 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
 
 }  // namespace
 
+namespace {
+// the reference's diagnostic test kernels (CLARITAS.py:3470-3485): mode 0
+// deposits segment t of data[n][7] (x0, y0, z0, vx, vy, vz, dist) with
+// particle order t & 1; mode 1 records event site t of data[n][3]
+__global__ void light_probe_kernel(const LightDiag* __restrict__ D, const float* __restrict__ data, int n, int mode) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (mode == 0) {
+    const float* q = data + 7 * (size_t)t;
+    light_segment(*D, (uint64_t)t, q[0], q[1], q[2], q[3], q[4], q[5], q[6], t & 1);
+  } else {
+    const float* q = data + 3 * (size_t)t;
+    light_site(*D, (uint64_t)t, q[0], q[1], q[2], D->event_weight, D->event_count, D->event_total);
+  }
+}
+}  // namespace
+
+hipError_t clr_launch_light_probe(const LightDiag* D, const float* data, int n, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(light_probe_kernel, dim3((n + 63) / 64), dim3(64), 0, s, D, data, n, mode);
+  return hipGetLastError();
+}
+
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s) {
   if (L.kind == CLR_KERNEL_PLAIN)
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_PLAIN>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   else if (L.kind == CLR_KERNEL_ROUGH)
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_ROUGH>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+  else if (L.kind == CLR_KERNEL_LIGHT)
+    hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_LIGHT>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   else
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_GENERIC>, dim3(L.grid), dim3(L.block), 0, s, L, L.params,
                        L.items);
@@ -2560,6 +2770,8 @@ int clr_transport_occupancy(int block, int kind) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_PLAIN>, block, 0);
   else if (kind == CLR_KERNEL_ROUGH)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_ROUGH>, block, 0);
+  else if (kind == CLR_KERNEL_LIGHT)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_LIGHT>, block, 0);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_GENERIC>, block, 0);
   if (e != hipSuccess) return 1;

@@ -18,7 +18,7 @@ from typing import Optional
 
 import numpy as np
 
-from . import abi, meridional
+from . import abi, light, meridional
 from ._lib import Context
 from .meridional import MeridionalSettings
 from .physics import (DETECTOR_ANGLES_DEG, Material, reference_material, vortex_surface_geometry)
@@ -188,6 +188,7 @@ class ForwardResult:
     symmetry_variance_reduction: bool
     batch_diagnostics: Optional[list] = None
     particle_diagnostics: Optional[dict] = None
+    light_transport_diagnostics: Optional[dict] = None
     # ---- extensions (no reference field) ----
     aux_detector_angles_deg: Optional[np.ndarray] = None
     aux_native_hits: Optional[np.ndarray] = None
@@ -626,9 +627,14 @@ class ClaritasForwardModel:
         l.block, l.blocks_per_cu, l.chunk = (int(self.launch.get(k, 0)) for k in ('block', 'blocks_per_cu', 'chunk'))
         return C.byref(l)
 
-    def run_tallies(self, n, seed, history_offset=0, n_records=0):
+    def run_tallies(self, n, seed, history_offset=0, n_records=0, light=None):
         """Trace n histories of the prepared slot 0; returns (tallies dict,
-        records or None, device ms)."""
+        records or None, device ms).  With a light.LightAccumulator the run
+        also records the light-transport voxel diagnostics (same tallies)."""
+        if light is not None:
+            t = light.run(n, seed, history_offset, self._launch())
+            ms = float(self.ctx.lib.clr_last_kernel_ms(self.ctx.h))
+            return abi.tallies_to_numpy(t), None, ms
         t = abi.Tallies()
         rec = (abi.Record * int(n_records))() if n_records else None
         self.ctx.check(self.ctx.lib.clr_run(self.ctx.h, int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, int(history_offset),
@@ -642,12 +648,16 @@ class ClaritasForwardModel:
                  weight_mode='mass_fraction', collect_rays=False, heatmap_size=0, light_transport_config=None,
                  target_detector_score=None, min_rays=None, max_rays=None, min_rear_channel_score=None,
                  min_h170_score=None, stability_l1_tolerance=None, stability_window=3, ray_id_start=0):
-        """Signature of CLARITAS.py:7160-7179.  Per-ray outputs, heat maps and
-        light-transport voxel diagnostics are not produced (collect_rays,
-        heatmap_size and light_transport_config must be off)."""
-        if collect_rays or heatmap_size or (light_transport_config and light_transport_config.get('enabled')):
-            raise ValueError('per-ray outputs and light diagnostics are not produced by the device path')
+        """Signature of CLARITAS.py:7160-7179.  Per-ray outputs and heat maps
+        are not produced (collect_rays and heatmap_size must be off); the
+        light-transport voxel diagnostics are (light_transport_config with
+        'enabled': True; keys as CLARITAS.py:7232-7247)."""
+        if collect_rays or heatmap_size:
+            raise ValueError('per-ray outputs and heat maps are not produced by the device path')
         mat, cfg = self.prepare(material, concentration_g_per_L, weights, weight_mode)
+        lacc = None
+        if light_transport_config and light_transport_config.get('enabled'):
+            lacc = light.LightAccumulator(self, light_transport_config, int(n_rays), int(ray_id_start))
         requested = int(n_rays)
         ray_id_start = int(ray_id_start)
         if ray_id_start < 0:
@@ -656,7 +666,7 @@ class ClaritasForwardModel:
         if not adaptive:
             if requested < 1:
                 raise ValueError('number of rays must be positive')
-            T, _, ms = self.run_tallies(requested, seed, ray_id_start)
+            T, _, ms = self.run_tallies(requested, seed, ray_id_start, light=lacc)
             batches = [T['det'][g] for g in range(T['n_groups'])]
             group_n = T['group_histories']
             stop_reason, total = 'fixed_n_rays', requested
@@ -675,7 +685,7 @@ class ClaritasForwardModel:
             prev_cum, recent = None, []
             while total < maxN:
                 b = min(self.statistics_batch_rays, maxN - total)
-                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total)
+                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total, light=lacc)
                 ms += msb
                 d = Tb['det'].sum(axis=0)
                 batches.append(d)
@@ -700,7 +710,11 @@ class ClaritasForwardModel:
             T['n_groups'] = len(batches)
             det_sum = np.stack(batches).sum(axis=0)
             group_n = np.asarray(group_n)
-        return self._result(mat, T, det_sum, batches, group_n, requested, total, adaptive, stop_reason, ms)
+        res = self._result(mat, T, det_sum, batches, group_n, requested, total, adaptive, stop_reason, ms)
+        if lacc is not None:
+            res.light_transport_diagnostics = lacc.result(total)
+            light.check_closure(res.light_transport_diagnostics)
+        return res
 
     @staticmethod
     def _merge(a, b):

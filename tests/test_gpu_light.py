@@ -1,0 +1,195 @@
+"""GPU parity of the light-transport voxel diagnostics (clr_run_light,
+clr_light_probe) against the oracle.
+
+Bars: the reference's test vectors (validate_claritas.py:1290-1402) on the
+device probes; device and oracle deposition of the same random segments and
+sites agree per voxel within FP32 accumulation order (1e-6 relative), with
+equal counts, snaps and errors; history by history on config 1 (the oracle
+traces the same histories) the totals agree within 1e-4, the grids within
+1e-3 in L1, event/absorption counts exactly up to histories that differ
+(< 0.1 %), and >= 99 % of representative trajectories are identical;
+tallies of clr_run_light equal clr_run's; on config 2 (the device tracks
+with slab majorants, the oracle with one) the track-length profile agrees
+bin by bin within 5 standard errors (32 batches a side).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from tardiis_and_claritas_amd import abi, light, workloads
+from tardiis_and_claritas_amd.model import ClaritasForwardModel
+
+pytestmark = pytest.mark.gpu
+
+R, ZMIN, ZMAX = 0.0465, -0.093, 0.074
+
+
+def _flush(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _lc(nx, ny, nz, rep=0, seg=1, max_rays=10 ** 12, zmax=ZMAX):
+    lc = abi.LightConfig()
+    lc.max_rays = max_rays
+    lc.nx, lc.ny, lc.nz = nx, ny, nz
+    lc.representative_rays, lc.max_segments = rep, seg
+    lc.zmax = zmax
+    return lc
+
+
+def _ctx():
+    return ClaritasForwardModel().ctx
+
+
+def test_probe_reference_vectors(gpu_available, oracle_lib):
+    _flush('light probe: reference vectors')
+    from test_oracle_light import reference_segments, reference_sites
+    ctx = _ctx()
+    seg = reference_segments()
+    a, sc = light.probe(ctx, 0, seg, _lc(12, 10, 14), R, ZMIN)
+    assert sc['error_count'] == 0 and sc['segment_count'] == len(seg)
+    expected = float(seg[:4, 6].sum(dtype=np.float64)) + 0.0165 + R + 2 * R + (ZMAX - ZMIN) + R
+    np.testing.assert_allclose(sc['segment_total'], expected, rtol=2e-7, atol=2e-8)
+    np.testing.assert_allclose(sc['deposited_total'], sc['segment_total'], rtol=2e-6, atol=2e-7)
+    np.testing.assert_allclose(a['track_length'].sum(dtype=np.float64), sc['segment_total'], rtol=2e-6, atol=2e-7)
+    assert np.all(a['post_scatter_track_length'] <= a['track_length'] + 1e-8)
+    o, osc = oracle_lib.light_probe(0, seg, _lc(12, 10, 14), R, ZMIN)
+    np.testing.assert_allclose(a['track_length'], o['track_length'], rtol=1e-6, atol=1e-10)
+    e, esc = light.probe(ctx, 1, reference_sites(), _lc(12, 10, 14), R, ZMIN)
+    assert esc['scatter_event_total'] == 5 and int(e['scatter_event_count'].sum()) == 4
+    assert esc['error_count'] == 1 and esc['outside'][0] == 1 and esc['boundary_adjustment_count'] >= 1
+    np.testing.assert_array_equal(e['scatter_event_weight'], e['scatter_event_count'].astype(np.float32))
+
+
+def test_probe_random_inputs_match_oracle(gpu_available, oracle_lib):
+    _flush('light probe: random segments and sites')
+    g = np.random.Generator(np.random.PCG64(5))
+    n = 4000
+    p0 = np.stack([g.uniform(-1.2 * R, 1.2 * R, n), g.uniform(-1.2 * R, 1.2 * R, n),
+                   g.uniform(ZMIN - 0.02, ZMAX + 0.02, n)], 1)
+    v = g.normal(size=(n, 3))
+    v[: n // 8, :2] = 0.0
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    seg = np.concatenate([p0, v, g.uniform(1e-5, 0.2, n)[:, None]], 1).astype(np.float32)
+    lc = _lc(20, 24, 28, rep=64, seg=4)
+    ctx = _ctx()
+    a, sc = light.probe(ctx, 0, seg, lc, R, ZMIN)
+    o, osc = oracle_lib.light_probe(0, seg, lc, R, ZMIN)
+    for k in ('segment_count', 'error_count', 'boundary_adjustment_count'):
+        assert sc[k] == osc[k], (k, sc[k], osc[k])
+    np.testing.assert_allclose(sc['segment_total'], osc['segment_total'], rtol=1e-12)
+    np.testing.assert_allclose(sc['deposited_total'], osc['deposited_total'], rtol=1e-12)
+    np.testing.assert_allclose(a['track_length'], o['track_length'], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(a['post_scatter_track_length'], o['post_scatter_track_length'], rtol=1e-6, atol=1e-9)
+    # history i < 64 stores its (single) segment in row i
+    for k in ('traj_x0', 'traj_y0', 'traj_z1', 'traj_order', 'traj_event', 'traj_count'):
+        np.testing.assert_array_equal(a[k], o[k])
+    sites = np.stack([g.uniform(-1.1 * R, 1.1 * R, n), g.uniform(-1.1 * R, 1.1 * R, n),
+                      g.uniform(ZMIN - 0.01, ZMAX + 0.01, n)], 1).astype(np.float32)
+    sites[:50, 0] = R + 1e-8
+    e, esc = light.probe(ctx, 1, sites, lc, R, ZMIN)
+    f, fsc = oracle_lib.light_probe(1, sites, lc, R, ZMIN)
+    np.testing.assert_array_equal(e['scatter_event_count'], f['scatter_event_count'])
+    for k in ('scatter_event_total', 'error_count', 'boundary_adjustment_count'):
+        assert esc[k] == fsc[k], k
+    assert esc['outside'][0] == fsc['outside'][0] > 0
+
+
+def _run_both(builder, n, lc, seed):
+    kw, mat, conc, _, _ = builder
+    m = ClaritasForwardModel(**kw)
+    m.prepare(mat, conc)
+    cfgd = {'enabled': True, 'max_rays': int(lc.max_rays), 'nx': lc.nx, 'ny': lc.ny, 'nz': lc.nz,
+            'representative_rays': lc.representative_rays, 'max_segments_per_ray': lc.max_segments}
+    acc = light.LightAccumulator(m, cfgd, n)
+    Tg, _, _ = m.run_tallies(n, seed, 0, light=acc)
+    return m, mat, conc, Tg, acc
+
+
+@pytest.mark.parametrize('law', ['plain_mie', 'hybrid_mie_exact_rate'])
+def test_history_by_history_config1(gpu_available, oracle_lib, law):
+    _flush(f'light history-by-history config1 {law}')
+    N = 20_000
+    builder = workloads.config1(law)
+    m0 = ClaritasForwardModel(**builder[0])
+    lc = _lc(32, 32, 40, rep=128, seg=96, max_rays=N, zmax=light.box_zmax(m0))
+    m, mat, conc, Tg, acc = _run_both(builder, N, lc, 91)
+    a, sc = acc.sum, acc.scalars
+    To, o, osc = oracle_lib.OracleModel(m, mat, conc).run_light(N, 91, lc)
+    assert sc['error_count'] == 0 and osc['error_count'] == 0
+    tol = 1e-3 * max(osc['segment_count'], 1) + 2
+    assert abs(sc['segment_count'] - osc['segment_count']) <= tol
+    np.testing.assert_allclose(sc['segment_total'], osc['segment_total'], rtol=1e-4)
+    l1 = np.abs(a['track_length'].astype(float) - o['track_length']).sum() / o['track_length'].sum()
+    assert l1 < 1e-3, l1
+    l1p = np.abs(a['post_scatter_track_length'].astype(float) - o['post_scatter_track_length']).sum()
+    assert l1p / max(o['post_scatter_track_length'].sum(), 1e-30) < 2e-3
+    for k in ('scatter_event_total', 'absorption_event_total'):
+        assert abs(sc[k] - osc[k]) <= 1e-3 * max(osc[k], 1) + 2, (k, sc[k], osc[k])
+    ec = np.abs(a['scatter_event_count'].astype(np.int64) - o['scatter_event_count']).sum()
+    assert ec <= 2e-3 * max(osc['scatter_event_total'], 1) + 4
+    same = 0
+    for r in range(lc.representative_rays):
+        k = a['traj_count'][r]
+        if k != o['traj_count'][r]:
+            continue
+        ok = np.array_equal(a['traj_event'][r], o['traj_event'][r]) and \
+            np.array_equal(a['traj_order'][r], o['traj_order'][r])
+        ok = ok and np.allclose(a['traj_x1'][r, :k], o['traj_x1'][r, :k], atol=2e-5)
+        same += int(ok)
+    assert same >= 0.99 * lc.representative_rays, same
+    # the diagnostics variant traces the same histories as the production one
+    Tp, _, _ = m.run_tallies(N, 91, 0)
+    d = np.abs(Tg['det'].sum(0) - Tp['det'].sum(0)).sum()
+    assert d <= 1e-3 * N, d
+    assert np.array_equal(Tg['status'], Tp['status']) or np.abs(Tg['status'] - Tp['status']).sum() <= 1e-3 * N
+
+
+def test_track_length_profile_config2_within_5_sigma(gpu_available, oracle_lib):
+    """Per-history track length is heavy tailed (a few long reflected paths),
+    so each side's standard error comes from 32 independent batches."""
+    _flush('light 5-sigma track-length profile config2')
+    kw, mat, conc, _, _ = workloads.config2(1000.0, 'plain_mie')
+    m = ClaritasForwardModel(**kw)
+    m.prepare(mat, conc)
+    lc = _lc(4, 4, 16, rep=0, seg=1, max_rays=10 ** 15, zmax=light.box_zmax(m))
+    cfgd = {'enabled': True, 'max_rays': 10 ** 15, 'nx': 4, 'ny': 4, 'nz': 16, 'representative_rays': 0,
+            'max_segments_per_ray': 1}
+    B, nb = 32, 6250
+    om = oracle_lib.OracleModel(m, mat, conc)
+    g, o = [], []
+    for b in range(B):
+        acc = light.LightAccumulator(m, cfgd, nb)
+        m.run_tallies(nb, 1234, b * nb, light=acc)
+        assert acc.scalars['error_count'] == 0
+        g.append(acc.sum['track_length'].astype(float).sum(axis=(1, 2)))
+        _, ob, osc = om.run_light(nb, 4321, lc, history_offset=b * nb)
+        assert osc['error_count'] == 0
+        o.append(ob['track_length'].astype(float).sum(axis=(1, 2)))
+    g, o = np.array(g), np.array(o)
+    se = np.sqrt((g.var(0, ddof=1) + o.var(0, ddof=1)) * B)
+    z = np.abs(g.sum(0) - o.sum(0)) / np.maximum(se, 1e-12)
+    assert np.all(z < 5.0), (z.max(), int(z.argmax()))
+
+
+def test_simulate_light_diagnostics(gpu_available, tmp_path):
+    _flush('simulate with light diagnostics')
+    kw, mat, conc, n, seed = workloads.config1('hybrid_mie_exact_rate')
+    m = ClaritasForwardModel(**kw)
+    cfgd = {'enabled': True, 'nx': 24, 'ny': 24, 'nz': 32, 'representative_rays': 32, 'max_segments_per_ray': 64}
+    r = m.simulate(mat, conc, n_rays=64_000, seed=seed, light_transport_config=cfgd)
+    d = r.light_transport_diagnostics
+    assert d is not None and d['diagnostic_rays_processed'] == 64_000
+    assert d['error_count'] == 0 and d['nonfinite_count'] == 0 and d['negative_value_count'] == 0
+    assert d['scatter_event_closure_error'] == 0 and d['absorption_event_closure_error'] == 0
+    assert d['deposition_closure_relative_error'] < 1e-9 and d['grid_closure_relative_error'] < 1e-4
+    assert d['track_length_m'].shape == (32, 24, 24)
+    s = light.write_outputs(tmp_path, d, m.sensor_height, cfgd)
+    assert (tmp_path / 'light_transport_volume.npz').exists()
+    assert s['representative_rays_with_segments'] > 0
+    # the same run without diagnostics gives the same detector tallies
+    r2 = m.simulate(mat, conc, n_rays=64_000, seed=seed)
+    assert np.abs(r.hardware_native_direct_hits - r2.hardware_native_direct_hits).sum() <= 64

@@ -155,3 +155,29 @@ def test_simulate_reports_solver(gpu_available):
     assert r.meridional_peak_speed_m_s > 0.0
     assert r.sediment_max_bin_density_scale == pytest.approx(float(np.max(m.meridional_result['scale_max'])))
     assert r.production_failure_count <= 2
+
+
+@pytest.mark.parametrize('law,surface', [('stokes_drift_diffusion', 'parabolic_vortex'),
+                                         ('schiller_naumann_drift_diffusion', 'localized_scully_vortex'),
+                                         ('schiller_naumann_drift_diffusion', 'parabolic_vortex')])
+def test_sediment_tables_match_oracle(gpu_available, oracle_lib, law, surface):
+    """Per-bin sediment coefficients (the meridional solve's inputs and the
+    analytic fields' tables): the device computes the radial integral by a
+    chunked workgroup scan, the oracle sequentially."""
+    import ctypes as C
+    from tardiis_and_claritas_amd import abi
+    _flush(f'sediment tables {law} {surface}')
+    m = ClaritasForwardModel(sediment_transport_model=law, free_surface_model=surface, particle_event_model='geometric')
+    mat = lognormal(30e-6, 2.0, 24, 1.46, 1e-4)
+    m.prepare(mat, 0.5)
+    n, nr = 24, m.sediment_radial_lookup_points
+    t = m.tables(n)
+    om = oracle_lib.OracleModel(m, mat, 0.5)
+    o = om.tables()
+    for k in ('sed_A', 'sed_B', 'sed_norm', 'sed_settling'):
+        np.testing.assert_allclose(t[k], o[k], rtol=1e-12, err_msg=k)
+    rl = np.zeros((n, nr))
+    tt = abi.TablesOut()
+    tt.sed_radial_log = rl.ctypes.data_as(C.POINTER(C.c_double))
+    m.ctx.check(m.ctx.lib.clr_get_tables(m.ctx.h, 0, C.byref(tt)), 'clr_get_tables')
+    np.testing.assert_allclose(rl, om.radial_log(nr), rtol=1e-12, atol=1e-14)
