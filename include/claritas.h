@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 4
+#define CLR_ABI_VERSION 5
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -119,6 +119,10 @@ enum {
   CLR_C_COND_INACCESSIBLE, /* inaccessible particle placements */
   CLR_C_RNG_DRAWS,         /* 32-bit Philox words consumed */
   CLR_C_N_STEPS,           /* state-machine steps (device) or loop passes */
+  /* reflecting source collimator (source_bore_model 1), CLARITAS.py:8677-8688 */
+  CLR_C_SRC_BORE_INTER,    /* source bore wall interactions */
+  CLR_C_SRC_BORE_REFL,     /* source bore wall reflections */
+  CLR_C_SRC_REFL_ACCEPT,   /* histories through the collimator after >= 1 reflection */
   CLR_C_COUNT_
 };
 
@@ -153,6 +157,15 @@ typedef struct clr_config {
   int32_t field_nbins;          /* 1 or n_bins (CLR_SED_GRID_PER_BIN) */
   int32_t spheroid_orientation; /* 0 isotropic, 1 local_tangential_axial_vmf, CLARITAS.py:5564 */
   int32_t pad_;
+  /* collimator bores, ring face, photodiode (CLARITAS.py:5558-5562, 5596-5603) */
+  double bore_n;                /* bore_refractive_index (> 1) */
+  double ring_face_n;           /* ring_inner_face_refractive_index (> 0) */
+  int32_t source_bore_model;    /* 0 absorbing, 1 smooth_fresnel_nylon */
+  int32_t detector_bore_model;  /* 0 absorbing, 1 smooth_fresnel_nylon */
+  int32_t ring_face_model;      /* 0 absorbing, 1 smooth_fresnel_nylon */
+  int32_t photodiode_model;     /* 0 ideal_hard_aperture, 1 sfh213_datasheet_fit */
+  int32_t max_bore_bounces;     /* >= 0 */
+  int32_t max_ring_face_bounces;/* >= 1 */
 } clr_config;
 
 /* ---- particle population (one PSD, CLARITAS.py:6036-6060) ---- */
@@ -228,7 +241,14 @@ typedef struct clr_tallies {
    * extra angles (ballistic histories skip windows at >= 90 deg) */
   uint64_t aux_window[CLR_MAX_GROUPS][2][CLR_MAX_AUX];
   uint64_t group_histories[CLR_MAX_GROUPS];
+  /* hardware hits after >= 1 bore or ring-face reflection, and those via
+   * the ring-face cavity (native, mirror; CLARITAS.py:4414-4423) */
+  uint64_t stray[2][CLR_N_CHANNELS];
+  uint64_t face_stray[2][CLR_N_CHANNELS];
   double incidence_sum[2][CLR_N_CHANNELS];   /* degrees, native / mirror */
+  /* photodiode-weighted hits per group (native, mirror): SFH213 relative
+   * response at the incidence angle, or 1 per hit for the ideal aperture */
+  double photodiode[CLR_MAX_GROUPS][2][CLR_N_CHANNELS];
   uint64_t status[CLR_N_STATUS];
   uint64_t failures[CLR_N_FAILURE];
   uint64_t counters[CLR_N_COUNTERS];

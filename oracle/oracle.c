@@ -301,6 +301,8 @@ typedef struct {
   int fn0, fn1, fn2, fnbins;
   float rough, aspect, kappa;   /* Beckmann alpha, spheroid a/b, vMF kappa */
   int orientation;              /* spheroid axis law */
+  int src_bore, det_bore, face_model, pd_model, max_bb, max_fb;  /* CLARITAS.py:5558-5562 */
+  float n_bore, n_face;
 } fparams;
 
 struct orc_model {
@@ -2307,6 +2309,205 @@ static int hardware_detector(const orc_model* m, float x, float y, float z, floa
 }
 
 /* ======================================================================
+ * Reflecting collimator bores, ring-face cavity and the SFH213 photodiode,
+ * CLARITAS.py:1263-1374 and 4283-4376.  Random numbers continue the
+ * history's stream: source collimator at the launch; at scoring native then
+ * mirror direct acceptance, then native and mirror ring-face paths.
+ * ====================================================================== */
+/* CLARITAS.py:1266-1279 */
+static float bore_side_hit(float q, float w, float vq, float vw, float radius) {
+  float A = vq * vq + vw * vw;
+  if (A <= 1.0e-20f) return INF_F;
+  float B = 2.0f * (q * vq + w * vw), Cc = q * q + w * w - radius * radius, D = B * B - 4.0f * A * Cc;
+  if (D < 0.0f) return INF_F;
+  float root = sqrtf(fmaxf(D, 0.0f));
+  float t1 = (-B - root) / (2.0f * A), t2 = (-B + root) / (2.0f * A), t = INF_F;
+  if (t1 > EPS_F) t = t1;
+  if (t2 > EPS_F && t2 < t) t = t2;
+  return t;
+}
+
+/* bore_wall_reflect + trace_bore_segment, CLARITAS.py:1281-1318 */
+static int bore_section(const fparams* P, rng_t* r, float s_end, float radius, float* s, float* q, float* w,
+                        float* vs, float* vq, float* vw, int* inter, int* refl) {
+  for (int b = 0; b <= P->max_bb; ++b) {
+    if (*vs <= 1.0e-12f) return 0;
+    float tp = (s_end - *s) / (*vs);
+    if (tp < -EPS_F) return 0;
+    if (tp < 0.0f) tp = 0.0f;
+    float twall = bore_side_hit(*q, *w, *vq, *vw, radius);
+    if (tp <= twall) {
+      *q += tp * (*vq); *w += tp * (*vw); *s = s_end;
+      return (*q) * (*q) + (*w) * (*w) <= radius * radius * (1.0f + 2.0e-5f);
+    }
+    if (twall >= INF_F * 0.5f) return 0;
+    *q += twall * (*vq); *w += twall * (*vw); *s += twall * (*vs);
+    (*inter)++;
+    float rr = sqrtf(fmaxf((*q) * (*q) + (*w) * (*w), 1.0e-30f));
+    float nq = -(*q) / rr, nw = -(*w) / rr;
+    float ci = fminf(fmaxf(-((*vs) * 0.0f + (*vq) * nq + (*vw) * nw), 0.0f), 1.0f);
+    int tir = 0;
+    float R = fresnel_R(ci, P->n_air, P->n_bore, &tir);
+    if (U(r) >= R) return 0;
+    float os, oq, ow;
+    reflect3(*vs, *vq, *vw, 0.0f, nq, nw, &os, &oq, &ow);
+    *vs = os; *vq = oq; *vw = ow;
+    (*refl)++;
+    *q -= EPS_F * (*q) / rr; *w -= EPS_F * (*w) / rr;
+  }
+  return 0;
+}
+
+/* CLARITAS.py:1321-1345 */
+static int source_collimator(const fparams* P, rng_t* r, float* x, float* y, float* z, float* vx, float* vy,
+                             float* vz, int* inter, int* refl) {
+  if (*vy <= 1.0e-12f) return 0;
+  float s = *y, q = *x, w = *z, vs = *vy, vq = *vx, vw = *vz;
+  if (q * q + w * w > P->cb_r * P->cb_r) return 0;
+  float shoulder = -P->throat_out;
+  if (s < shoulder - EPS_F) {
+    if (!bore_section(P, r, shoulder, P->cb_r, &s, &q, &w, &vs, &vq, &vw, inter, refl)) return 0;
+  } else if (s > shoulder + EPS_F) {
+    return 0;
+  }
+  if (q * q + w * w > P->throat_r * P->throat_r) return 0;
+  if (!bore_section(P, r, -P->ring_in, P->throat_r, &s, &q, &w, &vs, &vq, &vw, inter, refl)) return 0;
+  *x = q; *y = s; *z = w; *vx = vq; *vy = vs; *vz = vw;
+  return 1;
+}
+
+/* CLARITAS.py:1347-1371 */
+static int detector_collimator(const orc_model* m, rng_t* r, float x, float y, float z, float vx, float vy,
+                               float vz, int mirror, int* refl_out) {
+  const fparams* P = &m->fp;
+  if (mirror) { x = -x; vx = -vx; }
+  float best_t = INF_F;
+  int best = -1;
+  for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+    float ux = m->det_ux[j], uy = m->det_uy[j], den = vx * ux + vy * uy;
+    if (den <= 1.0e-10f) continue;
+    float t = (P->ring_in - (x * ux + y * uy)) / den;
+    if (t <= 0.0f) continue;
+    float hx = x + t * vx, hy = y + t * vy, hz = z + t * vz;
+    float q = hx * uy - hy * ux, w = hz;
+    if (q * q + w * w <= P->throat_r * P->throat_r && t < best_t) { best_t = t; best = j; }
+  }
+  if (best < 0) return -1;
+  float ux = m->det_ux[best], uy = m->det_uy[best];
+  float hx = x + best_t * vx, hy = y + best_t * vy;
+  float s = P->ring_in, q = hx * uy - hy * ux, w = z + best_t * vz;
+  float vs = vx * ux + vy * uy, vq = vx * uy - vy * ux, vw = vz;
+  int inter = 0, refl = 0;
+  if (!bore_section(P, r, P->throat_out, P->throat_r, &s, &q, &w, &vs, &vq, &vw, &inter, &refl)) return -1;
+  if (!bore_section(P, r, P->ring_out, P->cb_r, &s, &q, &w, &vs, &vq, &vw, &inter, &refl)) return -1;
+  *refl_out = refl;
+  return best;
+}
+
+/* CLARITAS.py:4304-4316 (the source throat faces -y) */
+static int throat_opening(const orc_model* m, float x, float y, float z, float vx, float vy, float vz) {
+  const fparams* P = &m->fp;
+  float best_t = INF_F, r2 = P->throat_r * P->throat_r;
+  int best = -1;
+  for (int j = 0; j <= CLR_N_CHANNELS; ++j) {
+    float ux = (j < CLR_N_CHANNELS) ? m->det_ux[j] : 0.0f, uy = (j < CLR_N_CHANNELS) ? m->det_uy[j] : -1.0f;
+    float den = vx * ux + vy * uy;
+    if (den <= 1.0e-10f) continue;
+    float t = (P->ring_in - (x * ux + y * uy)) / den;
+    if (t <= 0.0f) continue;
+    float qx = x + t * vx - P->ring_in * ux, qy = y + t * vy - P->ring_in * uy, qz = z + t * vz;
+    if (qx * qx + qy * qy + qz * qz <= r2 && t < best_t) { best_t = t; best = j; }
+  }
+  return best;
+}
+
+/* cylinder_hit of the reference: nearest x-y circle hit beyond EPS_F */
+static float circle_hit(float x, float y, float vx, float vy, float R) {
+  float a = vx * vx + vy * vy;
+  if (a < 1e-18f) return INF_F;
+  float b = x * vx + y * vy, c = x * x + y * y - R * R, d = b * b - a * c;
+  if (d < 0.0f) return INF_F;
+  float sd = sqrtf(fmaxf(d, 0.0f)), t1 = (-b - sd) / a, t2 = (-b + sd) / a, t = INF_F;
+  if (t1 > EPS_F) t = t1;
+  if (t2 > EPS_F && t2 < t) t = t2;
+  return t;
+}
+
+/* CLARITAS.py:4318-4328 */
+static int detector_from_air(const orc_model* m, rng_t* r, float x, float y, float z, float vx, float vy, float vz,
+                             int* bounces) {
+  *bounces = 0;
+  if (m->fp.det_bore == 0) return hardware_detector(m, x, y, z, vx, vy, vz, 0);
+  return detector_collimator(m, r, x, y, z, vx, vy, vz, 0, bounces);
+}
+
+/* CLARITAS.py:4330-4376 */
+static int ring_face_path(const orc_model* m, rng_t* r, float x, float y, float z, float vx, float vy, float vz,
+                          int mirror, int* bounces, int* face_refl) {
+  const fparams* P = &m->fp;
+  if (mirror) { x = -x; vx = -vx; }
+  *bounces = 0; *face_refl = 0;
+  if (P->face_model == 0) return -1;
+  if (throat_opening(m, x, y, z, vx, vy, vz) >= 0) return -1;
+  for (int b = 0; b < P->max_fb; ++b) {
+    float tr = circle_hit(x, y, vx, vy, P->ring_in);
+    if (tr >= INF_F * 0.5f) return -1;
+    x += tr * vx; y += tr * vy; z += tr * vz;
+    float nx, ny, a, c, d;
+    radial_normal(x, y, &nx, &ny);
+    float ci = fminf(fmaxf(vx * nx + vy * ny, 0.0f), 1.0f);
+    int tir = 0;
+    float R = fresnel_R(ci, P->n_air, P->n_face, &tir);
+    if (U(r) >= R) return -1;
+    reflect3(vx, vy, vz, -nx, -ny, 0.0f, &a, &c, &d);
+    vx = a; vy = c; vz = d;
+    (*face_refl)++;
+    x -= EPS_F * nx; y -= EPS_F * ny;
+    float ta = circle_hit(x, y, vx, vy, P->rout);
+    if (ta >= INF_F * 0.5f) return -1;
+    x += ta * vx; y += ta * vy; z += ta * vz;
+    radial_normal(x, y, &nx, &ny);
+    ci = fminf(fmaxf(-dot3(vx, vy, vz, nx, ny, 0.0f), 0.0f), 1.0f);
+    R = fresnel_R(ci, P->n_air, P->n_acrylic, &tir);
+    if (U(r) >= R) return -1;
+    reflect3(vx, vy, vz, nx, ny, 0.0f, &a, &c, &d);
+    vx = a; vy = c; vz = d;
+    x += EPS_F * nx; y += EPS_F * ny;
+    int db = 0;
+    int det = detector_from_air(m, r, x, y, z, vx, vy, vz, &db);
+    if (det >= 0) { *bounces = db; return det; }
+    if (throat_opening(m, x, y, z, vx, vy, vz) >= 0) return -1;
+  }
+  return -1;
+}
+
+/* bore_section on explicit rays (s, q, w, vs, vq, vw), ray i with stream i of
+ * `seed`, walls of cfg->bore_n, cfg->max_bore_bounces (test probe) */
+int orc_bore_probe(const clr_config* cfg, int n, uint64_t seed, const float* rays6, float s_end, float radius,
+                   int* pass, int* inter, int* refl) {
+  fparams P;
+  memset(&P, 0, sizeof(P));
+  P.n_air = (float)cfg->n_air; P.n_bore = (float)cfg->bore_n; P.max_bb = cfg->max_bore_bounces;
+  for (int i = 0; i < n; ++i) {
+    rng_t r;
+    rng_seed(&r, seed, (uint64_t)i);
+    const float* q = rays6 + 6 * (size_t)i;
+    float s = q[0], qq = q[1], w = q[2], vs = q[3], vq = q[4], vw = q[5];
+    inter[i] = 0; refl[i] = 0;
+    pass[i] = bore_section(&P, &r, s_end, radius, &s, &qq, &w, &vs, &vq, &vw, &inter[i], &refl[i]);
+  }
+  return 0;
+}
+
+/* CLARITAS.py:4283-4290 */
+static float sfh213_response(float phi_deg) {
+  float a = fabsf(phi_deg);
+  if (a >= 90.0f) return 0.0f;
+  return 1.0f / (1.0f + powf(a / 10.0f, 3.169925001442312f));
+}
+float orc_sfh213(float phi_deg) { return sfh213_response(phi_deg); }
+
+/* ======================================================================
  * One history, CLARITAS.py:4606-5068
  * ====================================================================== */
 typedef struct {
@@ -2321,6 +2522,8 @@ typedef struct {
   int auxw_native[CLR_MAX_AUX], auxw_mirror[CLR_MAX_AUX];
   int legacy_native[CLR_N_CHANNELS], legacy_mirror[CLR_N_CHANNELS];
   float inc_native, inc_mirror;
+  int sbi, sbr, nb, mb, nfb, mfb;  /* collimator / ring-face interactions and reflections */
+  float pd_native, pd_mirror;      /* SFH213 weights of the accepted hits */
 } hist_t;
 
 /* ======================================================================
@@ -2544,8 +2747,11 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
   normalize3(&vx, &vy, &vz);
   p->x = x; p->y = y; p->z = z; p->vx = vx; p->vy = vy; p->vz = vz;
   if (vy <= 1e-12f) { st = 3; goto WRITE_OUT; }
-  /* absorbing bore: hard apertures, CLARITAS.py:4681-4692 */
-  {
+  if (P->src_bore) {
+    /* reflecting source collimator, CLARITAS.py:4693-4697 */
+    if (!source_collimator(P, &r, &x, &y, &z, &vx, &vy, &vz, &o->sbi, &o->sbr)) { st = 3; goto WRITE_OUT; }
+  } else {
+    /* absorbing bore: hard apertures, CLARITAS.py:4681-4692 */
     if (x * x + z * z > P->cb_r * P->cb_r) { st = 3; goto WRITE_OUT; }
     float t = (-P->throat_out - y) / vy;
     if (t < -EPS_F) { st = 3; goto WRITE_OUT; }
@@ -2837,15 +3043,32 @@ AIR_EXIT:
   {
     float ex = p->x, ey = p->y, ez = p->z, evx = p->vx, evy = p->vy, evz = p->vz;
     if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
-      o->native = hardware_detector(m, ex, ey, ez, evx, evy, evz, 0);
-      o->mirror = hardware_detector(m, ex, ey, ez, evx, evy, evz, 1);
+      if (P->det_bore || P->face_model) {
+        /* direct acceptance first, then the ring-face cavity, CLARITAS.py:4398-4413 */
+        if (P->det_bore) {
+          o->native = detector_collimator(m, &r, ex, ey, ez, evx, evy, evz, 0, &o->nb);
+          o->mirror = detector_collimator(m, &r, ex, ey, ez, evx, evy, evz, 1, &o->mb);
+        } else {
+          o->native = hardware_detector(m, ex, ey, ez, evx, evy, evz, 0);
+          o->mirror = hardware_detector(m, ex, ey, ez, evx, evy, evz, 1);
+        }
+        if (o->native < 0 && P->face_model)
+          o->native = ring_face_path(m, &r, ex, ey, ez, evx, evy, evz, 0, &o->nb, &o->nfb);
+        if (o->mirror < 0 && P->face_model)
+          o->mirror = ring_face_path(m, &r, ex, ey, ez, evx, evy, evz, 1, &o->mb, &o->mfb);
+      } else {
+        o->native = hardware_detector(m, ex, ey, ez, evx, evy, evz, 0);
+        o->mirror = hardware_detector(m, ex, ey, ez, evx, evy, evz, 1);
+      }
       if (o->native >= 0) {
         float c = fminf(fmaxf(evx * m->det_ux[o->native] + evy * m->det_uy[o->native], -1.0f), 1.0f);
         o->inc_native = acosf(c) * (180.0f / PI_F);
+        o->pd_native = P->pd_model ? sfh213_response(o->inc_native) : 1.0f;
       }
       if (o->mirror >= 0) {
         float c = fminf(fmaxf(-evx * m->det_ux[o->mirror] + evy * m->det_uy[o->mirror], -1.0f), 1.0f);
         o->inc_mirror = acosf(c) * (180.0f / PI_F);
+        o->pd_mirror = P->pd_model ? sfh213_response(o->inc_mirror) : 1.0f;
       }
       for (int a = 0; a < P->n_aux; ++a) {
         float t;
@@ -2925,11 +3148,20 @@ static void tally(clr_tallies* T, const hist_t* h, int g) {
   if (h->native >= 0) {
     T->det[g][0][h->native]++;
     T->incidence_sum[0][h->native] += h->inc_native;
+    T->photodiode[g][0][h->native] += (double)h->pd_native;
+    if (h->nb > 0 || h->nfb > 0) T->stray[0][h->native]++;
+    if (h->nfb > 0) T->face_stray[0][h->native]++;
   }
   if (h->mirror >= 0) {
     T->det[g][1][h->mirror]++;
     T->incidence_sum[1][h->mirror] += h->inc_mirror;
+    T->photodiode[g][1][h->mirror] += (double)h->pd_mirror;
+    if (h->mb > 0 || h->mfb > 0) T->stray[1][h->mirror]++;
+    if (h->mfb > 0) T->face_stray[1][h->mirror]++;
   }
+  T->counters[CLR_C_SRC_BORE_INTER] += (uint64_t)h->sbi;
+  T->counters[CLR_C_SRC_BORE_REFL] += (uint64_t)h->sbr;
+  if (h->st != 3 && h->sbr > 0) T->counters[CLR_C_SRC_REFL_ACCEPT]++;
   for (int j = 0; j < CLR_N_CHANNELS; ++j) {
     if (h->legacy_native[j]) T->det[g][2][j]++;
     if (h->legacy_mirror[j]) T->det[g][3][j]++;
@@ -2952,7 +3184,14 @@ static void merge(clr_tallies* D, const clr_tallies* S) {
   for (size_t i = 0; i < sizeof(S->aux_window) / 8; ++i) d[i] += s[i];
   for (int g = 0; g < CLR_MAX_GROUPS; ++g) D->group_histories[g] += S->group_histories[g];
   for (int k = 0; k < 2; ++k)
-    for (int c = 0; c < CLR_N_CHANNELS; ++c) D->incidence_sum[k][c] += S->incidence_sum[k][c];
+    for (int c = 0; c < CLR_N_CHANNELS; ++c) {
+      D->incidence_sum[k][c] += S->incidence_sum[k][c];
+      D->stray[k][c] += S->stray[k][c];
+      D->face_stray[k][c] += S->face_stray[k][c];
+    }
+  for (int g = 0; g < CLR_MAX_GROUPS; ++g)
+    for (int k = 0; k < 2; ++k)
+      for (int c = 0; c < CLR_N_CHANNELS; ++c) D->photodiode[g][k][c] += S->photodiode[g][k][c];
   for (int i = 0; i < CLR_N_STATUS; ++i) D->status[i] += S->status[i];
   for (int i = 0; i < CLR_N_FAILURE; ++i) D->failures[i] += S->failures[i];
   for (int i = 0; i < CLR_N_COUNTERS; ++i) D->counters[i] += S->counters[i];
@@ -3084,6 +3323,9 @@ static void fill_fparams(orc_model* m) {
   P->max_pb = c->max_particle_bounces; P->max_iter = c->max_iterations;
   P->rough = (float)c->rough_alpha; P->aspect = (float)c->spheroid_aspect; P->kappa = (float)c->spheroid_kappa;
   P->orientation = c->spheroid_orientation;
+  P->src_bore = c->source_bore_model; P->det_bore = c->detector_bore_model; P->face_model = c->ring_face_model;
+  P->pd_model = c->photodiode_model; P->max_bb = c->max_bore_bounces; P->max_fb = c->max_ring_face_bounces;
+  P->n_bore = (float)c->bore_n; P->n_face = (float)c->ring_face_n;
   P->cond_trials = c->conditional_max_trials; P->n_aux = c->n_aux;
   P->n_groups = (c->n_groups >= 1 && c->n_groups <= CLR_MAX_GROUPS) ? c->n_groups : CLR_MAX_GROUPS;
   P->fn0 = c->field_n0; P->fn1 = c->field_n1; P->fn2 = c->field_n2; P->fnbins = c->field_nbins;

@@ -53,6 +53,14 @@ int orc_light_segment_probe(const clr_light_config* lc, float rin, float zmin, i
 int orc_light_event_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* xyz3,
                           clr_light_out* lo);
 
+/* One reflecting bore section (CLARITAS.py:1296-1318) on explicit rays
+ * rays6[n] = (s, q, w, vs, vq, vw) in the bore frame up to s_end, walls of
+ * refractive index cfg->bore_n, at most cfg->max_bore_bounces + 1 wall hits;
+ * ray i draws from stream i of `seed`.  And the SFH213 relative response. */
+int orc_bore_probe(const clr_config* cfg, int n, uint64_t seed, const float* rays6, float s_end, float radius,
+                   int* pass, int* inter, int* refl);
+float orc_sfh213(float phi_deg);
+
 /* Geometry probes on the reference's test-kernel semantics
  * (free_surface_geometry_test_kernel, CLARITAS.py:4438-4455). */
 int orc_free_surface_probe(const clr_config* cfg, const float* rays6, int n,

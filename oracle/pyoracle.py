@@ -81,6 +81,11 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = [P(abi.LightConfig), C.c_float, C.c_float, C.c_int, P(C.c_float), P(abi.LightOut)]
         fn.restype = C.c_int
+    lib.orc_bore_probe.argtypes = [P(abi.Config), C.c_int, C.c_uint64, P(C.c_float), C.c_float, C.c_float,
+                                   P(C.c_int), P(C.c_int), P(C.c_int)]
+    lib.orc_bore_probe.restype = C.c_int
+    lib.orc_sfh213.argtypes = [C.c_float]
+    lib.orc_sfh213.restype = C.c_float
     _lib = lib
     return lib
 
@@ -378,3 +383,12 @@ def light_probe(mode, data, light_config, rin, zmin):
     fn = load().orc_light_segment_probe if mode == 0 else load().orc_light_event_probe
     fn(C.byref(lc), float(rin), float(zmin), int(data.shape[0]), _pf(data), C.byref(out))
     return arrs, abi.light_scalars(out)
+
+
+def bore_probe(cfg, rays6, s_end, radius, seed=1):
+    """One reflecting bore section per ray: (pass, interactions, reflections)."""
+    rays6 = np.ascontiguousarray(rays6, dtype=np.float32)
+    n = rays6.shape[0]
+    out = [np.zeros(n, dtype=np.int32) for _ in range(3)]
+    load().orc_bore_probe(C.byref(cfg), n, int(seed), _pf(rays6), float(s_end), float(radius), *[_pi(a) for a in out])
+    return out

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -55,7 +55,8 @@ COUNTER_NAMES = [
     'ballistic', 'sum_pr', 'rays_per', 'rays_pir', 'rays_pabs', 'sum_cir', 'sum_cor',
     'rays_ctir', 'sum_topr', 'rays_toptir', 'sum_bir', 'sum_bor', 'rays_btir',
     'n_flight', 'n_null', 'n_scat', 'n_iface', 'n_abs', 'n_exit', 'cond_trials',
-    'cond_reject', 'cond_inaccessible', 'rng_draws', 'n_steps',
+    'cond_reject', 'cond_inaccessible', 'rng_draws', 'n_steps', 'src_bore_interactions', 'src_bore_reflections',
+    'src_reflected_accept',
 ]
 C_IDX = {name: i for i, name in enumerate(COUNTER_NAMES)}
 
@@ -101,6 +102,9 @@ class Config(C.Structure):
         ('n_groups', I32), ('field_n0', I32), ('field_n1', I32), ('field_n2', I32),
         ('field_nbins', I32),
         ('spheroid_orientation', I32), ('pad_', I32),
+        ('bore_n', D), ('ring_face_n', D), ('source_bore_model', I32), ('detector_bore_model', I32),
+        ('ring_face_model', I32), ('photodiode_model', I32), ('max_bore_bounces', I32),
+        ('max_ring_face_bounces', I32),
     ]
 
 
@@ -143,7 +147,10 @@ class Tallies(C.Structure):
         ('aux', U64 * MAX_AUX * 2 * MAX_GROUPS),
         ('aux_window', U64 * MAX_AUX * 2 * MAX_GROUPS),
         ('group_histories', U64 * MAX_GROUPS),
+        ('stray', U64 * N_CHANNELS * 2),
+        ('face_stray', U64 * N_CHANNELS * 2),
         ('incidence_sum', D * N_CHANNELS * 2),
+        ('photodiode', D * N_CHANNELS * 2 * MAX_GROUPS),
         ('status', U64 * N_STATUS),
         ('failures', U64 * N_FAILURE),
         ('counters', U64 * N_COUNTERS),
@@ -285,6 +292,9 @@ def tallies_to_numpy(t: Tallies):
         'aux_window': auxw[:g].copy(),
         'group_histories': np.ctypeslib.as_array(t.group_histories)[:g].astype(np.int64).copy(),
         'incidence_sum': np.ctypeslib.as_array(t.incidence_sum).reshape(2, N_CHANNELS).copy(),
+        'stray': np.ctypeslib.as_array(t.stray).reshape(2, N_CHANNELS).astype(np.int64).copy(),
+        'face_stray': np.ctypeslib.as_array(t.face_stray).reshape(2, N_CHANNELS).astype(np.int64).copy(),
+        'photodiode': np.ctypeslib.as_array(t.photodiode).reshape(MAX_GROUPS, 2, N_CHANNELS)[:g].copy(),
         'status': np.ctypeslib.as_array(t.status).astype(np.int64).copy(),
         'failures': np.ctypeslib.as_array(t.failures).astype(np.int64).copy(),
         'counters': np.ctypeslib.as_array(t.counters).astype(np.int64).copy(),

@@ -234,6 +234,19 @@ int validate_config(clr_ctx* c, const clr_config* g, const clr_particles* p) {
     return fail(c, CLR_E_ARG, "spheroid_kappa must be in [0, 100]");
   if (g->spheroid_orientation != 0 && g->spheroid_orientation != 1)
     return fail(c, CLR_E_ARG, "spheroid_orientation must be 0 (isotropic) or 1 (local tangential vMF)");
+  // CLARITAS.py:5966-5991
+  if ((g->source_bore_model != 0 && g->source_bore_model != 1) ||
+      (g->detector_bore_model != 0 && g->detector_bore_model != 1))
+    return fail(c, CLR_E_ARG, "bore surface model must be absorbing or smooth_fresnel_nylon");
+  if (g->ring_face_model != 0 && g->ring_face_model != 1)
+    return fail(c, CLR_E_ARG, "ring inner face surface model must be absorbing or smooth_fresnel_nylon");
+  if (g->photodiode_model != 0 && g->photodiode_model != 1)
+    return fail(c, CLR_E_ARG, "photodiode response model must be ideal_hard_aperture or sfh213_datasheet_fit");
+  if (!(g->ring_face_n > 0.0) || !std::isfinite(g->ring_face_n))
+    return fail(c, CLR_E_ARG, "ring_inner_face_refractive_index must be finite and positive");
+  if (g->max_ring_face_bounces < 1) return fail(c, CLR_E_ARG, "max_ring_face_bounces must be >=1");
+  if (!(g->bore_n > 1.0)) return fail(c, CLR_E_ARG, "bore_refractive_index must be >1");
+  if (g->max_bore_bounces < 0) return fail(c, CLR_E_ARG, "max_bore_bounces must be >=0");
   const int em = g->event_model;
   if (!(em == 0 || em == 1 || em == 2 || em == 3 || em == 4 || em == 6))
     return fail(c, CLR_E_UNSUPPORTED, "unsupported particle event model");
@@ -323,6 +336,10 @@ void fill_params(Slot& s, const clr_config& g) {
   P.max_pb = g.max_particle_bounces; P.max_iter = g.max_iterations; P.cond_trials = g.conditional_max_trials;
   P.rough = (float)g.rough_alpha; P.aspect = (float)g.spheroid_aspect; P.kappa = (float)g.spheroid_kappa;
   P.orientation = g.spheroid_orientation;
+  P.src_bore = g.source_bore_model; P.det_bore = g.detector_bore_model; P.face_model = g.ring_face_model;
+  P.pd_model = g.photodiode_model; P.max_bore_bounces = g.max_bore_bounces;
+  P.max_face_bounces = g.max_ring_face_bounces;
+  P.n_bore = (float)g.bore_n; P.n_face = (float)g.ring_face_n;
   P.n_aux = g.n_aux; P.n_groups = g.n_groups; P.n_bins = s.n; P.n_theta = s.nt; P.n_radial = s.nrad;
   P.fn0 = g.field_n0; P.fn1 = g.field_n1; P.fn2 = g.field_n2;
   // lowest free-surface height (host float arithmetic of the device formula)
@@ -399,6 +416,15 @@ int ensure_slots(clr_ctx* c, int need) {
       HIPCHK(c, hipMemcpy(c->params.as<DevParams>() + k, &c->slots[k]->params, sizeof(DevParams),
                           hipMemcpyHostToDevice));
   return 0;
+}
+
+// The ideal hard-aperture photodiode weighs every hit 1 (CLARITAS.py:4418):
+// its sums are the hardware counts, filled here rather than on the device.
+void ideal_photodiode(const clr_config& g, clr_tallies& t) {
+  if (g.photodiode_model != 0) return;
+  for (int q = 0; q < CLR_MAX_GROUPS; ++q)
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < CLR_N_CHANNELS; ++j) t.photodiode[q][k][j] = (double)t.det[q][k][j];
 }
 
 void group_sizes(uint64_t n, int G, uint64_t* out) {
@@ -531,6 +557,12 @@ int kernel_kind(clr_ctx* c, const std::vector<int>& used) {
           g.sediment_model == CLR_SED_XYZ_FIELD))
       plain = false;
     if (g.rough_alpha > 1e-7 || (g.event_model != CLR_EVENT_PLAIN_MIE && s.params.coherent < 1.0f)) rough = true;
+    // reflecting collimators, the ring-face cavity and the SFH213 weights
+    // are compiled into the ROUGH variant only
+    if (g.source_bore_model || g.detector_bore_model || g.ring_face_model || g.photodiode_model) {
+      plain = false;
+      rough = true;
+    }
   }
   return plain ? CLR_KERNEL_PLAIN : rough ? CLR_KERNEL_ROUGH : CLR_KERNEL_GENERIC;
 }
@@ -714,10 +746,11 @@ int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr
     HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
     const uint64_t* src = reinterpret_cast<const uint64_t*>(&host[0]);
     uint64_t* dst = reinterpret_cast<uint64_t*>(&acc[0]);
-    // every field before incidence_sum and after it is u64; add field-wise
+    // u64 fields up to incidence_sum, FP64 sums up to status, u64 after
     for (size_t k = 0; k < offsetof(clr_tallies, incidence_sum) / 8; ++k) dst[k] += src[k];
-    for (int k = 0; k < 2; ++k)
-      for (int j = 0; j < CLR_N_CHANNELS; ++j) acc[0].incidence_sum[k][j] += host[0].incidence_sum[k][j];
+    const double* fs = reinterpret_cast<const double*>(&host[0]);
+    double* fd = reinterpret_cast<double*>(&acc[0]);
+    for (size_t k = offsetof(clr_tallies, incidence_sum) / 8; k < offsetof(clr_tallies, status) / 8; ++k) fd[k] += fs[k];
     const size_t a0 = offsetof(clr_tallies, status) / 8, a1 = offsetof(clr_tallies, n_groups) / 8;
     for (size_t k = a0; k < a1; ++k) dst[k] += src[k];
   }
@@ -725,6 +758,7 @@ int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr
   *out = acc[0];
   out->n_groups = G;
   group_sizes(n, G, out->group_histories);
+  ideal_photodiode(c->slots[0]->cfg, *out);
   out->counters[CLR_C_HISTORIES] = n;  // launched histories (the device counts per-event work only)
   if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
   return 0;
@@ -1009,6 +1043,7 @@ int clr_run_batch(clr_ctx* c, const clr_batch_item* bi, int n_items, const clr_l
     out[i] = host[k];
     out[i].n_groups = c->slots[k]->cfg.n_groups;
     group_sizes(bi[i].n_histories, out[i].n_groups, out[i].group_histories);
+    ideal_photodiode(c->slots[k]->cfg, out[i]);
     out[i].counters[CLR_C_HISTORIES] = bi[i].n_histories;
   }
   return 0;

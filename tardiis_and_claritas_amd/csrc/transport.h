@@ -23,6 +23,10 @@ struct DevParams {
   // shared FP32 fields (sediment models 4, 5): inverse cell sizes and the
   // bin-summed channel rates in float
   float field_inv0, field_inv1, field_inv2, sum_g_f, sum_w_f;
+  // collimator bores, ring face and photodiode (CLARITAS.py:5558-5562):
+  // 0 absorbing / ideal, 1 smooth Fresnel nylon / SFH213 fit
+  int src_bore, det_bore, face_model, pd_model, max_bore_bounces, max_face_bounces;
+  float n_bore, n_face;
   int fs_model, src_model, event_model, sed_model;
   int max_pb, max_iter, cond_trials, n_aux;
   float rough, aspect, kappa;     // particle Beckmann alpha, spheroid a/b, axis-law kappa

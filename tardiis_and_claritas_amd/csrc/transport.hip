@@ -1670,6 +1670,199 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
   return best;
 }
 
+// ---- reflective collimator bores, ring-face cavity, SFH213 photodiode ----
+// (ROUGH and LIGHT variants only; CLARITAS.py:1263-1360, 4283-4376).  A bore
+// is a cylinder of radius `radius` along its local axis s; (q, w) are the
+// transverse coordinates.  A smooth opaque nylon wall reflects with the
+// unpolarised Fresnel probability (one uniform per wall hit) and absorbs
+// otherwise.
+
+// nearest wall hit beyond kEpsF of the transverse circle, CLARITAS.py:1266-1279
+__device__ __forceinline__ float bore_side_hit(float q, float w, float vq, float vw, float radius) {
+  const float A = vq * vq + vw * vw;
+  if (A <= 1.0e-20f) return kInfF;
+  const float B = 2.0f * (q * vq + w * vw), Cc = q * q + w * w - radius * radius;
+  const float D = B * B - 4.0f * A * Cc;
+  if (D < 0.0f) return kInfF;
+  const float root = sqrtf(fmaxf(D, 0.0f));
+  const float t1 = (-B - root) / (2.0f * A), t2 = (-B + root) / (2.0f * A);
+  float t = kInfF;
+  if (t1 > kEpsF) t = t1;
+  if (t2 > kEpsF && t2 < t) t = t2;
+  return t;
+}
+
+// one bore section from s to s_end (CLARITAS.py:1296-1318): reflections off
+// the wall until the exit plane; true when the ray leaves inside the radius
+__device__ bool bore_section(Rng& r, const DevParams& P, float s_end, float radius, float& s, float& q, float& w,
+                             float& vs, float& vq, float& vw, int& interactions, int& reflections) {
+  for (int b = 0; b <= P.max_bore_bounces; ++b) {
+    if (vs <= 1.0e-12f) return false;
+    float tp = (s_end - s) / vs;
+    if (tp < -kEpsF) return false;
+    if (tp < 0.0f) tp = 0.0f;
+    const float twall = bore_side_hit(q, w, vq, vw, radius);
+    if (tp <= twall) {
+      q += tp * vq; w += tp * vw; s = s_end;
+      return q * q + w * w <= radius * radius * (1.0f + 2.0e-5f);
+    }
+    if (twall >= kInfF * 0.5f) return false;
+    q += twall * vq; w += twall * vw; s += twall * vs;
+    // wall normal from the solid into the air cavity, CLARITAS.py:1281-1294
+    interactions++;  // callers reach here only with a reflecting bore model
+    const float rr = sqrtf(fmaxf(q * q + w * w, 1.0e-30f));
+    const float nq = -q / rr, nw = -w / rr;
+    const float ci = fminf(fmaxf(-(vq * nq + vw * nw), 0.0f), 1.0f);
+    bool tir;
+    const float R = fresnel_R(ci, P.n_air, P.n_bore, tir);
+    if (U(r) >= R) return false;  // transmitted into the nylon: lost
+    reflect3(vs, vq, vw, 0.0f, nq, nw);
+    reflections++;
+    q -= kEpsF * q / rr; w -= kEpsF * w / rr;
+  }
+  return false;
+}
+
+// source collimator, local axis +y (q = x, w = z): the counterbore from the
+// luminous plane to the shoulder, the opaque shoulder, then the throat to
+// the ring face, CLARITAS.py:1321-1345
+__device__ bool source_collimator(Rng& r, const DevParams& P, float& x, float& y, float& z, float& vx, float& vy,
+                                  float& vz, int& interactions, int& reflections) {
+  if (vy <= 1.0e-12f) return false;
+  float s = y, q = x, w = z, vs = vy, vq = vx, vw = vz;
+  if (q * q + w * w > P.cb_r * P.cb_r) return false;
+  const float shoulder = -P.throat_out;
+  if (s < shoulder - kEpsF) {
+    if (!bore_section(r, P, shoulder, P.cb_r, s, q, w, vs, vq, vw, interactions, reflections)) return false;
+  } else if (s > shoulder + kEpsF) {
+    return false;
+  }
+  if (q * q + w * w > P.throat_r * P.throat_r) return false;
+  if (!bore_section(r, P, -P.ring_in, P.throat_r, s, q, w, vs, vq, vw, interactions, reflections)) return false;
+  x = q; y = s; z = w; vx = vq; vy = vs; vz = vw;
+  return true;
+}
+
+// first detector throat entrance the air ray reaches, then the throat and
+// the counterbore with reflecting walls (CLARITAS.py:1347-1371); -1 if lost
+__device__ int detector_collimator(Rng& r, const DevParams& P, float x, float y, float z, float vx, float vy,
+                                   float vz, bool mirror, int& reflections) {
+  if (mirror) { x = -x; vx = -vx; }
+  float best_t = kInfF;
+  int best = -1;
+  for (int j = 0; j < CLR_N_CHANNELS; ++j) {
+    const float ux = P.det_ux[j], uy = P.det_uy[j];
+    const float den = vx * ux + vy * uy;
+    if (den <= 1.0e-10f) continue;
+    const float t = (P.ring_in - (x * ux + y * uy)) / den;
+    if (t <= 0.0f) continue;
+    const float hx = x + t * vx, hy = y + t * vy, hz = z + t * vz;
+    const float q = hx * uy - hy * ux, w = hz;
+    if (q * q + w * w <= P.throat_r * P.throat_r && t < best_t) { best_t = t; best = j; }
+  }
+  if (best < 0) return -1;
+  const float ux = P.det_ux[best], uy = P.det_uy[best];
+  const float hx = x + best_t * vx, hy = y + best_t * vy;
+  float s = P.ring_in, q = hx * uy - hy * ux, w = z + best_t * vz;
+  float vs = vx * ux + vy * uy, vq = vx * uy - vy * ux, vw = vz;
+  int inter = 0, refl = 0;
+  if (!bore_section(r, P, P.throat_out, P.throat_r, s, q, w, vs, vq, vw, inter, refl)) return -1;
+  if (!bore_section(r, P, P.ring_out, P.cb_r, s, q, w, vs, vq, vw, inter, refl)) return -1;
+  reflections = refl;
+  return best;
+}
+
+// throat opening (0..17 detectors, 18 the source) the ray enters first at
+// the ring face, or -1 for the solid face, CLARITAS.py:4304-4316
+__device__ int throat_opening(const DevParams& P, float x, float y, float z, float vx, float vy, float vz) {
+  float best_t = kInfF;
+  int best = -1;
+  const float r2 = P.throat_r * P.throat_r;
+  for (int j = 0; j <= CLR_N_CHANNELS; ++j) {
+    const float ux = (j < CLR_N_CHANNELS) ? P.det_ux[j] : 0.0f, uy = (j < CLR_N_CHANNELS) ? P.det_uy[j] : -1.0f;
+    const float den = vx * ux + vy * uy;
+    if (den <= 1.0e-10f) continue;
+    const float t = (P.ring_in - (x * ux + y * uy)) / den;
+    if (t <= 0.0f) continue;
+    const float qx = x + t * vx - P.ring_in * ux, qy = y + t * vy - P.ring_in * uy, qz = z + t * vz;
+    if (qx * qx + qy * qy + qz * qz <= r2 && t < best_t) { best_t = t; best = j; }
+  }
+  return best;
+}
+
+// nearest circle hit beyond kEpsF in x-y, CLARITAS.py cylinder_hit
+__device__ __forceinline__ float circle_hit(float x, float y, float vx, float vy, float R) {
+  const float a = vx * vx + vy * vy;
+  if (a < 1e-18f) return kInfF;
+  const float b = x * vx + y * vy, c = x * x + y * y - R * R, d = b * b - a * c;
+  if (d < 0.0f) return kInfF;
+  const float sd = sqrtf(fmaxf(d, 0.0f));
+  const float t1 = (-b - sd) / a, t2 = (-b + sd) / a;
+  float t = kInfF;
+  if (t1 > kEpsF) t = t1;
+  if (t2 > kEpsF && t2 < t) t = t2;
+  return t;
+}
+
+// a detector reached from the air gap once: hard apertures or the
+// reflecting collimator, CLARITAS.py:4318-4328
+__device__ __forceinline__ int detector_from_air(Rng& r, const DevParams& P, float x, float y, float z, float vx,
+                                                 float vy, float vz, int& bounces) {
+  bounces = 0;
+  if (P.det_bore == 0) return hardware_detector(P, x, y, z, vx, vy, vz, false);
+  return detector_collimator(r, P, x, y, z, vx, vy, vz, false, bounces);
+}
+
+// stray path between the nylon ring face and the acrylic outer wall: the
+// ray reflects off the ring face (Fresnel, else lost), off the acrylic
+// outer surface (Fresnel, else it re-enters the cell and is not counted),
+// and is offered to the detectors after each round trip,
+// CLARITAS.py:4330-4376
+__device__ int ring_face_path(Rng& r, const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
+                              bool mirror, int& bounces, int& face_reflections) {
+  if (mirror) { x = -x; vx = -vx; }
+  bounces = 0;
+  face_reflections = 0;
+  if (P.face_model == 0) return -1;
+  if (throat_opening(P, x, y, z, vx, vy, vz) >= 0) return -1;
+  for (int b = 0; b < P.max_face_bounces; ++b) {
+    const float tr = circle_hit(x, y, vx, vy, P.ring_in);
+    if (tr >= kInfF * 0.5f) return -1;
+    x += tr * vx; y += tr * vy; z += tr * vz;
+    float nx, ny;
+    radial_normal(x, y, nx, ny);
+    bool tir;
+    float ci = fminf(fmaxf(vx * nx + vy * ny, 0.0f), 1.0f);
+    float R = fresnel_R(ci, P.n_air, P.n_face, tir);
+    if (U(r) >= R) return -1;
+    reflect3(vx, vy, vz, -nx, -ny, 0.0f);
+    face_reflections++;
+    x -= kEpsF * nx; y -= kEpsF * ny;
+    const float ta = circle_hit(x, y, vx, vy, P.rout);
+    if (ta >= kInfF * 0.5f) return -1;
+    x += ta * vx; y += ta * vy; z += ta * vz;
+    radial_normal(x, y, nx, ny);
+    ci = fminf(fmaxf(-(vx * nx + vy * ny), 0.0f), 1.0f);
+    R = fresnel_R(ci, P.n_air, P.n_acrylic, tir);
+    if (U(r) >= R) return -1;
+    reflect3(vx, vy, vz, nx, ny, 0.0f);
+    x += kEpsF * nx; y += kEpsF * ny;
+    int db = 0;
+    const int det = detector_from_air(r, P, x, y, z, vx, vy, vz, db);
+    if (det >= 0) { bounces = db; return det; }
+    if (throat_opening(P, x, y, z, vx, vy, vz) >= 0) return -1;
+  }
+  return -1;
+}
+
+// SFH213 relative response 1 / (1 + (phi / 10 deg)^P), P = ln 9 / ln 2 (the
+// response is 0.1 at 20 deg), zero from 90 deg, CLARITAS.py:4283-4290
+__device__ __forceinline__ float sfh213_response(float phi_deg) {
+  const float a = fabsf(phi_deg);
+  if (a >= 90.0f) return 0.0f;
+  return 1.0f / (1.0f + powf(a / 10.0f, 3.169925001442312f));
+}
+
 // ------------------------------------------------------------ lane state --
 // R_WATER: next free flight; R_WPART / R_WIFACE: the flight ended at a
 // particle event / at a boundary of the water (event handled by the next pass)
@@ -1703,8 +1896,12 @@ __device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
   L.region = R_DONE;
 }
 
-// launch + hard apertures + air->acrylic entry, CLARITAS.py:4650-4735
-__device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A) {
+// launch + collimator + air->acrylic entry, CLARITAS.py:4650-4735.  The
+// reflecting source collimator (ROUGH and LIGHT variants) reports its wall
+// interactions and reflections in sbi / sbr.
+template <int kKind>
+__device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A, int& sbi, int& sbr) {
+  constexpr bool kBores = (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT);
   Rng& r = L.r;
   float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
   if (P.src_model == CLR_SOURCE_COLLIMATED_DISK) {
@@ -1733,16 +1930,22 @@ __device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc&
   normalize3(vx, vy, vz);
   L.x = x; L.y = y; L.z = z; L.vx = vx; L.vy = vy; L.vz = vz;
   if (vy <= 1e-12f) { terminate(L, 3); return; }
-  if (x * x + z * z > P.cb_r * P.cb_r) { terminate(L, 3); return; }
-  float t = (-P.throat_out - y) / vy;
-  if (t < -kEpsF) { terminate(L, 3); return; }
-  if (t < 0.0f) t = 0.0f;
-  const float xt = x + t * vx, zt = z + t * vz;
-  if (xt * xt + zt * zt > P.throat_r * P.throat_r) { terminate(L, 3); return; }
-  t = (-P.ring_in - y) / vy;
-  if (t <= 0.0f) { terminate(L, 3); return; }
-  x += t * vx; y += t * vy; z += t * vz;
-  if (x * x + z * z > P.throat_r * P.throat_r) { L.x = x; L.y = y; L.z = z; terminate(L, 3); return; }
+  bool reflecting = false;
+  if constexpr (kBores) reflecting = P.src_bore != 0;
+  if (reflecting) {
+    if (!source_collimator(r, P, x, y, z, vx, vy, vz, sbi, sbr)) { terminate(L, 3); return; }
+  } else {
+    if (x * x + z * z > P.cb_r * P.cb_r) { terminate(L, 3); return; }
+    float t = (-P.throat_out - y) / vy;
+    if (t < -kEpsF) { terminate(L, 3); return; }
+    if (t < 0.0f) t = 0.0f;
+    const float xt = x + t * vx, zt = z + t * vz;
+    if (xt * xt + zt * zt > P.throat_r * P.throat_r) { terminate(L, 3); return; }
+    t = (-P.ring_in - y) / vy;
+    if (t <= 0.0f) { terminate(L, 3); return; }
+    x += t * vx; y += t * vy; z += t * vz;
+    if (x * x + z * z > P.throat_r * P.throat_r) { L.x = x; L.y = y; L.z = z; terminate(L, 3); return; }
+  }
   const double td = finite_cylinder_forward_d(P, x, y, z, vx, vy, vz, P.rout, P.zlow, P.zwall_top);
   const float tw = (td < 0.5 * (double)kInfF) ? (float)td : kInfF;
   if (tw >= kInfF * 0.5f) { L.x = x; L.y = y; L.z = z; terminate(L, 3); return; }
@@ -2414,28 +2617,61 @@ __device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
 }
 
 // Score a history that has ended (CLARITAS.py:4397-4435) and fold its
-// per-ray reductions (CLARITAS.py:8659-8733) into the workgroup's LDS
+// per-ray reductions (CLARITAS.py:8659-8733) into the workgroup's LDS.  In
+// the ROUGH and LIGHT variants the reflecting detector bores, the ring-face
+// cavity and the SFH213 weights draw from the history's stream and write
+// their (rare) stray counts and photodiode sums straight to `Gt`
 // tallies.  Lanes of one wave that hit the same bin are serialised by the
 // LDS atomic unit, which is cheaper than aggregating them in software.
+template <int kKind>
 __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
-                                            uint64_t n_rec, uint64_t hidx, int g) {
+                                            uint64_t n_rec, uint64_t hidx, int g, clr_tallies* __restrict__ Gt) {
+  constexpr bool kBores = (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT);
   int nat = -1, mir = -1, aux_bits = 0;
   bool scored = false;
   float inc_n = 0.0f, inc_m = 0.0f, ang_n = -1.0f, ang_m = -1.0f;
   {
-    A.draws += rng_draws(L.r);
     if (L.flags & F_ENTERED) A.entered++;
     if (L.st == 1) {
       const float ex = L.x, ey = L.y, ez = L.z, evx = L.vx, evy = L.vy, evz = L.vz;
       A.n_exit++;
       if (isfinite(ex) && isfinite(evx)) A.valid_air++;
       if (isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(evx) && isfinite(evy) && isfinite(evz)) {
-        nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
-        mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+        int nb = 0, mb = 0, nfb = 0, mfb = 0;
+        bool bores = false;
+        if constexpr (kBores) bores = (P.det_bore | P.face_model) != 0;
+        if (bores) {
+          // direct acceptance first (native then mirror), then the ring-face
+          // cavity for a channel that failed, CLARITAS.py:4398-4413
+          if (P.det_bore) {
+            nat = detector_collimator(L.r, P, ex, ey, ez, evx, evy, evz, false, nb);
+            mir = detector_collimator(L.r, P, ex, ey, ez, evx, evy, evz, true, mb);
+          } else {
+            nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
+            mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+          }
+          if (nat < 0 && P.face_model) nat = ring_face_path(L.r, P, ex, ey, ez, evx, evy, evz, false, nb, nfb);
+          if (mir < 0 && P.face_model) mir = ring_face_path(L.r, P, ex, ey, ez, evx, evy, evz, true, mb, mfb);
+        } else {
+          nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
+          mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+        }
         if (nat >= 0)
           inc_n = acosf(fminf(fmaxf(evx * P.det_ux[nat] + evy * P.det_uy[nat], -1.0f), 1.0f)) * (180.0f / kPiF);
         if (mir >= 0)
           inc_m = acosf(fminf(fmaxf(-evx * P.det_ux[mir] + evy * P.det_uy[mir], -1.0f), 1.0f)) * (180.0f / kPiF);
+        if constexpr (kBores) {
+          if (nat >= 0) {
+            if (nb > 0 || nfb > 0) atomicAdd(&Gt->stray[0][nat], 1ull);
+            if (nfb > 0) atomicAdd(&Gt->face_stray[0][nat], 1ull);
+            if (P.pd_model) atomicAdd(&Gt->photodiode[g][0][nat], (double)sfh213_response(inc_n));
+          }
+          if (mir >= 0) {
+            if (mb > 0 || mfb > 0) atomicAdd(&Gt->stray[1][mir], 1ull);
+            if (mfb > 0) atomicAdd(&Gt->face_stray[1][mir], 1ull);
+            if (P.pd_model) atomicAdd(&Gt->photodiode[g][1][mir], (double)sfh213_response(inc_m));
+          }
+        }
         for (int a = 0; a < P.n_aux; ++a) {
           float t;
           if (bore_accept(P, ex, ey, ez, evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t)) aux_bits |= 1 << a;
@@ -2469,6 +2705,7 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
       if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
     }
     lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
+    A.draws += rng_draws(L.r);
     if (rec && hidx < n_rec) {
       clr_record q;
       q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
@@ -2665,7 +2902,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         if (fin) {
           // history number and jackknife group from the stream counter
           const uint64_t hidx = rng_counter(L.r) - w_offset;
-          finish_lane(L, P, T, A, rec, Lp.n_records, hidx, (int)((hidx * G) / w_total));
+          finish_lane<kKind>(L, P, T, A, rec, Lp.n_records, hidx, (int)((hidx * G) / w_total), Lp.tallies + cfg);
           alive = false;
         }
         // refill: one LDS atomic per wave hands consecutive history numbers
@@ -2687,7 +2924,15 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
               L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
               L.flags = 0u;
               L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
-              launch_history(L, P, A);  // may end it at once: scored by a later RECYCLE pass
+              int sbi = 0, sbr = 0;
+              launch_history<kKind>(L, P, A, sbi, sbr);  // may end it at once: scored by a later RECYCLE pass
+              if constexpr (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT) {
+                if (P.src_bore) {
+                  lds_add64(&T.cnt[CLR_C_SRC_BORE_INTER], (uint32_t)sbi);
+                  lds_add64(&T.cnt[CLR_C_SRC_BORE_REFL], (uint32_t)sbr);
+                  if (L.st != 3 && sbr > 0) atomicAdd(&T.cnt[CLR_C_SRC_REFL_ACCEPT], 1ull);
+                }
+              }
               alive = true;
             } else {
               exhausted = true;

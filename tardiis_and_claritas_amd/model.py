@@ -164,7 +164,15 @@ class ForwardResult:
     bottom_exit_fraction: float
     source_bore_surface_model: str
     detector_bore_surface_model: str
+    bore_refractive_index: float
+    ring_inner_face_surface_model: str
+    ring_inner_face_refractive_index: float
+    max_ring_face_bounces: int
     source_collimator_acceptance_fraction: float
+    source_direct_acceptance_fraction: float
+    source_reflected_acceptance_fraction: float
+    source_bore_interactions_per_ray: float
+    source_bore_reflections_per_ray: float
     hardware_native_direct_hits: np.ndarray
     hardware_mirror_direct_hits: np.ndarray
     hardware_symmetry_direct_scores: np.ndarray
@@ -172,10 +180,22 @@ class ForwardResult:
     hardware_mirror_stray_hits: np.ndarray
     hardware_symmetry_stray_scores: np.ndarray
     detector_stray_fraction_by_channel: np.ndarray
+    detector_stray_detection_fraction: float
+    hardware_native_ring_face_stray_hits: np.ndarray
+    hardware_mirror_ring_face_stray_hits: np.ndarray
+    hardware_symmetry_ring_face_stray_scores: np.ndarray
+    detector_ring_face_stray_fraction_by_channel: np.ndarray
+    detector_ring_face_stray_detection_fraction: float
     photodiode_response_model: str
+    sfh213_half_angle_deg: float
+    sfh213_directional_fit_exponent: float
     photodiode_native_weighted_scores: np.ndarray
     photodiode_mirror_weighted_scores: np.ndarray
     photodiode_symmetry_weighted_scores: np.ndarray
+    normalized_photodiode_native_response: np.ndarray
+    normalized_photodiode_mirror_response: np.ndarray
+    normalized_photodiode_symmetry_response: np.ndarray
+    photodiode_relative_response_by_channel: np.ndarray
     detector_mean_incidence_angle_deg_by_channel: np.ndarray
     hardware_candidate_detection_fraction: float
     entered_water_fraction: float
@@ -215,7 +235,9 @@ class ClaritasForwardModel:
     physics: every event law except the moment-constrained one, smooth and
     rough (Smith microsurface walk, CLARITAS.py:2314-2430) particle surfaces,
     the coherent Mie and the incoherent spheroid-morphology wave channels
-    (CLARITAS.py:2655-2866), absorbing bores, ideal photodiodes.
+    (CLARITAS.py:2655-2866), absorbing or reflecting (smooth Fresnel nylon)
+    collimator bores and ring face, ideal or SFH213-weighted photodiodes
+    (CLARITAS.py:1263-1374, 4283-4376).
     Extensions: event law 'plain_mie',
     source 'collimated_disk', surface 'rankine_vortex', sediment models
     'rz_field' / 'xyz_field' / 'grid_per_bin' with a caller-supplied FP32
@@ -227,8 +249,9 @@ class ClaritasForwardModel:
     SOURCE_ANGULAR_MODEL_CODES = abi.SOURCE_ANGULAR_MODEL_CODES
     FREE_SURFACE_MODEL_CODES = abi.FREE_SURFACE_MODEL_CODES
     SEDIMENT_TRANSPORT_MODEL_CODES = abi.SEDIMENT_TRANSPORT_MODEL_CODES
-    BORE_SURFACE_MODEL_CODES = {'absorbing': 0}
-    PHOTODIODE_RESPONSE_MODEL_CODES = {'ideal_hard_aperture': 0}
+    BORE_SURFACE_MODEL_CODES = {'absorbing': 0, 'smooth_fresnel_nylon': 1}       # CLARITAS.py:5558
+    RING_FACE_SURFACE_MODEL_CODES = {'absorbing': 0, 'smooth_fresnel_nylon': 1}  # CLARITAS.py:5559
+    PHOTODIODE_RESPONSE_MODEL_CODES = {'ideal_hard_aperture': 0, 'sfh213_datasheet_fit': 1}  # CLARITAS.py:5560
     SPHEROID_ORIENTATION_MODEL_CODES = {'isotropic': 0, 'local_tangential_axial_vmf': 1}   # CLARITAS.py:5564
 
     def __init__(
@@ -252,6 +275,11 @@ class ClaritasForwardModel:
         source_beam_sigma_m=1.0e-5,
         source_bore_surface_model='absorbing',
         detector_bore_surface_model='absorbing',
+        bore_refractive_index=1.53,
+        max_bore_bounces=16,
+        ring_inner_face_surface_model='absorbing',
+        ring_inner_face_refractive_index=1.53,
+        max_ring_face_bounces=16,
         photodiode_response_model='ideal_hard_aperture',
         particle_wavelength_m=622e-9,
         particle_optical_constants_by_material=None,
@@ -324,6 +352,11 @@ class ClaritasForwardModel:
         self.beam_radius = float(source_beam_radius_m)
         self.source_bore_surface_model = str(source_bore_surface_model).strip().lower()
         self.detector_bore_surface_model = str(detector_bore_surface_model).strip().lower()
+        self.bore_refractive_index = float(bore_refractive_index)
+        self.max_bore_bounces = int(max_bore_bounces)
+        self.ring_inner_face_surface_model = str(ring_inner_face_surface_model).strip().lower()
+        self.ring_inner_face_refractive_index = float(ring_inner_face_refractive_index)
+        self.max_ring_face_bounces = int(max_ring_face_bounces)
         self.photodiode_response_model = str(photodiode_response_model).strip().lower()
         self.particle_wavelength = float(particle_wavelength_m)
         self.particle_optical_constants_by_material = particle_optical_constants_by_material
@@ -399,11 +432,22 @@ class ClaritasForwardModel:
             raise ValueError(f'unsupported free_surface_model {self.free_surface_model!r}')
         if self.sediment_transport_model not in self.SEDIMENT_TRANSPORT_MODEL_CODES:
             raise ValueError(f'unsupported sediment_transport_model {self.sediment_transport_model!r}')
+        # CLARITAS.py:5966-5991
         if self.source_bore_surface_model not in self.BORE_SURFACE_MODEL_CODES or \
                 self.detector_bore_surface_model not in self.BORE_SURFACE_MODEL_CODES:
-            raise ValueError('only absorbing bores are modelled')
+            raise ValueError('bore surface model must be absorbing or smooth_fresnel_nylon')
+        if self.ring_inner_face_surface_model not in self.RING_FACE_SURFACE_MODEL_CODES:
+            raise ValueError('ring inner face surface model must be absorbing or smooth_fresnel_nylon')
+        if not (self.ring_inner_face_refractive_index > 0.0 and math.isfinite(self.ring_inner_face_refractive_index)):
+            raise ValueError('ring_inner_face_refractive_index must be finite and positive')
+        if self.max_ring_face_bounces < 1:
+            raise ValueError('max_ring_face_bounces must be >=1')
+        if not self.bore_refractive_index > 1.0:
+            raise ValueError('bore_refractive_index must be >1')
+        if self.max_bore_bounces < 0:
+            raise ValueError('max_bore_bounces must be >=0')
         if self.photodiode_response_model not in self.PHOTODIODE_RESPONSE_MODEL_CODES:
-            raise ValueError('only the ideal hard-aperture photodiode is modelled')
+            raise ValueError('photodiode_response_model must be ideal_hard_aperture or sfh213_datasheet_fit')
         # CLARITAS.py:5797-5826
         if not (math.isfinite(self.particle_surface_rms_slope_deg) and 0.0 <= self.particle_surface_rms_slope_deg < 80.0):
             raise ValueError('particle_surface_rms_slope_deg must be finite and in [0,80) degrees')
@@ -466,6 +510,12 @@ class ClaritasForwardModel:
         c.spheroid_aspect = self.wave_spheroid_aspect_ratio
         c.spheroid_kappa = self.wave_spheroid_orientation_kappa
         c.spheroid_orientation = self.SPHEROID_ORIENTATION_MODEL_CODES[self.wave_spheroid_orientation_model]
+        c.bore_n, c.ring_face_n = self.bore_refractive_index, self.ring_inner_face_refractive_index
+        c.source_bore_model = self.BORE_SURFACE_MODEL_CODES[self.source_bore_surface_model]
+        c.detector_bore_model = self.BORE_SURFACE_MODEL_CODES[self.detector_bore_surface_model]
+        c.ring_face_model = self.RING_FACE_SURFACE_MODEL_CODES[self.ring_inner_face_surface_model]
+        c.photodiode_model = self.PHOTODIODE_RESPONSE_MODEL_CODES[self.photodiode_response_model]
+        c.max_bore_bounces, c.max_ring_face_bounces = self.max_bore_bounces, self.max_ring_face_bounces
         for i, a in enumerate(self.aux_detector_angles_deg):
             c.aux_angles_deg[i] = float(a)
         c.n_aux = int(self.aux_detector_angles_deg.size)
@@ -668,6 +718,7 @@ class ClaritasForwardModel:
                 raise ValueError('number of rays must be positive')
             T, _, ms = self.run_tallies(requested, seed, ray_id_start, light=lacc)
             batches = [T['det'][g] for g in range(T['n_groups'])]
+            pbatches = [T['photodiode'][g] for g in range(T['n_groups'])]
             group_n = T['group_histories']
             stop_reason, total = 'fixed_n_rays', requested
             det_sum = T['det'].sum(axis=0)
@@ -679,7 +730,7 @@ class ClaritasForwardModel:
             minN = int(min_rays if min_rays is not None else min(self.statistics_batch_rays, maxN))
             if minN < 0 or maxN < minN or maxN < 1:
                 raise ValueError('invalid min_rays/max_rays')
-            total, ms, batches, group_n = 0, 0.0, [], []
+            total, ms, batches, pbatches, group_n = 0, 0.0, [], [], []
             acc = None
             stop_reason = 'max_rays'
             prev_cum, recent = None, []
@@ -689,10 +740,11 @@ class ClaritasForwardModel:
                 ms += msb
                 d = Tb['det'].sum(axis=0)
                 batches.append(d)
+                pbatches.append(Tb['photodiode'].sum(axis=0))
                 group_n.append(b)
                 acc = Tb if acc is None else self._merge(acc, Tb)
                 total += b
-                sym = self._sym(np.stack(batches).sum(axis=0))
+                sym = self._selected(np.stack(batches).sum(axis=0), np.stack(pbatches).sum(axis=0))
                 l1 = float(np.sum(np.abs(_norm(sym) - prev_cum))) if prev_cum is not None else float('nan')
                 if np.isfinite(l1):
                     recent.append(l1)
@@ -710,7 +762,7 @@ class ClaritasForwardModel:
             T['n_groups'] = len(batches)
             det_sum = np.stack(batches).sum(axis=0)
             group_n = np.asarray(group_n)
-        res = self._result(mat, T, det_sum, batches, group_n, requested, total, adaptive, stop_reason, ms)
+        res = self._result(mat, T, det_sum, batches, pbatches, group_n, requested, total, adaptive, stop_reason, ms)
         if lacc is not None:
             res.light_transport_diagnostics = lacc.result(total)
             light.check_closure(res.light_transport_diagnostics)
@@ -719,7 +771,7 @@ class ClaritasForwardModel:
     @staticmethod
     def _merge(a, b):
         out = dict(a)
-        for k in ('aux', 'incidence_sum', 'status', 'failures', 'counters', 'ic_hist'):
+        for k in ('aux', 'incidence_sum', 'status', 'failures', 'counters', 'ic_hist', 'stray', 'face_stray'):
             out[k] = a[k] + b[k]
         return out
 
@@ -729,17 +781,37 @@ class ClaritasForwardModel:
             return 0.5 * (det[0].astype(float) + det[1].astype(float))
         return det[0].astype(float)
 
-    def _result(self, mat, T, det, batches, group_n, requested, total, adaptive, stop_reason, ms):
+    def _pair(self, a, b):
+        """Symmetry-averaged (or native) pair of per-channel arrays."""
+        return 0.5 * (np.asarray(a, float) + np.asarray(b, float)) if self.symmetry_average_exact \
+            else np.asarray(a, float)
+
+    def _selected(self, det, pd):
+        """The score the reference selects: hardware counts for the ideal
+        aperture, photodiode-weighted hits otherwise (CLARITAS.py:8798-8802)."""
+        if self.photodiode_response_model == 'ideal_hard_aperture':
+            return self._sym(det)
+        return self._pair(pd[0], pd[1])
+
+    def _result(self, mat, T, det, batches, pbatches, group_n, requested, total, adaptive, stop_reason, ms):
         cnt = T['counters']
         ci = abi.C_IDX
         native, mirror = det[0].astype(np.int64), det[1].astype(np.int64)
         lnat, lmir = det[2].astype(np.int64), det[3].astype(np.int64)
-        sym = self._sym(det)
+        sym = self._sym(det)                      # hardware counts
+        pd = np.stack(pbatches).sum(axis=0)       # photodiode-weighted [2][18]
+        pd_sym = self._pair(pd[0], pd[1])
+        sel = self._selected(det, pd)
         lsym = 0.5 * (lnat + lmir) if self.symmetry_average_exact else lnat.astype(float)
         hw_batches = [self._sym(b) for b in batches]
+        sel_batches = [self._selected(b, p) for b, p in zip(batches, pbatches)]
         lg_batches = [(0.5 * (b[2] + b[3]) if self.symmetry_average_exact else b[2].astype(float)) for b in batches]
-        se = _jackknife_normalized_se(hw_batches)
+        se = _jackknife_normalized_se(sel_batches)
+        hw_se = _jackknife_normalized_se(hw_batches)
         lse = _jackknife_normalized_se(lg_batches)
+        stray, face = T['stray'], T['face_stray']
+        stray_sym, face_sym = self._pair(stray[0], stray[1]), self._pair(face[0], face[1])
+        frac = lambda a: np.divide(a, sym, out=np.zeros(18), where=sym > 0)
         status = T['status']
         fails = T['failures']
         n_fail = int(fails.sum())
@@ -765,17 +837,16 @@ class ClaritasForwardModel:
                        'batch_native_hits': int(batches[g][0].sum()), 'batch_mirror_hits': int(batches[g][1].sum()),
                        'batch_hardware_candidate_score': float(hw_batches[g].sum())}
                       for g in range(len(batches))]
-        zero = np.zeros(18, dtype=np.int64)
         return ForwardResult(
             detector_angles_deg=DETECTOR_ANGLES_DEG.copy(),
             raw_hits=native.copy(),
-            normalized_response=_norm(sym),
+            normalized_response=_norm(sel),
             native_exact_hits=native.copy(),
             mirror_exact_hits=mirror.copy(),
-            symmetry_exact_scores=sym.copy(),
+            symmetry_exact_scores=sel.copy(),
             normalized_native_exact_response=_norm(native.astype(float)),
             normalized_mirror_exact_response=_norm(mirror.astype(float)),
-            normalized_symmetry_exact_response=_norm(sym),
+            normalized_symmetry_exact_response=_norm(sel),
             normalized_response_jackknife_se=se,
             hardware_native_hits=native.copy(),
             hardware_mirror_hits=mirror.copy(),
@@ -783,7 +854,7 @@ class ClaritasForwardModel:
             normalized_hardware_native_response=_norm(native.astype(float)),
             normalized_hardware_mirror_response=_norm(mirror.astype(float)),
             normalized_hardware_symmetry_response=_norm(sym),
-            hardware_normalized_jackknife_se=se,
+            hardware_normalized_jackknife_se=hw_se,
             legacy_native_channel_counts=lnat,
             legacy_mirror_channel_counts=lmir,
             legacy_symmetry_channel_scores=lsym,
@@ -858,18 +929,38 @@ class ClaritasForwardModel:
             bottom_exit_fraction=status[7] / max(total, 1),
             source_bore_surface_model=self.source_bore_surface_model,
             detector_bore_surface_model=self.detector_bore_surface_model,
+            bore_refractive_index=self.bore_refractive_index,
+            ring_inner_face_surface_model=self.ring_inner_face_surface_model,
+            ring_inner_face_refractive_index=self.ring_inner_face_refractive_index,
+            max_ring_face_bounces=self.max_ring_face_bounces,
             source_collimator_acceptance_fraction=(total - status[3]) / max(total, 1),
-            hardware_native_direct_hits=native.copy(),
-            hardware_mirror_direct_hits=mirror.copy(),
-            hardware_symmetry_direct_scores=sym.copy(),
-            hardware_native_stray_hits=zero.copy(),
-            hardware_mirror_stray_hits=zero.copy(),
-            hardware_symmetry_stray_scores=np.zeros(18),
-            detector_stray_fraction_by_channel=np.zeros(18),
+            source_direct_acceptance_fraction=(total - status[3] - cnt[ci['src_reflected_accept']]) / max(total, 1),
+            source_reflected_acceptance_fraction=cnt[ci['src_reflected_accept']] / max(total, 1),
+            source_bore_interactions_per_ray=cnt[ci['src_bore_interactions']] / max(total, 1),
+            source_bore_reflections_per_ray=cnt[ci['src_bore_reflections']] / max(total, 1),
+            hardware_native_direct_hits=native - stray[0],
+            hardware_mirror_direct_hits=mirror - stray[1],
+            hardware_symmetry_direct_scores=sym - stray_sym,
+            hardware_native_stray_hits=stray[0].copy(),
+            hardware_mirror_stray_hits=stray[1].copy(),
+            hardware_symmetry_stray_scores=stray_sym,
+            detector_stray_fraction_by_channel=frac(stray_sym),
+            detector_stray_detection_fraction=float(stray_sym.sum()) / max(total, 1),
+            hardware_native_ring_face_stray_hits=face[0].copy(),
+            hardware_mirror_ring_face_stray_hits=face[1].copy(),
+            hardware_symmetry_ring_face_stray_scores=face_sym,
+            detector_ring_face_stray_fraction_by_channel=frac(face_sym),
+            detector_ring_face_stray_detection_fraction=float(face_sym.sum()) / max(total, 1),
             photodiode_response_model=self.photodiode_response_model,
-            photodiode_native_weighted_scores=native.astype(float),
-            photodiode_mirror_weighted_scores=mirror.astype(float),
-            photodiode_symmetry_weighted_scores=sym.copy(),
+            sfh213_half_angle_deg=10.0,
+            sfh213_directional_fit_exponent=3.169925001442312,
+            photodiode_native_weighted_scores=pd[0].copy(),
+            photodiode_mirror_weighted_scores=pd[1].copy(),
+            photodiode_symmetry_weighted_scores=pd_sym,
+            normalized_photodiode_native_response=_norm(pd[0]),
+            normalized_photodiode_mirror_response=_norm(pd[1]),
+            normalized_photodiode_symmetry_response=_norm(pd_sym),
+            photodiode_relative_response_by_channel=frac(pd_sym),
             detector_mean_incidence_angle_deg_by_channel=mean_inc,
             hardware_candidate_detection_fraction=float(sym.sum()) / max(total, 1),
             entered_water_fraction=entered / max(total, 1),
