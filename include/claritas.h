@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 5
+#define CLR_ABI_VERSION 6
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -355,6 +355,52 @@ typedef struct clr_light_out {
                                         history id, tolerance (FP32 bits), recorded flag */
 } clr_light_out;
 
+/* ---- causal path provenance of one hardware-detected history: the
+ * per-ray fields trace_kernel writes for the reference's detector-path,
+ * causal and H170 diagnostics (CLARITAS.py:4629-4647 state, 5076-5103
+ * write-out, with its sentinels: -1 for an undefined angle, path or
+ * radius, 0 for an undefined position or class).  Event type codes
+ * (CLARITAS.py:5325-5334): 1 geometric external (entry) reflection,
+ * 2 geometric transmission/refraction, 3 geometric internal-reflection
+ * path, 4 coherent Mie wave, 5-7 the morphology-wave counterparts of 1-3.
+ * Bins are particle-size (PSD) bins, -1 when none. ---- */
+typedef struct clr_causal_record {
+  uint64_t history;             /* history id (Philox counter) */
+  int32_t native_id, mirror_id; /* hardware detector channels, -1 none */
+  int32_t ic, gic, wic, wcoh;   /* particle events: all, geometric, wave, coherent wave */
+  int32_t per, pir, pabs;       /* entry reflections, internal reflections, absorptions */
+  int32_t acrylic_reflections;  /* cell inner + outer + bottom inner + outer reflections */
+  int32_t acrylic_returns;      /* returns to the water from the acrylic or headspace after a particle event */
+  int32_t events_after_return;  /* particle events after such a return */
+  int32_t first_type, last_type, last_order, first_backscatter, first_source_hemisphere;
+  int32_t first_bin, last_bin, first_geometric_bin, last_geometric_bin, first_wave_bin, last_wave_bin;
+  int32_t branch_transitions, type_transitions;
+  int32_t small_deflections, medium_deflections, large_deflections;  /* < 30, < 90, >= 90 deg */
+  int32_t max_type, max_order, max_bin, max_macro_bounces, max_micro_order;
+  float first_deflection, last_deflection;       /* deg */
+  float internal_path, internal_opl;             /* summed over the history's particle events (m) */
+  float water_path_after_last;                   /* water path after the last particle event (m) */
+  float cumulative_deflection, cumulative_deflection_sq, max_deflection;
+  float turn_x, turn_y, turn_z;  /* site of the largest turn (m) */
+  float first_x, first_y, first_z, last_x, last_y, last_z;  /* first / last event sites (m) */
+  float mean_r, mean_z, min_r, max_r, min_z, max_z;
+  float water_path_first, water_path_last;       /* water path at the first / last event (m) */
+  float max_internal_path, max_path_over_diameter, max_incidence, max_exit;  /* largest turn (m, -, deg, deg) */
+} clr_causal_record;
+
+/* Caller-owned record buffer of clr_run_causal.  The call traces the
+ * histories [range_start, range_start + range_count) of an n-history run
+ * (range_count 0: all n), with the run's jackknife groups, so that the
+ * tallies of consecutive ranges add up to those of the whole run.  count
+ * receives the number of detected histories in the range; min(count,
+ * capacity) records are written, in no particular order. */
+typedef struct clr_causal_out {
+  clr_causal_record* records;
+  uint64_t capacity;
+  uint64_t count;
+  uint64_t range_start, range_count;
+} clr_causal_out;
+
 struct clr_ctx;
 typedef struct clr_ctx clr_ctx;
 
@@ -362,7 +408,7 @@ typedef struct clr_ctx clr_ctx;
  * launch-arity check validate_claritas.py:972-1001). id: 0 config,
  * 1 particles, 2 sediment, 3 mie_out, 4 tables_out, 5 tallies, 6 record,
  * 7 batch_item, 8 launch, 9 meridional, 10 meridional_out, 11 light_config,
- * 12 light_out. Returns 0 for an unknown id. */
+ * 12 light_out, 13 causal_record. Returns 0 for an unknown id. */
 int clr_abi_version(void);
 size_t clr_sizeof(int struct_id);
 
@@ -436,6 +482,17 @@ int clr_run_light(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offs
  * (x0, y0, z0, vx, vy, vz, dist; history id i, particle order i & 1), mode 1
  * records event site i of data[n][3] into the scatter-event grids, in the
  * box [-rin, rin]^2 x [zmin, light->zmax].  Outputs as clr_run_light. */
+/* clr_run with the causal path provenance of every history that reaches a
+ * hardware detector (native or mirror), for the reference's detector-path,
+ * causal and H170 diagnostics (CLARITAS.py:7352-7613 state, 8017-8480
+ * reductions).  The provenance is observational: the histories, their
+ * random numbers and the tallies are those of clr_run.  With light non-NULL
+ * the light-transport diagnostics are recorded in the same pass (as
+ * clr_run_light).  Returns CLR_E_ARG when causal is NULL. */
+int clr_run_causal(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offset,
+                   const clr_launch* launch, const clr_light_config* light, clr_tallies* out,
+                   clr_light_out* light_out, clr_causal_out* causal);
+
 int clr_light_probe(clr_ctx* ctx, int mode, int n, const float* data, const clr_light_config* light, double rin,
                     double zmin, clr_light_out* light_out);
 

@@ -1874,16 +1874,33 @@ static int smooth_interface(rng_t* r, float bx, float by, float bz, float n_top,
  * TOP side has normal b: returns MS_ESCAPE with out_top = 1 (left on the
  * top side) or 0 (crossed), else MS_NUMERIC / MS_WATCHDOG with fc set.
  * CLARITAS.py:2314-2430 */
+static int micro_interface_ord(rng_t* r, float alpha, float bx, float by, float bz, float n_top, float k_top,
+                               float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top, int* fc,
+                               int* order);
+/* the walk without its scattering order */
 static int micro_interface(rng_t* r, float alpha, float bx, float by, float bz, float n_top, float k_top,
                            float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top, int* fc) {
+  int order = 0;
+  return micro_interface_ord(r, alpha, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc, &order);
+}
+/* order: microfacet scatterings of the walk (the smooth limit counts 1),
+ * CLARITAS.py:2328-2373 */
+static int micro_interface_ord(rng_t* r, float alpha, float bx, float by, float bz, float n_top, float k_top,
+                               float n_bot, float k_bot, float* dx, float* dy, float* dz, int* out_top, int* fc,
+                               int* order) {
   *fc = 0;
+  *order = 0;
   normalize3(&bx, &by, &bz);
   normalize3(dx, dy, dz);
   if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(*dx) || !isfinite(*dy) || !isfinite(*dz)) {
     *fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC;
   }
   if (dot3(*dx, *dy, *dz, bx, by, bz) >= -1.0e-7f) { *fc = MF_SIDE_STATE; return MS_NUMERIC; }
-  if (alpha <= 1.0e-7f) return smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+  if (alpha <= 1.0e-7f) {
+    int st = smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+    if (st == MS_ESCAPE) *order = 1;
+    return st;
+  }
   float h = 1.0f + (2.0f * 0.999f - 1.0f);
   int outside = 1;
   for (int it = 0; it < MICRO_WATCHDOG_LIMIT; ++it) {
@@ -1901,6 +1918,7 @@ static int micro_interface(rng_t* r, float alpha, float bx, float by, float bz, 
     }
     if (hs != MS_HIT || !isfinite(nh)) { *fc = f2 ? f2 : MF_HEIGHT_SAMPLE; return MS_NUMERIC; }
     h = nh;
+    (*order)++;
     const float wix = -(*dx), wiy = -(*dy), wiz = -(*dz);
     double mx, my, mz;
     f2 = 0;
@@ -1950,7 +1968,20 @@ static int micro_interface(rng_t* r, float alpha, float bx, float by, float bz, 
   return MS_WATCHDOG;
 }
 
-typedef struct { int pr, per, pir, pabs; uint32_t n_abs; } pcount_t;
+/* reflection / absorption counts, and the last encounter's record
+ * (CLARITAS.py:2503-2599): internal path (float sum of chords), entry
+ * incidence and exit angles (deg, -1 undefined), last chord index and the
+ * microfacet order of the last walk */
+typedef struct {
+  int pr, per, pir, pabs;
+  uint32_t n_abs;
+  float path, inc, ext;
+  int macro, micro;
+} pcount_t;
+static void pc_event_reset(pcount_t* pc) {
+  pc->path = 0.0f; pc->inc = -1.0f; pc->ext = -1.0f; pc->macro = 0; pc->micro = 0;
+}
+static float angle_deg(float c) { return acosf(fminf(fmaxf(c, 0.0f), 1.0f)) * (180.0f / PI_F); }
 
 /* Fresnel sphere (smooth or rough surface): entry, internal chords with
  * Beer-Lambert absorption 4 pi k / lambda, exit or internal reflection.
@@ -1960,11 +1991,13 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
                               double cx, double cy, double cz, float* x, float* y, float* z,
                               float* vx, float* vy, float* vz, pcount_t* pc, int* pf, int* pfs) {
   *pf = PF_OK; *pfs = SUB_NONE;
+  pc_event_reset(pc);
   if (radius <= 0.0f) return 1;
   int top = 1, fc = 0;
-  int ms = micro_interface(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc);
+  pc->inc = angle_deg(-dot3(*vx, *vy, *vz, gnx, gny, gnz));
+  int ms = micro_interface_ord(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc, &pc->micro);
   if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
-  if (top) { pc->pr++; pc->per++; return 1; }
+  if (top) { pc->ext = angle_deg(dot3(*vx, *vy, *vz, gnx, gny, gnz)); pc->pr++; pc->per++; return 1; }
   if (dot3(*vx, *vy, *vz, gnx, gny, gnz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_ENTRY_SIDE; return 0; }
   const double rr = (double)radius, rr2 = rr * rr;
   double qx = rr * (double)gnx, qy = rr * (double)gny, qz = rr * (double)gnz;
@@ -1973,6 +2006,7 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
   qx *= rr / qn0; qy *= rr / qn0; qz *= rr / qn0;
   float alpha_abs = (P->wavelength > 0.0f && k_p > 0.0f) ? (4.0f * PI_F * k_p / P->wavelength) : 0.0f;
   for (int bounce = 0; bounce <= max_bounces; ++bounce) {
+    pc->macro = bounce;
     double dvx = *vx, dvy = *vy, dvz = *vz;
     double A = dvx * dvx + dvy * dvy + dvz * dvz, q2 = qx * qx + qy * qy + qz * qz;
     if (!(A > 0.0) || !isfinite(A) || !isfinite(q2)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
@@ -1995,11 +2029,13 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
       if (dist < chord) {
         qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
         *x = (float)(cx + qx); *y = (float)(cy + qy); *z = (float)(cz + qz);
+        pc->path += (float)dist;
         pc->pabs++;
         return 2;
       }
     }
     qx = tqx; qy = tqy; qz = tqz;
+    pc->path += (float)chord;
     double qn = sqrt(qx * qx + qy * qy + qz * qz);
     if (!(qn > 0.0) || !isfinite(qn)) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_CHORD; return 0; }
     double pj = rr / qn;
@@ -2008,10 +2044,11 @@ static int sphere_interaction(const fparams* P, rng_t* r, float n_m, float n_p, 
     float nox = (float)(qx / rr), noy = (float)(qy / rr), noz = (float)(qz / rr);
     normalize3(&nox, &noy, &noz);
     top = 1;
-    ms = micro_interface(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc);
+    ms = micro_interface_ord(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc, &pc->micro);
     if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
     if (!top) {
       if (dot3(*vx, *vy, *vz, nox, noy, noz) <= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_EXIT_SIDE; return 0; }
+      pc->ext = angle_deg(dot3(*vx, *vy, *vz, nox, noy, noz));
       return 1;
     }
     if (dot3(*vx, *vy, *vz, nox, noy, noz) >= 0.0f) { *pf = PF_MACRO_GEOMETRY; *pfs = SUB_SPHERE_INTERNAL_SIDE; return 0; }
@@ -2118,6 +2155,7 @@ static int wave_particle_event(const fparams* P, rng_t* r, float n_m, float n_p,
                                float fy, float fz, float* vx, float* vy, float* vz, pcount_t* pc, int* pf,
                                int* pfs) {
   *pf = PF_OK; *pfs = SUB_NONE;
+  pc_event_reset(pc);
   if (r_eq <= 0.0f) return 1;
   const float q = P->aspect;
   if (!(q > 0.0f) || !isfinite(q)) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_ASPECT; return 0; }
@@ -2140,27 +2178,31 @@ static int wave_particle_event(const fparams* P, rng_t* r, float n_m, float n_p,
   }
   const float invb2 = 1.0f / (b * b), delta = 1.0f / (a * a) - invb2, scale = fmaxf(a, b);
   int top = 1, fc = 0;
-  int ms = micro_interface(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc);
+  pc->inc = angle_deg(-dot3(*vx, *vy, *vz, gnx, gny, gnz));
+  int ms = micro_interface_ord(r, P->rough, gnx, gny, gnz, n_m, 0.0f, n_p, k_p, vx, vy, vz, &top, &fc, &pc->micro);
   if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
-  if (top) { pc->pr++; pc->per++; return 1; }
+  if (top) { pc->ext = angle_deg(dot3(*vx, *vy, *vz, gnx, gny, gnz)); pc->pr++; pc->per++; return 1; }
   if (dot3(*vx, *vy, *vz, gnx, gny, gnz) >= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_ENTRY; return 0; }
   const float alpha_abs = (P->wavelength > 0.0f && k_p > 0.0f) ? (4.0f * PI_F * k_p / P->wavelength) : 0.0f;
   for (int bounce = 0; bounce <= P->max_pb; ++bounce) {
+    pc->macro = bounce;
     const float chord = spheroid_chord(x, y, z, *vx, *vy, *vz, ux, uy, uz, invb2, delta, scale);
     if (chord <= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_CHORD; return 0; }
     if (alpha_abs > 0.0f) {
       pc->n_abs++;
       const double dist = -log(Uopen(r)) / (double)alpha_abs;
-      if (dist < (double)chord) { pc->pabs++; return 2; }
+      if (dist < (double)chord) { pc->path += (float)dist; pc->pabs++; return 2; }
     }
     x += chord * (*vx); y += chord * (*vy); z += chord * (*vz);
+    pc->path += chord;
     float nox, noy, noz;
     spheroid_normal(x, y, z, ux, uy, uz, invb2, delta, &nox, &noy, &noz);
     top = 1;
-    ms = micro_interface(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc);
+    ms = micro_interface_ord(r, P->rough, -nox, -noy, -noz, n_p, k_p, n_m, 0.0f, vx, vy, vz, &top, &fc, &pc->micro);
     if (ms != MS_ESCAPE) { *pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; *pfs = fc; return 0; }
     if (!top) {
       if (dot3(*vx, *vy, *vz, nox, noy, noz) <= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_EXIT_SIDE; return 0; }
+      pc->ext = angle_deg(dot3(*vx, *vy, *vz, nox, noy, noz));
       return 1;
     }
     if (dot3(*vx, *vy, *vz, nox, noy, noz) >= 0.0f) { *pf = PF_SPHEROID; *pfs = SUB_SPHEROID_INTERNAL_SIDE; return 0; }
@@ -2238,6 +2280,7 @@ static cond_t conditional_event(const orc_model* m, rng_t* r, int pidx, float x,
     trials++;
     cond_t c;
     memset(&c, 0, sizeof(c));
+    pc_event_reset(&c.pc);
     c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
     c.channel = (U53(r) < wave) ? ((coherent >= 1.0f || U53(r) < coherent) ? 1 : 2) : 0;
     if (c.channel == 1) {
@@ -2524,7 +2567,120 @@ typedef struct {
   float inc_native, inc_mirror;
   int sbi, sbr, nb, mb, nfb, mfb;  /* collimator / ring-face interactions and reflections */
   float pd_native, pd_mirror;      /* SFH213 weights of the accepted hits */
+  /* causal path provenance (CLARITAS.py:4629-4647), observational */
+  struct {
+    float wp, first_defl, last_defl, ipath, iopl, wp_last, cum, cum2, maxd;
+    float site_max[3], site_first[3], site_last[3];
+    float sum_r, sum_z, min_r, max_r, min_z, max_z, wp_first;
+    float m_path, m_pod, m_inc, m_exit;
+    int ret, after_ret, first_type, last_type, last_order, first_back, first_hemi;
+    int bins[6];  /* first / last particle, geometric, wave bin */
+    int btrans, etrans, nsmall, nmed, nlarge, prev_branch, prev_type;
+    int m_type, m_order, m_bin, m_macro, m_micro;
+  } pv;
 } hist_t;
+
+static void prov_reset(hist_t* o) {
+  memset(&o->pv, 0, sizeof(o->pv));
+  o->pv.first_defl = o->pv.last_defl = o->pv.wp_last = o->pv.maxd = o->pv.wp_first = -1.0f;
+  o->pv.min_r = o->pv.min_z = INF_F;
+  o->pv.max_r = o->pv.max_z = -INF_F;
+  o->pv.m_inc = o->pv.m_exit = -1.0f;
+  for (int k = 0; k < 6; ++k) o->pv.bins[k] = -1;
+  o->pv.m_bin = -1;
+}
+
+/* a surviving particle event of code `type` on bin `pidx` at site s, incoming
+ * direction vin, outgoing direction p->v; e holds the encounter's record.
+ * CLARITAS.py:4936-4980 */
+static void prov_event(const orc_model* m, hist_t* o, const float vin[3], const float s[3], int type, int pidx,
+                       const pcount_t* e) {
+  const photon_t* p = &o->ph;
+  float c = dot3(vin[0], vin[1], vin[2], p->vx, p->vy, p->vz);
+  c = fminf(fmaxf(c, -1.0f), 1.0f);
+  const float defl = acosf(c) * (180.0f / PI_F);
+  const float sr = sqrtf(fmaxf(s[0] * s[0] + s[1] * s[1], 0.0f));
+  o->pv.ipath += e->path;
+  if (pidx >= 0) o->pv.iopl += e->path * m->n_f[pidx];
+  o->pv.cum += defl;
+  o->pv.cum2 += defl * defl;
+  if (defl > o->pv.maxd) {
+    o->pv.maxd = defl;
+    memcpy(o->pv.site_max, s, sizeof(float) * 3);
+    o->pv.m_type = type; o->pv.m_order = o->ic; o->pv.m_bin = pidx;
+    o->pv.m_macro = e->macro; o->pv.m_micro = e->micro; o->pv.m_path = e->path;
+    o->pv.m_pod = (pidx >= 0 && m->r_f[pidx] > 0.0f) ? e->path / (2.0f * m->r_f[pidx]) : 0.0f;
+    o->pv.m_inc = e->inc; o->pv.m_exit = e->ext;
+  }
+  if (defl < 30.0f) o->pv.nsmall++;
+  else if (defl < 90.0f) o->pv.nmed++;
+  else o->pv.nlarge++;
+  int branch = (type >= 1 && type <= 3) ? 1 : 2;
+  if (o->pv.prev_branch > 0 && branch != o->pv.prev_branch) o->pv.btrans++;
+  if (o->pv.prev_type > 0 && type != o->pv.prev_type) o->pv.etrans++;
+  o->pv.prev_branch = branch; o->pv.prev_type = type;
+  o->pv.sum_r += sr; o->pv.sum_z += s[2];
+  o->pv.min_r = fminf(o->pv.min_r, sr); o->pv.max_r = fmaxf(o->pv.max_r, sr);
+  o->pv.min_z = fminf(o->pv.min_z, s[2]); o->pv.max_z = fmaxf(o->pv.max_z, s[2]);
+  if (o->ic == 1) { memcpy(o->pv.site_first, s, sizeof(float) * 3); o->pv.wp_first = o->pv.wp; }
+  memcpy(o->pv.site_last, s, sizeof(float) * 3);
+  o->pv.wp_last = o->pv.wp;
+  if (pidx >= 0) {
+    int geo = (type >= 1 && type <= 3), wav = (type >= 4 && type <= 7);
+    if (o->pv.bins[0] < 0) o->pv.bins[0] = pidx;
+    o->pv.bins[1] = pidx;
+    if (geo) { if (o->pv.bins[2] < 0) o->pv.bins[2] = pidx; o->pv.bins[3] = pidx; }
+    if (wav) { if (o->pv.bins[4] < 0) o->pv.bins[4] = pidx; o->pv.bins[5] = pidx; }
+    if (o->pv.ret > 0) o->pv.after_ret++;
+  }
+  o->pv.last_type = type; o->pv.last_order = o->ic; o->pv.last_defl = defl;
+  if (o->ic == 1) {
+    o->pv.first_type = type; o->pv.first_defl = defl;
+    o->pv.first_back = (c < 0.0f); o->pv.first_hemi = (p->vy < 0.0f);
+  }
+}
+
+/* the record of a history, with the write-out sentinels of
+ * CLARITAS.py:5076-5103 */
+static void prov_record(const hist_t* o, uint64_t hist, clr_causal_record* q) {
+  const int any = o->ic > 0;
+  const photon_t* p = &o->ph;
+  memset(q, 0, sizeof(*q));
+  q->history = hist; q->native_id = o->native; q->mirror_id = o->mirror;
+  q->ic = o->ic; q->gic = o->gic; q->wic = o->wic; q->wcoh = o->wcoh;
+  q->per = o->pc.per; q->pir = o->pc.pir; q->pabs = o->pc.pabs > 0;
+  q->acrylic_reflections = p->cir + p->cor + p->bir + p->bor;
+  q->acrylic_returns = o->pv.ret; q->events_after_return = o->pv.after_ret;
+  q->first_type = o->pv.first_type; q->last_type = o->pv.last_type; q->last_order = o->pv.last_order;
+  q->first_backscatter = o->pv.first_back; q->first_source_hemisphere = o->pv.first_hemi;
+  q->first_bin = o->pv.bins[0]; q->last_bin = o->pv.bins[1];
+  q->first_geometric_bin = o->pv.bins[2]; q->last_geometric_bin = o->pv.bins[3];
+  q->first_wave_bin = o->pv.bins[4]; q->last_wave_bin = o->pv.bins[5];
+  q->branch_transitions = o->pv.btrans; q->type_transitions = o->pv.etrans;
+  q->small_deflections = o->pv.nsmall; q->medium_deflections = o->pv.nmed; q->large_deflections = o->pv.nlarge;
+  q->max_type = any ? o->pv.m_type : 0; q->max_order = any ? o->pv.m_order : 0;
+  q->max_bin = any ? o->pv.m_bin : -1;
+  q->max_macro_bounces = any ? o->pv.m_macro : 0; q->max_micro_order = any ? o->pv.m_micro : 0;
+  q->first_deflection = o->pv.first_defl; q->last_deflection = o->pv.last_defl;
+  q->internal_path = o->pv.ipath; q->internal_opl = o->pv.iopl;
+  q->water_path_after_last =
+      (o->pv.last_order > 0 && o->pv.wp_last >= 0.0f) ? fmaxf(o->pv.wp - o->pv.wp_last, 0.0f) : -1.0f;
+  q->cumulative_deflection = any ? o->pv.cum : -1.0f;
+  q->cumulative_deflection_sq = any ? o->pv.cum2 : -1.0f;
+  q->max_deflection = any ? o->pv.maxd : -1.0f;
+  if (any) {
+    q->turn_x = o->pv.site_max[0]; q->turn_y = o->pv.site_max[1]; q->turn_z = o->pv.site_max[2];
+    q->first_x = o->pv.site_first[0]; q->first_y = o->pv.site_first[1]; q->first_z = o->pv.site_first[2];
+    q->last_x = o->pv.site_last[0]; q->last_y = o->pv.site_last[1]; q->last_z = o->pv.site_last[2];
+  }
+  q->mean_r = any ? o->pv.sum_r / (float)o->ic : -1.0f;
+  q->mean_z = any ? o->pv.sum_z / (float)o->ic : -1.0f;
+  q->min_r = any ? o->pv.min_r : -1.0f; q->max_r = any ? o->pv.max_r : -1.0f;
+  q->min_z = any ? o->pv.min_z : -1.0f; q->max_z = any ? o->pv.max_z : -1.0f;
+  q->water_path_first = any ? o->pv.wp_first : -1.0f; q->water_path_last = any ? o->pv.wp_last : -1.0f;
+  q->max_internal_path = any ? o->pv.m_path : 0.0f; q->max_path_over_diameter = any ? o->pv.m_pod : 0.0f;
+  q->max_incidence = any ? o->pv.m_inc : -1.0f; q->max_exit = any ? o->pv.m_exit : -1.0f;
+}
 
 /* ======================================================================
  * Light-transport voxel diagnostics, CLARITAS.py:3312-3467 (observational:
@@ -2716,6 +2872,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
   rng_t r;
   rng_seed(&r, seed, hist);
   memset(o, 0, sizeof(*o));
+  prov_reset(o);
   o->native = o->mirror = -1;
   for (int a = 0; a < CLR_MAX_AUX; ++a) o->aux_native[a] = o->aux_mirror[a] = -1;
   photon_t* p = &o->ph;
@@ -2849,6 +3006,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
       if (Ld) light_segment(Ld, hist, p->x, p->y, p->z, p->vx, p->vy, p->vz, travel, o->ic);
       o->n_flight++;
       p->x += travel * p->vx; p->y += travel * p->vy; p->z += travel * p->vz;
+      o->pv.wp += travel;
       if (do_particle) {
         double er2 = (double)p->x * p->x + (double)p->y * p->y;
         if (er2 > ri2w + wtol || p->z < P->zmin - EPS_F || p->z > free_surface_z(P, p->x, p->y) + EPS_F) {
@@ -2877,6 +3035,10 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         }
         o->ic++;
         const float sx0 = p->x, sy0 = p->y, sz0 = p->z;  /* event site */
+        const float site[3] = {sx0, sy0, sz0}, vin[3] = {p->vx, p->vy, p->vz};
+        pcount_t none;
+        memset(&none, 0, sizeof(none));
+        pc_event_reset(&none);
         int use_wave = (LW > 0.0f && U(&r) * LT >= LG);
         int em = P->event_model;
         if (em == CLR_EVENT_PLAIN_MIE) {
@@ -2891,6 +3053,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->wic++; o->wcoh++;
           mie_scatter(&r, m, b, &p->vx, &p->vy, &p->vz);
           if (Ld) light_event(Ld, hist, sx0, sy0, sz0, 4);
+          prov_event(m, o, vin, site, 4, b, &none);
           continue;
         }
         int cond = (em == CLR_EVENT_HYBRID_EXACT_RATE);
@@ -2909,6 +3072,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
             break;
           }
           if (Ld) light_event(Ld, hist, sx0, sy0, sz0, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2));
+          prov_event(m, o, vin, site, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2), sel, &h.pc);
           continue;
         }
         if (use_wave) {
@@ -2925,11 +3089,13 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
             if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
             if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
             if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 1));
+            prov_event(m, o, vin, site, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 1), pidx, &o->pc);
             continue;
           }
           o->wcoh++;
           mie_scatter(&r, m, pidx, &p->vx, &p->vy, &p->vz);
           if (Ld) light_event(Ld, hist, sx0, sy0, sz0, 4);
+          prov_event(m, o, vin, site, 4, pidx, &none);
           continue;
         }
         o->gic++;
@@ -2950,6 +3116,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->gic--;
           if (h.channel == 0) o->gic++; else { o->wic++; if (h.channel == 1) o->wcoh++; }
           if (Ld) light_event(Ld, hist, sx0, sy0, sz0, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2));
+          prov_event(m, o, vin, site, h.channel == 1 ? 4 : light_code(h.pc.per, h.pc.pir, h.channel == 2), pidx, &h.pc);
           continue;
         }
         if (!accessible) { o->ic--; o->gic--; o->n_null++; if (Ld) light_mark(Ld, hist, 8); continue; }
@@ -2959,6 +3126,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
         if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
         if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 0));
+        prov_event(m, o, vin, site, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 0), pidx, &o->pc);
         continue;
       }
       if (axial_top || axial_bot) {
@@ -2979,20 +3147,20 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           p->vx = a; p->vy = b; p->vz = c;
           p->x -= EPS_F * snx; p->y -= EPS_F * sny; p->z -= EPS_F * snz;
           int hsub = 0, hw = headspace(P, &r, p, &hsub);
-          if (hw == 1) continue;
+          if (hw == 1) { if (o->ic > 0) o->pv.ret++; continue; }
           if (hw == 2) { st = 1; goto AIR_EXIT; }
           if (hw == 3) { st = 7; goto WRITE_OUT; }
           if (hw == 4) { st = 6; goto WRITE_OUT; }
           pf = PF_HEADSPACE_NO_HIT; pfs = hsub; st = 4; goto WRITE_OUT;
         } else {
           int bsub = 0, where = bottom_disc(P, &r, p, &bsub);
-          if (where == 1) continue;
+          if (where == 1) { if (o->ic > 0) o->pv.ret++; continue; }
           if (where == 2) { st = 1; goto AIR_EXIT; }
           if (where == 3) { st = 7; break; }
           if (where == 5) { st = 8; break; }
           if (where == 4) {
             int hsub = 0, hw = headspace(P, &r, p, &hsub);
-            if (hw == 1) continue;
+            if (hw == 1) { if (o->ic > 0) o->pv.ret++; continue; }
             if (hw == 2) { st = 1; goto AIR_EXIT; }
             if (hw == 3) { st = 7; goto WRITE_OUT; }
             if (hw == 4) { st = 6; goto WRITE_OUT; }
@@ -3018,13 +3186,13 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         }
         p->vx = a; p->vy = b; p->vz = c; p->x += EPS_F * nx; p->y += EPS_F * ny;
         int sub = 0, where = acrylic_solid(P, &r, p, &sub);
-        if (where == 1) continue;
+        if (where == 1) { if (o->ic > 0) o->pv.ret++; continue; }
         if (where == 2) { st = 1; goto AIR_EXIT; }
         if (where == 3) { st = 7; break; }
         if (where == 5) { st = 8; break; }
         if (where == 4) {
           int hsub = 0, hw = headspace(P, &r, p, &hsub);
-          if (hw == 1) continue;
+          if (hw == 1) { if (o->ic > 0) o->pv.ret++; continue; }
           if (hw == 2) { st = 1; goto AIR_EXIT; }
           if (hw == 3) { st = 7; goto WRITE_OUT; }
           if (hw == 4) { st = 6; goto WRITE_OUT; }
@@ -3199,7 +3367,8 @@ static void merge(clr_tallies* D, const clr_tallies* S) {
 }
 
 static int run_impl(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
-                    clr_tallies* out, clr_record* rec, uint64_t n_rec, const light_t* Ld) {
+                    clr_tallies* out, clr_record* rec, uint64_t n_rec, const light_t* Ld,
+                    clr_causal_record* crec, uint8_t* cdet) {
   if (!m || !out) return CLR_E_ARG;
   memset(out, 0, sizeof(*out));
   int G = m->fp.n_groups;
@@ -3238,6 +3407,10 @@ static int run_impl(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offs
         q->status = h.st; q->native_id = h.native; q->mirror_id = h.mirror; q->interactions = h.ic;
         q->failure = h.pf; q->draws = (int32_t)h.draws;
       }
+      if (crec) {
+        prov_record(&h, hist, &crec[i]);
+        cdet[i] = (h.native >= 0 || h.mirror >= 0);
+      }
     }
   }
   for (int t = 0; t < nt; ++t) merge(out, &part[t]);
@@ -3247,7 +3420,16 @@ static int run_impl(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offs
 
 int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
             clr_tallies* out, clr_record* rec, uint64_t n_rec) {
-  return run_impl(m, n, seed, offset, nthreads, out, rec, n_rec, NULL);
+  return run_impl(m, n, seed, offset, nthreads, out, rec, n_rec, NULL, NULL, NULL);
+}
+
+/* orc_run with the causal provenance record of every history (crec[n]);
+ * detected[i] = 1 when history i reached a hardware detector, the histories
+ * clr_run_causal records */
+int orc_run_causal(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+                   clr_tallies* out, clr_causal_record* crec, uint8_t* detected) {
+  if (!crec || !detected) return CLR_E_ARG;
+  return run_impl(m, n, seed, offset, nthreads, out, NULL, 0, NULL, crec, detected);
 }
 
 /* orc_run plus the light-transport diagnostics (clr_run_light contract):
@@ -3276,7 +3458,7 @@ int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset
   lo->error_count = lo->boundary_adjustment_count = 0;
   memset(lo->outside, 0, sizeof(lo->outside));
   light_t L = {lc, lo, (float)m->cfg.rin, (float)m->cfg.zmin, (float)lc->zmax};
-  return run_impl(m, n, seed, offset, nthreads, out, NULL, 0, &L);
+  return run_impl(m, n, seed, offset, nthreads, out, NULL, 0, &L, NULL, NULL);
 }
 
 /* the reference's segment and event test kernels (CLARITAS.py:3470-3485)

@@ -42,6 +42,10 @@ int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t history_offs
 
 /* orc_run plus the light-transport voxel diagnostics (clr_run_light
  * contract; the oracle initialises the caller's output arrays). */
+/* orc_run plus the causal provenance record of every history (crec[n],
+ * detected[n]: 1 when the history reached a hardware detector) */
+int orc_run_causal(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
+                   clr_tallies* out, clr_causal_record* crec, uint8_t* detected);
 int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
                   const clr_light_config* lc, clr_tallies* out, clr_light_out* lo);
 /* The reference's diagnostic test kernels (CLARITAS.py:3470-3485): segment i

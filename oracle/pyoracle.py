@@ -77,6 +77,9 @@ def load():
     lib.orc_run_light.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(abi.LightConfig),
                                   P(abi.Tallies), P(abi.LightOut)]
     lib.orc_run_light.restype = C.c_int
+    lib.orc_run_causal.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(abi.Tallies),
+                                   C.c_void_p, P(C.c_uint8)]
+    lib.orc_run_causal.restype = C.c_int
     for name in ('orc_light_segment_probe', 'orc_light_event_probe'):
         fn = getattr(lib, name)
         fn.argtypes = [P(abi.LightConfig), C.c_float, C.c_float, C.c_int, P(C.c_float), P(abi.LightOut)]
@@ -286,6 +289,18 @@ class OracleModel:
         if rc:
             raise RuntimeError(f'oracle light run failed with code {rc}')
         return abi.tallies_to_numpy(t), arrs, abi.light_scalars(out)
+
+    def run_causal(self, n, seed, history_offset=0, threads=0):
+        """orc_run_causal: (tallies dict, records of the detected histories
+        in history order)."""
+        recs = np.zeros(int(n), dtype=abi.causal_dtype())
+        det = np.zeros(int(n), dtype=np.uint8)
+        t = abi.Tallies()
+        rc = self.lib.orc_run_causal(self.h, int(n), int(seed), int(history_offset), int(threads), C.byref(t),
+                                     C.c_void_p(recs.ctypes.data), det.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc:
+            raise RuntimeError(f'oracle causal run failed with code {rc}')
+        return abi.tallies_to_numpy(t), recs[det.astype(bool)]
 
     def sediment_scale(self, bin_index, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float32)

@@ -35,6 +35,8 @@ _SIGS = {
     'clr_last_kernel_ms': (C.c_double, [C.c_void_p]),
     'clr_run_light': (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, P(abi.Launch), P(abi.LightConfig),
                                 P(abi.Tallies), P(abi.LightOut)]),
+    'clr_run_causal': (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, P(abi.Launch), P(abi.LightConfig),
+                                 P(abi.Tallies), P(abi.LightOut), P(abi.CausalOut)]),
     'clr_light_probe': (C.c_int, [C.c_void_p, C.c_int, C.c_int, P(C.c_float), P(abi.LightConfig), C.c_double,
                                   C.c_double, P(abi.LightOut)]),
     'clr_meridional_solve': (C.c_int, [C.c_void_p, P(abi.Config), P(abi.Particles), P(abi.Sediment),

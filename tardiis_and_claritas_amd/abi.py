@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -261,9 +261,43 @@ def light_scalars(o: LightOut) -> dict:
     return d
 
 
+# clr_causal_record field by field (include/claritas.h): the provenance of
+# one hardware-detected history, read as a numpy structured array
+CAUSAL_INT_FIELDS = (
+    'native_id', 'mirror_id', 'ic', 'gic', 'wic', 'wcoh', 'per', 'pir', 'pabs', 'acrylic_reflections',
+    'acrylic_returns', 'events_after_return', 'first_type', 'last_type', 'last_order', 'first_backscatter',
+    'first_source_hemisphere', 'first_bin', 'last_bin', 'first_geometric_bin', 'last_geometric_bin',
+    'first_wave_bin', 'last_wave_bin', 'branch_transitions', 'type_transitions', 'small_deflections',
+    'medium_deflections', 'large_deflections', 'max_type', 'max_order', 'max_bin', 'max_macro_bounces',
+    'max_micro_order')
+CAUSAL_FLOAT_FIELDS = (
+    'first_deflection', 'last_deflection', 'internal_path', 'internal_opl', 'water_path_after_last',
+    'cumulative_deflection', 'cumulative_deflection_sq', 'max_deflection', 'turn_x', 'turn_y', 'turn_z',
+    'first_x', 'first_y', 'first_z', 'last_x', 'last_y', 'last_z', 'mean_r', 'mean_z', 'min_r', 'max_r',
+    'min_z', 'max_z', 'water_path_first', 'water_path_last', 'max_internal_path', 'max_path_over_diameter',
+    'max_incidence', 'max_exit')
+
+
+def causal_dtype():
+    import numpy as np
+    return np.dtype([('history', '<u8')] + [(k, '<i4') for k in CAUSAL_INT_FIELDS] +
+                    [(k, '<f4') for k in CAUSAL_FLOAT_FIELDS])
+
+
+class CausalRecord(C.Structure):
+    _fields_ = [('history', C.c_uint64)] + [(k, C.c_int32) for k in CAUSAL_INT_FIELDS] + \
+        [(k, C.c_float) for k in CAUSAL_FLOAT_FIELDS]
+
+
+class CausalOut(C.Structure):
+    _fields_ = [('records', C.c_void_p), ('capacity', C.c_uint64), ('count', C.c_uint64),
+                ('range_start', C.c_uint64), ('range_count', C.c_uint64)]
+
+
 STRUCT_IDS = {
     0: Config, 1: Particles, 2: Sediment, 3: MieOut, 4: TablesOut, 5: Tallies, 6: Record,
     7: BatchItem, 8: Launch, 9: Meridional, 10: MeridionalOut, 11: LightConfig, 12: LightOut,
+    13: CausalRecord,
 }
 
 

@@ -107,6 +107,11 @@ struct clr_ctx {
   GraphCache graph;
   MlpState* mlp = nullptr;
   LightBufs light;
+  // causal provenance records of the run in progress (clr_run_causal)
+  DevBuf causal;
+  clr_causal_record* causal_rec = nullptr;
+  unsigned long long* causal_count = nullptr;
+  uint64_t causal_cap = 0;
 };
 
 namespace {
@@ -587,6 +592,9 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   L.block = sh.block;
   L.kind = sh.kind;
   L.light = light;
+  L.causal = c->causal_rec;
+  L.causal_count = c->causal_count;
+  L.causal_cap = c->causal_cap;
   const size_t words = sizeof(clr_tallies) * n_slots / 8;
   if (use_graph) {
     GraphCache& G = c->graph;
@@ -714,12 +722,17 @@ int light_readback(clr_ctx* c, const clr_light_config* lc, const LightDiag& D, c
 
 // Trace n histories of slot 0 on kernel variant `kind` (with the light
 // diagnostics when `light` is given) and reduce the tallies into *out.
+// histories [r0, r0 + rn) of an n-history run (rn 0: all), jackknife
+// groups numbered over the whole run
 int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_launch* launch, clr_tallies* out,
-               clr_record* records, uint64_t n_records, int kind, const LightDiag* light) {
+               clr_record* records, uint64_t n_records, int kind, const LightDiag* light, uint64_t r0 = 0,
+               uint64_t rn = 0) {
   if (n == 0) return fail(c, CLR_E_ARG, "n must be positive");
   if (n > (1ull << 56) || offset > (1ull << 62)) return fail(c, CLR_E_RANGE, "history range too large");
+  if (rn == 0) { r0 = 0; rn = n; }
+  if (r0 >= n || rn > n - r0) return fail(c, CLR_E_RANGE, "history sub-range outside the run");
   HIPCHK(c, hipSetDevice(c->device));
-  const RunShape sh = pick_shape(c, launch, n, kind);
+  const RunShape sh = pick_shape(c, launch, rn, kind);
   // one launch covers at most 2^31 histories so LDS u32 tallies cannot wrap
   const uint64_t per_launch = 1ull << 31;
   const int n_slots = (int)c->slots.size();
@@ -734,13 +747,13 @@ int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr
   }
   double total_ms = 0.0;
   const int G = c->slots[0]->cfg.n_groups;
-  for (uint64_t done = 0; done < n; done += per_launch) {
-    const uint64_t m = std::min(per_launch, n - done);
+  for (uint64_t done = r0; done < r0 + rn; done += per_launch) {
+    const uint64_t m = std::min(per_launch, r0 + rn - done);
     // histories of this launch keep their global group by numbering them
     // within the whole run: items carry start offsets into [0, n)
     QueueBuilder q(sh, (double)m);
     q.add(0, done, m, n, seed, offset);
-    if (int e = launch_items(c, q.items, sh, false, done == 0 ? d_rec : nullptr, done == 0 ? nrec : 0, light))
+    if (int e = launch_items(c, q.items, sh, false, done == r0 ? d_rec : nullptr, done == r0 ? nrec : 0, light))
       return e;
     total_ms += c->last_ms;
     HIPCHK(c, hipMemcpy(host.data(), c->tallies.p, sizeof(clr_tallies), hipMemcpyDeviceToHost));
@@ -758,8 +771,19 @@ int run_single(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr
   *out = acc[0];
   out->n_groups = G;
   group_sizes(n, G, out->group_histories);
+  if (rn != n) {  // the range's share of each group: i in [r0, r0 + rn) with floor(i G / n) == g
+    uint64_t lo[CLR_MAX_GROUPS + 1] = {0};
+    for (int g = 0; g <= G; ++g) {  // first history of group g: ceil(g n / G)
+      const unsigned __int128 v = (unsigned __int128)g * n;
+      lo[g] = (uint64_t)((v + (unsigned)G - 1) / (unsigned)G);
+    }
+    for (int g = 0; g < G; ++g) {
+      const uint64_t a = std::max(lo[g], r0), b = std::min(lo[g + 1], r0 + rn);
+      out->group_histories[g] = b > a ? b - a : 0;
+    }
+  }
   ideal_photodiode(c->slots[0]->cfg, *out);
-  out->counters[CLR_C_HISTORIES] = n;  // launched histories (the device counts per-event work only)
+  out->counters[CLR_C_HISTORIES] = rn;  // launched histories (the device counts per-event work only)
   if (d_rec) HIPCHK(c, hipMemcpy(records, d_rec, sizeof(clr_record) * nrec, hipMemcpyDeviceToHost));
   return 0;
 }
@@ -785,6 +809,7 @@ size_t clr_sizeof(int id) {
     case 10: return sizeof(clr_meridional_out);
     case 11: return sizeof(clr_light_config);
     case 12: return sizeof(clr_light_out);
+    case 13: return sizeof(clr_causal_record);
     default: return 0;
   }
 }
@@ -990,6 +1015,44 @@ int clr_run_light(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const 
   if (int e = run_single(c, n, seed, offset, launch, out, nullptr, 0, CLR_KERNEL_LIGHT, c->light.desc.as<LightDiag>()))
     return e;
   return light_readback(c, lc, D, lo);
+}
+
+int clr_run_causal(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const clr_launch* launch,
+                   const clr_light_config* lc, clr_tallies* out, clr_light_out* lo, clr_causal_out* co) {
+  if (!c || !out || !co || (lc && !lo)) return fail(c, CLR_E_ARG, "null argument");
+  if (co->capacity && !co->records) return fail(c, CLR_E_ARG, "causal records buffer is NULL");
+  if (co->capacity > (1ull << 40)) return fail(c, CLR_E_RANGE, "causal capacity too large");
+  if (c->slots.empty() || !c->slots[0]->ready) return fail(c, CLR_E_STATE, "clr_prepare(slot 0) has not run");
+  HIPCHK(c, hipSetDevice(c->device));
+  // record area followed by the 8-byte append counter
+  const size_t rec_bytes = sizeof(clr_causal_record) * (size_t)co->capacity;
+  HIPCHK(c, c->causal.ensure(rec_bytes + 256));
+  char* base = static_cast<char*>(c->causal.p);
+  c->causal_rec = reinterpret_cast<clr_causal_record*>(base);
+  c->causal_count = reinterpret_cast<unsigned long long*>(base + rec_bytes);
+  c->causal_cap = co->capacity;
+  HIPCHK(c, hipMemsetAsync(c->causal_count, 0, sizeof(unsigned long long), c->stream));
+  LightDiag D{};
+  int e = 0;
+  if (lc) {
+    const clr_config& g = c->slots[0]->cfg;
+    e = light_setup(c, lc, (float)g.rin, (float)g.zmin, D);
+    if (!e)
+      e = run_single(c, n, seed, offset, launch, out, nullptr, 0, CLR_KERNEL_LIGHT, c->light.desc.as<LightDiag>(),
+                     co->range_start, co->range_count);
+    if (!e) e = light_readback(c, lc, D, lo);
+  } else {
+    e = run_single(c, n, seed, offset, launch, out, nullptr, 0, CLR_KERNEL_CAUSAL, nullptr, co->range_start,
+                   co->range_count);
+  }
+  c->causal_rec = nullptr; c->causal_count = nullptr; c->causal_cap = 0;
+  if (e) return e;
+  unsigned long long cnt = 0;
+  HIPCHK(c, hipMemcpy(&cnt, base + rec_bytes, sizeof(cnt), hipMemcpyDeviceToHost));
+  co->count = cnt;
+  const uint64_t w = std::min<uint64_t>(cnt, co->capacity);
+  if (w) HIPCHK(c, hipMemcpy(co->records, base, sizeof(clr_causal_record) * w, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int clr_light_probe(clr_ctx* c, int mode, int n, const float* data, const clr_light_config* lc, double rin,

@@ -100,6 +100,11 @@ struct TransportLaunch {
   int grid, block;
   int kind;   // kernel variant, CLR_KERNEL_*
   const LightDiag* light;    // CLR_KERNEL_LIGHT only
+  // causal provenance records of detected histories (CAUSAL, and LIGHT when
+  // non-null): appended at *causal_count, written while below causal_cap
+  clr_causal_record* causal;
+  unsigned long long* causal_count;
+  uint64_t causal_cap;
 };
 
 // Kernel variants: PLAIN when every configuration uses the plain-Mie law and a
@@ -108,7 +113,11 @@ struct TransportLaunch {
 // otherwise.  The rare, FP64-heavy rough-surface code lives only in ROUGH, so
 // it costs the other variants no registers.  LIGHT is ROUGH plus the
 // light-transport voxel diagnostics; it runs only when they are requested.
-enum : int { CLR_KERNEL_GENERIC = 0, CLR_KERNEL_PLAIN = 1, CLR_KERNEL_ROUGH = 2, CLR_KERNEL_LIGHT = 3 };
+// LIGHT and CAUSAL also keep each history's causal path provenance (event
+// types, deflections, interface angles, sites, acrylic returns) and record
+// it for detected histories; CAUSAL is ROUGH plus that alone.
+enum : int { CLR_KERNEL_GENERIC = 0, CLR_KERNEL_PLAIN = 1, CLR_KERNEL_ROUGH = 2, CLR_KERNEL_LIGHT = 3,
+             CLR_KERNEL_CAUSAL = 4 };
 
 hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s);
 // Diagnostic deposition of explicit inputs (mode 0 segments [n][7], mode 1

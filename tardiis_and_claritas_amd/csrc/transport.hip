@@ -1176,7 +1176,7 @@ __device__ int smith_height(double U, float wx, float wy, float wz, float bx, fl
 // direction in/out and the outcome of one interface encounter
 struct MicroIO {
   float dx, dy, dz;
-  int top, fc;
+  int top, fc, order;  // order: microfacet scatterings of the walk
 };
 // rough-surface walk (alpha > 1e-7) from an incident direction that points
 // into the TOP side, CLARITAS.py:2358-2430
@@ -1185,7 +1185,7 @@ __device__ __forceinline__ int rough_walk(Rng* __restrict__ rp, float alpha, flo
                                        MicroIO* __restrict__ io) {
   Rng r = *rp;
   float dx = io->dx, dy = io->dy, dz = io->dz;
-  int status = MS_WATCHDOG, fc = MF_WATCHDOG, top = 0;
+  int status = MS_WATCHDOG, fc = MF_WATCHDOG, top = 0, order = 0;
   float h = 1.0f + (2.0f * 0.999f - 1.0f);
   bool outside = true;
   for (int it = 0; it < kMicroWatchdog; ++it) {
@@ -1203,6 +1203,7 @@ __device__ __forceinline__ int rough_walk(Rng* __restrict__ rp, float alpha, flo
     }
     if (hs != MS_HIT || !isfinite(nh)) { status = MS_NUMERIC; fc = f2 ? f2 : MF_HEIGHT_SAMPLE; break; }
     h = nh;
+    ++order;
     const float wix = -dx, wiy = -dy, wiz = -dz;
     double mx = 0.0, my = 0.0, mz = 0.0;
     f2 = 0;
@@ -1248,7 +1249,7 @@ __device__ __forceinline__ int rough_walk(Rng* __restrict__ rp, float alpha, flo
     dx = fx; dy = fy; dz = fz;
   }
   *rp = r;
-  io->dx = dx; io->dy = dy; io->dz = dz; io->top = top; io->fc = fc;
+  io->dx = dx; io->dy = dy; io->dz = dz; io->top = top; io->fc = fc; io->order = order;
   return status;
 }
 // smooth limit, CLARITAS.py:2336-2356
@@ -1281,21 +1282,25 @@ __device__ __forceinline__ int smooth_interface(Rng& r, float bx, float by, floa
 template <bool kRough>
 __device__ __forceinline__ int micro_interface(Rng& r, float alpha, float bx, float by, float bz, float n_top,
                                                float k_top, float n_bot, float k_bot, float& dx, float& dy,
-                                               float& dz, bool& out_top, int& fc) {
+                                               float& dz, bool& out_top, int& fc, int& order) {
   fc = 0;
+  order = 0;
   normalize3(bx, by, bz);
   normalize3(dx, dy, dz);
   if (!isfinite(bx) || !isfinite(by) || !isfinite(bz) || !isfinite(dx) || !isfinite(dy) || !isfinite(dz)) {
     fc = MF_NONFINITE_DIRECTION; return MS_NUMERIC;
   }
   if (dot3(dx, dy, dz, bx, by, bz) >= -1.0e-7f) { fc = MF_SIDE_STATE; return MS_NUMERIC; }
-  if (!kRough || alpha <= 1.0e-7f)
-    return smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+  if (!kRough || alpha <= 1.0e-7f) {
+    const int st = smooth_interface(r, bx, by, bz, n_top, k_top, n_bot, k_bot, dx, dy, dz, out_top, fc);
+    order = (st == MS_ESCAPE) ? 1 : 0;
+    return st;
+  }
   Rng t = r;
-  MicroIO io{dx, dy, dz, 0, 0};
+  MicroIO io{dx, dy, dz, 0, 0, 0};
   const int st = rough_walk(&t, alpha, bx, by, bz, n_top, k_top, n_bot, k_bot, &io);
   r = t;
-  dx = io.dx; dy = io.dy; dz = io.dz; out_top = io.top != 0; fc = io.fc;
+  dx = io.dx; dy = io.dy; dz = io.dz; out_top = io.top != 0; fc = io.fc; order = io.order;
   return st;
 }
 
@@ -1408,6 +1413,11 @@ __device__ __forceinline__ float spheroid_chord(float x, float y, float z, float
 struct PCount {
   int pr, per, pir, pabs;
   uint32_t n_abs;
+  // the encounter's observational record (CLARITAS.py:2503-2599): internal
+  // path (float sum of chords), entry incidence and exit angles (deg, -1
+  // undefined), last chord index, microfacet order of the last walk
+  float path, inc, ext;
+  int macro, micro;
 };
 // One finite-particle encounter with smooth or rough surfaces: a Fresnel
 // sphere of the given radius whose centre is c (the entry normal g is the
@@ -1424,6 +1434,7 @@ __device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool s
                                               float& z, float& vx, float& vy, float& vz, PCount& pc, int& pf,
                                               int& pfs) {
   pf = PF_OK; pfs = SUB_NONE;
+  pc.path = 0.0f; pc.inc = -1.0f; pc.ext = -1.0f; pc.macro = 0; pc.micro = 0;
   if (radius <= 0.0f) return 1;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, invb2 = 0.0f, delta = 0.0f, scale = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
   if (!kRough) spheroid = false;  // the morphology channel is reached only in the ROUGH variant
@@ -1445,11 +1456,18 @@ __device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool s
   float n_top = n_m, k_top = 0.0f, n_bot = n_p, k_bot = k_p;
   for (int stage = 0;; ++stage) {
     bool top = true;
-    int fc = 0;
-    const int ms = micro_interface<kRough>(r, P.rough, bx, by, bz, n_top, k_top, n_bot, k_bot, vx, vy, vz, top, fc);
+    int fc = 0, order = 0;
+    if (stage == 0) pc.inc = acosf(fminf(fmaxf(-dot3(vx, vy, vz, gx, gy, gz), 0.0f), 1.0f)) * (180.0f / kPiF);
+    const int ms = micro_interface<kRough>(r, P.rough, bx, by, bz, n_top, k_top, n_bot, k_bot, vx, vy, vz, top, fc,
+                                           order);
+    pc.micro = order;
     if (ms != MS_ESCAPE) { pf = (ms == MS_WATCHDOG) ? PF_MICRO_WATCHDOG : PF_MICRO_NUMERICAL; pfs = fc; return 0; }
     if (stage == 0) {
-      if (top) { pc.pr++; pc.per++; return 1; }
+      if (top) {
+        pc.ext = acosf(fminf(fmaxf(dot3(vx, vy, vz, gx, gy, gz), 0.0f), 1.0f)) * (180.0f / kPiF);
+        pc.pr++; pc.per++;
+        return 1;
+      }
       if (dot3(vx, vy, vz, gx, gy, gz) >= 0.0f) {
         pf = pf_geo; pfs = spheroid ? SUB_SPHEROID_ENTRY : SUB_SPHERE_ENTRY_SIDE; return 0;
       }
@@ -1465,6 +1483,7 @@ __device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool s
         if (dot3(vx, vy, vz, nox, noy, noz) <= 0.0f) {
           pf = pf_geo; pfs = spheroid ? SUB_SPHEROID_EXIT_SIDE : SUB_SPHERE_EXIT_SIDE; return 0;
         }
+        pc.ext = acosf(fminf(fmaxf(dot3(vx, vy, vz, nox, noy, noz), 0.0f), 1.0f)) * (180.0f / kPiF);
         return 1;
       }
       if (dot3(vx, vy, vz, nox, noy, noz) >= 0.0f) {
@@ -1477,15 +1496,17 @@ __device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool s
     }
     if (stage > P.max_pb) { pf = PF_OTHER; pfs = SUB_NONE; return 0; }
     // chord of internal bounce `stage` to the next encounter
+    pc.macro = stage;
     if (kRough && spheroid) {
       const float chord = spheroid_chord(sx, sy, sz, vx, vy, vz, ux, uy, uz, invb2, delta, scale);
       if (chord <= 0.0f) { pf = PF_SPHEROID; pfs = SUB_SPHEROID_CHORD; return 0; }
       if (alpha_abs > 0.0f) {
         pc.n_abs++;
         const double dist = -log(Uopen(r)) / (double)alpha_abs;
-        if (dist < (double)chord) { pc.pabs++; return 2; }
+        if (dist < (double)chord) { pc.path += (float)dist; pc.pabs++; return 2; }
       }
       sx += chord * vx; sy += chord * vy; sz += chord * vz;
+      pc.path += chord;
       spheroid_normal(sx, sy, sz, ux, uy, uz, invb2, delta, nox, noy, noz);
     } else {
       const double dvx = vx, dvy = vy, dvz = vz;
@@ -1510,11 +1531,13 @@ __device__ __forceinline__ int particle_event(const DevParams& P, Rng& r, bool s
         if (dist < chord) {
           qx += dist * dvx; qy += dist * dvy; qz += dist * dvz;
           x = (float)(cx + qx); y = (float)(cy + qy); z = (float)(cz + qz);
+          pc.path += (float)dist;
           pc.pabs++;
           return 2;
         }
       }
       qx = tqx; qy = tqy; qz = tqz;
+      pc.path += (float)chord;
       const double qn = sqrt(qx * qx + qy * qy + qz * qz);
       if (!(qn > 0.0) || !isfinite(qn)) { pf = PF_MACRO_GEOMETRY; pfs = SUB_SPHERE_CHORD; return 0; }
       const double pj = rr / qn;
@@ -1540,7 +1563,7 @@ __device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r,
   CondOut h;
   h.x = x; h.y = y; h.z = z; h.vx = vx; h.vy = vy; h.vz = vz;
   h.status = 0; h.channel = 0; h.failure = PF_OTHER; h.subcode = 9002;
-  h.pc = PCount{0, 0, 0, 0, 0u};
+  h.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
   h.trials = h.rejects = h.inacc = 0;
   const double qs = P.qsca[pidx], qe = P.qext[pidx];
   const float ki = P.k_imag[pidx], nr = P.n_real[pidx], radius = P.radius[pidx];
@@ -1561,7 +1584,7 @@ __device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r,
     trials++;
     CondOut c;
     c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
-    c.pc = PCount{0, 0, 0, 0, 0u};
+    c.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
     c.failure = PF_OK; c.subcode = 0; c.status = 0;
     c.channel = (U53(r) < wave) ? ((P.coherent >= 1.0f || U53(r) < (double)P.coherent) ? 1 : 2) : 0;
     if (c.channel == 1) {
@@ -1884,6 +1907,113 @@ struct Lane {
   int ic, gic, wic, wcoh, pr, cir, cor, tr, bir, bor;
   unsigned flags;
 };
+
+// Causal path provenance of one history, kept by the LIGHT and CAUSAL
+// variants (CLARITAS.py:4629-4647).  Observational: it draws no random
+// numbers and feeds no transport decision.
+struct Prov {
+  float wp;                                   // water path travelled so far
+  float first_defl, last_defl, ipath, iopl, wp_last;
+  float cum, cum2, maxd;
+  float mx, my, mz, fx, fy, fz, lx, ly, lz;   // largest-turn, first and last event sites
+  float sum_r, sum_z, min_r, max_r, min_z, max_z, wp_first;
+  float m_path, m_pod, m_inc, m_exit;         // the largest turn's particle record
+  int per, pir, ret, after_ret;  // ret: returns to the water from the acrylic or headspace after an event
+  int first_type, last_type, last_order, first_back, first_hemi;
+  int fpbin, lpbin, fgbin, lgbin, fwbin, lwbin;
+  int btrans, etrans, nsmall, nmed, nlarge, prev_branch, prev_type;
+  int m_type, m_order, m_bin, m_macro, m_micro;
+};
+__device__ __forceinline__ void prov_init(Prov& V) {
+  V.wp = 0.0f;
+  V.first_defl = V.last_defl = -1.0f; V.ipath = V.iopl = 0.0f; V.wp_last = -1.0f;
+  V.cum = V.cum2 = 0.0f; V.maxd = -1.0f;
+  V.mx = V.my = V.mz = V.fx = V.fy = V.fz = V.lx = V.ly = V.lz = 0.0f;
+  V.sum_r = V.sum_z = 0.0f;
+  V.min_r = kInfF; V.max_r = -kInfF; V.min_z = kInfF; V.max_z = -kInfF; V.wp_first = -1.0f;
+  V.m_path = V.m_pod = 0.0f; V.m_inc = V.m_exit = -1.0f;
+  V.per = V.pir = V.ret = V.after_ret = 0;
+  V.first_type = V.last_type = V.last_order = V.first_back = V.first_hemi = 0;
+  V.fpbin = V.lpbin = V.fgbin = V.lgbin = V.fwbin = V.lwbin = -1;
+  V.btrans = V.etrans = V.nsmall = V.nmed = V.nlarge = V.prev_branch = V.prev_type = 0;
+  V.m_type = V.m_order = 0; V.m_bin = -1; V.m_macro = V.m_micro = 0;
+}
+// One surviving particle event of type `type` (1-7) on bin `pidx` at site s
+// with incoming direction vin (the lane holds the outgoing one),
+// CLARITAS.py:4936-4980.
+__device__ __forceinline__ void prov_commit(Prov& V, const Lane& L, const DevParams& P, float vix, float viy,
+                                            float viz, float sx, float sy, float sz, int type, int pidx,
+                                            const PCount& pc) {
+  const float dcos = fminf(fmaxf(dot3(vix, viy, viz, L.vx, L.vy, L.vz), -1.0f), 1.0f);
+  const float defl = acosf(dcos) * (180.0f / kPiF);
+  const float sr = sqrtf(fmaxf(sx * sx + sy * sy, 0.0f));
+  V.ipath += pc.path;
+  if (pidx >= 0) V.iopl += pc.path * P.n_real[pidx];
+  V.cum += defl;
+  V.cum2 += defl * defl;
+  if (defl > V.maxd) {  // strict: the first of equal turns is kept
+    V.maxd = defl; V.mx = sx; V.my = sy; V.mz = sz;
+    V.m_type = type; V.m_order = L.ic; V.m_bin = pidx; V.m_macro = pc.macro; V.m_micro = pc.micro;
+    V.m_path = pc.path;
+    V.m_pod = (pidx >= 0 && P.radius[pidx] > 0.0f) ? pc.path / (2.0f * P.radius[pidx]) : 0.0f;
+    V.m_inc = pc.inc; V.m_exit = pc.ext;
+  }
+  if (defl < 30.0f) V.nsmall++; else if (defl < 90.0f) V.nmed++; else V.nlarge++;
+  const int branch = (type >= 1 && type <= 3) ? 1 : 2;
+  if (V.prev_branch > 0 && branch != V.prev_branch) V.btrans++;
+  if (V.prev_type > 0 && type != V.prev_type) V.etrans++;
+  V.prev_branch = branch; V.prev_type = type;
+  V.sum_r += sr; V.sum_z += sz;
+  if (sr < V.min_r) V.min_r = sr;
+  if (sr > V.max_r) V.max_r = sr;
+  if (sz < V.min_z) V.min_z = sz;
+  if (sz > V.max_z) V.max_z = sz;
+  if (L.ic == 1) { V.fx = sx; V.fy = sy; V.fz = sz; V.wp_first = V.wp; }
+  V.lx = sx; V.ly = sy; V.lz = sz; V.wp_last = V.wp;
+  if (pidx >= 0) {
+    if (V.fpbin < 0) V.fpbin = pidx;
+    V.lpbin = pidx;
+    if (type >= 1 && type <= 3) { if (V.fgbin < 0) V.fgbin = pidx; V.lgbin = pidx; }
+    else if (type >= 4 && type <= 7) { if (V.fwbin < 0) V.fwbin = pidx; V.lwbin = pidx; }
+    if (V.ret > 0) V.after_ret++;
+  }
+  V.last_type = type; V.last_order = L.ic; V.last_defl = defl;
+  if (L.ic == 1) { V.first_type = type; V.first_defl = defl; V.first_back = dcos < 0.0f; V.first_hemi = L.vy < 0.0f; }
+}
+// the record of a detected history (sentinels of CLARITAS.py:5076-5103)
+__device__ void prov_record(clr_causal_record* __restrict__ q, const Lane& L, const Prov& V, uint64_t hist, int nat,
+                            int mir) {
+  const bool any = L.ic > 0;
+  clr_causal_record c;
+  c.history = hist; c.native_id = nat; c.mirror_id = mir;
+  c.ic = L.ic; c.gic = L.gic; c.wic = L.wic; c.wcoh = L.wcoh;
+  c.per = V.per; c.pir = V.pir; c.pabs = (L.flags & F_PABS) ? 1 : 0;
+  c.acrylic_reflections = L.cir + L.cor + L.bir + L.bor;
+  c.acrylic_returns = V.ret; c.events_after_return = V.after_ret;
+  c.first_type = V.first_type; c.last_type = V.last_type; c.last_order = V.last_order;
+  c.first_backscatter = V.first_back; c.first_source_hemisphere = V.first_hemi;
+  c.first_bin = V.fpbin; c.last_bin = V.lpbin; c.first_geometric_bin = V.fgbin; c.last_geometric_bin = V.lgbin;
+  c.first_wave_bin = V.fwbin; c.last_wave_bin = V.lwbin;
+  c.branch_transitions = V.btrans; c.type_transitions = V.etrans;
+  c.small_deflections = V.nsmall; c.medium_deflections = V.nmed; c.large_deflections = V.nlarge;
+  c.max_type = any ? V.m_type : 0; c.max_order = any ? V.m_order : 0; c.max_bin = any ? V.m_bin : -1;
+  c.max_macro_bounces = any ? V.m_macro : 0; c.max_micro_order = any ? V.m_micro : 0;
+  c.first_deflection = V.first_defl; c.last_deflection = V.last_defl;
+  c.internal_path = V.ipath; c.internal_opl = V.iopl;
+  c.water_path_after_last = (V.last_order > 0 && V.wp_last >= 0.0f) ? fmaxf(V.wp - V.wp_last, 0.0f) : -1.0f;
+  c.cumulative_deflection = any ? V.cum : -1.0f; c.cumulative_deflection_sq = any ? V.cum2 : -1.0f;
+  c.max_deflection = any ? V.maxd : -1.0f;
+  c.turn_x = any ? V.mx : 0.0f; c.turn_y = any ? V.my : 0.0f; c.turn_z = any ? V.mz : 0.0f;
+  c.first_x = any ? V.fx : 0.0f; c.first_y = any ? V.fy : 0.0f; c.first_z = any ? V.fz : 0.0f;
+  c.last_x = any ? V.lx : 0.0f; c.last_y = any ? V.ly : 0.0f; c.last_z = any ? V.lz : 0.0f;
+  c.mean_r = any ? V.sum_r / (float)L.ic : -1.0f; c.mean_z = any ? V.sum_z / (float)L.ic : -1.0f;
+  c.min_r = any ? V.min_r : -1.0f; c.max_r = any ? V.max_r : -1.0f;
+  c.min_z = any ? V.min_z : -1.0f; c.max_z = any ? V.max_z : -1.0f;
+  c.water_path_first = any ? V.wp_first : -1.0f; c.water_path_last = any ? V.wp_last : -1.0f;
+  c.max_internal_path = any ? V.m_path : 0.0f; c.max_path_over_diameter = any ? V.m_pod : 0.0f;
+  c.max_incidence = any ? V.m_inc : -1.0f; c.max_exit = any ? V.m_exit : -1.0f;
+  *q = c;
+}
 // per-lane work counters accumulated over all histories of a launch
 struct Acc {
   uint32_t n_flight, n_null, n_iface, n_abs, steps, draws, trials, rejects, inacc;
@@ -1899,9 +2029,16 @@ __device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
 // launch + collimator + air->acrylic entry, CLARITAS.py:4650-4735.  The
 // reflecting source collimator (ROUGH and LIGHT variants) reports its wall
 // interactions and reflections in sbi / sbr.
+// variants with the rough-particle physics and the detector-side models
+template <int kKind>
+constexpr bool kind_rough() { return kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT || kKind == CLR_KERNEL_CAUSAL; }
+// variants that keep the causal provenance
+template <int kKind>
+constexpr bool kind_diag() { return kKind == CLR_KERNEL_LIGHT || kKind == CLR_KERNEL_CAUSAL; }
+
 template <int kKind>
 __device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A, int& sbi, int& sbr) {
-  constexpr bool kBores = (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT);
+  constexpr bool kBores = kind_rough<kKind>();
   Rng& r = L.r;
   float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
   if (P.src_model == CLR_SOURCE_COLLIMATED_DISK) {
@@ -2320,7 +2457,8 @@ __device__ __forceinline__ float slab_free_path(const DevParams& P, float z, flo
 }
 
 template <int kKind>
-__device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D) {
+__device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
+                                             Prov& V) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
   if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
   L.wstep++;
@@ -2369,6 +2507,7 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
     light_segment(*D, rng_counter(L.r), L.x, L.y, L.z, L.vx, L.vy, L.vz, travel, L.ic);
   A.n_flight++;
   L.x += travel * L.vx; L.y += travel * L.vy; L.z += travel * L.vz;
+  if constexpr (kind_diag<kKind>()) V.wp += travel;
   if (do_particle) { L.region = R_WPART; return; }
   L.ev = axial_top ? EV_TOP : (axial_bot ? EV_BOTTOM : EV_SIDE);
   L.region = R_WIFACE;
@@ -2376,12 +2515,15 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
 
 // the particle event at the end of a flight (CLARITAS.py:4774-4981)
 template <int kKind>
-__device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D) {
+__device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
+                                               Prov& V) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
-  constexpr bool kLight = (kKind == CLR_KERNEL_LIGHT), kRough = (kKind == CLR_KERNEL_ROUGH) || kLight;
+  constexpr bool kLight = (kKind == CLR_KERNEL_LIGHT), kRough = kind_rough<kKind>();
+  constexpr bool kDiag = kind_diag<kKind>();
   L.region = R_WATER;  // a terminate() below overrides
-  // event site and history id for the light-transport diagnostics
+  // event site and incoming direction (diagnostics), history id (light grids)
   const float sx0 = L.x, sy0 = L.y, sz0 = L.z;
+  const float vix = L.vx, viy = L.vy, viz = L.vz;
   const uint64_t gid = kLight ? rng_counter(L.r) : 0ull;
   const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
   const float MU_TOTAL = P.mu_geom + P.mu_wave;
@@ -2434,6 +2576,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     L.wic++; L.wcoh++;
     mie_scatter(L.r, P, b, L.vx, L.vy, L.vz);
     if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, 4);
+    if constexpr (kDiag) prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, 4, b, PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0});
     return;
   }
   if constexpr (!kPlain) {
@@ -2458,6 +2601,8 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
       L.wcoh++;
       mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
       if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, 4);
+      if constexpr (kDiag)
+        prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, 4, pidx, PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0});
       return;
     }
     // morphology-wave event about the local azimuthal flow direction,
@@ -2503,6 +2648,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     }
     L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
     L.pr += h.pc.pr;
+    if constexpr (kDiag) { V.per += h.pc.per; V.pir += h.pc.pir; }
     if (h.pc.per) L.flags |= F_PER;
     if (h.pc.pir) L.flags |= F_PIR;
     if (h.pc.pabs) L.flags |= F_PABS;
@@ -2513,9 +2659,11 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
       terminate(L, 5);
       return;
     }
-    if constexpr (kLight) {
+    {
       const int fres = h.pc.per ? 1 : (h.pc.pir ? 3 : 2);  // external reflection / internal reflection / transmission
-      light_event(*D, gid, sx0, sy0, sz0, (h.channel == 1) ? 4 : (h.channel == 2 ? fres + 4 : fres));
+      const int type = (h.channel == 1) ? 4 : (h.channel == 2 ? fres + 4 : fres);
+      if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, type);
+      if constexpr (kDiag) prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, type, cbin, h.pc);
     }
     return;
   }
@@ -2529,6 +2677,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   // a direction-only sphere keeps the incoming direction unless it escaped
   if (!dir_only || sph || ps == 1) { L.vx = pvx; L.vy = pvy; L.vz = pvz; }
   L.pr += pc.pr;
+  if constexpr (kDiag) { V.per += pc.per; V.pir += pc.pir; }
   if (pc.per) L.flags |= F_PER;
   if (pc.pir) L.flags |= F_PIR;
   if (pc.pabs) L.flags |= F_PABS;
@@ -2543,9 +2692,11 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     terminate(L, 4, ppf ? ppf : PF_OTHER);
     return;
   }
-  if constexpr (kLight) {
+  {
     const int fres = pc.per ? 1 : (pc.pir ? 3 : 2);
-    light_event(*D, gid, sx0, sy0, sz0, dir_only ? fres + 4 : fres);
+    const int type = dir_only ? fres + 4 : fres;
+    if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, type);
+    if constexpr (kDiag) prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, type, pidx, pc);
   }
   }
   return;
@@ -2625,8 +2776,9 @@ __device__ __forceinline__ void lds_add64(unsigned long long* p, uint32_t v) {
 // LDS atomic unit, which is cheaper than aggregating them in software.
 template <int kKind>
 __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTally& T, Acc& A, clr_record* rec,
-                                            uint64_t n_rec, uint64_t hidx, int g, clr_tallies* __restrict__ Gt) {
-  constexpr bool kBores = (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT);
+                                            uint64_t n_rec, uint64_t hidx, int g, clr_tallies* __restrict__ Gt,
+                                            const Prov& V, const TransportLaunch& Lp) {
+  constexpr bool kBores = kind_rough<kKind>();
   int nat = -1, mir = -1, aux_bits = 0;
   bool scored = false;
   float inc_n = 0.0f, inc_m = 0.0f, ang_n = -1.0f, ang_m = -1.0f;
@@ -2705,6 +2857,19 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
       if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
     }
     lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
+    if constexpr (kind_diag<kKind>()) {
+      // causal record of a detected history: one append per wave
+      const bool want = Lp.causal != nullptr && (nat >= 0 || mir >= 0);
+      const uint64_t m = __ballot(want);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned long long base = 0ull;
+        if ((int)__lane_id() == leader) base = atomicAdd(Lp.causal_count, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
+        const uint64_t k = base + (uint64_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        if (want && k < Lp.causal_cap) prov_record(Lp.causal + k, L, V, rng_counter(L.r), nat, mir);
+      }
+    }
     A.draws += rng_draws(L.r);
     if (rec && hidx < n_rec) {
       clr_record q;
@@ -2867,6 +3032,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
     const uint64_t G = (uint64_t)P.n_groups;
     clr_record* rec = (cfg == 0) ? Lp.records : nullptr;
     Lane L;
+    Prov V;
     bool alive = false, exhausted = false;
     for (;;) {
       // Region-coherent scheduling: each pass runs one handler for one group
@@ -2902,7 +3068,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         if (fin) {
           // history number and jackknife group from the stream counter
           const uint64_t hidx = rng_counter(L.r) - w_offset;
-          finish_lane<kKind>(L, P, T, A, rec, Lp.n_records, hidx, (int)((hidx * G) / w_total), Lp.tallies + cfg);
+          finish_lane<kKind>(L, P, T, A, rec, Lp.n_records, hidx, (int)((hidx * G) / w_total), Lp.tallies + cfg, V, Lp);
           alive = false;
         }
         // refill: one LDS atomic per wave hands consecutive history numbers
@@ -2923,10 +3089,11 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
               L.region = R_WATER; L.st = 0; L.pf = PF_OK; L.wstep = 0;
               L.ic = L.gic = L.wic = L.wcoh = L.pr = L.cir = L.cor = L.tr = L.bir = L.bor = 0;
               L.flags = 0u;
+              if constexpr (kind_diag<kKind>()) prov_init(V);
               L.actx = A_LAUNCH; L.hctx = H_LAUNCH; L.it = 0; L.hit = 0; L.stuck = 0; L.last_surf = -1;
               int sbi = 0, sbr = 0;
               launch_history<kKind>(L, P, A, sbi, sbr);  // may end it at once: scored by a later RECYCLE pass
-              if constexpr (kKind == CLR_KERNEL_ROUGH || kKind == CLR_KERNEL_LIGHT) {
+              if constexpr (kind_rough<kKind>()) {
                 if (P.src_bore) {
                   lds_add64(&T.cnt[CLR_C_SRC_BORE_INTER], (uint32_t)sbi);
                   lds_add64(&T.cnt[CLR_C_SRC_BORE_REFL], (uint32_t)sbr);
@@ -2941,11 +3108,12 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         }
       } else if (op == K_WATER) {
         if (alive && (L.region == R_WPART || L.region == R_WATER)) rng_topup(L.r);
-        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A, Lp.light);
-        if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light);
+        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A, Lp.light, V);
+        if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light, V);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
       } else if (op == K_ACRYLIC) {
+        const bool was_acrylic = alive && L.region == R_ACRYLIC;
         if (alive && L.region == R_ACRYLIC) acrylic_step(L, P, A);
         // A chain of kLongChain bounces in one acrylic visit is almost always
         // trapped by total internal reflection and runs on to the emergency
@@ -2956,8 +3124,14 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         while (__any(alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain))) {
           if (alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain)) acrylic_step(L, P, A);
         }
+        // a return to the water after a particle event (CLARITAS.py:5009-5058)
+        if constexpr (kind_diag<kKind>())
+          if (was_acrylic && L.region == R_WATER && L.ic > 0) V.ret++;
       } else {
+        const bool was_head = alive && L.region == R_HEADSPACE;
         if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
+        if constexpr (kind_diag<kKind>())
+          if (was_head && L.region == R_WATER && L.ic > 0) V.ret++;
       }
       // running estimate of the pass cost (cycles, 1/8 weight to the new pass)
       const int64_t dt = (int64_t)(__builtin_amdgcn_s_memtime() - t_pass);
@@ -3002,6 +3176,8 @@ hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_ROUGH>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   else if (L.kind == CLR_KERNEL_LIGHT)
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_LIGHT>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
+  else if (L.kind == CLR_KERNEL_CAUSAL)
+    hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_CAUSAL>, dim3(L.grid), dim3(L.block), 0, s, L, L.params, L.items);
   else
     hipLaunchKernelGGL(transport_kernel<CLR_KERNEL_GENERIC>, dim3(L.grid), dim3(L.block), 0, s, L, L.params,
                        L.items);
@@ -3017,6 +3193,8 @@ int clr_transport_occupancy(int block, int kind) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_ROUGH>, block, 0);
   else if (kind == CLR_KERNEL_LIGHT)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_LIGHT>, block, 0);
+  else if (kind == CLR_KERNEL_CAUSAL)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_CAUSAL>, block, 0);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, transport_kernel<CLR_KERNEL_GENERIC>, block, 0);
   if (e != hipSuccess) return 1;

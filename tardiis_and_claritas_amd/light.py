@@ -115,6 +115,11 @@ class LightAccumulator:
             raise ValueError(f'light diagnostic allocation is too large ({memory_bytes(c) / 1e9:.3g} GB); reduce '
                              'grid dimensions, representative rays, or segments per ray')
 
+    def shape(self):
+        """(nx, ny, nz, representative rays, segments per ray) of the output arrays."""
+        c = self.cfg
+        return c['nx'], c['ny'], c['nz'], c['representative_rays'], c['max_segments_per_ray']
+
     def run(self, n, seed, history_offset, launch):
         """clr_run_light over histories [history_offset, history_offset + n)."""
         c = self.cfg

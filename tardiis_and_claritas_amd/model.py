@@ -18,7 +18,7 @@ from typing import Optional
 
 import numpy as np
 
-from . import abi, light, meridional
+from . import abi, causal, light, meridional
 from ._lib import Context
 from .meridional import MeridionalSettings
 from .physics import (DETECTOR_ANGLES_DEG, Material, reference_material, vortex_surface_geometry)
@@ -209,6 +209,7 @@ class ForwardResult:
     batch_diagnostics: Optional[list] = None
     particle_diagnostics: Optional[dict] = None
     light_transport_diagnostics: Optional[dict] = None
+    detector_particle_path_diagnostics: Optional[dict] = None
     # ---- extensions (no reference field) ----
     aux_detector_angles_deg: Optional[np.ndarray] = None
     aux_native_hits: Optional[np.ndarray] = None
@@ -677,10 +678,14 @@ class ClaritasForwardModel:
         l.block, l.blocks_per_cu, l.chunk = (int(self.launch.get(k, 0)) for k in ('block', 'blocks_per_cu', 'chunk'))
         return C.byref(l)
 
-    def run_tallies(self, n, seed, history_offset=0, n_records=0, light=None):
+    def run_tallies(self, n, seed, history_offset=0, n_records=0, light=None, causal=None):
         """Trace n histories of the prepared slot 0; returns (tallies dict,
         records or None, device ms).  With a light.LightAccumulator the run
-        also records the light-transport voxel diagnostics (same tallies)."""
+        also records the light-transport voxel diagnostics, with a
+        causal.CausalAccumulator the detector-path provenance (same tallies)."""
+        if causal is not None:
+            T = causal.run(n, seed, history_offset, self._launch(), light_acc=light)
+            return T, None, float(self.ctx.lib.clr_last_kernel_ms(self.ctx.h))
         if light is not None:
             t = light.run(n, seed, history_offset, self._launch())
             ms = float(self.ctx.lib.clr_last_kernel_ms(self.ctx.h))
@@ -697,7 +702,8 @@ class ClaritasForwardModel:
     def simulate(self, material, concentration_g_per_L, weights=None, n_rays=100000, seed=12345,
                  weight_mode='mass_fraction', collect_rays=False, heatmap_size=0, light_transport_config=None,
                  target_detector_score=None, min_rays=None, max_rays=None, min_rear_channel_score=None,
-                 min_h170_score=None, stability_l1_tolerance=None, stability_window=3, ray_id_start=0):
+                 min_h170_score=None, stability_l1_tolerance=None, stability_window=3, ray_id_start=0,
+                 path_diagnostics=True):
         """Signature of CLARITAS.py:7160-7179.  Per-ray outputs and heat maps
         are not produced (collect_rays and heatmap_size must be off); the
         light-transport voxel diagnostics are (light_transport_config with
@@ -708,6 +714,9 @@ class ClaritasForwardModel:
         lacc = None
         if light_transport_config and light_transport_config.get('enabled'):
             lacc = light.LightAccumulator(self, light_transport_config, int(n_rays), int(ray_id_start))
+        # detector-path / causal / H170 diagnostics (always on in the
+        # reference; path_diagnostics=False runs the production kernel only)
+        cacc = causal.CausalAccumulator(self, len(mat.diameter_m)) if path_diagnostics else None
         requested = int(n_rays)
         ray_id_start = int(ray_id_start)
         if ray_id_start < 0:
@@ -716,7 +725,7 @@ class ClaritasForwardModel:
         if not adaptive:
             if requested < 1:
                 raise ValueError('number of rays must be positive')
-            T, _, ms = self.run_tallies(requested, seed, ray_id_start, light=lacc)
+            T, _, ms = self.run_tallies(requested, seed, ray_id_start, light=lacc, causal=cacc)
             batches = [T['det'][g] for g in range(T['n_groups'])]
             pbatches = [T['photodiode'][g] for g in range(T['n_groups'])]
             group_n = T['group_histories']
@@ -736,7 +745,7 @@ class ClaritasForwardModel:
             prev_cum, recent = None, []
             while total < maxN:
                 b = min(self.statistics_batch_rays, maxN - total)
-                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total, light=lacc)
+                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total, light=lacc, causal=cacc)
                 ms += msb
                 d = Tb['det'].sum(axis=0)
                 batches.append(d)
@@ -766,6 +775,17 @@ class ClaritasForwardModel:
         if lacc is not None:
             res.light_transport_diagnostics = lacc.result(total)
             light.check_closure(res.light_transport_diagnostics)
+        if cacc is not None:
+            cond = self.particle_event_model in ('hybrid_mie_exact_rate', 'hybrid_mie_exact_rate_moment_constrained')
+            closure = ('Mie extinction and terminal outcomes; conditional finite-particle angular/path law' if cond
+                       else self.particle_event_model + '; configured unconditional event law')
+            cnt = T['counters']
+            counts = dict.fromkeys(causal.CONDITIONAL_COUNT_NAMES)
+            counts.update(candidate_trials=int(cnt[abi.C_IDX['cond_trials']]),
+                          outcome_rejections=int(cnt[abi.C_IDX['cond_reject']]),
+                          inaccessible_trials=int(cnt[abi.C_IDX['cond_inaccessible']]))
+            res.detector_particle_path_diagnostics = causal.diagnostics_dict(
+                cacc.tally, self._sym(det_sum), closure, counts, CONDITIONAL_MAX_TRIALS)
         return res
 
     @staticmethod
