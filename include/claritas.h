@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 6
+#define CLR_ABI_VERSION 7
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -228,6 +228,17 @@ typedef struct clr_tables_out {
   double majorants[4];          /* CLARITAS.py:4035 layout */
 } clr_tables_out;
 
+/* Production failure subcodes (CLARITAS.py:464-521) by tally slot: slot k
+ * counts the failed histories whose subcode is CLR_SUBCODE_VALUES[k]; the
+ * last slot counts any other subcode. */
+#define CLR_N_SUBCODE 64
+#define CLR_SUBCODE_VALUES                                                                            \
+  {0,    10,   11,   12,   13,   14,   20,   21,   22,   23,   24,   25,   26,   27,   28,   29,   30, \
+   31,   32,   33,   34,   35,   36,   37,   38,   39,   40,   41,   1001, 1002, 1003, 1004, 1005,   \
+   1101, 1102, 1103, 1104, 1105, 1106, 1107, 2001, 2002, 2101, 2102, 2103, 2104, 3001, 3002, 3003, \
+   3004, 3005, 3006, 4001, 9001, 9002, 9003}
+#define CLR_N_SUBCODE_KNOWN 56
+
 /* ---- tallies for one configuration ----
  * det[g][k][c]: k = 0 hardware native, 1 hardware mirror, 2 legacy native,
  * 3 legacy mirror; channel c = detector at 10*c degrees (CLARITAS.py:4378-4436).
@@ -251,6 +262,7 @@ typedef struct clr_tallies {
   double photodiode[CLR_MAX_GROUPS][2][CLR_N_CHANNELS];
   uint64_t status[CLR_N_STATUS];
   uint64_t failures[CLR_N_FAILURE];
+  uint64_t failure_subcodes[CLR_N_SUBCODE];  /* failed histories by subcode slot (CLR_SUBCODE_VALUES) */
   uint64_t counters[CLR_N_COUNTERS];
   uint64_t ic_hist[CLR_N_ICHIST];
   int32_t n_groups;

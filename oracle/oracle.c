@@ -3274,11 +3274,20 @@ WRITE_OUT:
 /* ======================================================================
  * Tally one history
  * ====================================================================== */
+/* tally slot of a failure subcode (CLR_SUBCODE_VALUES; the last slot: any other) */
+static int subcode_slot(int sub) {
+  static const int vals[CLR_N_SUBCODE_KNOWN] = CLR_SUBCODE_VALUES;
+  for (int k = 0; k < CLR_N_SUBCODE_KNOWN; ++k)
+    if (vals[k] == sub) return k;
+  return CLR_N_SUBCODE - 1;
+}
+
 static void tally(clr_tallies* T, const hist_t* h, int g) {
   T->counters[CLR_C_HISTORIES]++;
   T->group_histories[g]++;
   if (h->st >= 0 && h->st < CLR_N_STATUS) T->status[h->st]++;
   if (h->st == 4 && h->pf >= 0 && h->pf < CLR_N_FAILURE) T->failures[h->pf]++;
+  if (h->st == 4) T->failure_subcodes[subcode_slot(h->pfs)]++;
   const photon_t* p = &h->ph;
   if (h->st == 1 && isfinite(p->x) && isfinite(p->vx)) T->counters[CLR_C_VALID_AIR]++;
   if (h->st == 1) T->counters[CLR_C_N_EXIT]++;
@@ -3362,6 +3371,7 @@ static void merge(clr_tallies* D, const clr_tallies* S) {
       for (int c = 0; c < CLR_N_CHANNELS; ++c) D->photodiode[g][k][c] += S->photodiode[g][k][c];
   for (int i = 0; i < CLR_N_STATUS; ++i) D->status[i] += S->status[i];
   for (int i = 0; i < CLR_N_FAILURE; ++i) D->failures[i] += S->failures[i];
+  for (int i = 0; i < CLR_N_SUBCODE; ++i) D->failure_subcodes[i] += S->failure_subcodes[i];
   for (int i = 0; i < CLR_N_COUNTERS; ++i) D->counters[i] += S->counters[i];
   for (int i = 0; i < CLR_N_ICHIST; ++i) D->ic_hist[i] += S->ic_hist[i];
 }

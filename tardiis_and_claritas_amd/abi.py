@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -16,6 +16,12 @@ N_COUNTERS = 64
 N_ICHIST = 64
 N_STATUS = 9
 N_FAILURE = 19
+# failure subcode tally slots (include/claritas.h CLR_SUBCODE_VALUES; the
+# last slot counts any other subcode), CLARITAS.py:464-521
+N_SUBCODE = 64
+SUBCODE_VALUES = (0, 10, 11, 12, 13, 14, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37,
+                  38, 39, 40, 41, 1001, 1002, 1003, 1004, 1005, 1101, 1102, 1103, 1104, 1105, 1106, 1107, 2001,
+                  2002, 2101, 2102, 2103, 2104, 3001, 3002, 3003, 3004, 3005, 3006, 4001, 9001, 9002, 9003)
 
 # event laws (CLARITAS.py:5543-5550); 6 = plain Mie (no reference code)
 EVENT_MODEL_CODES = {
@@ -153,6 +159,7 @@ class Tallies(C.Structure):
         ('photodiode', D * N_CHANNELS * 2 * MAX_GROUPS),
         ('status', U64 * N_STATUS),
         ('failures', U64 * N_FAILURE),
+        ('failure_subcodes', U64 * N_SUBCODE),
         ('counters', U64 * N_COUNTERS),
         ('ic_hist', U64 * N_ICHIST),
         ('n_groups', I32),
@@ -331,6 +338,7 @@ def tallies_to_numpy(t: Tallies):
         'photodiode': np.ctypeslib.as_array(t.photodiode).reshape(MAX_GROUPS, 2, N_CHANNELS)[:g].copy(),
         'status': np.ctypeslib.as_array(t.status).astype(np.int64).copy(),
         'failures': np.ctypeslib.as_array(t.failures).astype(np.int64).copy(),
+        'failure_subcodes': np.ctypeslib.as_array(t.failure_subcodes).astype(np.int64).copy(),
         'counters': np.ctypeslib.as_array(t.counters).astype(np.int64).copy(),
         'ic_hist': np.ctypeslib.as_array(t.ic_hist).astype(np.int64).copy(),
     }

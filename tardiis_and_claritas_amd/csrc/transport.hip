@@ -2020,11 +2020,34 @@ struct Acc {
   uint32_t entered, n_exit, valid_air;
 };
 
-__device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK) {
+// L.pf keeps the failure code in its low 8 bits and the subcode above them
+__device__ __forceinline__ void terminate(Lane& L, int st, int pf = PF_OK, int pfs = SUB_NONE) {
   L.st = st;
-  L.pf = pf;
+  L.pf = pf | (pfs << 8);
   L.region = R_DONE;
 }
+// tally slot of a failure subcode: the position of the subcode in
+// CLR_SUBCODE_VALUES (runs of consecutive codes), the last slot for any other
+__host__ __device__ constexpr int subcode_slot(int s) {
+  return s == 0                    ? 0
+         : (s >= 10 && s <= 14)     ? 1 + (s - 10)
+         : (s >= 20 && s <= 41)     ? 6 + (s - 20)
+         : (s >= 1001 && s <= 1005) ? 28 + (s - 1001)
+         : (s >= 1101 && s <= 1107) ? 33 + (s - 1101)
+         : (s >= 2001 && s <= 2002) ? 40 + (s - 2001)
+         : (s >= 2101 && s <= 2104) ? 42 + (s - 2101)
+         : (s >= 3001 && s <= 3006) ? 46 + (s - 3001)
+         : (s == 4001)              ? 52
+         : (s >= 9001 && s <= 9003) ? 53 + (s - 9001)
+                                    : CLR_N_SUBCODE - 1;
+}
+constexpr bool subcode_slots_match_table() {
+  constexpr int vals[CLR_N_SUBCODE_KNOWN] = CLR_SUBCODE_VALUES;
+  for (int k = 0; k < CLR_N_SUBCODE_KNOWN; ++k)
+    if (subcode_slot(vals[k]) != k) return false;
+  return subcode_slot(5) == CLR_N_SUBCODE - 1 && subcode_slot(9004) == CLR_N_SUBCODE - 1;
+}
+static_assert(subcode_slots_match_table(), "subcode slots disagree with CLR_SUBCODE_VALUES");
 
 // launch + collimator + air->acrylic entry, CLARITAS.py:4650-4735.  The
 // reflecting source collimator (ROUGH and LIGHT variants) reports its wall
@@ -2107,19 +2130,19 @@ __device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc&
 
 // headspace return code -> next state (callers differ, CLARITAS.py:4720-4731,
 // 4995-5002, 5024-5031, 5051-5059)
-__device__ __forceinline__ void headspace_return(Lane& L, int hw) {
+__device__ __forceinline__ void headspace_return(Lane& L, int hw, int sub = SUB_HEADSPACE_NO_HIT) {
   if (hw == 1) { L.region = R_WATER; L.flags |= F_ENTERED; return; }
   if (hw == 2) { terminate(L, 1); return; }
   if (hw == 3) { terminate(L, 7); return; }
   if (hw == 4) { terminate(L, 6); return; }
   if (hw == 5 && (L.hctx == H_LAUNCH || L.hctx == H_WATERWALL)) { terminate(L, 8); return; }
-  terminate(L, 4, PF_HEADSPACE_NO_HIT);
+  terminate(L, 4, PF_HEADSPACE_NO_HIT, sub);
 }
 // acrylic return code -> next state
 __device__ __forceinline__ void acrylic_return(Lane& L, int where, int sub) {
   if (L.actx == A_HEAD) {
     if (where == 4) { L.region = R_HEADSPACE; return; }  // same headspace walk continues
-    headspace_return(L, where);
+    headspace_return(L, where, sub);
     return;
   }
   if (where == 1) { L.region = R_WATER; L.flags |= F_ENTERED; return; }
@@ -2135,7 +2158,7 @@ __device__ __forceinline__ void acrylic_return(Lane& L, int where, int sub) {
   int pf;
   if (L.actx == A_BOTTOM) pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_BOTTOM_GEOMETRY;
   else pf = (sub == SUB_ACRYLIC_BOUNCE) ? PF_ACRYLIC_BOUNCE : PF_ACRYLIC_NO_HIT;
-  terminate(L, 4, pf);
+  terminate(L, 4, pf, sub);
 }
 __device__ __forceinline__ void enter_acrylic(Lane& L, int ctx) {
   L.region = R_ACRYLIC; L.actx = ctx; L.it = 0; L.stuck = 0; L.last_surf = -1;
@@ -2224,7 +2247,7 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
   if ((double)L.z <= (double)zs - tol && rr < (double)P.rin - tol) { headspace_return(L, 1); return; }
   if ((double)L.z > (double)P.zwall_top + tol) { headspace_return(L, 4); return; }
   const double tside = bore_wall_hit_d(P, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.rin, P.zmin, P.zwall_top);
-  if (tside < 0.0) { headspace_return(L, 0); return; }
+  if (tside < 0.0) { headspace_return(L, 0, SUB_WATER_OUTSIDE); return; }
   const double tsurf = (double)free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
   double ttop = 1.0e300;
   if (L.vz > 1.0e-14f) {
@@ -2254,7 +2277,7 @@ __device__ __forceinline__ void headspace_step(Lane& L, const DevParams& P, Acc&
       L.x -= kEpsF * snx; L.y -= kEpsF * sny; L.z -= kEpsF * snz;
       return;
     }
-    if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) { headspace_return(L, 0); return; }
+    if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) { headspace_return(L, 0, SUB_WATER_OUTSIDE); return; }
     headspace_return(L, 1);
     return;
   }
@@ -2460,18 +2483,22 @@ template <int kKind>
 __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
                                              Prov& V) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
-  if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT); return; }
+  if (L.wstep >= P.max_iter) { terminate(L, 4, PF_ITERATION_LIMIT, SUB_ITERATION_LIMIT); return; }
   L.wstep++;
   A.steps++;
-  if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) { terminate(L, 4, PF_NONFINITE_POSITION); return; }
-  if (!isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) { terminate(L, 4, PF_NONFINITE_DIRECTION); return; }
+  if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) {
+    terminate(L, 4, PF_NONFINITE_POSITION, SUB_NONFINITE_POSITION); return;
+  }
+  if (!isfinite(L.vx) || !isfinite(L.vy) || !isfinite(L.vz)) {
+    terminate(L, 4, PF_NONFINITE_DIRECTION, SUB_NONFINITE_DIRECTION); return;
+  }
   const float zsh = surface_z_vs(P, L.x, L.y, L.z);
   if (L.z <= P.zmin || L.z >= zsh) { terminate(L, 2); return; }
   const double ri2w = (double)P.rin * (double)P.rin, wtol = 8.0 * kFEps * ri2w;
   const double rr2 = (double)L.x * (double)L.x + (double)L.y * (double)L.y;
-  if (rr2 > ri2w + wtol) { terminate(L, 4, PF_INVALID_MEDIUM); return; }
+  if (rr2 > ri2w + wtol) { terminate(L, 4, PF_INVALID_MEDIUM, SUB_WATER_OUTSIDE); return; }
   const double tsd = bore_wall_hit_d(P, L.x, L.y, L.z, L.vx, L.vy, L.vz, P.rin, P.zmin, P.zwall_top);
-  if (tsd < 0.0) { terminate(L, 4, PF_INVALID_MEDIUM); return; }
+  if (tsd < 0.0) { terminate(L, 4, PF_INVALID_MEDIUM, SUB_WATER_OUTSIDE); return; }
   const float tside = (tsd < 0.5 * (double)kInfF) ? (float)tsd : kInfF;
   float ttop = kInfF, tbot = kInfF;
   if (L.vz < -1e-14f) tbot = (P.zmin - L.z) / L.vz;
@@ -2498,7 +2525,7 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   const bool axial_top = (ttop <= tbot && ttop < tside);
   const bool axial_bot = (tbot < ttop && tbot < tside);
   const float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
-  if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY); return; }
+  if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY, SUB_WATER_NO_BOUNDARY); return; }
   if (!early && tbound > 0.0f && MU_TRACK > 0.0f)
     freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tbound) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
@@ -2530,7 +2557,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   const float MU_TRACK = P.mu_track;
   const double er2 = (double)L.x * L.x + (double)L.y * L.y;
   if (er2 > ri2w + wtol || L.z < P.zmin - kEpsF || L.z > surface_z_vs(P, L.x, L.y, L.z) + kEpsF) {
-    terminate(L, 4, PF_INVALID_MEDIUM);
+    terminate(L, 4, PF_INVALID_MEDIUM, SUB_WATER_OUTSIDE);
     return;
   }
   float LG = P.mu_geom, LW = P.mu_wave, LT = MU_TOTAL;
@@ -2557,7 +2584,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     // the majorant the collision was sampled with: the slab's, or mu_track
     const float MU_LOC = (P.n_slabs > 0) ? P.slab_mu[slab_of(P, L.z)] : MU_TRACK;
     const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_LOC), 1.0f);
-    if (LT > MU_LOC + slack) { terminate(L, 4, PF_MAJORANT); return; }
+    if (LT > MU_LOC + slack) { terminate(L, 4, PF_MAJORANT, SUB_MAJORANT); return; }
     const float pacc = fminf(LT / fmaxf(MU_LOC, 1e-30f), 1.0f);
     if (U(L.r) >= pacc) { A.n_null++; return; }
   }
@@ -2614,7 +2641,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     if (fr > 1e-12f) { fx = -L.y / fr; fy = L.x / fr; }
     rad = P.radius[pidx];
     if (rad <= 0.0f) return;
-    if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) { terminate(L, 4, PF_SPHEROID); return; }
+    if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) { terminate(L, 4, PF_SPHEROID, SUB_SPHEROID_ASPECT); return; }
     if (fabsf(P.aspect - 1.0f) < 1e-7f) {
       sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
       dcx = -(double)rad * (double)gx; dcy = -(double)rad * (double)gy; dcz = -(double)rad * (double)gz;
@@ -2643,7 +2670,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     CondOut h = conditional_event<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials);
     A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
     if (fallback ? (h.status != 1) : (h.status != 1 && h.status != 2)) {
-      terminate(L, 4, h.failure ? h.failure : PF_OTHER);
+      terminate(L, 4, h.failure ? h.failure : PF_OTHER, h.subcode);
       return;
     }
     L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
@@ -2689,7 +2716,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   }
   if (ps != 1) {
     if constexpr (kLight) light_mark(*D, gid, 10);
-    terminate(L, 4, ppf ? ppf : PF_OTHER);
+    terminate(L, 4, ppf ? ppf : PF_OTHER, ppfs);
     return;
   }
   {
@@ -2714,7 +2741,7 @@ __device__ __forceinline__ void water_iface(Lane& L, const DevParams& P, Acc& A)
     if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, snx, sny, snz, P.n_water, P.n_air, tir)) {
       L.tr++;
       if (tir) L.flags |= F_TTIR;
-      if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) terminate(L, 4, PF_INVALID_MEDIUM);
+      if (!water_surface_offset(P, L.x, L.y, L.z, snx, sny, snz)) terminate(L, 4, PF_INVALID_MEDIUM, SUB_WATER_OUTSIDE);
       return;
     }
     L.x -= kEpsF * snx; L.y -= kEpsF * sny; L.z -= kEpsF * snz;
@@ -2724,7 +2751,7 @@ __device__ __forceinline__ void water_iface(Lane& L, const DevParams& P, Acc& A)
   if (axial_bot) {
     // water -> bottom disc, CLARITAS.py:3211-3228
     const double r2 = (double)L.x * (double)L.x + (double)L.y * (double)L.y;
-    if (!(r2 < (double)P.rin * (double)P.rin)) { terminate(L, 4, PF_BOTTOM_GEOMETRY); return; }
+    if (!(r2 < (double)P.rin * (double)P.rin)) { terminate(L, 4, PF_BOTTOM_GEOMETRY, SUB_BOTTOM_TOPOLOGY); return; }
     if (!fresnel_transmit(L.r, L.vx, L.vy, L.vz, 0.0f, 0.0f, 1.0f, P.n_water, P.n_acrylic, tir)) {
       L.bir++;
       if (tir) L.flags |= F_BTIR;
@@ -2875,12 +2902,16 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
       clr_record q;
       q.x = L.x; q.y = L.y; q.z = L.z; q.vx = L.vx; q.vy = L.vy; q.vz = L.vz;
       q.status = L.st; q.native_id = nat; q.mirror_id = mir; q.interactions = L.ic;
-      q.failure = L.pf; q.draws = (int32_t)rng_draws(L.r);
+      q.failure = L.pf & 255; q.draws = (int32_t)rng_draws(L.r);
       rec[hidx] = q;
     }
   }
   atomicAdd(&T.status[L.st], 1u);
-  if (L.st == 4) atomicAdd(&T.fail[L.pf < CLR_N_FAILURE ? L.pf : PF_OTHER], 1u);
+  if (L.st == 4) {
+    const int code = L.pf & 255;
+    atomicAdd(&T.fail[code < CLR_N_FAILURE ? code : PF_OTHER], 1u);
+    atomicAdd(&Gt->failure_subcodes[subcode_slot(L.pf >> 8)], 1ull);  // rare: global
+  }
   if (L.flags & F_ENTERED) atomicAdd(&T.ich[L.ic < CLR_N_ICHIST - 1 ? L.ic : CLR_N_ICHIST - 1], 1u);
   if (nat >= 0) {
     atomicAdd(&T.det[(g * 4 + 0) * CLR_N_CHANNELS + nat], 1u);

@@ -104,6 +104,11 @@ class ForwardResult:
     numerical_failure_fraction: float
     production_failure_count: int
     production_failure_code_counts: dict
+    production_failure_subcode_counts: dict
+    production_watchdog_count: int
+    production_geometry_failure_count: int
+    production_nonfinite_count: int
+    production_failure_diagnostics: dict
     n_rays: int
     requested_n_rays: int
     stopped_adaptively: bool
@@ -227,6 +232,46 @@ class ForwardResult:
             elif isinstance(v, np.generic):
                 d[k] = v.item()
         return d
+
+
+# failure-code groups, CLARITAS.py:461-463
+WATCHDOG_CODES = (2, 4, 6, 8, 10)
+GEOMETRY_FAILURE_CODES = (3, 5, 7, 9, 13, 16, 17)
+NONFINITE_CODES = (11, 12)
+FAILURE_SAMPLE_LIMIT = 512
+
+
+def _failure_summary(T, fails, total):
+    """Subcode counts, code-group counts and the production-failure
+    dictionary of CLARITAS.py:9608-9655.  Per-ray failure samples with the
+    particle drill-down are not recorded (the list is empty), and the
+    particle proposal / accessibility counts are not tallied (None)."""
+    sub = T['failure_subcodes']
+    sub_counts = {str(abi.SUBCODE_VALUES[k]): int(sub[k]) for k in range(len(abi.SUBCODE_VALUES)) if sub[k]}
+    if sub[abi.N_SUBCODE - 1]:
+        sub_counts['other'] = int(sub[abi.N_SUBCODE - 1])
+    sub_counts = dict(sorted(sub_counts.items(), key=lambda kv: (kv[0] == 'other', int(kv[0]) if kv[0] != 'other' else 0)))
+    code_counts = {abi.PRODUCTION_FAILURE_NAMES[i]: int(fails[i]) for i in range(1, abi.N_FAILURE) if fails[i]}
+    n_fail = int(np.sum(fails[1:]))
+    trapped = int(T['status'][8])
+    return dict(
+        production_failure_subcode_counts=sub_counts,
+        production_watchdog_count=int(sum(fails[i] for i in WATCHDOG_CODES)),
+        production_geometry_failure_count=int(sum(fails[i] for i in GEOMETRY_FAILURE_CODES)),
+        production_nonfinite_count=int(sum(fails[i] for i in NONFINITE_CODES)),
+        production_failure_diagnostics={
+            'code_counts': code_counts,
+            'subcode_counts': dict(sub_counts),
+            'samples': [],
+            'sample_limit': FAILURE_SAMPLE_LIMIT,
+            'count_closure_ok': n_fail == int(T['status'][4]) and int(np.sum(sub)) == n_fail,
+            'particle_proposal_count': None,
+            'particle_accessible_count': None,
+            'particle_null_count': None,
+            'acrylic_trapped_tir_count': trapped,
+            'acrylic_trapped_tir_fraction': float(trapped / max(int(total), 1)),
+        },
+    )
 
 
 class ClaritasForwardModel:
@@ -791,7 +836,8 @@ class ClaritasForwardModel:
     @staticmethod
     def _merge(a, b):
         out = dict(a)
-        for k in ('aux', 'incidence_sum', 'status', 'failures', 'counters', 'ic_hist', 'stray', 'face_stray'):
+        for k in ('aux', 'incidence_sum', 'status', 'failures', 'failure_subcodes', 'counters', 'ic_hist', 'stray',
+                  'face_stray'):
             out[k] = a[k] + b[k]
         return out
 
@@ -889,6 +935,7 @@ class ClaritasForwardModel:
             production_failure_count=n_fail,
             production_failure_code_counts={abi.PRODUCTION_FAILURE_NAMES[i]: int(fails[i])
                                             for i in range(1, abi.N_FAILURE) if fails[i]},
+            **_failure_summary(T, fails, total),
             n_rays=int(total),
             requested_n_rays=int(requested),
             stopped_adaptively=bool(adaptive and stop_reason == 'target_detector_score'),

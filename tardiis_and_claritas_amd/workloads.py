@@ -146,6 +146,32 @@ def config5(event_model='plain_mie', n=(128, 128, 256), seed=SEED):
     return kw, mat, 20.0, 10_000_000_000, seed
 
 
+def measure_causal(case, n):
+    """Throughput of the production kernel and of the causal-provenance
+    variant (clr_run_causal, records copied and binned) on the config-1
+    cell: case 'hybrid' (smooth spheres, hybrid law) or 'rough' (the
+    reference's default rough-spheroid physics)."""
+    import time
+    from . import causal
+    from .model import ClaritasForwardModel
+    kw, mat, conc, _, seed = config1('hybrid_mie_exact_rate') if case == 'hybrid' else config1_rough()
+    m = ClaritasForwardModel(**kw)
+    mat, _ = m.prepare(mat, conc)
+    m.run_tallies(min(n, 100_000), seed)  # warm-up
+    t0 = time.time()
+    _, _, ms_prod = m.run_tallies(n, seed)
+    wall_prod = time.time() - t0
+    acc = causal.CausalAccumulator(m, len(mat.diameter_m))
+    t1 = time.time()
+    acc.run(n, seed, 0, m._launch())
+    wall_causal = time.time() - t1
+    ms_causal = float(m.ctx.lib.clr_last_kernel_ms(m.ctx.h))
+    return dict(workload=f'causal:{case}', histories=n, records=acc.count,
+                production_histories_per_s=n / (ms_prod * 1e-3), production_wall_s=wall_prod,
+                causal_last_range_kernel_ms=ms_causal, causal_wall_s=wall_causal,
+                causal_histories_per_s_wall=n / wall_causal)
+
+
 def measure(name, histories=None, event_model='plain_mie'):
     """Run one workload on the GPU and return a summary dict (throughput from
     the library's HIP-event kernel time, detector signals, failure counts)."""
@@ -154,6 +180,8 @@ def measure(name, histories=None, event_model='plain_mie'):
     if name == 'meridional':
         from .meridional import measure as measure_solve
         return measure_solve(repeats=int(histories or 5))
+    if name.startswith('causal:'):
+        return measure_causal(name.split(':')[1], int(histories or 2_000_000))
     if name == 'config1':
         kw, mat, conc, n, seed = config1(event_model)
     elif name == 'config5':
@@ -185,7 +213,8 @@ if __name__ == '__main__':
     import argparse
     import json
     ap = argparse.ArgumentParser(description='run one benchmark workload on the GPU')
-    ap.add_argument('name', help="config1 | config5 | config2:<mg/L> | meridional (device sediment solve, "
+    ap.add_argument('name', help="config1 | config5 | config2:<mg/L> | causal:<hybrid|rough> (production vs "
+                                 "causal-provenance throughput) | meridional (device sediment solve, "
                                  "--histories = repeats)")
     ap.add_argument('--histories', type=float, default=None)
     ap.add_argument('--event-model', default='plain_mie')

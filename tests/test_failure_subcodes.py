@@ -1,0 +1,55 @@
+"""Production failure subcodes (CLARITAS.py:464-521, 8483-8505, 9608-9655):
+the tallies count failed histories by subcode slot; the slots add up to the
+failure codes and to status 4; a forced trace-iteration limit lands in
+subcode 3002 (TRACE_ITERATION_LIMIT) and a forced sphere bounce limit in
+1005 (SPHERE_BOUNCE_LIMIT); device and oracle agree on the same histories.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tardiis_and_claritas_amd import abi, workloads
+from tardiis_and_claritas_amd.model import ClaritasForwardModel
+
+SLOT = {v: k for k, v in enumerate(abi.SUBCODE_VALUES)}
+
+
+def _forced():
+    """config-1 cell, geometric law on 40 um smooth spheres at 5 g/L, with a
+    trace-iteration cap of 6 and no internal reflection allowed"""
+    kw, _, _, _, _ = workloads.config1('geometric')
+    kw.update(particle_surface_rms_slope_deg=0.0, wave_surface_rms_height_m=0.0, max_iterations=6,
+              max_internal_bounces=0)
+    return kw, workloads.monodisperse(40e-6, 1.46, 0.0), 5.0
+
+
+def _check(T):
+    sub, fails = T['failure_subcodes'], T['failures']
+    assert int(sub.sum()) == int(fails.sum()) == int(T['status'][4]) > 0
+    return sub
+
+
+def test_oracle_subcode_slots_close():
+    kw, mat, conc = _forced()
+    m = ClaritasForwardModel(**kw)
+    T, _ = po.OracleModel(m, mat, conc).run(20_000, 3, 0)
+    sub = _check(T)
+    assert sub[SLOT[3002]] > 0 and sub[SLOT[1005]] > 0
+    assert sub[abi.N_SUBCODE - 1] == 0
+
+
+@pytest.mark.gpu
+def test_device_subcodes_match_oracle(gpu_available, oracle_lib):
+    kw, mat, conc = _forced()
+    m = ClaritasForwardModel(**kw)
+    m.prepare(mat, conc)
+    N = 40_000
+    Tg, _, _ = m.run_tallies(N, 3, 0)
+    To, _ = oracle_lib.OracleModel(m, mat, conc).run(N, 3, 0)
+    sg, so = _check(Tg), _check(To)
+    assert np.abs(sg - so).sum() <= 1e-3 * N, (np.flatnonzero(sg), sg[sg > 0], so[so > 0])
+    r = m.simulate(mat, conc, n_rays=N, seed=3, path_diagnostics=False)
+    d = r.production_failure_diagnostics
+    assert d['count_closure_ok'] and r.production_failure_subcode_counts['3002'] > 0
+    assert r.production_watchdog_count + r.production_geometry_failure_count + r.production_nonfinite_count <= \
+        r.production_failure_count
