@@ -102,6 +102,28 @@ class ForwardResult:
     valid_exit_count: int
     numerical_failure_count: int
     numerical_failure_fraction: float
+    aggregation_model: str
+    aggregation_collision_exposure: float
+    aggregation_number_reduction_fraction: float
+    aggregation_mass_conservation_error: float
+    aggregation_projected_area_ratio: float
+    aggregation_largest_bin_mass_fraction: float
+    aggregation_overflow_event_fraction: float
+    aggregation_steps: int
+    population_number_reduction_fraction: float
+    fragmentation_breakup_exposure: float
+    fragmentation_reference_diameter_m: float
+    fragmentation_size_exponent: float
+    fragmentation_underflow_event_fraction: float
+    aggregation_events_per_initial_particle: float
+    fragmentation_events_per_initial_particle: float
+    wave_coherent_fraction_nominal_cuda: float
+    mean_morphology_wave_interactions: float
+    morphology_fraction_of_wave_interactions: float
+    vortex_asymptotic_height_parameter_mm: float
+    vortex_area_mean_uncorrected_rise_mm: float
+    vortex_equivalent_core_rpm: float
+    vortex_equivalent_rigid_body_rpm: float
     production_failure_count: int
     production_failure_code_counts: dict
     production_failure_subcode_counts: dict
@@ -215,6 +237,10 @@ class ForwardResult:
     particle_diagnostics: Optional[dict] = None
     light_transport_diagnostics: Optional[dict] = None
     detector_particle_path_diagnostics: Optional[dict] = None
+    # per-ray outputs and heat maps (collect_rays / heatmap_size) are not produced
+    ray_data: Optional[dict] = None
+    heatmap_xy: Optional[np.ndarray] = None
+    heatmap_zy: Optional[np.ndarray] = None
     # ---- extensions (no reference field) ----
     aux_detector_angles_deg: Optional[np.ndarray] = None
     aux_native_hits: Optional[np.ndarray] = None
@@ -360,6 +386,11 @@ class ClaritasForwardModel:
         density_kg_per_m3=2600.0,
         bottom_disc_thickness_m=0.003,
         bottom_external_refractive_index=1.0,
+        aggregation_model='none',
+        aggregation_collision_exposure=0.0,
+        fragmentation_breakup_exposure=0.0,
+        fragmentation_reference_diameter_m=100e-6,
+        fragmentation_size_exponent=1.0,
         max_iterations=int(1e6),
         max_internal_bounces=64,
         max_cell_bounces=64,
@@ -444,6 +475,17 @@ class ClaritasForwardModel:
         self.bottom_disc_thickness = float(bottom_disc_thickness_m)
         self.bottom_external_n = float(bottom_external_refractive_index)
         self.max_iterations = int(max_iterations)
+        # population balance (CLARITAS.py:3599-3843): only the unaggregated PSD
+        # (model 'none', exposure 0: the weights are normalised) is built
+        self.aggregation_model = str(aggregation_model).strip().lower()
+        self.aggregation_collision_exposure = float(aggregation_collision_exposure)
+        self.fragmentation_breakup_exposure = float(fragmentation_breakup_exposure)
+        self.fragmentation_reference_diameter_m = float(fragmentation_reference_diameter_m)
+        self.fragmentation_size_exponent = float(fragmentation_size_exponent)
+        if self.aggregation_model != 'none' or self.aggregation_collision_exposure != 0.0 or \
+                self.fragmentation_breakup_exposure != 0.0:
+            raise NotImplementedError('orthokinetic aggregation / fragmentation of the PSD is not built: '
+                                      "aggregation_model must be 'none' with zero exposures")
         self.max_internal_bounces = int(max_internal_bounces)
         self.max_cell_bounces = int(max_cell_bounces)
         self.chunk_size = int(chunk_size)
@@ -588,6 +630,7 @@ class ClaritasForwardModel:
     def particles_struct(self, material: Material, concentration_g_per_L: float, weights, weight_mode):
         d = np.ascontiguousarray(material.diameter_m, dtype=np.float64)
         w = np.ascontiguousarray(material.weights if weights is None else weights, dtype=np.float64)
+        self.last_psd_weights = w
         if w.ndim != 1 or w.size != d.size:
             raise ValueError(f'{material.name} requires {d.size} PSD weights')
         if np.any(~np.isfinite(w)) or np.any(w < 0.0) or not np.any(w > 0.0):
@@ -859,6 +902,30 @@ class ClaritasForwardModel:
             return self._sym(det)
         return self._pair(pd[0], pd[1])
 
+    def _population_fields(self):
+        """Population-balance result fields of the unaggregated PSD
+        (CLARITAS.py:3627-3631, 6878-6895 with exposure 0): no reduction, no
+        events, unchanged area; the largest bin's share of the normalised
+        input weights."""
+        w = np.asarray(getattr(self, 'last_psd_weights', [1.0]), dtype=np.float64)
+        return dict(
+            aggregation_model=self.aggregation_model,
+            aggregation_collision_exposure=self.aggregation_collision_exposure,
+            aggregation_number_reduction_fraction=0.0,
+            aggregation_mass_conservation_error=0.0,
+            aggregation_projected_area_ratio=1.0,
+            aggregation_largest_bin_mass_fraction=float(w[-1] / w.sum()) if w.sum() > 0 else 0.0,
+            aggregation_overflow_event_fraction=0.0,
+            aggregation_steps=0,
+            population_number_reduction_fraction=0.0,
+            fragmentation_breakup_exposure=self.fragmentation_breakup_exposure,
+            fragmentation_reference_diameter_m=self.fragmentation_reference_diameter_m,
+            fragmentation_size_exponent=self.fragmentation_size_exponent,
+            fragmentation_underflow_event_fraction=0.0,
+            aggregation_events_per_initial_particle=0.0,
+            fragmentation_events_per_initial_particle=0.0,
+        )
+
     def _result(self, mat, T, det, batches, pbatches, group_n, requested, total, adaptive, stop_reason, ms):
         cnt = T['counters']
         ci = abi.C_IDX
@@ -955,10 +1022,15 @@ class ClaritasForwardModel:
             mu_ext_per_m=float(mus[4]),
             geometric_optical_depth_diameter=float(mus[2]) * 2 * self.rin,
             optical_depth_diameter=float(mus[6]) * 2 * self.rin,
+            **self._population_fields(),
             mean_interactions=per_entered('sum_ic'),
             mean_geometric_interactions=per_entered('sum_gic'),
             mean_wave_interactions=per_entered('sum_wic'),
             mean_coherent_wave_interactions=per_entered('sum_wcoh'),
+            wave_coherent_fraction_nominal_cuda=float(mus[7]),
+            mean_morphology_wave_interactions=((cnt[ci['sum_wic']] - cnt[ci['sum_wcoh']]) / den) if entered else nan,
+            morphology_fraction_of_wave_interactions=(
+                (cnt[ci['sum_wic']] - cnt[ci['sum_wcoh']]) / cnt[ci['sum_wic']] if cnt[ci['sum_wic']] > 0 else 0.0),
             median_interactions=_histogram_median(T['ic_hist']),
             ballistic_fraction=per_entered('ballistic'),
             mean_fresnel_reflections=per_entered('sum_pr'),
@@ -975,6 +1047,10 @@ class ClaritasForwardModel:
             free_surface_model=self.free_surface_model,
             vortex_wall_center_height_difference_mm=self.vortex_delta_h * 1e3,
             vortex_core_radius_mm=self.vortex_core_radius * 1e3,
+            vortex_asymptotic_height_parameter_mm=vg['h_inf'] * 1e3,
+            vortex_area_mean_uncorrected_rise_mm=vg['h_bar'] * 1e3,
+            vortex_equivalent_core_rpm=vg['equivalent_core_rpm'],
+            vortex_equivalent_rigid_body_rpm=vg['equivalent_rigid_body_rpm'],
             vortex_center_surface_z_mm=vg['center_z'] * 1e3,
             vortex_wall_surface_z_mm=vg['wall_z'] * 1e3,
             sediment_transport_model=self.sediment_transport_model,

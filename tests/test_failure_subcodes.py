@@ -53,3 +53,14 @@ def test_device_subcodes_match_oracle(gpu_available, oracle_lib):
     assert d['count_closure_ok'] and r.production_failure_subcode_counts['3002'] > 0
     assert r.production_watchdog_count + r.production_geometry_failure_count + r.production_nonfinite_count <= \
         r.production_failure_count
+
+
+def test_population_balance_is_the_unaggregated_psd():
+    """Only aggregation_model 'none' with zero exposures is built
+    (CLARITAS.py:5632-5636 defaults); anything else raises."""
+    m = ClaritasForwardModel()
+    assert m.aggregation_model == 'none' and m.fragmentation_reference_diameter_m == 100e-6
+    with pytest.raises(NotImplementedError):
+        ClaritasForwardModel(aggregation_model='orthokinetic_compact')
+    with pytest.raises(NotImplementedError):
+        ClaritasForwardModel(aggregation_collision_exposure=0.5)

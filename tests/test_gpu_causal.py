@@ -179,6 +179,13 @@ def test_simulate_path_diagnostics(gpu_available, builder):
     # mean interactions >= fraction of histories with any interaction
     assert np.all(d['mean_particle_interactions_by_channel'][seen] >= d['any_particle_fraction'][seen] - 1e-12)
     assert d['conditional_max_trials'] == 16384 and d['conditional_sampler_counts']['candidate_trials'] >= 0
+    # result fields of the unaggregated PSD, the morphology share and the vortex geometry
+    assert r.aggregation_model == 'none' and r.aggregation_steps == 0 and r.aggregation_projected_area_ratio == 1.0
+    assert 0.0 < r.aggregation_largest_bin_mass_fraction <= 1.0
+    assert 0.0 <= r.morphology_fraction_of_wave_interactions <= 1.0
+    assert r.mean_morphology_wave_interactions == pytest.approx(
+        r.mean_wave_interactions - r.mean_coherent_wave_interactions, abs=1e-12)
+    assert r.ray_data is None and r.heatmap_xy is None
     r2 = m.simulate(mat, conc, n_rays=200_000, seed=seed, path_diagnostics=False)
     assert r2.detector_particle_path_diagnostics is None
     assert np.abs(r.hardware_native_direct_hits - r2.hardware_native_direct_hits).sum() <= 200
