@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 7
+#define CLR_ABI_VERSION 8
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -332,11 +332,12 @@ typedef struct clr_meridional_out {
 /* ---- light-transport voxel diagnostics (the reference's
  * light_transport_config, CLARITAS.py:7232-7300) ---- */
 typedef struct clr_light_config {
-  uint64_t max_rays;            /* histories with id < max_rays are recorded (>= 1) */
+  uint64_t max_rays;            /* histories with id < max_rays are recorded (0: no voxel
+                                   diagnostics, only the heat maps) */
   int32_t nx, ny, nz;           /* voxels over [-rin, rin]^2 x [zmin, zmax], each >= 4 */
   int32_t representative_rays;  /* the first histories whose segments are stored (<= max_rays) */
   int32_t max_segments;         /* segments stored per representative history (>= 1) */
-  int32_t pad_;
+  int32_t heatmap_size;         /* side of the water-path heat maps (0: none), CLARITAS.py:3296-3309 */
   double zmax;                  /* top of the box: max(centre, wall) height of the free surface */
 } clr_light_config;
 
@@ -363,6 +364,10 @@ typedef struct clr_light_out {
   uint64_t segment_count, scatter_event_total, absorption_event_total;
   uint32_t error_count;              /* traversal-integrity failures and out-of-box sites */
   uint32_t boundary_adjustment_count;/* benign endpoint snaps */
+  float* heatmap_xy;                 /* [heatmap_size][heatmap_size] water-path samples every 10 um
+                                        of every history, x-y projection (row: y from the top) */
+  float* heatmap_zy;                 /* the z-y projection (column: z from zmin to the surface
+                                        height at the wall) */
   uint32_t outside[8];               /* out-of-box sites: count, first x, y, z (FP32 bits),
                                         history id, tolerance (FP32 bits), recorded flag */
 } clr_light_out;
@@ -492,8 +497,10 @@ int clr_run_light(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_offs
  * kernels light_diag_segment_test_kernel / light_diag_event_test_kernel
  * (CLARITAS.py:3470-3485): mode 0 deposits segment i of data[n][7]
  * (x0, y0, z0, vx, vy, vz, dist; history id i, particle order i & 1), mode 1
- * records event site i of data[n][3] into the scatter-event grids, in the
- * box [-rin, rin]^2 x [zmin, light->zmax].  Outputs as clr_run_light. */
+ * records event site i of data[n][3] into the scatter-event grids, mode 2
+ * rasters segment i of data[n][7] into the heat maps (z axis up to
+ * light->zmax), in the box [-rin, rin]^2 x [zmin, light->zmax].  Outputs as
+ * clr_run_light. */
 /* clr_run with the causal path provenance of every history that reaches a
  * hardware detector (native or mirror), for the reference's detector-path,
  * causal and H170 diagnostics (CLARITAS.py:7352-7613 state, 8017-8480

@@ -2808,6 +2808,35 @@ static void light_segment(const light_t* L, uint64_t gid, float x0, float y0, fl
 }
 
 /* CLARITAS.py:3402-3407 */
+/* water-path heat maps: a sample every 10 um along the segment (at most
+ * 20000) into the x-y and z-y projections, z up to zvis.
+ * CLARITAS.py:3296-3309 */
+static void heat_raster(const light_t* L, float zvis, float x0, float y0, float z0, float vx, float vy, float vz,
+                        float dist) {
+  const int vis = L->c->heatmap_size;
+  if (vis <= 0 || !(dist > 0.0f) || !L->o->heatmap_xy || !L->o->heatmap_zy) return;
+  const float spacing = 1.0e-5f, Rin = L->rin, zmin = L->zmin;
+  int n = (int)ceilf(dist / spacing);
+  n = n < 1 ? 1 : (n > 20000 ? 20000 : n);
+  for (int k = 1; k <= n; ++k) {
+    float f = (float)k / (float)n;
+    float x = x0 + vx * dist * f, y = y0 + vy * dist * f, z = z0 + vz * dist * f;
+    int ix = (int)(((x + Rin) / (2.0f * Rin)) * vis);
+    int iy = vis - 1 - (int)(((y + Rin) / (2.0f * Rin)) * vis);
+    int iz = (int)(((z - zmin) / (zvis - zmin)) * vis);
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    L->o->heatmap_xy[(ix >= 0 && ix < vis && iy >= 0 && iy < vis) ? iy * vis + ix : 0] +=
+        (ix >= 0 && ix < vis && iy >= 0 && iy < vis) ? 1.0f : 0.0f;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    L->o->heatmap_zy[(iz >= 0 && iz < vis && iy >= 0 && iy < vis) ? iy * vis + iz : 0] +=
+        (iz >= 0 && iz < vis && iy >= 0 && iy < vis) ? 1.0f : 0.0f;
+  }
+}
+
 static void light_mark(const light_t* L, uint64_t gid, int code) {
   const clr_light_config* c = L->c;
   if (gid >= c->max_rays || gid >= (uint64_t)c->representative_rays || c->max_segments <= 0) return;
@@ -3003,7 +3032,11 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
       }
       int do_particle = freep < tbound;
       float travel = do_particle ? freep : tbound;
-      if (Ld) light_segment(Ld, hist, p->x, p->y, p->z, p->vx, p->vy, p->vz, travel, o->ic);
+      if (Ld) {
+        if (Ld->c->heatmap_size > 0)
+          heat_raster(Ld, free_surface_z(P, P->rin, 0.0f), p->x, p->y, p->z, p->vx, p->vy, p->vz, travel);
+        light_segment(Ld, hist, p->x, p->y, p->z, p->vx, p->vy, p->vz, travel, o->ic);
+      }
       o->n_flight++;
       p->x += travel * p->vx; p->y += travel * p->vy; p->z += travel * p->vz;
       o->pv.wp += travel;
@@ -3448,7 +3481,8 @@ int orc_run_causal(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offse
 int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
                   const clr_light_config* lc, clr_tallies* out, clr_light_out* lo) {
   if (!m || !lc || !lo || !out) return CLR_E_ARG;
-  if (lc->max_rays < 1 || lc->nx < 4 || lc->ny < 4 || lc->nz < 4 || lc->representative_rays < 0 ||
+  if ((lc->max_rays < 1 && lc->heatmap_size < 1) || lc->heatmap_size < 0 || lc->nx < 4 || lc->ny < 4 || lc->nz < 4 ||
+      lc->representative_rays < 0 ||
       (uint64_t)lc->representative_rays > lc->max_rays || lc->max_segments < 1 || !(lc->zmax > m->cfg.zmin))
     return CLR_E_ARG;
   size_t vox = (size_t)lc->nx * lc->ny * lc->nz, rep = (size_t)lc->representative_rays;
@@ -3462,6 +3496,9 @@ int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset
   if (lo->traj_order) for (size_t i = 0; i < tb; ++i) lo->traj_order[i] = -1;
   if (lo->traj_event) for (size_t i = 0; i < tb; ++i) lo->traj_event[i] = -1;
   if (lo->traj_count) memset(lo->traj_count, 0, rep * 4);
+  size_t vis2 = (size_t)lc->heatmap_size * (size_t)lc->heatmap_size;
+  if (lo->heatmap_xy) memset(lo->heatmap_xy, 0, vis2 * 4);
+  if (lo->heatmap_zy) memset(lo->heatmap_zy, 0, vis2 * 4);
   if (lo->traj_overflow) memset(lo->traj_overflow, 0, rep * 4);
   lo->segment_total = lo->deposited_total = 0.0;
   lo->segment_count = lo->scatter_event_total = lo->absorption_event_total = 0;
@@ -3474,6 +3511,18 @@ int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset
 /* the reference's segment and event test kernels (CLARITAS.py:3470-3485)
  * on explicit inputs: segments7 [n][7] = x0, y0, z0, vx, vy, vz, dist with
  * particle order i & 1, xyz3 [n][3] event sites */
+/* heat-map raster of segments7 [n][7] with the z axis up to lc->zmax (the
+ * caller zeroes the maps) */
+int orc_light_heat_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* segments7,
+                         clr_light_out* lo) {
+  light_t L = {lc, lo, rin, zmin, (float)lc->zmax};
+  for (int i = 0; i < n; ++i) {
+    const float* q = segments7 + 7 * (size_t)i;
+    heat_raster(&L, (float)lc->zmax, q[0], q[1], q[2], q[3], q[4], q[5], q[6]);
+  }
+  return 0;
+}
+
 int orc_light_segment_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* segments7,
                             clr_light_out* lo) {
   light_t L = {lc, lo, rin, zmin, (float)lc->zmax};

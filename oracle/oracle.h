@@ -46,6 +46,8 @@ int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t history_offs
  * detected[n]: 1 when the history reached a hardware detector) */
 int orc_run_causal(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
                    clr_tallies* out, clr_causal_record* crec, uint8_t* detected);
+int orc_light_heat_probe(const clr_light_config* lc, float rin, float zmin, int n, const float* segments7,
+                         clr_light_out* lo);
 int orc_run_light(const orc_model* m, uint64_t n, uint64_t seed, uint64_t offset, int nthreads,
                   const clr_light_config* lc, clr_tallies* out, clr_light_out* lo);
 /* The reference's diagnostic test kernels (CLARITAS.py:3470-3485): segment i

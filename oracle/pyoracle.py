@@ -80,7 +80,7 @@ def load():
     lib.orc_run_causal.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(abi.Tallies),
                                    C.c_void_p, P(C.c_uint8)]
     lib.orc_run_causal.restype = C.c_int
-    for name in ('orc_light_segment_probe', 'orc_light_event_probe'):
+    for name in ('orc_light_segment_probe', 'orc_light_event_probe', 'orc_light_heat_probe'):
         fn = getattr(lib, name)
         fn.argtypes = [P(abi.LightConfig), C.c_float, C.c_float, C.c_int, P(C.c_float), P(abi.LightOut)]
         fn.restype = C.c_int
@@ -282,7 +282,7 @@ class OracleModel:
         """orc_run_light: (tallies dict, arrays dict, scalars dict) with the
         clr_run_light contract."""
         lc = light_config
-        out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments)
+        out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments, lc.heatmap_size)
         t = abi.Tallies()
         rc = self.lib.orc_run_light(self.h, int(n), int(seed), int(history_offset), int(threads), C.byref(lc),
                                     C.byref(t), C.byref(out))
@@ -387,15 +387,16 @@ def meridional_solve(model, material, coeff=True, settings=None):
 # ----------------------------------------------------------- light probes
 def light_probe(mode, data, light_config, rin, zmin):
     """The reference's diagnostic test kernels on the oracle: mode 0 deposits
-    segments [n][7], mode 1 event sites [n][3]; returns (arrays, scalars)."""
+    segments [n][7], mode 1 event sites [n][3], mode 2 rasters segments [n][7]
+    into the heat maps; returns (arrays, scalars)."""
     lc = light_config
     data = np.ascontiguousarray(data, dtype=np.float32)
-    out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments)
+    out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments, lc.heatmap_size)
     for k in ('traj_x0', 'traj_y0', 'traj_z0', 'traj_x1', 'traj_y1', 'traj_z1'):
         arrs[k][:] = np.nan
     for k in ('traj_order', 'traj_event'):
         arrs[k][:] = -1
-    fn = load().orc_light_segment_probe if mode == 0 else load().orc_light_event_probe
+    fn = {0: load().orc_light_segment_probe, 1: load().orc_light_event_probe, 2: load().orc_light_heat_probe}[mode]
     fn(C.byref(lc), float(rin), float(zmin), int(data.shape[0]), _pf(data), C.byref(out))
     return arrs, abi.light_scalars(out)
 

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -224,7 +224,7 @@ PU32 = C.POINTER(C.c_uint32)
 
 class LightConfig(C.Structure):
     _fields_ = [('max_rays', U64), ('nx', I32), ('ny', I32), ('nz', I32), ('representative_rays', I32),
-                ('max_segments', I32), ('pad_', I32), ('zmax', D)]
+                ('max_segments', I32), ('heatmap_size', I32), ('zmax', D)]
 
 
 class LightOut(C.Structure):
@@ -234,7 +234,8 @@ class LightOut(C.Structure):
                 ('traj_order', PI), ('traj_event', PI), ('traj_count', PI), ('traj_overflow', PI),
                 ('segment_total', D), ('deposited_total', D), ('segment_count', U64),
                 ('scatter_event_total', U64), ('absorption_event_total', U64), ('error_count', C.c_uint32),
-                ('boundary_adjustment_count', C.c_uint32), ('outside', C.c_uint32 * 8)]
+                ('boundary_adjustment_count', C.c_uint32), ('heatmap_xy', PF), ('heatmap_zy', PF),
+                ('outside', C.c_uint32 * 8)]
 
 
 LIGHT_GRIDS = ('track_length', 'post_scatter_track_length', 'scatter_event_weight', 'scatter_event_count',
@@ -244,11 +245,14 @@ LIGHT_SCALARS = ('segment_total', 'deposited_total', 'segment_count', 'scatter_e
                  'absorption_event_total', 'error_count', 'boundary_adjustment_count')
 
 
-def light_out(nx, ny, nz, rep, max_segments):
+def light_out(nx, ny, nz, rep, max_segments, vis=0):
     """Allocate host outputs of one light-diagnostic call; returns (struct, arrays)."""
     import numpy as np
 
     arrs = {}
+    if vis > 0:
+        arrs['heatmap_xy'] = np.zeros((vis, vis), dtype=np.float32)
+        arrs['heatmap_zy'] = np.zeros((vis, vis), dtype=np.float32)
     for k in LIGHT_GRIDS:
         arrs[k] = np.zeros((nz, ny, nx), dtype=np.uint32 if k.endswith('count') else np.float32)
     for k in LIGHT_TRAJ:

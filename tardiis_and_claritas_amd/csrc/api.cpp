@@ -92,7 +92,7 @@ struct GraphCache {
 
 // Device buffers of the light-transport diagnostics (grown on demand).
 struct LightBufs {
-  DevBuf grids, scalars, traj, desc, probe;
+  DevBuf grids, scalars, traj, desc, probe, heat;
 };
 
 struct clr_ctx {
@@ -632,7 +632,9 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
 // Validate a light-diagnostic configuration, size and clear its device
 // buffers and upload the descriptor (CLARITAS.py:7232-7300).
 int light_setup(clr_ctx* c, const clr_light_config* lc, float rin, float zmin, LightDiag& D) {
-  if (lc->max_rays < 1) return fail(c, CLR_E_ARG, "light diagnostic max_rays must be >=1");
+  if (lc->heatmap_size < 0 || lc->heatmap_size > 4096) return fail(c, CLR_E_ARG, "heatmap_size must be in [0, 4096]");
+  if (lc->max_rays < 1 && lc->heatmap_size < 1)
+    return fail(c, CLR_E_ARG, "light diagnostic max_rays must be >=1 (or heat maps requested)");
   if (std::min(lc->nx, std::min(lc->ny, lc->nz)) < 4)
     return fail(c, CLR_E_ARG, "light diagnostic grid dimensions must each be >=4");
   if (lc->representative_rays < 0 || (uint64_t)lc->representative_rays > lc->max_rays || lc->max_segments < 1)
@@ -650,6 +652,8 @@ int light_setup(clr_ctx* c, const clr_light_config* lc, float rin, float zmin, L
   HIPCHK(c, B.scalars.ensure(128));
   HIPCHK(c, B.traj.ensure(traj * 32 + rep1 * 8));
   HIPCHK(c, B.desc.ensure(sizeof(LightDiag)));
+  const size_t vis2 = (size_t)lc->heatmap_size * lc->heatmap_size;
+  HIPCHK(c, B.heat.ensure(std::max<size_t>(vis2, 1) * 8));
   char* gp = B.grids.as<char>();
   char* sp = B.scalars.as<char>();
   D = LightDiag{};
@@ -678,6 +682,10 @@ int light_setup(clr_ctx* c, const clr_light_config* lc, float rin, float zmin, L
   D.nx = lc->nx; D.ny = lc->ny; D.nz = lc->nz;
   D.rep_rays = lc->representative_rays; D.max_segments = lc->max_segments;
   D.rin = rin; D.zmin = zmin; D.zmax = (float)lc->zmax;
+  D.vis = lc->heatmap_size;
+  D.heat_xy = B.heat.as<float>();
+  D.heat_zy = D.heat_xy + vis2;
+  if (vis2) HIPCHK(c, hipMemsetAsync(D.heat_xy, 0, vis2 * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(gp, 0, vox * 20, c->stream));
   HIPCHK(c, hipMemsetAsync(sp, 0, 128, c->stream));
   HIPCHK(c, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(tf), 0x7fc00000, traj * 6, c->stream));  // NaN
@@ -705,6 +713,11 @@ int light_readback(clr_ctx* c, const clr_light_config* lc, const LightDiag& D, c
     HIPCHK(c, get(lo->traj_y1, D.ty1, tb * 4)); HIPCHK(c, get(lo->traj_z1, D.tz1, tb * 4));
     HIPCHK(c, get(lo->traj_order, D.torder, tb * 4)); HIPCHK(c, get(lo->traj_event, D.tevent, tb * 4));
     HIPCHK(c, get(lo->traj_count, D.tcount, rep * 4)); HIPCHK(c, get(lo->traj_overflow, D.toverflow, rep * 4));
+  }
+  const size_t vis2 = (size_t)lc->heatmap_size * lc->heatmap_size;
+  if (vis2) {
+    HIPCHK(c, get(lo->heatmap_xy, D.heat_xy, vis2 * 4));
+    HIPCHK(c, get(lo->heatmap_zy, D.heat_zy, vis2 * 4));
   }
   unsigned char sc[80];
   HIPCHK(c, hipMemcpyAsync(sc, D.segment_total, 80, hipMemcpyDeviceToHost, c->stream));
@@ -1057,11 +1070,12 @@ int clr_run_causal(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const
 
 int clr_light_probe(clr_ctx* c, int mode, int n, const float* data, const clr_light_config* lc, double rin,
                     double zmin, clr_light_out* lo) {
-  if (!c || !data || !lc || !lo || n <= 0 || (mode != 0 && mode != 1)) return fail(c, CLR_E_ARG, "null argument");
+  if (!c || !data || !lc || !lo || n <= 0 || mode < 0 || mode > 2) return fail(c, CLR_E_ARG, "null argument");
+  if (mode == 2 && lc->heatmap_size < 1) return fail(c, CLR_E_ARG, "heat-map probe needs heatmap_size >= 1");
   HIPCHK(c, hipSetDevice(c->device));
   LightDiag D{};
   if (int e = light_setup(c, lc, (float)rin, (float)zmin, D)) return e;
-  const size_t nb = sizeof(float) * (size_t)n * (mode == 0 ? 7 : 3);
+  const size_t nb = sizeof(float) * (size_t)n * (mode == 1 ? 3 : 7);
   HIPCHK(c, c->light.probe.ensure(nb));
   HIPCHK(c, hipMemcpyAsync(c->light.probe.p, data, nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, clr_launch_light_probe(c->light.desc.as<LightDiag>(), c->light.probe.as<float>(), n, mode, c->stream));

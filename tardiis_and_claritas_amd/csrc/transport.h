@@ -87,6 +87,8 @@ struct LightDiag {
   uint64_t max_rays;
   int nx, ny, nz, rep_rays, max_segments, pad_;
   float rin, zmin, zmax, pad2_;
+  float *heat_xy, *heat_zy;  // water-path heat maps [vis][vis] (vis 0: none)
+  int vis, pad3_;
 };
 
 struct TransportLaunch {

@@ -2400,6 +2400,27 @@ __device__ void light_segment(const LightDiag& D, uint64_t gid, float x0, float 
 // CLARITAS.py:3402-3407 (0 propagation/boundary, 1-3 geometric external
 // reflection / transmission / internal reflection, 4 coherent Mie, 5-7 the
 // same for morphology, 8 inaccessible placement, 9 absorption, 10 failure)
+// Water-path heat maps: a sample every 10 um along the segment (at most
+// 20000), binned into the x-y and z-y projections; z runs from zmin to the
+// surface height at the wall.  CLARITAS.py:3296-3309, 4764-4765
+__device__ void heat_raster(const LightDiag& D, float zvis, float x0, float y0, float z0, float vx, float vy,
+                            float vz, float dist) {
+  const int vis = D.vis;
+  if (vis <= 0 || !(dist > 0.0f)) return;
+  const float spacing = 1.0e-5f, Rin = D.rin, zmin = D.zmin;
+  int n = (int)ceilf(dist / spacing);
+  n = n < 1 ? 1 : (n > 20000 ? 20000 : n);
+  for (int k = 1; k <= n; ++k) {
+    const float f = (float)k / (float)n;
+    const float x = x0 + vx * dist * f, y = y0 + vy * dist * f, z = z0 + vz * dist * f;
+    const int ix = (int)(((x + Rin) / (2.0f * Rin)) * vis);
+    const int iy = vis - 1 - (int)(((y + Rin) / (2.0f * Rin)) * vis);
+    const int iz = (int)(((z - zmin) / (zvis - zmin)) * vis);
+    if (ix >= 0 && ix < vis && iy >= 0 && iy < vis) atomicAdd(&D.heat_xy[iy * vis + ix], 1.0f);
+    if (iz >= 0 && iz < vis && iy >= 0 && iy < vis) atomicAdd(&D.heat_zy[iy * vis + iz], 1.0f);
+  }
+}
+
 __device__ __forceinline__ void light_mark(const LightDiag& D, uint64_t gid, int code) {
   if (gid >= D.max_rays || gid >= (uint64_t)D.rep_rays || D.max_segments <= 0) return;
   const int g = (int)gid, n = D.tcount[g];
@@ -2530,8 +2551,10 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
     freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tbound) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
   const float travel = do_particle ? freep : tbound;
-  if constexpr (kKind == CLR_KERNEL_LIGHT)
+  if constexpr (kKind == CLR_KERNEL_LIGHT) {
+    if (D->vis > 0) heat_raster(*D, free_surface_z(P, P.rin, 0.0f), L.x, L.y, L.z, L.vx, L.vy, L.vz, travel);
     light_segment(*D, rng_counter(L.r), L.x, L.y, L.z, L.vx, L.vy, L.vz, travel, L.ic);
+  }
   A.n_flight++;
   L.x += travel * L.vx; L.y += travel * L.vy; L.z += travel * L.vz;
   if constexpr (kind_diag<kKind>()) V.wp += travel;
@@ -3185,7 +3208,10 @@ namespace {
 __global__ void light_probe_kernel(const LightDiag* __restrict__ D, const float* __restrict__ data, int n, int mode) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  if (mode == 0) {
+  if (mode == 2) {  // heat-map raster of segment t, z axis to D->zmax
+    const float* q = data + 7 * (size_t)t;
+    heat_raster(*D, D->zmax, q[0], q[1], q[2], q[3], q[4], q[5], q[6]);
+  } else if (mode == 0) {
     const float* q = data + 7 * (size_t)t;
     light_segment(*D, (uint64_t)t, q[0], q[1], q[2], q[3], q[4], q[5], q[6], t & 1);
   } else {

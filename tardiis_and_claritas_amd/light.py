@@ -50,7 +50,8 @@ def resolve(config, n_rays):
     c['nx'], c['ny'], c['nz'] = int(c['nx']), int(c['ny']), int(c['nz'])
     c['max_segments_per_ray'] = int(c['max_segments_per_ray'])
     c['representative_rays'] = min(int(c['representative_rays']), c['max_rays'])
-    if c['max_rays'] < 1:
+    c['heatmap_size'] = int(c.get('heatmap_size', 0))
+    if c['max_rays'] < 1 and not c.get('heatmap_only'):
         raise ValueError('light diagnostic max_rays must be >=1')
     if min(c['nx'], c['ny'], c['nz']) < 4:
         raise ValueError('light diagnostic grid dimensions must each be >=4')
@@ -81,16 +82,25 @@ def config_struct(c, zmax) -> abi.LightConfig:
     s.nx, s.ny, s.nz = c['nx'], c['ny'], c['nz']
     s.representative_rays = int(c['representative_rays'])
     s.max_segments = int(c['max_segments_per_ray'])
+    s.heatmap_size = int(c.get('heatmap_size', 0))
     s.zmax = float(zmax)
     return s
 
 
+def heatmap_only_config(vis):
+    """Configuration of a run that keeps only the water-path heat maps
+    (simulate's heatmap_size without light diagnostics): no voxel histories."""
+    return dict(DEFAULTS, enabled=True, max_rays=0, nx=4, ny=4, nz=4, representative_rays=0,
+                max_segments_per_ray=1, heatmap_size=int(vis), heatmap_only=True)
+
+
 def probe(ctx, mode, data, light_config, rin, zmin):
     """clr_light_probe: the deposition on explicit inputs (mode 0 segments
-    [n][7], mode 1 event sites [n][3]); returns (arrays, scalars)."""
+    [n][7], mode 1 event sites [n][3], mode 2 the heat-map raster of segments
+    [n][7]); returns (arrays, scalars)."""
     lc = light_config
     data = np.ascontiguousarray(data, dtype=np.float32)
-    out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments)
+    out, arrs = abi.light_out(lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments, lc.heatmap_size)
     ctx.check(ctx.lib.clr_light_probe(ctx.h, int(mode), int(data.shape[0]), data.ctypes.data_as(abi.PF), C.byref(lc),
                                       float(rin), float(zmin), C.byref(out)), 'clr_light_probe')
     return arrs, abi.light_scalars(out)
@@ -116,14 +126,20 @@ class LightAccumulator:
                              'grid dimensions, representative rays, or segments per ray')
 
     def shape(self):
-        """(nx, ny, nz, representative rays, segments per ray) of the output arrays."""
+        """(nx, ny, nz, representative rays, segments per ray, heat-map side) of the output arrays."""
         c = self.cfg
-        return c['nx'], c['ny'], c['nz'], c['representative_rays'], c['max_segments_per_ray']
+        return c['nx'], c['ny'], c['nz'], c['representative_rays'], c['max_segments_per_ray'], c['heatmap_size']
+
+    def heatmaps(self):
+        """(x-y, z-y) water-path heat maps summed over the runs, or (None, None)."""
+        if not self.cfg['heatmap_size'] or self.sum is None:
+            return None, None
+        return self.sum['heatmap_xy'], self.sum['heatmap_zy']
 
     def run(self, n, seed, history_offset, launch):
         """clr_run_light over histories [history_offset, history_offset + n)."""
         c = self.cfg
-        out, arrs = abi.light_out(c['nx'], c['ny'], c['nz'], c['representative_rays'], c['max_segments_per_ray'])
+        out, arrs = abi.light_out(*self.shape())
         t = abi.Tallies()
         ctx = self.model.ctx
         ctx.check(ctx.lib.clr_run_light(ctx.h, int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, int(history_offset), launch,
@@ -135,7 +151,7 @@ class LightAccumulator:
         if self.sum is None:
             self.sum, self.scalars = {k: v.copy() for k, v in arrs.items()}, dict(sc)
             return
-        for k in abi.LIGHT_GRIDS:
+        for k in abi.LIGHT_GRIDS + (('heatmap_xy', 'heatmap_zy') if 'heatmap_xy' in arrs else ()):
             self.sum[k] = self.sum[k] + arrs[k]
         rows = arrs['traj_count'] > 0
         for k in abi.LIGHT_TRAJ:

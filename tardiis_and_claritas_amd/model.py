@@ -796,12 +796,18 @@ class ClaritasForwardModel:
         are not produced (collect_rays and heatmap_size must be off); the
         light-transport voxel diagnostics are (light_transport_config with
         'enabled': True; keys as CLARITAS.py:7232-7247)."""
-        if collect_rays or heatmap_size:
-            raise ValueError('per-ray outputs and heat maps are not produced by the device path')
+        if collect_rays:
+            raise ValueError('per-ray outputs are not produced by the device path')
         mat, cfg = self.prepare(material, concentration_g_per_L, weights, weight_mode)
         lacc = None
+        vis = int(heatmap_size or 0)
+        if vis < 0:
+            raise ValueError('heatmap_size must be >= 0')
         if light_transport_config and light_transport_config.get('enabled'):
-            lacc = light.LightAccumulator(self, light_transport_config, int(n_rays), int(ray_id_start))
+            lacc = light.LightAccumulator(self, dict(light_transport_config, heatmap_size=vis), int(n_rays),
+                                          int(ray_id_start))
+        elif vis > 0:  # water-path heat maps alone (CLARITAS.py:7229-7231, 9453-9454)
+            lacc = light.LightAccumulator(self, light.heatmap_only_config(vis), int(n_rays), int(ray_id_start))
         # detector-path / causal / H170 diagnostics (always on in the
         # reference; path_diagnostics=False runs the production kernel only)
         cacc = causal.CausalAccumulator(self, len(mat.diameter_m)) if path_diagnostics else None
@@ -861,8 +867,10 @@ class ClaritasForwardModel:
             group_n = np.asarray(group_n)
         res = self._result(mat, T, det_sum, batches, pbatches, group_n, requested, total, adaptive, stop_reason, ms)
         if lacc is not None:
-            res.light_transport_diagnostics = lacc.result(total)
-            light.check_closure(res.light_transport_diagnostics)
+            res.heatmap_xy, res.heatmap_zy = lacc.heatmaps()
+            if not lacc.cfg.get('heatmap_only'):
+                res.light_transport_diagnostics = lacc.result(total)
+                light.check_closure(res.light_transport_diagnostics)
         if cacc is not None:
             cond = self.particle_event_model in ('hybrid_mie_exact_rate', 'hybrid_mie_exact_rate_moment_constrained')
             closure = ('Mie extinction and terminal outcomes; conditional finite-particle angular/path law' if cond

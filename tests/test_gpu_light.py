@@ -193,3 +193,34 @@ def test_simulate_light_diagnostics(gpu_available, tmp_path):
     # the same run without diagnostics gives the same detector tallies
     r2 = m.simulate(mat, conc, n_rays=64_000, seed=seed)
     assert np.abs(r.hardware_native_direct_hits - r2.hardware_native_direct_hits).sum() <= 64
+
+
+def test_heat_maps_match_oracle(gpu_available, oracle_lib):
+    """Water-path heat maps (CLARITAS.py:3296-3309): the device raster of
+    random segments equals the oracle's up to FP32 rounding at bin faces;
+    simulate returns them alone or beside the light diagnostics, the same
+    maps either way."""
+    _flush('heat maps: probe and simulate')
+    g = np.random.Generator(np.random.PCG64(12))
+    n = 3000
+    p0 = np.stack([g.uniform(-R, R, n), g.uniform(-R, R, n), g.uniform(ZMIN, ZMAX, n)], 1)
+    v = g.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    seg = np.concatenate([p0, v, g.uniform(1e-5, 0.05, n)[:, None]], 1).astype(np.float32)
+    lc = _lc(4, 4, 4, max_rays=0)
+    lc.heatmap_size = 48
+    a, _ = light.probe(_ctx(), 2, seg, lc, R, ZMIN)
+    o, _ = oracle_lib.light_probe(2, seg, lc, R, ZMIN)
+    for k in ('heatmap_xy', 'heatmap_zy'):
+        assert abs(float(a[k].sum()) - float(o[k].sum())) <= 1e-5 * float(o[k].sum()) + 8, k
+        assert np.abs(a[k] - o[k]).sum() <= 1e-4 * float(o[k].sum()) + 16, k
+    kw, mat, conc, n_, seed = workloads.config1('hybrid_mie_exact_rate')
+    m = ClaritasForwardModel(**kw)
+    r = m.simulate(mat, conc, n_rays=50_000, seed=seed, heatmap_size=64, path_diagnostics=False)
+    assert r.heatmap_xy.shape == (64, 64) and r.heatmap_xy.sum() > 0 and r.heatmap_zy.sum() > 0
+    assert r.light_transport_diagnostics is None
+    cfgd = {'enabled': True, 'nx': 8, 'ny': 8, 'nz': 8, 'representative_rays': 0, 'max_segments_per_ray': 1}
+    r2 = m.simulate(mat, conc, n_rays=50_000, seed=seed, heatmap_size=64, light_transport_config=cfgd,
+                    path_diagnostics=False)
+    np.testing.assert_array_equal(r2.heatmap_xy, r.heatmap_xy)
+    assert r2.light_transport_diagnostics is not None

@@ -236,3 +236,66 @@ def test_config_validation():
     c = light.resolve({'enabled': True, 'representative_rays': 500}, 100)
     assert c['max_rays'] == 100 and c['representative_rays'] == 100
     assert light.memory_bytes(light.resolve({'enabled': True}, 10)) < 1e9
+
+
+# ---- water-path heat maps, CLARITAS.py:3296-3309 ----
+def _heat_lc(vis, zmax=0.074):
+    from tardiis_and_claritas_amd import abi
+    lc = abi.LightConfig()
+    lc.max_rays, lc.nx, lc.ny, lc.nz, lc.representative_rays, lc.max_segments = 0, 4, 4, 4, 0, 1
+    lc.heatmap_size, lc.zmax = vis, zmax
+    return lc
+
+
+def test_heat_raster_samples_and_bins():
+    """A segment gets ceil(dist / 10 um) samples (at most 20000) at
+    fractions k/n; each lands in bin floor((x + R) / 2R * vis) of the x-y map
+    (rows from +y down) and floor((z - zmin) / (zmax - zmin) * vis) of the
+    z-y map.  Brute force on tiny inputs, FP32 arithmetic, +-1 bin for
+    rounding at a face."""
+    Rr, zmin, zmax, vis = 0.0465, -0.093, 0.074, 16
+    g = np.random.Generator(np.random.PCG64(8))
+    n = 40
+    p0 = np.stack([g.uniform(-0.9 * Rr, 0.9 * Rr, n), g.uniform(-0.9 * Rr, 0.9 * Rr, n),
+                   g.uniform(zmin, zmax, n)], 1)
+    v = g.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    seg = np.concatenate([p0, v, g.uniform(1e-4, 3e-3, n)[:, None]], 1).astype(np.float32)
+    seg[0, 6] = 0.5  # longer than 20000 samples: capped
+    arrs, _ = po.light_probe(2, seg, _heat_lc(vis, zmax), Rr, zmin)
+    xy = np.zeros((vis, vis))
+    zy = np.zeros((vis, vis))
+    for q in seg.astype(np.float32):
+        dist = np.float32(q[6])
+        m = min(max(int(np.ceil(dist / np.float32(1e-5))), 1), 20000)
+        f = (np.arange(1, m + 1, dtype=np.float32) / np.float32(m)).astype(np.float32)
+        x, y, z = (q[i] + q[3 + i] * dist * f for i in range(3))
+        ix = ((x + np.float32(Rr)) / np.float32(2 * Rr) * vis).astype(np.float32)
+        iy = ((y + np.float32(Rr)) / np.float32(2 * Rr) * vis).astype(np.float32)
+        iz = ((z - np.float32(zmin)) / np.float32(zmax - zmin) * vis).astype(np.float32)
+        ix, iy, iz = np.trunc(ix).astype(int), vis - 1 - np.trunc(iy).astype(int), np.trunc(iz).astype(int)
+        ok = (ix >= 0) & (ix < vis) & (iy >= 0) & (iy < vis)
+        np.add.at(xy, (iy[ok], ix[ok]), 1.0)
+        ok = (iz >= 0) & (iz < vis) & (iy >= 0) & (iy < vis)
+        np.add.at(zy, (iy[ok], iz[ok]), 1.0)
+    for got, want in ((arrs['heatmap_xy'], xy), (arrs['heatmap_zy'], zy)):
+        assert abs(got.sum() - want.sum()) <= 4
+        assert np.abs(got - want).sum() <= 2e-3 * want.sum() + 8
+
+
+def test_heat_map_total_matches_track_length():
+    """Over a transport run, samples x 10 um equal the water track length in
+    the box up to the per-segment rounding (ceil) and the 20000-sample cap."""
+    from tardiis_and_claritas_amd import workloads
+    from tardiis_and_claritas_amd.light import box_zmax
+    from tardiis_and_claritas_amd.model import ClaritasForwardModel
+    kw, mat, conc, _, _ = workloads.config1('plain_mie')
+    m = ClaritasForwardModel(**kw)
+    lc = _heat_lc(32, box_zmax(m))
+    lc.max_rays, lc.nx, lc.ny, lc.nz = 10 ** 9, 8, 8, 8
+    T, a, sc = po.OracleModel(m, mat, conc).run_light(3000, 5, lc)
+    samples = float(a['heatmap_xy'].sum())
+    assert samples > 0 and float(a['heatmap_zy'].sum()) > 0.9 * samples
+    track = sc['segment_total']
+    assert track <= samples * 1e-5 + 1e-9
+    assert samples * 1e-5 - track <= sc['segment_count'] * 1e-5 + 1e-6
