@@ -107,13 +107,13 @@ class PathTally:
     # mirror channel each weighted 1/2 (or native alone), CLARITAS.py:8020-8031, 8070-8074
     def _tab(self, rec, key, K, mask, w=None):
         out = np.zeros((K, N_CH))
-        sides = ((rec['native_id'], 0.5), (rec['mirror_id'], 0.5)) if self.sym else ((rec['native_id'], 1.0),)
         key = np.asarray(key, dtype=np.int64)
-        for ids, scale in sides:
-            ok = mask & (ids >= 0) & (ids < N_CH) & (key >= 0) & (key < K)
-            idx = key[ok] * N_CH + ids[ok].astype(np.int64)
-            ww = None if w is None else np.asarray(w, dtype=np.float64)[ok]
-            out += scale * np.bincount(idx, weights=ww, minlength=K * N_CH).reshape(K, N_CH)
+        kok = mask & (key >= 0) & (key < K)
+        ww = None if w is None else np.asarray(w, dtype=np.float64)
+        # entries outside the table go to a sink bin (index K * 18), dropped
+        for ids, ok_ch, scale in self._sides:
+            idx = np.where(kok & ok_ch, key * N_CH + ids, K * N_CH)
+            out += scale * np.bincount(idx, weights=ww, minlength=K * N_CH + 1)[:K * N_CH].reshape(K, N_CH)
         return out
 
     def _add(self, name, val):
@@ -126,8 +126,12 @@ class PathTally:
         """Bin one array of clr_causal_record (numpy, abi.causal_dtype())."""
         if len(rec) == 0:
             return
-        self.n_records += len(rec)
-        n = len(rec)
+        # contiguous columns: the record array is 256 bytes per row
+        rec = {k: np.ascontiguousarray(rec[k]) for k in rec.dtype.names}
+        n = len(rec['history'])
+        self.n_records += n
+        sides = (('native_id', 0.5), ('mirror_id', 0.5)) if self.sym else (('native_id', 1.0),)
+        self._sides = [(rec[k].astype(np.int64), (rec[k] >= 0) & (rec[k] < N_CH), sc) for k, sc in sides]
         zero = np.zeros(n, dtype=np.int64)
         ones = np.ones(n, dtype=bool)
         T = lambda mask, w=None: self._tab(rec, zero, 1, mask, w)[0]
