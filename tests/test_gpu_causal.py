@@ -189,3 +189,20 @@ def test_simulate_path_diagnostics(gpu_available, builder):
     r2 = m.simulate(mat, conc, n_rays=200_000, seed=seed, path_diagnostics=False)
     assert r2.detector_particle_path_diagnostics is None
     assert np.abs(r.hardware_native_direct_hits - r2.hardware_native_direct_hits).sum() <= 200
+
+
+def test_adaptive_simulate_with_all_diagnostics(gpu_available):
+    """Adaptive stopping (batches of statistics_batch_rays) with the path
+    diagnostics and the heat maps: every batch's records and maps are
+    accumulated, and the closures hold over the whole run."""
+    _flush('adaptive simulate with path diagnostics and heat maps')
+    kw, mat, conc, n, seed = workloads.config1('hybrid_mie_exact_rate', source='beta_radiance_3d')
+    m = ClaritasForwardModel(statistics_batch_rays=50_000, **kw)
+    r = m.simulate(mat, conc, n_rays=50_000, seed=seed, target_detector_score=1e12, max_rays=150_000,
+                   heatmap_size=32)
+    assert r.n_rays == 150_000 and r.stop_reason == 'max_rays'
+    d = r.detector_particle_path_diagnostics
+    causal.check_closure(d)
+    np.testing.assert_allclose(d['hardware_symmetry_scores'], r.hardware_symmetry_scores)
+    assert r.heatmap_xy.shape == (32, 32) and r.heatmap_xy.sum() > 0
+    assert r.production_failure_diagnostics['count_closure_ok']
