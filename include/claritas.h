@@ -512,6 +512,19 @@ int clr_run_causal(clr_ctx* ctx, uint64_t n, uint64_t seed, uint64_t history_off
                    const clr_launch* launch, const clr_light_config* light, clr_tallies* out,
                    clr_light_out* light_out, clr_causal_out* causal);
 
+/* Host binning of causal records into the per-channel score tables of the
+ * detector-path / causal / H170 diagnostics (replaces the reductions of
+ * CLARITAS.py:8017-8480).  Table t of clr_causal_layout has K[t] keys x 18
+ * channels of FP64, laid out one after another; clr_causal_bin adds the n
+ * records into `tables` (the caller zeroes it once), each at its native and
+ * mirror channel with weight 1/2 (symmetric != 0) or at the native channel
+ * with weight 1.  r_edges and z_edges are the 6 radial and axial edges of
+ * the H170 collision maps.  clr_causal_layout returns the number of tables
+ * and, for 0 <= index < that number, the table's name and K. */
+int clr_causal_layout(int n_bins, int index, char* name, int name_cap, int* K);
+int clr_causal_bin(const clr_causal_record* records, uint64_t n, int n_bins, int symmetric, const double* r_edges,
+                   const double* z_edges, double* tables);
+
 int clr_light_probe(clr_ctx* ctx, int mode, int n, const float* data, const clr_light_config* light, double rin,
                     double zmin, clr_light_out* light_out);
 

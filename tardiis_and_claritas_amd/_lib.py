@@ -37,6 +37,9 @@ _SIGS = {
                                 P(abi.Tallies), P(abi.LightOut)]),
     'clr_run_causal': (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, P(abi.Launch), P(abi.LightConfig),
                                  P(abi.Tallies), P(abi.LightOut), P(abi.CausalOut)]),
+    'clr_causal_layout': (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_int, P(C.c_int)]),
+    'clr_causal_bin': (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                                 P(C.c_double)]),
     'clr_light_probe': (C.c_int, [C.c_void_p, C.c_int, C.c_int, P(C.c_float), P(abi.LightConfig), C.c_double,
                                   C.c_double, P(abi.LightOut)]),
     'clr_meridional_solve': (C.c_int, [C.c_void_p, P(abi.Config), P(abi.Particles), P(abi.Sediment),

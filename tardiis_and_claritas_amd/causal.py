@@ -92,11 +92,22 @@ def spatial_edges(model):
     return r, z
 
 
+# tables of the channel-17 (H170) histograms: their shape after the channel is taken
+def _h170_shapes(nb):
+    return {'h170_cumulative': (13,), 'h170_first_map': (25,), 'h170_last_map': (25,), 'h170_max_map': (25,),
+            'h170_type_by_macro': (8, 5), 'h170_ld': (12,), 'h170_incidence': (7,), 'h170_exit': (7,),
+            'h170_type_by_incidence': (8, 7), 'h170_type_by_exit': (8, 7), 'h170_psd_by_incidence': (nb, 7),
+            'h170_psd_by_exit': (nb, 7), 'h170_type_by_psd': (8, nb), 'h170_first_angle': (6,)}
+
+
 class PathTally:
     """Score tables over detected histories (the accumulators of
-    CLARITAS.py:7352-7600), fed one record array at a time."""
+    CLARITAS.py:7352-7600), fed one record array at a time.  engine
+    'native' bins in one pass in the library (clr_causal_bin); 'numpy' is
+    the same reduction written with numpy masks and bincounts."""
 
-    def __init__(self, n_bins, symmetric, r_edges, z_edges):
+    def __init__(self, n_bins, symmetric, r_edges, z_edges, engine='native'):
+        self.engine = engine
         self.nb = int(n_bins)
         self.sym = bool(symmetric)
         self.r_edges, self.z_edges = np.asarray(r_edges, float), np.asarray(z_edges, float)
@@ -126,6 +137,55 @@ class PathTally:
         """Bin one array of clr_causal_record (numpy, abi.causal_dtype())."""
         if len(rec) == 0:
             return
+        if self.engine == 'native':
+            self._add_native(rec)
+        else:
+            self._add_numpy(rec)
+
+    def _layout(self, lib):
+        if getattr(self, '_lay', None) is None:
+            n = lib.clr_causal_layout(self.nb, -1, None, 0, None)
+            lay, off = [], 0
+            buf = C.create_string_buffer(128)
+            for i in range(n):
+                K = C.c_int()
+                lib.clr_causal_layout(self.nb, i, buf, 128, C.byref(K))
+                lay.append((buf.value.decode(), K.value, off))
+                off += K.value * N_CH
+            self._lay, self._lay_total = lay, off
+        return self._lay, self._lay_total
+
+    def _add_native(self, rec):
+        from . import _lib
+        lib = _lib.load()
+        lay, total = self._layout(lib)
+        rec = np.ascontiguousarray(rec)
+        tab = np.zeros(total)
+        re_, ze_ = np.ascontiguousarray(self.r_edges, float), np.ascontiguousarray(self.z_edges, float)
+        pd = C.POINTER(C.c_double)
+        rc = lib.clr_causal_bin(C.c_void_p(rec.ctypes.data), len(rec), self.nb, int(self.sym), re_.ctypes.data_as(pd),
+                                ze_.ctypes.data_as(pd), tab.ctypes.data_as(pd))
+        if rc:
+            raise RuntimeError(f'clr_causal_bin failed with code {rc}')
+        self.n_records += len(rec)
+        h170 = _h170_shapes(self.nb)
+        last = None
+        for name, K, off in lay:
+            a = tab[off:off + K * N_CH].reshape(K, N_CH)
+            if name in h170:
+                self._add(name, a[:, H170].reshape(h170[name]).copy())
+            elif name == 'h170_last_angle_type':
+                last = a[:, H170].reshape(8, 6).copy() if last is None else last + a[:, H170].reshape(8, 6)
+            elif name == 'h170_last_angle_all':
+                row = np.zeros((8, 6))
+                row[0] = a[:, H170]
+                last = row if last is None else last + row
+            else:
+                per_bin = name.startswith('bin:') or name == 'max_bin'  # (n_bins, 18) even for one bin
+                self._add(name, a[0].copy() if K == 1 and not per_bin else a.copy())
+        self._add('h170_last_angle', last)
+
+    def _add_numpy(self, rec):
         # contiguous columns: the record array is 256 bytes per row
         rec = {k: np.ascontiguousarray(rec[k]) for k in rec.dtype.names}
         n = len(rec['history'])

@@ -171,3 +171,26 @@ def test_causal_run_keeps_the_tallies():
     Tb, _ = om.run(10_000, 3, 0)
     for k in ('det', 'status', 'counters'):
         np.testing.assert_array_equal(Ta[k], Tb[k])
+
+
+@pytest.mark.parametrize('case', ['plain', 'hybrid', 'rough'])
+def test_native_binning_equals_numpy(case):
+    """clr_causal_bin (one pass in the library) equals the numpy reduction on
+    the same records, table by table (only summation order differs)."""
+    if case == 'plain':
+        kw, mat, conc, _, _ = workloads.config1('plain_mie')
+    elif case == 'hybrid':
+        kw, mat, conc, _, _ = workloads.config1('hybrid_mie_exact_rate', source='beta_radiance_3d')
+    else:
+        kw, mat, conc, _, _ = workloads.config1_rough()
+    m, T, rec, nb = _run(kw, mat, conc, n=20_000)
+    r, z = causal.spatial_edges(m)
+    for sym in (True, False):
+        A = causal.PathTally(nb, sym, r, z, engine='native')
+        B = causal.PathTally(nb, sym, r, z, engine='numpy')
+        A.add(rec[:5000]); A.add(rec[5000:])
+        B.add(rec[:5000]); B.add(rec[5000:])
+        assert set(A.t) == set(B.t), set(A.t) ^ set(B.t)
+        for k in B.t:
+            assert A.t[k].shape == B.t[k].shape, k
+            np.testing.assert_allclose(A.t[k], B.t[k], rtol=1e-12, atol=1e-12, err_msg=k)
