@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 8
+#define CLR_ABI_VERSION 9
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -405,6 +405,16 @@ typedef struct clr_causal_record {
   float max_internal_path, max_path_over_diameter, max_incidence, max_exit;  /* largest turn (m, -, deg, deg) */
 } clr_causal_record;
 
+/* One failed history (status 4) of a causal run: the reference's failure
+ * sample fields (CLARITAS.py:8505-8640) without the particle drill-down. */
+typedef struct clr_failure_sample {
+  uint64_t history;
+  int32_t code, subcode;        /* production failure code and subcode */
+  int32_t trace_iteration;      /* water-loop iterations completed */
+  int32_t pad_;
+  float x, y, z, vx, vy, vz;    /* photon state at the failure */
+} clr_failure_sample;
+
 /* Caller-owned record buffer of clr_run_causal.  The call traces the
  * histories [range_start, range_start + range_count) of an n-history run
  * (range_count 0: all n), with the run's jackknife groups, so that the
@@ -416,6 +426,9 @@ typedef struct clr_causal_out {
   uint64_t capacity;
   uint64_t count;
   uint64_t range_start, range_count;
+  clr_failure_sample* failures;  /* optional: the first failed histories appended (any order) */
+  uint64_t failure_capacity;
+  uint64_t failure_count;        /* failed histories in the range */
 } clr_causal_out;
 
 struct clr_ctx;
@@ -425,7 +438,7 @@ typedef struct clr_ctx clr_ctx;
  * launch-arity check validate_claritas.py:972-1001). id: 0 config,
  * 1 particles, 2 sediment, 3 mie_out, 4 tables_out, 5 tallies, 6 record,
  * 7 batch_item, 8 launch, 9 meridional, 10 meridional_out, 11 light_config,
- * 12 light_out, 13 causal_record. Returns 0 for an unknown id. */
+ * 12 light_out, 13 causal_record, 14 failure_sample. Returns 0 for an unknown id. */
 int clr_abi_version(void);
 size_t clr_sizeof(int struct_id);
 

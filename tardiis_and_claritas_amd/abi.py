@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -300,15 +300,28 @@ class CausalRecord(C.Structure):
         [(k, C.c_float) for k in CAUSAL_FLOAT_FIELDS]
 
 
+class FailureSample(C.Structure):
+    _fields_ = [('history', C.c_uint64), ('code', C.c_int32), ('subcode', C.c_int32),
+                ('trace_iteration', C.c_int32), ('pad_', C.c_int32)] + [(k, C.c_float) for k in
+                                                                       ('x', 'y', 'z', 'vx', 'vy', 'vz')]
+
+
+def failure_dtype():
+    import numpy as np
+    return np.dtype([('history', '<u8'), ('code', '<i4'), ('subcode', '<i4'), ('trace_iteration', '<i4'),
+                     ('pad_', '<i4')] + [(k, '<f4') for k in ('x', 'y', 'z', 'vx', 'vy', 'vz')])
+
+
 class CausalOut(C.Structure):
     _fields_ = [('records', C.c_void_p), ('capacity', C.c_uint64), ('count', C.c_uint64),
-                ('range_start', C.c_uint64), ('range_count', C.c_uint64)]
+                ('range_start', C.c_uint64), ('range_count', C.c_uint64), ('failures', C.c_void_p),
+                ('failure_capacity', C.c_uint64), ('failure_count', C.c_uint64)]
 
 
 STRUCT_IDS = {
     0: Config, 1: Particles, 2: Sediment, 3: MieOut, 4: TablesOut, 5: Tallies, 6: Record,
     7: BatchItem, 8: Launch, 9: Meridional, 10: MeridionalOut, 11: LightConfig, 12: LightOut,
-    13: CausalRecord,
+    13: CausalRecord, 14: FailureSample,
 }
 
 

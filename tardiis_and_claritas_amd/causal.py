@@ -599,6 +599,9 @@ class CausalAccumulator:
         r, z = spatial_edges(model)
         self.tally = PathTally(n_bins, model.symmetry_average_exact, r, z)
         self.count = 0
+        self.sample_limit = 512
+        self.failures = []  # the first failed histories' samples (numpy rows)
+        self.failure_count = 0
 
     def run(self, n, seed, history_offset, launch, light_acc=None):
         """clr_run_causal over histories [history_offset, history_offset + n):
@@ -613,6 +616,9 @@ class CausalAccumulator:
             co.records = recs.ctypes.data
             co.capacity = rn
             co.range_start, co.range_count = r0, rn
+            room = max(self.sample_limit - sum(len(f) for f in self.failures), 0)
+            fails = np.zeros(max(room, 1), dtype=abi.failure_dtype())
+            co.failures, co.failure_capacity = fails.ctypes.data, room
             t = abi.Tallies()
             if light_acc is not None:
                 out, arrs = abi.light_out(*light_acc.shape())
@@ -627,5 +633,9 @@ class CausalAccumulator:
                 raise RuntimeError('causal record buffer overflow')
             self.tally.add(recs[:int(co.count)])
             self.count += int(co.count)
+            self.failure_count += int(co.failure_count)
+            if room:
+                f = fails[:min(int(co.failure_count), room)]
+                self.failures.append(np.sort(f, order='history'))
             total = sum_tallies(total, abi.tallies_to_numpy(t))
         return total

@@ -112,6 +112,9 @@ struct clr_ctx {
   clr_causal_record* causal_rec = nullptr;
   unsigned long long* causal_count = nullptr;
   uint64_t causal_cap = 0;
+  clr_failure_sample* fail_rec = nullptr;
+  unsigned long long* fail_count = nullptr;
+  uint64_t fail_cap = 0;
 };
 
 namespace {
@@ -595,6 +598,9 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   L.causal = c->causal_rec;
   L.causal_count = c->causal_count;
   L.causal_cap = c->causal_cap;
+  L.fail_rec = c->fail_rec;
+  L.fail_count = c->fail_count;
+  L.fail_cap = c->fail_cap;
   const size_t words = sizeof(clr_tallies) * n_slots / 8;
   if (use_graph) {
     GraphCache& G = c->graph;
@@ -823,6 +829,7 @@ size_t clr_sizeof(int id) {
     case 11: return sizeof(clr_light_config);
     case 12: return sizeof(clr_light_out);
     case 13: return sizeof(clr_causal_record);
+    case 14: return sizeof(clr_failure_sample);
     default: return 0;
   }
 }
@@ -1037,14 +1044,20 @@ int clr_run_causal(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const
   if (co->capacity > (1ull << 40)) return fail(c, CLR_E_RANGE, "causal capacity too large");
   if (c->slots.empty() || !c->slots[0]->ready) return fail(c, CLR_E_STATE, "clr_prepare(slot 0) has not run");
   HIPCHK(c, hipSetDevice(c->device));
-  // record area followed by the 8-byte append counter
+  if (co->failure_capacity && !co->failures) return fail(c, CLR_E_ARG, "failure samples buffer is NULL");
+  if (co->failure_capacity > (1ull << 24)) return fail(c, CLR_E_RANGE, "failure sample capacity too large");
+  // record area, failure samples, then the two 8-byte append counters
   const size_t rec_bytes = sizeof(clr_causal_record) * (size_t)co->capacity;
-  HIPCHK(c, c->causal.ensure(rec_bytes + 256));
+  const size_t fs_bytes = sizeof(clr_failure_sample) * (size_t)co->failure_capacity;
+  HIPCHK(c, c->causal.ensure(rec_bytes + fs_bytes + 256));
   char* base = static_cast<char*>(c->causal.p);
   c->causal_rec = reinterpret_cast<clr_causal_record*>(base);
-  c->causal_count = reinterpret_cast<unsigned long long*>(base + rec_bytes);
+  c->fail_rec = reinterpret_cast<clr_failure_sample*>(base + rec_bytes);
+  c->causal_count = reinterpret_cast<unsigned long long*>(base + rec_bytes + fs_bytes);
+  c->fail_count = c->causal_count + 1;
   c->causal_cap = co->capacity;
-  HIPCHK(c, hipMemsetAsync(c->causal_count, 0, sizeof(unsigned long long), c->stream));
+  c->fail_cap = co->failure_capacity;
+  HIPCHK(c, hipMemsetAsync(c->causal_count, 0, 2 * sizeof(unsigned long long), c->stream));
   LightDiag D{};
   int e = 0;
   if (lc) {
@@ -1059,10 +1072,15 @@ int clr_run_causal(clr_ctx* c, uint64_t n, uint64_t seed, uint64_t offset, const
                    co->range_count);
   }
   c->causal_rec = nullptr; c->causal_count = nullptr; c->causal_cap = 0;
+  c->fail_rec = nullptr; c->fail_count = nullptr; c->fail_cap = 0;
   if (e) return e;
-  unsigned long long cnt = 0;
-  HIPCHK(c, hipMemcpy(&cnt, base + rec_bytes, sizeof(cnt), hipMemcpyDeviceToHost));
+  unsigned long long cnts[2] = {0, 0};
+  HIPCHK(c, hipMemcpy(cnts, base + rec_bytes + fs_bytes, sizeof(cnts), hipMemcpyDeviceToHost));
+  const unsigned long long cnt = cnts[0];
   co->count = cnt;
+  co->failure_count = cnts[1];
+  const uint64_t fw = std::min<uint64_t>(cnts[1], co->failure_capacity);
+  if (fw) HIPCHK(c, hipMemcpy(co->failures, base + rec_bytes, sizeof(clr_failure_sample) * fw, hipMemcpyDeviceToHost));
   const uint64_t w = std::min<uint64_t>(cnt, co->capacity);
   if (w) HIPCHK(c, hipMemcpy(co->records, base, sizeof(clr_causal_record) * w, hipMemcpyDeviceToHost));
   return 0;

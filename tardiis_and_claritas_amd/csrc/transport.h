@@ -107,6 +107,10 @@ struct TransportLaunch {
   clr_causal_record* causal;
   unsigned long long* causal_count;
   uint64_t causal_cap;
+  // failed-history samples (CAUSAL/LIGHT with a causal target)
+  clr_failure_sample* fail_rec;
+  unsigned long long* fail_count;
+  uint64_t fail_cap;
 };
 
 // Kernel variants: PLAIN when every configuration uses the plain-Mie law and a

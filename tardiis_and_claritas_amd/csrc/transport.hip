@@ -2908,6 +2908,17 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
     }
     lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
     if constexpr (kind_diag<kKind>()) {
+      // a failed history's sample (rare: one atomic per failure)
+      if (L.st == 4 && Lp.fail_count != nullptr) {
+        const unsigned long long k = atomicAdd(Lp.fail_count, 1ull);
+        if (k < Lp.fail_cap) {
+          clr_failure_sample f;
+          f.history = rng_counter(L.r); f.code = L.pf & 255; f.subcode = L.pf >> 8;
+          f.trace_iteration = L.wstep; f.pad_ = 0;
+          f.x = L.x; f.y = L.y; f.z = L.z; f.vx = L.vx; f.vy = L.vy; f.vz = L.vz;
+          Lp.fail_rec[k] = f;
+        }
+      }
       // causal record of a detected history: one append per wave
       const bool want = Lp.causal != nullptr && (nat >= 0 || mir >= 0);
       const uint64_t m = __ballot(want);

@@ -267,10 +267,25 @@ NONFINITE_CODES = (11, 12)
 FAILURE_SAMPLE_LIMIT = 512
 
 
+def _failure_samples(cacc):
+    """The first failed histories of a causal run as the reference's sample
+    dicts (CLARITAS.py:8505-8640), without the particle drill-down fields."""
+    out = []
+    for f in cacc.failures:
+        for q in f:
+            code, sub = int(q['code']), int(q['subcode'])
+            out.append({'ray_id': int(q['history']), 'failure_code': code,
+                        'failure_name': abi.PRODUCTION_FAILURE_NAMES.get(code, f'CODE_{code}'),
+                        'failure_subcode': sub, 'trace_iteration': int(q['trace_iteration']),
+                        'x': float(q['x']), 'y': float(q['y']), 'z': float(q['z']),
+                        'vx': float(q['vx']), 'vy': float(q['vy']), 'vz': float(q['vz'])})
+    return out[:FAILURE_SAMPLE_LIMIT]
+
+
 def _failure_summary(T, fails, total):
     """Subcode counts, code-group counts and the production-failure
-    dictionary of CLARITAS.py:9608-9655.  Per-ray failure samples with the
-    particle drill-down are not recorded (the list is empty), and the
+    dictionary of CLARITAS.py:9608-9655.  The failure samples are filled by
+    simulate's causal run (without the particle drill-down fields); the
     particle proposal / accessibility counts are not tallied (None)."""
     sub = T['failure_subcodes']
     sub_counts = {str(abi.SUBCODE_VALUES[k]): int(sub[k]) for k in range(len(abi.SUBCODE_VALUES)) if sub[k]}
@@ -882,6 +897,7 @@ class ClaritasForwardModel:
                           inaccessible_trials=int(cnt[abi.C_IDX['cond_inaccessible']]))
             res.detector_particle_path_diagnostics = causal.diagnostics_dict(
                 cacc.tally, self._sym(det_sum), closure, counts, CONDITIONAL_MAX_TRIALS)
+            res.production_failure_diagnostics['samples'] = _failure_samples(cacc)
         return res
 
     @staticmethod

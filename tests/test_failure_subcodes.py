@@ -64,3 +64,36 @@ def test_population_balance_is_the_unaggregated_psd():
         ClaritasForwardModel(aggregation_model='orthokinetic_compact')
     with pytest.raises(NotImplementedError):
         ClaritasForwardModel(aggregation_collision_exposure=0.5)
+
+
+@pytest.mark.gpu
+def test_failure_samples(gpu_available, oracle_lib):
+    """Failed-history samples of the causal run: with room for all of them,
+    their codes and subcodes histogram to the tallies' failure and subcode
+    counts, and each sampled history failed with the same code in the
+    oracle's trace of the same history (up to < 0.1 % of histories)."""
+    import ctypes as C
+    kw, mat, conc = _forced()
+    m = ClaritasForwardModel(**kw)
+    m.prepare(mat, conc)
+    N = 20_000
+    recs = np.zeros(N, dtype=abi.causal_dtype())
+    fails = np.zeros(N, dtype=abi.failure_dtype())
+    co = abi.CausalOut()
+    co.records, co.capacity, co.failures, co.failure_capacity = recs.ctypes.data, N, fails.ctypes.data, N
+    t = abi.Tallies()
+    m.ctx.check(m.ctx.lib.clr_run_causal(m.ctx.h, N, 3, 0, m._launch(), None, C.byref(t), None, C.byref(co)),
+                'clr_run_causal')
+    T = abi.tallies_to_numpy(t)
+    f = fails[:co.failure_count]
+    assert co.failure_count == T['status'][4] > 0
+    np.testing.assert_array_equal(np.bincount(f['code'], minlength=abi.N_FAILURE), T['failures'])
+    slots = np.array([SLOT.get(int(s), abi.N_SUBCODE - 1) for s in f['subcode']])
+    np.testing.assert_array_equal(np.bincount(slots, minlength=abi.N_SUBCODE), T['failure_subcodes'])
+    To, ro = oracle_lib.OracleModel(m, mat, conc).run(N, 3, 0, n_records=N)
+    o_code = ro['failure'][f['history'].astype(np.int64)]
+    o_st = ro['status'][f['history'].astype(np.int64)]
+    assert np.mean((o_st == 4) & (o_code == f['code'])) >= 0.999
+    r = m.simulate(mat, conc, n_rays=N, seed=3)
+    s = r.production_failure_diagnostics['samples']
+    assert 0 < len(s) <= 512 and all(q['failure_code'] > 0 for q in s)
