@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 9
+#define CLR_ABI_VERSION 10
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -540,6 +540,55 @@ int clr_causal_bin(const clr_causal_record* records, uint64_t n, int n_bins, int
 
 int clr_light_probe(clr_ctx* ctx, int mode, int n, const float* data, const clr_light_config* light, double rin,
                     double zmin, clr_light_out* light_out);
+
+/* ---- device probes of the transport's building blocks on explicit
+ * inputs.  Replaces the reference's test kernels
+ * free_surface_geometry_test_kernel (CLARITAS.py:4438-4455),
+ * sediment_field_test_kernel (CLARITAS.py:4228-4235), bore_wall_test_kernel
+ * (CLARITAS.py:2952-2956) and water_surface_offset_test_kernel
+ * (CLARITAS.py:2980-2985), and exposes the samplers and the detector
+ * acceptance of trace_kernel for statistical and closed-form tests.  Item i
+ * reads in[i*IN ..] and writes out[i*OUT ..] (IN floats, OUT doubles, see
+ * clr_probe_shape); random draws come from the Philox stream (seed,
+ * history i), exactly as a history's first draws.  Every mode uses the
+ * prepared configuration `slot`.
+ *   SURFACE     in x y z vx vy vz -> surface z at (x,y), hit distance (>= 3e38: none),
+ *                                     surface z at the hit, outward normal (3)
+ *   WALL        in x y z vx vy vz -> water-side exit through r = rin on [zmin, wall top],
+ *                                     forward hit of r = rout on [bottom of the disc, wall top],
+ *                                     next acrylic surface distance and its code (1 inner wall,
+ *                                     2 outer wall, 3 disc bottom, 4 wall top, 5 water base);
+ *                                     1e300: no hit, -1: start outside the water
+ *   OFFSET      in x y z nx ny nz    -> accepted (1/0), offset position (3)
+ *   SEDIMENT    in bin x y z         -> local density scale of the bin (FP64 path of
+ *                                     CLARITAS.py:1537-1609), FP32 shared-field scale
+ *   DETECTOR    in x y z vx vy vz    -> hardware bore of the exit ray, native and mirror (-1 none)
+ *   SAMPLER     in kind a b          -> draw, words used; kind 0 gamma(a), 1 beta(a, b),
+ *                                     2 Beta-radiance polar angle (a, b), 3 standard normal
+ *                                     (CLARITAS.py:1149-1212)
+ *   SOURCE      in -                 -> launch position and direction before the collimator
+ *   MIE_THETA   in bin               -> polar scattering angle sampled from the bin's table
+ *                                     (CLARITAS.py:2632-2641)
+ *   CONDITIONAL in bin x y z vx vy vz -> status (1 scattered, 2 absorbed, other: failed),
+ *                                     channel, trials, outcome rejections, inaccessible
+ *                                     placements, failure code, direction (3) of the
+ *                                     conditional closure (CLARITAS.py:4479-4536) ---- */
+#define CLR_PROBE_SURFACE 0
+#define CLR_PROBE_WALL 1
+#define CLR_PROBE_OFFSET 2
+#define CLR_PROBE_SEDIMENT 3
+#define CLR_PROBE_DETECTOR 4
+#define CLR_PROBE_SAMPLER 5
+#define CLR_PROBE_SOURCE 6
+#define CLR_PROBE_MIE_THETA 7
+#define CLR_PROBE_CONDITIONAL 8
+#define CLR_PROBE_N_MODES 9
+/* Input floats and output doubles per item of a probe mode; returns 0, or
+ * CLR_E_ARG for an unknown mode. */
+int clr_probe_shape(int mode, int* in_stride, int* out_stride);
+/* Run n items of probe `mode` on slot `slot` (host arrays in[n*IN],
+ * out[n*OUT]); at most 2^24 items per call. */
+int clr_probe(clr_ctx* ctx, int slot, int mode, int n, uint64_t seed, const float* in, double* out);
 
 /* Time one launch shape: trace n histories of slot 0 and return device
  * milliseconds in *ms (HIP events on the library stream). */

@@ -2896,17 +2896,12 @@ static void light_absorb(const light_t* L, uint64_t gid, float x, float y, float
  * 3 internal reflection, 2 transmission (+4 for the morphology channel) */
 static int light_code(int per, int pir, int morph) { return (per ? 1 : (pir ? 3 : 2)) + (morph ? 4 : 0); }
 
-static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist_t* o, const light_t* Ld) {
-  const fparams* P = &m->fp;
-  rng_t r;
-  rng_seed(&r, seed, hist);
-  memset(o, 0, sizeof(*o));
-  prov_reset(o);
-  o->native = o->mirror = -1;
-  for (int a = 0; a < CLR_MAX_AUX; ++a) o->aux_native[a] = o->aux_mirror[a] = -1;
-  photon_t* p = &o->ph;
-  int st = 0, pf = PF_OK, pfs = SUB_NONE;
-  /* ---- launch, CLARITAS.py:4650-4697 ---- */
+/* source position on the launch plane y = -source_r and direction before
+ * the collimator, CLARITAS.py:4650-4680 (the collimated disk has no
+ * reference counterpart) */
+static void source_sample(const fparams* P, rng_t* rp, float* xo, float* yo, float* zo, float* vxo, float* vyo,
+                          float* vzo) {
+  rng_t r = *rp;
   float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
   if (P->src_model == CLR_SOURCE_COLLIMATED_DISK) {
     float u1 = U(&r), u2 = U(&r);
@@ -2931,6 +2926,23 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
     vx = sinf(h); vy = cosf(h); vz = 0.0f;
   }
   normalize3(&vx, &vy, &vz);
+  *rp = r;
+  *xo = x; *yo = y; *zo = z; *vxo = vx; *vyo = vy; *vzo = vz;
+}
+
+static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist_t* o, const light_t* Ld) {
+  const fparams* P = &m->fp;
+  rng_t r;
+  rng_seed(&r, seed, hist);
+  memset(o, 0, sizeof(*o));
+  prov_reset(o);
+  o->native = o->mirror = -1;
+  for (int a = 0; a < CLR_MAX_AUX; ++a) o->aux_native[a] = o->aux_mirror[a] = -1;
+  photon_t* p = &o->ph;
+  int st = 0, pf = PF_OK, pfs = SUB_NONE;
+  /* ---- launch, CLARITAS.py:4650-4697 ---- */
+  float x, y, z, vx, vy, vz;
+  source_sample(P, &r, &x, &y, &z, &vx, &vy, &vz);
   p->x = x; p->y = y; p->z = z; p->vx = vx; p->vy = vy; p->vz = vz;
   if (vy <= 1e-12f) { st = 3; goto WRITE_OUT; }
   if (P->src_bore) {
@@ -3782,6 +3794,100 @@ int orc_walk_probe(int n, uint64_t seed, float alpha, const float* b, float n_to
     top[i] = -1; fc[i] = 0;
     status[i] = micro_interface(&r, alpha, bb[0], bb[1], bb[2], n_top, k_top, n_bot, k_bot, &v[0], &v[1], &v[2],
                                 &top[i], &fc[i]);
+  }
+  return 0;
+}
+
+/* ======================================================================
+ * Probes of the building blocks on explicit inputs (clr_probe modes)
+ * ====================================================================== */
+int orc_probe_shape(int mode, int* in_stride, int* out_stride) {
+  static const int kIn[CLR_PROBE_N_MODES] = {6, 6, 6, 4, 6, 3, 1, 1, 7};
+  static const int kOut[CLR_PROBE_N_MODES] = {6, 4, 4, 2, 2, 2, 6, 1, 9};
+  if (mode < 0 || mode >= CLR_PROBE_N_MODES) return CLR_E_ARG;
+  *in_stride = kIn[mode];
+  *out_stride = kOut[mode];
+  return 0;
+}
+
+int orc_probe(const orc_model* m, int mode, int n, uint64_t seed, const float* in, double* out) {
+  int si, so;
+  if (!m || orc_probe_shape(mode, &si, &so)) return CLR_E_ARG;
+  const fparams* P = &m->fp;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const float* q = in + (size_t)i * si;
+    double* o = out + (size_t)i * so;
+    rng_t r;
+    rng_seed(&r, seed, (uint64_t)i);
+    switch (mode) {
+      case CLR_PROBE_SURFACE: {
+        o[0] = free_surface_z(P, q[0], q[1]);
+        float t = free_surface_hit(P, q[0], q[1], q[2], q[3], q[4], q[5]), nx = 0.0f, ny = 0.0f, nz = 0.0f,
+              zh = INF_F;
+        o[1] = t;
+        if (t < 0.5f * INF_F) {
+          float hx = q[0] + t * q[3], hy = q[1] + t * q[4];
+          zh = free_surface_z(P, hx, hy);
+          free_surface_normal(P, hx, hy, &nx, &ny, &nz);
+        }
+        o[2] = zh; o[3] = nx; o[4] = ny; o[5] = nz;
+        break;
+      }
+      case CLR_PROBE_WALL: {
+        o[0] = bore_wall_hit_d(q[0], q[1], q[2], q[3], q[4], q[5], P->rin, P->zmin, P->zwall_top);
+        o[1] = finite_cylinder_forward_d(q[0], q[1], q[2], q[3], q[4], q[5], P->rout, P->zlow, P->zwall_top);
+        double ta = 0.0;
+        int surf = acrylic_next_surface(P, q[0], q[1], q[2], q[3], q[4], q[5], &ta);
+        o[2] = ta; o[3] = surf;
+        break;
+      }
+      case CLR_PROBE_OFFSET: {
+        float x = q[0], y = q[1], z = q[2];
+        int ok = water_surface_offset(P, &x, &y, &z, q[3], q[4], q[5]);
+        o[0] = ok; o[1] = x; o[2] = y; o[3] = z;
+        break;
+      }
+      case CLR_PROBE_SEDIMENT: {
+        int b = (int)q[0];
+        o[0] = (b >= 0 && b < m->n) ? sediment_local_scale(m, b, q[1], q[2], q[3]) : -1.0;
+        o[1] = (P->sed_model == CLR_SED_RZ_FIELD || P->sed_model == CLR_SED_XYZ_FIELD)
+                   ? shared_field_scale(m, q[1], q[2], q[3]) : 1.0;
+        break;
+      }
+      case CLR_PROBE_DETECTOR:
+        o[0] = hardware_detector(m, q[0], q[1], q[2], q[3], q[4], q[5], 0);
+        o[1] = hardware_detector(m, q[0], q[1], q[2], q[3], q[4], q[5], 1);
+        break;
+      case CLR_PROBE_SAMPLER: {
+        int k = (int)q[0];
+        o[0] = k == 0 ? gamma_draw(&r, q[1]) : k == 1 ? beta_draw(&r, q[1], q[2])
+                                           : k == 2 ? beta_radiance_polar(&r, q[1], q[2]) : normal_bm(&r);
+        o[1] = r.draws;
+        break;
+      }
+      case CLR_PROBE_SOURCE: {
+        float x, y, z, vx, vy, vz;
+        source_sample(P, &r, &x, &y, &z, &vx, &vy, &vz);
+        o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+        break;
+      }
+      case CLR_PROBE_MIE_THETA: {
+        int b = (int)q[0];
+        o[0] = (b >= 0 && b < m->n) ? sample_mie_theta(&r, m, b) : -1.0;
+        break;
+      }
+      case CLR_PROBE_CONDITIONAL: {
+        int b = (int)q[0];
+        if (b < 0 || b >= m->n) { o[0] = -1.0; break; }
+        cond_t c = conditional_event(m, &r, b, q[1], q[2], q[3], q[4], q[5], q[6], P->cond_trials);
+        o[0] = c.status; o[1] = c.channel; o[2] = c.trials; o[3] = c.rejects; o[4] = c.inaccessible;
+        o[5] = c.failure; o[6] = c.vx; o[7] = c.vy; o[8] = c.vz;
+        break;
+      }
+      default:
+        break;
+    }
   }
   return 0;
 }

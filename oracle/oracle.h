@@ -75,6 +75,12 @@ int orc_free_surface_probe(const clr_config* cfg, const float* rays6, int n,
 /* Local sediment density scale of bin `bin` at n points (CLARITAS.py:4228-4235). */
 int orc_sediment_probe(const orc_model* m, int bin, const float* xyz, int n, double* out);
 
+/* The probe modes of clr_probe (include/claritas.h: CLR_PROBE_*, item
+ * layouts from clr_probe_shape's table, restated here) on a prepared model;
+ * the detector acceptance is the reference's full 18-bore scan. */
+int orc_probe_shape(int mode, int* in_stride, int* out_stride);
+int orc_probe(const orc_model* m, int mode, int n, uint64_t seed, const float* in, double* out);
+
 /* closed-form probes: unpolarised Fresnel R, complex-index interface,
  * Snell refraction, water-side wall exit distance */
 int orc_fresnel_probe(int n, const float* ci, float n1, float n2, float* R, int* tir);

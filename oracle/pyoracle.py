@@ -87,6 +87,10 @@ def load():
     lib.orc_bore_probe.argtypes = [P(abi.Config), C.c_int, C.c_uint64, P(C.c_float), C.c_float, C.c_float,
                                    P(C.c_int), P(C.c_int), P(C.c_int)]
     lib.orc_bore_probe.restype = C.c_int
+    lib.orc_probe_shape.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
+    lib.orc_probe_shape.restype = C.c_int
+    lib.orc_probe.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, P(C.c_float), P(C.c_double)]
+    lib.orc_probe.restype = C.c_int
     lib.orc_sfh213.argtypes = [C.c_float]
     lib.orc_sfh213.restype = C.c_float
     _lib = lib
@@ -260,6 +264,21 @@ class OracleModel:
         t = abi.TablesOut()
         t.sed_radial_log = out.ctypes.data_as(PD)
         self.lib.orc_get_tables(self.h, C.byref(t))
+        return out
+
+    def probe(self, mode, items, seed=0):
+        """orc_probe `mode` (abi.PROBE_MODES name or code) on items [n][IN];
+        returns [n][OUT] doubles (clr_probe's layouts)."""
+        code = abi.PROBE_MODES[mode] if isinstance(mode, str) else int(mode)
+        si, so = C.c_int(), C.c_int()
+        if self.lib.orc_probe_shape(code, C.byref(si), C.byref(so)):
+            raise ValueError(f'unknown probe mode {mode!r}')
+        a = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, si.value)
+        out = np.zeros((a.shape[0], so.value), dtype=np.float64)
+        rc = self.lib.orc_probe(self.h, code, a.shape[0], int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                a.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc:
+            raise RuntimeError(f'oracle probe failed with code {rc}')
         return out
 
     def set_phase_table(self, theta, cdf):

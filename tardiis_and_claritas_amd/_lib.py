@@ -44,6 +44,8 @@ _SIGS = {
                                   C.c_double, P(abi.LightOut)]),
     'clr_meridional_solve': (C.c_int, [C.c_void_p, P(abi.Config), P(abi.Particles), P(abi.Sediment),
                                        P(abi.Meridional), P(abi.MeridionalOut)]),
+    'clr_probe_shape': (C.c_int, [C.c_int, P(C.c_int), P(C.c_int)]),
+    'clr_probe': (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, P(C.c_float), P(C.c_double)]),
     'clr_mlp_init': (C.c_int, [C.c_void_p, P(abi.MlpShape), C.c_uint64]),
     'clr_mlp_param_count': (C.c_size_t, [C.c_void_p]),
     'clr_mlp_get_params': (C.c_int, [C.c_void_p, P(C.c_float), C.c_size_t]),
@@ -112,6 +114,20 @@ class Context:
         keys = ['compute_units', 'wavefront', 'lds_per_block', 'clock_khz', 'gfx', 'hbm_mib',
                 'plain_blocks_per_cu', 'generic_blocks_per_cu']
         return dict(zip(keys, list(info)))
+
+    def probe(self, mode: int, items, slot: int = 0, seed: int = 0):
+        """Run clr_probe `mode` (abi.PROBE_MODES name or code) on items
+        [n][IN] of slot `slot`; returns [n][OUT] doubles."""
+        import numpy as np
+        code = abi.PROBE_MODES[mode] if isinstance(mode, str) else int(mode)
+        si, so = C.c_int(), C.c_int()
+        self.check(self.lib.clr_probe_shape(code, C.byref(si), C.byref(so)), 'clr_probe_shape')
+        a = np.ascontiguousarray(items, dtype=np.float32).reshape(-1, si.value)
+        out = np.zeros((a.shape[0], so.value), dtype=np.float64)
+        self.check(self.lib.clr_probe(self.h, int(slot), code, a.shape[0], int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                      a.ctypes.data_as(P(C.c_float)), out.ctypes.data_as(P(C.c_double))),
+                   'clr_probe')
+        return out
 
     def close(self):
         if getattr(self, 'h', None):

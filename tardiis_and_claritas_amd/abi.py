@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -54,6 +54,10 @@ SEDIMENT_TRANSPORT_MODEL_CODES = {
     'rz_field': 4,
     'xyz_field': 5,
 }
+
+# clr_probe modes (claritas.h CLR_PROBE_*)
+PROBE_MODES = {'surface': 0, 'wall': 1, 'offset': 2, 'sediment': 3, 'detector': 4, 'sampler': 5, 'source': 6,
+               'mie_theta': 7, 'conditional': 8}
 
 # counter indices (claritas.h enum)
 COUNTER_NAMES = [

@@ -34,7 +34,9 @@ class BatchedRunner:
         self.n_prepared = len(specs)
         return models
 
-    def run_prepared(self, budgets, seed, history_offsets=None):
+    def run_prepared(self, budgets, seed, history_offsets=None, seeds=None):
+        """Trace budgets[i] histories of prepared configuration i, all in one
+        launch; configuration i uses Philox key seeds[i] (default seed + i)."""
         n = len(budgets)
         if n > self.n_prepared:
             raise ValueError('more budgets than prepared configurations')
@@ -42,7 +44,7 @@ class BatchedRunner:
         for i, b in enumerate(budgets):
             items[i].config = i
             items[i].n_histories = int(b)
-            items[i].seed = (int(seed) + i) & 0xFFFFFFFFFFFFFFFF
+            items[i].seed = (int(seeds[i]) if seeds is not None else int(seed) + i) & 0xFFFFFFFFFFFFFFFF
             items[i].history_offset = int(history_offsets[i]) if history_offsets is not None else 0
         out = (abi.Tallies * n)()
         lp = None

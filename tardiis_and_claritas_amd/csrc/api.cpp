@@ -17,6 +17,7 @@
 #include "meridional.h"
 #include "mie.h"
 #include "mlp.h"
+#include "graph_key.h"
 #include "precompute.h"
 #include "transport.h"
 
@@ -75,16 +76,18 @@ struct Slot {
   }
 };
 
+// The captured clear + transport graph of clr_run_batch, replayed while
+// every argument it captured is unchanged (graph_key.h).
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  int n_items = -1, grid = 0, block = 0, kind = -1, n_slots = 0;
+  GraphKey key{};
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
-    n_items = -1;
+    key = GraphKey{};
   }
 };
 
@@ -231,6 +234,34 @@ float det_u(double deg, bool cosine) {
   return cosine ? (float)std::cos((double)th) : (float)std::sin((double)th);
 }
 
+// Bores hardware_detector (transport.hip) must test either side of the bore
+// nearest to the azimuth phi where an exit ray crosses r = ring_in.  In the
+// frame of bore j (axis y, lateral x) an accepted ray meets the ring_in plane
+// at lateral offset |l| <= throat_r and the throat_out plane within throat_r
+// of the axis, so its x-y slope against the axis is at most
+// s = 2 throat_r / (throat_out - ring_in).  Between the circle crossing C and
+// the plane the ray advances tau = ring_in - sqrt(ring_in^2 - x_C^2) along the
+// axis, so |x_C| <= X with X the fixed point of X = throat_r + s tau(X), and
+// |phi - 10 j| <= delta = asin(X / ring_in).  The nearest bore jc = rint(phi/10)
+// then satisfies |jc - j| <= floor(1/2 + delta/10); one bore and half a degree
+// are added against FP32 rounding of phi.  The window covers the full scan of
+// CLARITAS.py:4241-4253 (18 bores) when the bound fails to converge.
+int window_for_cell(double ring_in, double throat_out, double throat_r) {
+  const double s = 2.0 * throat_r / (throat_out - ring_in);
+  double X = throat_r;
+  for (int it = 0; it < 200; ++it) {
+    if (!(X < ring_in)) return CLR_N_CHANNELS;
+    const double tau = ring_in - std::sqrt(ring_in * ring_in - X * X);
+    const double nx = throat_r + s * tau;
+    if (std::fabs(nx - X) <= 1e-15 * ring_in) { X = nx; break; }
+    X = nx;
+  }
+  if (!(X < ring_in)) return CLR_N_CHANNELS;
+  const double delta = std::asin(X / ring_in) * (180.0 / kPi) + 0.5;
+  const int w = (int)std::floor(0.5 + delta / 10.0) + 1;
+  return std::min(w, CLR_N_CHANNELS);
+}
+
 int validate_config(clr_ctx* c, const clr_config* g, const clr_particles* p) {
   if (!g) return fail(c, CLR_E_ARG, "config is NULL");
   // CLARITAS.py:5797-5826
@@ -325,6 +356,7 @@ void fill_params(Slot& s, const clr_config& g) {
     P.det_ux[j] = det_u(10.0 * j, false);
     P.det_uy[j] = det_u(10.0 * j, true);
   }
+  P.det_window = window_for_cell(g.ring_in, g.throat_out, g.throat_r);
   for (int a = 0; a < CLR_MAX_AUX; ++a) {
     P.aux_ux[a] = det_u(g.aux_angles_deg[a], false);
     P.aux_uy[a] = det_u(g.aux_angles_deg[a], true);
@@ -604,8 +636,8 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
   const size_t words = sizeof(clr_tallies) * n_slots / 8;
   if (use_graph) {
     GraphCache& G = c->graph;
-    if (!(G.exec && G.n_items == L.n_items && G.grid == L.grid && G.block == L.block && G.kind == L.kind &&
-          G.n_slots == n_slots)) {
+    const GraphKey key = graph_key(L, n_slots);
+    if (!(G.exec && same_graph_key(G.key, key))) {
       G.reset();
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       hipLaunchKernelGGL(clear_kernel, dim3(256), dim3(256), 0, c->stream, c->tallies.as<unsigned long long>(), words,
@@ -615,7 +647,7 @@ int launch_items(clr_ctx* c, const std::vector<WorkItem>& items, const RunShape&
       if (le != hipSuccess) return fail(c, CLR_E_HIP, std::string("transport launch: ") + hipGetErrorString(le));
       if (ce != hipSuccess) return fail(c, CLR_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
       HIPCHK(c, hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
-      G.n_items = L.n_items; G.grid = L.grid; G.block = L.block; G.kind = L.kind; G.n_slots = n_slots;
+      G.key = key;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
@@ -1015,6 +1047,40 @@ int clr_get_tables(clr_ctx* c, int slot, clr_tables_out* o) {
   if (o->sed_radial_log) HIPCHK(c, hipMemcpy(o->sed_radial_log, s.radlog.p, nb * s.nrad, hipMemcpyDeviceToHost));
   memcpy(o->mu_scalars, s.mu_scalars_h, sizeof(s.mu_scalars_h));
   memcpy(o->majorants, s.majorants_h, sizeof(s.majorants_h));
+  return 0;
+}
+
+int clr_probe_shape(int mode, int* in_stride, int* out_stride) {
+  static const int kIn[CLR_PROBE_N_MODES] = {6, 6, 6, 4, 6, 3, 1, 1, 7};
+  static const int kOut[CLR_PROBE_N_MODES] = {6, 4, 4, 2, 2, 2, 6, 1, 9};
+  if (mode < 0 || mode >= CLR_PROBE_N_MODES) return CLR_E_ARG;
+  if (in_stride) *in_stride = kIn[mode];
+  if (out_stride) *out_stride = kOut[mode];
+  return 0;
+}
+
+int clr_probe(clr_ctx* c, int slot, int mode, int n, uint64_t seed, const float* in, double* out) {
+  if (!c || !out || (n > 0 && !in)) return fail(c, CLR_E_ARG, "null argument");
+  int si = 0, so = 0;
+  if (clr_probe_shape(mode, &si, &so) != 0) return fail(c, CLR_E_ARG, "unknown probe mode");
+  if (n < 0 || n > (1 << 24)) return fail(c, CLR_E_RANGE, "probe item count must be in [0, 2^24]");
+  if (slot < 0 || slot >= (int)c->slots.size() || !c->slots[slot]->ready) return fail(c, CLR_E_STATE, "slot not prepared");
+  if (n == 0) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  void* din = nullptr;
+  void* dout = nullptr;
+  const size_t bin = sizeof(float) * (size_t)n * si, bout = sizeof(double) * (size_t)n * so;
+  HIPCHK(c, hipMalloc(&din, bin));
+  hipError_t e = hipMalloc(&dout, bout);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, in, bin, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = clr_launch_probe(c->params.as<DevParams>() + slot, mode, n, static_cast<const float*>(din), si,
+                         static_cast<double*>(dout), so, seed, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, CLR_E_HIP, std::string("probe: ") + hipGetErrorString(e));
   return 0;
 }
 

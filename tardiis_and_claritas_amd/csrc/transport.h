@@ -38,7 +38,9 @@ struct DevParams {
   int n_slabs;
   int orientation;                // spheroid axis law (0 isotropic, 1 local-tangential vMF)
   int n_groups, n_bins, n_theta, n_radial;
-  int fn0, fn1, fn2, pad0;
+  int fn0, fn1, fn2;
+  // bores tested either side of the exit azimuth's nearest bore (hardware_detector)
+  int det_window;
   // per-bin tables
   const float* radius;
   const float* n_real;
@@ -129,5 +131,10 @@ hipError_t clr_launch_transport(const TransportLaunch& L, hipStream_t s);
 // Diagnostic deposition of explicit inputs (mode 0 segments [n][7], mode 1
 // event sites [n][3]) into the buffers of D (device pointers).
 hipError_t clr_launch_light_probe(const LightDiag* D, const float* data, int n, int mode, hipStream_t s);
+// Device probes of the transport's building blocks on explicit inputs
+// (clr_probe, include/claritas.h): mode, n items of `in_stride` floats in,
+// `out_stride` doubles out per item, random streams keyed by seed.
+hipError_t clr_launch_probe(const DevParams* P, int mode, int n, const float* in, int in_stride, double* out,
+                            int out_stride, uint64_t seed, hipStream_t s);
 // Resident workgroups per CU for `block` threads (occupancy query).
 int clr_transport_occupancy(int block, int kind);

@@ -1666,12 +1666,12 @@ __device__ __forceinline__ bool bore_accept(const DevParams& P, float x, float y
   return false;
 }
 // First bore (smallest t0) whose three apertures the exit ray passes.  Only
-// bores within +-20 deg of the azimuth where the ray crosses r = ring_in are
-// tested: an accepted ray crosses that circle before its ring_in aperture
-// plane, at most ~0.2 mm away laterally, so its crossing azimuth lies within
-// asin(throat_r/ring_in) + 0.5 deg of the bore axis (< 10 deg for the cell
-// geometries here); the tested set therefore contains every bore the full
-// 18-bore scan of CLARITAS.py:4241-4253 could accept, in the same order.
+// the bores within P.det_window of the bore nearest to the azimuth where the
+// ray crosses r = ring_in are tested.  The host derives the window from the
+// cell (window_for_cell in api.cpp): an accepted ray passes both throat
+// apertures, which bounds its crossing azimuth's distance from the bore axis;
+// the tested set then contains every bore the full 18-bore scan of
+// CLARITAS.py:4241-4253 could accept, in the same order.
 __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
                                  bool mirror) {
   if (mirror) { x = -x; vx = -vx; }
@@ -1682,10 +1682,10 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
   if (!(disc >= 0.0f)) return -1;
   const float t = (-b + sqrtf(disc)) / a;
   const float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
-  const int jc = (int)rintf(deg * 0.1f);
+  const int jc = (int)rintf(deg * 0.1f), W = P.det_window;
   float best_t = kInfF, tt;
   int best = -1;
-  for (int k = -2; k <= 2; ++k) {
+  for (int k = -W; k <= W; ++k) {
     const int j = jc + k;
     if (j < 0 || j >= CLR_N_CHANNELS) continue;
     if (bore_accept(P, x, y, z, vx, vy, vz, P.det_ux[j], P.det_uy[j], tt) && tt < best_t) { best_t = tt; best = j; }
@@ -2059,11 +2059,12 @@ constexpr bool kind_rough() { return kKind == CLR_KERNEL_ROUGH || kKind == CLR_K
 template <int kKind>
 constexpr bool kind_diag() { return kKind == CLR_KERNEL_LIGHT || kKind == CLR_KERNEL_CAUSAL; }
 
-template <int kKind>
-__device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A, int& sbi, int& sbr) {
-  constexpr bool kBores = kind_rough<kKind>();
-  Rng& r = L.r;
-  float x, y, z, vx = 0.0f, vy = 1.0f, vz = 0.0f;
+// Source position on the launch plane y = -source_r and direction, before
+// the collimator, CLARITAS.py:4650-4680 (the collimated disk has no reference
+// counterpart).
+__device__ __forceinline__ void source_sample(Rng& r, const DevParams& P, float& x, float& y, float& z, float& vx,
+                                              float& vy, float& vz) {
+  vx = 0.0f; vy = 1.0f; vz = 0.0f;
   if (P.src_model == CLR_SOURCE_COLLIMATED_DISK) {
     const float u1 = U(r), u2 = U(r);
     const float rho = P.beam_r * sqrtf(u1), ang = 2.0f * kPiF * u2;
@@ -2088,6 +2089,14 @@ __device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc&
     vx = sinf(h); vy = cosf(h); vz = 0.0f;
   }
   normalize3(vx, vy, vz);
+}
+
+template <int kKind>
+__device__ __forceinline__ void launch_history(Lane& L, const DevParams& P, Acc& A, int& sbi, int& sbr) {
+  constexpr bool kBores = kind_rough<kKind>();
+  Rng& r = L.r;
+  float x, y, z, vx, vy, vz;
+  source_sample(r, P, x, y, z, vx, vy, vz);
   L.x = x; L.y = y; L.z = z; L.vx = vx; L.vy = vy; L.vz = vz;
   if (vy <= 1e-12f) { terminate(L, 3); return; }
   bool reflecting = false;
@@ -3230,10 +3239,112 @@ __global__ void light_probe_kernel(const LightDiag* __restrict__ D, const float*
     light_site(*D, (uint64_t)t, q[0], q[1], q[2], D->event_weight, D->event_count, D->event_total);
   }
 }
+
+// The transport's building blocks on explicit inputs, one item per thread
+// (the counterpart of the reference's test kernels free_surface_geometry_test_kernel,
+// sediment_field_test_kernel, bore_wall_test_kernel and
+// water_surface_offset_test_kernel, CLARITAS.py:4438-4455, 4228-4235,
+// 2952-2956, 2980-2985).  Item i reads in[i * in_stride ..] and writes
+// out[i * out_stride ..]; random streams are (seed, history i).  Modes and
+// layouts are listed with clr_probe in include/claritas.h.
+__global__ void probe_kernel(const DevParams* __restrict__ PP, int mode, int n, const float* __restrict__ in,
+                             int in_stride, double* __restrict__ out, int out_stride, uint64_t seed) {
+  if (threadIdx.x == 0) { s_key[0] = (uint32_t)seed; s_key[1] = (uint32_t)(seed >> 32); }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevParams& P = *PP;
+  const float* q = in + (size_t)i * in_stride;
+  double* o = out + (size_t)i * out_stride;
+  Rng r;
+  rng_seed(r, (uint64_t)i);
+  switch (mode) {
+    case CLR_PROBE_SURFACE: {
+      o[0] = free_surface_z(P, q[0], q[1]);
+      const float t = free_surface_hit(P, q[0], q[1], q[2], q[3], q[4], q[5]);
+      o[1] = t;
+      float nx = 0.0f, ny = 0.0f, nz = 0.0f, zh = kInfF;
+      if (t < 0.5f * kInfF) {
+        const float hx = q[0] + t * q[3], hy = q[1] + t * q[4];
+        zh = free_surface_z(P, hx, hy);
+        free_surface_normal(P, hx, hy, nx, ny, nz);
+      }
+      o[2] = zh; o[3] = nx; o[4] = ny; o[5] = nz;
+      break;
+    }
+    case CLR_PROBE_WALL: {
+      o[0] = bore_wall_hit_d(P, q[0], q[1], q[2], q[3], q[4], q[5], P.rin, P.zmin, P.zwall_top);
+      o[1] = finite_cylinder_forward_d(P, q[0], q[1], q[2], q[3], q[4], q[5], P.rout, P.zlow, P.zwall_top);
+      double ta = 0.0;
+      const int surf = acrylic_next_surface(P, q[0], q[1], q[2], q[3], q[4], q[5], ta);
+      o[2] = ta; o[3] = surf;
+      break;
+    }
+    case CLR_PROBE_OFFSET: {
+      float x = q[0], y = q[1], z = q[2];
+      const bool ok = water_surface_offset(P, x, y, z, q[3], q[4], q[5]);
+      o[0] = ok ? 1.0 : 0.0; o[1] = x; o[2] = y; o[3] = z;
+      break;
+    }
+    case CLR_PROBE_SEDIMENT: {
+      const int b = (int)q[0];
+      o[0] = (b >= 0 && b < P.n_bins) ? sediment_scale(P, b, q[1], q[2], q[3]) : -1.0;
+      float f = 1.0f;
+      if (P.sed_model == CLR_SED_RZ_FIELD)
+        f = rz_field_f32(P.field, P.fn0, P.fn1, q[1], q[2], q[3], P.rin, P.zmin, P.field_inv0, P.field_inv1);
+      else if (P.sed_model == CLR_SED_XYZ_FIELD)
+        f = xyz_field_f32(P.field, P.fn0, P.fn1, P.fn2, q[1], q[2], q[3], P.field_half, P.zmin, P.field_inv0,
+                          P.field_inv1, P.field_inv2);
+      o[1] = f;
+      break;
+    }
+    case CLR_PROBE_DETECTOR:
+      o[0] = hardware_detector(P, q[0], q[1], q[2], q[3], q[4], q[5], false);
+      o[1] = hardware_detector(P, q[0], q[1], q[2], q[3], q[4], q[5], true);
+      break;
+    case CLR_PROBE_SAMPLER: {
+      const int k = (int)q[0];
+      o[0] = k == 0 ? gamma_draw(r, q[1]) : k == 1 ? beta_draw(r, q[1], q[2])
+                                         : k == 2 ? beta_radiance_polar(r, q[1], q[2]) : normal_bm(r);
+      o[1] = rng_draws(r);
+      break;
+    }
+    case CLR_PROBE_SOURCE: {
+      float x, y, z, vx, vy, vz;
+      source_sample(r, P, x, y, z, vx, vy, vz);
+      o[0] = x; o[1] = y; o[2] = z; o[3] = vx; o[4] = vy; o[5] = vz;
+      break;
+    }
+    case CLR_PROBE_MIE_THETA: {
+      const int b = (int)q[0];
+      o[0] = (b >= 0 && b < P.n_bins) ? sample_mie_theta(r, P, b) : -1.0;
+      break;
+    }
+    case CLR_PROBE_CONDITIONAL: {
+      const int b = (int)q[0];
+      if (b < 0 || b >= P.n_bins) { o[0] = -1.0; break; }
+      const bool rough = P.rough > 1e-7f || P.coherent < 1.0f;
+      const CondOut c = rough ? conditional_event<true>(P, r, b, q[1], q[2], q[3], q[4], q[5], q[6], P.cond_trials)
+                              : conditional_event<false>(P, r, b, q[1], q[2], q[3], q[4], q[5], q[6], P.cond_trials);
+      o[0] = c.status; o[1] = c.channel; o[2] = c.trials; o[3] = c.rejects; o[4] = c.inacc; o[5] = c.failure;
+      o[6] = c.vx; o[7] = c.vy; o[8] = c.vz;
+      break;
+    }
+    default:
+      break;
+  }
+}
 }  // namespace
 
 hipError_t clr_launch_light_probe(const LightDiag* D, const float* data, int n, int mode, hipStream_t s) {
   hipLaunchKernelGGL(light_probe_kernel, dim3((n + 63) / 64), dim3(64), 0, s, D, data, n, mode);
+  return hipGetLastError();
+}
+
+hipError_t clr_launch_probe(const DevParams* P, int mode, int n, const float* in, int in_stride, double* out,
+                            int out_stride, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL(probe_kernel, dim3((n + 63) / 64), dim3(64), 0, s, P, mode, n, in, in_stride, out, out_stride,
+                     seed);
   return hipGetLastError();
 }
 

@@ -29,39 +29,40 @@ CONDITIONAL_MAX_TRIALS = 16384  # CLARITAS.py:1117
 
 
 def _jackknife_normalized_se(batch_scores):
-    """Leave-one-batch-out SE of the normalised 18-channel response,
-    CLARITAS.py:5244-5261 (NaN with fewer than 3 usable batches)."""
+    """Delete-one-batch jackknife standard error of the normalised response
+    r = T / sum(T), T = sum of the per-batch channel scores; the statistic of
+    CLARITAS.py:5244-5261.
+
+    With R_b = (T - s_b) / sum(T - s_b) the leave-one-out responses over the
+    B' batches whose complement has a positive sum, the estimate is
+    sqrt((B' - 1) * Var(R_b)), the population variance taken per channel
+    (Quenouille-Tukey jackknife).  NaN per channel when fewer than 3
+    batches are given or usable."""
     nch = len(DETECTOR_ANGLES_DEG)
-    if len(batch_scores) < 3:
-        return np.full(nch, np.nan)
     s = np.asarray(batch_scores, dtype=float)
-    total = s.sum(axis=0)
-    loo = []
-    for b in range(s.shape[0]):
-        q = total - s[b]
-        den = q.sum()
-        if den > 0:
-            loo.append(q / den)
-    if len(loo) < 3:
+    if s.ndim != 2 or s.shape[0] < 3:
         return np.full(nch, np.nan)
-    loo = np.asarray(loo)
-    m = loo.mean(axis=0)
-    B = loo.shape[0]
-    return np.sqrt((B - 1.0) / B * np.sum((loo - m) ** 2, axis=0))
+    comp = s.sum(axis=0)[None, :] - s                  # each batch's complement
+    mass = comp.sum(axis=1)
+    keep = mass > 0
+    if np.count_nonzero(keep) < 3:
+        return np.full(nch, np.nan)
+    loo = comp[keep] / mass[keep, None]
+    return np.sqrt((loo.shape[0] - 1) * loo.var(axis=0))
 
 
 def _histogram_median(hist):
-    """Median of an integer histogram, CLARITAS.py:5228-5240."""
+    """Median of the sample in which value v occurs hist[v] times
+    (CLARITAS.py:5231-5242): the mean of the order statistics of ranks
+    floor((n+1)/2) and floor(n/2)+1 (1-based), each the smallest v whose
+    cumulative count reaches the rank.  NaN for an empty histogram."""
     hist = np.asarray(hist, dtype=np.int64)
     n = int(hist.sum())
     if n <= 0:
         return float('nan')
-    cs = np.cumsum(hist)
-    if n % 2:
-        return float(np.searchsorted(cs, n // 2 + 1))
-    a = int(np.searchsorted(cs, n // 2))
-    b = int(np.searchsorted(cs, n // 2 + 1))
-    return 0.5 * (a + b)
+    cum = np.cumsum(hist)
+    lo, hi = np.searchsorted(cum, [(n + 1) // 2, n // 2 + 1])
+    return 0.5 * float(lo + hi)
 
 
 def _norm(v):
@@ -373,6 +374,7 @@ class ClaritasForwardModel:
         particle_event_model='geometric',
         hybrid_wave_scale=1.0,
         mie_phase_grid_size=2049,
+        moment_constraint_by_material=None,
         particle_surface_rms_slope_deg=25.0,
         wave_surface_rms_height_m=100e-9,
         wave_spheroid_aspect_ratio=1.5,
@@ -406,6 +408,9 @@ class ClaritasForwardModel:
         fragmentation_breakup_exposure=0.0,
         fragmentation_reference_diameter_m=100e-6,
         fragmentation_size_exponent=1.0,
+        aggregation_step_safety=0.2,
+        aggregation_max_steps=200000,
+        max_axial_bounces=64,
         max_iterations=int(1e6),
         max_internal_bounces=64,
         max_cell_bounces=64,
@@ -497,10 +502,16 @@ class ClaritasForwardModel:
         self.fragmentation_breakup_exposure = float(fragmentation_breakup_exposure)
         self.fragmentation_reference_diameter_m = float(fragmentation_reference_diameter_m)
         self.fragmentation_size_exponent = float(fragmentation_size_exponent)
-        if self.aggregation_model != 'none' or self.aggregation_collision_exposure != 0.0 or \
-                self.fragmentation_breakup_exposure != 0.0:
-            raise NotImplementedError('orthokinetic aggregation / fragmentation of the PSD is not built: '
-                                      "aggregation_model must be 'none' with zero exposures")
+        # integrator controls of the population balance; validated as the
+        # reference does (CLARITAS.py:5958-5961) although model 'none' never steps
+        self.aggregation_step_safety = float(aggregation_step_safety)
+        self.aggregation_max_steps = int(aggregation_max_steps)
+        self._validate_population()
+        self.moment_constraint_by_material = moment_constraint_by_material or {}
+        # the acrylic loops are bounded by their own emergency cap (16384,
+        # CLARITAS.py:3143-3147); the reference validates this value and never
+        # reads it afterwards, and so does this model
+        self.max_axial_bounces = int(max_axial_bounces)
         self.max_internal_bounces = int(max_internal_bounces)
         self.max_cell_bounces = int(max_cell_bounces)
         self.chunk_size = int(chunk_size)
@@ -524,6 +535,33 @@ class ClaritasForwardModel:
         return self._ctx
 
     # ---------------------------------------------------------------- checks
+    def _validate_population(self):
+        """Population-balance settings: the reference's rules
+        (CLARITAS.py:5920-5961), then the one model built here ('none' with
+        zero exposures: the PSD weights are only normalised)."""
+        if self.aggregation_model not in ('none', 'orthokinetic_compact', 'orthokinetic_compact_fragmentation'):
+            raise ValueError("aggregation_model must be 'none', 'orthokinetic_compact', or "
+                             "'orthokinetic_compact_fragmentation'")
+        if not (math.isfinite(self.aggregation_collision_exposure) and self.aggregation_collision_exposure >= 0.0):
+            raise ValueError('aggregation_collision_exposure must be finite and >=0')
+        if not (math.isfinite(self.fragmentation_breakup_exposure) and self.fragmentation_breakup_exposure >= 0.0):
+            raise ValueError('fragmentation_breakup_exposure must be finite and >=0')
+        if not (math.isfinite(self.fragmentation_reference_diameter_m) and self.fragmentation_reference_diameter_m > 0):
+            raise ValueError('fragmentation_reference_diameter_m must be finite and >0')
+        if not (math.isfinite(self.fragmentation_size_exponent) and 0.0 <= self.fragmentation_size_exponent <= 6.0):
+            raise ValueError('fragmentation_size_exponent must be finite and in [0,6]')
+        if self.aggregation_model != 'orthokinetic_compact_fragmentation' and self.fragmentation_breakup_exposure > 0:
+            raise ValueError('fragmentation_breakup_exposure requires '
+                             'aggregation_model=orthokinetic_compact_fragmentation')
+        if not (0.0 < self.aggregation_step_safety <= 0.25):
+            raise ValueError('aggregation_step_safety must be in (0,0.25]')
+        if self.aggregation_max_steps < 1:
+            raise ValueError('aggregation_max_steps must be >=1')
+        if self.aggregation_model != 'none' or self.aggregation_collision_exposure != 0.0 or \
+                self.fragmentation_breakup_exposure != 0.0:
+            raise NotImplementedError('orthokinetic aggregation / fragmentation of the PSD is not built: '
+                                      "aggregation_model must be 'none' with zero exposures")
+
     def _validate(self):
         """Fail-closed validation, following CLARITAS.py:5754-5999 for the
         supported physics and refusing what the device path does not model."""
@@ -570,6 +608,15 @@ class ClaritasForwardModel:
             raise ValueError('bore radii must satisfy 0 < throat < counterbore')
         if self.mie_phase_grid_size < 257:
             raise ValueError('mie_phase_grid_size must be >= 257')
+        # CLARITAS.py:5992-5999
+        if not (self.zmin < 0 < self.zmean):
+            raise ValueError('sensor plane must lie within water column')
+        if self.bottom_disc_thickness <= 0.0:
+            raise ValueError('bottom_disc_thickness_m must be positive')
+        if self.bottom_external_n <= 0.0:
+            raise ValueError('bottom_external_refractive_index must be positive')
+        if self.max_axial_bounces < 1:
+            raise ValueError('max_axial_bounces must be >=1')
         if not (1 <= self.jackknife_groups <= abi.MAX_GROUPS):
             raise ValueError('jackknife_groups must be 1..64')
         if self.aux_detector_angles_deg.size > abi.MAX_AUX:
@@ -813,6 +860,11 @@ class ClaritasForwardModel:
         'enabled': True; keys as CLARITAS.py:7232-7247)."""
         if collect_rays:
             raise ValueError('per-ray outputs are not produced by the device path')
+        requested = int(n_rays)
+        ray_id_start = int(ray_id_start)
+        adaptive = target_detector_score is not None
+        plan = self._adaptive_plan(requested, ray_id_start, target_detector_score, min_rays, max_rays,
+                                   min_rear_channel_score, min_h170_score)
         mat, cfg = self.prepare(material, concentration_g_per_L, weights, weight_mode)
         lacc = None
         vis = int(heatmap_size or 0)
@@ -826,14 +878,7 @@ class ClaritasForwardModel:
         # detector-path / causal / H170 diagnostics (always on in the
         # reference; path_diagnostics=False runs the production kernel only)
         cacc = causal.CausalAccumulator(self, len(mat.diameter_m)) if path_diagnostics else None
-        requested = int(n_rays)
-        ray_id_start = int(ray_id_start)
-        if ray_id_start < 0:
-            raise ValueError('ray_id_start must be >= 0')
-        adaptive = target_detector_score is not None
         if not adaptive:
-            if requested < 1:
-                raise ValueError('number of rays must be positive')
             T, _, ms = self.run_tallies(requested, seed, ray_id_start, light=lacc, causal=cacc)
             batches = [T['det'][g] for g in range(T['n_groups'])]
             pbatches = [T['photodiode'][g] for g in range(T['n_groups'])]
@@ -841,45 +886,10 @@ class ClaritasForwardModel:
             stop_reason, total = 'fixed_n_rays', requested
             det_sum = T['det'].sum(axis=0)
         else:
-            target = float(target_detector_score)
-            if target <= 0:
-                raise ValueError('target_detector_score must be >0')
-            maxN = int(max_rays if max_rays is not None else max(requested, self.statistics_batch_rays))
-            minN = int(min_rays if min_rays is not None else min(self.statistics_batch_rays, maxN))
-            if minN < 0 or maxN < minN or maxN < 1:
-                raise ValueError('invalid min_rays/max_rays')
-            total, ms, batches, pbatches, group_n = 0, 0.0, [], [], []
-            acc = None
-            stop_reason = 'max_rays'
-            prev_cum, recent = None, []
-            while total < maxN:
-                b = min(self.statistics_batch_rays, maxN - total)
-                Tb, _, msb = self.run_tallies(b, seed, ray_id_start + total, light=lacc, causal=cacc)
-                ms += msb
-                d = Tb['det'].sum(axis=0)
-                batches.append(d)
-                pbatches.append(Tb['photodiode'].sum(axis=0))
-                group_n.append(b)
-                acc = Tb if acc is None else self._merge(acc, Tb)
-                total += b
-                sym = self._selected(np.stack(batches).sum(axis=0), np.stack(pbatches).sum(axis=0))
-                l1 = float(np.sum(np.abs(_norm(sym) - prev_cum))) if prev_cum is not None else float('nan')
-                if np.isfinite(l1):
-                    recent.append(l1)
-                prev_cum = _norm(sym)
-                if total >= minN and sym.sum() >= target:
-                    ok = (min_rear_channel_score is None or float(np.min(sym[15:18])) >= float(min_rear_channel_score))
-                    ok &= (min_h170_score is None or float(sym[17]) >= float(min_h170_score))
-                    if stability_l1_tolerance is not None:
-                        w = max(1, int(stability_window))
-                        ok &= len(recent) >= w and all(q <= float(stability_l1_tolerance) for q in recent[-w:])
-                    if ok:
-                        stop_reason = 'target_detector_score'
-                        break
-            T = acc
-            T['n_groups'] = len(batches)
+            run = lambda b, off: self.run_tallies(b, seed, off, light=lacc, causal=cacc)
+            T, batches, pbatches, group_n, total, ms, stop_reason = self._run_adaptive(
+                run, ray_id_start, plan, stability_l1_tolerance, stability_window)
             det_sum = np.stack(batches).sum(axis=0)
-            group_n = np.asarray(group_n)
         res = self._result(mat, T, det_sum, batches, pbatches, group_n, requested, total, adaptive, stop_reason, ms)
         if lacc is not None:
             res.heatmap_xy, res.heatmap_zy = lacc.heatmaps()
@@ -899,6 +909,88 @@ class ClaritasForwardModel:
                 cacc.tally, self._sym(det_sum), closure, counts, CONDITIONAL_MAX_TRIALS)
             res.production_failure_diagnostics['samples'] = _failure_samples(cacc)
         return res
+
+    def _adaptive_plan(self, requested, ray_id_start, target_detector_score, min_rays, max_rays,
+                       min_rear_channel_score, min_h170_score):
+        """Ray budget and stop targets of simulate, with the reference's
+        argument checks (CLARITAS.py:7188-7227): history ids are the
+        reference's uint32 ray ids, so a range past 2^32 is refused here (the
+        64-bit range is open through run_tallies / clr_run)."""
+        if ray_id_start < 0 or ray_id_start + requested > 2 ** 32:
+            raise ValueError('ray_id_start range exceeds uint32 global IDs')
+        plan = dict(target=None, rear=None, h170=None, maxN=requested, minN=requested)
+        if target_detector_score is not None:
+            target = float(target_detector_score)
+            if target <= 0:
+                raise ValueError('target_detector_score must be >0')
+            rear = None if min_rear_channel_score is None else float(min_rear_channel_score)
+            h170 = None if min_h170_score is None else float(min_h170_score)
+            if rear is not None and rear <= 0:
+                raise ValueError('min_rear_channel_score must be >0')
+            if h170 is not None and h170 <= 0:
+                raise ValueError('min_h170_score must be >0')
+            maxN = int(max_rays if max_rays is not None else max(requested, self.statistics_batch_rays))
+            minN = int(min_rays if min_rays is not None else min(self.statistics_batch_rays, maxN))
+            if minN < 0 or maxN < minN:
+                raise ValueError('invalid min_rays/max_rays')
+            plan.update(target=target, rear=rear, h170=h170, maxN=maxN, minN=minN)
+        if plan['maxN'] < 1:
+            raise ValueError('number of rays must be positive')
+        if ray_id_start + plan['maxN'] > 2 ** 32:
+            raise ValueError('global ray range exceeds uint32')
+        return plan
+
+    def _run_adaptive(self, run, ray_id_start, plan, stability_l1_tolerance, stability_window):
+        """Batches of statistics_batch_rays until the selected detector score
+        reaches its target (and the optional rear-channel, H170 and L1
+        stability criteria hold) or max_rays is spent; the stop reason names
+        the criteria, as CLARITAS.py:8818-8881 does.  `run(b, offset)`
+        traces b histories from history id `offset` and returns
+        (tallies, records, ms)."""
+        total, ms, batches, pbatches, group_n = 0, 0.0, [], [], []
+        acc, prev, recent = None, None, []
+        stop_reason = 'fixed_n_rays'
+        maxN, minN, target = plan['maxN'], plan['minN'], plan['target']
+        rear, h170 = plan['rear'], plan['h170']
+        sym = np.zeros(len(DETECTOR_ANGLES_DEG))
+        while total < maxN:
+            b = min(self.statistics_batch_rays, maxN - total)
+            Tb, _, msb = run(b, ray_id_start + total)
+            ms += msb
+            batches.append(Tb['det'].sum(axis=0))
+            pbatches.append(Tb['photodiode'].sum(axis=0))
+            group_n.append(b)
+            acc = Tb if acc is None else self._merge(acc, Tb)
+            total += b
+            sym = self._selected(np.stack(batches).sum(axis=0), np.stack(pbatches).sum(axis=0))
+            cum = _norm(sym)
+            if prev is not None:
+                recent.append(float(np.abs(cum - prev).sum()))
+            prev = cum
+            if total < minN or sym.sum() < target:
+                continue
+            met = (rear is None or float(sym[15:18].min()) >= rear) and (h170 is None or float(sym[17]) >= h170)
+            if stability_l1_tolerance is not None:
+                w = max(1, int(stability_window))
+                met = met and len(recent) >= w and max(recent[-w:]) <= float(stability_l1_tolerance)
+            if met:
+                names = ['target_detector_score']
+                names += ['rear_channels'] if rear is not None else []
+                names += ['H170'] if h170 is not None else []
+                names += ['stability'] if stability_l1_tolerance is not None else []
+                stop_reason = '_and_'.join(names)
+                break
+        if total >= maxN:
+            if sym.sum() < target:
+                stop_reason = 'max_rays_before_target'
+            elif rear is not None and float(sym[15:18].min()) < rear:
+                stop_reason = 'max_rays_before_rear_channel_target'
+            elif h170 is not None and float(sym[17]) < h170:
+                stop_reason = 'max_rays_before_H170_target'
+            elif stop_reason.startswith('fixed'):
+                stop_reason = 'max_rays_before_stability' if stability_l1_tolerance is not None else 'max_rays'
+        acc['n_groups'] = len(batches)
+        return acc, batches, pbatches, np.asarray(group_n), total, ms, stop_reason
 
     @staticmethod
     def _merge(a, b):

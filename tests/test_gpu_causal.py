@@ -200,7 +200,7 @@ def test_adaptive_simulate_with_all_diagnostics(gpu_available):
     m = ClaritasForwardModel(statistics_batch_rays=50_000, **kw)
     r = m.simulate(mat, conc, n_rays=50_000, seed=seed, target_detector_score=1e12, max_rays=150_000,
                    heatmap_size=32)
-    assert r.n_rays == 150_000 and r.stop_reason == 'max_rays'
+    assert r.n_rays == 150_000 and r.stop_reason == 'max_rays_before_target'  # CLARITAS.py:8872-8873
     d = r.detector_particle_path_diagnostics
     causal.check_closure(d)
     np.testing.assert_allclose(d['hardware_symmetry_scores'], r.hardware_symmetry_scores)

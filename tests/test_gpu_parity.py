@@ -4,11 +4,16 @@ Bars: Mie tables within 1e-6 relative of the FP64 oracle; transport tables
 bit-for-bit where they are bookkeeping; history-by-history agreement of the
 discrete outcome (status, detector channels, interaction count, Philox words
 consumed) on the same Philox streams for >= 99.9 % of histories; detector
-tallies of independent runs within 3 combined standard errors.
+tallies of independent runs within 3 combined standard errors (family-wise,
+tests/tally_cases.py) at >= 2,000 hardware 90-degree hits per side.
 """
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
+import tally_cases
 from tardiis_and_claritas_amd import ClaritasForwardModel, workloads
 from tardiis_and_claritas_amd import physics
 
@@ -117,67 +122,51 @@ def test_history_by_history_rough(gpu_available, oracle_lib, over):
         assert abs(a - b) <= 0.002 * max(a, b) + 2, (k, a, b)
 
 
-def _sym(T):
-    d = T['det'].sum(0)
-    return 0.5 * (d[0] + d[1]), 0.5 * (d[2] + d[3])
+GOLDEN_TALLIES = json.loads((Path(__file__).parent / 'golden' / 'oracle_tallies.json').read_text())
 
 
-@pytest.mark.parametrize('builder', ['config1_hybrid', 'config1_plain', 'config2_plain_1000', 'config2_hybrid_316',
-                                     'config1_rough', 'config2_plain_5000'])
-def test_tallies_within_3_sigma(gpu_available, oracle_lib, builder):
-    _flush(f'3-sigma tallies {builder}')
-    if builder == 'config1_hybrid':
-        kw, mat, conc, n, seed = workloads.config1('hybrid_mie_exact_rate')
-        N = 400_000
-    elif builder == 'config1_plain':
-        kw, mat, conc, n, seed = workloads.config1('plain_mie')
-        N = 400_000
-    elif builder == 'config2_plain_1000':
-        kw, mat, conc, n, seed = workloads.config2(1000.0, 'plain_mie')
-        kw['aux_detector_angles_deg'] = (135.0,)
-        N = 200_000
-    elif builder == 'config2_plain_5000':
-        # shared Rouse field at the sweep's densest point: the device tracks
-        # with a piecewise (z-slab) majorant, the oracle with the single one
-        kw, mat, conc, n, seed = workloads.config2(5000.0, 'plain_mie')
-        kw['aux_detector_angles_deg'] = (135.0,)
-        N = 200_000
-    elif builder == 'config1_rough':
-        kw, mat, conc, n, seed = workloads.config1_rough()
-        N = 200_000
-    else:
-        kw, mat, conc, n, seed = workloads.config2(316.0, 'hybrid_mie_exact_rate')
-        N = 100_000
+@pytest.mark.parametrize('case', tally_cases.CASES)
+def test_tallies_within_3_sigma(gpu_available, case):
+    """Device tallies at >= 2,000 hardware 90-degree hits per side against
+    the oracle's committed tallies of an independent stream (same law, same
+    N), every channel at three combined standard errors family-wise (rule
+    in tests/tally_cases.py); status and failure-code rates likewise."""
+    fx = GOLDEN_TALLIES[case]
+    kw, mat, conc, seed, N = tally_cases.case_spec(case)
+    assert fx['n'] == N and fx['key'] == seed + 2
+    _flush(f'3-sigma tallies {case}: {N:,} histories')
     m = _model(kw)
     m.prepare(mat, conc)
     Tg, _, _ = m.run_tallies(N, seed + 1, 0)
-    To, _ = oracle_lib.OracleModel(m, mat, conc).run(N, seed + 2, 0)
-    for a, b in zip(_sym(Tg), _sym(To)):
-        # per-channel counts: independent binomials, combined sd sqrt(a+b)
-        sd = np.sqrt(np.maximum(a + b, 1.0))
-        z = np.abs(a - b) / sd
-        assert np.all(z < 4.0), f'{builder}: max z {z.max():.2f} at channel {z.argmax()}'
-    aw_g, aw_o = Tg['aux_window'].sum(0).sum(0), To['aux_window'].sum(0).sum(0)
-    assert np.all(np.abs(aw_g - aw_o) <= 4.0 * np.sqrt(np.maximum(aw_g + aw_o, 1.0))), (aw_g, aw_o)
-    if builder == 'config1_rough':
-        # rough walks may end in the reference's explicit numerical/watchdog or
-        # spheroid-geometry failures: rare on both sides, same rate per code
-        for code in range(1, len(Tg['failures'])):
-            a, b = Tg['failures'][code], To['failures'][code]
-            assert abs(a - b) <= 4 * np.sqrt(a + b + 1), f'failure code {code}: {a} vs {b}'
-        assert Tg['failures'].sum() <= 1e-3 * N
-    else:
-        # the only failure the reference law produces here is the conditional
-        # closure's explicit retry exhaustion (code 15, ~1e-6 per history,
-        # CLARITAS.py:4534); every other code must stay at zero on both sides
-        for T in (Tg, To):
-            others = np.delete(T['failures'], 15)
-            assert others.sum() == 0, f'failures by code {T["failures"]}'
-        a, b = Tg['failures'][15], To['failures'][15]
-        assert abs(a - b) <= 4 * np.sqrt(a + b + 1)
+    det_g = Tg['det'].sum(0)
+    det_o = np.asarray(fx['det'])
+    assert tally_cases.power_ok(det_g) and tally_cases.power_ok(det_o), (det_g[:2, 9], det_o[:2, 9])
+    naux = len(kw.get('aux_detector_angles_deg', ()))
+    a = tally_cases.tally_vector(det_g, Tg['aux_window'].sum(0), naux)
+    b = tally_cases.tally_vector(det_o, fx['aux_window'], naux)
+    ok, rep = tally_cases.compare_counts(a, b)
+    _flush(f'  {case}: {rep}')
+    assert ok, rep
+    # terminal status rates, and failure codes: rare on both sides, equal rates
+    st_g, st_o = Tg['status'], np.asarray(fx['status'])
+    assert st_g.sum() == N and st_o.sum() == N
+    zk = tally_cases.sidak_z(8 + 18)
     for k in range(1, 9):
-        a, b = Tg['status'][k], To['status'][k]
-        assert abs(a - b) <= 4 * np.sqrt(max(a + b, 1)), f'status {k}: {a} vs {b}'
+        z = tally_cases.binomial_z(st_g[k], st_o[k], N)
+        assert abs(z) <= zk, f'status {k}: {st_g[k]} vs {st_o[k]} (z {z:.2f})'
+    f_g, f_o = Tg['failures'], np.asarray(fx['failures'])
+    for code in range(1, len(f_g)):
+        z = tally_cases.binomial_z(f_g[code], f_o[code], N)
+        assert abs(z) <= zk, f'failure code {code}: {f_g[code]} vs {f_o[code]}'
+    if case == 'config1_rough':
+        assert f_g.sum() <= 1e-3 * N
+    else:
+        # these laws fail by the conditional closure's retry exhaustion (code
+        # 15, ~1e-6 per history, CLARITAS.py:4534) and, about once in 1e8
+        # histories, an acrylic chord with no exit intersection at the FP32
+        # edge (code 5); nothing else
+        lim = max(3.0, 3e-8 * N)
+        assert np.delete(f_g, 15).sum() <= lim and np.delete(f_o, 15).sum() <= lim, (f_g, f_o)
 
 
 def test_simulate_forward_result(gpu_available):
