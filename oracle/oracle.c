@@ -288,6 +288,7 @@ static void perpendicular_basis(float vx, float vy, float vz, float* e1x, float*
 /* ======================================================================
  * Model: float parameters as the transport kernel receives them
  * ====================================================================== */
+#define CLR_ORC_MAX_SLABS 32
 typedef struct {
   float n_water, n_acrylic, n_air, n_bottom_ext;
   float rin, rout, zmin, zmean, zwall_top, bdt, zlow;
@@ -303,6 +304,10 @@ typedef struct {
   int orientation;              /* spheroid axis law */
   int src_bore, det_bore, face_model, pd_model, max_bb, max_fb;  /* CLARITAS.py:5558-5562 */
   float n_bore, n_face;
+  /* piecewise-constant majorant over z slabs (orc_set_slab_majorant; 0 slabs:
+   * the single majorant of the reference) */
+  int n_slabs;
+  float slab_mu[CLR_ORC_MAX_SLABS], slab_z0, slab_w, slab_inv_w;
 } fparams;
 
 struct orc_model {
@@ -2896,6 +2901,36 @@ static void light_absorb(const light_t* L, uint64_t gid, float x, float y, float
  * 3 internal reflection, 2 transmission (+4 for the morphology channel) */
 static int light_code(int per, int pir, int morph) { return (per ? 1 : (pir ? 3 : 2)) + (morph ? 4 : 0); }
 
+/* ----------------------------------------------------------------------
+ * Woodcock tracking with a majorant that is constant on z slabs: the
+ * device's tracking of shared FP32 fields (no reference counterpart; the
+ * reference tracks with one majorant, CLARITAS.py:4757-4799).  The optical
+ * depth -log u is spent slab by slab (the exponential is memoryless at the
+ * slab faces) and a collision is kept with probability mu_local / mu_slab.
+ * ---------------------------------------------------------------------- */
+static int slab_of(const fparams* P, float z) {
+  int j = (int)floorf((z - P->slab_z0) * P->slab_inv_w);
+  return j < 0 ? 0 : (j > P->n_slabs - 1 ? P->n_slabs - 1 : j);
+}
+static float slab_free_path(const fparams* P, float z, float vz, float u, float limit) {
+  float tau = -logf(u), t = 0.0f;
+  int j = slab_of(P, z);
+  const int step = (vz > 0.0f) ? 1 : -1;
+  for (int n = 0; n < P->n_slabs; ++n) {
+    float t_exit = INF_F;
+    if (vz > 1e-14f && j < P->n_slabs - 1) t_exit = (P->slab_z0 + (float)(j + 1) * P->slab_w - z) / vz;
+    else if (vz < -1e-14f && j > 0) t_exit = (P->slab_z0 + (float)j * P->slab_w - z) / vz;
+    t_exit = fmaxf(t_exit, t);
+    const float mu = P->slab_mu[j];
+    if (mu > 0.0f && tau < mu * (t_exit - t)) return t + tau / mu;
+    if (t_exit >= limit) return INF_F;
+    tau -= mu * (t_exit - t);
+    t = t_exit;
+    j += step;
+  }
+  return INF_F;
+}
+
 /* source position on the launch plane y = -source_r and direction before
  * the collimator, CLARITAS.py:4650-4680 (the collimated disk has no
  * reference counterpart) */
@@ -3040,7 +3075,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
       float freep = INF_F;
       if (tbound > 0.0f && MU_TRACK > 0.0f) {
         float u = fmaxf(U(&r), 1e-12f);
-        freep = -logf(u) / MU_TRACK;
+        freep = (P->n_slabs > 0) ? slab_free_path(P, p->z, p->vz, u, tbound) : -logf(u) / MU_TRACK;
       }
       int do_particle = freep < tbound;
       float travel = do_particle ? freep : tbound;
@@ -3072,9 +3107,10 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           }
           LT = LG + LW;
           if (!(LT > 0.0f)) { o->n_null++; continue; }
-          float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_TRACK), 1.0f);
-          if (LT > MU_TRACK + slack) { pf = PF_MAJORANT; pfs = SUB_MAJORANT; st = 4; break; }
-          float pacc = LT / fmaxf(MU_TRACK, 1e-30f);
+          const float MU_LOC = (P->n_slabs > 0) ? P->slab_mu[slab_of(P, p->z)] : MU_TRACK;
+          float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_LOC), 1.0f);
+          if (LT > MU_LOC + slack) { pf = PF_MAJORANT; pfs = SUB_MAJORANT; st = 4; break; }
+          float pacc = LT / fmaxf(MU_LOC, 1e-30f);
           if (pacc > 1.0f) pacc = 1.0f;
           if (U(&r) >= pacc) { o->n_null++; continue; }
         }
@@ -3889,5 +3925,41 @@ int orc_probe(const orc_model* m, int mode, int n, uint64_t seed, const float* i
         break;
     }
   }
+  return 0;
+}
+
+/* Slab j covers [zmin + j w, zmin + (j+1) w), w = (field_zmax - zmin) / S,
+ * S = min(32, nz); its majorant is the trace rate times the largest field
+ * node over all r (x, y) and the z nodes whose interpolation weight can
+ * reach the slab (floor(u) and floor(u) + 1 for u = (z - zmin) nz/(zmax -
+ * zmin) - 1/2), widened by one node each side against rounding; the end
+ * slabs reach the clamped edges. */
+int orc_set_slab_majorant(orc_model* m, int on) {
+  if (!m) return CLR_E_ARG;
+  fparams* P = &m->fp;
+  P->n_slabs = 0;
+  if (!on) return 0;
+  const int rz = P->sed_model == CLR_SED_RZ_FIELD, xyz = P->sed_model == CLR_SED_XYZ_FIELD;
+  if (!(rz || xyz) || !m->field) return CLR_E_ARG;
+  const int nz = rz ? P->fn1 : P->fn2;
+  const size_t planes = rz ? (size_t)P->fn0 : (size_t)P->fn0 * P->fn1;
+  const double zmin = m->cfg.zmin, zmax = m->cfg.field_zmax;
+  if (nz < 2 || planes == 0 || !(zmax > zmin)) return CLR_E_ARG;
+  const int S = nz < CLR_ORC_MAX_SLABS ? nz : CLR_ORC_MAX_SLABS;
+  for (int j = 0; j < S; ++j) {
+    int k0 = (int)floor((double)j * nz / S - 0.5) - 1, k1 = (int)floor((double)(j + 1) * nz / S - 0.5) + 2;
+    if (j == 0) k0 = 0;
+    if (j == S - 1) k1 = nz - 1;
+    if (k0 < 0) k0 = 0;
+    if (k1 > nz - 1) k1 = nz - 1;
+    float mx = 0.0f;
+    for (size_t pl = 0; pl < planes; ++pl)
+      for (int k = k0; k <= k1; ++k) mx = fmaxf(mx, m->field[pl * nz + k]);
+    P->slab_mu[j] = (float)(m->sum_trace_g * (double)mx) + (float)(m->sum_trace_w * (double)mx);
+  }
+  P->n_slabs = S;
+  P->slab_z0 = P->zmin;
+  P->slab_w = (float)((zmax - zmin) / S);
+  P->slab_inv_w = (float)(S / (zmax - zmin));
   return 0;
 }

@@ -36,6 +36,11 @@ int orc_get_tables(const orc_model* m, clr_tables_out* out);
  * oracle on exactly the device's float tables in history-by-history tests). */
 int orc_set_phase_table(orc_model* m, const double* theta, const double* cdf, int n_theta);
 
+/* Track a shared FP32 field (sediment models 4, 5) with the device's
+ * piecewise-constant majorant over z slabs instead of the single majorant
+ * (on = 0 restores it). */
+int orc_set_slab_majorant(orc_model* m, int on);
+
 /* Trace n histories; tallies zeroed then filled.  nthreads<=0 uses all. */
 int orc_run(const orc_model* m, uint64_t n, uint64_t seed, uint64_t history_offset,
             int nthreads, clr_tallies* out, clr_record* records, uint64_t n_records);

@@ -87,6 +87,8 @@ def load():
     lib.orc_bore_probe.argtypes = [P(abi.Config), C.c_int, C.c_uint64, P(C.c_float), C.c_float, C.c_float,
                                    P(C.c_int), P(C.c_int), P(C.c_int)]
     lib.orc_bore_probe.restype = C.c_int
+    lib.orc_set_slab_majorant.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_set_slab_majorant.restype = C.c_int
     lib.orc_probe_shape.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
     lib.orc_probe_shape.restype = C.c_int
     lib.orc_probe.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, P(C.c_float), P(C.c_double)]
@@ -265,6 +267,12 @@ class OracleModel:
         t.sed_radial_log = out.ctypes.data_as(PD)
         self.lib.orc_get_tables(self.h, C.byref(t))
         return out
+
+    def use_slab_majorant(self, on=True):
+        """Track the shared field with the device's z-slab majorants."""
+        if self.lib.orc_set_slab_majorant(self.h, int(bool(on))):
+            raise ValueError('slab majorants need a shared r-z or x-y-z field')
+        return self
 
     def probe(self, mode, items, seed=0):
         """orc_probe `mode` (abi.PROBE_MODES name or code) on items [n][IN];

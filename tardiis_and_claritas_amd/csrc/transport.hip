@@ -1681,7 +1681,8 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
   const float disc = b * b - a * c;
   if (!(disc >= 0.0f)) return -1;
   const float t = (-b + sqrtf(disc)) / a;
-  const float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
+  float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
+  if (deg < -90.0f) deg += 360.0f;  // bores lie at 0..170 deg: keep the azimuth continuous past 180
   const int jc = (int)rintf(deg * 0.1f), W = P.det_window;
   float best_t = kInfF, tt;
   int best = -1;

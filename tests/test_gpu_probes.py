@@ -104,10 +104,11 @@ def test_free_surface_probe_against_mpmath(gpu_available, oracle_lib, surface, c
         checked += 1
         assert abs(g[i, 1] - root) < 1e-8, (i, g[i, 1], root)
     assert checked > 30
-    # heights and normals: FP32 evaluation on both sides
-    np.testing.assert_allclose(g[:, 0], o[:, 0], rtol=0, atol=3e-9)
+    # heights and normals: FP32 evaluation on both sides (log1pf, division:
+    # within 2 ulp)
+    np.testing.assert_allclose(g[:, 0], o[:, 0], rtol=2.5e-7, atol=0)
     hit = g[:, 1] < 1e30
-    np.testing.assert_allclose(g[hit, 2], o[hit, 2], rtol=0, atol=3e-9)
+    np.testing.assert_allclose(g[hit, 2], o[hit, 2], rtol=2.5e-7, atol=0)
     np.testing.assert_allclose(g[hit, 3:6], o[hit, 3:6], rtol=0, atol=2e-6)
     # the hit lies on the surface: z(t) equals the surface height there
     zt = rays[hit, 2] + g[hit, 1] * rays[hit, 5]
@@ -125,7 +126,9 @@ def test_wall_probe_against_mpmath(gpu_available, oracle_lib, cell):
     rays = pins.water_rays(m, 2000, seed=5)
     g = m.ctx.probe('wall', rays)
     o = oracle_lib.OracleModel(m, mat, 0.1).probe('wall', rays)
-    exact = pins.wall_exit_mp(rays, m.rin)
+    # the transport's geometry is the FP32 radius (trace_kernel takes float
+    # radii, CLARITAS.py:4551-4605)
+    exact = pins.wall_exit_mp(rays, float(np.float32(m.rin)))
     zh = rays[:, 2] + exact * rays[:, 5]
     inside = (zh > m.zmin + 1e-6) & (zh < m.zwall_top - 1e-6)
     assert inside.sum() > 500
@@ -187,7 +190,8 @@ def test_sediment_probe(gpu_available, oracle_lib, model, surface):
     g = m.ctx.probe('sediment', items)
     o = oracle_lib.OracleModel(m, mat, conc).probe('sediment', items)
     np.testing.assert_allclose(g[:, 0], o[:, 0], rtol=3e-11, atol=1e-300)
-    np.testing.assert_array_equal(g[:, 1], o[:, 1])
+    # FP32 interpolation, contracted differently by the two compilers: 2 ulp
+    np.testing.assert_allclose(g[:, 1], o[:, 1], rtol=2.5e-7, atol=0)
     assert np.all(g[:, 0] >= 0) and g[:, 0].max() > 0
 
 
@@ -204,8 +208,10 @@ def test_sampler_probe(gpu_available, oracle_lib, kind, a, b, law):
     # same words consumed, same value up to the device/host libm difference
     assert np.mean(g[:, 1] == o[:, 1]) >= 0.999
     same = g[:, 1] == o[:, 1]
-    rel = np.abs(g[same, 0] - o[same, 0]) / np.maximum(np.abs(o[same, 0]), 1e-30)
-    assert np.mean(rel < 1e-5) >= 0.999, np.quantile(rel, 0.999)
+    # FP32 transcendentals of the device and host libraries; 1 - (1-u)^(1/b)
+    # cancels near u = 0, so the bound is on the value scale, not relative
+    err = np.abs(g[same, 0] - o[same, 0]) / np.maximum(np.abs(o[same, 0]), 1.0)
+    assert np.mean(err < 2e-6) >= 0.999, np.quantile(err, 0.999)
     cdf = law.cdf if law is not None else pins.beta_radiance_cdf(a, b)
     assert stats.kstest(g[:, 0], cdf).pvalue > 1e-3
 
