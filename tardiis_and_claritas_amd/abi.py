@@ -23,6 +23,9 @@ SUBCODE_VALUES = (0, 10, 11, 12, 13, 14, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
                   38, 39, 40, 41, 1001, 1002, 1003, 1004, 1005, 1101, 1102, 1103, 1104, 1105, 1106, 1107, 2001,
                   2002, 2101, 2102, 2103, 2104, 3001, 3002, 3003, 3004, 3005, 3006, 4001, 9001, 9002, 9003)
 
+# error codes (claritas.h CLR_E_*)
+E_ARG, E_HIP, E_NOMEM, E_STATE, E_RANGE, E_UNSUPPORTED, E_NOT_CONVERGED = -1, -2, -3, -4, -5, -6, -7
+
 # event laws (CLARITAS.py:5543-5550); 6 = plain Mie (no reference code)
 EVENT_MODEL_CODES = {
     'geometric': 0,

@@ -19,3 +19,10 @@ int mlp_grad(MlpState* st, const float* x, const float* y, const float* w, int32
              hipStream_t s, std::string& err);
 int mlp_infer(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s, std::string& err);
 int mlp_time_infer(MlpState* st, int64_t n, int repeats, double* ms, hipStream_t s, std::string& err);
+// Shape of the ensemble; whether the fused inference kernel serves it
+// (hidden <= 256, in_dim <= 32, members * out_dim <= 80); one fused pass over
+// n device-resident rows x[n][in_dim] into mean/stdev[n][out_dim].
+const clr_mlp_shape* mlp_get_shape(const MlpState* st);
+bool mlp_fused_ok(const MlpState* st);
+int mlp_infer_device(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s,
+                     std::string& err);

@@ -349,6 +349,142 @@ __global__ void fill_f32_kernel(float* p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
+// ---- fused ensemble inference ----
+// One workgroup (4 waves) runs every member of the ensemble on 64 query rows
+// with the activations held in LDS: the first layer on the VALU, each hidden
+// matrix product on MFMA (wave w owns output columns 64w .. 64w+63 of all 64
+// rows as 4 x 4 tiles of 16 x 16), the output layer from LDS, then the
+// ensemble mean and spread.  Every value is rounded and summed in the order of
+// the unfused path (first_layer_kernel, gemm_kernel<EPI_INFER>,
+// last_layer_kernel, ensemble_kernel), so both give identical results; only
+// the per-layer activations no longer travel through HBM.  Shapes: hidden
+// <= 256, in_dim <= 32, members * out_dim <= 80 (the config-4 ensemble is
+// 256 / 8 / 15).
+constexpr int kFRows = 64;
+constexpr int kFMaxH = 256, kFMaxIn = 32, kFMaxOutVals = 80, kFMaxLayers = 17;
+
+struct FusedArgs {
+  const float* x;  // [n][din]
+  long long n;
+  int din, H, dout, members, n_layers;
+  const bf16* wbf;
+  const float* params;
+  long long sP;
+  long long w_off[kFMaxLayers], b_off[kFMaxLayers];
+  float* mean;   // [n][dout]
+  float* stdev;  // [n][dout] or nullptr
+};
+
+__device__ __forceinline__ void fused_gemm(const bf16* __restrict__ A, bf16* __restrict__ Y, int S,
+                                           const bf16* __restrict__ W, const float* __restrict__ bias, int H,
+                                           int wave, int lane) {
+  const int n0 = 64 * wave;
+  if (n0 >= H) return;
+  const int r = lane & 15, kb = (lane >> 4) * 8;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16* brow = W + (size_t)(n0 + r) * H + kb;
+  bf16x8 bn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bn[j] = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * j * H);
+  for (int k0 = 0; k0 < H; k0 += kK) {
+    bf16x8 bv[4], av[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bv[j] = bn[j];
+    if (k0 + kK < H) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bn[j] = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * j * H + k0 + kK);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const bf16x8*>(A + (16 * i + r) * S + kb + k0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + 16 * j + r;
+      const float b = bias[n];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = 16 * i + (lane >> 4) * 4 + v;
+        Y[m * S + n] = (bf16)gelu(acc[i][j][v] + b);
+      }
+    }
+}
+
+__global__ __launch_bounds__(256) void fused_infer_kernel(FusedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int S = a.H + 8;  // padded row: rows 0..7 of a fragment fall in distinct LDS banks
+  bf16* act0 = reinterpret_cast<bf16*>(smem);
+  bf16* act1 = act0 + kFRows * S;
+  float* xs = reinterpret_cast<float*>(act1 + kFRows * S);
+  float* outv = xs + kFRows * a.din;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const long long row0 = (long long)blockIdx.x * kFRows;
+  const int nv = (int)min((long long)kFRows, a.n - row0);
+  for (int i = tid; i < kFRows * a.din; i += 256) {
+    const int r = i / a.din, c = i - r * a.din;
+    xs[i] = (r < nv) ? a.x[(row0 + r) * a.din + c] : 0.0f;  // padding rows read 0, as the batched path
+  }
+  __syncthreads();
+  const int L = a.n_layers;
+  for (int m = 0; m < a.members; ++m) {
+    const bf16* Wm = a.wbf + m * a.sP;
+    const float* Pm = a.params + m * a.sP;
+    for (int o = tid; o < a.H; o += 256) {
+      const bf16* W0 = Wm + a.w_off[0] + (size_t)o * a.din;
+      float w[kFMaxIn];
+#pragma unroll
+      for (int i = 0; i < kFMaxIn; ++i) w[i] = (i < a.din) ? (float)W0[i] : 0.0f;
+      const float b0 = Pm[a.b_off[0] + o];
+      for (int r = 0; r < kFRows; ++r) {
+        float z = b0;
+#pragma unroll
+        for (int i = 0; i < kFMaxIn; ++i)
+          if (i < a.din) z += xs[r * a.din + i] * w[i];
+        act0[r * S + o] = (bf16)gelu(z);
+      }
+    }
+    __syncthreads();
+    bf16* cur = act0;
+    bf16* nxt = act1;
+    for (int l = 1; l < L - 1; ++l) {
+      fused_gemm(cur, nxt, S, Wm + a.w_off[l], Pm + a.b_off[l], a.H, wave, lane);
+      __syncthreads();
+      bf16* t = cur; cur = nxt; nxt = t;
+    }
+    for (int t = tid; t < kFRows * a.dout; t += 256) {
+      const int r = t / a.dout, j = t - r * a.dout;
+      const bf16* W = Wm + a.w_off[L - 1] + (size_t)j * a.H;
+      float sum = Pm[a.b_off[L - 1] + j];
+      const bf16* y = cur + r * S;
+      for (int o = 0; o < a.H; ++o) sum += (float)y[o] * (float)W[o];
+      outv[(m * kFRows + r) * a.dout + j] = sum;
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < nv * a.dout; t += 256) {
+    const int r = t / a.dout, j = t - r * a.dout;
+    float sum = 0.0f;
+    for (int m = 0; m < a.members; ++m) sum += outv[(m * kFRows + r) * a.dout + j];
+    const float mu = sum / (float)a.members;
+    float q = 0.0f;
+    for (int m = 0; m < a.members; ++m) {
+      const float d = outv[(m * kFRows + r) * a.dout + j] - mu;
+      q += d * d;
+    }
+    a.mean[(row0 + r) * a.dout + j] = mu;
+    if (a.stdev) a.stdev[(row0 + r) * a.dout + j] = a.members > 1 ? sqrtf(q / (float)(a.members - 1)) : 0.0f;
+  }
+}
+
 inline unsigned int blocks(long long n, int b = 256) { return (unsigned int)((n + b - 1) / b); }
 inline long long round_up(long long v, long long a) { return (v + a - 1) / a * a; }
 
@@ -529,6 +665,36 @@ void refresh_bf16(MlpState* st, hipStream_t s) {
   for (const Tensor& w : st->W)
     hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks((long long)w.out * w.in), M), dim3(256), 0, s, st->params.p,
                        st->wbf.p, st->wtbf.p, w.dev_off, w.out, w.in, st->sP);
+}
+
+bool fused_ok(const MlpState* st) {
+  const clr_mlp_shape& h = st->sh;
+  return h.hidden <= kFMaxH && h.in_dim <= kFMaxIn && h.n_members * h.out_dim <= kFMaxOutVals &&
+         st->n_layers <= kFMaxLayers;
+}
+
+size_t fused_lds_bytes(const MlpState* st) {
+  const clr_mlp_shape& h = st->sh;
+  return (size_t)2 * kFRows * (h.hidden + 8) * sizeof(bf16) + (size_t)kFRows * h.in_dim * sizeof(float) +
+         (size_t)h.n_members * kFRows * h.out_dim * sizeof(float);
+}
+
+// one fused pass over n device-resident query rows (fused_ok(st) holds)
+hipError_t launch_fused(MlpState* st, const float* x, long long n, float* mean, float* stdev, hipStream_t s) {
+  FusedArgs a{};
+  a.x = x; a.n = n; a.din = st->sh.in_dim; a.H = st->sh.hidden; a.dout = st->sh.out_dim;
+  a.members = st->sh.n_members; a.n_layers = st->n_layers; a.wbf = st->wbf.p; a.params = st->params.p;
+  a.sP = st->sP; a.mean = mean; a.stdev = stdev;
+  for (int l = 0; l < st->n_layers; ++l) {
+    a.w_off[l] = st->W[l].dev_off;
+    a.b_off[l] = st->b[l].dev_off;
+  }
+  const size_t lds = fused_lds_bytes(st);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_infer_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fused_infer_kernel, dim3((unsigned)((n + kFRows - 1) / kFRows)), dim3(256), lds, s, a);
+  return hipGetLastError();
 }
 
 // host flat layout <-> device padded layout, tensor by tensor
@@ -717,8 +883,10 @@ constexpr int kInferChunk = 65536;  // rows per forward pass of inference
 int mlp_infer(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s, std::string& err) {
   if (n < 1) { err = "inference needs n >= 1"; return CLR_E_ARG; }
   const int M = st->sh.n_members, dout = st->sh.out_dim;
+  const bool fused = fused_ok(st);
   const int chunk = (int)std::min<int64_t>(kInferChunk, round_up(n, kTile));
-  if (int rc = ensure_batch(st, chunk, err)) return rc;
+  if (!fused)
+    if (int rc = ensure_batch(st, chunk, err)) return rc;
   MLPCHK(st->x_d.ensure((size_t)chunk * st->sh.in_dim));
   MLPCHK(st->mean_d.ensure((size_t)chunk * dout));
   MLPCHK(st->std_d.ensure((size_t)chunk * dout));
@@ -727,9 +895,13 @@ int mlp_infer(MlpState* st, const float* x, int64_t n, float* mean, float* stdev
     const int B = (int)round_up(nv, kTile);
     MLPCHK(hipMemcpyAsync(st->x_d.p, x + (size_t)r0 * st->sh.in_dim, sizeof(float) * nv * st->sh.in_dim,
                           hipMemcpyHostToDevice, s));
-    forward(st, st->x_d.p, nullptr, 0, 0, nv, B, false, s);
-    hipLaunchKernelGGL(ensemble_kernel, dim3(blocks((long long)nv * dout)), dim3(256), 0, s, st->out.p, M, B, dout,
-                       nv, st->mean_d.p, st->std_d.p);
+    if (fused) {
+      MLPCHK(launch_fused(st, st->x_d.p, nv, st->mean_d.p, st->std_d.p, s));
+    } else {
+      forward(st, st->x_d.p, nullptr, 0, 0, nv, B, false, s);
+      hipLaunchKernelGGL(ensemble_kernel, dim3(blocks((long long)nv * dout)), dim3(256), 0, s, st->out.p, M, B, dout,
+                         nv, st->mean_d.p, st->std_d.p);
+    }
     MLPCHK(hipGetLastError());
     MLPCHK(hipMemcpyAsync(mean + (size_t)r0 * dout, st->mean_d.p, sizeof(float) * nv * dout, hipMemcpyDeviceToHost, s));
     if (stdev)
@@ -747,24 +919,43 @@ int mlp_time_infer(MlpState* st, int64_t n, int repeats, double* ms, hipStream_t
   }
   const int M = st->sh.n_members, dout = st->sh.out_dim;
   const int B = (int)round_up(n, kTile);
-  if (int rc = ensure_batch(st, B, err)) return rc;
+  const bool fused = fused_ok(st);
+  if (!fused)
+    if (int rc = ensure_batch(st, B, err)) return rc;
   MLPCHK(st->x_d.ensure((size_t)B * st->sh.in_dim));
   MLPCHK(st->mean_d.ensure((size_t)B * dout));
   MLPCHK(st->std_d.ensure((size_t)B * dout));
   hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks((long long)B * st->sh.in_dim)), dim3(256), 0, s, st->x_d.p,
                      (long long)B * st->sh.in_dim, 0.25f);
-  forward(st, st->x_d.p, nullptr, 0, 0, (int)n, B, false, s);  // warm-up
-  MLPCHK(hipEventRecord(st->e0, s));
-  for (int r = 0; r < repeats; ++r) {
+  auto pass = [&]() -> hipError_t {
+    if (fused) return launch_fused(st, st->x_d.p, n, st->mean_d.p, st->std_d.p, s);
     forward(st, st->x_d.p, nullptr, 0, 0, (int)n, B, false, s);
     hipLaunchKernelGGL(ensemble_kernel, dim3(blocks((long long)n * dout)), dim3(256), 0, s, st->out.p, M, B, dout,
                        (int)n, st->mean_d.p, st->std_d.p);
-  }
+    return hipGetLastError();
+  };
+  MLPCHK(pass());  // warm-up
+  MLPCHK(hipEventRecord(st->e0, s));
+  for (int r = 0; r < repeats; ++r) MLPCHK(pass());
   MLPCHK(hipEventRecord(st->e1, s));
   MLPCHK(hipEventSynchronize(st->e1));
   MLPCHK(hipGetLastError());
   float t = 0.0f;
   MLPCHK(hipEventElapsedTime(&t, st->e0, st->e1));
   *ms = (double)t / repeats;
+  return 0;
+}
+
+const clr_mlp_shape* mlp_get_shape(const MlpState* st) { return &st->sh; }
+bool mlp_fused_ok(const MlpState* st) { return fused_ok(st); }
+
+int mlp_infer_device(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s,
+                     std::string& err) {
+  if (!fused_ok(st)) {
+    err = "device-resident inference needs the fused shape: hidden <= 256, in_dim <= 32, members * out_dim <= 80";
+    return CLR_E_UNSUPPORTED;
+  }
+  if (n < 1) { err = "inference needs n >= 1"; return CLR_E_ARG; }
+  MLPCHK(launch_fused(st, x, n, mean, stdev, s));
   return 0;
 }
