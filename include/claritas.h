@@ -646,6 +646,17 @@ int clr_mlp_grad(clr_ctx* ctx, const float* x, const float* y, const float* w,
 /* Inference: mean[n][out_dim] and std[n][out_dim] over the members. */
 int clr_mlp_infer(clr_ctx* ctx, const float* x, int64_t n, float* mean,
                   float* stdev);
+/* Inference that allocates no device memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same results as
+ * clr_mlp_infer (the same fused kernel); the queries and outputs are staged
+ * in the library's static 4 MiB device array, one per GPU, so one call takes
+ * at most n rows with n * (in_dim + 2 out_dim) * 4 bytes <= 4 MiB - 768
+ * (74,898 queries of the config-4 ensemble) and CLR_E_RANGE beyond that;
+ * ensembles outside the fused shape (hidden <= 256, in_dim <= 32,
+ * members * out_dim <= 80) get CLR_E_UNSUPPORTED.  Calls that overlap on
+ * one GPU share the array and race. */
+int clr_mlp_infer_static(clr_ctx* ctx, const float* x, int64_t n, float* mean, float* stdev);
 /* Device-resident inference timing: n queries, `repeats` launches; *ms is
  * the average per launch (HIP events). */
 int clr_mlp_time_infer(clr_ctx* ctx, int64_t n, int repeats, double* ms);

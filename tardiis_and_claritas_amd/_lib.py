@@ -56,6 +56,7 @@ _SIGS = {
                                P(C.c_float)]),
     'clr_mlp_infer': (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
     'clr_mlp_time_infer': (C.c_int, [C.c_void_p, C.c_int64, C.c_int, P(C.c_double)]),
+    'clr_mlp_infer_static': (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]),
 }
 EXPORTED = tuple(_SIGS)
 

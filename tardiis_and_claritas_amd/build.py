@@ -24,7 +24,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / 'csrc'
 LIB = HERE / 'libclaritas_hip.so'
 ARCH = os.environ.get('CLARITAS_OFFLOAD_ARCH', 'gfx950')
-SOURCES = ['mie.hip', 'precompute.hip', 'meridional.hip', 'transport.hip', 'mlp.hip', 'api.cpp', 'causal_bin.cpp']
+SOURCES = ['mie.hip', 'precompute.hip', 'meridional.hip', 'transport.hip', 'mlp.hip', 'mlp_static.hip', 'api.cpp',
+           'causal_bin.cpp']
 FLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-fPIC', '-std=c++17', '-munsafe-fp-atomics',
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',

@@ -109,6 +109,7 @@ struct clr_ctx {
   double last_ms = 0.0;
   GraphCache graph;
   MlpState* mlp = nullptr;
+  void* static_scratch = nullptr;  // this device's copy of mlp_static.hip's staging array
   LightBufs light;
   // causal provenance records of the run in progress (clr_run_causal)
   DevBuf causal;
@@ -881,6 +882,10 @@ int clr_create(clr_ctx** out, int device) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  if (mlp_static_scratch(device, &c->static_scratch, c->err) != 0) {
+    clr_destroy(c);
+    return CLR_E_HIP;
+  }
   *out = c;
   return 0;
 }
@@ -1409,6 +1414,12 @@ int clr_mlp_infer(clr_ctx* c, const float* x, int64_t n, float* mean, float* std
   if (!c || !c->mlp || !x || !mean) return fail(c, CLR_E_STATE, "no ensemble");
   HIPCHK(c, hipSetDevice(c->device));
   return mlp_infer(c->mlp, x, n, mean, stdev, c->stream, c->err);
+}
+int clr_mlp_infer_static(clr_ctx* c, const float* x, int64_t n, float* mean, float* stdev) {
+  if (!c || !c->mlp || !x || !mean) return fail(c, CLR_E_STATE, "no ensemble");
+  if (!c->static_scratch) return fail(c, CLR_E_STATE, "the static staging array is not set up on this device");
+  HIPCHK(c, hipSetDevice(c->device));
+  return mlp_infer_static(c->mlp, c->static_scratch, x, n, mean, stdev, c->stream, c->err);
 }
 int clr_mlp_time_infer(clr_ctx* c, int64_t n, int repeats, double* ms) {
   if (!c || !c->mlp || !ms) return fail(c, CLR_E_STATE, "no ensemble");

@@ -26,3 +26,11 @@ const clr_mlp_shape* mlp_get_shape(const MlpState* st);
 bool mlp_fused_ok(const MlpState* st);
 int mlp_infer_device(MlpState* st, const float* x, int64_t n, float* mean, float* stdev, hipStream_t s,
                      std::string& err);
+// Allocation-free inference staged in the library's static device array
+// (mlp_static.hip): the array's address on `device` (hipGetSymbolAddress,
+// cached per device), the largest query count one call can stage, and the
+// inference itself.
+int mlp_static_scratch(int device, void** out, std::string& err);
+size_t mlp_static_max_rows(const MlpState* st);
+int mlp_infer_static(MlpState* st, void* scratch_ptr, const float* x, int64_t n, float* mean, float* stdev,
+                     hipStream_t s, std::string& err);
