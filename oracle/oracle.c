@@ -307,7 +307,7 @@ typedef struct {
   /* piecewise-constant majorant over z slabs (orc_set_slab_majorant; 0 slabs:
    * the single majorant of the reference) */
   int n_slabs;
-  float slab_mu[CLR_ORC_MAX_SLABS], slab_z0, slab_w, slab_inv_w;
+  float slab_mu[CLR_ORC_MAX_SLABS], slab_z0, slab_w, slab_inv_w, slab_mu_max;
 } fparams;
 
 struct orc_model {
@@ -2914,12 +2914,15 @@ static int slab_of(const fparams* P, float z) {
 }
 static float slab_free_path(const fparams* P, float z, float vz, float u, float limit) {
   float tau = -logf(u), t = 0.0f;
+  /* no slab majorant exceeds slab_mu_max: no collision before limit */
+  if (!(tau < P->slab_mu_max * limit)) return INF_F;
   int j = slab_of(P, z);
   const int step = (vz > 0.0f) ? 1 : -1;
+  const float ivz = 1.0f / vz;
   for (int n = 0; n < P->n_slabs; ++n) {
     float t_exit = INF_F;
-    if (vz > 1e-14f && j < P->n_slabs - 1) t_exit = (P->slab_z0 + (float)(j + 1) * P->slab_w - z) / vz;
-    else if (vz < -1e-14f && j > 0) t_exit = (P->slab_z0 + (float)j * P->slab_w - z) / vz;
+    if (vz > 1e-14f && j < P->n_slabs - 1) t_exit = (P->slab_z0 + (float)(j + 1) * P->slab_w - z) * ivz;
+    else if (vz < -1e-14f && j > 0) t_exit = (P->slab_z0 + (float)j * P->slab_w - z) * ivz;
     t_exit = fmaxf(t_exit, t);
     const float mu = P->slab_mu[j];
     if (mu > 0.0f && tau < mu * (t_exit - t)) return t + tau / mu;
@@ -3958,6 +3961,8 @@ int orc_set_slab_majorant(orc_model* m, int on) {
     P->slab_mu[j] = (float)(m->sum_trace_g * (double)mx) + (float)(m->sum_trace_w * (double)mx);
   }
   P->n_slabs = S;
+  P->slab_mu_max = 0.0f;
+  for (int j = 0; j < S; ++j) P->slab_mu_max = fmaxf(P->slab_mu_max, P->slab_mu[j]);
   P->slab_z0 = P->zmin;
   P->slab_w = (float)((zmax - zmin) / S);
   P->slab_inv_w = (float)(S / (zmax - zmin));

@@ -358,6 +358,12 @@ void fill_params(Slot& s, const clr_config& g) {
     P.det_uy[j] = det_u(10.0 * j, true);
   }
   P.det_window = window_for_cell(g.ring_in, g.throat_out, g.throat_r);
+  {
+    // cos(atan(2 r_t / D)) with D the distance between the throat planes,
+    // lowered by 1e-4 so that rounding never rejects a ray the apertures pass
+    const double D = g.throat_out - g.ring_in, w = 2.0 * g.throat_r;
+    P.det_cos_min = (float)(D / std::sqrt(D * D + w * w) - 1e-4);
+  }
   for (int a = 0; a < CLR_MAX_AUX; ++a) {
     P.aux_ux[a] = det_u(g.aux_angles_deg[a], false);
     P.aux_uy[a] = det_u(g.aux_angles_deg[a], true);
@@ -431,6 +437,8 @@ void fill_params(Slot& s, const clr_config& g) {
     for (int j = 0; j < S; ++j)
       P.slab_mu[j] = (float)(sums[0] * (double)s.slab_max[j]) + (float)(sums[1] * (double)s.slab_max[j]);
   }
+  P.slab_mu_max = 0.0f;
+  for (int j = 0; j < P.n_slabs; ++j) P.slab_mu_max = std::max(P.slab_mu_max, P.slab_mu[j]);
   // inverse cell sizes of the shared fields, in float as the oracle computes them
   P.field_inv0 = P.field_inv1 = P.field_inv2 = 0.0f;
   if (g.sediment_model == CLR_SED_RZ_FIELD && g.field_n0 > 0 && g.field_n1 > 0) {

@@ -34,13 +34,15 @@ struct DevParams {
   // [slab_z0 + j w, slab_z0 + (j+1) w) (the end slabs are open), n_slabs = 0
   // when the single majorant mu_track is used
   float slab_mu[CLR_MAJ_SLABS];
-  float slab_z0, slab_w, slab_inv_w;
+  float slab_z0, slab_w, slab_inv_w, slab_mu_max;
   int n_slabs;
   int orientation;                // spheroid axis law (0 isotropic, 1 local-tangential vMF)
   int n_groups, n_bins, n_theta, n_radial;
   int fn0, fn1, fn2;
   // bores tested either side of the exit azimuth's nearest bore (hardware_detector)
   int det_window;
+  // least alignment u.v of a ray that can pass both throat apertures of a bore
+  float det_cos_min;
   // per-bin tables
   const float* radius;
   const float* n_real;

@@ -691,23 +691,25 @@ __device__ __forceinline__ int acrylic_next_surface(const DevParams& P, float x,
     const double zh = (double)z + r1 * (double)vz;
     if (zh >= (double)P.zlow - tol && zh <= (double)P.zwall_top + tol && r1 > 0.0 && r1 < best) { best = r1; surf = 2; }
   }
+  // planes: the radius of the crossing is needed only for a nearer hit
   if (vz < 0.0f) {
     const double t = ((double)P.zlow - (double)z) / (double)vz;
-    if (t > 0.0 && isfinite(t)) {
+    if (t > 0.0 && isfinite(t) && t < best) {
       const double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy;
-      if (hypot(X, Y) <= (double)P.rout + tol && t < best) { best = t; surf = 3; }
+      if (hypot(X, Y) <= (double)P.rout + tol) { best = t; surf = 3; }
     }
   }
   if (vz > 0.0f) {
-    const double t = ((double)P.zwall_top - (double)z) / (double)vz;
-    if (t > 0.0 && isfinite(t)) {
+    const double iv = 1.0 / (double)vz;
+    const double t = ((double)P.zwall_top - (double)z) * iv;
+    if (t > 0.0 && isfinite(t) && t < best) {
       const double X = (double)x + t * (double)vx, Y = (double)y + t * (double)vy, rr = hypot(X, Y);
-      if (rr >= (double)P.rin - tol && rr <= (double)P.rout + tol && t < best) { best = t; surf = 4; }
+      if (rr >= (double)P.rin - tol && rr <= (double)P.rout + tol) { best = t; surf = 4; }
     }
-    const double ta = ((double)P.zmin - (double)z) / (double)vz;
-    if (ta > 0.0 && isfinite(ta)) {
+    const double ta = ((double)P.zmin - (double)z) * iv;
+    if (ta > 0.0 && isfinite(ta) && ta < best) {
       const double X = (double)x + ta * (double)vx, Y = (double)y + ta * (double)vy;
-      if (hypot(X, Y) < (double)P.rin - tol && ta < best) { best = ta; surf = 5; }
+      if (hypot(X, Y) < (double)P.rin - tol) { best = ta; surf = 5; }
     }
   }
   t_out = best;
@@ -1646,24 +1648,36 @@ __device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r,
 }
 
 // --------------------------------------------------------------- detectors --
-// three-plane hard aperture of one bore, CLARITAS.py:4237-4255
+// three-plane hard aperture of one bore, CLARITAS.py:4237-4255.  The ray
+// meets plane k (u.p = a_k: ring_in, throat_out, ring_out) at
+// t_k = (a_k - u.p) / (u.v); with e_k = a_k - u.p the offset from the axis
+// point there, times u.v > 0, is (p - a_k u) (u.v) + e_k v, so each aperture
+// test compares squared lengths scaled by (u.v)^2 and needs no division
+// (the order t2 > t1 > t0 > 0 reduces to e_0 > 0 since ring_out > throat_out
+// > ring_in).  A ray through both throats of radius r_t, a distance D apart
+// along the axis, makes an angle with the axis of at most atan(2 r_t / D):
+// rays less aligned than that (P.det_cos_min, with a margin) are rejected
+// first.  Only an accepted ray's t_0 is divided out.
 __device__ __forceinline__ bool bore_accept(const DevParams& P, float x, float y, float z, float vx,
                                             float vy, float vz, float ux, float uy, float& t_out) {
   const float den = vx * ux + vy * uy;
-  if (den <= 1e-10f) return false;
+  if (den <= 1e-10f || den < P.det_cos_min) return false;
   const float pdot = x * ux + y * uy;
-  const float t0 = (P.ring_in - pdot) / den, t1 = (P.throat_out - pdot) / den, t2 = (P.ring_out - pdot) / den;
-  if (!(t0 > 0.0f && t1 > t0 && t2 > t1)) return false;
-  const float rth2 = P.throat_r * P.throat_r, rcb2 = P.cb_r * P.cb_r;
-  const float q0x = x + t0 * vx - P.ring_in * ux, q0y = y + t0 * vy - P.ring_in * uy, q0z = z + t0 * vz;
-  const float q1x = x + t1 * vx - P.throat_out * ux, q1y = y + t1 * vy - P.throat_out * uy, q1z = z + t1 * vz;
-  const float q2x = x + t2 * vx - P.ring_out * ux, q2y = y + t2 * vy - P.ring_out * uy, q2z = z + t2 * vz;
-  if (q0x * q0x + q0y * q0y + q0z * q0z <= rth2 && q1x * q1x + q1y * q1y + q1z * q1z <= rth2 &&
-      q2x * q2x + q2y * q2y + q2z * q2z <= rcb2) {
-    t_out = t0;
-    return true;
-  }
-  return false;
+  const float e0 = P.ring_in - pdot;
+  if (!(e0 > 0.0f)) return false;
+  const float rth2 = P.throat_r * P.throat_r * (den * den), rcb2 = P.cb_r * P.cb_r * (den * den);
+  const float xd = x * den, yd = y * den, zd = z * den;
+  const float ax = ux * den, ay = uy * den;
+  const float q0x = xd - P.ring_in * ax + e0 * vx, q0y = yd - P.ring_in * ay + e0 * vy, q0z = zd + e0 * vz;
+  if (!(q0x * q0x + q0y * q0y + q0z * q0z <= rth2)) return false;
+  const float e1 = P.throat_out - pdot;
+  const float q1x = xd - P.throat_out * ax + e1 * vx, q1y = yd - P.throat_out * ay + e1 * vy, q1z = zd + e1 * vz;
+  if (!(q1x * q1x + q1y * q1y + q1z * q1z <= rth2)) return false;
+  const float e2 = P.ring_out - pdot;
+  const float q2x = xd - P.ring_out * ax + e2 * vx, q2y = yd - P.ring_out * ay + e2 * vy, q2z = zd + e2 * vz;
+  if (!(q2x * q2x + q2y * q2y + q2z * q2z <= rcb2)) return false;
+  t_out = e0 / den;
+  return true;
 }
 // First bore (smallest t0) whose three apertures the exit ray passes.  Only
 // the bores within P.det_window of the bore nearest to the azimuth where the
@@ -2492,13 +2506,17 @@ __device__ __forceinline__ int slab_of(const DevParams& P, float z) {
 // kInfF when the path reaches `limit` first.
 __device__ __forceinline__ float slab_free_path(const DevParams& P, float z, float vz, float u, float limit) {
   float tau = -logf(fmaxf(u, 1e-12f));
+  // every slab's majorant is at most slab_mu_max: a depth that the path
+  // cannot spend before `limit` even at that rate means no collision first
+  if (!(tau < P.slab_mu_max * limit)) return kInfF;
   int j = slab_of(P, z);
   const int step = (vz > 0.0f) ? 1 : -1;
+  const float ivz = 1.0f / vz;
   float t = 0.0f;
   for (int n = 0; n < P.n_slabs; ++n) {
     float t_exit = kInfF;
-    if (vz > 1e-14f && j < P.n_slabs - 1) t_exit = (P.slab_z0 + (float)(j + 1) * P.slab_w - z) / vz;
-    else if (vz < -1e-14f && j > 0) t_exit = (P.slab_z0 + (float)j * P.slab_w - z) / vz;
+    if (vz > 1e-14f && j < P.n_slabs - 1) t_exit = (P.slab_z0 + (float)(j + 1) * P.slab_w - z) * ivz;
+    else if (vz < -1e-14f && j > 0) t_exit = (P.slab_z0 + (float)j * P.slab_w - z) * ivz;
     t_exit = fmaxf(t_exit, t);
     const float mu = P.slab_mu[j];
     if (mu > 0.0f && tau < mu * (t_exit - t)) return t + tau / mu;
@@ -2542,14 +2560,21 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   // uses no random numbers) lets the surface be skipped exactly when the
   // segment up to min(free path, tc) stays below its lowest point: then the
   // surface can neither come first nor change the outcome of the flight.
-  const bool early = (tside > 0.0f && tc < 0.5f * kInfF && MU_TRACK > 0.0f);
+  // With tside > 0 the flight always draws its free path (the boundary is
+  // at a positive distance): a vertical ray (tc infinite) has the surface
+  // ahead, and the draw's outcome against the boundary does not depend on
+  // which of the two limits it was measured with.  With tside = 0 the ray is
+  // leaving through the wall here and draws nothing.
+  const bool early = (tside > 0.0f && MU_TRACK > 0.0f);
   const bool slabs = P.n_slabs > 0;
   float freep = kInfF;
   if (early) freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tc) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   {
+    // the surface may be skipped only when the side wall or the bottom
+    // bounds the flight (tc finite): otherwise it is the only boundary
     const float d = fminf(freep, tc);
     bool skip = false;
-    if (P.fs_model != 0 && d < 0.5f * kInfF) skip = fmaxf(L.z, L.z + d * L.vz) < P.zs_min;
+    if (P.fs_model != 0 && tc < 0.5f * kInfF) skip = fmaxf(L.z, L.z + d * L.vz) < P.zs_min;
     if (!skip) ttop = free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
   }
   if (!(ttop > 0.0f)) ttop = kInfF;
@@ -2557,8 +2582,6 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   const bool axial_bot = (tbot < ttop && tbot < tside);
   const float tbound = axial_top ? ttop : (axial_bot ? tbot : tside);
   if (tbound >= kInfF * 0.5f) { terminate(L, 4, PF_WATER_NO_BOUNDARY, SUB_WATER_NO_BOUNDARY); return; }
-  if (!early && tbound > 0.0f && MU_TRACK > 0.0f)
-    freep = slabs ? slab_free_path(P, L.z, L.vz, U(L.r), tbound) : -logf(fmaxf(U(L.r), 1e-12f)) / MU_TRACK;
   const bool do_particle = freep < tbound;
   const float travel = do_particle ? freep : tbound;
   if constexpr (kKind == CLR_KERNEL_LIGHT) {
