@@ -333,7 +333,8 @@ def _cat(parts):
 
 
 def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
-                 test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True):
+                 test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True,
+                 generate_only=False, hi=200_000_000):
     """Generate (n_optical x n_variants training and n_test_optical x
     n_test_variants held-out configurations; 0 skips), merge earlier
     `datasets` (paths of dataset.npz), train, evaluate; writes summary.json,
@@ -354,7 +355,7 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
             for v in range(n_var):
                 params = {k: np.asarray(p_opt[k])[sel] for k in OPTICAL}
                 params.update(transport_params(len(sel), g))
-                budget = photon_budget(params['conc_mg_l'], cnt)
+                budget = photon_budget(params['conc_mg_l'], cnt, hi=hi)
                 specs = [workloads.config3_item(params, i) for i in range(len(sel))]
                 runner.prepare(specs)
                 T = runner.run_prepared(budget.tolist(), sd + 1_000_003 * (v + 1) + a0)
@@ -376,6 +377,12 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
     np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr, train_hist=n_tr,
              **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te, test_hist=n_te)
     mc_rate = (h_tr + h_te) / (s_tr + s_te) if (s_tr + s_te) > 0 else None
+    if generate_only:
+        summary = dict(n_train=len(S_tr), n_test=len(S_te), train_histories=int(n_tr.sum()),
+                       test_histories=int(n_te.sum()), mc_histories_per_s=mc_rate, seconds=time.time() - t0)
+        (out / 'summary.json').write_text(json.dumps(summary, indent=1))
+        log(json.dumps(summary))
+        return summary
     for path in datasets:
         a_P, a_S, a_n, b_P, b_S, b_n = load_dataset(path, counts, test_counts)
         P_tr, S_tr, n_tr = _cat([P_tr, a_P]) if len(S_tr) else a_P, np.concatenate([S_tr, a_S]), \
@@ -447,6 +454,9 @@ if __name__ == '__main__':
     ap.add_argument('--seed', type=int, default=20261204)
     ap.add_argument('--dataset', action='append', default=[], help='merge an earlier dataset.npz (repeatable)')
     ap.add_argument('--unweighted', action='store_true', help='plain MSE instead of inverse-variance weights')
+    ap.add_argument('--generate-only', action='store_true', help='write dataset.npz and stop (no training)')
+    ap.add_argument('--max-photons', type=float, default=2e8, help='photon budget cap per configuration')
     a = ap.parse_args()
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
-                 a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted)
+                 a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted,
+                 generate_only=a.generate_only, hi=int(a.max_photons))
