@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CLR_ABI_VERSION 10
+#define CLR_ABI_VERSION 11
 
 #define CLR_N_CHANNELS 18      /* detectors at 0..170 deg, CLARITAS.py:420 */
 #define CLR_MAX_GROUPS 64      /* jackknife batches per configuration */
@@ -114,7 +114,8 @@ enum {
   CLR_C_N_IFACE,           /* Fresnel interface decisions */
   CLR_C_N_ABS,             /* absorption roulette samples */
   CLR_C_N_EXIT,            /* scored air exits */
-  CLR_C_COND_TRIALS,       /* conditional-closure candidate trials */
+  CLR_C_COND_TRIALS,       /* conditional-law candidate trials: closure trials plus one per
+                              unconditional encounter (CLARITAS.py conditional_stats[0]) */
   CLR_C_COND_REJECT,       /* conditional outcome rejections */
   CLR_C_COND_INACCESSIBLE, /* inaccessible particle placements */
   CLR_C_RNG_DRAWS,         /* 32-bit Philox words consumed */
@@ -123,6 +124,15 @@ enum {
   CLR_C_SRC_BORE_INTER,    /* source bore wall interactions */
   CLR_C_SRC_BORE_REFL,     /* source bore wall reflections */
   CLR_C_SRC_REFL_ACCEPT,   /* histories through the collimator after >= 1 reflection */
+  /* the rest of the conditional law's sampler counts, CLARITAS.py:9119-9125
+     (conditional_stats[3..9]; counted only under hybrid_mie_exact_rate) */
+  CLR_C_COND_ACC_GEOM,     /* accepted geometric-channel encounters */
+  CLR_C_COND_ACC_COHERENT, /* accepted coherent-wave encounters */
+  CLR_C_COND_ACC_MORPH,    /* accepted morphology-wave encounters */
+  CLR_C_COND_ACC_ABS,      /* accepted encounters that absorb */
+  CLR_C_COND_ACC_SCAT,     /* accepted encounters that scatter */
+  CLR_C_COND_NUMERICAL,    /* encounters ended by a numerical failure */
+  CLR_C_COND_EXHAUSTED,    /* closure calls that ran out of trials */
   CLR_C_COUNT_
 };
 

@@ -2565,6 +2565,10 @@ typedef struct {
   photon_t ph;
   uint32_t n_flight, n_null, trials, rejects, inacc, steps;
   uint32_t draws;
+  /* conditional-law sampler outcomes, CLARITAS.py:4491-4536, 4854-4933:
+     accepted geometric / coherent / morphology, accepted absorbing /
+     scattering, numerical failures, trial exhaustions */
+  uint32_t cs[7];
   int native, mirror;
   int aux_native[CLR_MAX_AUX], aux_mirror[CLR_MAX_AUX];
   int auxw_native[CLR_MAX_AUX], auxw_mirror[CLR_MAX_AUX];
@@ -2584,6 +2588,17 @@ typedef struct {
     int m_type, m_order, m_bin, m_macro, m_micro;
   } pv;
 } hist_t;
+
+/* a closure call's outcome in the sampler counts (every trial's own
+   outcome is in h->trials / rejects / inaccessible) */
+static void count_closure(hist_t* o, const cond_t* h) {
+  if (h->status == 1 || h->status == 2) {
+    o->cs[h->channel]++;
+    o->cs[h->status == 2 ? 3 : 4]++;
+  } else {
+    o->cs[h->subcode == 9001 ? 6 : 5]++;
+  }
+}
 
 static void prov_reset(hist_t* o) {
   memset(&o->pv, 0, sizeof(o->pv));
@@ -3146,6 +3161,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         if (cond && m->ki[sel] > 0.0) {
           cond_t h = conditional_event(m, &r, sel, p->x, p->y, p->z, p->vx, p->vy, p->vz, P->cond_trials);
           o->trials += h.trials; o->rejects += h.rejects; o->inacc += h.inaccessible; o->pc.n_abs += h.pc.n_abs;
+          count_closure(o, &h);
           if (h.status != 1 && h.status != 2) { pf = h.failure ? h.failure : PF_OTHER; pfs = h.subcode; st = 4; break; }
           p->x = h.x; p->y = h.y; p->z = h.z; p->vx = h.vx; p->vy = h.vy; p->vz = h.vz;
           o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
@@ -3163,6 +3179,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->wic++;
           int pidx = sel >= 0 ? sel : select_bin(&r, m, 1, p->x, p->y, p->z, LW);
           int coherent = (P->coherent >= 1.0f) || (U(&r) < P->coherent);
+          if (cond) o->trials++;
           if (!coherent) {
             /* morphology-wave event about the local azimuthal flow direction */
             float fr = sqrtf(p->x * p->x + p->y * p->y), ffx = 1.0f, ffy = 0.0f;
@@ -3170,6 +3187,8 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
             int ppf = 0, ppfs = 0, per_b = o->pc.per, pir_b = o->pc.pir;
             int ps = wave_particle_event(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], m->r_f[pidx], ffx, ffy, 0.0f,
                                          &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
+            if (cond && ps == 1) { o->cs[2]++; o->cs[4]++; }
+            if (cond && ps != 1 && ps != 2) o->cs[5]++;
             if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
             if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
             if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 1));
@@ -3177,12 +3196,14 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
             continue;
           }
           o->wcoh++;
+          if (cond) { o->cs[1]++; o->cs[4]++; }
           mie_scatter(&r, m, pidx, &p->vx, &p->vy, &p->vz);
           if (Ld) light_event(Ld, hist, sx0, sy0, sz0, 4);
           prov_event(m, o, vin, site, 4, pidx, &none);
           continue;
         }
         o->gic++;
+        if (cond) o->trials++;
         int pidx = sel >= 0 ? sel : select_bin(&r, m, 0, p->x, p->y, p->z, LG);
         float rad = m->r_f[pidx], gx, gy, gz;
         sphere_sample(&r, p->vx, p->vy, p->vz, &gx, &gy, &gz);
@@ -3194,6 +3215,7 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
           o->inacc++;
           cond_t h = conditional_event(m, &r, pidx, p->x, p->y, p->z, p->vx, p->vy, p->vz, P->cond_trials - 1);
           o->trials += h.trials; o->rejects += h.rejects; o->inacc += h.inaccessible; o->pc.n_abs += h.pc.n_abs;
+          count_closure(o, &h);
           if (h.status != 1) { pf = h.failure ? h.failure : PF_OTHER; pfs = h.subcode; st = 4; break; }
           p->x = h.x; p->y = h.y; p->z = h.z; p->vx = h.vx; p->vy = h.vy; p->vz = h.vz;
           o->pc.pr += h.pc.pr; o->pc.per += h.pc.per; o->pc.pir += h.pc.pir; o->pc.pabs += h.pc.pabs;
@@ -3207,6 +3229,8 @@ static void trace_history(const orc_model* m, uint64_t seed, uint64_t hist, hist
         int ppf = 0, ppfs = 0, per_b = o->pc.per, pir_b = o->pc.pir;
         int ps = sphere_interaction(P, &r, P->n_water, m->n_f[pidx], m->k_f[pidx], rad, P->max_pb, gx, gy, gz,
                                     dcx, dcy, dcz, &p->x, &p->y, &p->z, &p->vx, &p->vy, &p->vz, &o->pc, &ppf, &ppfs);
+        if (cond && ps == 1) { o->cs[0]++; o->cs[4]++; }
+        if (cond && ps != 1 && ps != 2) o->cs[5]++;
         if (ps == 2) { if (Ld) light_mark(Ld, hist, 9); st = 5; break; }
         if (ps != 1) { if (Ld) light_mark(Ld, hist, 10); pf = ppf ? ppf : PF_OTHER; pfs = ppfs; st = 4; break; }
         if (Ld) light_event(Ld, hist, sx0, sy0, sz0, light_code(o->pc.per > per_b, o->pc.pir > pir_b, 0));
@@ -3404,6 +3428,7 @@ static void tally(clr_tallies* T, const hist_t* h, int g) {
   T->counters[CLR_C_COND_TRIALS] += h->trials;
   T->counters[CLR_C_COND_REJECT] += h->rejects;
   T->counters[CLR_C_COND_INACCESSIBLE] += h->inacc;
+  for (int k = 0; k < 7; ++k) T->counters[CLR_C_COND_ACC_GEOM + k] += h->cs[k];
   T->counters[CLR_C_RNG_DRAWS] += h->draws;
   T->counters[CLR_C_N_STEPS] += h->steps;
   if (h->native >= 0) {

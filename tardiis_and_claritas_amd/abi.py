@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 N_CHANNELS = 18
 MAX_GROUPS = 64
 MAX_AUX = 4
@@ -69,7 +69,8 @@ COUNTER_NAMES = [
     'rays_ctir', 'sum_topr', 'rays_toptir', 'sum_bir', 'sum_bor', 'rays_btir',
     'n_flight', 'n_null', 'n_scat', 'n_iface', 'n_abs', 'n_exit', 'cond_trials',
     'cond_reject', 'cond_inaccessible', 'rng_draws', 'n_steps', 'src_bore_interactions', 'src_bore_reflections',
-    'src_reflected_accept',
+    'src_reflected_accept', 'cond_accepted_geometric', 'cond_accepted_coherent', 'cond_accepted_morphology',
+    'cond_accepted_absorptions', 'cond_accepted_scatterings', 'cond_numerical_failures', 'cond_retry_exhaustions',
 ]
 C_IDX = {name: i for i, name in enumerate(COUNTER_NAMES)}
 

@@ -75,6 +75,18 @@ CONDITIONAL_COUNT_NAMES = ('candidate_trials', 'outcome_rejections', 'inaccessib
                            'accepted_scatterings', 'numerical_failures', 'retry_exhaustions')
 
 
+# the device counters behind each name (claritas.h CLR_C_COND_*)
+CONDITIONAL_COUNTERS = ('cond_trials', 'cond_reject', 'cond_inaccessible', 'cond_accepted_geometric',
+                        'cond_accepted_coherent', 'cond_accepted_morphology', 'cond_accepted_absorptions',
+                        'cond_accepted_scatterings', 'cond_numerical_failures', 'cond_retry_exhaustions')
+
+
+def conditional_counts(counters):
+    """The reference's conditional_sampler_counts (CLARITAS.py:9119-9125) from
+    a run's counter array: all zero unless the conditional law ran."""
+    return {name: int(counters[abi.C_IDX[c]]) for name, c in zip(CONDITIONAL_COUNT_NAMES, CONDITIONAL_COUNTERS)}
+
+
 def _bin_index(v, edges):
     """Index b with edges[b] <= v < edges[b + 1], else -1 (v compared in FP64)."""
     v = np.asarray(v, dtype=np.float64)
@@ -600,8 +612,9 @@ class CausalAccumulator:
         self.tally = PathTally(n_bins, model.symmetry_average_exact, r, z)
         self.count = 0
         self.sample_limit = 512
-        self.failures = []  # the first failed histories' samples (numpy rows)
+        self.failures = []  # the failed histories' samples with the lowest ids (numpy rows)
         self.failure_count = 0
+        self.failure_device_capacity = None  # samples one launch may write (None: 4x the room, >= 1024)
 
     def run(self, n, seed, history_offset, launch, light_acc=None):
         """clr_run_causal over histories [history_offset, history_offset + n):
@@ -617,8 +630,9 @@ class CausalAccumulator:
             co.capacity = rn
             co.range_start, co.range_count = r0, rn
             room = max(self.sample_limit - sum(len(f) for f in self.failures), 0)
-            fails = np.zeros(max(room, 1), dtype=abi.failure_dtype())
-            co.failures, co.failure_capacity = fails.ctypes.data, room
+            cap = self._failure_cap(room, rn)
+            fails = np.zeros(max(cap, 1), dtype=abi.failure_dtype())
+            co.failures, co.failure_capacity = fails.ctypes.data, cap
             t = abi.Tallies()
             if light_acc is not None:
                 out, arrs = abi.light_out(*light_acc.shape())
@@ -634,8 +648,46 @@ class CausalAccumulator:
             self.tally.add(recs[:int(co.count)])
             self.count += int(co.count)
             self.failure_count += int(co.failure_count)
-            if room:
-                f = fails[:min(int(co.failure_count), room)]
-                self.failures.append(np.sort(f, order='history'))
+            if room and co.failure_count:
+                if co.failure_count <= cap:
+                    f = np.sort(fails[:int(co.failure_count)], order='history')[:room]
+                else:
+                    f = self._lowest_failures(n, seed, history_offset, launch, r0, rn, room, cap)
+                self.failures.append(f)
             total = sum_tallies(total, abi.tallies_to_numpy(t))
         return total
+
+    def _failure_cap(self, room, rn):
+        if not room:
+            return 0
+        c = self.failure_device_capacity or max(4 * room, 1024)
+        return int(min(max(c, room), rn))
+
+    def _lowest_failures(self, n, seed, history_offset, launch, r0, rn, room, cap):
+        """The `room` failed histories with the lowest ids in the range
+        [r0, r0 + rn) of a run whose launch wrote more failures than `cap`
+        (in completion order).  The range's histories are traced again in
+        prefixes short enough for all their failures to fit (the traces are
+        the same: one Philox stream per history); tallies and records of
+        these passes are discarded."""
+        ctx = self.model.ctx
+        out, need, start, end = [], room, r0, r0 + rn
+        while need > 0 and start < end:
+            length = end - start
+            while True:
+                fails = np.zeros(cap, dtype=abi.failure_dtype())
+                co = abi.CausalOut()
+                co.records, co.capacity = None, 0
+                co.range_start, co.range_count = start, length
+                co.failures, co.failure_capacity = fails.ctypes.data, cap
+                t = abi.Tallies()
+                ctx.check(ctx.lib.clr_run_causal(ctx.h, int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, int(history_offset),
+                                                 launch, None, C.byref(t), None, C.byref(co)), 'clr_run_causal')
+                if co.failure_count <= cap or length == 1:
+                    break
+                length = max(length // 2, 1)
+            f = np.sort(fails[:min(int(co.failure_count), cap)], order='history')[:need]
+            out.append(f)
+            need -= len(f)
+            start += length
+        return np.concatenate(out) if out else np.zeros(0, dtype=abi.failure_dtype())

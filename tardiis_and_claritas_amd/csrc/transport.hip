@@ -2596,10 +2596,32 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
   L.region = R_WIFACE;
 }
 
-// the particle event at the end of a flight (CLARITAS.py:4774-4981)
+// Adds the number of active lanes with `pred` to the workgroup's LDS counter
+// cc[idx], with one atomic per wave (the conditional law's rarer sampler
+// counts, which keep no per-lane register).
+__device__ __forceinline__ void wave_count(unsigned long long* cc, int idx, bool pred) {
+  const uint64_t m = __ballot(pred);
+  const uint64_t act = __ballot(1);
+  if (m && (int)__lane_id() == __ffsll((unsigned long long)act) - 1) atomicAdd(cc + idx, (unsigned long long)__popcll(m));
+}
+// a closure call's outcome in the sampler counts (CLARITAS.py:4498-4535)
+__device__ __forceinline__ void count_closure(unsigned long long* cc, const CondOut& h) {
+  const bool ok = (h.status == 1 || h.status == 2);
+  wave_count(cc, CLR_C_COND_ACC_GEOM, ok && h.channel == 0);
+  wave_count(cc, CLR_C_COND_ACC_COHERENT, ok && h.channel == 1);
+  wave_count(cc, CLR_C_COND_ACC_MORPH, ok && h.channel == 2);
+  wave_count(cc, CLR_C_COND_ACC_ABS, h.status == 2);
+  wave_count(cc, CLR_C_COND_ACC_SCAT, h.status == 1);
+  wave_count(cc, CLR_C_COND_EXHAUSTED, !ok && h.subcode == 9001);
+  wave_count(cc, CLR_C_COND_NUMERICAL, !ok && h.subcode != 9001);
+}
+
+// the particle event at the end of a flight (CLARITAS.py:4774-4981).  Under
+// the conditional law the sampler counts of CLARITAS.py:4854-4933 go to cc
+// (the workgroup's LDS counters).
 template <int kKind>
 __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
-                                               Prov& V) {
+                                               Prov& V, unsigned long long* cc) {
   constexpr bool kPlain = (kKind == CLR_KERNEL_PLAIN);
   constexpr bool kLight = (kKind == CLR_KERNEL_LIGHT), kRough = kind_rough<kKind>();
   constexpr bool kDiag = kind_diag<kKind>();
@@ -2680,6 +2702,11 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     L.wic++;
     pidx = sel >= 0 ? sel : select_bin(L.r, P, true, L.x, L.y, L.z, LW);
     const bool coherent = (P.coherent >= 1.0f) || (U(L.r) < P.coherent);
+    if (cond) {
+      A.trials++;
+      wave_count(cc, CLR_C_COND_ACC_COHERENT, coherent);
+      wave_count(cc, CLR_C_COND_ACC_SCAT, coherent);
+    }
     if (coherent) {
       L.wcoh++;
       mie_scatter(L.r, P, pidx, L.vx, L.vy, L.vz);
@@ -2691,13 +2718,24 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     // morphology-wave event about the local azimuthal flow direction,
     // CLARITAS.py:4862-4881, 2783-2799 (configurations with a coherent
     // fraction below 1 run on the ROUGH variant)
-    if constexpr (!kRough) { terminate(L, 4, PF_SPHEROID); return; }
+    if constexpr (!kRough) {
+      if (cond) wave_count(cc, CLR_C_COND_NUMERICAL, true);
+      terminate(L, 4, PF_SPHEROID);
+      return;
+    }
     dir_only = true;
     const float fr = sqrtf(L.x * L.x + L.y * L.y);
     if (fr > 1e-12f) { fx = -L.y / fr; fy = L.x / fr; }
     rad = P.radius[pidx];
-    if (rad <= 0.0f) return;
-    if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) { terminate(L, 4, PF_SPHEROID, SUB_SPHEROID_ASPECT); return; }
+    if (rad <= 0.0f) {  // a point particle: accepted, direction kept
+      if (cond) { wave_count(cc, CLR_C_COND_ACC_MORPH, true); wave_count(cc, CLR_C_COND_ACC_SCAT, true); }
+      return;
+    }
+    if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) {
+      if (cond) wave_count(cc, CLR_C_COND_NUMERICAL, true);
+      terminate(L, 4, PF_SPHEROID, SUB_SPHEROID_ASPECT);
+      return;
+    }
     if (fabsf(P.aspect - 1.0f) < 1e-7f) {
       sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
       dcx = -(double)rad * (double)gx; dcy = -(double)rad * (double)gy; dcz = -(double)rad * (double)gz;
@@ -2706,6 +2744,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     }
   } else {
     L.gic++;
+    if (cond) A.trials++;
     pidx = sel >= 0 ? sel : select_bin(L.r, P, false, L.x, L.y, L.z, LG);
     rad = P.radius[pidx];
     sphere_sample(L.r, L.vx, L.vy, L.vz, gx, gy, gz);
@@ -2725,6 +2764,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   if (cbin >= 0) {
     CondOut h = conditional_event<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials);
     A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+    count_closure(cc, h);
     if (fallback ? (h.status != 1) : (h.status != 1 && h.status != 2)) {
       terminate(L, 4, h.failure ? h.failure : PF_OTHER, h.subcode);
       return;
@@ -2765,6 +2805,12 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
   if (pc.pir) L.flags |= F_PIR;
   if (pc.pabs) L.flags |= F_PABS;
   A.n_abs += pc.n_abs;
+  if (cond) {
+    wave_count(cc, CLR_C_COND_ACC_GEOM, ps == 1 && !dir_only);
+    wave_count(cc, CLR_C_COND_ACC_MORPH, ps == 1 && dir_only);
+    wave_count(cc, CLR_C_COND_ACC_SCAT, ps == 1);
+    wave_count(cc, CLR_C_COND_NUMERICAL, ps != 1 && ps != 2);
+  }
   if (ps == 2) {
     if constexpr (kLight) light_mark(*D, gid, 9);
     terminate(L, 5);
@@ -3206,7 +3252,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         }
       } else if (op == K_WATER) {
         if (alive && (L.region == R_WPART || L.region == R_WATER)) rng_topup(L.r);
-        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A, Lp.light, V);
+        if (alive && L.region == R_WPART) water_particle<kKind>(L, P, A, Lp.light, V, T.cnt);
         if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light, V);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);

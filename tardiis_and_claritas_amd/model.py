@@ -901,10 +901,7 @@ class ClaritasForwardModel:
             closure = ('Mie extinction and terminal outcomes; conditional finite-particle angular/path law' if cond
                        else self.particle_event_model + '; configured unconditional event law')
             cnt = T['counters']
-            counts = dict.fromkeys(causal.CONDITIONAL_COUNT_NAMES)
-            counts.update(candidate_trials=int(cnt[abi.C_IDX['cond_trials']]),
-                          outcome_rejections=int(cnt[abi.C_IDX['cond_reject']]),
-                          inaccessible_trials=int(cnt[abi.C_IDX['cond_inaccessible']]))
+            counts = causal.conditional_counts(cnt)
             res.detector_particle_path_diagnostics = causal.diagnostics_dict(
                 cacc.tally, self._sym(det_sum), closure, counts, CONDITIONAL_MAX_TRIALS)
             res.production_failure_diagnostics['samples'] = _failure_samples(cacc)

@@ -97,3 +97,15 @@ def test_failure_samples(gpu_available, oracle_lib):
     r = m.simulate(mat, conc, n_rays=N, seed=3)
     s = r.production_failure_diagnostics['samples']
     assert 0 < len(s) <= 512 and all(q['failure_code'] > 0 for q in s)
+    # more failures than one launch may write: the samples kept are those of
+    # the lowest failed history ids (CLARITAS.py:8504-8514), also across chunks
+    from tardiis_and_claritas_amd import causal
+    assert co.failure_count > 40
+    want = np.sort(f['history'])[:24]
+    nb = len(m.resolve_material(mat).diameter_m)
+    for chunk in (1 << 20, 3000):
+        acc = causal.CausalAccumulator(m, nb, chunk=chunk)
+        acc.sample_limit, acc.failure_device_capacity = 24, 32
+        acc.run(N, 3, 0, m._launch())
+        got = np.concatenate(acc.failures)['history']
+        np.testing.assert_array_equal(got, want)

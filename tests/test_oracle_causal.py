@@ -194,3 +194,36 @@ def test_native_binning_equals_numpy(case):
         for k in B.t:
             assert A.t[k].shape == B.t[k].shape, k
             np.testing.assert_allclose(A.t[k], B.t[k], rtol=1e-12, atol=1e-12, err_msg=k)
+
+
+def test_conditional_sampler_counts():
+    """All ten conditional_sampler_counts (CLARITAS.py:9119-9125): every
+    acceptance is counted once by channel and once by outcome, so the two
+    families have equal sums; a run's counts are the sum of its halves'
+    (they are per-history); the unconditional laws count nothing; the
+    closure's own counts follow its probe outcomes."""
+    kw, mat, conc, n, seed = workloads.config1('hybrid_mie_exact_rate')
+    m = ClaritasForwardModel(**kw)
+    om = po.OracleModel(m, mat, conc)
+    T, _ = om.run(8000, seed, 0)
+    c = causal.conditional_counts(T['counters'])
+    assert list(c) == list(causal.CONDITIONAL_COUNT_NAMES)
+    by_channel = c['accepted_geometric'] + c['accepted_coherent'] + c['accepted_morphology']
+    assert by_channel == c['accepted_absorptions'] + c['accepted_scatterings'] > 1000
+    # rejections and inaccessible placements are trials that were not accepted
+    assert c['candidate_trials'] >= by_channel + c['outcome_rejections'] + c['inaccessible_trials']
+    A, _ = om.run(3000, seed, 0)
+    B, _ = om.run(5000, seed, 3000)
+    cab = causal.conditional_counts(A['counters'] + B['counters'])
+    assert cab == c
+    # the spheroid-morphology channel appears with a coherent fraction below 1
+    kw2, mat2, conc2, _, seed2 = workloads.config1_rough()
+    m2 = ClaritasForwardModel(**kw2)
+    c2 = causal.conditional_counts(po.OracleModel(m2, mat2, conc2).run(4000, seed2, 0)[0]['counters'])
+    assert c2['accepted_morphology'] > 0 and c2['accepted_geometric'] > 0
+    assert (c2['accepted_geometric'] + c2['accepted_coherent'] + c2['accepted_morphology'] ==
+            c2['accepted_absorptions'] + c2['accepted_scatterings'])
+    kw3, mat3, conc3, _, seed3 = workloads.config1('plain_mie')
+    m3 = ClaritasForwardModel(**kw3)
+    c3 = causal.conditional_counts(po.OracleModel(m3, mat3, conc3).run(2000, seed3, 0)[0]['counters'])
+    assert all(v == 0 for v in c3.values())
