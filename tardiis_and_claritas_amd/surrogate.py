@@ -334,11 +334,14 @@ def _cat(parts):
 
 def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
                  test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True,
-                 generate_only=False, hi=200_000_000):
+                 generate_only=False, hi=200_000_000, spot_datasets=()):
     """Generate (n_optical x n_variants training and n_test_optical x
     n_test_variants held-out configurations; 0 skips), merge earlier
-    `datasets` (paths of dataset.npz), train, evaluate; writes summary.json,
-    weights.npz and dataset.npz (this run's own samples)."""
+    `datasets` (paths of dataset.npz), train, evaluate on the held-out part
+    and on the held-out parts of `spot_datasets` (Monte Carlo spot checks at
+    high counts, scored separately, with the Monte Carlo fallback run on
+    them); writes summary.json, weights.npz and dataset.npz (this run's own
+    samples)."""
     from .batched import BatchedRunner
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
@@ -403,41 +406,123 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
     losses = sur.fit(F_tr[keep], S_tr[keep], epochs=epochs, batch=256, lr=1e-3, seed=seed,
                      n_hist=n_tr[keep] if weighted else None)
     train_s = time.time() - t1
-    F_te = features(P_te)
-    S_pred, spread = sur.predict(F_te)
-    valid = np.all(S_te > 0, axis=1)
-    err = ntu_relative_error(S_pred[valid], S_te[valid])
-    # Monte Carlo statistical error of the held-out labels themselves
-    label_se = np.sqrt(label_variance(S_te[valid], n_te[valid])[:, 0])
-    ratio_spread = spread[valid, 0]
     ms = sur.ens.time_infer(65536, 20)
     q_s = ms * 1e-3 / 65536
-    n_match = mc_matched_photons(S_te[valid])
-    t_mc = n_match / mc_rate
     summary = dict(
-        n_train=int(keep.sum()), n_test=int(valid.sum()), train_histories=int(n_tr.sum()),
-        test_histories=int(n_te.sum()), datasets=[str(p) for p in datasets], weighted=bool(weighted),
-        mc_histories_per_s=mc_rate, train_seconds=train_s, epochs=epochs, final_member_losses=losses[-1],
-        ntu_error_median=float(np.median(err)), ntu_error_p90=float(np.percentile(err, 90)),
-        ntu_error_mean=float(np.mean(err)), frac_within_1pct=float(np.mean(err <= 0.01)),
-        label_se_median=float(np.median(label_se)),
-        surrogate_query_s=q_s, surrogate_queries_per_s=1.0 / q_s,
-        mc_matched_photons_median=float(np.median(n_match)), mc_query_s_median=float(np.median(t_mc)),
-        speedup_median=float(np.median(t_mc) / q_s),
-    )
-    # Monte Carlo fallback where the members disagree: error and effective speed-up
-    fb = {}
-    for thr in (0.005, 0.01, 0.02, 0.05):
-        use_mc = ratio_spread > thr
-        e = np.where(use_mc, label_se, err)   # a fallback answer carries MC's own 1 % error budget
-        t = q_s + np.where(use_mc, t_mc, 0.0)
-        fb[str(thr)] = dict(fallback_fraction=float(use_mc.mean()), ntu_error_p90=float(np.percentile(e, 90)),
-                            speedup=float(np.mean(t_mc) / np.mean(t)))
-    summary['fallback'] = fb
+        n_train=int(keep.sum()), train_histories=int(n_tr.sum()), datasets=[str(p) for p in datasets],
+        spot_datasets=[str(p) for p in spot_datasets], weighted=bool(weighted), mc_histories_per_s=mc_rate,
+        train_seconds=train_s, epochs=epochs, final_member_losses=losses[-1],
+        surrogate_query_s=q_s, surrogate_queries_per_s=1.0 / q_s)
+    thresholds = {}
+    if len(S_te):
+        S_pred, spread = sur.predict(features(P_te))
+        summary['held_out'] = set_metrics(S_pred, S_te, n_te)
+        summary['held_out']['test_histories'] = int(n_te.sum())
+        # fallback thresholds: quantiles of the members' spread of log(S90/S0)
+        # on the held-out set, for fallback fractions of 0.1 %, 1 % and 5 %
+        for f in FALLBACK_FRACTIONS:
+            thresholds[str(f)] = float(np.quantile(spread[:, 0], 1.0 - f))
+        summary['fallback_thresholds'] = thresholds
+        summary['held_out']['spread_calibration'] = spread_calibration(S_pred, spread, S_te, n_te)
+    if spot_datasets:
+        parts = [load_dataset(pth)[3:] for pth in spot_datasets]
+        P_sp = _cat([a for a, _, _ in parts])
+        S_sp = np.concatenate([b for _, b, _ in parts])
+        n_sp = np.concatenate([c for _, _, c in parts])
+        S_pred, spread = sur.predict(features(P_sp))
+        summary['spot'] = spot = set_metrics(S_pred, S_sp, n_sp)
+        spot['histories'] = int(n_sp.sum())
+        spot.update(matched_speedup(S_pred, spot['surrogate_rms_est'], q_s, mc_rate))
+        (out / 'summary.json').write_text(json.dumps(summary, indent=1))
+        np.savez(out / 'weights.npz', params=sur.ens.get_params(), **{k: v for k, v in sur.norm.__dict__.items()})
+        log(json.dumps(summary))
+        if thresholds:
+            spot['fallback'] = fallback_eval(P_sp, S_pred, spread, S_sp, n_sp, thresholds, q_s, mc_rate, runner,
+                                             seed + 4242)
     (out / 'summary.json').write_text(json.dumps(summary, indent=1))
     np.savez(out / 'weights.npz', params=sur.ens.get_params(), **{k: v for k, v in sur.norm.__dict__.items()})
     log(json.dumps(summary))
     return summary
+
+
+FALLBACK_FRACTIONS = (0.001, 0.01, 0.05)
+
+
+def set_metrics(S_pred, S_ref, n_ref):
+    """NTU error of predictions against Monte Carlo labels that carry their
+    own counting noise.  d = log of the predicted over the labelled S90/S0;
+    E[d^2] = (network error)^2 + (label variance), so the network's own rms
+    error is estimated as sqrt(mean d^2 - mean label variance)."""
+    S_pred, S_ref = np.asarray(S_pred, dtype=np.float64), np.asarray(S_ref, dtype=np.float64)
+    valid = np.all(S_ref > 0, axis=1) & np.all(S_pred > 0, axis=1)
+    a, b = S_pred[valid], S_ref[valid]
+    err = ntu_relative_error(a, b)
+    d = np.log((a[:, 0] / a[:, 1]) / (b[:, 0] / b[:, 1]))
+    v = label_variance(b, np.asarray(n_ref)[valid])[:, 0]
+    return dict(n=int(valid.sum()), n_without_counts=int((~valid).sum()),
+                ntu_error_median=float(np.median(err)), ntu_error_p90=float(np.percentile(err, 90)),
+                ntu_error_mean=float(np.mean(err)), frac_within_1pct=float(np.mean(err <= 0.01)),
+                rms_vs_labels=float(np.sqrt(np.mean(d * d))), label_rms=float(np.sqrt(np.mean(v))),
+                label_se_median=float(np.median(np.sqrt(v))),
+                surrogate_rms_est=float(np.sqrt(max(np.mean(d * d) - np.mean(v), 0.0))))
+
+
+def spread_calibration(S_pred, spread, S_ref, n_ref, n_bins=10):
+    """Estimated network rms error of log(S90/S0) per decile of the members'
+    spread (label variance subtracted bin by bin)."""
+    valid = np.all(S_ref > 0, axis=1)
+    sp = spread[valid, 0]
+    a, b = np.asarray(S_pred, dtype=np.float64)[valid], np.asarray(S_ref, dtype=np.float64)[valid]
+    d2 = np.log((a[:, 0] / a[:, 1]) / (b[:, 0] / b[:, 1])) ** 2
+    v = label_variance(b, np.asarray(n_ref)[valid])[:, 0]
+    edges = np.quantile(sp, np.linspace(0.0, 1.0, n_bins + 1))
+    rows = []
+    for k in range(n_bins):
+        m = (sp >= edges[k]) & ((sp < edges[k + 1]) if k < n_bins - 1 else (sp <= edges[k + 1]))
+        if m.any():
+            rows.append(dict(spread_lo=float(edges[k]), spread_hi=float(edges[k + 1]), n=int(m.sum()),
+                             rms_est=float(np.sqrt(max(d2[m].mean() - v[m].mean(), 0.0)))))
+    return rows
+
+
+def matched_speedup(S_pred, rms, q_s, mc_rate):
+    """Batched Monte Carlo time per query for 1 % relative standard error of
+    S90/S0 (histories sized from the predicted signals), and for the
+    network's own rms error, against the network's time per query."""
+    n1 = mc_matched_photons(S_pred, 0.01)
+    out = dict(mc_matched_photons_median=float(np.median(n1)), mc_query_s_median=float(np.median(n1) / mc_rate),
+               speedup_at_1pct=float(np.median(n1) / mc_rate / q_s), meets_1pct=bool(rms <= 0.01))
+    if rms > 0:
+        nr = mc_matched_photons(S_pred, rms)
+        out['speedup_at_matched_rms'] = float(np.median(nr) / mc_rate / q_s)
+    return out
+
+
+def fallback_eval(P_sp, S_pred, spread, S_ref, n_ref, thresholds, q_s, mc_rate, runner, seed, rel=0.01):
+    """Answer the spot-check queries whose spread exceeds each threshold by
+    batched Monte Carlo run now, histories sized for `rel` relative error from
+    the network's own prediction, and score every answer against the
+    spot-check labels."""
+    S_pred = np.asarray(S_pred, dtype=np.float64)
+    lo = min(thresholds.values())
+    flag = spread[:, 0] > lo
+    answers = S_pred.copy()
+    n_mc = np.clip(mc_matched_photons(S_pred, rel), 200_000, 2_000_000_000).astype(np.int64)
+    if flag.any():
+        sub = {k: np.asarray(v)[flag] for k, v in P_sp.items()}
+        answers[flag] = monte_carlo(sub, n_mc[flag], seed, runner=runner)
+    t_mc = n_mc / mc_rate
+    out = {}
+    for name, thr in thresholds.items():
+        use = spread[:, 0] > thr
+        S = np.where(use[:, None], answers, S_pred)
+        m = set_metrics(S, S_ref, n_ref)
+        t = q_s + np.where(use, t_mc, 0.0)
+        out[name] = dict(threshold=thr, fallback_fraction=float(use.mean()), ntu_error_median=m['ntu_error_median'],
+                         ntu_error_p90=m['ntu_error_p90'], rms_est=m['surrogate_rms_est'],
+                         frac_within_1pct=m['frac_within_1pct'],
+                         speedup_vs_mc_at_1pct=float(np.mean(t_mc) / np.mean(t)))
+    return out
 
 
 if __name__ == '__main__':
@@ -456,7 +541,9 @@ if __name__ == '__main__':
     ap.add_argument('--unweighted', action='store_true', help='plain MSE instead of inverse-variance weights')
     ap.add_argument('--generate-only', action='store_true', help='write dataset.npz and stop (no training)')
     ap.add_argument('--max-photons', type=float, default=2e8, help='photon budget cap per configuration')
+    ap.add_argument('--spot-dataset', action='append', default=[],
+                    help='dataset.npz whose held-out part is scored as a spot check (repeatable)')
     a = ap.parse_args()
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
                  a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted,
-                 generate_only=a.generate_only, hi=int(a.max_photons))
+                 generate_only=a.generate_only, hi=int(a.max_photons), spot_datasets=a.spot_dataset)

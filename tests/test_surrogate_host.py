@@ -45,3 +45,42 @@ def test_photon_budget_clips():
     b = sg.photon_budget(np.array([1.0, 1000.0, 1e7]), counts=100.0)
     assert b[0] == 200_000_000 and b[2] == 200_000
     assert b[1] == int(100.0 / (1e-7 * 1000.0))
+
+
+def test_set_metrics_recovers_network_error_under_label_noise():
+    # predictions off by a known 2 % rms in S90/S0, labels with Poisson noise of
+    # a comparable size: the label variance is subtracted from mean d^2
+    g = np.random.default_rng(1)
+    n = 40000
+    true = np.stack([g.uniform(1e-5, 1e-4, n), g.uniform(1e-2, 5e-2, n), g.uniform(1e-5, 1e-4, n)], axis=1)
+    n_hist = np.full(n, 5e6)
+    labels = g.poisson(true * n_hist[:, None]) / n_hist[:, None]
+    pred = true.copy()
+    pred[:, 0] *= np.exp(g.normal(0.0, 0.02, n))
+    m = sg.set_metrics(pred, labels, n_hist)
+    assert m['n'] == n
+    assert abs(m['surrogate_rms_est'] - 0.02) < 0.002
+    assert m['rms_vs_labels'] > m['surrogate_rms_est']
+    # exact predictions: the estimate is at the sampling noise of the subtraction
+    assert sg.set_metrics(true, labels, n_hist)['surrogate_rms_est'] < 0.006
+
+
+def test_spread_calibration_bins_and_matched_speedup():
+    g = np.random.default_rng(2)
+    n = 20000
+    true = np.stack([np.full(n, 5e-5), np.full(n, 2e-2), np.full(n, 5e-5)], axis=1)
+    n_hist = np.full(n, 1e12)                      # labels practically exact
+    spread = np.zeros((n, 3))
+    spread[:, 0] = g.uniform(0.0, 0.1, n)
+    pred = true.copy()
+    pred[:, 0] *= np.exp(g.normal(0.0, 1.0, n) * spread[:, 0])   # error grows with the spread
+    rows = sg.spread_calibration(pred, spread, true, n_hist)
+    assert len(rows) == 10 and sum(r['n'] for r in rows) == n
+    est = [r['rms_est'] for r in rows]
+    assert est[-1] > 5 * est[0]
+    mid = [0.5 * (r['spread_lo'] + r['spread_hi']) for r in rows]
+    np.testing.assert_allclose(est[2:], np.asarray(mid[2:]) * 1.0, rtol=0.15)
+    s = sg.matched_speedup(true, 0.02, 1e-8, 1e9)
+    # 1 % needs 4x the histories of 2 %
+    np.testing.assert_allclose(s['speedup_at_1pct'] / s['speedup_at_matched_rms'], 4.0, rtol=1e-9)
+    assert not s['meets_1pct'] and sg.matched_speedup(true, 0.009, 1e-8, 1e9)['meets_1pct']
