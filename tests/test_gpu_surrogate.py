@@ -126,3 +126,59 @@ def test_inference_timing(gpu_available):
     ens = MlpEnsemble(_ctx(), n_members=5, hidden=256, n_hidden_layers=3, seed=0)
     ms = ens.time_infer(65536, 5)
     assert 0.0 < ms < 1000.0
+
+
+def test_surrogate_gain_over_monte_carlo(gpu_available):
+    """What the trained network gains over batched Monte Carlo, the method
+    without it.  A 5-member ensemble is trained on the GPU on the committed
+    config-3 training fixture (8192 configurations, labels from batched Monte
+    Carlo at ~150 expected 90-degree window counts) and scored on the 256
+    spot-check configurations (labels at ~4000 counts, 0.3 % noise):
+
+    - at equal cost (the Monte Carlo photons one surrogate query's time buys,
+      from the live inference rate and the live batched Monte Carlo rate on
+      the same configurations) Monte Carlo's NTU error is many times the
+      network's;
+    - Monte Carlo needs more than 1000x the network's time per query to
+      match the network's own rms error;
+    - the members' spread ranks the errors: the 10 % most uncertain queries
+      err more than the rest, which is what the Monte Carlo fallback uses."""
+    _flush('surrogate gain over Monte Carlo (config-3 fixture)')
+    from pathlib import Path
+    from tardiis_and_claritas_amd import surrogate as sg
+    from tardiis_and_claritas_amd.batched import BatchedRunner
+    d = np.load(Path(__file__).parent / 'golden' / 'surrogate_config3.npz')
+    keys = sg.OPTICAL + sg.TRANSPORT
+    P_tr = {k: d[f'train_{k}'] for k in keys}
+    P_sp = {k: d[f'spot_{k}'] for k in keys}
+    S_tr, n_tr, S_sp, n_sp = d['train_S'], d['train_hist'], d['spot_S'], d['spot_hist']
+    runner = BatchedRunner(0)
+    sur = sg.Surrogate(runner.ctx, seed=7)
+    keep = np.all(S_tr > 0, axis=1)
+    sur.fit(sg.features(P_tr)[keep], S_tr[keep], epochs=120, batch=256, lr=1e-3, seed=7, n_hist=n_tr[keep])
+    pred, spread = sur.predict(sg.features(P_sp))
+    ok = np.all(S_sp > 0, axis=1)
+    err = sg.ntu_relative_error(pred[ok], S_sp[ok])
+    q_s = sur.ens.time_infer(65536, 10) * 1e-3 / 65536
+    # live batched Monte Carlo rate on the spot configurations at ~1e6 histories each
+    specs_n = np.full(len(S_sp), 1_000_000, dtype=np.int64)
+    sg.monte_carlo(P_sp, specs_n, 11, runner=runner)
+    t0 = __import__('time').perf_counter()
+    sg.monte_carlo(P_sp, specs_n, 12, runner=runner)
+    rate = specs_n.sum() / (__import__('time').perf_counter() - t0)
+    n_eq = max(int(q_s * rate), 1)
+    S_mc = sg.monte_carlo(P_sp, n_eq, 13, runner=runner)
+    mc_ok = np.all(S_mc[ok][:, :2] > 0, axis=1)
+    err_mc = np.ones(int(ok.sum()))
+    err_mc[mc_ok] = sg.ntu_relative_error(S_mc[ok][mc_ok], S_sp[ok][mc_ok])
+    m = sg.set_metrics(pred, S_sp, n_sp)
+    t_match = np.median(sg.mc_matched_photons(pred[ok], m['surrogate_rms_est'])) / rate
+    print(f'network: median {np.median(err):.4f} p90 {np.percentile(err, 90):.4f} rms {m["surrogate_rms_est"]:.4f}; '
+          f'{1 / q_s:.3g} queries/s; Monte Carlo {rate:.3g} histories/s, {n_eq} histories per query at equal '
+          f'cost: median {np.median(err_mc):.3f}; matched-error Monte Carlo {t_match / q_s:.3g}x the query time',
+          flush=True)
+    assert np.median(err) < 0.05
+    assert np.median(err_mc) > 5 * np.median(err)
+    assert t_match / q_s > 1000
+    top = spread[ok, 0] >= np.quantile(spread[ok, 0], 0.9)
+    assert err[top].mean() > err[~top].mean()
