@@ -397,6 +397,16 @@ void fill_params(Slot& s, const clr_config& g) {
     zs = P.zmean - H * (1.0f - (a2 / R2) * std::log1p(R2 / a2));
   } else if (P.fs_model == CLR_SURFACE_RANKINE && P.dh > 0.0f) zs = (float)P.rk_C0;
   P.zs_min = zs - 1e-6f * std::fmax(std::fabs(zs), 1e-3f);
+  P.fs_c0 = P.zmean; P.fs_k = 0.0f; P.fs_a2 = 1.0f;
+  P.fs_qpad = 1e-6f * P.rin * P.rin;
+  if (P.fs_model == CLR_SURFACE_PARABOLIC && P.dh > 0.0f) {
+    P.fs_c0 = P.zmean - 0.5f * P.dh; P.fs_k = P.dh / (P.rin * P.rin);
+  } else if (P.fs_model == CLR_SURFACE_SCULLY && P.dh > 0.0f) {
+    const float a2 = P.core_r * P.core_r;
+    P.fs_c0 = zs; P.fs_k = P.dh * (a2 + P.rin * P.rin) / (P.rin * P.rin); P.fs_a2 = a2;
+  } else if (P.fs_model == CLR_SURFACE_RANKINE && P.dh > 0.0f) {
+    P.fs_c0 = (float)P.rk_C0; P.fs_k = (float)P.rk_K; P.fs_a2 = P.core_r * P.core_r;
+  }
   P.radius = s.radius_f.as<float>();
   P.n_real = s.n_f.as<float>();
   P.k_imag = s.k_f.as<float>();

@@ -43,6 +43,10 @@ struct DevParams {
   int det_window;
   // least alignment u.v of a ray that can pass both throat apertures of a bore
   float det_cos_min;
+  // free-surface height as a function of q = x^2 + y^2 in float (surface_floor):
+  // Rankine z = fs_c0 + fs_k q / fs_a2 (q < fs_a2), fs_c0 + fs_k (2 - fs_a2 / q);
+  // Scully z = fs_c0 + fs_k q / (fs_a2 + q); parabolic z = fs_c0 + fs_k q
+  float fs_c0, fs_k, fs_a2, fs_qpad;
   // per-bin tables
   const float* radius;
   const float* n_real;

@@ -70,6 +70,9 @@ enum : int {
 // The key (the run's seed) is the same for every lane of a work item, so it
 // lives once per workgroup in LDS rather than in two VGPRs per lane.
 __shared__ uint32_t s_key[2];
+// the z-slab majorants of the configuration a workgroup is tracing (read per
+// lane with a lane-dependent index, so from LDS rather than the parameter block)
+__shared__ float s_slab_mu[CLR_MAJ_SLABS];
 // The lane keeps an 8-word queue of its stream: words are consumed from b0
 // and the queue shifts down (a dynamically indexed pick would make the
 // compiler keep it in scratch memory).  rng_topup appends the next block when
@@ -350,6 +353,30 @@ __device__ __forceinline__ float surface_z_vs(const DevParams& P, float x, float
   float zs = P.zs_min;
   if (!(z < P.zs_min)) zs = free_surface_z(P, x, y);
   return zs;
+}
+// A lower bound of the free-surface height over the horizontal track of the
+// segment p + t v, 0 <= t <= d: every surface model here rises with
+// q = x^2 + y^2, so its lowest point over the track is at the track's least q
+// (the end points, or the closest approach to the axis when that lies inside
+// the segment).  q is rounded down by a relative and an absolute pad and the
+// height by a relative pad, which cover the float rounding of both.
+__device__ __forceinline__ float surface_floor(const DevParams& P, float x, float y, float vx, float vy, float d) {
+  const float q0 = x * x + y * y, q1 = x * vx + y * vy, q2 = vx * vx + vy * vy;
+  float q = fminf(q0, q0 + d * (2.0f * q1 + d * q2));
+  if (q1 < 0.0f && -q1 < q2 * d) {
+    const float cr = x * vy - y * vx;
+    q = cr * cr * __builtin_amdgcn_rcpf(q2);
+  }
+  q = fmaxf(q * (1.0f - 1e-5f) - P.fs_qpad, 0.0f);
+  float z;
+  if (P.fs_model == CLR_SURFACE_RANKINE)
+    z = (q < P.fs_a2) ? P.fs_c0 + P.fs_k * q * __builtin_amdgcn_rcpf(P.fs_a2)
+                      : P.fs_c0 + P.fs_k * (2.0f - P.fs_a2 * __builtin_amdgcn_rcpf(q));
+  else if (P.fs_model == CLR_SURFACE_SCULLY)
+    z = P.fs_c0 + P.fs_k * q * __builtin_amdgcn_rcpf(P.fs_a2 + q);
+  else
+    z = P.fs_c0 + P.fs_k * q;
+  return z - 1e-6f * fmaxf(fabsf(z), 1e-3f);
 }
 __device__ double quadratic_min_positive(double A, double B, double C) {
   if (fabs(A) < 1e-24) {
@@ -1686,16 +1713,9 @@ __device__ __forceinline__ bool bore_accept(const DevParams& P, float x, float y
 // apertures, which bounds its crossing azimuth's distance from the bore axis;
 // the tested set then contains every bore the full 18-bore scan of
 // CLARITAS.py:4241-4253 could accept, in the same order.
-__device__ __forceinline__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
-                                 bool mirror) {
-  if (mirror) { x = -x; vx = -vx; }
-  const float a = vx * vx + vy * vy;
-  if (!(a > 0.0f)) return -1;
-  const float b = x * vx + y * vy, c = x * x + y * y - P.ring_in * P.ring_in;
-  const float disc = b * b - a * c;
-  if (!(disc >= 0.0f)) return -1;
-  const float t = (-b + sqrtf(disc)) / a;
-  float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
+// bores tested around window centre jc for the ray (x, y, z, v)
+__device__ __forceinline__ int bore_scan(const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
+                                         float deg) {
   if (deg < -90.0f) deg += 360.0f;  // bores lie at 0..170 deg: keep the azimuth continuous past 180
   const int jc = (int)rintf(deg * 0.1f), W = P.det_window;
   float best_t = kInfF, tt;
@@ -1706,6 +1726,33 @@ __device__ __forceinline__ int hardware_detector(const DevParams& P, float x, fl
     if (bore_accept(P, x, y, z, vx, vy, vz, P.det_ux[j], P.det_uy[j], tt) && tt < best_t) { best_t = tt; best = j; }
   }
   return best;
+}
+__device__ __forceinline__ int hardware_detector(const DevParams& P, float x, float y, float z, float vx, float vy, float vz,
+                                 bool mirror) {
+  if (mirror) { x = -x; vx = -vx; }
+  const float a = vx * vx + vy * vy;
+  if (!(a > 0.0f)) return -1;
+  const float b = x * vx + y * vy, c = x * x + y * y - P.ring_in * P.ring_in;
+  const float disc = b * b - a * c;
+  if (!(disc >= 0.0f)) return -1;
+  const float t = (-b + sqrtf(disc)) / a;
+  return bore_scan(P, x, y, z, vx, vy, vz, atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF));
+}
+// Native and mirrored detection of one exit ray.  The mirror (x, vx negated)
+// meets r = ring_in at the same t, at the reflected point, whose azimuth is
+// the negated one, so the crossing is solved once.
+__device__ __forceinline__ void hardware_detectors(const DevParams& P, float x, float y, float z, float vx, float vy,
+                                                   float vz, int& nat, int& mir) {
+  nat = -1; mir = -1;
+  const float a = vx * vx + vy * vy;
+  if (!(a > 0.0f)) return;
+  const float b = x * vx + y * vy, c = x * x + y * y - P.ring_in * P.ring_in;
+  const float disc = b * b - a * c;
+  if (!(disc >= 0.0f)) return;
+  const float t = (-b + sqrtf(disc)) / a;
+  const float deg = atan2f(x + t * vx, y + t * vy) * (180.0f / kPiF);
+  nat = bore_scan(P, x, y, z, vx, vy, vz, deg);
+  mir = bore_scan(P, -x, y, z, -vx, vy, vz, -deg);
 }
 
 // ---- reflective collimator bores, ring-face cavity, SFH213 photodiode ----
@@ -2518,7 +2565,7 @@ __device__ __forceinline__ float slab_free_path(const DevParams& P, float z, flo
     if (vz > 1e-14f && j < P.n_slabs - 1) t_exit = (P.slab_z0 + (float)(j + 1) * P.slab_w - z) * ivz;
     else if (vz < -1e-14f && j > 0) t_exit = (P.slab_z0 + (float)j * P.slab_w - z) * ivz;
     t_exit = fmaxf(t_exit, t);
-    const float mu = P.slab_mu[j];
+    const float mu = s_slab_mu[j];
     if (mu > 0.0f && tau < mu * (t_exit - t)) return t + tau / mu;
     if (t_exit >= limit) return kInfF;
     tau -= mu * (t_exit - t);
@@ -2574,7 +2621,12 @@ __device__ __forceinline__ void water_flight(Lane& L, const DevParams& P, Acc& A
     // bounds the flight (tc finite): otherwise it is the only boundary
     const float d = fminf(freep, tc);
     bool skip = false;
-    if (P.fs_model != 0 && tc < 0.5f * kInfF) skip = fmaxf(L.z, L.z + d * L.vz) < P.zs_min;
+    if (P.fs_model != 0 && tc < 0.5f * kInfF) {
+      // below the surface's lowest point, or below its lowest point over
+      // this segment's track
+      const float ztop = fmaxf(L.z, L.z + d * L.vz);
+      skip = ztop < P.zs_min || ztop < surface_floor(P, L.x, L.y, L.vx, L.vy, d);
+    }
     if (!skip) ttop = free_surface_hit(P, L.x, L.y, L.z, L.vx, L.vy, L.vz);
   }
   if (!(ttop > 0.0f)) ttop = kInfF;
@@ -2660,7 +2712,7 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     LT = LG + LW;
     if (!(LT > 0.0f)) { A.n_null++; return; }
     // the majorant the collision was sampled with: the slab's, or mu_track
-    const float MU_LOC = (P.n_slabs > 0) ? P.slab_mu[slab_of(P, L.z)] : MU_TRACK;
+    const float MU_LOC = (P.n_slabs > 0) ? s_slab_mu[slab_of(P, L.z)] : MU_TRACK;
     const float slack = 64.0f * 1.1920928955078125e-7f * fmaxf(fabsf(MU_LOC), 1.0f);
     if (LT > MU_LOC + slack) { terminate(L, 4, PF_MAJORANT, SUB_MAJORANT); return; }
     const float pacc = fminf(LT / fmaxf(MU_LOC, 1e-30f), 1.0f);
@@ -2928,14 +2980,12 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
             nat = detector_collimator(L.r, P, ex, ey, ez, evx, evy, evz, false, nb);
             mir = detector_collimator(L.r, P, ex, ey, ez, evx, evy, evz, true, mb);
           } else {
-            nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
-            mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+            hardware_detectors(P, ex, ey, ez, evx, evy, evz, nat, mir);
           }
           if (nat < 0 && P.face_model) nat = ring_face_path(L.r, P, ex, ey, ez, evx, evy, evz, false, nb, nfb);
           if (mir < 0 && P.face_model) mir = ring_face_path(L.r, P, ex, ey, ez, evx, evy, evz, true, mb, mfb);
         } else {
-          nat = hardware_detector(P, ex, ey, ez, evx, evy, evz, false);
-          mir = hardware_detector(P, ex, ey, ez, evx, evy, evz, true);
+          hardware_detectors(P, ex, ey, ez, evx, evy, evz, nat, mir);
         }
         if (nat >= 0)
           inc_n = acosf(fminf(fmaxf(evx * P.det_ux[nat] + evy * P.det_uy[nat], -1.0f), 1.0f)) * (180.0f / kPiF);
@@ -2958,9 +3008,10 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
           if (bore_accept(P, ex, ey, ez, evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t)) aux_bits |= 1 << a;
           if (bore_accept(P, -ex, ey, ez, -evx, evy, evz, P.aux_ux[a], P.aux_uy[a], t)) aux_bits |= 16 << a;
         }
+        // the mirrored exit point's azimuth is the negated one
         ang_n = atan2f(ex, ey) * (180.0f / kPiF);
+        ang_m = -ang_n;
         if (ang_n < 0.0f) ang_n += 360.0f;
-        ang_m = atan2f(-ex, ey) * (180.0f / kPiF);
         if (ang_m < 0.0f) ang_m += 360.0f;
         scored = true;
       }
@@ -2977,14 +3028,16 @@ __device__ __forceinline__ void finish_lane(Lane& L, const DevParams& P, LdsTall
       lds_add64(&T.cnt[CLR_C_SUM_TOPR], (uint32_t)L.tr);
       lds_add64(&T.cnt[CLR_C_SUM_BIR], (uint32_t)L.bir);
       lds_add64(&T.cnt[CLR_C_SUM_BOR], (uint32_t)L.bor);
-      if (L.flags & F_PER) atomicAdd(&T.cnt[CLR_C_RAYS_PER], 1ull);
-      if (L.flags & F_PIR) atomicAdd(&T.cnt[CLR_C_RAYS_PIR], 1ull);
-      if (L.flags & F_PABS) atomicAdd(&T.cnt[CLR_C_RAYS_PABS], 1ull);
-      if (L.flags & F_CTIR) atomicAdd(&T.cnt[CLR_C_RAYS_CTIR], 1ull);
-      if (L.flags & F_TTIR) atomicAdd(&T.cnt[CLR_C_RAYS_TOPTIR], 1ull);
-      if (L.flags & F_BTIR) atomicAdd(&T.cnt[CLR_C_RAYS_BTIR], 1ull);
-      if (L.ic == 0) atomicAdd(&T.cnt[CLR_C_BALLISTIC], 1ull);
     }
+    // history flags: one LDS atomic per wave and counter
+    const bool ent = (L.flags & F_ENTERED) != 0;
+    wave_count(T.cnt, CLR_C_RAYS_PER, ent && (L.flags & F_PER));
+    wave_count(T.cnt, CLR_C_RAYS_PIR, ent && (L.flags & F_PIR));
+    wave_count(T.cnt, CLR_C_RAYS_PABS, ent && (L.flags & F_PABS));
+    wave_count(T.cnt, CLR_C_RAYS_CTIR, ent && (L.flags & F_CTIR));
+    wave_count(T.cnt, CLR_C_RAYS_TOPTIR, ent && (L.flags & F_TTIR));
+    wave_count(T.cnt, CLR_C_RAYS_BTIR, ent && (L.flags & F_BTIR));
+    wave_count(T.cnt, CLR_C_BALLISTIC, ent && L.ic == 0);
     lds_add64(&T.cnt[CLR_C_N_SCAT], (uint32_t)(L.ic > 0 ? L.ic : 0));
     if constexpr (kind_diag<kKind>()) {
       // a failed history's sample (rare: one atomic per failure)
@@ -3122,12 +3175,13 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Occupancy targets (waves per SIMD).  The kernel is latency bound (long
-// dependent FP64/transcendental chains and table loads), so the register
-// budget is capped to keep more waves resident: a few spills cost less than
-// the latency that one more wave per SIMD hides.
+// Occupancy targets (waves per SIMD).  Long dependent FP64/transcendental
+// chains and table loads leave issue slots that only other resident waves
+// fill, so the register budget is capped to keep more waves resident: the
+// plain kernel at 128 VGPRs (4 waves) spills a little and runs faster than at
+// 168 (3 waves), while 96 VGPRs (5 waves) spills too much.
 #ifndef CLR_PLAIN_WAVES_PER_SIMD
-#define CLR_PLAIN_WAVES_PER_SIMD 3
+#define CLR_PLAIN_WAVES_PER_SIMD 4
 #endif
 #ifndef CLR_GENERIC_WAVES_PER_SIMD
 #define CLR_GENERIC_WAVES_PER_SIMD 2
@@ -3171,6 +3225,8 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         flush_lds(T, Lp.tallies + cur_cfg);
       }
       cur_cfg = cfg;
+      for (int i = threadIdx.x; i < CLR_MAJ_SLABS; i += blockDim.x) s_slab_mu[i] = params[cfg].slab_mu[i];
+      __syncthreads();
     }
     const DevParams& P = params[cfg];
     const uint64_t G = (uint64_t)P.n_groups;
@@ -3183,9 +3239,11 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
       // of lanes in the wave.  A pass costs about the same however many
       // lanes are active, so handlers that a lane runs back to back share a
       // pass (RECYCLE scores finished histories and starts new ones; WATER
-      // runs a pending particle event, then the next flight), and the group
-      // chosen is the one with the most lanes per cycle of its handler's
-      // measured pass cost.
+      // runs a pending particle event, then the next flight; RECYCLE and
+      // IFACE passes end with one step of every lane in the acrylic, which
+      // new histories and most water-boundary transmissions enter), and the
+      // group chosen is the one with the most lanes per cycle of its
+      // handler's measured pass cost.
       const bool fin = alive && L.region == R_DONE;
       const uint64_t m_rc = __ballot(fin || (!alive && !exhausted));
       const uint64_t m_w = __ballot(alive && (L.region == R_WATER || L.region == R_WPART));
@@ -3205,7 +3263,8 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
 #pragma unroll
       for (int k = 0; k < K_N; ++k) {
         const uint64_t ck = (k == K_WATER) ? (cost[k] * CLR_WATER_BIAS) >> 2 : cost[k];
-        if (cnt[k] * bc > bn * ck) { op = k; bn = cnt[k]; bc = ck; }
+        const uint64_t ek = (cnt[k] && (k == K_RECYCLE || k == K_IFACE)) ? cnt[k] + cnt[K_ACRYLIC] : cnt[k];
+        if (ek * bc > bn * ck) { op = k; bn = ek; bc = ck; }
       }
       const uint64_t t_pass = __builtin_amdgcn_s_memtime();
       if (op == K_RECYCLE) {
@@ -3256,7 +3315,13 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light, V);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
-      } else if (op == K_ACRYLIC) {
+      } else if (op == K_HEAD) {
+        const bool was_head = alive && L.region == R_HEADSPACE;
+        if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
+        if constexpr (kind_diag<kKind>())
+          if (was_head && L.region == R_WATER && L.ic > 0) V.ret++;
+      }
+      if (op == K_ACRYLIC || op == K_RECYCLE || op == K_IFACE) {
         const bool was_acrylic = alive && L.region == R_ACRYLIC;
         if (alive && L.region == R_ACRYLIC) acrylic_step(L, P, A);
         // A chain of kLongChain bounces in one acrylic visit is almost always
@@ -3264,18 +3329,13 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         // cap; stepping it to the end now (the rest of the wave waits) keeps
         // such a chain from trailing past the end of the launch.  The same
         // applies when acrylic lanes are all this wave has left.
-        const bool only = !(m_rc | m_w | m_i | m_h);
+        const bool only = op == K_ACRYLIC && !(m_rc | m_w | m_i | m_h);
         while (__any(alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain))) {
           if (alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain)) acrylic_step(L, P, A);
         }
         // a return to the water after a particle event (CLARITAS.py:5009-5058)
         if constexpr (kind_diag<kKind>())
           if (was_acrylic && L.region == R_WATER && L.ic > 0) V.ret++;
-      } else {
-        const bool was_head = alive && L.region == R_HEADSPACE;
-        if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
-        if constexpr (kind_diag<kKind>())
-          if (was_head && L.region == R_WATER && L.ic > 0) V.ret++;
       }
       // running estimate of the pass cost (cycles, 1/8 weight to the new pass)
       const int64_t dt = (int64_t)(__builtin_amdgcn_s_memtime() - t_pass);
