@@ -26,8 +26,6 @@
 #include "precompute.h"
 #include "transport.h"
 
-#define CLR_COLD __attribute__((noinline))
-
 namespace {
 
 constexpr float kInfF = 3.402823466e+38F;
@@ -394,7 +392,7 @@ __device__ double quadratic_min_positive(double A, double B, double C) {
   if (t2 > 0.0 && t2 < best) best = t2;
   return best;
 }
-CLR_COLD __device__ double cubic_min_positive(double A3, double A2, double A1, double A0) {
+__device__ __forceinline__ double cubic_min_positive(double A3, double A2, double A1, double A0) {
   const double scale = fmax(fmax(fabs(A3), fabs(A2)), fmax(fabs(A1), fabs(A0)));
   if (scale <= 0.0) return 1.0e300;
   if (fabs(A3) <= 1e-12) return quadratic_min_positive(A2, A1, A0);
@@ -425,7 +423,7 @@ CLR_COLD __device__ double cubic_min_positive(double A3, double A2, double A1, d
 }
 // first sign change of a cubic on [lo,hi] by monotone-piece bisection,
 // CLARITAS.py:1690-1722
-CLR_COLD __device__ double bounded_cubic_root(double a3, double a2, double a1, double a0, double lo,
+__device__ __forceinline__ double bounded_cubic_root(double a3, double a2, double a1, double a0, double lo,
                                      double hi) {
   if (!(hi > lo) || !isfinite(hi)) return 1.0e300;
   double c0 = lo, c1 = hi, c2 = hi, c3 = hi;
@@ -468,7 +466,7 @@ CLR_COLD __device__ double bounded_cubic_root(double a3, double a2, double a1, d
   }
   return 1.0e300;
 }
-CLR_COLD __device__ double rankine_hit(const DevParams& P, double X, double Y, double Z, double VX,
+__device__ __forceinline__ double rankine_hit(const DevParams& P, double X, double Y, double Z, double VX,
                               double VY, double VZ) {
   const double a2 = (double)P.core_r * (double)P.core_r, K = P.rk_K, C0 = P.rk_C0;
   const double RR = (double)P.rin * (double)P.rin;
@@ -519,7 +517,7 @@ CLR_COLD __device__ double rankine_hit(const DevParams& P, double X, double Y, d
   }
   return 1.0e300;
 }
-CLR_COLD __device__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
+__device__ __forceinline__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
                                                   float vy, float vz);
 __device__ __forceinline__ float free_surface_hit(const DevParams& P, float x, float y, float z, float vx,
                                                   float vy, float vz) {
@@ -530,7 +528,7 @@ __device__ __forceinline__ float free_surface_hit(const DevParams& P, float x, f
   }
   return free_surface_hit_curved(P, x, y, z, vx, vy, vz);
 }
-CLR_COLD __device__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
+__device__ __forceinline__ float free_surface_hit_curved(const DevParams& P, float x, float y, float z, float vx,
                                                   float vy, float vz) {
   const int M = P.fs_model;
   const float R = P.rin;
@@ -748,7 +746,7 @@ __device__ __forceinline__ bool tir_for_ci(double ci, double n1, double n2) {
   return (n1 / n2) * sqrt(fmax(0.0, 1.0 - ci * ci)) >= 1.0;
 }
 // conserved-orbit trap classifier at the emergency cap, CLARITAS.py:3083-3135
-CLR_COLD __device__ bool acrylic_trapped(const DevParams& P, float x, float y, float vx, float vy, float vz) {
+__device__ __forceinline__ bool acrylic_trapped(const DevParams& P, float x, float y, float vx, float vy, float vz) {
   const double dx = vx, dy = vy, dz = vz, sp2 = dx * dx + dy * dy + dz * dz;
   if (!(sp2 > 0.0) || !isfinite(sp2)) return false;
   const double vxy2 = dx * dx + dy * dy, L = (double)x * dy - (double)y * dx, rs = sqrt(sp2);
@@ -856,7 +854,7 @@ __device__ double xyz_sample(const float* __restrict__ G, int NX, int NY, int NZ
   const double c0 = (1 - t1) * c00 + t1 * c01, c1 = (1 - t1) * c10 + t1 * c11;
   return fmax((1 - t0) * c0 + t0 * c1, 0.0);
 }
-CLR_COLD __device__ double sediment_scale(const DevParams& P, int pidx, float x, float y, float z) {
+__device__ __forceinline__ double sediment_scale(const DevParams& P, int pidx, float x, float y, float z) {
   const int tm = P.sed_model;
   if (tm == CLR_SED_UNIFORM) return 1.0;
   if (tm == CLR_SED_GRID_PER_BIN)
@@ -949,11 +947,14 @@ __device__ __forceinline__ bool sphere_accessible(const DevParams& P, double cx,
 // Smith microsurface random walk over a Beckmann slope distribution (Heitz,
 // Hanika, d'Eon & Dachsbacher, ACM TOG 35(4), 2016), in the signed,
 // log-domain form of CLARITAS.py:1792-2430, FP64 where the reference uses it.
-// Everything here is force-inlined: the persistent kernels contain no
-// function calls (with calls present, the generic kernel computed wrong
-// results on ROCm 7.2 even along paths that never executed a call).  The
-// finite-particle event (particle_event) holds the only micro_interface site
-// of each caller, so the walk is inlined twice per kernel.
+// The per-event helpers are force-inlined: a call inside the pass loop makes
+// the compiler save the whole live photon state around it, and those spills
+// and reloads land on the common path of every pass.  The only calls left in
+// the persistent kernels are the LDS tally clear and flush at configuration
+// changes and, in the ROUGH/LIGHT/CAUSAL variants, the ring-face throat test
+// and the LIGHT variant's voxel traversal.  The finite-particle event
+// (particle_event) holds the only micro_interface site of each caller, so
+// the walk is inlined twice per kernel.
 constexpr int kMicroWatchdog = 4096;
 constexpr double kSqrtPi = 1.77245385090551602729816748334114518;
 constexpr double kInv2SqrtPi = 0.28209479177387814347403972578038629;
