@@ -90,28 +90,96 @@ def cpu_baseline(histories_per_conc: int, seed: int):
             'seconds': t_total}
 
 
-def surrogate_line(ctx) -> dict:
-    """Second half of the metric: surrogate queries/s measured live on the
-    device (the 5-member BF16 ensemble, 65536 device-resident queries per
-    launch; timing does not depend on the weights) against batched Monte Carlo
-    at matched 1 % NTU accuracy, with the accuracy and Monte Carlo cost of the
-    held-out evaluation shipped in tardiis_and_claritas_amd/surrogate_eval.json."""
-    from tardiis_and_claritas_amd.surrogate import MlpEnsemble
-    ev = json.loads((Path(__file__).resolve().parent / 'tardiis_and_claritas_amd' / 'surrogate_eval.json').read_text())
-    ens = MlpEnsemble(ctx, n_members=5, hidden=256, n_hidden_layers=3, seed=0)
-    ms = ens.time_infer(65536, 50)
-    qps = 65536 / (ms * 1e-3)
-    mc_s = ev['mc_matched_photons_median'] / ev['mc_histories_per_s']
+SURROGATE_DIR = ROOT / 'tardiis_and_claritas_amd'
+SURROGATE_WEIGHTS = SURROGATE_DIR / 'surrogate_weights.npz'
+SURROGATE_HELDOUT = SURROGATE_DIR / 'surrogate_heldout.npz'
+SURROGATE_EVAL = SURROGATE_DIR / 'surrogate_eval.json'
+
+
+def surrogate_line(seed: int) -> dict:
+    """Second half of the metric, measured in this run: the trained 5-member
+    BF16 ensemble (committed weights) answers the committed held-out config-3
+    queries, whose Monte Carlo labels carry ~4000 expected 90-degree window
+    counts; queries/s is the inference kernel on 65536 device-resident
+    queries; the batched Monte Carlo rate is measured on the same held-out
+    configurations; the speed-up is Monte Carlo seconds per query for 1 %
+    relative error on S90/S0 times queries/s.  Queries whose ensemble spread
+    exceeds the committed fallback threshold are answered by Monte Carlo run
+    now, at 1 % relative error."""
+    import numpy as np
+    from tardiis_and_claritas_amd import surrogate as sg
+    from tardiis_and_claritas_amd.batched import BatchedRunner
+    ev = json.loads(SURROGATE_EVAL.read_text())
+    runner = BatchedRunner(0)
+    sur = sg.Surrogate.load(runner.ctx, SURROGATE_WEIGHTS)
+    d = np.load(SURROGATE_HELDOUT)
+    keys = sg.OPTICAL + sg.TRANSPORT
+    P = {k: d[f'spot_{k}'] for k in keys}
+    S_ref, n_ref = d['spot_S'], d['spot_hist']
+    pred, spread = sur.predict(sg.features(P))
+    m = sg.set_metrics(pred, S_ref, n_ref)
+    q_s = sur.ens.time_infer(65536, 50) * 1e-3 / 65536
+    # live batched Monte Carlo rate on 256 held-out configurations (one launch)
+    sub = {k: v[:256] for k, v in P.items()}
+    n_mc = np.full(256, 2_000_000, dtype=np.int64)
+    sg.monte_carlo(sub, n_mc, seed + 900, runner=runner)
+    sg.monte_carlo(sub, n_mc, seed + 901, runner=runner)   # one batch: its launch is timed by HIP events
+    rate = float(n_mc.sum()) / (runner.last_ms * 1e-3)
+    speed = sg.matched_speedup(pred, m['surrogate_rms_est'], q_s, rate)
+    thr = float(ev['bench_fallback_threshold'])
+    fb = sg.fallback_eval(P, pred, spread, S_ref, n_ref, {'threshold': thr}, q_s, rate, runner, seed + 902)['threshold']
+    print(f'bench: surrogate {1 / q_s:.3g} queries/s; held-out median {m["ntu_error_median"]:.4f} '
+          f'p90 {m["ntu_error_p90"]:.4f}; Monte Carlo {rate:.3g} histories/s', file=sys.stderr, flush=True)
     return {
-        'queries_per_s': qps,
-        'mc_seconds_per_query_at_1pct': mc_s,
-        'speedup_vs_batched_mc': qps * mc_s,
-        'ntu_error_median': ev['ntu_error_median'],
-        'ntu_error_p90': ev['ntu_error_p90'],
-        'held_out_configs': ev['n_test'],
+        'queries_per_s': 1.0 / q_s,
+        'mc_histories_per_s': rate,
+        'mc_seconds_per_query_at_1pct': speed['mc_query_s_median'],
+        'speedup_vs_batched_mc': speed['speedup_at_1pct'],
+        'speedup_vs_batched_mc_at_matched_rms': speed.get('speedup_at_matched_rms'),
+        'ntu_error_median': m['ntu_error_median'],
+        'ntu_error_p90': m['ntu_error_p90'],
+        'surrogate_rms_est': m['surrogate_rms_est'],
+        'label_se_median': m['label_se_median'],
+        'held_out_configs': m['n'],
         'training_configs': ev['n_train'],
-        'evaluation': 'tardiis_and_claritas_amd/surrogate_eval.json (speed-up = live queries/s x Monte Carlo '
-                      'seconds for 1 % relative error on S90/S0 at the held-out median)',
+        'fallback': fb,
+        'evaluation': 'live: committed weights (' + SURROGATE_WEIGHTS.name + ') on the committed held-out set ('
+                      + SURROGATE_HELDOUT.name + '); speed-up = live Monte Carlo seconds for 1 % relative error '
+                      'on S90/S0 (median over the held-out queries) x live queries/s; the fallback answers '
+                      'queries above the spread threshold by Monte Carlo now',
+    }
+
+
+def hybrid_line(h: int, seed: int) -> dict:
+    """The same sweep under the reference's production event law
+    (hybrid_mie_exact_rate, CLARITAS.py:9941): h histories per concentration in
+    one launch, one warm-up launch and two timed ones (HIP events)."""
+    import numpy as np
+    from tardiis_and_claritas_amd import workloads
+    from tardiis_and_claritas_amd.batched import BatchedRunner, flops_per_history_model, total_counters
+    specs = [workloads.config2(c, 'hybrid_mie_exact_rate')[:3] for c in workloads.CONFIG2_CONCENTRATIONS_MG_L]
+    r = BatchedRunner(0)
+    r.prepare(specs)
+    budgets = [h] * len(specs)
+    r.run_prepared(budgets, seed + 700)
+    ms, fl = [], []
+    for k in range(2):
+        T = r.run_prepared(budgets, seed + 701 + k)
+        ms.append(r.last_ms)
+        fl.append(flops_per_history_model(total_counters(T)))
+    n = h * len(specs)
+    t = float(np.mean(ms)) * 1e-3
+    ach = float(np.mean(fl)) / t / 1e12
+    print(f'bench: hybrid-law sweep {n:.3g} histories: {", ".join("%.1f" % m for m in ms)} ms', file=sys.stderr,
+          flush=True)
+    return {
+        'value': n / t, 'unit': 'photon histories/s', 'histories_per_launch': n, 'launch_ms': t * 1e3,
+        'workload': 'the config-2 sweep with the reference production law hybrid_mie_exact_rate (smooth '
+                    'spheres; conditional exact-rate closure for absorbing bins), GENERIC kernel variant',
+        'roofline': {'bound': 'valu', 'achieved': ach, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': ach / FP32_PEAK_TFLOPS, 'flops_per_history': float(np.mean(fl)) / n,
+                     'flop_model': 'the plain-law model above; the FP64 ray tracing of the conditional '
+                                   'closure trials is not counted, so frac is a lower bound'},
     }
 
 
@@ -128,6 +196,11 @@ def main():
     ap.add_argument('--chunk', type=int, default=0)
     ap.add_argument('--depth', type=int, default=2, help='launches in flight (separate HIP streams)')
     ap.add_argument('--queue-order', type=int, default=0, help='0 as given, 1 most expensive configuration first')
+    ap.add_argument('--roofline-launches', type=int, default=2,
+                    help='launches timed alone (one in flight) for the roofline figure')
+    ap.add_argument('--hybrid-histories', type=float, default=1e7,
+                    help='histories per concentration of the hybrid-law sweep (0 skips it)')
+    ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (counter calibration) and exit')
     args = ap.parse_args()
@@ -168,21 +241,32 @@ def main():
     sync()
     elapsed = time.perf_counter() - t0
     kernel_ms = list(pipe.last_ms)
-    flops, totals = 0.0, None
+    flops = 0.0
     for k, T in enumerate(results):
-        cnt = total_counters(T)
-        flops += flops_per_history_model(cnt)
-        totals = cnt if totals is None else totals + cnt
+        flops += flops_per_history_model(total_counters(T))
         print(f'bench: step {k + 1}/{args.steps}: launch {kernel_ms[k]:.1f} ms', file=sys.stderr, flush=True)
     n_hist = args.steps * H * len(specs)
     value = n_hist / elapsed
-    avg_ms = float(np.mean(kernel_ms))
     step_ms = elapsed / args.steps * 1e3
     tph = traffic_per_history()
-    # The device is busy through the whole timed region; with several launches
-    # in flight a launch's own event interval includes time it shares the
-    # compute units with its neighbours, so the rate uses the wall time per step.
-    achieved = flops / args.steps / (step_ms * 1e-3) / 1e12
+    # Roofline of the dominant kernel: the same launch timed alone (one launch
+    # in flight, so its HIP-event interval is its own duration), algorithmic
+    # FLOP of that launch from its event counters over that duration.
+    # (--roofline-launches 0 with --depth 1: the timed steps are such launches)
+    stage = pipe.stages[0]
+    r_ms, r_flops = [], []
+    for k in range(args.roofline_launches):
+        T = stage.run_prepared(budgets, seed + 500 + k)
+        r_ms.append(stage.last_ms)
+        r_flops.append(flops_per_history_model(total_counters(T)))
+    if not r_ms:
+        if args.depth != 1:
+            raise SystemExit('--roofline-launches 0 needs --depth 1')
+        r_ms = kernel_ms
+        r_flops = [flops_per_history_model(total_counters(T)) for T in results]
+    launch_ms = float(np.mean(r_ms))
+    achieved = float(np.mean(r_flops)) / (launch_ms * 1e-3) / 1e12
+    print(f'bench: roofline launches alone: {", ".join("%.1f" % m for m in r_ms)} ms', file=sys.stderr, flush=True)
     out = {
         'metric': 'photon histories/s (1 MI355X) + surrogate-vs-MC speedup at <=1% NTU error',
         'value': value,
@@ -206,23 +290,30 @@ def main():
                       'HIP streams' % args.depth,
         },
         'roofline': {
-            'bound': 'mfma',
+            'bound': 'valu',
             'achieved': achieved,
             'peak': FP32_PEAK_TFLOPS,
             'unit': 'TFLOP/s',
             'frac': achieved / FP32_PEAK_TFLOPS,
             'traffic': (tph * H * len(specs)) if tph else None,
-            'kernel': 'transport_kernel<true> (plain-Mie persistent kernel)',
-            'kernel_ms_avg': avg_ms,
-            'launches_in_flight': args.depth,
-            'time_basis': 'wall time per step (device busy throughout; with launches in flight each '
-                          'launch interval overlaps its neighbours)',
+            'traffic_source': 'bytes per launch = HBM bytes per history x histories per launch; bytes per '
+                              'history from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this '
+                              'workload, copy-calibrated (' + TRAFFIC_PROFILE.name + ')',
+            'kernel': 'transport_kernel<1> (plain-Mie persistent kernel)',
+            'launch_ms': launch_ms,
+            'launches_timed': len(r_ms),
+            'time_basis': 'HIP-event duration of one launch of the step workload run alone (one launch in '
+                          'flight), on the launch stream; the timed steps above keep two in flight',
             'flop_model': 'F = 16 + 30 N_flight + 16 N_null + 60 N_scat + 40 N_iface + 10 N_abs + 45 N_exit '
-                          'per history (FP32; FMA=2); peak is the FP32 figure, issued on the VALU',
-            'flops_per_history': flops / n_hist,
+                          'per history (FP32; FMA=2), event counts from the launch counters; peak is the '
+                          'FP32 figure, issued on the VALU',
+            'flops_per_history': float(np.mean(r_flops)) / (H * len(specs)),
         },
     }
-    out['surrogate'] = surrogate_line(pipe.stages[0].ctx)
+    if args.hybrid_histories > 0:
+        out['hybrid_law'] = hybrid_line(int(args.hybrid_histories), seed)
+    if not args.no_surrogate:
+        out['surrogate'] = surrogate_line(seed)
     if not args.no_cpu:
         per = args.cpu_histories or int(2e7)
         out['cpu_baseline'] = cpu_baseline(per, seed)

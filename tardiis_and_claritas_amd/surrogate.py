@@ -269,6 +269,21 @@ class Surrogate:
             losses.append(self.ens.train(x, y, batch, per, lr * f, seed + k, w=w).tolist())
         return losses
 
+    def save(self, path):
+        """Write the ensemble parameters (FP32 master copy) and the
+        feature/target normaliser to an .npz file."""
+        np.savez(path, params=self.ens.get_params(), **{k: v for k, v in self.norm.__dict__.items()})
+
+    @classmethod
+    def load(cls, ctx: Context, path, n_members=5, hidden=256, n_hidden_layers=3):
+        """A trained surrogate from a file written by `save` (or run_pipeline's
+        weights.npz)."""
+        d = np.load(path)
+        sur = cls(ctx, n_members=n_members, hidden=hidden, n_hidden_layers=n_hidden_layers)
+        sur.ens.set_params(np.asarray(d['params'], dtype=np.float32))
+        sur.norm = Normaliser(*(np.asarray(d[k], dtype=np.float64) for k in ('x_mean', 'x_std', 'y_mean', 'y_std')))
+        return sur
+
     def predict(self, F):
         """Signals [n][3] and the members' spread of the log targets [n][3];
         column 0 is the relative spread of the NTU reading S90/S0."""
@@ -326,6 +341,18 @@ def load_dataset(path, counts=None, test_counts=None):
             raise ValueError(f'{path}: no {part}_hist; give the counts it was generated with')
         out += [prm, d[f'{part}_S'], hist]
     return tuple(out)
+
+
+def write_heldout(dataset_path, out_path):
+    """Copy the held-out part of a dataset.npz (parameters, signals and the
+    histories behind them) to `out_path` as the spot_* arrays the benchmark's
+    live surrogate evaluation reads."""
+    d = np.load(dataset_path)
+    out = {f'spot_{k}': np.asarray(d[f'test_{k}'], dtype=np.float64) for k in OPTICAL + TRANSPORT}
+    out['spot_S'] = np.asarray(d['test_S'], dtype=np.float64)
+    out['spot_hist'] = np.asarray(d['test_hist'], dtype=np.int64)
+    np.savez_compressed(out_path, **out)
+    return len(out['spot_S'])
 
 
 def _cat(parts):
@@ -434,18 +461,18 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         spot['histories'] = int(n_sp.sum())
         spot.update(matched_speedup(S_pred, spot['surrogate_rms_est'], q_s, mc_rate))
         (out / 'summary.json').write_text(json.dumps(summary, indent=1))
-        np.savez(out / 'weights.npz', params=sur.ens.get_params(), **{k: v for k, v in sur.norm.__dict__.items()})
+        sur.save(out / 'weights.npz')
         log(json.dumps(summary))
         if thresholds:
             spot['fallback'] = fallback_eval(P_sp, S_pred, spread, S_sp, n_sp, thresholds, q_s, mc_rate, runner,
                                              seed + 4242)
     (out / 'summary.json').write_text(json.dumps(summary, indent=1))
-    np.savez(out / 'weights.npz', params=sur.ens.get_params(), **{k: v for k, v in sur.norm.__dict__.items()})
+    sur.save(out / 'weights.npz')
     log(json.dumps(summary))
     return summary
 
 
-FALLBACK_FRACTIONS = (0.001, 0.01, 0.05)
+FALLBACK_FRACTIONS = (0.01, 0.05, 0.1, 0.2, 0.3)
 
 
 def set_metrics(S_pred, S_ref, n_ref):
@@ -543,7 +570,12 @@ if __name__ == '__main__':
     ap.add_argument('--max-photons', type=float, default=2e8, help='photon budget cap per configuration')
     ap.add_argument('--spot-dataset', action='append', default=[],
                     help='dataset.npz whose held-out part is scored as a spot check (repeatable)')
+    ap.add_argument('--write-heldout', nargs=2, metavar=('DATASET', 'OUT'),
+                    help='write the held-out part of DATASET as the benchmark held-out file OUT and stop')
     a = ap.parse_args()
+    if a.write_heldout:
+        print(write_heldout(*a.write_heldout))
+        raise SystemExit(0)
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
                  a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted,
                  generate_only=a.generate_only, hi=int(a.max_photons), spot_datasets=a.spot_dataset)
