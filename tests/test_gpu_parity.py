@@ -239,3 +239,66 @@ def test_batch_equals_single(gpu_available):
         assert np.array_equal(T['status'], res[i]['status']), f'config {i}'
         assert np.array_equal(T['counters'], res[i]['counters']), f'config {i}'
         assert res[i]['counters'][__import__('tardiis_and_claritas_amd.abi', fromlist=['C_IDX']).C_IDX['n_flight']] > 0
+
+
+@pytest.mark.parametrize('law,weight_mode', [('hybrid_mie_exact_rate', 'mass_fraction'),
+                                             ('plain_mie', 'number_fraction')])
+def test_transport_tables_match_oracle(gpu_available, oracle_lib, law, weight_mode):
+    """transport_precompute_kernel (CLARITAS.py:3845-3921) on the device
+    against the oracle's restatement, bin by bin: radii, number densities,
+    per-bin and total attenuation, the channel rates, their sampling CDFs and
+    the majorants (FP64 where the reference computes in FP64)."""
+    _flush(f'transport tables {law} {weight_mode}')
+    kw, mat, conc, _, _ = workloads.config2(1000.0, law)
+    m = _model(kw)
+    m.prepare(mat, conc, weight_mode=weight_mode)
+    n = len(mat.diameter_m)
+    t = m.tables(n)
+    o = oracle_lib.OracleModel(m, mat, conc, weight_mode=weight_mode).tables()
+    for k in ('radius', 'number_density', 'mu_geom_bin', 'mu_sca_bin', 'mu_ext_bin', 'trace_geom_bin',
+              'trace_wave_bin', 'geom_cdf', 'wave_cdf'):
+        np.testing.assert_allclose(t[k], o[k], rtol=1e-12, atol=1e-300, err_msg=k)
+    np.testing.assert_allclose(t['mu_scalars'], o['mu_scalars'], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(t['majorants'], o['majorants'], rtol=1e-6)
+    assert np.all(np.diff(t['geom_cdf']) >= 0) and abs(t['geom_cdf'][-1] - 1.0) < 1e-12
+
+
+def _jackknife_se(batches):
+    """Leave-one-batch-out standard error of the normalised response,
+    sqrt((G-1)/G sum_i (r_(i) - mean r_(.))^2) over the G batches."""
+    b = np.asarray(batches, dtype=float)
+    G = len(b)
+    tot = b.sum(0)
+    loo = np.array([(tot - b[i]) / (tot - b[i]).sum() for i in range(G)])
+    return np.sqrt((G - 1) / G * ((loo - loo.mean(0)) ** 2).sum(0))
+
+
+def test_simulate_normalisation_matches_oracle(gpu_available, oracle_lib):
+    """simulate's normalised responses, fractions and jackknife errors equal
+    the same quantities formed here, from their definitions, out of the
+    oracle's tallies for the same histories (the device and the oracle trace
+    the same Philox streams, so their tallies differ by the few histories
+    whose FP32 rounding differs)."""
+    _flush('simulate normalisation vs oracle')
+    kw, mat, conc, _, seed = workloads.config1('hybrid_mie_exact_rate')
+    m = _model(kw)
+    n = 400_000
+    r = m.simulate(mat, conc, n_rays=n, seed=seed, path_diagnostics=False)
+    To, _ = oracle_lib.OracleModel(m, mat, conc).run(n, seed, 0)
+    det = To['det'][:To['n_groups']]
+    sym = 0.5 * (det[:, 0].sum(0) + det[:, 1].sum(0))
+    leg = 0.5 * (det[:, 2].sum(0) + det[:, 3].sum(0))
+    tol = dict(rtol=2e-3, atol=2e-5)
+    np.testing.assert_allclose(r.normalized_hardware_symmetry_response, sym / sym.sum(), **tol)
+    np.testing.assert_allclose(r.normalized_legacy_symmetry_response, leg / leg.sum(), **tol)
+    np.testing.assert_allclose(r.normalized_hardware_native_response, det[:, 0].sum(0) / det[:, 0].sum(), **tol)
+    se_o = _jackknife_se([0.5 * (det[g, 0] + det[g, 1]) for g in range(len(det))])
+    ok = sym > 200
+    np.testing.assert_allclose(r.hardware_normalized_jackknife_se[ok], se_o[ok], rtol=0.05)
+    ci = __import__('tardiis_and_claritas_amd.abi', fromlist=['C_IDX']).C_IDX
+    c = To['counters']
+    ent = c[ci['entered']]
+    for field, key in (('ballistic_fraction', 'ballistic'), ('mean_interactions', 'sum_ic'),
+                       ('mean_cell_outer_reflections', 'sum_cor'), ('particle_absorption_fraction', 'rays_pabs')):
+        np.testing.assert_allclose(getattr(r, field), c[ci[key]] / ent, rtol=2e-3, atol=2e-5, err_msg=field)
+    assert r.valid_exit_count == pytest.approx(c[ci['valid_air']], rel=1e-3)
