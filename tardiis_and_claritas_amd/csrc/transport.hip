@@ -71,6 +71,10 @@ __shared__ uint32_t s_key[2];
 // the z-slab majorants of the configuration a workgroup is tracing (read per
 // lane with a lane-dependent index, so from LDS rather than the parameter block)
 __shared__ float s_slab_mu[CLR_MAJ_SLABS];
+// conditional-closure state of a lane in R_WCLOSE (one slot per thread of the
+// workgroup; blocks have at most 256 threads): bin | fallback << 16 |
+// target-absorbed << 17, trial budget, trials, rejects, inaccessible, absorptions
+__shared__ uint32_t s_clo[6][256];
 // The lane keeps an 8-word queue of its stream: words are consumed from b0
 // and the queue shifts down (a dynamically indexed pick would make the
 // compiler keep it in scratch memory).  rng_topup appends the next block when
@@ -1586,92 +1590,135 @@ struct CondOut {
   PCount pc;
   uint32_t trials, rejects, inacc;
 };
-// conditional finite-particle closure, CLARITAS.py:4479-4536
+// conditional finite-particle closure, CLARITAS.py:4479-4536, in two parts:
+// cond_begin checks the bin and draws the target outcome (absorbed with
+// probability 1 - albedo); cond_trial runs one trial.  A trial samples the
+// channel (coherent Mie, Fresnel sphere at an accessible centre, or the
+// morphology wave) and the event; it is kept when its outcome (scattered /
+// absorbed) matches the target.  The transport kernel runs one trial per
+// CLOSE pass for every lane in the closure, so a lane that needs many trials
+// (an absorbing target, accepted with the event's small absorption
+// probability) no longer holds the rest of its wave; conditional_event runs
+// the whole loop at once (device probes).
+struct CondState {
+  uint32_t trials, rejects, inacc, nabs;
+  bool target_abs;
+};
+// 0: trials follow (st filled); otherwise the closure has ended with h
 template <bool kRough>
-__device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
-                                     float vx, float vy, float vz, int max_trials) {
-  CondOut h;
+__device__ __forceinline__ int cond_begin(const DevParams& P, Rng& r, int pidx, float x, float y, float z, float vx,
+                                          float vy, float vz, int max_trials, CondState& st, CondOut& h) {
   h.x = x; h.y = y; h.z = z; h.vx = vx; h.vy = vy; h.vz = vz;
   h.status = 0; h.channel = 0; h.failure = PF_OTHER; h.subcode = 9002;
   h.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
   h.trials = h.rejects = h.inacc = 0;
+  st.trials = st.rejects = st.inacc = st.nabs = 0;
+  st.target_abs = false;
   const double qs = P.qsca[pidx], qe = P.qext[pidx];
-  const float ki = P.k_imag[pidx], nr = P.n_real[pidx], radius = P.radius[pidx];
+  const float ki = P.k_imag[pidx];
   const double tol = 1e-10 * fmax(1.0, fmax(fabs(qs), fabs(qe)));
-  if (max_trials == 0) { h.subcode = 9001; return h; }
-  if (!isfinite(qs) || !isfinite(qe) || qe <= 0.0 || qs < -tol || qs > qe + tol || max_trials < 1) return h;
+  if (max_trials == 0) { h.subcode = 9001; return 1; }
+  if (!isfinite(qs) || !isfinite(qe) || qe <= 0.0 || qs < -tol || qs > qe + tol || max_trials < 1) return 1;
   const bool lossless = (ki == 0.0f && fabs(qe - qs) <= tol);
   const double albedo = lossless ? 1.0 : fmin(fmax(qs / qe, 0.0), 1.0);
   const double pabs = lossless ? 0.0 : fmin(fmax((qe - qs) / qe, 0.0), 1.0);
   if ((ki > 0.0f && (pabs <= 0.0 || albedo >= 1.0)) || (albedo > 0.0 && albedo < 2.3283064365386962890625e-10) ||
       (pabs > 0.0 && pabs < 2.3283064365386962890625e-10)) {
-    h.subcode = 9003; return h;
+    h.subcode = 9003; return 1;
   }
-  const bool target_abs = (albedo < 1.0 && U53(r) >= albedo);
+  st.target_abs = (albedo < 1.0 && U53(r) >= albedo);
+  return 0;
+}
+// One trial from the closure's entry state (x, v).  Returns 0 to go on
+// (rejected or inaccessible), 1 when the closure has ended with c (accepted,
+// failed, or out of trials).
+template <bool kRough>
+__device__ __forceinline__ int cond_trial(const DevParams& P, Rng& r, int pidx, float x, float y, float z, float vx,
+                                          float vy, float vz, int max_trials, CondState& st, CondOut& c) {
+  const double qs = P.qsca[pidx];
+  const float ki = P.k_imag[pidx], nr = P.n_real[pidx], radius = P.radius[pidx];
   const double wave = qs > 0.0 ? fmax(qs - 1.0, 0.0) / qs : 0.0;
-  uint32_t trials = 0, rejects = 0, inacc = 0, nabs = 0;
-  for (int trial = 0; trial < max_trials; ++trial) {
-    trials++;
-    CondOut c;
-    c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
-    c.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
-    c.failure = PF_OK; c.subcode = 0; c.status = 0;
-    c.channel = (U53(r) < wave) ? ((P.coherent >= 1.0f || U53(r) < (double)P.coherent) ? 1 : 2) : 0;
-    if (c.channel == 1) {
-      mie_scatter(r, P, pidx, c.vx, c.vy, c.vz);
-      c.status = 1;
-    } else {
-      // channel 0: Fresnel sphere at a sampled accessible centre; channel 2:
-      // morphology-wave event (direction only), CLARITAS.py:4512-4531
-      const bool dir_only = (c.channel == 2);
-      bool sph = false;
-      float gx = 0.0f, gy = 0.0f, gz = 0.0f, fx = 1.0f, fy = 0.0f;
-      double dcx = 0.0, dcy = 0.0, dcz = 0.0;
-      int pre = -1;  // outcome settled before the event
-      if (dir_only) {
-        const float fr = sqrtf(x * x + y * y);
-        if (fr > 1e-12f) { fx = -y / fr; fy = x / fr; }
-        if (radius <= 0.0f) {
-          pre = 1;
-        } else if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) {
-          pre = 0; c.failure = PF_SPHEROID; c.subcode = SUB_SPHEROID_ASPECT;
-        } else if (fabsf(P.aspect - 1.0f) < 1e-7f) {
-          sphere_sample(r, vx, vy, vz, gx, gy, gz);
-          dcx = -(double)radius * (double)gx; dcy = -(double)radius * (double)gy; dcz = -(double)radius * (double)gz;
-        } else {
-          sph = true;
-        }
-      } else {
+  st.trials++;
+  c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
+  c.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
+  c.failure = PF_OK; c.subcode = 0; c.status = 0;
+  c.channel = (U53(r) < wave) ? ((P.coherent >= 1.0f || U53(r) < (double)P.coherent) ? 1 : 2) : 0;
+  bool go_on = false;
+  if (c.channel == 1) {
+    mie_scatter(r, P, pidx, c.vx, c.vy, c.vz);
+    c.status = 1;
+  } else {
+    // channel 0: Fresnel sphere at a sampled accessible centre; channel 2:
+    // morphology-wave event (direction only), CLARITAS.py:4512-4531
+    const bool dir_only = (c.channel == 2);
+    bool sph = false;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f, fx = 1.0f, fy = 0.0f;
+    double dcx = 0.0, dcy = 0.0, dcz = 0.0;
+    int pre = -1;  // outcome settled before the event
+    if (dir_only) {
+      const float fr = sqrtf(x * x + y * y);
+      if (fr > 1e-12f) { fx = -y / fr; fy = x / fr; }
+      if (radius <= 0.0f) {
+        pre = 1;
+      } else if (!(P.aspect > 0.0f) || !isfinite(P.aspect)) {
+        pre = 0; c.failure = PF_SPHEROID; c.subcode = SUB_SPHEROID_ASPECT;
+      } else if (fabsf(P.aspect - 1.0f) < 1e-7f) {
         sphere_sample(r, vx, vy, vz, gx, gy, gz);
-        dcx = (double)x - (double)radius * gx; dcy = (double)y - (double)radius * gy;
-        dcz = (double)z - (double)radius * gz;
-        if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { inacc++; continue; }
+        dcx = -(double)radius * (double)gx; dcy = -(double)radius * (double)gy; dcz = -(double)radius * (double)gz;
+      } else {
+        sph = true;
       }
+    } else {
+      sphere_sample(r, vx, vy, vz, gx, gy, gz);
+      dcx = (double)x - (double)radius * gx; dcy = (double)y - (double)radius * gy;
+      dcz = (double)z - (double)radius * gz;
+      if (!sphere_accessible(P, dcx, dcy, dcz, radius)) { st.inacc++; go_on = true; }
+    }
+    if (!go_on) {
       if (pre >= 0) {
         c.status = pre;
       } else {
         float px = dir_only ? 0.0f : c.x, py = dir_only ? 0.0f : c.y, pz = dir_only ? 0.0f : c.z;
         float pvx = c.vx, pvy = c.vy, pvz = c.vz;
         int pf = 0, pfs = 0;
-        c.status = particle_event<kRough>(P, r, sph, P.n_water, nr, ki, radius, gx, gy, gz, dcx, dcy, dcz, fx, fy, 0.0f, px,
-                                  py, pz, pvx, pvy, pvz, c.pc, pf, pfs);
+        c.status = particle_event<kRough>(P, r, sph, P.n_water, nr, ki, radius, gx, gy, gz, dcx, dcy, dcz, fx, fy,
+                                          0.0f, px, py, pz, pvx, pvy, pvz, c.pc, pf, pfs);
         if (!dir_only) { c.x = px; c.y = py; c.z = pz; }
         if (!dir_only || sph || c.status == 1) { c.vx = pvx; c.vy = pvy; c.vz = pvz; }
         c.failure = pf; c.subcode = pfs;
       }
     }
-    nabs += c.pc.n_abs;
-    if (c.status != 1 && c.status != 2) {
-      c.trials = trials; c.rejects = rejects; c.inacc = inacc; c.pc.n_abs = nabs;
-      return c;
-    }
-    if ((c.status == 2) != target_abs) { rejects++; continue; }
-    c.trials = trials; c.rejects = rejects; c.inacc = inacc; c.pc.n_abs = nabs;
-    c.failure = PF_OK; c.subcode = 0;
-    return c;
   }
-  h.status = 0; h.failure = PF_OTHER; h.subcode = 9001;
-  h.trials = trials; h.rejects = rejects; h.inacc = inacc; h.pc.n_abs = nabs;
+  if (!go_on) {
+    st.nabs += c.pc.n_abs;
+    if (c.status != 1 && c.status != 2) {
+      c.trials = st.trials; c.rejects = st.rejects; c.inacc = st.inacc; c.pc.n_abs = st.nabs;
+      return 1;
+    }
+    if ((c.status == 2) != st.target_abs) {
+      st.rejects++;
+    } else {
+      c.trials = st.trials; c.rejects = st.rejects; c.inacc = st.inacc; c.pc.n_abs = st.nabs;
+      c.failure = PF_OK; c.subcode = 0;
+      return 1;
+    }
+  }
+  if ((int)st.trials >= max_trials) {
+    c.x = x; c.y = y; c.z = z; c.vx = vx; c.vy = vy; c.vz = vz;
+    c.status = 0; c.channel = 0; c.failure = PF_OTHER; c.subcode = 9001;
+    c.pc = PCount{0, 0, 0, 0, 0u, 0.0f, -1.0f, -1.0f, 0, 0};
+    c.trials = st.trials; c.rejects = st.rejects; c.inacc = st.inacc; c.pc.n_abs = st.nabs;
+    return 1;
+  }
+  return 0;
+}
+template <bool kRough>
+__device__ __forceinline__ CondOut conditional_event(const DevParams& P, Rng& r, int pidx, float x, float y, float z,
+                                                     float vx, float vy, float vz, int max_trials) {
+  CondOut h;
+  CondState st;
+  if (cond_begin<kRough>(P, r, pidx, x, y, z, vx, vy, vz, max_trials, st, h)) return h;
+  while (!cond_trial<kRough>(P, r, pidx, x, y, z, vx, vy, vz, max_trials, st, h)) {}
   return h;
 }
 
@@ -1952,10 +1999,11 @@ __device__ __forceinline__ float sfh213_response(float phi_deg) {
 // ------------------------------------------------------------ lane state --
 // R_WATER: next free flight; R_WPART / R_WIFACE: the flight ended at a
 // particle event / at a boundary of the water (event handled by the next pass)
-enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4, R_WPART = 5, R_WIFACE = 6 };
+// R_WCLOSE: a particle event whose conditional closure is running its trials (CLOSE passes)
+enum : int { R_WATER = 1, R_ACRYLIC = 2, R_HEADSPACE = 3, R_DONE = 4, R_WPART = 5, R_WIFACE = 6, R_WCLOSE = 7 };
 enum : int { EV_TOP = 1, EV_BOTTOM = 2, EV_SIDE = 3 };
 // kinds of scheduler pass
-enum : int { K_RECYCLE = 0, K_WATER = 1, K_IFACE = 2, K_ACRYLIC = 3, K_HEAD = 4, K_N = 5 };
+enum : int { K_RECYCLE = 0, K_WATER = 1, K_IFACE = 2, K_ACRYLIC = 3, K_HEAD = 4, K_CLOSE = 5, K_N = 6 };
 // who entered the acrylic / headspace (maps their return codes)
 enum : int { A_LAUNCH = 0, A_WATER = 1, A_BOTTOM = 2, A_HEAD = 3 };
 enum : int { H_LAUNCH = 0, H_WATERWALL = 1, H_TOP = 2, H_BOTTOM = 3 };
@@ -2669,6 +2717,65 @@ __device__ __forceinline__ void count_closure(unsigned long long* cc, const Cond
   wave_count(cc, CLR_C_COND_NUMERICAL, !ok && h.subcode != 9001);
 }
 
+// The end of a conditional closure (CLARITAS.py:4854-4933): the sampler
+// counts, a failure, or the accepted event applied to the lane.  The lane is
+// still at the event site with its incoming direction.
+template <int kKind>
+__device__ __forceinline__ void closure_end(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
+                                            Prov& V, unsigned long long* cc, const CondOut& h, bool fallback,
+                                            int cbin) {
+  constexpr bool kLight = (kKind == CLR_KERNEL_LIGHT), kDiag = kind_diag<kKind>();
+  const float sx0 = L.x, sy0 = L.y, sz0 = L.z, vix = L.vx, viy = L.vy, viz = L.vz;
+  const uint64_t gid = kLight ? rng_counter(L.r) : 0ull;
+  A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
+  count_closure(cc, h);
+  if (fallback ? (h.status != 1) : (h.status != 1 && h.status != 2)) {
+    terminate(L, 4, h.failure ? h.failure : PF_OTHER, h.subcode);
+    return;
+  }
+  L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
+  L.pr += h.pc.pr;
+  if constexpr (kDiag) { V.per += h.pc.per; V.pir += h.pc.pir; }
+  if (h.pc.per) L.flags |= F_PER;
+  if (h.pc.pir) L.flags |= F_PIR;
+  if (h.pc.pabs) L.flags |= F_PABS;
+  if (fallback) L.gic--;
+  if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
+  if (!fallback && h.status == 2) {
+    if constexpr (kLight) light_absorb(*D, gid, sx0, sy0, sz0);
+    terminate(L, 5);
+    return;
+  }
+  {
+    const int fres = h.pc.per ? 1 : (h.pc.pir ? 3 : 2);  // external reflection / internal reflection / transmission
+    const int type = (h.channel == 1) ? 4 : (h.channel == 2 ? fres + 4 : fres);
+    if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, type);
+    if constexpr (kDiag) prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, type, cbin, h.pc);
+  }
+}
+// CLOSE pass: one trial of the lane's conditional closure (state in s_clo)
+template <int kKind>
+__device__ __forceinline__ void water_closure(Lane& L, const DevParams& P, Acc& A, const LightDiag* __restrict__ D,
+                                              Prov& V, unsigned long long* cc) {
+  if constexpr (kKind != CLR_KERNEL_PLAIN) {
+    constexpr bool kRough = kind_rough<kKind>();
+    const int tid = threadIdx.x;
+    const uint32_t w0 = s_clo[0][tid];
+    const int cbin = (int)(w0 & 0xFFFFu), ctrials = (int)s_clo[1][tid];
+    const bool fallback = (w0 & 0x10000u) != 0;
+    CondState st;
+    st.target_abs = (w0 & 0x20000u) != 0;
+    st.trials = s_clo[2][tid]; st.rejects = s_clo[3][tid]; st.inacc = s_clo[4][tid]; st.nabs = s_clo[5][tid];
+    CondOut h;
+    if (cond_trial<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials, st, h)) {
+      L.region = R_WATER;
+      closure_end<kKind>(L, P, A, D, V, cc, h, fallback, cbin);
+    } else {
+      s_clo[2][tid] = st.trials; s_clo[3][tid] = st.rejects; s_clo[4][tid] = st.inacc; s_clo[5][tid] = st.nabs;
+    }
+  }
+}
+
 // the particle event at the end of a flight (CLARITAS.py:4774-4981).  Under
 // the conditional law the sampler counts of CLARITAS.py:4854-4933 go to cc
 // (the workgroup's LDS counters).
@@ -2815,32 +2922,18 @@ __device__ __forceinline__ void water_particle(Lane& L, const DevParams& P, Acc&
     }
   }
   if (cbin >= 0) {
-    CondOut h = conditional_event<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials);
-    A.trials += h.trials; A.rejects += h.rejects; A.inacc += h.inacc; A.n_abs += h.pc.n_abs;
-    count_closure(cc, h);
-    if (fallback ? (h.status != 1) : (h.status != 1 && h.status != 2)) {
-      terminate(L, 4, h.failure ? h.failure : PF_OTHER, h.subcode);
+    CondOut h;
+    CondState st;
+    if (cond_begin<kRough>(P, L.r, cbin, L.x, L.y, L.z, L.vx, L.vy, L.vz, ctrials, st, h)) {
+      closure_end<kKind>(L, P, A, D, V, cc, h, fallback, cbin);
       return;
     }
-    L.x = h.x; L.y = h.y; L.z = h.z; L.vx = h.vx; L.vy = h.vy; L.vz = h.vz;
-    L.pr += h.pc.pr;
-    if constexpr (kDiag) { V.per += h.pc.per; V.pir += h.pc.pir; }
-    if (h.pc.per) L.flags |= F_PER;
-    if (h.pc.pir) L.flags |= F_PIR;
-    if (h.pc.pabs) L.flags |= F_PABS;
-    if (fallback) L.gic--;
-    if (h.channel == 0) L.gic++; else { L.wic++; if (h.channel == 1) L.wcoh++; }
-    if (!fallback && h.status == 2) {
-      if constexpr (kLight) light_absorb(*D, gid, sx0, sy0, sz0);
-      terminate(L, 5);
-      return;
-    }
-    {
-      const int fres = h.pc.per ? 1 : (h.pc.pir ? 3 : 2);  // external reflection / internal reflection / transmission
-      const int type = (h.channel == 1) ? 4 : (h.channel == 2 ? fres + 4 : fres);
-      if constexpr (kLight) light_event(*D, gid, sx0, sy0, sz0, type);
-      if constexpr (kDiag) prov_commit(V, L, P, vix, viy, viz, sx0, sy0, sz0, type, cbin, h.pc);
-    }
+    // the trials run in CLOSE passes, one per pass
+    const int tid = threadIdx.x;
+    s_clo[0][tid] = (uint32_t)cbin | (fallback ? 0x10000u : 0u) | (st.target_abs ? 0x20000u : 0u);
+    s_clo[1][tid] = (uint32_t)ctrials;
+    s_clo[2][tid] = 0u; s_clo[3][tid] = 0u; s_clo[4][tid] = 0u; s_clo[5][tid] = 0u;
+    L.region = R_WCLOSE;
     return;
   }
   PCount pc{0, 0, 0, 0, 0u};
@@ -3197,7 +3290,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
   zero_lds(T);
   // per-wave pass-cost estimates in cycles (initial values are typical
   // costs; they adapt to the configuration within a few passes)
-  uint64_t cost[K_N] = {12000, 12000, 3000, 6000, 12000};
+  uint64_t cost[K_N] = {12000, 12000, 3000, 6000, 12000, 12000};
   int cur_cfg = -1;
   Acc A{};
   for (;;) {
@@ -3251,11 +3344,12 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
       const uint64_t m_i = __ballot(alive && L.region == R_WIFACE);
       const uint64_t m_a = __ballot(alive && L.region == R_ACRYLIC);
       const uint64_t m_h = __ballot(alive && L.region == R_HEADSPACE);
-      if (!(m_rc | m_w | m_i | m_a | m_h)) break;
+      const uint64_t m_c = (kKind == CLR_KERNEL_PLAIN) ? 0ull : __ballot(alive && L.region == R_WCLOSE);
+      if (!(m_rc | m_w | m_i | m_a | m_h | m_c)) break;
       int op = K_ACRYLIC;
       uint64_t bn = (uint64_t)__popcll(m_a), bc = cost[K_ACRYLIC];
       const uint64_t cnt[K_N] = {(uint64_t)__popcll(m_rc), (uint64_t)__popcll(m_w), (uint64_t)__popcll(m_i),
-                                 (uint64_t)__popcll(m_a), (uint64_t)__popcll(m_h)};
+                                 (uint64_t)__popcll(m_a), (uint64_t)__popcll(m_h), (uint64_t)__popcll(m_c)};
 #ifndef CLR_WATER_BIAS
 #define CLR_WATER_BIAS 8
 #endif
@@ -3316,6 +3410,11 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         if (alive && L.region == R_WATER) water_flight<kKind>(L, P, A, Lp.light, V);
       } else if (op == K_IFACE) {
         if (alive && L.region == R_WIFACE) water_iface(L, P, A);
+      } else if (op == K_CLOSE) {
+        if constexpr (kKind != CLR_KERNEL_PLAIN) {
+          if (alive && L.region == R_WCLOSE) rng_topup(L.r);
+          if (alive && L.region == R_WCLOSE) water_closure<kKind>(L, P, A, Lp.light, V, T.cnt);
+        }
       } else if (op == K_HEAD) {
         const bool was_head = alive && L.region == R_HEADSPACE;
         if (alive && L.region == R_HEADSPACE) headspace_step(L, P, A);
@@ -3330,7 +3429,7 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         // cap; stepping it to the end now (the rest of the wave waits) keeps
         // such a chain from trailing past the end of the launch.  The same
         // applies when acrylic lanes are all this wave has left.
-        const bool only = op == K_ACRYLIC && !(m_rc | m_w | m_i | m_h);
+        const bool only = op == K_ACRYLIC && !(m_rc | m_w | m_i | m_h | m_c);
         while (__any(alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain))) {
           if (alive && L.region == R_ACRYLIC && (only || L.it >= kLongChain)) acrylic_step(L, P, A);
         }
