@@ -97,6 +97,32 @@ def test_history_by_history_config1(gpu_available, oracle_lib, event_model):
     _compare_conditional_counts(Tg, To, event_model == 'hybrid_mie_exact_rate')
 
 
+@pytest.mark.parametrize('conc_mg_l', [3162.0, 316.0])
+def test_history_by_history_hybrid_dense(gpu_available, oracle_lib, conc_mg_l):
+    """The hybrid law where most histories run conditional closures (the
+    config-2 cell and PSD at 316 and 3162 mg/L, uniform field so the oracle
+    tracks with the same single majorant): the device parks each closure
+    and runs its trials in CLOSE passes, the oracle runs them in one loop;
+    the histories must still agree one by one."""
+    _flush(f'history-by-history hybrid dense {conc_mg_l}')
+    kw, mat, conc, _, seed = workloads.config2(conc_mg_l, 'hybrid_mie_exact_rate')
+    kw['sediment_transport_model'] = 'uniform'
+    for k in ('sediment_field', 'sediment_field_zmax_m'):
+        kw.pop(k, None)
+    m = _model(kw)
+    N = 20000
+    m.prepare(mat, conc)
+    Tg, rg, _ = m.run_tallies(N, seed, 0, n_records=N)
+    To, ro = oracle_lib.OracleModel(m, mat, conc).run(N, seed, 0, n_records=N)
+    same = _compare_records(rg, ro)
+    assert same.mean() >= 0.999, f'only {same.mean():.5f} agree; first mismatch {np.nonzero(~same)[0][:5]}'
+    assert np.abs(Tg['det'].sum(0) - To['det'].sum(0)).sum() <= 0.004 * N
+    _compare_conditional_counts(Tg, To, True)
+    from tardiis_and_claritas_amd import causal
+    cg = causal.conditional_counts(Tg['counters'])
+    assert cg['outcome_rejections'] > 0        # absorbing targets ran multi-trial closures
+
+
 def _compare_conditional_counts(Tg, To, active):
     """The ten conditional_sampler_counts of the device and the oracle on the
     same streams (per-history counts: equal up to the < 0.1 % of histories
