@@ -208,8 +208,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is '
-                                   'described in DESIGN.md, not run)'}))
+        print(json.dumps({'error': 'the transport runs on one GPU: --gpus must be 1 (sharding history ranges '
+                                   'over several GPUs is designed, not implemented)'}))
         sys.exit(2)
     import numpy as np
     from tardiis_and_claritas_amd import workloads

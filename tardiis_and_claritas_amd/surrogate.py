@@ -555,7 +555,7 @@ def fallback_eval(P_sp, S_pred, spread, S_ref, n_ref, thresholds, q_s, mc_rate, 
 if __name__ == '__main__':
     import argparse
     ap = argparse.ArgumentParser(description='surrogate pipeline: batched MC data, GPU training, held-out evaluation')
-    ap.add_argument('--out', default='gpurun_out/surrogate')
+    ap.add_argument('--out', default='results/surrogate')
     ap.add_argument('--n-optical', type=int, default=64)
     ap.add_argument('--n-variants', type=int, default=64)
     ap.add_argument('--n-test-optical', type=int, default=16)
