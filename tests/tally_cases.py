@@ -3,9 +3,8 @@ they are judged by.
 
 Each case is a workload of tardiis_and_claritas_amd.workloads at a photon
 count that gives at least 2,000 hardware hits per side in the 90-degree
-channel (bore 9), the weakest of the detector channels compared (0, 90
-and 135 degrees).  The oracle's
-tallies for histories [0, N) with Philox key seed + 2 are committed in
+channel (bore 9), the weakest channel among those the tolerance is set for
+(0, 90 and 135 degrees).  The oracle's tallies for histories [0, N) with Philox key seed + 2 are committed in
 golden/oracle_tallies.json (script golden/make_oracle_tallies.py); the
 device runs the same N with key seed + 1, so the two are independent
 samples of the same law.
