@@ -3289,8 +3289,11 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
   __shared__ unsigned int s_next;
   zero_lds(T);
   // per-wave pass-cost estimates in cycles (initial values are typical
-  // costs; they adapt to the configuration within a few passes)
-  uint64_t cost[K_N] = {12000, 12000, 3000, 6000, 12000, 12000};
+  // costs; they adapt to the configuration within a few passes).  Costs stay
+  // in [64, 2^24] and a group has at most 64 lanes, so the 32-bit products
+  // of the group choice below stay under 2^31: the whole choice runs on the
+  // scalar unit (gfx9 has no scalar 64-bit magnitude compare)
+  uint32_t cost[K_N] = {12000, 12000, 3000, 6000, 12000, 12000};
   int cur_cfg = -1;
   Acc A{};
   for (;;) {
@@ -3347,9 +3350,9 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
       const uint64_t m_c = (kKind == CLR_KERNEL_PLAIN) ? 0ull : __ballot(alive && L.region == R_WCLOSE);
       if (!(m_rc | m_w | m_i | m_a | m_h | m_c)) break;
       int op = K_ACRYLIC;
-      uint64_t bn = (uint64_t)__popcll(m_a), bc = cost[K_ACRYLIC];
-      const uint64_t cnt[K_N] = {(uint64_t)__popcll(m_rc), (uint64_t)__popcll(m_w), (uint64_t)__popcll(m_i),
-                                 (uint64_t)__popcll(m_a), (uint64_t)__popcll(m_h), (uint64_t)__popcll(m_c)};
+      uint32_t bn = (uint32_t)__popcll(m_a), bc = cost[K_ACRYLIC];
+      const uint32_t cnt[K_N] = {(uint32_t)__popcll(m_rc), (uint32_t)__popcll(m_w), (uint32_t)__popcll(m_i),
+                                 (uint32_t)__popcll(m_a), (uint32_t)__popcll(m_h), (uint32_t)__popcll(m_c)};
 #ifndef CLR_WATER_BIAS
 #define CLR_WATER_BIAS 8
 #endif
@@ -3357,8 +3360,8 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
       // at cheap handlers return to the water group sooner
 #pragma unroll
       for (int k = 0; k < K_N; ++k) {
-        const uint64_t ck = (k == K_WATER) ? (cost[k] * CLR_WATER_BIAS) >> 2 : cost[k];
-        const uint64_t ek = (cnt[k] && (k == K_RECYCLE || k == K_IFACE)) ? cnt[k] + cnt[K_ACRYLIC] : cnt[k];
+        const uint32_t ck = (k == K_WATER) ? (cost[k] * CLR_WATER_BIAS) >> 2 : cost[k];
+        const uint32_t ek = (cnt[k] && (k == K_RECYCLE || k == K_IFACE)) ? cnt[k] + cnt[K_ACRYLIC] : cnt[k];
         if (ek * bc > bn * ck) { op = k; bn = ek; bc = ck; }
       }
       const uint64_t t_pass = __builtin_amdgcn_s_memtime();
@@ -3437,10 +3440,13 @@ __global__ __launch_bounds__(256, kKind == CLR_KERNEL_PLAIN ? CLR_PLAIN_WAVES_PE
         if constexpr (kind_diag<kKind>())
           if (was_acrylic && L.region == R_WATER && L.ic > 0) V.ret++;
       }
-      // running estimate of the pass cost (cycles, 1/8 weight to the new pass)
-      const int64_t dt = (int64_t)(__builtin_amdgcn_s_memtime() - t_pass);
-      const int64_t c0 = (int64_t)cost[op];
-      cost[op] = (uint64_t)max(c0 + ((min(dt, (int64_t)(1 << 24)) - c0) >> 3), (int64_t)64);
+      // running estimate of the pass cost (cycles, 1/8 weight to the new
+      // pass).  Constant indices keep cost[] in scalar registers: an index
+      // by op would move the array through vector registers every pass.
+      const int32_t dt = (int32_t)min((int64_t)(__builtin_amdgcn_s_memtime() - t_pass), (int64_t)(1 << 24));
+#pragma unroll
+      for (int k = 0; k < K_N; ++k)
+        if (k == op) cost[k] = (uint32_t)max((int32_t)cost[k] + ((dt - (int32_t)cost[k]) >> 3), 64);
     }
   }
   if (cur_cfg >= 0) {
