@@ -317,6 +317,19 @@ def photon_budget(conc_mg_l, counts=200.0, p90_per_mg_l=1e-7, lo=200_000, hi=200
     return np.clip(n, lo, hi).astype(np.int64)
 
 
+def adaptive_budget(s90_pilot, pilot_photons, counts, lo=200_000, hi=400_000_000):
+    """Histories per configuration so that the 90 deg window collects about
+    `counts` photons, from a pilot estimate of its yield S90 (pilot counts /
+    `pilot_photons`).  The 90 deg yield varies about a hundredfold across the
+    optical parameters at one concentration, so a budget from the
+    concentration alone leaves the weakly scattering, strongly absorbing
+    configurations with a few hundred counts (and the noisiest labels) while
+    spending far more than needed on the rest.  A pilot with no count is
+    taken to have half a count."""
+    s90 = np.maximum(np.asarray(s90_pilot, dtype=float), 0.5 / float(pilot_photons))
+    return np.clip(counts / s90, lo, hi).astype(np.int64)
+
+
 def mc_matched_photons(S, rel=0.01):
     """Histories for a relative standard error `rel` on S90/S0 (binomial
     counts; S0 and S90 are disjoint windows)."""
@@ -361,21 +374,25 @@ def _cat(parts):
 
 def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
                  test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True,
-                 generate_only=False, hi=200_000_000, spot_datasets=()):
+                 generate_only=False, hi=200_000_000, spot_datasets=(), adaptive_counts=0.0,
+                 pilot_photons=1_000_000):
     """Generate (n_optical x n_variants training and n_test_optical x
     n_test_variants held-out configurations; 0 skips), merge earlier
     `datasets` (paths of dataset.npz), train, evaluate on the held-out part
     and on the held-out parts of `spot_datasets` (Monte Carlo spot checks at
     high counts, scored separately, with the Monte Carlo fallback run on
     them); writes summary.json, weights.npz and dataset.npz (this run's own
-    samples)."""
+    samples).  With `adaptive_counts` > 0 the training configurations get
+    adaptive_budget histories from a pilot of `pilot_photons` each (its own
+    random streams, so a label is unbiased given its budget); otherwise the
+    concentration-only photon_budget with `counts`."""
     from .batched import BatchedRunner
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     runner = BatchedRunner(0)
     t0 = time.time()
 
-    def gen(n_opt, n_var, cnt, sd):
+    def gen(n_opt, n_var, cnt, sd, adapt=0.0):
         p_opt = workloads.config3_params(n_opt, sd)
         g = np.random.Generator(np.random.PCG64(sd + 7919))
         cols = {k: [] for k in OPTICAL + TRANSPORT}
@@ -385,9 +402,16 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
             for v in range(n_var):
                 params = {k: np.asarray(p_opt[k])[sel] for k in OPTICAL}
                 params.update(transport_params(len(sel), g))
-                budget = photon_budget(params['conc_mg_l'], cnt, hi=hi)
                 specs = [workloads.config3_item(params, i) for i in range(len(sel))]
                 runner.prepare(specs)
+                if adapt > 0:
+                    Tp = runner.run_prepared([pilot_photons] * len(sel), sd + 7_000_003 * (v + 1) + a0)
+                    hist += pilot_photons * len(sel)
+                    mc_s += runner.last_ms * 1e-3
+                    s90 = [signals_from_tallies(Tp[k], pilot_photons)[0] for k in range(len(sel))]
+                    budget = adaptive_budget(s90, pilot_photons, adapt, hi=hi)
+                else:
+                    budget = photon_budget(params['conc_mg_l'], cnt, hi=hi)
                 T = runner.run_prepared(budget.tolist(), sd + 1_000_003 * (v + 1) + a0)
                 hist += int(budget.sum())
                 mc_s += runner.last_ms * 1e-3
@@ -402,7 +426,7 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         return ({k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig), np.concatenate(hists), hist,
                 mc_s)
 
-    P_tr, S_tr, n_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed)
+    P_tr, S_tr, n_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed, adaptive_counts)
     P_te, S_te, n_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1)
     np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr, train_hist=n_tr,
              **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te, test_hist=n_te)
@@ -568,6 +592,9 @@ if __name__ == '__main__':
     ap.add_argument('--unweighted', action='store_true', help='plain MSE instead of inverse-variance weights')
     ap.add_argument('--generate-only', action='store_true', help='write dataset.npz and stop (no training)')
     ap.add_argument('--max-photons', type=float, default=2e8, help='photon budget cap per configuration')
+    ap.add_argument('--adaptive-counts', type=float, default=0.0,
+                    help='target 90 deg counts per training configuration from a pilot run (0: budget from --counts)')
+    ap.add_argument('--pilot-photons', type=int, default=1_000_000, help='pilot histories per configuration')
     ap.add_argument('--spot-dataset', action='append', default=[],
                     help='dataset.npz whose held-out part is scored as a spot check (repeatable)')
     ap.add_argument('--write-heldout', nargs=2, metavar=('DATASET', 'OUT'),
@@ -578,4 +605,5 @@ if __name__ == '__main__':
         raise SystemExit(0)
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
                  a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted,
-                 generate_only=a.generate_only, hi=int(a.max_photons), spot_datasets=a.spot_dataset)
+                 generate_only=a.generate_only, hi=int(a.max_photons), spot_datasets=a.spot_dataset,
+                 adaptive_counts=a.adaptive_counts, pilot_photons=a.pilot_photons)

@@ -47,6 +47,20 @@ def test_photon_budget_clips():
     assert b[1] == int(100.0 / (1e-7 * 1000.0))
 
 
+def test_adaptive_budget_targets_counts_from_pilot():
+    # pilot yields 1e-3, 1e-6 and none (half a count of 1e6 -> 5e-7), and one
+    # above the counts/lo ratio: budgets counts/S90, clipped to [lo, hi]
+    b = sg.adaptive_budget(np.array([1e-3, 1e-6, 0.0, 0.5]), 1_000_000, 20000.0, lo=200_000, hi=400_000_000)
+    assert b[0] == 20_000_000
+    assert b[1] == 400_000_000            # 2e10 clipped to hi
+    assert b[2] == 400_000_000            # 4e10 clipped to hi
+    assert b[3] == 200_000                # 4e4 clipped to lo
+    # the expected 90 deg count of an unclipped budget is the target
+    s90 = np.array([3e-4, 2e-5])
+    bb = sg.adaptive_budget(s90, 1_000_000, 5000.0)
+    np.testing.assert_allclose(bb * s90, 5000.0, rtol=1e-6)
+
+
 def test_set_metrics_recovers_network_error_under_label_noise():
     # predictions off by a known 2 % rms in S90/S0, labels with Poisson noise of
     # a comparable size: the label variance is subtracted from mean d^2
