@@ -317,17 +317,25 @@ def photon_budget(conc_mg_l, counts=200.0, p90_per_mg_l=1e-7, lo=200_000, hi=200
     return np.clip(n, lo, hi).astype(np.int64)
 
 
-def adaptive_budget(s90_pilot, pilot_photons, counts, lo=200_000, hi=400_000_000):
-    """Histories per configuration so that the 90 deg window collects about
-    `counts` photons, from a pilot estimate of its yield S90 (pilot counts /
-    `pilot_photons`).  The 90 deg yield varies about a hundredfold across the
-    optical parameters at one concentration, so a budget from the
-    concentration alone leaves the weakly scattering, strongly absorbing
-    configurations with a few hundred counts (and the noisiest labels) while
-    spending far more than needed on the rest.  A pilot with no count is
-    taken to have half a count."""
-    s90 = np.maximum(np.asarray(s90_pilot, dtype=float), 0.5 / float(pilot_photons))
-    return np.clip(counts / s90, lo, hi).astype(np.int64)
+def adaptive_budget(s90_pilot, pilot_photons, counts, lo=200_000, hi=400_000_000, s0_pilot=None):
+    """Histories per configuration for a label of about `counts` effective
+    counts, from pilot estimates of the window yields (pilot counts /
+    `pilot_photons`).  With the 90 deg yield alone the 90 deg window collects
+    about `counts` photons.  With the 0 deg yield as well, the relative
+    variance of the NTU reading S90/S0 (disjoint windows, so
+    1/(n S90) + 1/(n S0)) is brought to 1/counts:
+    n = counts (1/S90 + 1/S0); dense suspensions, whose transmitted beam is
+    nearly extinct, then get the histories their 0 deg label needs.  The
+    yields vary about a hundredfold across the optical parameters at one
+    concentration, so a budget from the concentration alone leaves the
+    weakly scattering, strongly absorbing configurations with a few hundred
+    counts (and the noisiest labels) while spending far more than needed on
+    the rest.  A pilot with no count is taken to have half a count."""
+    floor = 0.5 / float(pilot_photons)
+    inv = 1.0 / np.maximum(np.asarray(s90_pilot, dtype=float), floor)
+    if s0_pilot is not None:
+        inv = inv + 1.0 / np.maximum(np.asarray(s0_pilot, dtype=float), floor)
+    return np.clip(counts * inv, lo, hi).astype(np.int64)
 
 
 def mc_matched_photons(S, rel=0.01):
@@ -356,6 +364,16 @@ def load_dataset(path, counts=None, test_counts=None):
     return tuple(out)
 
 
+def load_spot(path):
+    """(params, signals, histories) of a spot-check set: the held-out part
+    of a dataset.npz, or the spot_* arrays of a file written by
+    write_heldout."""
+    d = np.load(path)
+    if 'spot_S' in d.files:
+        return {k: d[f'spot_{k}'] for k in OPTICAL + TRANSPORT}, d['spot_S'], d['spot_hist']
+    return load_dataset(path)[3:]
+
+
 def write_heldout(dataset_path, out_path):
     """Copy the held-out part of a dataset.npz (parameters, signals and the
     histories behind them) to `out_path` as the spot_* arrays the benchmark's
@@ -375,7 +393,7 @@ def _cat(parts):
 def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test_variants=32, counts=200.0,
                  test_counts=2000.0, epochs=60, seed=20261204, log=print, datasets=(), weighted=True,
                  generate_only=False, hi=200_000_000, spot_datasets=(), adaptive_counts=0.0,
-                 pilot_photons=1_000_000):
+                 pilot_photons=1_000_000, time_budget=None, ratio_budget=False, adaptive_test=False):
     """Generate (n_optical x n_variants training and n_test_optical x
     n_test_variants held-out configurations; 0 skips), merge earlier
     `datasets` (paths of dataset.npz), train, evaluate on the held-out part
@@ -385,14 +403,19 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
     samples).  With `adaptive_counts` > 0 the training configurations get
     adaptive_budget histories from a pilot of `pilot_photons` each (its own
     random streams, so a label is unbiased given its budget); otherwise the
-    concentration-only photon_budget with `counts`."""
+    concentration-only photon_budget with `counts`; `ratio_budget` sizes the
+    adaptive budget for the S90/S0 ratio instead of S90, and `adaptive_test`
+    gives the held-out configurations that budget with `test_counts`.  With
+    `time_budget`
+    (seconds) the training generation stops after the first batch that ends
+    past it and keeps the configurations done so far."""
     from .batched import BatchedRunner
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     runner = BatchedRunner(0)
     t0 = time.time()
 
-    def gen(n_opt, n_var, cnt, sd, adapt=0.0):
+    def gen(n_opt, n_var, cnt, sd, adapt=0.0, budget_s=None):
         p_opt = workloads.config3_params(n_opt, sd)
         g = np.random.Generator(np.random.PCG64(sd + 7919))
         cols = {k: [] for k in OPTICAL + TRANSPORT}
@@ -400,6 +423,8 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         for a0 in range(0, n_opt, 256):
             sel = np.arange(a0, min(n_opt, a0 + 256))
             for v in range(n_var):
+                if budget_s is not None and time.time() - t0 > budget_s:
+                    break
                 params = {k: np.asarray(p_opt[k])[sel] for k in OPTICAL}
                 params.update(transport_params(len(sel), g))
                 specs = [workloads.config3_item(params, i) for i in range(len(sel))]
@@ -408,8 +433,9 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
                     Tp = runner.run_prepared([pilot_photons] * len(sel), sd + 7_000_003 * (v + 1) + a0)
                     hist += pilot_photons * len(sel)
                     mc_s += runner.last_ms * 1e-3
-                    s90 = [signals_from_tallies(Tp[k], pilot_photons)[0] for k in range(len(sel))]
-                    budget = adaptive_budget(s90, pilot_photons, adapt, hi=hi)
+                    Sp = np.stack([signals_from_tallies(Tp[k], pilot_photons) for k in range(len(sel))])
+                    budget = adaptive_budget(Sp[:, 0], pilot_photons, adapt, hi=hi,
+                                             s0_pilot=Sp[:, 1] if ratio_budget else None)
                 else:
                     budget = photon_budget(params['conc_mg_l'], cnt, hi=hi)
                 T = runner.run_prepared(budget.tolist(), sd + 1_000_003 * (v + 1) + a0)
@@ -426,8 +452,9 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         return ({k: np.concatenate(v) for k, v in cols.items()}, np.concatenate(sig), np.concatenate(hists), hist,
                 mc_s)
 
-    P_tr, S_tr, n_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed, adaptive_counts)
-    P_te, S_te, n_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1)
+    P_tr, S_tr, n_tr, h_tr, s_tr = gen(n_optical, n_variants, counts, seed, adaptive_counts, time_budget)
+    P_te, S_te, n_te, h_te, s_te = gen(n_test_optical, n_test_variants, test_counts, seed + 1,
+                                       test_counts if adaptive_test else 0.0)
     np.savez(out / 'dataset.npz', **{f'train_{k}': v for k, v in P_tr.items()}, train_S=S_tr, train_hist=n_tr,
              **{f'test_{k}': v for k, v in P_te.items()}, test_S=S_te, test_hist=n_te)
     mc_rate = (h_tr + h_te) / (s_tr + s_te) if (s_tr + s_te) > 0 else None
@@ -476,7 +503,7 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
         summary['fallback_thresholds'] = thresholds
         summary['held_out']['spread_calibration'] = spread_calibration(S_pred, spread, S_te, n_te)
     if spot_datasets:
-        parts = [load_dataset(pth)[3:] for pth in spot_datasets]
+        parts = [load_spot(pth) for pth in spot_datasets]
         P_sp = _cat([a for a, _, _ in parts])
         S_sp = np.concatenate([b for _, b, _ in parts])
         n_sp = np.concatenate([c for _, _, c in parts])
@@ -497,6 +524,40 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
 
 
 FALLBACK_FRACTIONS = (0.01, 0.05, 0.1, 0.2, 0.3)
+
+
+def evaluate_weights(weights, spot_paths, thresholds, seed=1, log=print):
+    """Score trained weights on each spot-check set separately: NTU error
+    against the Monte Carlo labels (label noise subtracted), the Monte Carlo
+    time per query for 1 % error against the network's, and the fallback at
+    each spread threshold (Monte Carlo run now).  The batched Monte Carlo
+    rate is timed on the first 256 configurations of the first set."""
+    from .batched import BatchedRunner
+    runner = BatchedRunner(0)
+    sur = Surrogate.load(runner.ctx, weights)
+    q_s = sur.ens.time_infer(65536, 20) * 1e-3 / 65536
+    out = dict(weights=str(weights), surrogate_query_s=q_s, surrogate_queries_per_s=1.0 / q_s, sets={})
+    mc_rate = None
+    for k, path in enumerate(spot_paths):
+        P_sp, S_sp, n_sp = load_spot(path)
+        if mc_rate is None:
+            sel = {q: np.asarray(v)[:256] for q, v in P_sp.items()}
+            n_mc = np.full(len(sel['rouse']), 2_000_000, dtype=np.int64)
+            monte_carlo(sel, n_mc, seed + 900, runner=runner)
+            monte_carlo(sel, n_mc, seed + 901, runner=runner)
+            mc_rate = float(n_mc.sum()) / (runner.last_ms * 1e-3)
+            out['mc_histories_per_s'] = mc_rate
+        S_pred, spread = sur.predict(features(P_sp))
+        m = set_metrics(S_pred, S_sp, n_sp)
+        m['histories'] = int(np.sum(n_sp))
+        m['distinct_optical'] = int(len({tuple(np.asarray(P_sp[q])[i] for q in OPTICAL) for i in range(len(S_sp))}))
+        m.update(matched_speedup(S_pred, m['surrogate_rms_est'], q_s, mc_rate))
+        m['fallback'] = fallback_eval(P_sp, S_pred, spread, S_sp, n_sp, thresholds, q_s, mc_rate, runner,
+                                      seed + 4242 + k)
+        out['sets'][str(path)] = m
+        log(json.dumps({str(path): {q: m[q] for q in ('n', 'ntu_error_median', 'ntu_error_p90', 'label_rms',
+                                                        'surrogate_rms_est', 'speedup_at_1pct')}}))
+    return out
 
 
 def set_metrics(S_pred, S_ref, n_ref):
@@ -594,16 +655,32 @@ if __name__ == '__main__':
     ap.add_argument('--max-photons', type=float, default=2e8, help='photon budget cap per configuration')
     ap.add_argument('--adaptive-counts', type=float, default=0.0,
                     help='target 90 deg counts per training configuration from a pilot run (0: budget from --counts)')
+    ap.add_argument('--ratio-budget', action='store_true',
+                    help='adaptive budget for the relative error of S90/S0 rather than the 90 deg counts')
     ap.add_argument('--pilot-photons', type=int, default=1_000_000, help='pilot histories per configuration')
+    ap.add_argument('--time-budget', type=float, default=None,
+                    help='seconds after which training-data generation stops (configurations done so far are kept)')
     ap.add_argument('--spot-dataset', action='append', default=[],
                     help='dataset.npz whose held-out part is scored as a spot check (repeatable)')
+    ap.add_argument('--adaptive-test', action='store_true',
+                    help='held-out configurations get the pilot ratio budget with --test-counts')
+    ap.add_argument('--evaluate', metavar='WEIGHTS',
+                    help='score WEIGHTS on each --spot-dataset separately (thresholds from --thresholds) and stop')
+    ap.add_argument('--thresholds', metavar='SUMMARY', help='summary.json whose fallback_thresholds --evaluate uses')
     ap.add_argument('--write-heldout', nargs=2, metavar=('DATASET', 'OUT'),
                     help='write the held-out part of DATASET as the benchmark held-out file OUT and stop')
     a = ap.parse_args()
     if a.write_heldout:
         print(write_heldout(*a.write_heldout))
         raise SystemExit(0)
+    if a.evaluate:
+        thr = json.loads(Path(a.thresholds).read_text())['fallback_thresholds'] if a.thresholds else {}
+        res = evaluate_weights(a.evaluate, a.spot_dataset, thr, a.seed, log=lambda s: print(s, flush=True))
+        Path(a.out).mkdir(parents=True, exist_ok=True)
+        (Path(a.out) / 'evaluate.json').write_text(json.dumps(res, indent=1))
+        raise SystemExit(0)
     run_pipeline(a.out, a.n_optical, a.n_variants, a.n_test_optical, a.n_test_variants, a.counts, a.test_counts,
                  a.epochs, a.seed, log=lambda s: print(s, flush=True), datasets=a.dataset, weighted=not a.unweighted,
                  generate_only=a.generate_only, hi=int(a.max_photons), spot_datasets=a.spot_dataset,
-                 adaptive_counts=a.adaptive_counts, pilot_photons=a.pilot_photons)
+                 adaptive_counts=a.adaptive_counts, pilot_photons=a.pilot_photons, time_budget=a.time_budget,
+                 ratio_budget=a.ratio_budget, adaptive_test=a.adaptive_test)

@@ -61,6 +61,19 @@ def test_adaptive_budget_targets_counts_from_pilot():
     np.testing.assert_allclose(bb * s90, 5000.0, rtol=1e-6)
 
 
+def test_ratio_budget_sets_the_ntu_label_variance():
+    # with the 0 deg yield as well, the budget makes the binomial relative
+    # variance of S90/S0, 1/(n S90) + 1/(n S0), equal to 1/counts
+    s90, s0 = np.array([3e-4, 2e-5, 1e-3]), np.array([0.4, 1e-5, 2e-3])
+    b = sg.adaptive_budget(s90, 1_000_000, 1000.0, s0_pilot=s0)
+    np.testing.assert_allclose(1.0 / (b * s90) + 1.0 / (b * s0), 1e-3, rtol=1e-5)
+    # an extinct transmitted beam (no pilot count) gets half a count of 1e6
+    b0 = sg.adaptive_budget(np.array([1e-4]), 1_000_000, 100.0, hi=10**12, s0_pilot=np.array([0.0]))
+    assert b0[0] == int(100.0 * (1e4 + 2e6))
+    # the ratio budget is never below the 90 deg one
+    assert np.all(b >= sg.adaptive_budget(s90, 1_000_000, 1000.0))
+
+
 def test_set_metrics_recovers_network_error_under_label_noise():
     # predictions off by a known 2 % rms in S90/S0, labels with Poisson noise of
     # a comparable size: the label variance is subtracted from mean d^2
