@@ -526,6 +526,48 @@ def run_pipeline(out_dir, n_optical=64, n_variants=64, n_test_optical=16, n_test
 FALLBACK_FRACTIONS = (0.01, 0.05, 0.1, 0.2, 0.3)
 
 
+def full_scale_queries(weights, n_queries=100_000, n_spot=512, counts=20_000.0, seed=49009,
+                       pilot_photons=1_000_000, hi=1_000_000_000, threshold=None, log=print):
+    """The surrogate leg of the full-scale workload: `n_queries` held-out
+    configurations drawn afresh from the config-3 space (a seed no training or
+    spot-check set used) answered by the trained ensemble in one call, and
+    the first `n_spot` of them spot-checked by batched Monte Carlo with pilot
+    ratio budgets for `counts` effective counts on S90/S0.  Returns the
+    query times (host to host, and the inference kernel alone on the device),
+    the spread distribution, the share above the fallback `threshold`, and
+    the spot-check errors with the labels' own variance subtracted."""
+    from .batched import BatchedRunner
+    runner = BatchedRunner(0)
+    sur = Surrogate.load(runner.ctx, weights)
+    p = workloads.config3_params(n_queries, seed)
+    P = {k: np.asarray(p[k], dtype=np.float64) for k in OPTICAL + TRANSPORT}
+    F = features(P)
+    sur.predict(F[:1024])                        # warm-up
+    t0 = time.time()
+    pred, spread = sur.predict(F)
+    wall = time.time() - t0
+    kernel_ms = sur.ens.time_infer(n_queries, 20)
+    log(f'full scale: {n_queries} queries in {wall * 1e3:.1f} ms host to host, {kernel_ms:.3f} ms on the device')
+    sub = {k: v[:n_spot] for k, v in P.items()}
+    t1 = time.time()
+    Sp = monte_carlo(sub, pilot_photons, seed + 11, runner=runner)
+    budget = adaptive_budget(Sp[:, 0], pilot_photons, counts, hi=hi, s0_pilot=Sp[:, 1])
+    S_mc = monte_carlo(sub, budget, seed + 13, runner=runner,
+                       progress=lambda d, n: log(f'full scale: Monte Carlo spot check {d}/{n}'))
+    mc_wall = time.time() - t1
+    m = set_metrics(pred[:n_spot], S_mc, budget)
+    out = dict(n_queries=int(n_queries), seed=int(seed), query_wall_s=wall, queries_per_s_host=n_queries / wall,
+               query_kernel_ms=kernel_ms, queries_per_s_kernel=n_queries / (kernel_ms * 1e-3),
+               spread_ntu_quantiles={str(q): float(np.quantile(spread[:, 0], q)) for q in (0.5, 0.9, 0.99, 0.999)},
+               predicted_positive=bool(np.all(pred > 0)), n_spot=int(n_spot), spot_counts=float(counts),
+               spot_histories=int(budget.sum()) + int(pilot_photons) * int(n_spot), spot_mc_wall_s=mc_wall,
+               spot=m)
+    if threshold is not None:
+        out['fallback_threshold'] = float(threshold)
+        out['fallback_fraction'] = float(np.mean(spread[:, 0] > threshold))
+    return out
+
+
 def evaluate_weights(weights, spot_paths, thresholds, seed=1, log=print):
     """Score trained weights on each spot-check set separately: NTU error
     against the Monte Carlo labels (label noise subtracted), the Monte Carlo
@@ -666,12 +708,23 @@ if __name__ == '__main__':
                     help='held-out configurations get the pilot ratio budget with --test-counts')
     ap.add_argument('--evaluate', metavar='WEIGHTS',
                     help='score WEIGHTS on each --spot-dataset separately (thresholds from --thresholds) and stop')
+    ap.add_argument('--full-scale', metavar='WEIGHTS',
+                    help='query WEIGHTS on 1e5 fresh configurations, spot-check 512 by Monte Carlo, and stop')
     ap.add_argument('--thresholds', metavar='SUMMARY', help='summary.json whose fallback_thresholds --evaluate uses')
     ap.add_argument('--write-heldout', nargs=2, metavar=('DATASET', 'OUT'),
                     help='write the held-out part of DATASET as the benchmark held-out file OUT and stop')
     a = ap.parse_args()
     if a.write_heldout:
         print(write_heldout(*a.write_heldout))
+        raise SystemExit(0)
+    if a.full_scale:
+        ev = Path(__file__).resolve().parent / 'surrogate_eval.json'
+        thr = json.loads(ev.read_text())['bench_fallback_threshold'] if ev.exists() else None
+        res = full_scale_queries(a.full_scale, seed=a.seed, threshold=thr, log=lambda s: print(s, flush=True))
+        Path(a.out).mkdir(parents=True, exist_ok=True)
+        (Path(a.out) / 'full_scale.json').write_text(json.dumps(res, indent=1))
+        print(json.dumps({k: v for k, v in res.items() if k != 'spot'}), flush=True)
+        print(json.dumps(res['spot']), flush=True)
         raise SystemExit(0)
     if a.evaluate:
         thr = json.loads(Path(a.thresholds).read_text())['fallback_thresholds'] if a.thresholds else {}

@@ -182,3 +182,25 @@ def test_surrogate_gain_over_monte_carlo(gpu_available):
     assert t_match / q_s > 1000
     top = spread[ok, 0] >= np.quantile(spread[ok, 0], 0.9)
     assert err[top].mean() > err[~top].mean()
+
+
+def test_committed_weights_meet_one_percent(gpu_available):
+    """The committed ensemble on the committed spot-check sets: the network's
+    own rms error on the NTU reading S90/S0 (the labels' Monte Carlo variance
+    subtracted from the mean squared log error) is within 1 %, and so is the
+    median error against the labels."""
+    _flush('committed surrogate weights on the spot-check sets')
+    from pathlib import Path
+    from tardiis_and_claritas_amd import surrogate as sg
+    d = Path(sg.__file__).resolve().parent
+    sur = sg.Surrogate.load(_ctx(), d / 'surrogate_weights.npz')
+    for name, bar in (('surrogate_spotcheck.npz', 0.01), ('surrogate_heldout.npz', 0.01)):
+        P, S, n = sg.load_spot(d / name)
+        pred, spread = sur.predict(sg.features(P))
+        m = sg.set_metrics(pred, S, n)
+        _flush(f'{name}: median {m["ntu_error_median"]:.4f} rms vs labels {m["rms_vs_labels"]:.4f} '
+               f'label rms {m["label_rms"]:.4f} network rms {m["surrogate_rms_est"]:.4f}')
+        assert m['n'] == 1024 and m['n_without_counts'] == 0
+        assert m['surrogate_rms_est'] <= bar
+        assert m['ntu_error_median'] <= bar
+        assert np.all(np.isfinite(spread)) and np.all(spread >= 0.0)

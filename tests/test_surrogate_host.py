@@ -111,3 +111,18 @@ def test_spread_calibration_bins_and_matched_speedup():
     # 1 % needs 4x the histories of 2 %
     np.testing.assert_allclose(s['speedup_at_1pct'] / s['speedup_at_matched_rms'], 4.0, rtol=1e-9)
     assert not s['meets_1pct'] and sg.matched_speedup(true, 0.009, 1e-8, 1e9)['meets_1pct']
+
+
+def test_committed_spot_check_set():
+    # the spot-check set the bench scores: 1,024 configurations of the
+    # config-3 space, each with its own optical draw, whose S90/S0 labels
+    # carry under 1 % relative Monte Carlo error
+    from pathlib import Path
+    P, S, n = sg.load_spot(Path(sg.__file__).resolve().parent / 'surrogate_spotcheck.npz')
+    assert len(S) == 1024 and np.all(S > 0)
+    assert len({tuple(P[k][i] for k in sg.OPTICAL) for i in range(len(S))}) == 1024
+    v = sg.label_variance(S, n)[:, 0]
+    assert np.sqrt(np.mean(v)) < 0.01
+    assert np.all((P['conc_mg_l'] >= 10.0) & (P['conc_mg_l'] <= 10000.0))
+    assert np.all((P['median_um'] >= 2.0) & (P['median_um'] <= 60.0))
+    assert set(np.unique(P['wavelength_nm'])) <= {530.0, 860.0}
