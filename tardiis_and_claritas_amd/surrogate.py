@@ -546,7 +546,8 @@ def full_scale_queries(weights, n_queries=100_000, n_spot=512, counts=20_000.0, 
     t0 = time.time()
     pred, spread = sur.predict(F)
     wall = time.time() - t0
-    kernel_ms = sur.ens.time_infer(n_queries, 20)
+    chunk = min(n_queries, 65536)               # one device-resident chunk per timed launch
+    kernel_ms = sur.ens.time_infer(chunk, 20) * n_queries / chunk
     log(f'full scale: {n_queries} queries in {wall * 1e3:.1f} ms host to host, {kernel_ms:.3f} ms on the device')
     sub = {k: v[:n_spot] for k, v in P.items()}
     t1 = time.time()
