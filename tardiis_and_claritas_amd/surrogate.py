@@ -536,6 +536,8 @@ def full_scale_queries(weights, n_queries=100_000, n_spot=512, counts=20_000.0, 
     query times (host to host, and the inference kernel alone on the device),
     the spread distribution, the share above the fallback `threshold`, and
     the spot-check errors with the labels' own variance subtracted."""
+    if not (1 <= n_spot <= n_queries):
+        raise ValueError('need 1 <= n_spot <= n_queries')
     from .batched import BatchedRunner
     runner = BatchedRunner(0)
     sur = Surrogate.load(runner.ctx, weights)
@@ -595,8 +597,9 @@ def evaluate_weights(weights, spot_paths, thresholds, seed=1, log=print):
         m['histories'] = int(np.sum(n_sp))
         m['distinct_optical'] = int(len({tuple(np.asarray(P_sp[q])[i] for q in OPTICAL) for i in range(len(S_sp))}))
         m.update(matched_speedup(S_pred, m['surrogate_rms_est'], q_s, mc_rate))
-        m['fallback'] = fallback_eval(P_sp, S_pred, spread, S_sp, n_sp, thresholds, q_s, mc_rate, runner,
-                                      seed + 4242 + k)
+        if thresholds:
+            m['fallback'] = fallback_eval(P_sp, S_pred, spread, S_sp, n_sp, thresholds, q_s, mc_rate, runner,
+                                          seed + 4242 + k)
         out['sets'][str(path)] = m
         log(json.dumps({str(path): {q: m[q] for q in ('n', 'ntu_error_median', 'ntu_error_p90', 'label_rms',
                                                         'surrogate_rms_est', 'speedup_at_1pct')}}))
