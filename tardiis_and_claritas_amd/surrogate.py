@@ -290,17 +290,33 @@ class Surrogate:
         m, s = self.ens.predict(self._x(F))
         return self.signals(m * self.norm.y_std + self.norm.y_mean), s * self.norm.y_std
 
-    def query(self, params, threshold=0.01, mc_photons=10_000_000, seed=1, runner=None):
+    def query(self, params, threshold=None, mc_photons=None, rel=0.01, seed=1, runner=None):
         """Signals for a parameter dict: network where the ensemble spread of
-        log(S90/S0) is within `threshold`, Monte Carlo elsewhere.  Returns
-        (signals, used_mc mask)."""
+        log(S90/S0) is within `threshold`, batched Monte Carlo elsewhere.
+        `threshold` defaults to the committed fallback threshold
+        (surrogate_eval.json, bench_fallback_threshold); the Monte Carlo
+        answers get `mc_photons` histories each, by default the histories
+        for a relative error `rel` on S90/S0 sized from the network's own
+        prediction (clipped to [2e5, 2e9]).  Returns (signals, used_mc
+        mask)."""
+        if threshold is None:
+            threshold = default_fallback_threshold()
         F = features(params)
         S, spread = self.predict(F)
         mc = spread[:, 0] > threshold
         if np.any(mc):
             sub = {k: np.asarray(v)[mc] for k, v in params.items()}
-            S[mc] = monte_carlo(sub, mc_photons, seed, runner=runner)
+            n = (np.clip(mc_matched_photons(S[mc], rel), 200_000, 2_000_000_000).astype(np.int64)
+                 if mc_photons is None else mc_photons)
+            S[mc] = monte_carlo(sub, n, seed, runner=runner)
         return S, mc
+
+
+def default_fallback_threshold():
+    """The committed fallback threshold: the held-out 99th percentile of the
+    ensemble spread of log(S90/S0) for the shipped weights."""
+    ev = Path(__file__).resolve().parent / 'surrogate_eval.json'
+    return float(json.loads(ev.read_text())['bench_fallback_threshold'])
 
 
 def ntu_relative_error(S_pred, S_ref):

@@ -204,3 +204,34 @@ def test_committed_weights_meet_one_percent(gpu_available):
         assert m['surrogate_rms_est'] <= bar
         assert m['ntu_error_median'] <= bar
         assert np.all(np.isfinite(spread)) and np.all(spread >= 0.0)
+
+
+def test_query_falls_back_to_monte_carlo(gpu_available):
+    """Surrogate.query answers from the network where the ensemble agrees and
+    by batched Monte Carlo (histories sized for 1 % on S90/S0) where it does
+    not: unflagged answers equal predict(), flagged ones are Monte Carlo
+    estimates that agree with the spot-check labels within their combined
+    counting error."""
+    _flush('surrogate query with Monte Carlo fallback')
+    from pathlib import Path
+    from tardiis_and_claritas_amd import surrogate as sg
+    from tardiis_and_claritas_amd.batched import BatchedRunner
+    d = Path(sg.__file__).resolve().parent
+    runner = BatchedRunner(0)
+    sur = sg.Surrogate.load(runner.ctx, d / 'surrogate_weights.npz')
+    P, S_ref, n_ref = sg.load_spot(d / 'surrogate_spotcheck.npz')
+    sel = np.arange(64)
+    P = {k: np.asarray(v)[sel] for k, v in P.items()}
+    S_ref, n_ref = S_ref[sel], n_ref[sel]
+    pred, spread = sur.predict(sg.features(P))
+    thr = float(np.sort(spread[:, 0])[-5])          # the four most uncertain queries fall back
+    S, mc = sur.query(P, threshold=thr, seed=5, runner=runner)
+    assert int(mc.sum()) == 4
+    np.testing.assert_array_equal(S[~mc], pred[~mc])
+    n_mc = np.clip(sg.mc_matched_photons(pred[mc], 0.01), 200_000, 2_000_000_000)
+    # relative standard error of the ratio S90/S0 from both Monte Carlo runs
+    se = np.sqrt(sg.label_variance(S_ref[mc], n_ref[mc])[:, 0] + sg.label_variance(S[mc], n_mc)[:, 0])
+    d_log = np.log((S[mc, 0] / S[mc, 1]) / (S_ref[mc, 0] / S_ref[mc, 1]))
+    _flush(f'fallback ratio errors {np.round(d_log, 4)} against standard errors {np.round(se, 4)}')
+    assert np.all(np.abs(d_log) < 4.0 * se)
+    assert sg.default_fallback_threshold() > 0.0
